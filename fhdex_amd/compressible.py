@@ -1,0 +1,138 @@
+"""Host-side mirror of src_compressible's interface over the C ABI.
+
+Function names and argument meaning follow the reference (compressible_functions.H:23-55):
+``conservedToPrimitive``, ``calculateTransportCoeffs``, ``calculateFlux``, ``RK3step``, plus
+the fast fused step ``rk3_step``.  Arrays are MultiFab-shaped float64 device tensors laid out
+(ncomp, z, y, x) with the reference's ghost widths; `DeviceState` allocates the whole argument
+list of RK3step.  torch is used only for device memory and streams.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+from .lib import Context, check, fab_of_tensor, lib
+from .params import Fab, Params
+
+
+class MultiFab:
+    """One FAB of a single-box periodic domain: a device tensor plus its index box."""
+
+    def __init__(self, t, lo):
+        self.t = t
+        self.lo = tuple(lo)
+
+    @property
+    def fab(self) -> Fab:
+        return fab_of_tensor(self.t, self.lo)
+
+    @staticmethod
+    def cell(p: Params, ncomp: int, device="cuda"):
+        import torch
+        n0, n1, n2 = p.n_cells
+        g = p.ngc
+        t = torch.zeros((ncomp, n2 + 2 * g, n1 + 2 * g, n0 + 2 * g), dtype=torch.float64, device=device)
+        return MultiFab(t, (-g, -g, -g))
+
+    @staticmethod
+    def face(p: Params, d: int, ncomp: int, device="cuda"):
+        import torch
+        n0, n1, n2 = p.n_cells
+        t = torch.zeros((ncomp, n2 + (d == 2), n1 + (d == 1), n0 + (d == 0)), dtype=torch.float64,
+                        device=device)
+        return MultiFab(t, (0, 0, 0))
+
+    @staticmethod
+    def node(p: Params, ncomp: int = 1, device="cuda"):
+        import torch
+        n0, n1, n2 = p.n_cells
+        t = torch.zeros((ncomp, n2 + 1, n1 + 1, n0 + 1), dtype=torch.float64, device=device)
+        return MultiFab(t, (0, 0, 0))
+
+
+def _fa(mfs):
+    return (Fab * 3)(*[m.fab for m in mfs])
+
+
+class DeviceState:
+    """Every MultiFab of RK3step's argument list (main_driver.cpp:160-297) on the device."""
+
+    def __init__(self, p: Params, device="cuda"):
+        self.p = p
+        nv, NS = p.nvars, p.nspecies
+        cell = lambda nc: MultiFab.cell(p, nc, device)  # noqa: E731
+        self.cu, self.cup, self.cup2 = cell(nv), cell(nv), cell(nv)
+        self.prim = cell(p.nprimvars)
+        self.eta, self.zeta, self.kappa = cell(1), cell(1), cell(1)
+        self.chi, self.D = cell(NS), cell(NS * NS)
+        self.flux = [MultiFab.face(p, d, nv + 4, device) for d in range(3)]
+        self.stochFlux = [MultiFab.face(p, d, nv, device) for d in range(3)]
+        self.cornx = [MultiFab.node(p, 1, device) for _ in range(3)]
+        self.corny = [MultiFab.node(p, 1, device) for _ in range(3)]
+        self.cornz = [MultiFab.node(p, 1, device) for _ in range(3)]
+        self.visccorn = MultiFab.node(p, 1, device)
+        self.rancorn = MultiFab.node(p, 1, device)
+
+
+class NoiseSets:
+    """The A/B noise of one step (stochFlux_A/B, rancorn_A/B of timeStep.cpp:38-53)."""
+
+    def __init__(self, p: Params, device="cuda"):
+        nv = p.nvars
+        self.A = [MultiFab.face(p, d, nv, device) for d in range(3)]
+        self.B = [MultiFab.face(p, d, nv, device) for d in range(3)]
+        self.rA = MultiFab.node(p, 1, device)
+        self.rB = MultiFab.node(p, 1, device)
+
+
+def conservedToPrimitive(ctx: Context, prim: MultiFab, cons: MultiFab, stream=None):
+    check(lib().fhdx_cons_to_prim(ctx.h, prim.fab, cons.fab, stream), "conservedToPrimitive")
+
+
+def FillBoundary(ctx: Context, mf: MultiFab, stream=None):
+    check(lib().fhdx_fill_boundary(ctx.h, mf.fab, stream), "FillBoundary")
+
+
+def calculateTransportCoeffs(ctx: Context, prim, eta, zeta, kappa, chi, D, stream=None):
+    check(lib().fhdx_transport(ctx.h, prim.fab, eta.fab, zeta.fab, kappa.fab, chi.fab, D.fab, stream),
+          "calculateTransportCoeffs")
+
+
+def calculateFlux(ctx: Context, cons, prim, eta, zeta, kappa, chi, D, flux, stochFlux, cornx, corny,
+                  cornz, visccorn, rancorn, dt, stream=None):
+    check(lib().fhdx_calculate_flux(ctx.h, cons.fab, prim.fab, eta.fab, zeta.fab, kappa.fab, chi.fab,
+                                    D.fab, _fa(flux), _fa(stochFlux), _fa(cornx), _fa(corny),
+                                    _fa(cornz), visccorn.fab, rancorn.fab, dt, stream),
+          "calculateFlux")
+
+
+def fill_noise(ctx: Context, noise: NoiseSets, seed: int, step: int, instance: int = 0, stream=None):
+    check(lib().fhdx_fill_noise(ctx.h, seed, step, instance, _fa(noise.A), _fa(noise.B),
+                                noise.rA.fab, noise.rB.fab, stream), "fill_noise")
+
+
+def RK3step(ctx: Context, st: DeviceState, dt: float, noise: NoiseSets | None = None, seed: int = 1,
+            step: int = 0, stream=None):
+    """RK3step (timeStep.cpp:11-427) on the reference's arrays; noise from `noise` if given,
+    else Philox (seed, step)."""
+    if noise is not None:
+        a, b = _fa(noise.A), _fa(noise.B)
+        ra, rb = noise.rA.fab, noise.rB.fab
+        args = (a, b, C.pointer(ra), C.pointer(rb))
+    else:
+        args = (None, None, None, None)
+    check(lib().fhdx_rk3_step_ref(ctx.h, st.cu.fab, st.cup.fab, st.cup2.fab, st.prim.fab, st.eta.fab,
+                                  st.zeta.fab, st.kappa.fab, st.chi.fab, st.D.fab, _fa(st.flux),
+                                  _fa(st.stochFlux), _fa(st.cornx), _fa(st.corny), _fa(st.cornz),
+                                  st.visccorn.fab, st.rancorn.fab, *args, dt, seed, step, stream),
+          "RK3step")
+
+
+def rk3_step(ctx: Context, cu: MultiFab, prim: MultiFab, dt: float, seed: int, step: int, stream=None):
+    """Fast fused RK3 step with in-kernel Philox noise."""
+    check(lib().fhdx_rk3_step(ctx.h, cu.fab, prim.fab, dt, seed, step, stream), "rk3_step")
+
+
+def rk3_step_injected(ctx: Context, cu: MultiFab, prim: MultiFab, noise: NoiseSets, dt: float,
+                      stream=None):
+    check(lib().fhdx_rk3_step_injected(ctx.h, cu.fab, prim.fab, _fa(noise.A), _fa(noise.B),
+                                       noise.rA.fab, noise.rB.fab, dt, stream), "rk3_step_injected")

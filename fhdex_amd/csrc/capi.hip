@@ -1,0 +1,505 @@
+// capi.hip -- the C ABI of include/fhdx.h.
+//
+// Host C++ over the HIP kernels: argument and shape checks, the RK3 stage driver of
+// timeStep.cpp:11-427, context-owned workspace (allocated once, never per step) and the
+// device error flag that replaces the reference's amrex::Abort calls.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fused.hpp"
+#include "kernels.hpp"
+
+namespace fhdx {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+}  // namespace fhdx
+
+using namespace fhdx;
+
+// Device buffers a context owns: the reference's scratch MultiFabs for the drop-in RK3 step
+// and the noise sets when the caller supplies none.
+struct RefWorkspace {
+    bool ready = false;
+    std::vector<double*> blocks;
+    fhdx_fab cup, cup2, prim, eta, zeta, kappa, chi, D;
+    fhdx_fab flux[3], stoch[3], sA[3], sB[3], corn[3][3];
+    fhdx_fab visccorn, rancorn, rA, rB;
+};
+
+struct fhdx_ctx {
+    fhdx_params p;
+    DevParams dp;
+    int device = 0;
+    int* d_err = nullptr;  // device-side error flags (bit 0: mass fraction outside (0,1))
+    RefWorkspace ref;
+    FusedWorkspace fused;
+};
+
+extern "C" {
+
+int fhdx_abi_version(void) { return FHDX_ABI_VERSION; }
+
+const char* fhdx_status_string(int s)
+{
+    switch (s) {
+        case FHDX_OK: return "ok";
+        case FHDX_ERR_ARG: return "invalid argument or array shape";
+        case FHDX_ERR_UNSUPPORTED: return "parameters outside the implemented scope";
+        case FHDX_ERR_HIP: return "HIP runtime error";
+        case FHDX_ERR_MASSFRAC: return "mass fraction outside (0,1) in the transport step";
+        case FHDX_ERR_NOMEM: return "device memory budget exhausted";
+        case FHDX_ERR_NONFINITE: return "non-finite state";
+        case FHDX_ERR_BUSY: return "no free slot";
+        default: return "unknown status";
+    }
+}
+
+const char* fhdx_last_error(void) { return g_last_error.c_str(); }
+
+void fhdx_params_default(fhdx_params* p)
+{
+    std::memset(p, 0, sizeof(*p));
+    p->nspecies = 2;
+    for (int d = 0; d < 3; ++d) {
+        p->n_cells[d] = 32;
+        p->dx[d] = 8e-6;
+        p->bc_vel_lo[d] = p->bc_vel_hi[d] = -1;
+        p->bc_mass_lo[d] = p->bc_mass_hi[d] = -1;
+        p->bc_therm_lo[d] = p->bc_therm_hi[d] = -1;
+    }
+    p->ngc = 2;
+    p->algorithm_type = 2;
+    p->advection_type = 2;
+    p->visc_type = 2;
+    p->transport_type = 1;
+    p->stoch_stress_form = 1;
+    for (int n = 0; n < FHDX_MAX_SPECIES; ++n) {
+        p->molmass[n] = 1.;
+        p->diameter[n] = 1.;
+        p->dof[n] = 3.;
+        p->hcv[n] = -1.;
+        p->hcp[n] = -1.;
+        p->e0[n] = 0.;
+    }
+    p->k_B = 1.38064852e-16;
+    p->Runiv = 8.314462175e7;
+    p->variance_coef_mom = p->variance_coef_ener = p->variance_coef_mass = 1.;
+    p->zeta_ratio = -1.;
+}
+
+void fhdx_params_finalize(fhdx_params* p)
+{
+    for (int i = 0; i < p->nspecies && i < FHDX_MAX_SPECIES; ++i) {
+        if (p->hcv[i] < 0.) p->hcv[i] = 0.5 * p->dof[i] * p->Runiv / p->molmass[i];
+        if (p->hcp[i] < 0.) p->hcp[i] = 0.5 * (2. + p->dof[i]) * p->Runiv / p->molmass[i];
+    }
+}
+
+int fhdx_params_check(const fhdx_params* p)
+{
+    if (!p) return FHDX_ERR_ARG;
+    if (p->nspecies < 2 || p->nspecies > 4) {
+        set_error("nspecies must be 2..4 (a single gas is written as two identical species)");
+        return FHDX_ERR_UNSUPPORTED;
+    }
+    for (int d = 0; d < 3; ++d) {
+        if (p->n_cells[d] < 4) {
+            set_error("n_cells must be >= 4 in every direction (3-D only)");
+            return FHDX_ERR_UNSUPPORTED;
+        }
+        if (!(p->dx[d] > 0.)) {
+            set_error("dx must be positive");
+            return FHDX_ERR_ARG;
+        }
+        if (p->bc_vel_lo[d] != -1 || p->bc_vel_hi[d] != -1) {
+            set_error("only fully periodic domains are implemented (bc_vel = -1)");
+            return FHDX_ERR_UNSUPPORTED;
+        }
+    }
+    if (p->ngc < 2) {
+        set_error("ngc must be >= 2 (the stencils reach two cells)");
+        return FHDX_ERR_UNSUPPORTED;
+    }
+    for (int d = 0; d < 3; ++d)
+        if (p->ngc > p->n_cells[d]) {
+            set_error("ngc must not exceed n_cells (periodic images wrap once)");
+            return FHDX_ERR_UNSUPPORTED;
+        }
+    if (p->transport_type != 1) {
+        set_error("only transport_type 1 (Giovangigli) is implemented");
+        return FHDX_ERR_UNSUPPORTED;
+    }
+    if (p->nreaction != 0) {
+        set_error("chemistry (nreaction > 0) is outside this path");
+        return FHDX_ERR_UNSUPPORTED;
+    }
+    if (p->advection_type < 0 || p->advection_type > 2) return FHDX_ERR_ARG;
+    if (p->stoch_stress_form != 0 && p->stoch_stress_form != 1) return FHDX_ERR_ARG;
+    for (int n = 0; n < p->nspecies; ++n)
+        if (!(p->hcv[n] > 0.) || !(p->hcp[n] > 0.) || !(p->molmass[n] > 0.) || !(p->diameter[n] > 0.)) {
+            set_error("species properties must be positive (call fhdx_params_finalize for hcv/hcp)");
+            return FHDX_ERR_ARG;
+        }
+    return FHDX_OK;
+}
+
+}  // extern "C"
+
+// ---- shape checks ----------------------------------------------------------------------
+
+namespace {
+
+bool check_cell(const fhdx_ctx* c, const fhdx_fab& f, int ncomp, const char* what)
+{
+    const int g = c->p.ngc;
+    bool ok = f.p != nullptr && f.ncomp >= ncomp;
+    for (int d = 0; d < 3; ++d) ok = ok && f.lo[d] == -g && f.hi[d] == c->p.n_cells[d] - 1 + g;
+    if (!ok) set_error(std::string("bad cell array shape: ") + what);
+    return ok;
+}
+
+bool check_face(const fhdx_ctx* c, const fhdx_fab& f, int dir, int ncomp, const char* what)
+{
+    bool ok = f.p != nullptr && f.ncomp >= ncomp;
+    for (int d = 0; d < 3; ++d) ok = ok && f.lo[d] == 0 && f.hi[d] == c->p.n_cells[d] - 1 + (d == dir);
+    if (!ok) set_error(std::string("bad face array shape: ") + what);
+    return ok;
+}
+
+bool check_node(const fhdx_ctx* c, const fhdx_fab& f, const char* what)
+{
+    bool ok = f.p != nullptr && f.ncomp >= 1;
+    for (int d = 0; d < 3; ++d) ok = ok && f.lo[d] == 0 && f.hi[d] == c->p.n_cells[d];
+    if (!ok) set_error(std::string("bad node array shape: ") + what);
+    return ok;
+}
+
+hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+int alloc_fab(std::vector<double*>& blocks, fhdx_fab& f, int l0, int l1, int l2, int h0, int h1, int h2, int nc)
+{
+    f.lo[0] = l0; f.lo[1] = l1; f.lo[2] = l2;
+    f.hi[0] = h0; f.hi[1] = h1; f.hi[2] = h2;
+    f.ncomp = nc;
+    f.pad = 0;
+    size_t n = (size_t)(h0 - l0 + 1) * (h1 - l1 + 1) * (h2 - l2 + 1) * nc;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, n * sizeof(double));
+    if (e != hipSuccess) {
+        set_error(std::string("hipMalloc of workspace failed: ") + hipGetErrorString(e));
+        return FHDX_ERR_NOMEM;
+    }
+    (void)hipMemset(p, 0, n * sizeof(double));
+    f.p = (double*)p;
+    blocks.push_back((double*)p);
+    return FHDX_OK;
+}
+
+int ensure_ref_workspace(fhdx_ctx* c)
+{
+    RefWorkspace& w = c->ref;
+    if (w.ready) return FHDX_OK;
+    const int g = c->p.ngc, NS = c->p.nspecies, nv = 5 + NS;
+    const int n0 = c->p.n_cells[0], n1 = c->p.n_cells[1], n2 = c->p.n_cells[2];
+    int r = 0;
+    auto cell = [&](fhdx_fab& f, int nc) {
+        if (!r) r = alloc_fab(w.blocks, f, -g, -g, -g, n0 - 1 + g, n1 - 1 + g, n2 - 1 + g, nc);
+    };
+    auto face = [&](fhdx_fab& f, int d, int nc) {
+        if (!r) r = alloc_fab(w.blocks, f, 0, 0, 0, n0 - 1 + (d == 0), n1 - 1 + (d == 1), n2 - 1 + (d == 2), nc);
+    };
+    auto node = [&](fhdx_fab& f) {
+        if (!r) r = alloc_fab(w.blocks, f, 0, 0, 0, n0, n1, n2, 1);
+    };
+    cell(w.cup, nv);
+    cell(w.cup2, nv);
+    cell(w.prim, 6 + 2 * NS);
+    cell(w.eta, 1);
+    cell(w.zeta, 1);
+    cell(w.kappa, 1);
+    cell(w.chi, NS);
+    cell(w.D, NS * NS);
+    for (int d = 0; d < 3; ++d) {
+        face(w.flux[d], d, nv + 4);
+        face(w.stoch[d], d, nv);
+        face(w.sA[d], d, nv);
+        face(w.sB[d], d, nv);
+        for (int a = 0; a < 3; ++a) node(w.corn[a][d]);
+    }
+    node(w.visccorn);
+    node(w.rancorn);
+    node(w.rA);
+    node(w.rB);
+    if (r) return r;
+    FHDX_HIP_TRY(hipDeviceSynchronize());
+    w.ready = true;
+    return FHDX_OK;
+}
+
+void free_ref_workspace(RefWorkspace& w)
+{
+    for (double* p : w.blocks) (void)hipFree(p);
+    w.blocks.clear();
+    w.ready = false;
+}
+
+long npts(const fhdx_fab& f)
+{
+    return (long)(f.hi[0] - f.lo[0] + 1) * (f.hi[1] - f.lo[1] + 1) * (f.hi[2] - f.lo[2] + 1);
+}
+
+// The reference RK3step with the reference's arrays (timeStep.cpp:11-427).
+int rk3_ref(fhdx_ctx* c, fhdx_fab cu, fhdx_fab cup, fhdx_fab cup2, fhdx_fab prim, fhdx_fab eta,
+            fhdx_fab zeta, fhdx_fab kappa, fhdx_fab chi, fhdx_fab D, const fhdx_fab flux[3],
+            const fhdx_fab stochFlux[3], const fhdx_fab cornx[3], const fhdx_fab corny[3],
+            const fhdx_fab cornz[3], fhdx_fab visccorn, fhdx_fab rancorn, const fhdx_fab sA[3],
+            const fhdx_fab sB[3], fhdx_fab rA, fhdx_fab rB, double dt, hipStream_t s)
+{
+    const DevParams& P = c->dp;
+    const int nv = P.nvars;
+    double ws[3];
+    stage_weights(ws);
+    fhdx_fab in[3] = {cu, cup, cup2};
+    fhdx_fab out[3] = {cup, cup2, cu};
+    FluxArgs a;
+    a.prim = make_arr(prim);
+    a.eta = make_arr(eta);
+    a.zeta = make_arr(zeta);
+    a.kappa = make_arr(kappa);
+    a.chi = make_arr(chi);
+    a.D = make_arr(D);
+    Arr fl[3];
+    for (int d = 0; d < 3; ++d) {
+        a.flux[d] = fl[d] = make_arr(flux[d]);
+        a.ran[d] = make_arr(stochFlux[d]);
+        a.corn[0][d] = make_arr(cornx[d]);
+        a.corn[1][d] = make_arr(corny[d]);
+        a.corn[2][d] = make_arr(cornz[d]);
+    }
+    a.visccorn = make_arr(visccorn);
+    a.rancorn = make_arr(rancorn);
+    a.dt = dt;
+    for (int st = 0; st < 3; ++st) {
+        int r;
+        if ((r = ref_transport(P, a.prim, a.eta, a.zeta, a.kappa, a.chi, a.D, c->d_err, s))) return r;
+        for (int d = 0; d < 3; ++d)
+            if ((r = ref_lincomb(stochFlux[d].p, sA[d].p, sB[d].p, npts(stochFlux[d]), nv, 1, ws[st], s))) return r;
+        if ((r = ref_lincomb(rancorn.p, rA.p, rB.p, npts(rancorn), 1, 0, ws[st], s))) return r;
+        a.cons = make_arr(in[st]);
+        if ((r = ref_calculate_flux(P, a, s))) return r;
+        if ((r = ref_stage(P, make_arr(out[st]), make_arr(cu), make_arr(in[st]), fl, dt, st, s))) return r;
+        if ((r = ref_c2p(P, a.prim, make_arr(out[st]), s))) return r;
+        if ((r = ref_fill_boundary(P, make_arr(out[st]), nv, s))) return r;
+        if ((r = ref_fill_boundary(P, a.prim, P.nprim, s))) return r;
+    }
+    return FHDX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fhdx_ctx_create(const fhdx_params* p, fhdx_ctx** out)
+{
+    if (!out) return FHDX_ERR_ARG;
+    *out = nullptr;
+    int r = fhdx_params_check(p);
+    if (r) return r;
+    fhdx_ctx* c = new fhdx_ctx();
+    c->p = *p;
+    c->dp = make_dev_params(*p);
+    hipError_t e = hipGetDevice(&c->device);
+    if (e == hipSuccess) e = hipMalloc((void**)&c->d_err, sizeof(int));
+    if (e == hipSuccess) e = hipMemset(c->d_err, 0, sizeof(int));
+    if (e != hipSuccess) {
+        set_error(std::string("context setup failed: ") + hipGetErrorString(e));
+        if (c->d_err) (void)hipFree(c->d_err);
+        delete c;
+        return FHDX_ERR_HIP;
+    }
+    *out = c;
+    return FHDX_OK;
+}
+
+int fhdx_ctx_destroy(fhdx_ctx* c)
+{
+    if (!c) return FHDX_OK;
+    (void)hipDeviceSynchronize();
+    free_ref_workspace(c->ref);
+    fused_free(c->fused);
+    if (c->d_err) (void)hipFree(c->d_err);
+    delete c;
+    return FHDX_OK;
+}
+
+int fhdx_ctx_check(fhdx_ctx* c, void* stream)
+{
+    if (!c) return FHDX_ERR_ARG;
+    FHDX_HIP_TRY(hipStreamSynchronize(S(stream)));
+    int h = 0;
+    FHDX_HIP_TRY(hipMemcpy(&h, c->d_err, sizeof(int), hipMemcpyDeviceToHost));
+    if (h) {
+        FHDX_HIP_TRY(hipMemset(c->d_err, 0, sizeof(int)));
+        if (h & 1) {
+            set_error("mass fraction outside (0,1) met in the transport step");
+            return FHDX_ERR_MASSFRAC;
+        }
+        return FHDX_ERR_NONFINITE;
+    }
+    return FHDX_OK;
+}
+
+int fhdx_cons_to_prim(fhdx_ctx* c, fhdx_fab prim, fhdx_fab cons, void* stream)
+{
+    if (!c) return FHDX_ERR_ARG;
+    if (!check_cell(c, prim, c->dp.nprim, "prim") || !check_cell(c, cons, c->dp.nvars, "cons")) return FHDX_ERR_ARG;
+    return ref_c2p(c->dp, make_arr(prim), make_arr(cons), S(stream));
+}
+
+int fhdx_transport(fhdx_ctx* c, fhdx_fab prim, fhdx_fab eta, fhdx_fab zeta, fhdx_fab kappa, fhdx_fab chi,
+                   fhdx_fab D, void* stream)
+{
+    if (!c) return FHDX_ERR_ARG;
+    const int NS = c->p.nspecies;
+    if (!check_cell(c, prim, c->dp.nprim, "prim") || !check_cell(c, eta, 1, "eta") ||
+        !check_cell(c, zeta, 1, "zeta") || !check_cell(c, kappa, 1, "kappa") || !check_cell(c, chi, NS, "chi") ||
+        !check_cell(c, D, NS * NS, "D"))
+        return FHDX_ERR_ARG;
+    return ref_transport(c->dp, make_arr(prim), make_arr(eta), make_arr(zeta), make_arr(kappa), make_arr(chi),
+                         make_arr(D), c->d_err, S(stream));
+}
+
+int fhdx_fill_boundary(fhdx_ctx* c, fhdx_fab mf, void* stream)
+{
+    if (!c) return FHDX_ERR_ARG;
+    if (!check_cell(c, mf, 1, "mf")) return FHDX_ERR_ARG;
+    return ref_fill_boundary(c->dp, make_arr(mf), mf.ncomp, S(stream));
+}
+
+int fhdx_calculate_flux(fhdx_ctx* c, fhdx_fab cons, fhdx_fab prim, fhdx_fab eta, fhdx_fab zeta,
+                        fhdx_fab kappa, fhdx_fab chi, fhdx_fab D, const fhdx_fab flux[3],
+                        const fhdx_fab stochFlux[3], const fhdx_fab cornx[3], const fhdx_fab corny[3],
+                        const fhdx_fab cornz[3], fhdx_fab visccorn, fhdx_fab rancorn, double dt, void* stream)
+{
+    if (!c || !flux || !stochFlux || !cornx || !corny || !cornz) return FHDX_ERR_ARG;
+    const int NS = c->p.nspecies, nv = c->dp.nvars;
+    bool ok = check_cell(c, cons, nv, "cons") && check_cell(c, prim, c->dp.nprim, "prim") &&
+              check_cell(c, eta, 1, "eta") && check_cell(c, zeta, 1, "zeta") && check_cell(c, kappa, 1, "kappa") &&
+              check_cell(c, chi, NS, "chi") && check_cell(c, D, NS * NS, "D") &&
+              check_node(c, visccorn, "visccorn") && check_node(c, rancorn, "rancorn");
+    for (int d = 0; ok && d < 3; ++d)
+        ok = check_face(c, flux[d], d, nv + 4, "flux") && check_face(c, stochFlux[d], d, nv, "stochFlux") &&
+             check_node(c, cornx[d], "cornx") && check_node(c, corny[d], "corny") && check_node(c, cornz[d], "cornz");
+    if (!ok) return FHDX_ERR_ARG;
+    if (!(dt > 0.)) return FHDX_ERR_ARG;
+    FluxArgs a;
+    a.cons = make_arr(cons);
+    a.prim = make_arr(prim);
+    a.eta = make_arr(eta);
+    a.zeta = make_arr(zeta);
+    a.kappa = make_arr(kappa);
+    a.chi = make_arr(chi);
+    a.D = make_arr(D);
+    for (int d = 0; d < 3; ++d) {
+        a.flux[d] = make_arr(flux[d]);
+        a.ran[d] = make_arr(stochFlux[d]);
+        a.corn[0][d] = make_arr(cornx[d]);
+        a.corn[1][d] = make_arr(corny[d]);
+        a.corn[2][d] = make_arr(cornz[d]);
+    }
+    a.visccorn = make_arr(visccorn);
+    a.rancorn = make_arr(rancorn);
+    a.dt = dt;
+    return ref_calculate_flux(c->dp, a, S(stream));
+}
+
+int fhdx_fill_noise(fhdx_ctx* c, uint64_t seed, uint64_t step, uint32_t instance, const fhdx_fab stochA[3],
+                    const fhdx_fab stochB[3], fhdx_fab rancornA, fhdx_fab rancornB, void* stream)
+{
+    if (!c || !stochA || !stochB) return FHDX_ERR_ARG;
+    const int nv = c->dp.nvars;
+    bool ok = check_node(c, rancornA, "rancornA") && check_node(c, rancornB, "rancornB");
+    for (int d = 0; ok && d < 3; ++d)
+        ok = check_face(c, stochA[d], d, nv, "stochA") && check_face(c, stochB[d], d, nv, "stochB");
+    if (!ok) return FHDX_ERR_ARG;
+    hipStream_t s = S(stream);
+    Arr A[3], B[3];
+    for (int d = 0; d < 3; ++d) {
+        A[d] = make_arr(stochA[d]);
+        B[d] = make_arr(stochB[d]);
+        int r = fill_value(stochA[d].p, A[d].sn * stochA[d].ncomp, 0.0, s);
+        if (!r) r = fill_value(stochB[d].p, B[d].sn * stochB[d].ncomp, 0.0, s);
+        if (r) return r;
+    }
+    return ref_fill_noise(c->dp, make_noise_key(seed, step, instance), A, B, make_arr(rancornA),
+                          make_arr(rancornB), s);
+}
+
+int fhdx_rk3_step_ref(fhdx_ctx* c, fhdx_fab cu, fhdx_fab cup, fhdx_fab cup2, fhdx_fab prim, fhdx_fab eta,
+                      fhdx_fab zeta, fhdx_fab kappa, fhdx_fab chi, fhdx_fab D, const fhdx_fab flux[3],
+                      const fhdx_fab stochFlux[3], const fhdx_fab cornx[3], const fhdx_fab corny[3],
+                      const fhdx_fab cornz[3], fhdx_fab visccorn, fhdx_fab rancorn, const fhdx_fab* stochA,
+                      const fhdx_fab* stochB, const fhdx_fab* rancornA, const fhdx_fab* rancornB, double dt,
+                      uint64_t seed, uint64_t step, void* stream)
+{
+    if (!c || !flux || !stochFlux || !cornx || !corny || !cornz) return FHDX_ERR_ARG;
+    const int NS = c->p.nspecies, nv = c->dp.nvars;
+    bool ok = check_cell(c, cu, nv, "cu") && check_cell(c, cup, nv, "cup") && check_cell(c, cup2, nv, "cup2") &&
+              check_cell(c, prim, c->dp.nprim, "prim") && check_cell(c, eta, 1, "eta") &&
+              check_cell(c, zeta, 1, "zeta") && check_cell(c, kappa, 1, "kappa") && check_cell(c, chi, NS, "chi") &&
+              check_cell(c, D, NS * NS, "D") && check_node(c, visccorn, "visccorn") &&
+              check_node(c, rancorn, "rancorn");
+    for (int d = 0; ok && d < 3; ++d)
+        ok = check_face(c, flux[d], d, nv + 4, "flux") && check_face(c, stochFlux[d], d, nv, "stochFlux") &&
+             check_node(c, cornx[d], "cornx") && check_node(c, corny[d], "corny") && check_node(c, cornz[d], "cornz");
+    if (!ok || !(dt > 0.)) return FHDX_ERR_ARG;
+    hipStream_t s = S(stream);
+    const bool injected = stochA != nullptr;
+    if (injected) {
+        if (!stochB || !rancornA || !rancornB) return FHDX_ERR_ARG;
+        for (int d = 0; d < 3; ++d)
+            if (!check_face(c, stochA[d], d, nv, "stochA") || !check_face(c, stochB[d], d, nv, "stochB"))
+                return FHDX_ERR_ARG;
+        if (!check_node(c, *rancornA, "rancornA") || !check_node(c, *rancornB, "rancornB")) return FHDX_ERR_ARG;
+        return rk3_ref(c, cu, cup, cup2, prim, eta, zeta, kappa, chi, D, flux, stochFlux, cornx, corny, cornz,
+                       visccorn, rancorn, stochA, stochB, *rancornA, *rancornB, dt, s);
+    }
+    int r = ensure_ref_workspace(c);
+    if (r) return r;
+    RefWorkspace& w = c->ref;
+    if ((r = fhdx_fill_noise(c, seed, step, 0, w.sA, w.sB, w.rA, w.rB, stream))) return r;
+    return rk3_ref(c, cu, cup, cup2, prim, eta, zeta, kappa, chi, D, flux, stochFlux, cornx, corny, cornz,
+                   visccorn, rancorn, w.sA, w.sB, w.rA, w.rB, dt, s);
+}
+
+int fhdx_rk3_step(fhdx_ctx* c, fhdx_fab cu, fhdx_fab prim, double dt, uint64_t seed, uint64_t step, void* stream)
+{
+    if (!c) return FHDX_ERR_ARG;
+    if (!check_cell(c, cu, c->dp.nvars, "cu") || !check_cell(c, prim, c->dp.nprim, "prim") || !(dt > 0.))
+        return FHDX_ERR_ARG;
+    return fused_rk3_step(c->p, c->dp, c->fused, cu, prim, dt, make_noise_key(seed, step, 0), nullptr, nullptr,
+                          nullptr, nullptr, c->d_err, S(stream));
+}
+
+int fhdx_rk3_step_injected(fhdx_ctx* c, fhdx_fab cu, fhdx_fab prim, const fhdx_fab stochA[3],
+                           const fhdx_fab stochB[3], fhdx_fab rancornA, fhdx_fab rancornB, double dt, void* stream)
+{
+    if (!c || !stochA || !stochB) return FHDX_ERR_ARG;
+    const int nv = c->dp.nvars;
+    bool ok = check_cell(c, cu, nv, "cu") && check_cell(c, prim, c->dp.nprim, "prim") &&
+              check_node(c, rancornA, "rancornA") && check_node(c, rancornB, "rancornB") && dt > 0.;
+    for (int d = 0; ok && d < 3; ++d)
+        ok = check_face(c, stochA[d], d, nv, "stochA") && check_face(c, stochB[d], d, nv, "stochB");
+    if (!ok) return FHDX_ERR_ARG;
+    return fused_rk3_step(c->p, c->dp, c->fused, cu, prim, dt, make_noise_key(0, 0, 0), stochA, stochB, &rancornA,
+                          &rancornB, c->d_err, S(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,30 @@
+// kernels.hpp -- host launchers of the device kernels (ref_kernels.hip, fused_kernels.hip).
+#pragma once
+
+#include "common.hpp"
+
+namespace fhdx {
+
+// Views of every array calculateFlux touches (flux.cpp:4-16).
+struct FluxArgs {
+    Arr cons, prim, eta, zeta, kappa, chi, D;
+    Arr flux[3], ran[3];
+    Arr corn[3][3];  // [derivative direction][velocity component]: cornx/corny/cornz
+    Arr visccorn, rancorn;
+    double dt;
+};
+
+int ref_c2p(const DevParams& P, Arr prim, Arr cons, hipStream_t s);
+int ref_transport(const DevParams& P, Arr prim, Arr eta, Arr zeta, Arr kappa, Arr chi, Arr Dm, int* err,
+                  hipStream_t s);
+int ref_fill_boundary(const DevParams& P, Arr a, int nc, hipStream_t s);
+int fill_value(double* p, long n, double v, hipStream_t s);
+int ref_calculate_flux(const DevParams& P, const FluxArgs& a, hipStream_t s);
+int ref_stage(const DevParams& P, Arr out, Arr u0, Arr us, const Arr flux[3], double dt, int mode,
+              hipStream_t s);
+int ref_lincomb(double* dst, const double* a, const double* b, long npts, int nc, int c0, double w,
+                hipStream_t s);
+int ref_fill_noise(const DevParams& P, const NoiseKey& K, const Arr A[3], const Arr B[3], Arr rA, Arr rB,
+                   hipStream_t s);
+
+}  // namespace fhdx

@@ -1,0 +1,463 @@
+// physics.hpp -- device-side FP64 physics of the compressible FHD step (gfx950).
+//
+// Every routine keeps the reference's floating-point operation order (left-to-right sums,
+// the same parenthesisation, a division wherever the reference divides) and the library is
+// compiled with -ffp-contract=off, so results are bit-identical to a faithful CPU evaluation
+// of the reference formulas.  That matters: the species-noise Cholesky factors a matrix that
+// is singular in exact arithmetic (its last pivot is rounding noise), so any reordering moves
+// rho*Y_k, j and rho*E by ~1e-11 relative per step.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/fhdx.h"
+
+namespace fhdx {
+
+constexpr int MS = FHDX_MAX_SPECIES;
+
+// Kernel-argument copy of the run parameters plus per-call constants.
+struct DevParams {
+    int ns, nvars, nprim, ngc;
+    int n[3];
+    int algorithm_type, advection_type, visc_type, stoch_on;
+    double dx[3];
+    double molmass[MS], diameter[MS], hcv[MS], hcp[MS], e0[MS];
+    double k_B, Runiv, zeta_ratio;
+    double grav[3];
+    double var_mom, var_ener, var_mass;  // sqrt(coef*coef), the MultiFabFillRandom scale
+};
+
+// Strided view of one FAB (AMReX Array4 layout: x fastest, then y, z, component).
+struct Arr {
+    double* p;
+    int lo0, lo1, lo2;
+    int sy, sz;
+    long sn;
+    __host__ __device__ inline double& operator()(int i, int j, int k, int c = 0) const
+    {
+        return p[(long)((i - lo0) + (j - lo1) * sy + (k - lo2) * sz) + (long)c * sn];
+    }
+};
+
+__host__ __device__ inline double dmax(double a, double b) { return a > b ? a : b; }
+__host__ __device__ inline double dmin(double a, double b) { return a < b ? a : b; }
+// amrex::max(0., amrex::min(1., y))
+__host__ __device__ inline double clamp01(double y) { return dmax(0., dmin(1., y)); }
+
+// ---- EOS (compressible_functions.H:138-199, 1497-1520) --------------------------------
+
+template <int NS>
+__device__ inline double eos_temperature(const DevParams& P, double energy, const double* Yk)
+{
+    double cvmix = 0., e0mix = 0.;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+        cvmix += Yk[n] * P.hcv[n];
+        e0mix += Yk[n] * P.e0[n];
+    }
+    return (energy - e0mix) / cvmix;
+}
+
+template <int NS>
+__device__ inline double eos_pressure(const DevParams& P, const double* Yk, double rho, double T)
+{
+    double molmix = 0.;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) molmix += Yk[n] / P.molmass[n];
+    molmix = 1. / molmix;
+    return rho * (P.Runiv / molmix) * T;
+}
+
+template <int NS>
+__device__ inline double eos_energy(const DevParams& P, const double* Yk, double T)
+{
+    double cvmix = 0., e0mix = 0.;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+        cvmix += Yk[n] * P.hcv[n];
+        e0mix += Yk[n] * P.e0[n];
+    }
+    return e0mix + T * cvmix;
+}
+
+// conservedToPrimitive for one cell (conservedPrimitiveConversions.cpp:57-88).
+// c[0..nvars), pr[0..nprim).
+template <int NS>
+__device__ inline void cons_to_prim_cell(const DevParams& P, const double* c, double* pr)
+{
+    double Yk[NS], Yf[NS];
+    pr[0] = c[0];
+    pr[1] = c[1] / c[0];
+    pr[2] = c[2] / c[0];
+    pr[3] = c[3] / c[0];
+    double vsqr = pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3];
+    double intenergy = c[4] / c[0] - 0.5 * vsqr;
+    double sumYk = 0.;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+        Yk[n] = c[5 + n] / c[0];
+        Yf[n] = clamp01(Yk[n]);
+        sumYk += Yf[n];
+    }
+#pragma unroll
+    for (int n = 0; n < NS; ++n) Yf[n] /= sumYk;
+    pr[4] = eos_temperature<NS>(P, intenergy, Yf);
+    // GetMolfrac of the raw mass fractions
+    double molmix = 0.;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) molmix += Yk[n] / P.molmass[n];
+    molmix = 1. / molmix;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+        pr[6 + n] = Yk[n];
+        pr[6 + NS + n] = Yk[n] * molmix / P.molmass[n];
+    }
+    pr[5] = eos_pressure<NS>(P, Yk, pr[0], pr[4]);
+}
+
+// ---- transport: IdealMixtureTransportGIO + D_GIO1 (compressible_functions.H:647-1352) ----
+
+// Gaussian elimination without pivoting and back substitution (H:1254-1271).
+template <int NS>
+__device__ inline void gio_solve(double (&a)[NS][NS], double* Bi)
+{
+#pragma unroll
+    for (int j = 0; j < NS - 1; ++j)
+#pragma unroll
+        for (int i = j + 1; i < NS; ++i) {
+            double fact1 = a[i][j] / a[j][j];
+#pragma unroll
+            for (int k = 0; k < NS; ++k) a[i][k] -= fact1 * a[j][k];
+            Bi[i] -= fact1 * Bi[j];
+        }
+    Bi[NS - 1] /= a[NS - 1][NS - 1];
+#pragma unroll
+    for (int i = NS - 2; i > -1; --i) {
+        double sum1 = 0.0;
+#pragma unroll
+        for (int j = NS - 1; j > i; --j) sum1 += a[i][j] * Bi[j];
+        Bi[i] = (Bi[i] - sum1) / a[i][i];
+    }
+}
+
+struct Transport {
+    double eta, zeta, kappa;
+};
+
+// Transport coefficients of one cell (transCoeffs.cpp:135-178).  chi[NS], D[NS*NS] out.
+// Returns true if a mass fraction lies outside (0,1) (the reference aborts there).
+template <int NS>
+__device__ inline bool transport_cell(const DevParams& P, double rho, double T, double p,
+                                      const double* Yraw, Transport& tr, double* chi, double* D)
+{
+    double Yk[NS];
+    double sumYk = 0.;
+    bool bad = false;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+        double y = Yraw[n];
+        bad |= (y <= 0.0) | (y >= 1.0);
+        Yk[n] = clamp01(y);
+        sumYk += Yk[n];
+    }
+#pragma unroll
+    for (int n = 0; n < NS; ++n) Yk[n] /= sumYk;
+
+    const double pi = 3.1415926535897932;
+    double mm[NS], Xk[NS], Xkp[NS], Ykp[NS], eta1[NS], rhs[NS], Bi[NS];
+    double mu[NS][NS], diam[NS][NS], Dbin[NS][NS], amat[NS][NS];
+#pragma unroll
+    for (int n = 0; n < NS; ++n) mm[n] = P.molmass[n] * (P.k_B / P.Runiv);
+    double mbar = 0.0;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) mbar += Yk[n] / mm[n];
+    mbar = 1.0 / mbar;
+    double Xksum = 0.0;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+        Xk[n] = Yk[n] * mbar / mm[n];
+        Xksum += Xk[n];
+    }
+#pragma unroll
+    for (int n = 0; n < NS; ++n) Xkp[n] = Xk[n] + 1.0e-16 * (Xksum / double(NS) - Xk[n]);
+#pragma unroll
+    for (int n = 0; n < NS; ++n) Ykp[n] = Xkp[n] * mm[n] / mbar;
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            diam[i][j] = 0.5 * (P.diameter[i] + P.diameter[j]);
+            mu[i][j] = mm[i] * mm[j] / (mm[i] + mm[j]);
+        }
+    const double kT = P.k_B * T;
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+#pragma unroll
+        for (int j = 0; j < NS; ++j)
+            Dbin[i][j] = (3.0 / 16.0) * sqrt(2.0 * pi * kT * kT * kT / mu[i][j]) /
+                         (p * pi * diam[i][j] * diam[i][j]);
+    const double AKL = 1.0, BKL = 1.0, CKL = 1.0;
+
+    // viscosity
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+        eta1[i] = 5.0 / (16.0 * diam[i][i] * diam[i][i]) * sqrt(mm[i] * P.k_B * T / pi);
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        rhs[i] = Xkp[i];
+        Bi[i] = rhs[i];
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            if (i == j) {
+                amat[i][i] = Xkp[i] * Xkp[i] / eta1[i];
+#pragma unroll
+                for (int k = 0; k < NS; ++k)
+                    if (k != i)
+                        amat[i][i] += 2.0 * kT * Xkp[i] * Xkp[k] /
+                                      (p * Dbin[i][k] * (mm[i] + mm[k])) *
+                                      (1.0 + 0.6 * mm[k] / mm[i] * AKL);
+            } else {
+                amat[i][j] = 2.0 * kT * Xkp[i] * Xkp[j] / (p * Dbin[i][j] * (mm[i] + mm[j])) *
+                             (0.6 * AKL - 1.0);
+            }
+        }
+    }
+    gio_solve<NS>(amat, Bi);
+    double e = 0.0;
+#pragma unroll
+    for (int i = 0; i < NS; ++i) e += Xkp[i] * Bi[i];
+    tr.eta = e;
+    tr.zeta = 0.;
+
+    // thermal conductivity
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        rhs[i] = 2.5 * Xkp[i];
+        Bi[i] = rhs[i];
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            if (i == j) {
+                amat[i][i] = Xkp[i] * Xkp[i] / Dbin[i][j] * 2.0 * AKL;
+#pragma unroll
+                for (int k = 0; k < NS; ++k)
+                    if (k != i)
+                        amat[i][i] += Xkp[i] * Xkp[k] / Dbin[i][k] * mu[i][k] / (mm[i] + mm[k]) *
+                                      (7.5 * mm[i] / mm[k] + 6.25 * mm[k] / mm[i] -
+                                       3.0 * mm[k] / mm[i] * BKL + 4.0 * AKL);
+            } else {
+                amat[i][j] = -1.0 * Xkp[i] * Xkp[j] / Dbin[i][j] * mu[i][j] / (mm[i] + mm[j]) *
+                             (55.0 / 4.0 - 3.0 * BKL - 4.0 * AKL);
+            }
+        }
+    }
+    gio_solve<NS>(amat, Bi);
+    double kap = 0.0;
+#pragma unroll
+    for (int i = 0; i < NS; ++i) kap += rhs[i] * Bi[i];
+    kap *= p / T;
+    tr.kappa = kap;
+
+    // thermal diffusion ratios
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            if (i == j) {
+                amat[i][i] = 0.0;
+#pragma unroll
+                for (int k = 0; k < NS; ++k)
+                    if (k != i)
+                        amat[i][i] -= 0.5 * Xkp[k] / Dbin[i][k] * mm[k] / (mm[i] + mm[k]) *
+                                      (6.0 * CKL - 5.0);
+            } else {
+                amat[i][j] = 0.5 * Xkp[j] / Dbin[i][j] * mm[i] / (mm[i] + mm[j]) * (6.0 * CKL - 5.0);
+            }
+        }
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        double c = 0.0;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) c += amat[i][j] * Bi[j];
+        chi[i] = c;
+    }
+
+    // D_GIO1 (H:647-778)
+    {
+        double Di[NS], Minv[NS], Mmat[NS];
+        double Pm[NS][NS], Zm[NS][NS], PJ[NS][NS], m1[NS][NS], m2[NS][NS], Diff[NS][NS];
+#pragma unroll
+        for (int i = 0; i < NS; ++i) {
+            double term2 = 0.;
+#pragma unroll
+            for (int j = 0; j < NS; ++j)
+                if (j != i) term2 = term2 + Xkp[j] / Dbin[i][j];
+            Di[i] = (1. - Ykp[i]) / term2;
+        }
+#pragma unroll
+        for (int i = 0; i < NS; ++i) {
+            Mmat[i] = Xkp[i] / Di[i];
+            Minv[i] = Di[i] / Xkp[i];
+        }
+#pragma unroll
+        for (int i = 0; i < NS; ++i)
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+                Pm[i][j] = -Ykp[j];
+                if (i == j) Pm[i][j] = Pm[i][j] + 1.;
+            }
+#pragma unroll
+        for (int i = 0; i < NS; ++i)
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+                double Delta;
+                if (i == j) {
+                    double term1 = 0.;
+#pragma unroll
+                    for (int k = 0; k < NS; ++k)
+                        if (k != i) term1 = term1 + Xkp[i] * Xkp[k] / Dbin[i][k];
+                    Delta = term1;
+                } else {
+                    Delta = -Xkp[i] * Xkp[j] / Dbin[i][j];
+                }
+                Zm[i][j] = -Delta;
+            }
+#pragma unroll
+        for (int i = 0; i < NS; ++i) Zm[i][i] = Zm[i][i] + Mmat[i];
+        // Jmat = Minv_i * Zmat_ij, PJ = P * J
+#pragma unroll
+        for (int i = 0; i < NS; ++i)
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+                double s = 0.;
+#pragma unroll
+                for (int k = 0; k < NS; ++k) s = s + Pm[i][k] * (Minv[k] * Zm[k][j]);
+                PJ[i][j] = s;
+            }
+#pragma unroll
+        for (int i = 0; i < NS; ++i)
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+                double scr = 0.;
+#pragma unroll
+                for (int k = 0; k < NS; ++k) scr = scr + Pm[i][k] * Minv[k] * Pm[j][k];
+                m2[i][j] = scr;
+                Diff[i][j] = scr;
+            }
+#pragma unroll
+        for (int ll = 0; ll < 3; ++ll) {
+#pragma unroll
+            for (int i = 0; i < NS; ++i)
+#pragma unroll
+                for (int j = 0; j < NS; ++j) {
+                    double scr = 0.;
+#pragma unroll
+                    for (int k = 0; k < NS; ++k) scr = scr + PJ[i][k] * Diff[k][j];
+                    m1[i][j] = scr + m2[i][j];
+                }
+#pragma unroll
+            for (int i = 0; i < NS; ++i)
+#pragma unroll
+                for (int j = 0; j < NS; ++j) Diff[i][j] = m1[i][j];
+        }
+#pragma unroll
+        for (int i = 0; i < NS; ++i)
+#pragma unroll
+            for (int j = 0; j < NS; ++j) D[j * NS + i] = Diff[i][j] * Ykp[i];
+    }
+#pragma unroll
+    for (int n = 0; n < NS * NS; ++n) D[n] *= rho;
+    if (P.visc_type == 3 || P.visc_type == -3) tr.zeta = P.zeta_ratio * tr.eta;
+    return bad;
+}
+
+// CholeskyDecomp (compressible_functions.H:460-509): lower factor in sqda (row-major NSxNS).
+// Entries the reference writes above the diagonal never feed the result and are skipped.
+template <int NS>
+__device__ inline void cholesky(double* a, double* sqda)
+{
+    double p[NS];
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        bool ising = false;
+#pragma unroll
+        for (int j = i; j < NS; ++j) {
+            double sum1 = a[i * NS + j];
+#pragma unroll
+            for (int k = i - 1; k >= 0; --k) sum1 = sum1 - a[i * NS + k] * a[j * NS + k];
+            if (i == j) {
+                if (sum1 <= 0.) {
+                    p[i] = 0.;
+                    ising = true;
+                } else {
+                    p[i] = sqrt(sum1);
+                }
+            }
+            a[j * NS + i] = ising ? 0. : sum1 / p[i];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+#pragma unroll
+        for (int j = 0; j < NS; ++j) sqda[i * NS + j] = (j < i) ? a[i * NS + j] : (j == i ? p[i] : 0.);
+}
+
+// ---- counter-based noise: Philox4x32-10 (Salmon et al., SC'11) + Box-Muller ------------
+
+struct U4 {
+    uint32_t x, y, z, w;
+};
+
+__host__ __device__ inline U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+        U4 o;
+        o.x = (uint32_t)(p1 >> 32) ^ c.y ^ k0;
+        o.y = (uint32_t)p1;
+        o.z = (uint32_t)(p0 >> 32) ^ c.w ^ k1;
+        o.w = (uint32_t)p0;
+        c = o;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return c;
+}
+
+// Two N(0,1) values from one Philox block: 53-bit u1 in (0,1), u2 in [0,1).
+__device__ inline void normal_pair(U4 r, double& z0, double& z1)
+{
+    const double two_m53 = 1.1102230246251565e-16;
+    uint64_t a = ((((uint64_t)r.x) << 32) | r.y) >> 11;
+    uint64_t b = ((((uint64_t)r.z) << 32) | r.w) >> 11;
+    double u1 = ((double)a + 0.5) * two_m53;
+    double u2 = (double)b * two_m53;
+    double rad = sqrt(-2.0 * log(u1));
+    double s, c;
+    sincospi(2.0 * u2, &s, &c);
+    z0 = rad * c;
+    z1 = rad * s;
+}
+
+// Noise keys: field = 16*dir + comp for face components 1..nvars-1, 48 for the node field.
+constexpr uint32_t NODE_FIELD = 48u;
+
+struct NoiseKey {
+    uint32_t k0, k1;     // seed
+    uint32_t step_lo;    // step
+    uint32_t field_hi;   // (step >> 32) << 8
+    uint32_t instance;
+};
+
+// A/B pair of the noise value at wrapped linear index `lin` of `field`.
+__device__ inline void noise_ab(const NoiseKey& K, uint32_t field, uint32_t lin, double& a,
+                                double& b)
+{
+    U4 c{lin, K.step_lo, field | K.field_hi, K.instance};
+    normal_pair(philox4x32_10(c, K.k0, K.k1), a, b);
+}
+
+}  // namespace fhdx

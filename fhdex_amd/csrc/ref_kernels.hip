@@ -1,0 +1,718 @@
+// ref_kernels.hip -- the reference-shaped kernels of calculateFlux / RK3step on gfx950.
+//
+// One kernel per pass of the reference (transport, stochastic, diffusive, corner, corner-to-
+// face, hyperbolic fluxes, stage combination, noise), each thread owning one cell, face or
+// node.  These back the drop-in entry points that fill the reference's own intermediate
+// arrays (fhdx_calculate_flux, fhdx_rk3_step_ref).  Arithmetic follows the reference
+// operation by operation (see physics.hpp for why that is required).
+#include "kernels.hpp"
+
+namespace fhdx {
+
+constexpr int BLK = 256;
+
+// Edge-viscosity summation order per face direction and edge (pp, mp, pm, mm): offsets of the
+// eta cell then of the T cell relative to the face index, flux.cpp:117-135 (x), 481-499 (y),
+// 847-865 (z, including the eta(k-2)*T(k) terms of 857-858).
+__constant__ signed char c_mu_order[3][4][8][6] = {
+    {{{0,0,0,0,0,0},{-1,0,0,-1,0,0},{0,1,0,0,1,0},{-1,1,0,-1,1,0},{0,0,1,0,0,1},{-1,0,1,-1,0,1},{0,1,1,0,1,1},{-1,1,1,-1,1,1}},
+     {{0,-1,0,0,-1,0},{-1,-1,0,-1,-1,0},{0,0,0,0,0,0},{-1,0,0,-1,0,0},{0,-1,1,0,-1,1},{-1,-1,1,-1,-1,1},{0,0,1,0,0,1},{-1,0,1,-1,0,1}},
+     {{0,0,-1,0,0,-1},{-1,0,-1,-1,0,-1},{0,1,-1,0,1,-1},{-1,1,-1,-1,1,-1},{0,0,0,0,0,0},{-1,0,0,-1,0,0},{0,1,0,0,1,0},{-1,1,0,-1,1,0}},
+     {{0,-1,-1,0,-1,-1},{-1,-1,-1,-1,-1,-1},{0,0,-1,0,0,-1},{-1,0,-1,-1,0,-1},{0,-1,0,0,-1,0},{-1,-1,0,-1,-1,0},{0,0,0,0,0,0},{-1,0,0,-1,0,0}}},
+    {{{1,-1,0,1,-1,0},{0,-1,0,0,-1,0},{1,0,0,1,0,0},{0,0,0,0,0,0},{1,-1,1,1,-1,1},{0,-1,1,0,-1,1},{1,0,1,1,0,1},{0,0,1,0,0,1}},
+     {{-1,0,0,-1,0,0},{0,0,0,0,0,0},{-1,-1,0,-1,-1,0},{0,-1,0,0,-1,0},{-1,0,1,-1,0,1},{0,0,1,0,0,1},{-1,-1,1,-1,-1,1},{0,-1,1,0,-1,1}},
+     {{1,-1,-1,1,-1,-1},{0,-1,-1,0,-1,-1},{1,0,-1,1,0,-1},{0,0,-1,0,0,-1},{1,-1,0,1,-1,0},{0,-1,0,0,-1,0},{1,0,0,1,0,0},{0,0,0,0,0,0}},
+     {{-1,0,-1,-1,0,-1},{0,0,-1,0,0,-1},{-1,-1,-1,-1,-1,-1},{0,-1,-1,0,-1,-1},{-1,0,0,-1,0,0},{0,0,0,0,0,0},{-1,-1,0,-1,-1,0},{0,-1,0,0,-1,0}}},
+    {{{1,0,-1,1,0,-1},{0,0,-1,0,0,-1},{1,1,-1,1,1,-1},{0,1,-1,0,1,-1},{1,0,0,1,0,0},{0,0,0,0,0,0},{1,1,0,1,1,0},{0,1,0,0,1,0}},
+     {{-1,1,-1,-1,1,-1},{0,1,-1,0,1,-1},{-1,0,-1,-1,0,-1},{0,0,-1,0,0,-1},{-1,1,0,-1,1,0},{0,1,0,0,1,0},{-1,0,0,-1,0,0},{0,0,0,0,0,0}},
+     {{1,0,0,1,0,0},{0,0,-2,0,0,0},{1,-1,0,1,-1,0},{0,-1,-2,0,-1,0},{1,0,-1,1,0,-1},{0,0,-1,0,0,-1},{1,-1,-1,1,-1,-1},{0,-1,-1,0,-1,-1}},
+     {{-1,-1,0,-1,-1,0},{0,-1,0,0,-1,0},{-1,0,0,-1,0,0},{0,0,0,0,0,0},{-1,-1,-1,-1,-1,-1},{0,-1,-1,0,-1,-1},{-1,0,-1,-1,0,-1},{0,0,-1,0,0,-1}}}};
+
+// Node (rancorn) offsets of the pp, mp, pm, mm edges (flux.cpp:277-280, 644-647, 1011-1014).
+__constant__ signed char c_rc_node[3][4][3] = {{{0, 1, 1}, {0, 0, 1}, {0, 1, 0}, {0, 0, 0}},
+                                               {{1, 0, 1}, {0, 0, 1}, {1, 0, 0}, {0, 0, 0}},
+                                               {{1, 1, 0}, {0, 1, 0}, {1, 0, 0}, {0, 0, 0}}};
+
+__device__ inline void face_index(long t, int fx, int fy, int& i, int& j, int& k)
+{
+    i = (int)(t % fx);
+    long r = t / fx;
+    j = (int)(r % fy);
+    k = (int)(r / fy);
+}
+
+// ---- cell kernels -------------------------------------------------------------------------
+
+template <int NS>
+__global__ void __launch_bounds__(BLK) k_c2p(DevParams P, Arr prim, Arr cons)
+{
+    long t = (long)blockIdx.x * BLK + threadIdx.x;
+    if (t >= (long)P.n[0] * P.n[1] * P.n[2]) return;
+    int i, j, k;
+    face_index(t, P.n[0], P.n[1], i, j, k);
+    double c[NS + 5], pr[2 * NS + 6];
+#pragma unroll
+    for (int n = 0; n < NS + 5; ++n) c[n] = cons(i, j, k, n);
+    cons_to_prim_cell<NS>(P, c, pr);
+#pragma unroll
+    for (int n = 0; n < 2 * NS + 6; ++n) prim(i, j, k, n) = pr[n];
+}
+
+template <int NS>
+__global__ void __launch_bounds__(BLK) k_transport(DevParams P, Arr prim, Arr eta, Arr zeta,
+                                                   Arr kappa, Arr chi, Arr Dm, int* err)
+{
+    const int g = P.ngc;
+    const int gx = P.n[0] + 2 * g, gy = P.n[1] + 2 * g, gz = P.n[2] + 2 * g;
+    long t = (long)blockIdx.x * BLK + threadIdx.x;
+    if (t >= (long)gx * gy * gz) return;
+    int i, j, k;
+    face_index(t, gx, gy, i, j, k);
+    i -= g;
+    j -= g;
+    k -= g;
+    double Y[NS], chiv[NS], Dv[NS * NS];
+#pragma unroll
+    for (int n = 0; n < NS; ++n) Y[n] = prim(i, j, k, 6 + n);
+    Transport tr;
+    bool bad = transport_cell<NS>(P, prim(i, j, k, 0), prim(i, j, k, 4), prim(i, j, k, 5), Y, tr,
+                                  chiv, Dv);
+    if (bad) atomicOr(err, 1);
+    eta(i, j, k) = tr.eta;
+    zeta(i, j, k) = tr.zeta;
+    kappa(i, j, k) = tr.kappa;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) chi(i, j, k, n) = chiv[n];
+#pragma unroll
+    for (int n = 0; n < NS * NS; ++n) Dm(i, j, k, n) = Dv[n];
+}
+
+// Periodic ghost fill of a cell FAB: ghost (i,j,k) <- valid (i mod n, ...).
+__global__ void __launch_bounds__(BLK) k_fill_boundary(Arr a, int nc, int g, int n0, int n1, int n2)
+{
+    const int gx = n0 + 2 * g, gy = n1 + 2 * g, gz = n2 + 2 * g;
+    long t = (long)blockIdx.x * BLK + threadIdx.x;
+    if (t >= (long)gx * gy * gz) return;
+    int i, j, k;
+    face_index(t, gx, gy, i, j, k);
+    i -= g;
+    j -= g;
+    k -= g;
+    int iv = (i + n0) % n0, jv = (j + n1) % n1, kv = (k + n2) % n2;
+    if (iv == i && jv == j && kv == k) return;
+    for (int n = 0; n < nc; ++n) a(i, j, k, n) = a(iv, jv, kv, n);
+}
+
+__global__ void __launch_bounds__(BLK) k_fill(double* p, long n, double v)
+{
+    long t = (long)blockIdx.x * BLK + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+// ---- stochastic face flux (flux.cpp:71-1145) -----------------------------------------------
+
+template <int NS, int D>
+__global__ void __launch_bounds__(BLK) k_stoch(DevParams P, Arr prim, Arr eta, Arr zeta, Arr kappa,
+                                               Arr chi, Arr Dm, Arr fl, Arr ran, Arr rc,
+                                               double volinv, double dtinv, double dt)
+{
+    const int fx = P.n[0] + (D == 0), fy = P.n[1] + (D == 1), fz = P.n[2] + (D == 2);
+    long t = (long)blockIdx.x * BLK + threadIdx.x;
+    if (t >= (long)fx * fy * fz) return;
+    int i, j, k;
+    face_index(t, fx, fy, i, j, k);
+    const int il = i - (D == 0), jl = j - (D == 1), kl = k - (D == 2);
+    const int nv = NS + 5;
+
+    double Tu = prim(i, j, k, 4), Tl = prim(il, jl, kl, 4);
+    double mup = eta(i, j, k) * Tu + eta(il, jl, kl) * Tl;
+    double kp = kappa(i, j, k) * Tu * Tu + kappa(il, jl, kl) * Tl * Tl;
+    double meanT = 0.5 * (Tu + Tl);
+    double w[NS + 5];
+    double fw1 = sqrt(P.k_B * mup * volinv * dtinv);
+    double fw4 = sqrt(P.k_B * kp * volinv * dtinv);
+    w[0] = 0 * ran(i, j, k, 0);
+    w[1] = fw1 * ran(i, j, k, 1);
+    w[2] = fw1 * ran(i, j, k, 2);
+    w[3] = fw1 * ran(i, j, k, 3);
+    w[4] = fw4 * ran(i, j, k, 4);
+    double nweight = sqrt(P.k_B * volinv * dtinv);
+
+    double sum = 0.;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        double s = 0.;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const signed char* o = c_mu_order[D][c][q];
+            double term = eta(i + o[0], j + o[1], k + o[2]) * prim(i + o[3], j + o[4], k + o[5], 4);
+            s = (q == 0) ? term : s + term;
+        }
+        double mu = 0.25 * s / 3.;
+        if (P.visc_type == 3 || P.visc_type == -3) {
+            double z = 0.;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const signed char* o = c_mu_order[D][c][q];
+                double term = zeta(i + o[0], j + o[1], k + o[2]) * prim(i + o[3], j + o[4], k + o[5], 4);
+                z = (q == 0) ? term : z + term;
+            }
+            mu += 0.25 * z;
+        }
+        const signed char* nd = c_rc_node[D][c];
+        double term = sqrt(mu) * rc(i + nd[0], j + nd[1], k + nd[2]);
+        sum = (c == 0) ? term : sum + term;
+    }
+    w[1 + D] = w[1 + D] + 0.25 * nweight * sum;
+
+#pragma unroll
+    for (int n = 1; n < 4; ++n) fl(i, j, k, n) = fl(i, j, k, n) + w[n];
+    fl(i, j, k, nv) = fl(i, j, k, nv) + w[4];
+
+    constexpr int t1 = (D == 0) ? 2 : 1, t2 = (D == 2) ? 2 : 3;
+    double phidiag = w[1 + D] * (prim(il, jl, kl, 1 + D) + prim(i, j, k, 1 + D));
+    double phishear = w[t1] * (prim(il, jl, kl, t1) + prim(i, j, k, t1)) +
+                      w[t2] * (prim(il, jl, kl, t2) + prim(i, j, k, t2));
+    phidiag = -0.5 * phidiag;
+    phishear = -0.5 * phishear;
+    fl(i, j, k, nv + 1) = fl(i, j, k, nv + 1) - phidiag;
+    fl(i, j, k, nv + 2) = fl(i, j, k, nv + 2) - phishear;
+
+    if (P.algorithm_type == 2) {
+        double yy[NS], yyp[NS], DijY[NS * NS], sqD[NS * NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            w[5 + s] = 0.;
+            yy[s] = clamp01(prim(il, jl, kl, 6 + s));
+            yyp[s] = clamp01(prim(i, j, k, 6 + s));
+        }
+        double sumy = 0., sumyp = 0.;
+#pragma unroll
+        for (int n = 0; n < NS; ++n) {
+            sumy += yy[n];
+            sumyp += yyp[n];
+        }
+#pragma unroll
+        for (int n = 0; n < NS; ++n) {
+            yy[n] /= sumy;
+            yyp[n] /= sumyp;
+        }
+        double MWmix = 0.;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            MWmix = MWmix + 0.5 * (yy[s] + yyp[s]) / P.molmass[s];
+#pragma unroll
+            for (int l = 0; l < NS; ++l)
+                DijY[s * NS + l] = 0.5 * (Dm(il, jl, kl, l * NS + s) * yy[l] + Dm(i, j, k, l * NS + s) * yyp[l] +
+                                          (Dm(il, jl, kl, s * NS + l) * yy[s] + Dm(i, j, k, s * NS + l) * yyp[s]));
+        }
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+            if (fabs(yy[s]) + fabs(yyp[s]) <= 1.e-12)
+#pragma unroll
+                for (int n = 0; n < NS; ++n) {
+                    DijY[s * NS + n] = 0.;
+                    DijY[n * NS + s] = 0.;
+                }
+        MWmix = 1. / MWmix;
+        cholesky<NS>(DijY, sqD);
+        double wsp = sqrt(P.k_B * MWmix * volinv / (P.Runiv * dt));
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+#pragma unroll
+            for (int l = 0; l <= s; ++l) w[5 + s] = w[5 + s] + (wsp * sqD[s * NS + l]) * ran(i, j, k, 5 + l);
+            fl(i, j, k, 5 + s) = w[5 + s];
+        }
+        double soret = 0.;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            double hk = P.e0[s] + P.hcp[s] * meanT;
+            double ss = (hk + P.Runiv * meanT / P.molmass[s] * 0.5 * (chi(il, jl, kl, s) + chi(i, j, k, s))) *
+                        w[5 + s];
+            soret = soret + ss;
+        }
+        fl(i, j, k, nv + 3) = fl(i, j, k, nv + 3) + soret;
+    }
+}
+
+// ---- diffusive face flux (flux.cpp:1193-1623) ----------------------------------------------
+
+template <int NS, int D>
+__global__ void __launch_bounds__(BLK) k_diff(DevParams P, Arr prim, Arr eta, Arr kappa, Arr chi,
+                                              Arr Dm, Arr fl)
+{
+    const int fx = P.n[0] + (D == 0), fy = P.n[1] + (D == 1), fz = P.n[2] + (D == 2);
+    long t = (long)blockIdx.x * BLK + threadIdx.x;
+    if (t >= (long)fx * fy * fz) return;
+    int i, j, k;
+    face_index(t, fx, fy, i, j, k);
+    const int il = i - (D == 0), jl = j - (D == 1), kl = k - (D == 2);
+    const int nv = NS + 5;
+    const double half = 0.5, h = P.dx[D];
+
+    double mup = half * (eta(i, j, k) + eta(il, jl, kl));
+    double kp = half * (kappa(i, j, k) + kappa(il, jl, kl));
+    double tau1 = mup * (prim(i, j, k, 1) - prim(il, jl, kl, 1)) / h;
+    double tau2 = mup * (prim(i, j, k, 2) - prim(il, jl, kl, 2)) / h;
+    double tau3 = mup * (prim(i, j, k, 3) - prim(il, jl, kl, 3)) / h;
+    double divp = 0.;
+    double s1 = prim(il, jl, kl, 1) + prim(i, j, k, 1);
+    double s2 = prim(il, jl, kl, 2) + prim(i, j, k, 2);
+    double s3 = prim(il, jl, kl, 3) + prim(i, j, k, 3);
+    double phiflx;
+    if (D == 0) phiflx = tau1 * s1 + divp * s1 + tau2 * s2 + tau3 * s3;
+    else if (D == 1) phiflx = tau1 * s1 + tau2 * s2 + divp * s2 + tau3 * s3;
+    else phiflx = tau1 * s1 + tau2 * s2 + tau3 * s3 + divp * s3;
+    double Qflux = kp * (prim(i, j, k, 4) - prim(il, jl, kl, 4)) / h;
+    fl(i, j, k, 1) = fl(i, j, k, 1) - (D == 0 ? tau1 + divp : tau1);
+    fl(i, j, k, 2) = fl(i, j, k, 2) - (D == 1 ? tau2 + divp : tau2);
+    fl(i, j, k, 3) = fl(i, j, k, 3) - (D == 2 ? tau3 + divp : tau3);
+    fl(i, j, k, nv) = fl(i, j, k, nv) - Qflux;
+    fl(i, j, k, nv + 1) = fl(i, j, k, nv + 1) - (half * phiflx);
+
+    double meanT = 0.5 * (prim(il, jl, kl, 4) + prim(i, j, k, 4));
+    double meanP = 0.5 * (prim(il, jl, kl, 5) + prim(i, j, k, 5));
+    if (P.algorithm_type == 2) {
+        double dk[NS], soret[NS], Fk[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            double XU = prim(i, j, k, 6 + NS + s), XL = prim(il, jl, kl, 6 + NS + s);
+            double term1 = (XU - XL) / h;
+            double mX = 0.5 * (XL + XU);
+            double mY = 0.5 * (prim(il, jl, kl, 6 + s) + prim(i, j, k, 6 + s));
+            double term2 = (mX - mY) * (prim(i, j, k, 5) - prim(il, jl, kl, 5)) / h / meanP;
+            dk[s] = term1 + term2;
+            double ChiX;
+            if (D == 2)  // flux.cpp:1561 reads chi at k and k+1
+                ChiX = 0.5 * (chi(i, j, k, s) * XL + chi(i, j, k + 1, s) * XU);
+            else
+                ChiX = 0.5 * (chi(il, jl, kl, s) * XL + chi(i, j, k, s) * XU);
+            soret[s] = ChiX * (prim(i, j, k, 4) - prim(il, jl, kl, 4)) / h / meanT;
+        }
+#pragma unroll
+        for (int kk = 0; kk < NS; ++kk) {
+            double f = 0.;
+#pragma unroll
+            for (int ll = 0; ll < NS; ++ll) {
+                double Fks = half * (Dm(il, jl, kl, ll * NS + kk) + Dm(i, j, k, ll * NS + kk)) * (dk[ll] + soret[ll]);
+                f = f - Fks;
+            }
+            Fk[kk] = f;
+        }
+        double Q5 = 0.;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            double hk = P.e0[s] + P.hcp[s] * meanT;
+            double csum = (D == 2) ? chi(i, j, k, s) + chi(i, j, k, s)  // flux.cpp:1604
+                                   : chi(il, jl, kl, s) + chi(i, j, k, s);
+            double Q5s = (hk + 0.5 * P.Runiv * meanT * csum / P.molmass[s]) * Fk[s];
+            Q5 = Q5 + Q5s;
+        }
+        fl(i, j, k, nv + 3) = fl(i, j, k, nv + 3) + Q5;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) fl(i, j, k, 5 + s) = fl(i, j, k, 5 + s) + Fk[s];
+    }
+}
+
+// ---- corner viscous terms at nodes (flux.cpp:1627-1713) ------------------------------------
+
+__global__ void __launch_bounds__(BLK) k_corner_node(DevParams P, Arr prim, Arr eta, Arr zeta,
+                                                     Arr cux, Arr cvx, Arr cwx, Arr cuy, Arr cvy,
+                                                     Arr cwy, Arr cuz, Arr cvz, Arr cwz, Arr vcorn)
+{
+    const int nx = P.n[0] + 1, ny = P.n[1] + 1, nz = P.n[2] + 1;
+    long t = (long)blockIdx.x * BLK + threadIdx.x;
+    if (t >= (long)nx * ny * nz) return;
+    int i, j, k;
+    face_index(t, nx, ny, i, j, k);
+    double muxp = 0.125 * (eta(i, j - 1, k - 1) + eta(i - 1, j - 1, k - 1) + eta(i, j, k - 1) +
+                           eta(i - 1, j, k - 1) + eta(i, j - 1, k) + eta(i - 1, j - 1, k) + eta(i, j, k) +
+                           eta(i - 1, j, k));
+    double zetaxp = 0.;
+    if (P.visc_type == 3 || P.visc_type == -3)
+        zetaxp = 0.125 * (zeta(i, j - 1, k - 1) + zeta(i - 1, j - 1, k - 1) + zeta(i, j, k - 1) +
+                          zeta(i - 1, j, k - 1) + zeta(i, j - 1, k) + zeta(i - 1, j - 1, k) + zeta(i, j, k) +
+                          zeta(i - 1, j, k));
+    const double DX0 = P.dx[0], DX1 = P.dx[1], DX2 = P.dx[2];
+    Arr gx[3] = {cux, cvx, cwx}, gy[3] = {cuy, cvy, cwy}, gz[3] = {cuz, cvz, cwz};
+#pragma unroll
+    for (int n = 1; n < 4; ++n) {
+        gx[n - 1](i, j, k) = 0.25 * muxp *
+                             (prim(i, j - 1, k - 1, n) - prim(i - 1, j - 1, k - 1, n) + prim(i, j, k - 1, n) -
+                              prim(i - 1, j, k - 1, n) + prim(i, j - 1, k, n) - prim(i - 1, j - 1, k, n) +
+                              prim(i, j, k, n) - prim(i - 1, j, k, n)) /
+                             DX0;
+        gy[n - 1](i, j, k) = 0.25 * muxp *
+                             (prim(i - 1, j, k - 1, n) - prim(i - 1, j - 1, k - 1, n) + prim(i, j, k - 1, n) -
+                              prim(i, j - 1, k - 1, n) + prim(i - 1, j, k, n) - prim(i - 1, j - 1, k, n) +
+                              prim(i, j, k, n) - prim(i, j - 1, k, n)) /
+                             DX1;
+        gz[n - 1](i, j, k) = 0.25 * muxp *
+                             (prim(i - 1, j - 1, k, n) - prim(i - 1, j - 1, k - 1, n) + prim(i, j - 1, k, n) -
+                              prim(i, j - 1, k - 1, n) + prim(i - 1, j, k, n) - prim(i - 1, j, k - 1, n) +
+                              prim(i, j, k, n) - prim(i, j, k - 1, n)) /
+                             DX2;
+    }
+    vcorn(i, j, k) =
+        (muxp / 12. + zetaxp / 4.) *
+        ((prim(i, j - 1, k - 1, 1) - prim(i - 1, j - 1, k - 1, 1)) / DX0 +
+         (prim(i, j, k - 1, 1) - prim(i - 1, j, k - 1, 1)) / DX0 +
+         (prim(i, j - 1, k, 1) - prim(i - 1, j - 1, k, 1)) / DX0 + (prim(i, j, k, 1) - prim(i - 1, j, k, 1)) / DX0 +
+         (prim(i - 1, j, k - 1, 2) - prim(i - 1, j - 1, k - 1, 2)) / DX1 +
+         (prim(i, j, k - 1, 2) - prim(i, j - 1, k - 1, 2)) / DX1 +
+         (prim(i - 1, j, k, 2) - prim(i - 1, j - 1, k, 2)) / DX1 + (prim(i, j, k, 2) - prim(i, j - 1, k, 2)) / DX1 +
+         (prim(i - 1, j - 1, k, 3) - prim(i - 1, j - 1, k - 1, 3)) / DX2 +
+         (prim(i, j - 1, k, 3) - prim(i, j - 1, k - 1, 3)) / DX2 +
+         (prim(i - 1, j, k, 3) - prim(i - 1, j, k - 1, 3)) / DX2 + (prim(i, j, k, 3) - prim(i, j, k - 1, 3)) / DX2);
+}
+
+// ---- corner-to-face terms (flux.cpp:1770-1980) -------------------------------------------
+// Sums over the 4 nodes of a face: visccorn in (hi,hi),(lo,hi),(hi,lo),(lo,lo) transverse
+// order on every face; the corner arrays in that order on x/y faces and in
+// (hi,hi),(hi,lo),(lo,hi),(lo,lo) order on z faces.
+
+// Transverse offsets of face direction D: the first transverse axis moves (AI, AJ, 0), the
+// second (0, BJ, BK): x faces (y, z), y faces (x, z), z faces (x, y).
+template <int D>
+struct Transverse {
+    static constexpr int AI = (D != 0), AJ = (D == 0), BJ = (D == 2), BK = (D != 2);
+};
+
+// Left-to-right sum of `a` over the 4 nodes of face (i,j,k), starting from `s0` unless
+// `first`; zorder selects (hi,hi),(hi,lo),(lo,hi),(lo,lo) instead of (hi,hi),(lo,hi),(hi,lo),(lo,lo).
+template <int D>
+__device__ inline double node4(const Arr& a, int i, int j, int k, bool zorder, double s0 = 0., bool first = true)
+{
+    using T = Transverse<D>;
+    const int oa[4] = {1, zorder ? 1 : 0, zorder ? 0 : 1, 0};
+    const int ob[4] = {1, zorder ? 0 : 1, zorder ? 1 : 0, 0};
+    double s = s0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        double v = a(i + oa[q] * T::AI, j + oa[q] * T::AJ + ob[q] * T::BJ, k + ob[q] * T::BK);
+        s = (first && q == 0) ? v : s + v;
+    }
+    return s;
+}
+
+template <int NS, int D>
+__global__ void __launch_bounds__(BLK) k_corner_face(DevParams P, Arr prim, Arr fl, Arr vcorn, Arr c1,
+                                                     Arr c2, Arr ta1, Arr ta2)
+{
+    const int fx = P.n[0] + (D == 0), fy = P.n[1] + (D == 1), fz = P.n[2] + (D == 2);
+    long t = (long)blockIdx.x * BLK + threadIdx.x;
+    if (t >= (long)fx * fy * fz) return;
+    int i, j, k;
+    face_index(t, fx, fy, i, j, k);
+    const int il = i - (D == 0), jl = j - (D == 1), kl = k - (D == 2);
+    const int nv = NS + 5;
+    const bool zo = (D == 2);
+    double vc4 = node4<D>(vcorn, i, j, k, false);
+    double div8 = node4<D>(c1, i, j, k, zo);
+    div8 = node4<D>(c2, i, j, k, zo, div8, false);
+    double s1 = node4<D>(ta1, i, j, k, zo);
+    double s2 = node4<D>(ta2, i, j, k, zo);
+    constexpr int cn = 1 + D;
+    constexpr int tc1 = (D == 0) ? 2 : 1, tc2 = (D == 2) ? 2 : 3;
+    fl(i, j, k, cn) = fl(i, j, k, cn) - 0.25 * vc4;
+    fl(i, j, k, cn) = fl(i, j, k, cn) + .25 * div8;
+    fl(i, j, k, tc1) = fl(i, j, k, tc1) - .25 * s1;
+    fl(i, j, k, tc2) = fl(i, j, k, tc2) - .25 * s2;
+    double phiflx = 0.25 * (vc4 - div8) * (prim(il, jl, kl, cn) + prim(i, j, k, cn));
+    phiflx = phiflx + .25 * s1 * (prim(il, jl, kl, tc1) + prim(i, j, k, tc1));
+    phiflx = phiflx + .25 * s2 * (prim(il, jl, kl, tc2) + prim(i, j, k, tc2));
+    fl(i, j, k, nv + 1) = fl(i, j, k, nv + 1) - 0.5 * phiflx;
+}
+
+// ---- hyperbolic face flux (flux.cpp:2016-2372) ---------------------------------------------
+
+template <int NS, int D>
+__global__ void __launch_bounds__(BLK) k_hyp(DevParams P, Arr cons, Arr prim, Arr fl)
+{
+    const int fx = P.n[0] + (D == 0), fy = P.n[1] + (D == 1), fz = P.n[2] + (D == 2);
+    long t = (long)blockIdx.x * BLK + threadIdx.x;
+    if (t >= (long)fx * fy * fz) return;
+    int i, j, k;
+    face_index(t, fx, fy, i, j, k);
+    constexpr int ex = (D == 0), ey = (D == 1), ez = (D == 2);
+    const int nv = NS + 5;
+    const double wgt2 = 1. / 12.;
+    const double wgt1 = 0.5 + wgt2;
+    double cv[NS + 5], pv[NS + 6], Yk[NS];
+    if (P.advection_type == 1) {
+#pragma unroll
+        for (int l = 0; l < NS + 6; ++l)
+            pv[l] = wgt1 * (prim(i, j, k, l) + prim(i - ex, j - ey, k - ez, l)) -
+                    wgt2 * (prim(i - 2 * ex, j - 2 * ey, k - 2 * ez, l) + prim(i + ex, j + ey, k + ez, l));
+        double rho = pv[0];
+        cv[0] = rho;
+#pragma unroll
+        for (int n = 0; n < NS; ++n) Yk[n] = pv[6 + n];
+        double ie = eos_energy<NS>(P, Yk, pv[4]);
+        double vsqr = pv[1] * pv[1] + pv[2] * pv[2] + pv[3] * pv[3];
+        cv[4] = rho * ie + 0.5 * rho * vsqr;
+#pragma unroll
+        for (int n = 0; n < NS; ++n) cv[5 + n] = cv[0] * pv[6 + n];
+    } else {
+#pragma unroll
+        for (int l = 0; l < NS + 5; ++l)
+            cv[l] = wgt1 * (cons(i, j, k, l) + cons(i - ex, j - ey, k - ez, l)) -
+                    wgt2 * (cons(i - 2 * ex, j - 2 * ey, k - 2 * ez, l) + cons(i + ex, j + ey, k + ez, l));
+#pragma unroll
+        for (int l = 1; l < 4; ++l) pv[l] = cv[l] / cv[0];
+#pragma unroll
+        for (int n = 0; n < NS; ++n) Yk[n] = cv[5 + n] / cv[0];
+        double vsqr = pv[1] * pv[1] + pv[2] * pv[2] + pv[3] * pv[3];
+        double ie = cv[4] / cv[0] - 0.5 * vsqr;
+        pv[4] = eos_temperature<NS>(P, ie, Yk);
+        pv[5] = eos_pressure<NS>(P, Yk, cv[0], pv[4]);
+    }
+    const double un = pv[1 + D];
+    fl(i, j, k, 0) += cv[0] * un;
+    if (D == 0) {
+        fl(i, j, k, 1) += cv[0] * (pv[1] * pv[1]) + pv[5];
+        fl(i, j, k, 2) += cv[0] * pv[1] * pv[2];
+        fl(i, j, k, 3) += cv[0] * pv[1] * pv[3];
+    } else if (D == 1) {
+        fl(i, j, k, 1) += cv[0] * pv[1] * pv[2];
+        fl(i, j, k, 2) += cv[0] * pv[2] * pv[2] + pv[5];
+        fl(i, j, k, 3) += cv[0] * pv[3] * pv[2];
+    } else {
+        fl(i, j, k, 1) += cv[0] * pv[1] * pv[3];
+        fl(i, j, k, 2) += cv[0] * pv[2] * pv[3];
+        fl(i, j, k, 3) += cv[0] * pv[3] * pv[3] + pv[5];
+    }
+    fl(i, j, k, 4) += un * cv[4] + pv[5] * un;
+    fl(i, j, k, 4) += fl(i, j, k, nv) + fl(i, j, k, nv + 1) + fl(i, j, k, nv + 2) + fl(i, j, k, nv + 3);
+    if (P.algorithm_type == 2) {
+        if (P.advection_type == 1) {
+#pragma unroll
+            for (int n = 0; n < NS; ++n) fl(i, j, k, 5 + n) += cv[0] * pv[6 + n] * un;
+        } else {
+#pragma unroll
+            for (int n = 0; n < NS; ++n) fl(i, j, k, 5 + n) += cv[5 + n] * un;
+        }
+    }
+}
+
+// ---- stage combination (timeStep.cpp:158-201, 266-306, 370-413) ----------------------------
+
+template <int NS>
+__global__ void __launch_bounds__(BLK) k_stage(DevParams P, Arr out, Arr u0, Arr us, Arr fx, Arr fy,
+                                               Arr fz, double dt, int mode)
+{
+    long t = (long)blockIdx.x * BLK + threadIdx.x;
+    if (t >= (long)P.n[0] * P.n[1] * P.n[2]) return;
+    int i, j, k;
+    face_index(t, P.n[0], P.n[1], i, j, k);
+    const double src = 0.0;
+    double o[NS + 5];
+#pragma unroll
+    for (int n = 0; n < NS + 5; ++n) {
+        double div = (fx(i + 1, j, k, n) - fx(i, j, k, n)) / P.dx[0] +
+                     (fy(i, j + 1, k, n) - fy(i, j, k, n)) / P.dx[1] +
+                     (fz(i, j, k + 1, n) - fz(i, j, k, n)) / P.dx[2];
+        if (mode == 0) o[n] = us(i, j, k, n) - dt * div + dt * src;
+        else if (mode == 1) o[n] = 0.25 * (3.0 * u0(i, j, k, n) + us(i, j, k, n) - dt * div + dt * src);
+        else o[n] = (2. / 3.) * (0.5 * u0(i, j, k, n) + us(i, j, k, n) - dt * div + dt * src);
+    }
+    const double* g = P.grav;
+    double r0 = us(i, j, k, 0), r1 = us(i, j, k, 1), r2 = us(i, j, k, 2), r3 = us(i, j, k, 3);
+    if (mode == 0) {
+#pragma unroll
+        for (int n = 0; n < 3; ++n) o[n + 1] += dt * r0 * g[n];
+        o[4] += dt * (g[0] * r1 + g[1] * r2 + g[2] * r3);
+    } else if (mode == 1) {
+#pragma unroll
+        for (int n = 0; n < 3; ++n) o[n + 1] += 0.25 * dt * r0 * g[n];
+        o[4] += 0.25 * dt * (g[0] * r1 + g[1] * r2 + g[2] * r3);
+    } else {
+#pragma unroll
+        for (int n = 0; n < 3; ++n) o[n + 1] += 2. / 3. * dt * r0 * g[n];
+        o[4] += 2. / 3. * dt * (g[0] * r1 + g[1] * r2 + g[2] * r3);
+    }
+#pragma unroll
+    for (int n = 0; n < NS + 5; ++n) out(i, j, k, n) = o[n];
+}
+
+// ---- noise --------------------------------------------------------------------------------
+
+// stochFlux = 1*A + w*B on comps 1..nvars-1 (comp 0 zeroed), MultiFab::LinComb.
+__global__ void __launch_bounds__(BLK) k_lincomb(double* dst, const double* a, const double* b, long npts,
+                                                 int nc, int c0, double w)
+{
+    long t = (long)blockIdx.x * BLK + threadIdx.x;
+    if (t >= npts) return;
+    for (int n = 0; n < nc; ++n) {
+        long q = t + (long)n * npts;
+        dst[q] = (n < c0) ? 0.0 : 1.0 * a[q] + w * b[q];
+    }
+}
+
+// Fills face component `comp` of direction D for sets A and B.
+template <int D>
+__global__ void __launch_bounds__(BLK) k_noise_face(DevParams P, NoiseKey K, Arr A, Arr B, int comp,
+                                                    double scale)
+{
+    const int fx = P.n[0] + (D == 0), fy = P.n[1] + (D == 1), fz = P.n[2] + (D == 2);
+    long t = (long)blockIdx.x * BLK + threadIdx.x;
+    if (t >= (long)fx * fy * fz) return;
+    int i, j, k;
+    face_index(t, fx, fy, i, j, k);
+    const int iw = (D == 0 && i == P.n[0]) ? 0 : i;
+    const int jw = (D == 1 && j == P.n[1]) ? 0 : j;
+    const int kw = (D == 2 && k == P.n[2]) ? 0 : k;
+    uint32_t lin = (uint32_t)(((long)kw * P.n[1] + jw) * P.n[0] + iw);
+    double za, zb;
+    noise_ab(K, (uint32_t)(16 * D + comp), lin, za, zb);
+    A(i, j, k, comp) = za * scale;
+    B(i, j, k, comp) = zb * scale;
+}
+
+__global__ void __launch_bounds__(BLK) k_noise_node(DevParams P, NoiseKey K, Arr A, Arr B, double scale)
+{
+    const int nx = P.n[0] + 1, ny = P.n[1] + 1, nz = P.n[2] + 1;
+    long t = (long)blockIdx.x * BLK + threadIdx.x;
+    if (t >= (long)nx * ny * nz) return;
+    int i, j, k;
+    face_index(t, nx, ny, i, j, k);
+    const int iw = (i == P.n[0]) ? 0 : i, jw = (j == P.n[1]) ? 0 : j, kw = (k == P.n[2]) ? 0 : k;
+    uint32_t lin = (uint32_t)(((long)kw * P.n[1] + jw) * P.n[0] + iw);
+    double za, zb;
+    noise_ab(K, NODE_FIELD, lin, za, zb);
+    A(i, j, k) = za * scale;
+    B(i, j, k) = zb * scale;
+}
+
+// ---- host launchers ---------------------------------------------------------------------
+
+#define NS_DISPATCH(NSV, CALL)                          \
+    switch (NSV) {                                      \
+        case 2: { constexpr int NS = 2; CALL; } break;  \
+        case 3: { constexpr int NS = 3; CALL; } break;  \
+        case 4: { constexpr int NS = 4; CALL; } break;  \
+        default: return FHDX_ERR_UNSUPPORTED;           \
+    }
+
+static long ncell(const DevParams& P) { return (long)P.n[0] * P.n[1] * P.n[2]; }
+static long nface(const DevParams& P, int d)
+{
+    return (long)(P.n[0] + (d == 0)) * (P.n[1] + (d == 1)) * (P.n[2] + (d == 2));
+}
+static long nnode(const DevParams& P) { return (long)(P.n[0] + 1) * (P.n[1] + 1) * (P.n[2] + 1); }
+static long ngrown(const DevParams& P)
+{
+    const int g = P.ngc;
+    return (long)(P.n[0] + 2 * g) * (P.n[1] + 2 * g) * (P.n[2] + 2 * g);
+}
+
+int ref_c2p(const DevParams& P, Arr prim, Arr cons, hipStream_t s)
+{
+    NS_DISPATCH(P.ns, (k_c2p<NS><<<grid_for(ncell(P), BLK), BLK, 0, s>>>(P, prim, cons)));
+    FHDX_HIP_TRY(hipGetLastError());
+    return FHDX_OK;
+}
+
+int ref_transport(const DevParams& P, Arr prim, Arr eta, Arr zeta, Arr kappa, Arr chi, Arr Dm,
+                  int* err, hipStream_t s)
+{
+    NS_DISPATCH(P.ns, (k_transport<NS><<<grid_for(ngrown(P), BLK), BLK, 0, s>>>(
+                          P, prim, eta, zeta, kappa, chi, Dm, err)));
+    FHDX_HIP_TRY(hipGetLastError());
+    return FHDX_OK;
+}
+
+int ref_fill_boundary(const DevParams& P, Arr a, int nc, hipStream_t s)
+{
+    k_fill_boundary<<<grid_for(ngrown(P), BLK), BLK, 0, s>>>(a, nc, P.ngc, P.n[0], P.n[1], P.n[2]);
+    FHDX_HIP_TRY(hipGetLastError());
+    return FHDX_OK;
+}
+
+int fill_value(double* p, long n, double v, hipStream_t s)
+{
+    if (n <= 0) return FHDX_OK;
+    k_fill<<<grid_for(n, BLK), BLK, 0, s>>>(p, n, v);
+    FHDX_HIP_TRY(hipGetLastError());
+    return FHDX_OK;
+}
+
+template <int NS>
+static void flux_passes(const DevParams& P, const FluxArgs& a, hipStream_t s)
+{
+    const double volinv = 1. / (P.dx[0] * P.dx[1] * P.dx[2]);
+    const double dtinv = 1. / a.dt;
+    if (P.stoch_on) {
+        k_stoch<NS, 0><<<grid_for(nface(P, 0), BLK), BLK, 0, s>>>(P, a.prim, a.eta, a.zeta, a.kappa, a.chi, a.D,
+                                                                  a.flux[0], a.ran[0], a.rancorn, volinv, dtinv, a.dt);
+        k_stoch<NS, 1><<<grid_for(nface(P, 1), BLK), BLK, 0, s>>>(P, a.prim, a.eta, a.zeta, a.kappa, a.chi, a.D,
+                                                                  a.flux[1], a.ran[1], a.rancorn, volinv, dtinv, a.dt);
+        k_stoch<NS, 2><<<grid_for(nface(P, 2), BLK), BLK, 0, s>>>(P, a.prim, a.eta, a.zeta, a.kappa, a.chi, a.D,
+                                                                  a.flux[2], a.ran[2], a.rancorn, volinv, dtinv, a.dt);
+    }
+    k_diff<NS, 0><<<grid_for(nface(P, 0), BLK), BLK, 0, s>>>(P, a.prim, a.eta, a.kappa, a.chi, a.D, a.flux[0]);
+    k_diff<NS, 1><<<grid_for(nface(P, 1), BLK), BLK, 0, s>>>(P, a.prim, a.eta, a.kappa, a.chi, a.D, a.flux[1]);
+    k_diff<NS, 2><<<grid_for(nface(P, 2), BLK), BLK, 0, s>>>(P, a.prim, a.eta, a.kappa, a.chi, a.D, a.flux[2]);
+    // corn[a][b]: derivative direction a, velocity component b (cornx/corny/cornz of flux.cpp)
+    const Arr(&c)[3][3] = a.corn;
+    k_corner_node<<<grid_for(nnode(P), BLK), BLK, 0, s>>>(P, a.prim, a.eta, a.zeta, c[0][0], c[0][1], c[0][2],
+                                                          c[1][0], c[1][1], c[1][2], c[2][0], c[2][1], c[2][2],
+                                                          a.visccorn);
+    // x face: div pair (vy, wz), tangential (uy, uz); y: (ux, wz), (vx, vz); z: (vy, ux), (wx, wy)
+    k_corner_face<NS, 0><<<grid_for(nface(P, 0), BLK), BLK, 0, s>>>(P, a.prim, a.flux[0], a.visccorn, c[1][1],
+                                                                    c[2][2], c[1][0], c[2][0]);
+    k_corner_face<NS, 1><<<grid_for(nface(P, 1), BLK), BLK, 0, s>>>(P, a.prim, a.flux[1], a.visccorn, c[0][0],
+                                                                    c[2][2], c[0][1], c[2][1]);
+    k_corner_face<NS, 2><<<grid_for(nface(P, 2), BLK), BLK, 0, s>>>(P, a.prim, a.flux[2], a.visccorn, c[1][1],
+                                                                    c[0][0], c[0][2], c[1][2]);
+    if (P.advection_type == 1 || P.advection_type == 2) {
+        k_hyp<NS, 0><<<grid_for(nface(P, 0), BLK), BLK, 0, s>>>(P, a.cons, a.prim, a.flux[0]);
+        k_hyp<NS, 1><<<grid_for(nface(P, 1), BLK), BLK, 0, s>>>(P, a.cons, a.prim, a.flux[1]);
+        k_hyp<NS, 2><<<grid_for(nface(P, 2), BLK), BLK, 0, s>>>(P, a.cons, a.prim, a.flux[2]);
+    }
+}
+
+int ref_calculate_flux(const DevParams& P, const FluxArgs& a, hipStream_t s)
+{
+    const int nc = P.nvars + 4;
+    for (int d = 0; d < 3; ++d) {
+        int r = fill_value(a.flux[d].p, a.flux[d].sn * nc, 0.0, s);
+        if (r) return r;
+    }
+    NS_DISPATCH(P.ns, flux_passes<NS>(P, a, s));
+    FHDX_HIP_TRY(hipGetLastError());
+    return FHDX_OK;
+}
+
+int ref_stage(const DevParams& P, Arr out, Arr u0, Arr us, const Arr flux[3], double dt, int mode,
+              hipStream_t s)
+{
+    NS_DISPATCH(P.ns, (k_stage<NS><<<grid_for(ncell(P), BLK), BLK, 0, s>>>(P, out, u0, us, flux[0], flux[1],
+                                                                             flux[2], dt, mode)));
+    FHDX_HIP_TRY(hipGetLastError());
+    return FHDX_OK;
+}
+
+int ref_lincomb(double* dst, const double* a, const double* b, long npts, int nc, int c0, double w,
+                hipStream_t s)
+{
+    k_lincomb<<<grid_for(npts, BLK), BLK, 0, s>>>(dst, a, b, npts, nc, c0, w);
+    FHDX_HIP_TRY(hipGetLastError());
+    return FHDX_OK;
+}
+
+int ref_fill_noise(const DevParams& P, const NoiseKey& K, const Arr A[3], const Arr B[3], Arr rA, Arr rB,
+                   hipStream_t s)
+{
+    for (int c = 1; c < P.nvars; ++c) {
+        double sc = (c <= 3) ? P.var_mom : (c == 4) ? P.var_ener : P.var_mass;
+        k_noise_face<0><<<grid_for(nface(P, 0), BLK), BLK, 0, s>>>(P, K, A[0], B[0], c, sc);
+        k_noise_face<1><<<grid_for(nface(P, 1), BLK), BLK, 0, s>>>(P, K, A[1], B[1], c, sc);
+        k_noise_face<2><<<grid_for(nface(P, 2), BLK), BLK, 0, s>>>(P, K, A[2], B[2], c, sc);
+    }
+    k_noise_node<<<grid_for(nnode(P), BLK), BLK, 0, s>>>(P, K, rA, rB, P.var_mom);
+    FHDX_HIP_TRY(hipGetLastError());
+    return FHDX_OK;
+}
+
+}  // namespace fhdx

@@ -1,0 +1,118 @@
+"""ctypes binding of libfhdx.so (the C ABI of include/fhdx.h) and device-array helpers.
+
+The library is the product: every call here goes to the HIP kernels.  There is no CPU
+fallback -- `lib()` raises if the built library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+from .params import Fab, Params
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfhdx.so")
+
+# every symbol include/fhdx.h declares (tests check the library exports all of them)
+EXPORTS = [
+    "fhdx_abi_version", "fhdx_status_string", "fhdx_last_error", "fhdx_params_default",
+    "fhdx_params_finalize", "fhdx_params_check", "fhdx_ctx_create", "fhdx_ctx_destroy",
+    "fhdx_ctx_check", "fhdx_cons_to_prim", "fhdx_transport", "fhdx_fill_boundary",
+    "fhdx_calculate_flux", "fhdx_fill_noise", "fhdx_rk3_step_ref", "fhdx_rk3_step",
+    "fhdx_rk3_step_injected",
+]
+
+_lib = None
+
+
+class FhdxError(RuntimeError):
+    def __init__(self, status: int, where: str):
+        L = lib()
+        msg = L.fhdx_status_string(status).decode()
+        detail = L.fhdx_last_error().decode()
+        super().__init__(f"{where}: {msg} ({status}) {detail}")
+        self.status = status
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run `python -m fhdex_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER(Params)
+        FA = Fab * 3
+        PF = C.POINTER(Fab)
+        vp = C.c_void_p
+        L.fhdx_abi_version.restype = C.c_int
+        L.fhdx_status_string.restype = C.c_char_p
+        L.fhdx_status_string.argtypes = [C.c_int]
+        L.fhdx_last_error.restype = C.c_char_p
+        L.fhdx_params_default.argtypes = [P]
+        L.fhdx_params_finalize.argtypes = [P]
+        L.fhdx_params_check.argtypes = [P]
+        L.fhdx_ctx_create.argtypes = [P, C.POINTER(vp)]
+        L.fhdx_ctx_destroy.argtypes = [vp]
+        L.fhdx_ctx_check.argtypes = [vp, vp]
+        L.fhdx_cons_to_prim.argtypes = [vp, Fab, Fab, vp]
+        L.fhdx_transport.argtypes = [vp, Fab, Fab, Fab, Fab, Fab, Fab, vp]
+        L.fhdx_fill_boundary.argtypes = [vp, Fab, vp]
+        L.fhdx_calculate_flux.argtypes = [vp, Fab, Fab, Fab, Fab, Fab, Fab, Fab, FA, FA, FA, FA, FA,
+                                          Fab, Fab, C.c_double, vp]
+        L.fhdx_fill_noise.argtypes = [vp, C.c_uint64, C.c_uint64, C.c_uint32, FA, FA, Fab, Fab, vp]
+        L.fhdx_rk3_step_ref.argtypes = [vp, Fab, Fab, Fab, Fab, Fab, Fab, Fab, Fab, Fab, FA, FA, FA, FA,
+                                        FA, Fab, Fab, PF, PF, PF, PF, C.c_double, C.c_uint64,
+                                        C.c_uint64, vp]
+        L.fhdx_rk3_step.argtypes = [vp, Fab, Fab, C.c_double, C.c_uint64, C.c_uint64, vp]
+        L.fhdx_rk3_step_injected.argtypes = [vp, Fab, Fab, FA, FA, Fab, Fab, C.c_double, vp]
+        for name in EXPORTS:
+            getattr(L, name).restype = C.c_int
+        L.fhdx_params_default.restype = None
+        L.fhdx_params_finalize.restype = None
+        L.fhdx_status_string.restype = C.c_char_p
+        L.fhdx_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def check(status: int, where: str):
+    if status != 0:
+        raise FhdxError(status, where)
+
+
+def fab_of_tensor(t, lo) -> Fab:
+    """fhdx_fab of a contiguous float64 torch tensor shaped (ncomp, nz, ny, nx) starting at
+    index `lo`."""
+    assert t.dtype.is_floating_point and t.element_size() == 8 and t.is_contiguous() and t.dim() == 4
+    f = Fab()
+    f.p = t.data_ptr()
+    nc, nz, ny, nx = t.shape
+    for d, n in enumerate((nx, ny, nz)):
+        f.lo[d] = lo[d]
+        f.hi[d] = lo[d] + n - 1
+    f.ncomp = nc
+    return f
+
+
+class Context:
+    """Owns an fhdx_ctx (device workspace, error flag) for one parameter set."""
+
+    def __init__(self, p: Params):
+        self.p = p
+        h = C.c_void_p()
+        check(lib().fhdx_ctx_create(C.byref(p), C.byref(h)), "fhdx_ctx_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().fhdx_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, stream=None):
+        check(lib().fhdx_ctx_check(self.h, stream), "fhdx_ctx_check")

@@ -1,0 +1,169 @@
+/*
+ * fhdx.h -- C ABI of the MI355X (gfx950) compressible fluctuating-hydrodynamics step.
+ *
+ * Every entry point takes plain pointers and sizes (no AMReX, HIP or torch types), returns an
+ * int error code (FHDX_OK == 0) and launches asynchronously on the HIP stream passed as
+ * `void* stream` (NULL = the null stream).  All arrays are FP64 device pointers laid out like
+ * an AMReX FArrayBox / Array4: x fastest, then y, then z, then component.
+ *
+ * Index conventions (reference: src_compressible/main_driver.cpp:160-297):
+ *   cell data  : indices [-ngc, n_d + ngc) per direction (ngc ghost layers);
+ *   face data  : direction d nodal, indices [0, n_d] in d and [0, n_t) across, no ghosts;
+ *   node data  : [0, n_d] in every direction, no ghosts.
+ * Component maps (reference: main_driver.cpp:172-190, 248-260; transCoeffs.cpp:168-173):
+ *   cons  : 0 rho, 1-3 momentum, 4 rho*E, 5.. rho*Y_k                         (nvars = 5+NS)
+ *   prim  : 0 rho, 1-3 velocity, 4 T, 5 p, 6.. Y_k, 6+NS.. X_k                 (nprimvars = 6+2NS)
+ *   flux  : 0..nvars-1 as cons, nvars heat, +1 viscous heating (diag), +2 (shear), +3 Dufour
+ *   D     : component ll*NS+kk, multiplied by rho
+ */
+#ifndef FHDX_H
+#define FHDX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FHDX_MAX_SPECIES 8
+#define FHDX_ABI_VERSION 1
+
+enum fhdx_status {
+    FHDX_OK = 0,
+    FHDX_ERR_ARG = 1,         /* bad argument or array shape */
+    FHDX_ERR_UNSUPPORTED = 2, /* parameter combination outside the implemented scope */
+    FHDX_ERR_HIP = 3,         /* HIP runtime failure (message in fhdx_last_error) */
+    FHDX_ERR_MASSFRAC = 4,    /* Y_k <= 0 or >= 1 met in the transport step: the reference aborts
+                                 there (src_compressible/transCoeffs.cpp:45-46) */
+    FHDX_ERR_NOMEM = 5,       /* device memory budget exhausted */
+    FHDX_ERR_NONFINITE = 6,   /* non-finite state detected */
+    FHDX_ERR_BUSY = 7         /* serving engine: no free slot for the request */
+};
+
+/* Run parameters: the values the reference reads from its AMREX_GPU_MANAGED namespaces
+ * (src_common/common_namespace.H:24-115, src_compressible/compressible_namespace.H:1-15). */
+typedef struct fhdx_params {
+    int32_t nspecies;          /* NS, 1..FHDX_MAX_SPECIES                                   */
+    int32_t n_cells[3];        /* domain cells per direction                                 */
+    int32_t ngc;               /* ghost layers of cell data (>= 2)                           */
+    int32_t algorithm_type;    /* 2: multispecies (species fluxes on), else species off      */
+    int32_t advection_type;    /* 0: none, 1: interpolate prim, 2: interpolate cons          */
+    int32_t visc_type;         /* |3| adds bulk viscosity zeta = zeta_ratio*eta              */
+    int32_t transport_type;    /* 1: Giovangigli (the only one implemented)                  */
+    int32_t stoch_stress_form; /* 1: stochastic fluxes on, 0: deterministic                  */
+    int32_t nreaction;         /* must be 0 (chemistry source is outside this path)          */
+    int32_t bc_vel_lo[3], bc_vel_hi[3];   /* -1 periodic (the only supported value)           */
+    int32_t bc_mass_lo[3], bc_mass_hi[3];
+    int32_t bc_therm_lo[3], bc_therm_hi[3];
+    int32_t pad0;
+    double dx[3];              /* cell sizes [cm]                                            */
+    double molmass[FHDX_MAX_SPECIES];  /* g/mol                                              */
+    double diameter[FHDX_MAX_SPECIES]; /* cm                                                 */
+    double dof[FHDX_MAX_SPECIES];
+    double hcv[FHDX_MAX_SPECIES];      /* < 0: derived from dof (compressible_functions.cpp:91-100) */
+    double hcp[FHDX_MAX_SPECIES];
+    double e0[FHDX_MAX_SPECIES];
+    double k_B;                /* 1.38064852e-16 by default                                  */
+    double Runiv;              /* 8.314462175e7 by default                                   */
+    double variance_coef_mom, variance_coef_ener, variance_coef_mass;
+    double grav[3];
+    double zeta_ratio;
+} fhdx_params;
+
+/* One FAB: data pointer plus the inclusive index box it covers (ghosts included). */
+typedef struct fhdx_fab {
+    double* p;
+    int32_t lo[3];
+    int32_t hi[3];
+    int32_t ncomp;
+    int32_t pad;
+} fhdx_fab;
+
+typedef struct fhdx_ctx fhdx_ctx;
+
+/* ---- library / parameters -------------------------------------------------------------- */
+int fhdx_abi_version(void);
+const char* fhdx_status_string(int status);
+/* Text of the last error raised on this host thread ("" if none). */
+const char* fhdx_last_error(void);
+
+/* Reference defaults of src_common/common_functions.cpp:320-490 with the compressible
+ * periodic settings every reference input uses (ngc 2, algorithm/visc/advection type 2). */
+void fhdx_params_default(fhdx_params* p);
+/* GetHcGas (src_compressible/compressible_functions.cpp:91-100): hcv/hcp < 0 -> from dof. */
+void fhdx_params_finalize(fhdx_params* p);
+/* 0 if the parameters are inside the implemented scope, else an fhdx_status. */
+int fhdx_params_check(const fhdx_params* p);
+
+/* A context owns per-run device workspace; one per host thread.  Creating it selects nothing:
+ * the current HIP device is used. */
+int fhdx_ctx_create(const fhdx_params* p, fhdx_ctx** out);
+int fhdx_ctx_destroy(fhdx_ctx* ctx);
+/* Waits for the work queued on `stream`, then reports (and clears) the device-side error
+ * flags raised by earlier calls on this context (e.g. FHDX_ERR_MASSFRAC). */
+int fhdx_ctx_check(fhdx_ctx* ctx, void* stream);
+
+/* ---- reference-shaped entry points ------------------------------------------------------ */
+
+/* conservedToPrimitive (src_compressible/conservedPrimitiveConversions.cpp:4-92): valid cells. */
+int fhdx_cons_to_prim(fhdx_ctx* ctx, fhdx_fab prim, fhdx_fab cons, void* stream);
+
+/* calculateTransportCoeffs (src_compressible/transCoeffs.cpp:7-90), transport_type 1, on the
+ * box grown by ngc.  A mass fraction outside (0,1) raises FHDX_ERR_MASSFRAC (see ctx_check). */
+int fhdx_transport(fhdx_ctx* ctx, fhdx_fab prim, fhdx_fab eta, fhdx_fab zeta, fhdx_fab kappa,
+                   fhdx_fab chi, fhdx_fab D, void* stream);
+
+/* MultiFab::FillBoundary(periodicity) for one periodic cell FAB (all components). */
+int fhdx_fill_boundary(fhdx_ctx* ctx, fhdx_fab mf, void* stream);
+
+/* calculateFlux (src_compressible/flux.cpp:4-2376; signature compressible_functions.H:23-35):
+ * stochastic, diffusive, corner and hyperbolic face fluxes into flux[3] (nvars+4 comps),
+ * the corner arrays and visccorn.  stochFlux/rancorn hold the already weighted noise. */
+int fhdx_calculate_flux(fhdx_ctx* ctx, fhdx_fab cons, fhdx_fab prim, fhdx_fab eta,
+                        fhdx_fab zeta, fhdx_fab kappa, fhdx_fab chi, fhdx_fab D,
+                        const fhdx_fab flux[3], const fhdx_fab stochFlux[3],
+                        const fhdx_fab cornx[3], const fhdx_fab corny[3],
+                        const fhdx_fab cornz[3], fhdx_fab visccorn, fhdx_fab rancorn,
+                        double dt, void* stream);
+
+/* The noise of one step (replaces the MultiFabFillRandom calls of timeStep.cpp:75-91):
+ * N(0,1)*|variance_coef| from a counter-based Philox4x32-10 keyed by (seed, step, instance),
+ * set A and set B, face components 1..nvars-1 and the nodal field.  Duplicate periodic
+ * faces/nodes (index n_d == index 0) carry the same value, as after OverrideSync. */
+int fhdx_fill_noise(fhdx_ctx* ctx, uint64_t seed, uint64_t step, uint32_t instance,
+                    const fhdx_fab stochA[3], const fhdx_fab stochB[3], fhdx_fab rancornA,
+                    fhdx_fab rancornB, void* stream);
+
+/* RK3step (src_compressible/timeStep.cpp:11-427; signature compressible_functions.H:41-51)
+ * with the reference's own arrays: cu is advanced in place, cup/cup2 are stage scratch, prim,
+ * eta..D, flux, stochFlux, corners, visccorn and rancorn end as the reference leaves them.
+ * Noise: if stochA is non-NULL the caller's A/B arrays are used (injected noise), otherwise
+ * Philox noise for (seed, step, instance 0) is generated into ctx workspace. */
+int fhdx_rk3_step_ref(fhdx_ctx* ctx, fhdx_fab cu, fhdx_fab cup, fhdx_fab cup2, fhdx_fab prim,
+                      fhdx_fab eta, fhdx_fab zeta, fhdx_fab kappa, fhdx_fab chi, fhdx_fab D,
+                      const fhdx_fab flux[3], const fhdx_fab stochFlux[3],
+                      const fhdx_fab cornx[3], const fhdx_fab corny[3], const fhdx_fab cornz[3],
+                      fhdx_fab visccorn, fhdx_fab rancorn,
+                      const fhdx_fab* stochA, const fhdx_fab* stochB,
+                      const fhdx_fab* rancornA, const fhdx_fab* rancornB,
+                      double dt, uint64_t seed, uint64_t step, void* stream);
+
+/* ---- fused fast path --------------------------------------------------------------------- */
+
+/* One RK3 step of RK3step's arithmetic with fused stage kernels: cu (nvars comps, ngc ghosts)
+ * advanced in place, prim (nprimvars comps) recomputed from the new cu; ghost layers of both
+ * are filled periodically, as after timeStep.cpp:417-425.  Noise is generated in-kernel from
+ * Philox (seed, step, instance 0), bit-identical to fhdx_fill_noise for the same key. */
+int fhdx_rk3_step(fhdx_ctx* ctx, fhdx_fab cu, fhdx_fab prim, double dt, uint64_t seed,
+                  uint64_t step, void* stream);
+
+/* As fhdx_rk3_step but the A/B noise is read from caller arrays (parity runs). */
+int fhdx_rk3_step_injected(fhdx_ctx* ctx, fhdx_fab cu, fhdx_fab prim,
+                           const fhdx_fab stochA[3], const fhdx_fab stochB[3],
+                           fhdx_fab rancornA, fhdx_fab rancornB, double dt, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FHDX_H */

@@ -1,0 +1,184 @@
+"""ctypes binding of the CPU oracle (oracle/liborc.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's CPU
+baseline leg as the checker.  The product never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from fhdex_amd.params import Fab, Params
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liborc.so")
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        P = C.POINTER(Params)
+        FA = Fab * 3
+        L.orc_cons_to_prim.argtypes = [P, Fab, Fab]
+        L.orc_transport.argtypes = [P, Fab, Fab, Fab, Fab, Fab, Fab]
+        L.orc_fill_boundary.argtypes = [P, Fab]
+        L.orc_calculate_flux.argtypes = [P, Fab, Fab, Fab, Fab, Fab, Fab, Fab, FA, FA, FA, FA,
+                                         FA, Fab, Fab, C.c_double]
+        L.orc_rk3_step.argtypes = [P, Fab, Fab, Fab, Fab, Fab, Fab, Fab, Fab, Fab, FA, FA, FA,
+                                   FA, FA, Fab, Fab, FA, FA, Fab, Fab, C.c_double]
+        L.orc_advance.argtypes = [P, Fab, Fab, C.c_int, C.c_double, C.c_void_p, C.c_uint64,
+                                  C.c_uint64, C.c_uint32]
+        L.orc_noise_doubles_per_step.argtypes = [P]
+        L.orc_noise_doubles_per_step.restype = C.c_long
+        L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                        C.POINTER(C.c_uint32)]
+        L.orc_normal_pair.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double)]
+        L.orc_fill_noise.argtypes = [P, C.c_uint64, C.c_uint64, C.c_uint32, FA, FA, Fab, Fab]
+        L.orc_init_cons.argtypes = [P, Fab, C.c_int, C.c_double, C.POINTER(C.c_double),
+                                    C.c_double]
+        L.orc_num_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+# ---- host FABs ---------------------------------------------------------------------------
+
+def fab_of(arr: np.ndarray, lo) -> Fab:
+    """Describe a C-contiguous float64 array of shape (ncomp, nz, ny, nx) whose first element
+    is index `lo`."""
+    assert arr.dtype == np.float64 and arr.flags["C_CONTIGUOUS"] and arr.ndim == 4
+    f = Fab()
+    f.p = arr.ctypes.data
+    nc, nz, ny, nx = arr.shape
+    for d, n in enumerate((nx, ny, nz)):
+        f.lo[d] = lo[d]
+        f.hi[d] = lo[d] + n - 1
+    f.ncomp = nc
+    return f
+
+
+class HostState:
+    """All arrays of the reference's RK3step argument list, on the host (numpy)."""
+
+    def __init__(self, p: Params):
+        self.p = p
+        n0, n1, n2 = p.n_cells
+        g = p.ngc
+        NS = p.nspecies
+        nv = p.nvars
+
+        def cell(nc):
+            return np.zeros((nc, n2 + 2 * g, n1 + 2 * g, n0 + 2 * g))
+
+        def face(d, nc):
+            return np.zeros((nc, n2 + (d == 2), n1 + (d == 1), n0 + (d == 0)))
+
+        def node(nc=1):
+            return np.zeros((nc, n2 + 1, n1 + 1, n0 + 1))
+
+        self.cu, self.cup, self.cup2 = cell(nv), cell(nv), cell(nv)
+        self.prim = cell(p.nprimvars)
+        self.eta, self.zeta, self.kappa = cell(1), cell(1), cell(1)
+        self.chi, self.D = cell(NS), cell(NS * NS)
+        self.flux = [face(d, nv + 4) for d in range(3)]
+        self.stoch = [face(d, nv) for d in range(3)]
+        self.corn = [[node() for _ in range(3)] for _ in range(3)]
+        self.visccorn, self.rancorn = node(), node()
+        self.sA = [face(d, nv) for d in range(3)]
+        self.sB = [face(d, nv) for d in range(3)]
+        self.rA, self.rB = node(), node()
+        self.glo = (-g, -g, -g)
+
+    def cf(self, a):
+        return fab_of(a, self.glo)
+
+    @staticmethod
+    def zf(a):
+        return fab_of(a, (0, 0, 0))
+
+    def fa(self, arrs):
+        return (Fab * 3)(*[self.zf(a) for a in arrs])
+
+
+def init_cons(p: Params, cu: np.ndarray, rho0, rhobar, T_init, prob_type=1):
+    g = p.ngc
+    rb = (C.c_double * 8)(*(list(rhobar) + [0.0] * (8 - len(rhobar))))
+    return lib().orc_init_cons(C.byref(p), fab_of(cu, (-g, -g, -g)), prob_type, rho0, rb,
+                               T_init)
+
+
+def cons_to_prim(p, prim, cons):
+    g = p.ngc
+    return lib().orc_cons_to_prim(C.byref(p), fab_of(prim, (-g,) * 3), fab_of(cons, (-g,) * 3))
+
+
+def fill_boundary(p, a):
+    g = p.ngc
+    return lib().orc_fill_boundary(C.byref(p), fab_of(a, (-g,) * 3))
+
+
+def transport(p, st: HostState):
+    return lib().orc_transport(C.byref(p), st.cf(st.prim), st.cf(st.eta), st.cf(st.zeta),
+                               st.cf(st.kappa), st.cf(st.chi), st.cf(st.D))
+
+
+def calculate_flux(p, st: HostState, cons, dt):
+    return lib().orc_calculate_flux(
+        C.byref(p), st.cf(cons), st.cf(st.prim), st.cf(st.eta), st.cf(st.zeta),
+        st.cf(st.kappa), st.cf(st.chi), st.cf(st.D), st.fa(st.flux), st.fa(st.stoch),
+        st.fa(st.corn[0]), st.fa(st.corn[1]), st.fa(st.corn[2]), st.zf(st.visccorn),
+        st.zf(st.rancorn), dt)
+
+
+def rk3_step(p, st: HostState, dt):
+    """One RK3step with the noise sets currently held in st.sA/sB/rA/rB."""
+    return lib().orc_rk3_step(
+        C.byref(p), st.cf(st.cu), st.cf(st.cup), st.cf(st.cup2), st.cf(st.prim),
+        st.cf(st.eta), st.cf(st.zeta), st.cf(st.kappa), st.cf(st.chi), st.cf(st.D),
+        st.fa(st.flux), st.fa(st.stoch), st.fa(st.corn[0]), st.fa(st.corn[1]),
+        st.fa(st.corn[2]), st.zf(st.visccorn), st.zf(st.rancorn), st.fa(st.sA), st.fa(st.sB),
+        st.zf(st.rA), st.zf(st.rB), dt)
+
+
+def fill_noise(p, st: HostState, seed, step, instance=0):
+    return lib().orc_fill_noise(C.byref(p), seed, step, instance, st.fa(st.sA), st.fa(st.sB),
+                                st.zf(st.rA), st.zf(st.rB))
+
+
+def advance(p, cu, prim, nsteps, dt, noise=None, seed=1, step0=0, instance=0):
+    g = p.ngc
+    ptr = None if noise is None else noise.ctypes.data
+    return lib().orc_advance(C.byref(p), fab_of(cu, (-g,) * 3), fab_of(prim, (-g,) * 3),
+                             nsteps, dt, ptr, seed, step0, instance)
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def normal_pair(words):
+    w = (C.c_uint32 * 4)(*words)
+    a, b = C.c_double(), C.c_double()
+    lib().orc_normal_pair(w, C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def num_threads():
+    return lib().orc_num_threads()

@@ -1,0 +1,1229 @@
+/*
+ * oracle.c -- FP64 CPU restatement of FHDeX's compressible RK3 step (src_compressible).
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path, never the product.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's CPU-baseline leg may load it.
+ *
+ * Each routine restates one reference routine and keeps its floating-point operation order
+ * (left-to-right sums, the same parenthesisation, divisions where the reference divides),
+ * because the species-noise Cholesky factors a matrix that is singular in exact arithmetic:
+ * its last pivot is rounding noise, so any reordering shows up at ~1e-11 in rho*Y_k, j and
+ * rho*E.  Build with -ffp-contract=off.  Only the fully periodic branches exist here (every
+ * is_*_dirichlet_mass flag of flux.cpp:29-34 is false and every bc_vel factor is 1).
+ *
+ * Functions and the reference code they follow:
+ *   cons_to_prim     conservedPrimitiveConversions.cpp:48-89, compressible_functions.H:138-199
+ *   transport        transCoeffs.cpp:129-181, compressible_functions.H:1153-1352 (GIO),
+ *                    :647-778 (D_GIO1)
+ *   stoch_face       flux.cpp:71-1145 (+ CholeskyDecomp compressible_functions.H:460-509)
+ *   diff_face        flux.cpp:1193-1623
+ *   corner_node      flux.cpp:1627-1713
+ *   corner_face      flux.cpp:1770-1980
+ *   hyp_face         flux.cpp:2016-2175 (advection_type 1), :2180-2372 (advection_type 2)
+ *   rk3_step         timeStep.cpp:11-427 (LinComb of the noise sets at :118-128 etc.)
+ *   fill_boundary    MultiFab::FillBoundary(periodicity) on one periodic box
+ *   init_cons        compressible_functions.cpp:131-147, :292-302
+ *   noise            replaces MultiFabFillRandom.cpp:5-31 (Philox4x32-10 + Box-Muller)
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define MS FHDX_MAX_SPECIES
+
+typedef struct {
+    double* p;
+    int lo0, lo1, lo2;
+    int hi0, hi1, hi2;
+    long sy, sz, sn;
+    int nc;
+} fab_t;
+
+static fab_t F(fhdx_fab f)
+{
+    fab_t r;
+    r.p = f.p;
+    r.lo0 = f.lo[0]; r.lo1 = f.lo[1]; r.lo2 = f.lo[2];
+    r.hi0 = f.hi[0]; r.hi1 = f.hi[1]; r.hi2 = f.hi[2];
+    long nx = f.hi[0] - f.lo[0] + 1, ny = f.hi[1] - f.lo[1] + 1, nz = f.hi[2] - f.lo[2] + 1;
+    r.sy = nx;
+    r.sz = nx * ny;
+    r.sn = nx * ny * nz;
+    r.nc = f.ncomp;
+    return r;
+}
+
+#define AT(f, i, j, k, n) \
+    ((f).p[((long)(i) - (f).lo0) + ((long)(j) - (f).lo1) * (f).sy + \
+           ((long)(k) - (f).lo2) * (f).sz + (long)(n) * (f).sn])
+
+static double dmax(double a, double b) { return a > b ? a : b; }
+static double dmin(double a, double b) { return a < b ? a : b; }
+/* amrex::max(0., amrex::min(1., y)) */
+static double clamp01(double y) { return dmax(0., dmin(1., y)); }
+
+static int nvars_of(const fhdx_params* P) { return 5 + P->nspecies; }
+
+/* ------------------------------------------------------------------------------------- */
+/* EOS helpers, compressible_functions.H:138-199, 1497-1520                               */
+/* ------------------------------------------------------------------------------------- */
+
+static double get_temperature(const fhdx_params* P, double energy, const double* Yk)
+{
+    double cvmix = 0., e0mix = 0.;
+    for (int n = 0; n < P->nspecies; ++n) {
+        cvmix += Yk[n] * P->hcv[n];
+        e0mix += Yk[n] * P->e0[n];
+    }
+    return (energy - e0mix) / cvmix;
+}
+
+static double get_pressure(const fhdx_params* P, const double* Yk, double rho, double T)
+{
+    double molmix = 0.;
+    for (int n = 0; n < P->nspecies; ++n) molmix += Yk[n] / P->molmass[n];
+    molmix = 1. / molmix;
+    return rho * (P->Runiv / molmix) * T;
+}
+
+static void get_molfrac(const fhdx_params* P, const double* Yk, double* Xk)
+{
+    double molmix = 0.;
+    for (int n = 0; n < P->nspecies; ++n) molmix += Yk[n] / P->molmass[n];
+    molmix = 1. / molmix;
+    for (int n = 0; n < P->nspecies; ++n) Xk[n] = Yk[n] * molmix / P->molmass[n];
+}
+
+static double get_energy(const fhdx_params* P, const double* Yk, double T)
+{
+    double cvmix = 0., e0mix = 0.;
+    for (int n = 0; n < P->nspecies; ++n) {
+        cvmix += Yk[n] * P->hcv[n];
+        e0mix += Yk[n] * P->e0[n];
+    }
+    return e0mix + T * cvmix;
+}
+
+/* ------------------------------------------------------------------------------------- */
+/* conservedToPrimitive                                                                    */
+/* ------------------------------------------------------------------------------------- */
+
+static void c2p_cell(const fhdx_params* P, const double* c, double* pr)
+{
+    const int NS = P->nspecies;
+    double Yk[MS], Yf[MS], Xk[MS];
+    pr[0] = c[0];
+    pr[1] = c[1] / c[0];
+    pr[2] = c[2] / c[0];
+    pr[3] = c[3] / c[0];
+    double vsqr = pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3];
+    double intenergy = c[4] / c[0] - 0.5 * vsqr;
+    double sumYk = 0.;
+    for (int n = 0; n < NS; ++n) {
+        Yk[n] = c[5 + n] / c[0];
+        Yf[n] = clamp01(Yk[n]);
+        sumYk += Yf[n];
+    }
+    for (int n = 0; n < NS; ++n) Yf[n] /= sumYk;
+    pr[4] = get_temperature(P, intenergy, Yf);
+    get_molfrac(P, Yk, Xk);
+    for (int n = 0; n < NS; ++n) {
+        pr[6 + n] = Yk[n];
+        pr[6 + NS + n] = Xk[n];
+    }
+    pr[5] = get_pressure(P, Yk, pr[0], pr[4]);
+}
+
+int orc_cons_to_prim(const fhdx_params* P, fhdx_fab prim_, fhdx_fab cons_)
+{
+    fab_t prim = F(prim_), cons = F(cons_);
+    const int nx = P->n_cells[0], ny = P->n_cells[1], nz = P->n_cells[2];
+    const int nv = nvars_of(P), npv = 6 + 2 * P->nspecies;
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                double c[MS + 5], pr[2 * MS + 6];
+                for (int n = 0; n < nv; ++n) c[n] = AT(cons, i, j, k, n);
+                c2p_cell(P, c, pr);
+                for (int n = 0; n < npv; ++n) AT(prim, i, j, k, n) = pr[n];
+            }
+    return FHDX_OK;
+}
+
+/* ------------------------------------------------------------------------------------- */
+/* Transport: IdealMixtureTransportGIO + D_GIO1                                            */
+/* ------------------------------------------------------------------------------------- */
+
+/* Gaussian elimination without pivoting followed by back substitution, as written at
+ * compressible_functions.H:1254-1271. */
+static void gio_solve(int NS, double amat[MS][MS], double* Bi)
+{
+    for (int j = 0; j < NS - 1; ++j)
+        for (int i = j + 1; i < NS; ++i) {
+            double fact1 = amat[i][j] / amat[j][j];
+            for (int k = 0; k < NS; ++k) amat[i][k] -= fact1 * amat[j][k];
+            Bi[i] -= fact1 * Bi[j];
+        }
+    Bi[NS - 1] /= amat[NS - 1][NS - 1];
+    for (int i = NS - 2; i > -1; --i) {
+        double sum1 = 0.0;
+        for (int j = NS - 1; j > i; --j) sum1 += amat[i][j] * Bi[j];
+        Bi[i] = (Bi[i] - sum1) / amat[i][i];
+    }
+}
+
+/* D_GIO1, compressible_functions.H:647-778; out[j*NS+i] = Diff_ij * Ykp_i */
+static void d_gio1(int NS, double Dbin[MS][MS], const double* Ykp, const double* Xkp,
+                   double* out)
+{
+    double Di[MS], Minv[MS], Mmat[MS];
+    double Pm[MS][MS], Zm[MS][MS], Jm[MS][MS], PJ[MS][MS], m1[MS][MS], m2[MS][MS];
+    double Diff[MS][MS];
+    for (int i = 0; i < NS; ++i) {
+        double term2 = 0.;
+        for (int j = 0; j < NS; ++j)
+            if (j != i) term2 = term2 + Xkp[j] / Dbin[i][j];
+        Di[i] = (1. - Ykp[i]) / term2;
+    }
+    for (int i = 0; i < NS; ++i) {
+        Mmat[i] = Xkp[i] / Di[i];
+        Minv[i] = Di[i] / Xkp[i];
+    }
+    for (int i = 0; i < NS; ++i)
+        for (int j = 0; j < NS; ++j) {
+            Pm[i][j] = -Ykp[j];
+            if (i == j) Pm[i][j] = Pm[i][j] + 1.;
+        }
+    for (int i = 0; i < NS; ++i)
+        for (int j = 0; j < NS; ++j) {
+            double Delta;
+            if (i == j) {
+                double term1 = 0.;
+                for (int k = 0; k < NS; ++k)
+                    if (k != i) term1 = term1 + Xkp[i] * Xkp[k] / Dbin[i][k];
+                Delta = term1;
+            } else {
+                Delta = -Xkp[i] * Xkp[j] / Dbin[i][j];
+            }
+            Zm[i][j] = -Delta;
+        }
+    for (int i = 0; i < NS; ++i) Zm[i][i] = Zm[i][i] + Mmat[i];
+    for (int i = 0; i < NS; ++i)
+        for (int j = 0; j < NS; ++j) Jm[i][j] = Minv[i] * Zm[i][j];
+    for (int i = 0; i < NS; ++i)
+        for (int j = 0; j < NS; ++j) {
+            PJ[i][j] = 0.;
+            for (int k = 0; k < NS; ++k) PJ[i][j] = PJ[i][j] + Pm[i][k] * Jm[k][j];
+        }
+    for (int i = 0; i < NS; ++i)
+        for (int j = 0; j < NS; ++j) {
+            double scr = 0.;
+            for (int k = 0; k < NS; ++k) scr = scr + Pm[i][k] * Minv[k] * Pm[j][k];
+            m2[i][j] = scr;
+            Diff[i][j] = scr;
+        }
+    for (int ll = 0; ll < 3; ++ll) {
+        for (int i = 0; i < NS; ++i)
+            for (int j = 0; j < NS; ++j) {
+                double scr = 0.;
+                for (int k = 0; k < NS; ++k) scr = scr + PJ[i][k] * Diff[k][j];
+                m1[i][j] = scr + m2[i][j];
+            }
+        for (int i = 0; i < NS; ++i)
+            for (int j = 0; j < NS; ++j) Diff[i][j] = m1[i][j];
+    }
+    for (int i = 0; i < NS; ++i)
+        for (int j = 0; j < NS; ++j) out[j * NS + i] = Diff[i][j] * Ykp[i];
+}
+
+/* IdealMixtureTransportGIO (compressible_functions.H:1153-1352) followed by the rho scaling
+ * and bulk viscosity of transCoeffs.cpp:167-178.  Returns 1 if a mass fraction is outside
+ * (0,1) (the reference aborts at transCoeffs.cpp:137-138). */
+static int transport_cell(const fhdx_params* P, const double* pr, double* eta, double* zeta,
+                          double* kappa, double* chi, double* D)
+{
+    const int NS = P->nspecies;
+    double Yk[MS];
+    double sumYk = 0.;
+    int bad = 0;
+    for (int n = 0; n < NS; ++n) {
+        double y = pr[6 + n];
+        if (y <= 0.0) bad = 1;
+        if (y >= 1.0) bad = 1;
+        Yk[n] = clamp01(y);
+        sumYk += Yk[n];
+    }
+    for (int n = 0; n < NS; ++n) Yk[n] /= sumYk;
+
+    const double T = pr[4], p = pr[5];
+    const double pi = 3.1415926535897932;
+    double mm[MS], Xk[MS], Xkp[MS], Ykp[MS], eta1[MS], rhs[MS], Bi[MS];
+    double mu[MS][MS], diam[MS][MS], Dbin[MS][MS], amat[MS][MS];
+    for (int n = 0; n < NS; ++n) mm[n] = P->molmass[n] * (P->k_B / P->Runiv);
+    double mbar = 0.0;
+    for (int n = 0; n < NS; ++n) mbar += Yk[n] / mm[n];
+    mbar = 1.0 / mbar;
+    double Xksum = 0.0;
+    for (int n = 0; n < NS; ++n) {
+        Xk[n] = Yk[n] * mbar / mm[n];
+        Xksum += Xk[n];
+    }
+    for (int n = 0; n < NS; ++n) Xkp[n] = Xk[n] + 1.0e-16 * (Xksum / (double)NS - Xk[n]);
+    for (int n = 0; n < NS; ++n) Ykp[n] = Xkp[n] * mm[n] / mbar;
+    for (int i = 0; i < NS; ++i)
+        for (int j = 0; j < NS; ++j) {
+            diam[i][j] = 0.5 * (P->diameter[i] + P->diameter[j]);
+            mu[i][j] = mm[i] * mm[j] / (mm[i] + mm[j]);
+        }
+    const double kT = P->k_B * T;
+    for (int i = 0; i < NS; ++i)
+        for (int j = 0; j < NS; ++j)
+            Dbin[i][j] = (3.0 / 16.0) * sqrt(2.0 * pi * kT * kT * kT / mu[i][j]) /
+                         (p * pi * diam[i][j] * diam[i][j]);
+    const double AKL = 1.0, BKL = 1.0, CKL = 1.0;
+
+    /* viscosity */
+    for (int i = 0; i < NS; ++i)
+        eta1[i] = 5.0 / (16.0 * diam[i][i] * diam[i][i]) * sqrt(mm[i] * P->k_B * T / pi);
+    for (int i = 0; i < NS; ++i) {
+        rhs[i] = Xkp[i];
+        Bi[i] = rhs[i];
+        for (int j = 0; j < NS; ++j) {
+            if (i == j) {
+                amat[i][i] = Xkp[i] * Xkp[i] / eta1[i];
+                for (int k = 0; k < NS; ++k)
+                    if (k != i)
+                        amat[i][i] += 2.0 * kT * Xkp[i] * Xkp[k] /
+                                      (p * Dbin[i][k] * (mm[i] + mm[k])) *
+                                      (1.0 + 0.6 * mm[k] / mm[i] * AKL);
+            } else {
+                amat[i][j] = 2.0 * kT * Xkp[i] * Xkp[j] / (p * Dbin[i][j] * (mm[i] + mm[j])) *
+                             (0.6 * AKL - 1.0);
+            }
+        }
+    }
+    gio_solve(NS, amat, Bi);
+    double e = 0.0;
+    for (int i = 0; i < NS; ++i) e += Xkp[i] * Bi[i];
+    *eta = e;
+    *zeta = 0.;
+
+    /* thermal conductivity */
+    for (int i = 0; i < NS; ++i) {
+        rhs[i] = 2.5 * Xkp[i];
+        Bi[i] = rhs[i];
+        for (int j = 0; j < NS; ++j) {
+            if (i == j) {
+                amat[i][i] = Xkp[i] * Xkp[i] / Dbin[i][j] * 2.0 * AKL;
+                for (int k = 0; k < NS; ++k)
+                    if (k != i)
+                        amat[i][i] += Xkp[i] * Xkp[k] / Dbin[i][k] * mu[i][k] / (mm[i] + mm[k]) *
+                                      (7.5 * mm[i] / mm[k] + 6.25 * mm[k] / mm[i] -
+                                       3.0 * mm[k] / mm[i] * BKL + 4.0 * AKL);
+            } else {
+                amat[i][j] = -1.0 * Xkp[i] * Xkp[j] / Dbin[i][j] * mu[i][j] / (mm[i] + mm[j]) *
+                             (55.0 / 4.0 - 3.0 * BKL - 4.0 * AKL);
+            }
+        }
+    }
+    gio_solve(NS, amat, Bi);
+    double kap = 0.0;
+    for (int i = 0; i < NS; ++i) kap += rhs[i] * Bi[i];
+    kap *= p / T;
+    *kappa = kap;
+
+    /* thermal diffusion ratios */
+    for (int i = 0; i < NS; ++i) {
+        for (int j = 0; j < NS; ++j) {
+            if (i == j) {
+                amat[i][i] = 0.0;
+                for (int k = 0; k < NS; ++k)
+                    if (k != i)
+                        amat[i][i] -= 0.5 * Xkp[k] / Dbin[i][k] * mm[k] / (mm[i] + mm[k]) *
+                                      (6.0 * CKL - 5.0);
+            } else {
+                amat[i][j] =
+                    0.5 * Xkp[j] / Dbin[i][j] * mm[i] / (mm[i] + mm[j]) * (6.0 * CKL - 5.0);
+            }
+        }
+    }
+    for (int i = 0; i < NS; ++i) {
+        double c = 0.0;
+        for (int j = 0; j < NS; ++j) c += amat[i][j] * Bi[j];
+        chi[i] = c;
+    }
+
+    d_gio1(NS, Dbin, Ykp, Xkp, D);
+    for (int kk = 0; kk < NS; ++kk)
+        for (int ll = 0; ll < NS; ++ll) D[kk * NS + ll] *= pr[0];
+    if (abs(P->visc_type) == 3) *zeta = P->zeta_ratio * *eta;
+    return bad;
+}
+
+int orc_transport(const fhdx_params* P, fhdx_fab prim_, fhdx_fab eta_, fhdx_fab zeta_,
+                  fhdx_fab kappa_, fhdx_fab chi_, fhdx_fab D_)
+{
+    fab_t prim = F(prim_), eta = F(eta_), zeta = F(zeta_), kappa = F(kappa_), chi = F(chi_),
+          D = F(D_);
+    const int NS = P->nspecies, npv = 6 + 2 * NS, g = P->ngc;
+    const int nx = P->n_cells[0], ny = P->n_cells[1], nz = P->n_cells[2];
+    int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+    for (int k = -g; k < nz + g; ++k)
+        for (int j = -g; j < ny + g; ++j)
+            for (int i = -g; i < nx + g; ++i) {
+                double pr[2 * MS + 6], c[MS], d[MS * MS], e, z, kp;
+                for (int n = 0; n < npv; ++n) pr[n] = AT(prim, i, j, k, n);
+                bad |= transport_cell(P, pr, &e, &z, &kp, c, d);
+                AT(eta, i, j, k, 0) = e;
+                AT(zeta, i, j, k, 0) = z;
+                AT(kappa, i, j, k, 0) = kp;
+                for (int n = 0; n < NS; ++n) AT(chi, i, j, k, n) = c[n];
+                for (int n = 0; n < NS * NS; ++n) AT(D, i, j, k, n) = d[n];
+            }
+    return bad ? FHDX_ERR_MASSFRAC : FHDX_OK;
+}
+
+/* ------------------------------------------------------------------------------------- */
+/* Periodic ghost fill                                                                     */
+/* ------------------------------------------------------------------------------------- */
+
+static int wrap(int i, int n) { return ((i % n) + n) % n; }
+
+int orc_fill_boundary(const fhdx_params* P, fhdx_fab mf_)
+{
+    fab_t f = F(mf_);
+    const int nx = P->n_cells[0], ny = P->n_cells[1], nz = P->n_cells[2];
+#pragma omp parallel for schedule(static)
+    for (int k = f.lo2; k <= f.hi2; ++k)
+        for (int j = f.lo1; j <= f.hi1; ++j)
+            for (int i = f.lo0; i <= f.hi0; ++i) {
+                int iv = wrap(i, nx), jv = wrap(j, ny), kv = wrap(k, nz);
+                if (iv == i && jv == j && kv == k) continue;
+                for (int n = 0; n < f.nc; ++n) AT(f, i, j, k, n) = AT(f, iv, jv, kv, n);
+            }
+    return FHDX_OK;
+}
+
+/* ------------------------------------------------------------------------------------- */
+/* calculateFlux                                                                           */
+/* ------------------------------------------------------------------------------------- */
+
+/* Offsets relative to the face index (i,j,k) of the 8 cells summed, in the reference's
+ * order, for the four edge viscosities of a face (pp, mp, pm, mm), x/y/z faces:
+ * flux.cpp:117-135 (x), :481-499 (y), :847-865 (z).  Each entry: eta offset then T offset
+ * (they differ only in the z-face pm quirk of flux.cpp:857-858). */
+static const int MU_ORDER[3][4][8][6] = {
+    { /* x */
+      {{0,0,0,0,0,0},{-1,0,0,-1,0,0},{0,1,0,0,1,0},{-1,1,0,-1,1,0},{0,0,1,0,0,1},{-1,0,1,-1,0,1},{0,1,1,0,1,1},{-1,1,1,-1,1,1}},
+      {{0,-1,0,0,-1,0},{-1,-1,0,-1,-1,0},{0,0,0,0,0,0},{-1,0,0,-1,0,0},{0,-1,1,0,-1,1},{-1,-1,1,-1,-1,1},{0,0,1,0,0,1},{-1,0,1,-1,0,1}},
+      {{0,0,-1,0,0,-1},{-1,0,-1,-1,0,-1},{0,1,-1,0,1,-1},{-1,1,-1,-1,1,-1},{0,0,0,0,0,0},{-1,0,0,-1,0,0},{0,1,0,0,1,0},{-1,1,0,-1,1,0}},
+      {{0,-1,-1,0,-1,-1},{-1,-1,-1,-1,-1,-1},{0,0,-1,0,0,-1},{-1,0,-1,-1,0,-1},{0,-1,0,0,-1,0},{-1,-1,0,-1,-1,0},{0,0,0,0,0,0},{-1,0,0,-1,0,0}} },
+    { /* y */
+      {{1,-1,0,1,-1,0},{0,-1,0,0,-1,0},{1,0,0,1,0,0},{0,0,0,0,0,0},{1,-1,1,1,-1,1},{0,-1,1,0,-1,1},{1,0,1,1,0,1},{0,0,1,0,0,1}},
+      {{-1,0,0,-1,0,0},{0,0,0,0,0,0},{-1,-1,0,-1,-1,0},{0,-1,0,0,-1,0},{-1,0,1,-1,0,1},{0,0,1,0,0,1},{-1,-1,1,-1,-1,1},{0,-1,1,0,-1,1}},
+      {{1,-1,-1,1,-1,-1},{0,-1,-1,0,-1,-1},{1,0,-1,1,0,-1},{0,0,-1,0,0,-1},{1,-1,0,1,-1,0},{0,-1,0,0,-1,0},{1,0,0,1,0,0},{0,0,0,0,0,0}},
+      {{-1,0,-1,-1,0,-1},{0,0,-1,0,0,-1},{-1,-1,-1,-1,-1,-1},{0,-1,-1,0,-1,-1},{-1,0,0,-1,0,0},{0,0,0,0,0,0},{-1,-1,0,-1,-1,0},{0,-1,0,0,-1,0}} },
+    { /* z */
+      {{1,0,-1,1,0,-1},{0,0,-1,0,0,-1},{1,1,-1,1,1,-1},{0,1,-1,0,1,-1},{1,0,0,1,0,0},{0,0,0,0,0,0},{1,1,0,1,1,0},{0,1,0,0,1,0}},
+      {{-1,1,-1,-1,1,-1},{0,1,-1,0,1,-1},{-1,0,-1,-1,0,-1},{0,0,-1,0,0,-1},{-1,1,0,-1,1,0},{0,1,0,0,1,0},{-1,0,0,-1,0,0},{0,0,0,0,0,0}},
+      {{1,0,0,1,0,0},{0,0,-2,0,0,0},{1,-1,0,1,-1,0},{0,-1,-2,0,-1,0},{1,0,-1,1,0,-1},{0,0,-1,0,0,-1},{1,-1,-1,1,-1,-1},{0,-1,-1,0,-1,-1}},
+      {{-1,-1,0,-1,-1,0},{0,-1,0,0,-1,0},{-1,0,0,-1,0,0},{0,0,0,0,0,0},{-1,-1,-1,-1,-1,-1},{0,-1,-1,0,-1,-1},{-1,0,-1,-1,0,-1},{0,0,-1,0,0,-1}} }
+};
+/* node (rancorn) offsets of the pp, mp, pm, mm edges: flux.cpp:277-280, 644-647, 1011-1014 */
+static const int RC_NODE[3][4][3] = {
+    {{0,1,1},{0,0,1},{0,1,0},{0,0,0}},
+    {{1,0,1},{0,0,1},{1,0,0},{0,0,0}},
+    {{1,1,0},{0,1,0},{1,0,0},{0,0,0}}
+};
+
+typedef struct {
+    const fhdx_params* P;
+    fab_t cons, prim, eta, zeta, kappa, chi, D;
+    fab_t flux[3], ran[3], corn[3][3], visccorn, rancorn;
+    double dt;
+    int nv, NS;
+} flux_args;
+
+/* CholeskyDecomp (compressible_functions.H:460-509).  Only the lower-triangle entries and the
+ * pivots feed the result; the reference's writes into the upper triangle never do. */
+static void cholesky(int NS, double* a, double* sqda)
+{
+    double p[MS];
+    for (int i = 0; i < NS; ++i) {
+        int ising = 0;
+        for (int j = i; j < NS; ++j) {
+            double sum1 = a[i * NS + j];
+            for (int k = i - 1; k >= 0; --k) sum1 = sum1 - a[i * NS + k] * a[j * NS + k];
+            if (i == j) {
+                if (sum1 <= 0.) {
+                    p[i] = 0.;
+                    ising = 1;
+                } else {
+                    p[i] = sqrt(sum1);
+                }
+            }
+            if (ising == 0) a[j * NS + i] = sum1 / p[i];
+            else a[j * NS + i] = 0.;
+        }
+    }
+    for (int i = 0; i < NS * NS; ++i) sqda[i] = 0.;
+    for (int i = 0; i < NS; ++i) {
+        for (int j = i - 1; j >= 0; --j) sqda[i * NS + j] = a[i * NS + j];
+        sqda[i * NS + i] = p[i];
+    }
+}
+
+/* Stochastic flux of one face (flux.cpp:71-1145, periodic branches). */
+static void stoch_face(const flux_args* A, int d, int i, int j, int k)
+{
+    const fhdx_params* P = A->P;
+    const int NS = A->NS, nv = A->nv;
+    const int di = (d == 0), dj = (d == 1), dk = (d == 2);
+    const int il = i - di, jl = j - dj, kl = k - dk; /* lower cell; upper cell is (i,j,k) */
+    const fab_t *eta = &A->eta, *zeta = &A->zeta, *prim = &A->prim, *kappa = &A->kappa;
+    const fab_t *ran = &A->ran[d], *rc = &A->rancorn;
+    fab_t* fl = (fab_t*)&A->flux[d];
+    const double volinv = 1. / (P->dx[0] * P->dx[1] * P->dx[2]);
+    const double dtinv = 1. / A->dt;
+
+    double Tu = AT(*prim, i, j, k, 4), Tl = AT(*prim, il, jl, kl, 4);
+    double mup = AT(*eta, i, j, k, 0) * Tu + AT(*eta, il, jl, kl, 0) * Tl;
+    double kp = AT(*kappa, i, j, k, 0) * Tu * Tu + AT(*kappa, il, jl, kl, 0) * Tl * Tl;
+    double meanT = 0.5 * (Tu + Tl);
+    double fw[MS + 5], w[MS + 5];
+    fw[0] = 0;
+    fw[1] = sqrt(P->k_B * mup * volinv * dtinv);
+    fw[2] = fw[1];
+    fw[3] = fw[1];
+    fw[4] = sqrt(P->k_B * kp * volinv * dtinv);
+    for (int n = 0; n < 5; ++n) w[n] = fw[n] * AT(*ran, i, j, k, n);
+    double nweight = sqrt(P->k_B * volinv * dtinv);
+
+    double mus[4];
+    for (int c = 0; c < 4; ++c) {
+        const int(*o)[6] = MU_ORDER[d][c];
+        double s = AT(*eta, i + o[0][0], j + o[0][1], k + o[0][2], 0) *
+                   AT(*prim, i + o[0][3], j + o[0][4], k + o[0][5], 4);
+        for (int t = 1; t < 8; ++t)
+            s = s + AT(*eta, i + o[t][0], j + o[t][1], k + o[t][2], 0) *
+                        AT(*prim, i + o[t][3], j + o[t][4], k + o[t][5], 4);
+        mus[c] = 0.25 * s / 3.;
+        if (abs(P->visc_type) == 3) {
+            double z = AT(*zeta, i + o[0][0], j + o[0][1], k + o[0][2], 0) *
+                       AT(*prim, i + o[0][3], j + o[0][4], k + o[0][5], 4);
+            for (int t = 1; t < 8; ++t)
+                z = z + AT(*zeta, i + o[t][0], j + o[t][1], k + o[t][2], 0) *
+                            AT(*prim, i + o[t][3], j + o[t][4], k + o[t][5], 4);
+            mus[c] += 0.25 * z;
+        }
+    }
+    double sum = 0.;
+    for (int c = 0; c < 4; ++c) {
+        const int* q = RC_NODE[d][c];
+        double term = sqrt(mus[c]) * AT(*rc, i + q[0], j + q[1], k + q[2], 0);
+        sum = (c == 0) ? term : sum + term;
+    }
+    w[1 + d] = w[1 + d] + 0.25 * nweight * sum;
+
+    for (int n = 1; n < 4; ++n) AT(*fl, i, j, k, n) = AT(*fl, i, j, k, n) + w[n];
+    AT(*fl, i, j, k, nv) = AT(*fl, i, j, k, nv) + w[4];
+
+    /* viscous heating: the normal component first, the two tangential ones after */
+    const int t1 = (d == 0) ? 2 : 1, t2 = (d == 2) ? 2 : 3;
+    double phidiag = w[1 + d] * (AT(*prim, il, jl, kl, 1 + d) + AT(*prim, i, j, k, 1 + d));
+    double phishear = w[t1] * (AT(*prim, il, jl, kl, t1) + AT(*prim, i, j, k, t1)) +
+                      w[t2] * (AT(*prim, il, jl, kl, t2) + AT(*prim, i, j, k, t2));
+    phidiag = -0.5 * phidiag;
+    phishear = -0.5 * phishear;
+    AT(*fl, i, j, k, nv + 1) = AT(*fl, i, j, k, nv + 1) - phidiag;
+    AT(*fl, i, j, k, nv + 2) = AT(*fl, i, j, k, nv + 2) - phishear;
+
+    if (P->algorithm_type == 2) {
+        double yy[MS], yyp[MS], DijY[MS * MS], sqD[MS * MS], hk[MS];
+        for (int n = 5; n < 5 + NS; ++n) w[n] = 0.;
+        for (int s = 0; s < NS; ++s) {
+            yy[s] = clamp01(AT(*prim, il, jl, kl, 6 + s));
+            yyp[s] = clamp01(AT(*prim, i, j, k, 6 + s));
+        }
+        double sumy = 0., sumyp = 0.;
+        for (int n = 0; n < NS; ++n) {
+            sumy += yy[n];
+            sumyp += yyp[n];
+        }
+        for (int n = 0; n < NS; ++n) {
+            yy[n] /= sumy;
+            yyp[n] /= sumyp;
+        }
+        double MWmix = 0.;
+        for (int s = 0; s < NS; ++s) {
+            MWmix = MWmix + 0.5 * (yy[s] + yyp[s]) / P->molmass[s];
+            for (int l = 0; l < NS; ++l)
+                DijY[s * NS + l] = 0.5 * (AT(A->D, il, jl, kl, l * NS + s) * yy[l] +
+                                          AT(A->D, i, j, k, l * NS + s) * yyp[l] +
+                                          (AT(A->D, il, jl, kl, s * NS + l) * yy[s] +
+                                           AT(A->D, i, j, k, s * NS + l) * yyp[s]));
+        }
+        for (int s = 0; s < NS; ++s)
+            if (fabs(yy[s]) + fabs(yyp[s]) <= 1.e-12)
+                for (int n = 0; n < NS; ++n) {
+                    DijY[s * NS + n] = 0.;
+                    DijY[n * NS + s] = 0.;
+                }
+        MWmix = 1. / MWmix;
+        cholesky(NS, DijY, sqD);
+        for (int s = 0; s < NS; ++s) {
+            for (int l = 0; l <= s; ++l) {
+                fw[5 + l] = sqrt(P->k_B * MWmix * volinv / (P->Runiv * A->dt)) * sqD[s * NS + l];
+                w[5 + s] = w[5 + s] + fw[5 + l] * AT(*ran, i, j, k, 5 + l);
+            }
+            AT(*fl, i, j, k, 5 + s) = w[5 + s];
+        }
+        for (int s = 0; s < NS; ++s) hk[s] = P->e0[s] + P->hcp[s] * meanT;
+        double soret = 0.;
+        for (int s = 0; s < NS; ++s) {
+            double ss = (hk[s] + P->Runiv * meanT / P->molmass[s] * 0.5 *
+                                     (AT(A->chi, il, jl, kl, s) + AT(A->chi, i, j, k, s))) *
+                        w[5 + s];
+            soret = soret + ss;
+        }
+        AT(*fl, i, j, k, nv + 3) = AT(*fl, i, j, k, nv + 3) + soret;
+    }
+}
+
+/* Diffusive flux of one face (flux.cpp:1193-1623, periodic branches). */
+static void diff_face(const flux_args* A, int d, int i, int j, int k)
+{
+    const fhdx_params* P = A->P;
+    const int NS = A->NS, nv = A->nv;
+    const int il = i - (d == 0), jl = j - (d == 1), kl = k - (d == 2);
+    const fab_t *prim = &A->prim, *eta = &A->eta, *kappa = &A->kappa, *chi = &A->chi;
+    fab_t* fl = (fab_t*)&A->flux[d];
+    const double half = 0.5, h = P->dx[d];
+#define PU(n) AT(*prim, i, j, k, n)
+#define PL(n) AT(*prim, il, jl, kl, n)
+    double mup = half * (AT(*eta, i, j, k, 0) + AT(*eta, il, jl, kl, 0));
+    double kp = half * (AT(*kappa, i, j, k, 0) + AT(*kappa, il, jl, kl, 0));
+    double tau[4];
+    for (int c = 1; c < 4; ++c) tau[c] = mup * (PU(c) - PL(c)) / h;
+    double divp = 0.;
+    double s1 = PL(1) + PU(1), s2 = PL(2) + PU(2), s3 = PL(3) + PU(3);
+    double phiflx;
+    if (d == 0) phiflx = tau[1] * s1 + divp * s1 + tau[2] * s2 + tau[3] * s3;
+    else if (d == 1) phiflx = tau[1] * s1 + tau[2] * s2 + divp * s2 + tau[3] * s3;
+    else phiflx = tau[1] * s1 + tau[2] * s2 + tau[3] * s3 + divp * s3;
+    double Qflux = kp * (PU(4) - PL(4)) / h;
+    for (int c = 1; c < 4; ++c) {
+        double t = (c == 1 + d) ? (tau[c] + divp) : tau[c];
+        AT(*fl, i, j, k, c) = AT(*fl, i, j, k, c) - t;
+    }
+    AT(*fl, i, j, k, nv) = AT(*fl, i, j, k, nv) - Qflux;
+    AT(*fl, i, j, k, nv + 1) = AT(*fl, i, j, k, nv + 1) - (half * phiflx);
+
+    double meanT = 0.5 * (PL(4) + PU(4));
+    double meanP = 0.5 * (PL(5) + PU(5));
+    if (P->algorithm_type == 2) {
+        double dk[MS], soret[MS], Fk[MS], hk[MS];
+        for (int s = 0; s < NS; ++s) {
+            double term1 = (PU(6 + NS + s) - PL(6 + NS + s)) / h;
+            double mX = 0.5 * (PL(6 + NS + s) + PU(6 + NS + s));
+            double mY = 0.5 * (PL(6 + s) + PU(6 + s));
+            double term2 = (mX - mY) * (PU(5) - PL(5)) / h / meanP;
+            dk[s] = term1 + term2;
+            double ChiX;
+            if (d == 2) /* flux.cpp:1561 reads chi at k and k+1 */
+                ChiX = 0.5 * (AT(*chi, i, j, k, s) * PL(6 + NS + s) +
+                              AT(*chi, i, j, k + 1, s) * PU(6 + NS + s));
+            else
+                ChiX = 0.5 * (AT(*chi, il, jl, kl, s) * PL(6 + NS + s) +
+                              AT(*chi, i, j, k, s) * PU(6 + NS + s));
+            soret[s] = ChiX * (PU(4) - PL(4)) / h / meanT;
+        }
+        for (int kk = 0; kk < NS; ++kk) {
+            Fk[kk] = 0.;
+            for (int ll = 0; ll < NS; ++ll) {
+                double Fks = half * (AT(A->D, il, jl, kl, ll * NS + kk) +
+                                     AT(A->D, i, j, k, ll * NS + kk)) *
+                             (dk[ll] + soret[ll]);
+                Fk[kk] = Fk[kk] - Fks;
+            }
+        }
+        for (int s = 0; s < NS; ++s) hk[s] = P->e0[s] + P->hcp[s] * meanT;
+        double Q5 = 0.;
+        for (int s = 0; s < NS; ++s) {
+            double csum = (d == 2) /* flux.cpp:1604 sums chi(i,j,k) twice */
+                              ? AT(*chi, i, j, k, s) + AT(*chi, i, j, k, s)
+                              : AT(*chi, il, jl, kl, s) + AT(*chi, i, j, k, s);
+            double Q5s = (hk[s] + 0.5 * P->Runiv * meanT * csum / P->molmass[s]) * Fk[s];
+            Q5 = Q5 + Q5s;
+        }
+        AT(*fl, i, j, k, nv + 3) = AT(*fl, i, j, k, nv + 3) + Q5;
+        for (int s = 0; s < NS; ++s) AT(*fl, i, j, k, 5 + s) = AT(*fl, i, j, k, 5 + s) + Fk[s];
+    }
+#undef PU
+#undef PL
+}
+
+/* Corner (nodal) viscous terms at node (i,j,k) (flux.cpp:1627-1713). */
+static void corner_node(const flux_args* A, int i, int j, int k)
+{
+    const fhdx_params* P = A->P;
+    const fab_t *eta = &A->eta, *zeta = &A->zeta, *pr = &A->prim;
+#define E(a, b, c) AT(*eta, a, b, c, 0)
+#define Z(a, b, c) AT(*zeta, a, b, c, 0)
+#define V(a, b, c, n) AT(*pr, a, b, c, n)
+    double muxp = 0.125 * (E(i, j - 1, k - 1) + E(i - 1, j - 1, k - 1) + E(i, j, k - 1) +
+                           E(i - 1, j, k - 1) + E(i, j - 1, k) + E(i - 1, j - 1, k) + E(i, j, k) +
+                           E(i - 1, j, k));
+    double zetaxp;
+    if (abs(P->visc_type) == 3)
+        zetaxp = 0.125 * (Z(i, j - 1, k - 1) + Z(i - 1, j - 1, k - 1) + Z(i, j, k - 1) +
+                          Z(i - 1, j, k - 1) + Z(i, j - 1, k) + Z(i - 1, j - 1, k) + Z(i, j, k) +
+                          Z(i - 1, j, k));
+    else
+        zetaxp = 0.;
+    const double DX0 = P->dx[0], DX1 = P->dx[1], DX2 = P->dx[2];
+    double gx[4], gy[4], gz[4]; /* index 1..3: u, v, w */
+    for (int n = 1; n < 4; ++n) {
+        gx[n] = 0.25 * muxp *
+                (V(i, j - 1, k - 1, n) - V(i - 1, j - 1, k - 1, n) + V(i, j, k - 1, n) -
+                 V(i - 1, j, k - 1, n) + V(i, j - 1, k, n) - V(i - 1, j - 1, k, n) + V(i, j, k, n) -
+                 V(i - 1, j, k, n)) /
+                DX0;
+        gy[n] = 0.25 * muxp *
+                (V(i - 1, j, k - 1, n) - V(i - 1, j - 1, k - 1, n) + V(i, j, k - 1, n) -
+                 V(i, j - 1, k - 1, n) + V(i - 1, j, k, n) - V(i - 1, j - 1, k, n) + V(i, j, k, n) -
+                 V(i, j - 1, k, n)) /
+                DX1;
+        gz[n] = 0.25 * muxp *
+                (V(i - 1, j - 1, k, n) - V(i - 1, j - 1, k - 1, n) + V(i, j - 1, k, n) -
+                 V(i, j - 1, k - 1, n) + V(i - 1, j, k, n) - V(i - 1, j, k - 1, n) + V(i, j, k, n) -
+                 V(i, j, k - 1, n)) /
+                DX2;
+    }
+    /* corn[a][b]: a = derivative direction x/y/z, b = component u/v/w (cornx_in[b] holds
+     * d/dx of component b, flux.cpp:1173-1181) */
+    for (int n = 1; n < 4; ++n) {
+        AT(A->corn[0][n - 1], i, j, k, 0) = gx[n];
+        AT(A->corn[1][n - 1], i, j, k, 0) = gy[n];
+        AT(A->corn[2][n - 1], i, j, k, 0) = gz[n];
+    }
+    AT(A->visccorn, i, j, k, 0) =
+        (muxp / 12. + zetaxp / 4.) *
+        ((V(i, j - 1, k - 1, 1) - V(i - 1, j - 1, k - 1, 1)) / DX0 +
+         (V(i, j, k - 1, 1) - V(i - 1, j, k - 1, 1)) / DX0 +
+         (V(i, j - 1, k, 1) - V(i - 1, j - 1, k, 1)) / DX0 + (V(i, j, k, 1) - V(i - 1, j, k, 1)) / DX0 +
+         (V(i - 1, j, k - 1, 2) - V(i - 1, j - 1, k - 1, 2)) / DX1 +
+         (V(i, j, k - 1, 2) - V(i, j - 1, k - 1, 2)) / DX1 +
+         (V(i - 1, j, k, 2) - V(i - 1, j - 1, k, 2)) / DX1 + (V(i, j, k, 2) - V(i, j - 1, k, 2)) / DX1 +
+         (V(i - 1, j - 1, k, 3) - V(i - 1, j - 1, k - 1, 3)) / DX2 +
+         (V(i, j - 1, k, 3) - V(i, j - 1, k - 1, 3)) / DX2 +
+         (V(i - 1, j, k, 3) - V(i - 1, j, k - 1, 3)) / DX2 + (V(i, j, k, 3) - V(i, j, k - 1, 3)) / DX2);
+#undef E
+#undef Z
+#undef V
+}
+
+/* Corner-to-face terms (flux.cpp:1770-1980).  Node offsets (transverse a,b) summed in the
+ * reference order: visccorn always (hi,hi),(lo,hi),(hi,lo),(lo,lo); the corner arrays in the
+ * same order on x/y faces and (hi,hi),(hi,lo),(lo,hi),(lo,lo) on z faces. */
+static void corner_face(const flux_args* A, int d, int i, int j, int k)
+{
+    const int nv = A->nv;
+    const fab_t* pr = &A->prim;
+    fab_t* fl = (fab_t*)&A->flux[d];
+    const int il = i - (d == 0), jl = j - (d == 1), kl = k - (d == 2);
+    /* transverse unit offsets */
+    int ta[3], tb[3];
+    if (d == 0) { ta[0] = 0; ta[1] = 1; ta[2] = 0; tb[0] = 0; tb[1] = 0; tb[2] = 1; }
+    else if (d == 1) { ta[0] = 1; ta[1] = 0; ta[2] = 0; tb[0] = 0; tb[1] = 0; tb[2] = 1; }
+    else { ta[0] = 1; ta[1] = 0; ta[2] = 0; tb[0] = 0; tb[1] = 1; tb[2] = 0; }
+    static const int vc_ord[4][2] = {{1, 1}, {0, 1}, {1, 0}, {0, 0}};
+    static const int zc_ord[4][2] = {{1, 1}, {1, 0}, {0, 1}, {0, 0}};
+    const int(*cord)[2] = (d == 2) ? zc_ord : vc_ord;
+#define NODE(f, o, q) AT(f, i + (o)[q][0] * ta[0] + (o)[q][1] * tb[0], \
+                         j + (o)[q][0] * ta[1] + (o)[q][1] * tb[1], \
+                         k + (o)[q][0] * ta[2] + (o)[q][1] * tb[2], 0)
+    double vc4 = NODE(A->visccorn, vc_ord, 0);
+    for (int q = 1; q < 4; ++q) vc4 = vc4 + NODE(A->visccorn, vc_ord, q);
+    /* normal momentum: transverse divergence pair; x: (vy, wz); y: (ux, wz); z: (vy, ux) */
+    const fab_t *c1, *c2, *ta1, *ta2;
+    int tc1, tc2; /* tangential momentum components */
+    if (d == 0) { c1 = &A->corn[1][1]; c2 = &A->corn[2][2]; ta1 = &A->corn[1][0]; ta2 = &A->corn[2][0]; tc1 = 2; tc2 = 3; }
+    else if (d == 1) { c1 = &A->corn[0][0]; c2 = &A->corn[2][2]; ta1 = &A->corn[0][1]; ta2 = &A->corn[2][1]; tc1 = 1; tc2 = 3; }
+    else { c1 = &A->corn[1][1]; c2 = &A->corn[0][0]; ta1 = &A->corn[0][2]; ta2 = &A->corn[1][2]; tc1 = 1; tc2 = 2; }
+    double div8 = NODE(*c1, cord, 0);
+    for (int q = 1; q < 4; ++q) div8 = div8 + NODE(*c1, cord, q);
+    for (int q = 0; q < 4; ++q) div8 = div8 + NODE(*c2, cord, q);
+    double s1 = NODE(*ta1, cord, 0), s2 = NODE(*ta2, cord, 0);
+    for (int q = 1; q < 4; ++q) {
+        s1 = s1 + NODE(*ta1, cord, q);
+        s2 = s2 + NODE(*ta2, cord, q);
+    }
+#undef NODE
+    const int cn = 1 + d;
+    AT(*fl, i, j, k, cn) = AT(*fl, i, j, k, cn) - 0.25 * vc4;
+    AT(*fl, i, j, k, cn) = AT(*fl, i, j, k, cn) + .25 * div8;
+    AT(*fl, i, j, k, tc1) = AT(*fl, i, j, k, tc1) - .25 * s1;
+    AT(*fl, i, j, k, tc2) = AT(*fl, i, j, k, tc2) - .25 * s2;
+    double phiflx = 0.25 * (vc4 - div8) * (AT(*pr, il, jl, kl, cn) + AT(*pr, i, j, k, cn));
+    phiflx = phiflx + .25 * s1 * (AT(*pr, il, jl, kl, tc1) + AT(*pr, i, j, k, tc1));
+    phiflx = phiflx + .25 * s2 * (AT(*pr, il, jl, kl, tc2) + AT(*pr, i, j, k, tc2));
+    AT(*fl, i, j, k, nv + 1) = AT(*fl, i, j, k, nv + 1) - 0.5 * phiflx;
+}
+
+/* Hyperbolic flux of one face (flux.cpp:2016-2372). */
+static void hyp_face(const flux_args* A, int d, int i, int j, int k)
+{
+    const fhdx_params* P = A->P;
+    const int NS = A->NS, nv = A->nv;
+    const int ex = (d == 0), ey = (d == 1), ez = (d == 2);
+    fab_t* fl = (fab_t*)&A->flux[d];
+    const double wgt2 = 1. / 12.;
+    const double wgt1 = 0.5 + wgt2;
+    double cv[MS + 5], pv[MS + 6], Yk[MS];
+    if (P->advection_type == 1) {
+        for (int l = 0; l < NS + 6; ++l)
+            pv[l] = wgt1 * (AT(A->prim, i, j, k, l) + AT(A->prim, i - ex, j - ey, k - ez, l)) -
+                    wgt2 * (AT(A->prim, i - 2 * ex, j - 2 * ey, k - 2 * ez, l) +
+                            AT(A->prim, i + ex, j + ey, k + ez, l));
+        double temp = pv[4], rho = pv[0];
+        cv[0] = rho;
+        for (int n = 0; n < NS; ++n) Yk[n] = pv[6 + n];
+        double ie = get_energy(P, Yk, temp);
+        double vsqr = pv[1] * pv[1] + pv[2] * pv[2] + pv[3] * pv[3];
+        cv[4] = rho * ie + 0.5 * rho * vsqr;
+    } else {
+        for (int l = 0; l < NS + 5; ++l)
+            cv[l] = wgt1 * (AT(A->cons, i, j, k, l) + AT(A->cons, i - ex, j - ey, k - ez, l)) -
+                    wgt2 * (AT(A->cons, i - 2 * ex, j - 2 * ey, k - 2 * ez, l) +
+                            AT(A->cons, i + ex, j + ey, k + ez, l));
+        for (int l = 1; l < 4; ++l) pv[l] = cv[l] / cv[0];
+        for (int n = 0; n < NS; ++n) Yk[n] = cv[5 + n] / cv[0];
+        double vsqr = pv[1] * pv[1] + pv[2] * pv[2] + pv[3] * pv[3];
+        double ie = cv[4] / cv[0] - 0.5 * vsqr;
+        pv[4] = get_temperature(P, ie, Yk);
+        pv[5] = get_pressure(P, Yk, cv[0], pv[4]);
+    }
+    const double un = pv[1 + d];
+    AT(*fl, i, j, k, 0) += cv[0] * un;
+    if (d == 0) {
+        AT(*fl, i, j, k, 1) += cv[0] * (pv[1] * pv[1]) + pv[5];
+        AT(*fl, i, j, k, 2) += cv[0] * pv[1] * pv[2];
+        AT(*fl, i, j, k, 3) += cv[0] * pv[1] * pv[3];
+    } else if (d == 1) {
+        AT(*fl, i, j, k, 1) += cv[0] * pv[1] * pv[2];
+        AT(*fl, i, j, k, 2) += cv[0] * pv[2] * pv[2] + pv[5];
+        AT(*fl, i, j, k, 3) += cv[0] * pv[3] * pv[2];
+    } else {
+        AT(*fl, i, j, k, 1) += cv[0] * pv[1] * pv[3];
+        AT(*fl, i, j, k, 2) += cv[0] * pv[2] * pv[3];
+        AT(*fl, i, j, k, 3) += cv[0] * pv[3] * pv[3] + pv[5];
+    }
+    AT(*fl, i, j, k, 4) += un * cv[4] + pv[5] * un;
+    AT(*fl, i, j, k, 4) += AT(*fl, i, j, k, nv) + AT(*fl, i, j, k, nv + 1) +
+                           AT(*fl, i, j, k, nv + 2) + AT(*fl, i, j, k, nv + 3);
+    if (P->algorithm_type == 2) {
+        if (P->advection_type == 1)
+            for (int n = 0; n < NS; ++n) AT(*fl, i, j, k, 5 + n) += cv[0] * pv[6 + n] * un;
+        else
+            for (int n = 0; n < NS; ++n) AT(*fl, i, j, k, 5 + n) += cv[5 + n] * un;
+    }
+}
+
+int orc_calculate_flux(const fhdx_params* P, fhdx_fab cons, fhdx_fab prim, fhdx_fab eta,
+                       fhdx_fab zeta, fhdx_fab kappa, fhdx_fab chi, fhdx_fab D,
+                       const fhdx_fab flux[3], const fhdx_fab stochFlux[3],
+                       const fhdx_fab cornx[3], const fhdx_fab corny[3],
+                       const fhdx_fab cornz[3], fhdx_fab visccorn, fhdx_fab rancorn,
+                       double dt)
+{
+    flux_args A;
+    A.P = P;
+    A.cons = F(cons); A.prim = F(prim); A.eta = F(eta); A.zeta = F(zeta);
+    A.kappa = F(kappa); A.chi = F(chi); A.D = F(D);
+    for (int d = 0; d < 3; ++d) {
+        A.flux[d] = F(flux[d]);
+        A.ran[d] = F(stochFlux[d]);
+        A.corn[0][d] = F(cornx[d]);
+        A.corn[1][d] = F(corny[d]);
+        A.corn[2][d] = F(cornz[d]);
+    }
+    A.visccorn = F(visccorn);
+    A.rancorn = F(rancorn);
+    A.dt = dt;
+    A.NS = P->nspecies;
+    A.nv = nvars_of(P);
+    const int n0 = P->n_cells[0], n1 = P->n_cells[1], n2 = P->n_cells[2];
+    const int nc = A.nv + 4;
+
+    /* flux_in[d].setVal(0) (flux.cpp:24-26) */
+    for (int d = 0; d < 3; ++d) {
+        long npts = A.flux[d].sn * nc;
+        memset(A.flux[d].p, 0, sizeof(double) * npts);
+    }
+    /* each pass visits faces in order; faces are independent, so threads split over k */
+    if (P->stoch_stress_form == 1)
+        for (int d = 0; d < 3; ++d) {
+            const int kx = n0 + (d == 0), ky = n1 + (d == 1), kz = n2 + (d == 2);
+#pragma omp parallel for schedule(static)
+            for (int k = 0; k < kz; ++k)
+                for (int j = 0; j < ky; ++j)
+                    for (int i = 0; i < kx; ++i) stoch_face(&A, d, i, j, k);
+        }
+    for (int d = 0; d < 3; ++d) {
+        const int kx = n0 + (d == 0), ky = n1 + (d == 1), kz = n2 + (d == 2);
+#pragma omp parallel for schedule(static)
+        for (int k = 0; k < kz; ++k)
+            for (int j = 0; j < ky; ++j)
+                for (int i = 0; i < kx; ++i) diff_face(&A, d, i, j, k);
+    }
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k <= n2; ++k)
+        for (int j = 0; j <= n1; ++j)
+            for (int i = 0; i <= n0; ++i) corner_node(&A, i, j, k);
+    for (int d = 0; d < 3; ++d) {
+        const int kx = n0 + (d == 0), ky = n1 + (d == 1), kz = n2 + (d == 2);
+#pragma omp parallel for schedule(static)
+        for (int k = 0; k < kz; ++k)
+            for (int j = 0; j < ky; ++j)
+                for (int i = 0; i < kx; ++i) corner_face(&A, d, i, j, k);
+    }
+    if (P->advection_type == 1 || P->advection_type == 2)
+        for (int d = 0; d < 3; ++d) {
+            const int kx = n0 + (d == 0), ky = n1 + (d == 1), kz = n2 + (d == 2);
+#pragma omp parallel for schedule(static)
+            for (int k = 0; k < kz; ++k)
+                for (int j = 0; j < ky; ++j)
+                    for (int i = 0; i < kx; ++i) hyp_face(&A, d, i, j, k);
+        }
+    return FHDX_OK;
+}
+
+/* ------------------------------------------------------------------------------------- */
+/* RK3 step (timeStep.cpp:11-427)                                                          */
+/* ------------------------------------------------------------------------------------- */
+
+/* stochFlux = 1*A + w*B over comps 1..nvars-1, rancorn likewise (MultiFab::LinComb). */
+static void weight_noise(const fhdx_params* P, const fhdx_fab stochFlux[3], fhdx_fab rancorn,
+                         const fhdx_fab sA[3], const fhdx_fab sB[3], fhdx_fab rA, fhdx_fab rB,
+                         double w)
+{
+    const int nv = nvars_of(P);
+    for (int d = 0; d < 3; ++d) {
+        fab_t s = F(stochFlux[d]), a = F(sA[d]), b = F(sB[d]);
+        memset(s.p, 0, sizeof(double) * s.sn * s.nc);
+        for (int n = 1; n < nv; ++n)
+            for (long q = 0; q < s.sn; ++q)
+                s.p[q + n * s.sn] = 1.0 * a.p[q + n * a.sn] + w * b.p[q + n * b.sn];
+    }
+    fab_t r = F(rancorn), a = F(rA), b = F(rB);
+    for (long q = 0; q < r.sn; ++q) r.p[q] = 1.0 * a.p[q] + w * b.p[q];
+}
+
+/* stage combination: mode 0 -> out = U - dt*div + dt*S;
+ * mode 1 -> out = 0.25*(3*U0 + U - dt*div + dt*S); mode 2 -> out = 2/3*(0.5*U0 + U - ...). */
+static void stage_update(const fhdx_params* P, fhdx_fab out_, fhdx_fab u0_, fhdx_fab us_,
+                         const fhdx_fab flux[3], double dt, int mode)
+{
+    fab_t out = F(out_), u0 = F(u0_), us = F(us_);
+    fab_t fx = F(flux[0]), fy = F(flux[1]), fz = F(flux[2]);
+    const int nv = nvars_of(P);
+    const int n0 = P->n_cells[0], n1 = P->n_cells[1], n2 = P->n_cells[2];
+    const double dx0 = P->dx[0], dx1 = P->dx[1], dx2 = P->dx[2];
+    const double src = 0.0; /* nreaction == 0: source stays zero (main_driver.cpp:241-242) */
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < n2; ++k)
+        for (int j = 0; j < n1; ++j)
+            for (int i = 0; i < n0; ++i) {
+                double o[MS + 5];
+                for (int n = 0; n < nv; ++n) {
+                    double div = (AT(fx, i + 1, j, k, n) - AT(fx, i, j, k, n)) / dx0 +
+                                 (AT(fy, i, j + 1, k, n) - AT(fy, i, j, k, n)) / dx1 +
+                                 (AT(fz, i, j, k + 1, n) - AT(fz, i, j, k, n)) / dx2;
+                    if (mode == 0)
+                        o[n] = AT(us, i, j, k, n) - dt * div + dt * src;
+                    else if (mode == 1)
+                        o[n] = 0.25 * (3.0 * AT(u0, i, j, k, n) + AT(us, i, j, k, n) - dt * div +
+                                       dt * src);
+                    else
+                        o[n] = (2. / 3.) * (0.5 * AT(u0, i, j, k, n) + AT(us, i, j, k, n) -
+                                            dt * div + dt * src);
+                }
+                /* gravity, using the previous stage's state (timeStep.cpp:190-201 etc.) */
+                const double *g = P->grav;
+                double r0 = AT(us, i, j, k, 0), r1 = AT(us, i, j, k, 1), r2 = AT(us, i, j, k, 2),
+                       r3 = AT(us, i, j, k, 3);
+                if (mode == 0) {
+                    for (int n = 0; n < 3; ++n) o[n + 1] += dt * r0 * g[n];
+                    o[4] += dt * (g[0] * r1 + g[1] * r2 + g[2] * r3);
+                } else if (mode == 1) {
+                    for (int n = 0; n < 3; ++n) o[n + 1] += 0.25 * dt * r0 * g[n];
+                    o[4] += 0.25 * dt * (g[0] * r1 + g[1] * r2 + g[2] * r3);
+                } else {
+                    for (int n = 0; n < 3; ++n) o[n + 1] += 2. / 3. * dt * r0 * g[n];
+                    o[4] += 2. / 3. * dt * (g[0] * r1 + g[1] * r2 + g[2] * r3);
+                }
+                for (int n = 0; n < nv; ++n) AT(out, i, j, k, n) = o[n];
+            }
+}
+
+int orc_rk3_step(const fhdx_params* P, fhdx_fab cu, fhdx_fab cup, fhdx_fab cup2,
+                 fhdx_fab prim, fhdx_fab eta, fhdx_fab zeta, fhdx_fab kappa, fhdx_fab chi,
+                 fhdx_fab D, const fhdx_fab flux[3], const fhdx_fab stochFlux[3],
+                 const fhdx_fab cornx[3], const fhdx_fab corny[3], const fhdx_fab cornz[3],
+                 fhdx_fab visccorn, fhdx_fab rancorn, const fhdx_fab stochA[3],
+                 const fhdx_fab stochB[3], fhdx_fab rancornA, fhdx_fab rancornB, double dt)
+{
+    const double sq2 = sqrt(2.0), sq3 = sqrt(3.0);
+    const double w1 = (2.0 * sq2 + 1.0 * sq3) / 5.0;
+    const double w2 = (-4.0 * sq2 + 3.0 * sq3) / 5.0;
+    const double w3 = (1.0 * sq2 - 2.0 * sq3) / 10.0;
+    const double ws[3] = {w1, w2, w3};
+    fhdx_fab in[3] = {cu, cup, cup2};
+    fhdx_fab outs[3] = {cup, cup2, cu};
+    int err = FHDX_OK;
+    for (int s = 0; s < 3; ++s) {
+        int e = orc_transport(P, prim, eta, zeta, kappa, chi, D);
+        if (e) err = e;
+        weight_noise(P, stochFlux, rancorn, stochA, stochB, rancornA, rancornB, ws[s]);
+        orc_calculate_flux(P, in[s], prim, eta, zeta, kappa, chi, D, flux, stochFlux, cornx,
+                           corny, cornz, visccorn, rancorn, dt);
+        stage_update(P, outs[s], cu, in[s], flux, dt, s);
+        orc_cons_to_prim(P, prim, outs[s]);
+        orc_fill_boundary(P, outs[s]);
+        orc_fill_boundary(P, prim);
+    }
+    return err;
+}
+
+/* ------------------------------------------------------------------------------------- */
+/* Noise: Philox4x32-10 (Salmon et al., SC'11) and a Box-Muller normal pair               */
+/* ------------------------------------------------------------------------------------- */
+
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        c0 = hi1 ^ c1 ^ k0;
+        c1 = lo1;
+        c2 = hi0 ^ c3 ^ k1;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* u1 in (0,1) from words 0,1; u2 in [0,1) from words 2,3 (53 bits each);
+ * z0 = sqrt(-2 ln u1) cos(2 pi u2), z1 = sqrt(-2 ln u1) sin(2 pi u2). */
+void orc_normal_pair(const uint32_t r[4], double* z0, double* z1)
+{
+    const double two_m53 = 1.1102230246251565e-16; /* 2^-53 */
+    uint64_t a = (((uint64_t)r[0] << 32) | r[1]) >> 11;
+    uint64_t b = (((uint64_t)r[2] << 32) | r[3]) >> 11;
+    double u1 = ((double)a + 0.5) * two_m53;
+    double u2 = (double)b * two_m53;
+    double rad = sqrt(-2.0 * log(u1));
+    double th = 6.283185307179586 * u2;
+    *z0 = rad * cos(th);
+    *z1 = rad * sin(th);
+}
+
+int orc_fill_noise(const fhdx_params* P, uint64_t seed, uint64_t step, uint32_t instance,
+                   const fhdx_fab stochA[3], const fhdx_fab stochB[3], fhdx_fab rancornA,
+                   fhdx_fab rancornB)
+{
+    const int nv = nvars_of(P);
+    const int n[3] = {P->n_cells[0], P->n_cells[1], P->n_cells[2]};
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    const uint32_t step_lo = (uint32_t)step, step_hi = (uint32_t)(step >> 32);
+    for (int d = 0; d < 3; ++d) {
+        fab_t a = F(stochA[d]), b = F(stochB[d]);
+        memset(a.p, 0, sizeof(double) * a.sn * a.nc);
+        memset(b.p, 0, sizeof(double) * b.sn * b.nc);
+        for (int c = 1; c < nv; ++c) {
+            double vc = (c <= 3) ? P->variance_coef_mom
+                        : (c == 4) ? P->variance_coef_ener : P->variance_coef_mass;
+            double sc = sqrt(vc * vc);
+            uint32_t field = (uint32_t)(d * 16 + c);
+#pragma omp parallel for schedule(static)
+            for (int k = 0; k <= a.hi2; ++k)
+                for (int j = 0; j <= a.hi1; ++j)
+                    for (int i = 0; i <= a.hi0; ++i) {
+                        uint32_t lin = (uint32_t)(((long)(k % n[2]) * n[1] + (j % n[1])) * n[0] +
+                                                  (i % n[0]));
+                        uint32_t ctr[4] = {lin, step_lo, field | (step_hi << 8), instance};
+                        uint32_t r[4];
+                        double z0, z1;
+                        orc_philox4x32_10(ctr, key, r);
+                        orc_normal_pair(r, &z0, &z1);
+                        AT(a, i, j, k, c) = z0 * sc;
+                        AT(b, i, j, k, c) = z1 * sc;
+                    }
+        }
+    }
+    fab_t a = F(rancornA), b = F(rancornB);
+    const double sc = sqrt(P->variance_coef_mom * P->variance_coef_mom);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k <= a.hi2; ++k)
+        for (int j = 0; j <= a.hi1; ++j)
+            for (int i = 0; i <= a.hi0; ++i) {
+                uint32_t lin =
+                    (uint32_t)(((long)(k % n[2]) * n[1] + (j % n[1])) * n[0] + (i % n[0]));
+                uint32_t ctr[4] = {lin, step_lo, 48u | (step_hi << 8), instance};
+                uint32_t r[4];
+                double z0, z1;
+                orc_philox4x32_10(ctr, key, r);
+                orc_normal_pair(r, &z0, &z1);
+                AT(a, i, j, k, 0) = z0 * sc;
+                AT(b, i, j, k, 0) = z1 * sc;
+            }
+    return FHDX_OK;
+}
+
+/* ------------------------------------------------------------------------------------- */
+/* Initial state and a self-allocating driver                                              */
+/* ------------------------------------------------------------------------------------- */
+
+int orc_init_cons(const fhdx_params* P, fhdx_fab cu_, int prob_type, double rho0,
+                  const double* rhobar, double T_init)
+{
+    fab_t cu = F(cu_);
+    const int NS = P->nspecies, nv = nvars_of(P);
+    double massvec[MS];
+    for (int n = 0; n < NS; ++n) massvec[n] = rhobar[n];
+    double ie = get_energy(P, massvec, T_init);
+    for (long q = 0; q < cu.sn * cu.nc; ++q) cu.p[q] = 0.0;
+    for (long q = 0; q < cu.sn; ++q) {
+        cu.p[q] = rho0;
+        cu.p[q + 1 * cu.sn] = 0;
+        cu.p[q + 2 * cu.sn] = 0;
+        cu.p[q + 3 * cu.sn] = 0;
+        cu.p[q + 4 * cu.sn] = rho0 * ie;
+        for (int n = 0; n < NS; ++n) cu.p[q + (5 + n) * cu.sn] = rho0 * rhobar[n];
+    }
+    if (prob_type == 100) {
+        const double pi = acos(-1.);
+        const double Ly = P->dx[1] * P->n_cells[1];
+        for (int k = 0; k < P->n_cells[2]; ++k)
+            for (int j = 0; j < P->n_cells[1]; ++j)
+                for (int i = 0; i < P->n_cells[0]; ++i) {
+                    double y = (j + 0.5) * P->dx[1];
+                    double Yk[MS];
+                    for (int l = 0; l < NS; ++l) Yk[l] = AT(cu, i, j, k, 5 + l) / AT(cu, i, j, k, 0);
+                    AT(cu, i, j, k, 0) = rho0 + 0.1 * rho0 * sin(2. * pi * y / Ly);
+                    for (int l = 0; l < NS; ++l) AT(cu, i, j, k, 5 + l) = AT(cu, i, j, k, 0) * Yk[l];
+                }
+        orc_fill_boundary(P, cu_);
+    } else if (prob_type != 1) {
+        return FHDX_ERR_UNSUPPORTED;
+    }
+    (void)nv;
+    return FHDX_OK;
+}
+
+long orc_noise_doubles_per_step(const fhdx_params* P)
+{
+    const long n0 = P->n_cells[0], n1 = P->n_cells[1], n2 = P->n_cells[2];
+    const long nv = nvars_of(P);
+    long faces = ((n0 + 1) * n1 * n2 + n0 * (n1 + 1) * n2 + n0 * n1 * (n2 + 1)) * nv;
+    long nodes = (n0 + 1) * (n1 + 1) * (n2 + 1);
+    return 2 * (faces + nodes);
+}
+
+static fhdx_fab mkfab(int l0, int l1, int l2, int h0, int h1, int h2, int nc)
+{
+    fhdx_fab f;
+    f.lo[0] = l0; f.lo[1] = l1; f.lo[2] = l2;
+    f.hi[0] = h0; f.hi[1] = h1; f.hi[2] = h2;
+    f.ncomp = nc;
+    f.pad = 0;
+    long n = (long)(h0 - l0 + 1) * (h1 - l1 + 1) * (h2 - l2 + 1) * nc;
+    f.p = (double*)calloc((size_t)n, sizeof(double));
+    return f;
+}
+
+int orc_advance(const fhdx_params* P, fhdx_fab cu, fhdx_fab prim, int nsteps, double dt,
+                const double* noise, uint64_t seed, uint64_t step0, uint32_t instance)
+{
+    const int NS = P->nspecies, nv = nvars_of(P), g = P->ngc;
+    const int n0 = P->n_cells[0], n1 = P->n_cells[1], n2 = P->n_cells[2];
+    fhdx_fab cup = mkfab(-g, -g, -g, n0 + g - 1, n1 + g - 1, n2 + g - 1, nv);
+    fhdx_fab cup2 = mkfab(-g, -g, -g, n0 + g - 1, n1 + g - 1, n2 + g - 1, nv);
+    fhdx_fab eta = mkfab(-g, -g, -g, n0 + g - 1, n1 + g - 1, n2 + g - 1, 1);
+    fhdx_fab zeta = mkfab(-g, -g, -g, n0 + g - 1, n1 + g - 1, n2 + g - 1, 1);
+    fhdx_fab kappa = mkfab(-g, -g, -g, n0 + g - 1, n1 + g - 1, n2 + g - 1, 1);
+    fhdx_fab chi = mkfab(-g, -g, -g, n0 + g - 1, n1 + g - 1, n2 + g - 1, NS);
+    fhdx_fab D = mkfab(-g, -g, -g, n0 + g - 1, n1 + g - 1, n2 + g - 1, NS * NS);
+    fhdx_fab flux[3], sf[3], sA[3], sB[3], cx[3], cy[3], cz[3];
+    for (int d = 0; d < 3; ++d) {
+        int h0 = n0 - 1 + (d == 0), h1 = n1 - 1 + (d == 1), h2 = n2 - 1 + (d == 2);
+        flux[d] = mkfab(0, 0, 0, h0, h1, h2, nv + 4);
+        sf[d] = mkfab(0, 0, 0, h0, h1, h2, nv);
+        sA[d] = mkfab(0, 0, 0, h0, h1, h2, nv);
+        sB[d] = mkfab(0, 0, 0, h0, h1, h2, nv);
+        cx[d] = mkfab(0, 0, 0, n0, n1, n2, 1);
+        cy[d] = mkfab(0, 0, 0, n0, n1, n2, 1);
+        cz[d] = mkfab(0, 0, 0, n0, n1, n2, 1);
+    }
+    fhdx_fab vcorn = mkfab(0, 0, 0, n0, n1, n2, 1);
+    fhdx_fab rc = mkfab(0, 0, 0, n0, n1, n2, 1);
+    fhdx_fab rA = mkfab(0, 0, 0, n0, n1, n2, 1);
+    fhdx_fab rB = mkfab(0, 0, 0, n0, n1, n2, 1);
+
+    orc_fill_boundary(P, cu);
+    orc_cons_to_prim(P, prim, cu);
+    orc_fill_boundary(P, prim);
+    int err = FHDX_OK;
+    const long per = orc_noise_doubles_per_step(P);
+    for (int s = 0; s < nsteps; ++s) {
+        if (noise) {
+            const double* q = noise + per * s;
+            fhdx_fab* sets[2] = {sA, sB};
+            fhdx_fab nodes[2] = {rA, rB};
+            for (int set = 0; set < 2; ++set) {
+                for (int d = 0; d < 3; ++d) {
+                    fab_t f = F(sets[set][d]);
+                    memcpy(f.p, q, sizeof(double) * f.sn * f.nc);
+                    q += f.sn * f.nc;
+                }
+                fab_t f = F(nodes[set]);
+                memcpy(f.p, q, sizeof(double) * f.sn);
+                q += f.sn;
+            }
+        } else {
+            orc_fill_noise(P, seed, step0 + (uint64_t)s, instance, sA, sB, rA, rB);
+        }
+        int e = orc_rk3_step(P, cu, cup, cup2, prim, eta, zeta, kappa, chi, D, flux, sf, cx,
+                             cy, cz, vcorn, rc, sA, sB, rA, rB, dt);
+        if (e) err = e;
+    }
+    free(cup.p); free(cup2.p); free(eta.p); free(zeta.p); free(kappa.p); free(chi.p); free(D.p);
+    for (int d = 0; d < 3; ++d) {
+        free(flux[d].p); free(sf[d].p); free(sA[d].p); free(sB[d].p);
+        free(cx[d].p); free(cy[d].p); free(cz[d].p);
+    }
+    free(vcorn.p); free(rc.p); free(rA.p); free(rB.p);
+    return err;
+}
+
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

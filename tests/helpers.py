@@ -1,0 +1,51 @@
+"""Shared state builders for the tests (host numpy arrays; the oracle is the checker)."""
+from __future__ import annotations
+
+import numpy as np
+
+from fhdex_amd.params import Params
+
+
+def perturbed_state(p: Params, cfg, orc, seed=0, amp=1e-2, vel=2e3):
+    """A periodic conserved state near the uniform start of `cfg`: density, velocity,
+    temperature and composition perturbed cell by cell (all mass fractions stay in (0,1))."""
+    rng = np.random.default_rng(seed)
+    st = orc.HostState(p)
+    orc.init_cons(p, st.cu, cfg.rho0, cfg.rhobar, cfg.T_init)
+    g = p.ngc
+    NS = p.nspecies
+    n0, n1, n2 = p.n_cells
+    sl = (slice(g, g + n2), slice(g, g + n1), slice(g, g + n0))
+    shape = (n2, n1, n0)
+    rho = cfg.rho0 * (1 + amp * rng.standard_normal(shape))
+    Y = np.array(cfg.rhobar)[:, None, None, None] * (1 + amp * rng.standard_normal((NS,) + shape))
+    Y /= Y.sum(axis=0)
+    u = vel * rng.standard_normal((3,) + shape)
+    T = cfg.T_init * (1 + amp * rng.standard_normal(shape))
+    cv = sum(Y[n] * p.hcv[n] for n in range(NS))
+    e = cv * T + sum(Y[n] * p.e0[n] for n in range(NS))
+    st.cu[0][sl] = rho
+    for d in range(3):
+        st.cu[1 + d][sl] = rho * u[d]
+    st.cu[4][sl] = rho * (e + 0.5 * (u ** 2).sum(axis=0))
+    for n in range(NS):
+        st.cu[5 + n][sl] = rho * Y[n]
+    orc.fill_boundary(p, st.cu)
+    orc.cons_to_prim(p, st.prim, st.cu)
+    orc.fill_boundary(p, st.prim)
+    return st
+
+
+def valid(p: Params, a):
+    g = p.ngc
+    n0, n1, n2 = p.n_cells
+    return a[:, g:g + n2, g:g + n1, g:g + n0]
+
+
+def rel_err(a, b):
+    """Per-component max|a-b| / max|b| (inf-norm relative, the parity metric)."""
+    a = np.asarray(a).reshape(a.shape[0], -1)
+    b = np.asarray(b).reshape(b.shape[0], -1)
+    den = np.abs(b).max(axis=1)
+    den[den == 0] = 1.0
+    return np.abs(a - b).max(axis=1) / den

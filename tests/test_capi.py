@@ -1,0 +1,89 @@
+"""The C ABI without a GPU: the library loads, exports every symbol include/fhdx.h declares,
+the struct layouts agree with the header, and the pure host functions behave."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from fhdex_amd import lib as L
+from fhdex_amd import params as P
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "fhdx.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(fhdx_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.lib()
+    names = header_functions()
+    assert len(names) >= 17
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing
+    assert sorted(L.EXPORTS) == sorted(names)
+
+
+def test_abi_version_and_status_strings():
+    lib = L.lib()
+    assert lib.fhdx_abi_version() == 1
+    assert lib.fhdx_status_string(0) == b"ok"
+    assert b"mass fraction" in lib.fhdx_status_string(4)
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """ctypes offsets equal the C compiler's offsetof for every field of both structs."""
+    fields = [f for f, _ in P.Params._fields_]
+    src = ["#include <stdio.h>", "#include <stddef.h>", '#include "fhdx.h"', "int main(void){"]
+    src.append('printf("%zu %zu\\n", sizeof(fhdx_params), sizeof(fhdx_fab));')
+    for f in fields:
+        src.append(f'printf("%zu\\n", offsetof(fhdx_params, {f}));')
+    for f, _ in P.Fab._fields_:
+        src.append(f'printf("%zu\\n", offsetof(fhdx_fab, {f}));')
+    src.append("return 0;}")
+    c = tmp_path / "layout.c"
+    c.write_text("\n".join(src))
+    exe = tmp_path / "layout"
+    import subprocess
+    subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), str(c), "-o", str(exe)])
+    out = subprocess.check_output([str(exe)]).decode().split()
+    vals = [int(v) for v in out]
+    assert vals[0] == C.sizeof(P.Params) and vals[1] == C.sizeof(P.Fab)
+    offs = vals[2:]
+    for i, f in enumerate(fields):
+        assert getattr(P.Params, f).offset == offs[i], f
+    for i, (f, _) in enumerate(P.Fab._fields_):
+        assert getattr(P.Fab, f).offset == offs[len(fields) + i], f
+
+
+def test_params_default_finalize_check():
+    lib = L.lib()
+    p = P.Params()
+    lib.fhdx_params_default(C.byref(p))
+    assert p.ngc == 2 and p.advection_type == 2 and p.algorithm_type == 2
+    assert p.hcv[0] < 0
+    assert lib.fhdx_params_check(C.byref(p)) == 1          # hcv not derived yet
+    p.molmass[0] = p.molmass[1] = 39.95
+    p.diameter[0] = p.diameter[1] = 3.66e-8
+    lib.fhdx_params_finalize(C.byref(p))
+    assert p.hcv[0] == pytest.approx(1.5 * P.R_UNIV / 39.95)
+    assert lib.fhdx_params_check(C.byref(p)) == 0
+    q = p.copy()
+    q.bc_vel_lo[0] = 1
+    assert lib.fhdx_params_check(C.byref(q)) == 2          # walls are out of scope
+    assert b"periodic" in lib.fhdx_last_error()
+    q = p.copy()
+    q.nspecies = 1
+    assert lib.fhdx_params_check(C.byref(q)) == 2
+    q = p.copy()
+    q.ngc = 1
+    assert lib.fhdx_params_check(C.byref(q)) == 2
+
+
+def test_config_params_pass_check():
+    lib = L.lib()
+    for cfg in (P.argon_gas(32), P.four_species(16), P.species_prefix(3, 16, 10)):
+        assert lib.fhdx_params_check(C.byref(cfg.params)) == 0

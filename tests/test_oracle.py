@@ -1,0 +1,371 @@
+"""CPU tests that pin the oracle (oracle/oracle.c) independently of the GPU path.
+
+The reference ships no golden outputs for this path, so each oracle function is pinned by
+known-answer vectors (Philox), closed forms (hard-sphere Chapman-Enskog transport, binary
+diffusion, shear-wave and sound-wave damping), invariants (conservation, vanishing column sums
+of D, uniform states, cyclic symmetry of the deterministic fluxes) and the reference's own
+quirks showing up exactly where the reference puts them.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from fhdex_amd import params as P
+from tests.helpers import perturbed_state, rel_err, valid
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ---- noise ------------------------------------------------------------------------------
+
+def test_philox_known_answers(orc):
+    kat = json.load(open(os.path.join(GOLDEN, "philox_kat.json")))
+    assert len(kat) == 3
+    for v in kat:
+        assert orc.philox(v["ctr"], v["key"]) == v["out"]
+
+
+def test_normal_pair_moments(orc):
+    n = 40000
+    z = np.empty(2 * n)
+    for c in range(n):
+        z[2 * c], z[2 * c + 1] = orc.normal_pair(orc.philox([c, 7, 3, 0], [12345, 0]))
+    m, v = z.mean(), z.var()
+    k4 = ((z - m) ** 4).mean() / v ** 2
+    assert abs(m) < 5 / math.sqrt(2 * n)
+    assert abs(v - 1) < 5 * math.sqrt(2 / (2 * n))
+    assert abs(k4 - 3) < 0.1
+    # the two members of a pair are uncorrelated
+    assert abs(np.corrcoef(z[0::2], z[1::2])[0, 1]) < 5 / math.sqrt(n)
+
+
+def test_noise_duplicate_faces_equal(orc):
+    """Face n_d and face 0 (and node n and node 0) carry the same value, as after
+    OverrideSync (MultiFabFillRandom.cpp:25); A and B sets differ."""
+    cfg = P.argon_gas(6)
+    p = cfg.params
+    st = orc.HostState(p)
+    orc.fill_noise(p, st, 99, 5)
+    nv = p.nvars
+    for d in range(3):
+        a = st.sA[d]
+        if d == 0:
+            np.testing.assert_array_equal(a[:, :, :, 0], a[:, :, :, -1])
+        elif d == 1:
+            np.testing.assert_array_equal(a[:, :, 0, :], a[:, :, -1, :])
+        else:
+            np.testing.assert_array_equal(a[:, 0], a[:, -1])
+        assert np.all(a[0] == 0)
+        assert not np.array_equal(st.sA[d][1:nv], st.sB[d][1:nv])
+    r = st.rA[0]
+    np.testing.assert_array_equal(r[0], r[-1])
+    np.testing.assert_array_equal(r[:, 0], r[:, -1])
+    np.testing.assert_array_equal(r[:, :, 0], r[:, :, -1])
+
+
+# ---- transport ----------------------------------------------------------------------------
+
+def _transport_of(orc, cfg, st):
+    p = cfg.params
+    assert orc.transport(p, st) == 0
+    return st
+
+
+def test_transport_hard_sphere_closed_forms(orc):
+    """Two identical species reduce GIO to the single-gas Chapman-Enskog values:
+    eta = 5/(16 d^2) sqrt(m kT/pi), kappa = 75/64 k_B/d^2 sqrt(kT/(pi m)), and the binary
+    diffusion flux uses D_12 = 3/16 sqrt(2 pi (kT)^3/mu)/(p pi d^2) with mu = m/2."""
+    cfg = P.argon_gas(4)
+    p = cfg.params
+    st = orc.HostState(p)
+    orc.init_cons(p, st.cu, cfg.rho0, cfg.rhobar, cfg.T_init)
+    orc.cons_to_prim(p, st.prim, st.cu)
+    orc.fill_boundary(p, st.prim)
+    _transport_of(orc, cfg, st)
+    kB, T, d = P.K_B, cfg.T_init, 3.66e-8
+    m = 39.95 * P.K_B / P.R_UNIV
+    eta = 5 / (16 * d * d) * math.sqrt(m * kB * T / math.pi)
+    kappa = 75 / 64 * kB / (d * d) * math.sqrt(kB * T / (math.pi * m))
+    pres = st.prim[5, 2, 2, 2]
+    D12 = 3 / 16 * math.sqrt(2 * math.pi * (kB * T) ** 3 / (m / 2)) / (pres * math.pi * d * d)
+    assert st.eta[0, 2, 2, 2] == pytest.approx(eta, rel=1e-13)
+    assert st.kappa[0, 2, 2, 2] == pytest.approx(kappa, rel=1e-13)
+    assert st.eta[0, 2, 2, 2] == pytest.approx(2.0812e-4, rel=1e-4)   # argon at 273 K
+    assert st.kappa[0, 2, 2, 2] == pytest.approx(1624.3, rel=1e-4)
+    Dv = st.D[:, 2, 2, 2]
+    # F_0 = -(D(0,0) d_0 + D(1,0) d_1), d_1 = -d_0  ->  effective rho*D_12 = D(0,0) - D(1*2+0)
+    assert Dv[0] - Dv[2] == pytest.approx(cfg.rho0 * D12, rel=1e-12)
+    assert np.all(np.abs(st.chi[:, 2, 2, 2]) <= 1e-12 * np.abs(Dv).max())
+
+
+def test_transport_D_columns_sum_to_zero(orc):
+    """Every term of D_GIO1's matrix starts with P and Y^T P = 0, so sum_k D(l*NS+k) = 0."""
+    cfg = P.four_species(4)
+    p = cfg.params
+    st = perturbed_state(p, cfg, orc, seed=3, amp=0.2)
+    _transport_of(orc, cfg, st)
+    NS = p.nspecies
+    D = st.D.reshape(NS, NS, -1)           # [l, k, cell]
+    colsum = D.sum(axis=1)
+    assert np.abs(colsum).max() <= 1e-13 * np.abs(D).max()
+    assert np.all(st.eta > 0) and np.all(st.kappa > 0)
+
+
+def test_transport_rejects_mass_fraction_outside_unit_interval(orc):
+    cfg = P.four_species(4)
+    p = cfg.params
+    st = perturbed_state(p, cfg, orc, seed=1)
+    st.prim[6, 3, 3, 3] = 1.0           # Y_0 >= 1: the reference aborts (transCoeffs.cpp:46)
+    assert orc.transport(p, st) != 0
+
+
+# ---- conversions ------------------------------------------------------------------------
+
+def test_cons_to_prim_recovers_construction(orc):
+    cfg = P.four_species(6)
+    p = cfg.params
+    rng = np.random.default_rng(5)
+    st = orc.HostState(p)
+    NS = p.nspecies
+    shape = st.cu.shape[1:]
+    rho = cfg.rho0 * (1 + 0.05 * rng.standard_normal(shape))
+    Y = 0.25 * (1 + 0.2 * rng.standard_normal((NS,) + shape))
+    Y /= Y.sum(axis=0)
+    u = 300 * rng.standard_normal((3,) + shape)
+    T = 300 * (1 + 0.05 * rng.standard_normal(shape))
+    cv = sum(Y[n] * p.hcv[n] for n in range(NS))
+    st.cu[0] = rho
+    st.cu[1:4] = rho * u
+    st.cu[4] = rho * (cv * T + 0.5 * (u ** 2).sum(axis=0))
+    st.cu[5:5 + NS] = rho * Y
+    orc.cons_to_prim(p, st.prim, st.cu)
+    pv = valid(p, st.prim)
+    sl = (slice(2, -2),) * 3
+    np.testing.assert_allclose(pv[1:4], u[(slice(None),) + sl], rtol=1e-12)
+    np.testing.assert_allclose(pv[4], T[sl], rtol=1e-12)
+    molmix = 1 / sum(Y[n][sl] / p.molmass[n] for n in range(NS))
+    np.testing.assert_allclose(pv[5], rho[sl] * P.R_UNIV / molmix * T[sl], rtol=1e-12)
+    X = np.stack([Y[n][sl] * molmix / p.molmass[n] for n in range(NS)])
+    np.testing.assert_allclose(pv[6 + NS:6 + 2 * NS], X, rtol=1e-12)
+    np.testing.assert_allclose(pv[6 + NS:6 + 2 * NS].sum(axis=0), 1.0, rtol=1e-13)
+
+
+# ---- whole step invariants --------------------------------------------------------------
+
+@pytest.mark.parametrize("maker,n", [(P.argon_gas, 8), (P.four_species, 6)])
+def test_step_conserves_totals(orc, maker, n):
+    cfg = maker(n)
+    p = cfg.params
+    st = perturbed_state(p, cfg, orc, seed=11)
+    tot0 = valid(p, st.cu).sum(axis=(1, 2, 3))
+    abs0 = np.abs(valid(p, st.cu)).sum(axis=(1, 2, 3))
+    for s in range(3):
+        orc.fill_noise(p, st, 7, s)
+        assert orc.rk3_step(p, st, cfg.dt) == 0
+    tot1 = valid(p, st.cu).sum(axis=(1, 2, 3))
+    assert np.all(np.abs(tot1 - tot0) <= 2e-14 * abs0)
+
+
+def test_uniform_state_stays_uniform_without_noise(orc):
+    cfg = P.four_species(6)
+    p = P.make_params(4, (6, 6, 6), 8e-6, [p for p in cfg.params.molmass[:4]],
+                      [d for d in cfg.params.diameter[:4]], variance_coef=(0.0, 0.0, 0.0))
+    st = orc.HostState(p)
+    orc.init_cons(p, st.cu, cfg.rho0, cfg.rhobar, cfg.T_init)
+    orc.fill_boundary(p, st.cu)
+    orc.cons_to_prim(p, st.prim, st.cu)
+    orc.fill_boundary(p, st.prim)
+    for s in range(3):
+        orc.fill_noise(p, st, 1, s)
+        assert orc.rk3_step(p, st, cfg.dt) == 0
+    v = valid(p, st.cu).reshape(p.nvars, -1)
+    assert np.all(v.max(axis=1) == v.min(axis=1))
+    assert np.all(v[1:4] == 0)
+
+
+def _rotate_cells(a):
+    """Cyclic relabelling of axes: new x = old z, new y = old x, new z = old y."""
+    return np.ascontiguousarray(np.transpose(a, (0, 2, 3, 1)))
+
+
+def _rotate_vec(a, first):
+    """Component block first..first+2 holds a vector (x, y, z): new (x,y,z) = old (z,x,y)."""
+    b = a.copy()
+    b[first] = a[first + 2]
+    b[first + 1] = a[first]
+    b[first + 2] = a[first + 1]
+    return b
+
+
+@pytest.mark.parametrize("maker", [P.argon_gas, P.four_species])
+def test_deterministic_fluxes_are_cyclic_symmetric(orc, maker):
+    """With the noise off, relabelling the axes cyclically maps every face flux onto the
+    corresponding one -- except the z-face Soret and Dufour terms, where the reference reads
+    chi(k), chi(k+1) and chi(k)+chi(k) (flux.cpp:1561, 1604).  Catches swapped indices, wrong
+    transverse pairs and sign slips in the restatement."""
+    cfg = maker(6)
+    p = cfg.params
+    p.stoch_stress_form = 0
+    st = perturbed_state(p, cfg, orc, seed=21, amp=0.05)
+    assert orc.transport(p, st) == 0
+    assert orc.calculate_flux(p, st, st.cu, cfg.dt) == 0
+    f_old = [f.copy() for f in st.flux]
+
+    st2 = orc.HostState(p)
+    st2.cu[:] = _rotate_vec(_rotate_cells(st.cu), 1)
+    orc.cons_to_prim(p, st2.prim, st2.cu)
+    orc.fill_boundary(p, st2.prim)
+    assert orc.transport(p, st2) == 0
+    assert orc.calculate_flux(p, st2, st2.cu, cfg.dt) == 0
+    nv = p.nvars
+    NS = p.nspecies
+    species_q = list(range(5, 5 + NS)) + [4, nv + 3]
+    # new x faces <- old z faces, new y <- old x, new z <- old y
+    for new_d, old_d in ((0, 2), (1, 0), (2, 1)):
+        expect = _rotate_vec(_rotate_cells(f_old[old_d]), 1)
+        got = st2.flux[new_d]
+        diff = np.abs(got - expect).reshape(nv + 4, -1).max(axis=1)
+        scale = np.abs(expect).reshape(nv + 4, -1).max(axis=1)
+        # Dufour sums cancel to rounding for identical species: judge them on the heat scale
+        scale[nv + 3] = max(scale[nv + 3], 1e-2 * scale[nv])
+        quirk = (new_d == 0 or new_d == 2) and maker is P.four_species   # z faces before or after
+        for c in range(nv + 4):
+            if quirk and c in species_q:
+                continue
+            assert diff[c] <= 1e-11 * scale[c], (new_d, c, diff[c], scale[c])
+    if maker is P.four_species:
+        # the chi(k)/chi(k+1) reads make the z-face species fluxes differ from the relabelled
+        # y-face ones (old y faces become new z faces)
+        expect = _rotate_vec(_rotate_cells(f_old[1]), 1)
+        err = rel_err(st2.flux[2], expect)
+        assert max(err[c] for c in range(5, 5 + NS)) > 1e-9
+
+
+def test_z_face_edge_viscosity_quirk_present(orc):
+    """flux.cpp:857-858 weights one edge of every z face with eta(k-2)*T(k).  Scaling eta in
+    cell layer 1 only therefore moves the stochastic z-momentum flux of z faces 1, 2 (their
+    own cells) and 3 (the quirk), while x faces move only for k = 0..2."""
+    cfg = P.argon_gas(6)
+    p = cfg.params
+    st = perturbed_state(p, cfg, orc, seed=4, amp=0.05)
+    orc.fill_noise(p, st, 3, 0)
+    assert orc.transport(p, st) == 0
+    for d in range(3):
+        st.stoch[d][:] = st.sA[d]
+    st.rancorn[:] = st.rA
+    assert orc.calculate_flux(p, st, st.cu, cfg.dt) == 0
+    fx, fz = st.flux[0].copy(), st.flux[2].copy()
+    g = p.ngc
+    st.eta[0, g + 1, :, :] *= 1.5
+    assert orc.calculate_flux(p, st, st.cu, cfg.dt) == 0
+    dz = np.abs(st.flux[2][3] - fz[3]).max(axis=(1, 2))     # per z-face layer k = 0..6
+    dx = np.abs(st.flux[0] - fx).max(axis=(0, 2, 3))          # per cell layer k = 0..5
+    assert dz[1] > 0 and dz[2] > 0 and dz[3] > 0
+    assert dz[0] == 0 and dz[4] == 0 and dz[5] == 0
+    assert dx[0] > 0 and dx[1] > 0 and dx[2] > 0
+    assert dx[3] == 0 and dx[4] == 0
+
+
+# ---- closed-form dynamics -----------------------------------------------------------------
+
+def _wave_run(orc, cfg, p, init, nsteps, probe):
+    st = orc.HostState(p)
+    orc.init_cons(p, st.cu, cfg.rho0, cfg.rhobar, cfg.T_init)
+    init(st)
+    orc.fill_boundary(p, st.cu)
+    orc.cons_to_prim(p, st.prim, st.cu)
+    orc.fill_boundary(p, st.prim)
+    out = [probe(st)]
+    for s in range(nsteps):
+        assert orc.rk3_step(p, st, cfg.dt) == 0
+        out.append(probe(st))
+    return np.array(out)
+
+
+def test_shear_wave_decays_at_kinematic_viscosity(orc):
+    """u_x = A sin(k y) decays as exp(-nu k_eff^2 t), nu = eta/rho, with the centred
+    Laplacian's k_eff^2 = (2 - 2 cos(k dy))/dy^2."""
+    cfg = P.argon_gas(4)
+    n = 32
+    p = P.make_params(2, (4, n, 4), 8e-6, [39.95, 39.95], [3.66e-8, 3.66e-8], stoch_stress_form=0)
+    g = p.ngc
+    dy = p.dx[1]
+    k = 2 * math.pi / (n * dy)
+    y = (np.arange(n) + 0.5) * dy
+    A = 100.0
+    s = np.sin(k * y)
+
+    def init(st):
+        st.cu[1, g:-g, g:-g, g:-g] = cfg.rho0 * A * s[None, :, None]
+        st.cu[4, g:-g, g:-g, g:-g] += 0.5 * cfg.rho0 * (A * s[None, :, None]) ** 2
+
+    def probe(st):
+        u = st.prim[1, g, g:-g, g]
+        return 2 * (u * s).mean()
+
+    amp = _wave_run(orc, cfg, p, init, 60, probe)
+    t = cfg.dt * np.arange(len(amp))
+    rate = -np.polyfit(t, np.log(amp), 1)[0]
+    eta = 2.0812e-4
+    keff2 = (2 - 2 * math.cos(k * dy)) / dy ** 2
+    assert rate == pytest.approx(eta / cfg.rho0 * keff2, rel=5e-3)
+
+
+def test_sound_wave_speed_and_damping(orc):
+    """A standing adiabatic sound wave oscillates at c_s k and decays at
+    Gamma = k^2/(2 rho) (4/3 eta + kappa (1/c_v - 1/c_p)): pins the hyperbolic flux, the
+    corner (bulk) viscous terms and heat conduction together."""
+    from scipy.optimize import curve_fit
+    cfg = P.argon_gas(4)
+    n = 32
+    p = P.make_params(2, (4, n, 4), 8e-6, [39.95, 39.95], [3.66e-8, 3.66e-8], stoch_stress_form=0)
+    g = p.ngc
+    dy = p.dx[1]
+    k = 2 * math.pi / (n * dy)
+    y = (np.arange(n) + 0.5) * dy
+    s = np.sin(k * y)[None, :, None]
+    eps = 1e-4
+    gam = 5.0 / 3.0
+    cv = p.hcv[0]
+
+    def init(st):
+        rho = cfg.rho0 * (1 + eps * s)
+        T = cfg.T_init * (1 + (gam - 1) * eps * s)
+        st.cu[0, g:-g, g:-g, g:-g] = rho
+        st.cu[4, g:-g, g:-g, g:-g] = rho * cv * T
+        st.cu[5, g:-g, g:-g, g:-g] = 0.5 * rho
+        st.cu[6, g:-g, g:-g, g:-g] = 0.5 * rho
+
+    def probe(st):
+        r = st.cu[0, g, g:-g, g] / cfg.rho0 - 1
+        return 2 * (r * s[0, :, 0]).mean()
+
+    nsteps = 240
+    a = _wave_run(orc, cfg, p, init, nsteps, probe)
+    t = cfg.dt * np.arange(nsteps + 1)
+    # linearised 1-D Navier-Stokes-Fourier in (drho, v, dT), with the scheme's wavenumbers:
+    # 4th-order centred first derivative, 2-point second derivative
+    h = dy
+    k1 = (8 * math.sin(k * h) - math.sin(2 * k * h)) / (6 * h)
+    k2sq = (2 - 2 * math.cos(k * h)) / h ** 2
+    eta, kappa = 2.0812e-4, 1624.3
+    rho0, T0 = cfg.rho0, cfg.T_init
+    Rm = P.R_UNIV / 39.95
+    p0 = rho0 * Rm * T0
+    A = np.array([[0, -1j * k1 * rho0, 0],
+                  [-1j * k1 * Rm * T0 / rho0, -4 / 3 * eta / rho0 * k2sq, -1j * k1 * Rm],
+                  [0, -1j * k1 * p0 / (rho0 * cv), -kappa * k2sq / (rho0 * cv)]])
+    ev = np.linalg.eigvals(A)
+    sound = ev[np.argmax(np.abs(ev.imag))]
+    G0, w0 = -sound.real, abs(sound.imag)
+    heat = ev[np.argmin(np.abs(ev.imag))].real
+
+    def model(t, A, G, w, ph, B):
+        return A * np.exp(-G * t) * np.cos(w * t + ph) + B * np.exp(heat * t)
+
+    popt, _ = curve_fit(model, t, a, p0=[eps, G0, w0, 0.0, 0.0])
+    assert popt[2] == pytest.approx(w0, rel=1e-3)
+    assert popt[1] == pytest.approx(G0, rel=1e-2)
