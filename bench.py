@@ -1,7 +1,7 @@
 """Benchmark of the compressible FHD RK3 step on one MI355X.
 
 Default workload: two-identical-species argon (the reference's single-gas form), 256^3 periodic
-cells, FP64, CFL 0.2, in-kernel Philox noise; one "step" is one full RK3 step (three stages)
+cells, FP64, CFL 0.1, in-kernel Philox noise; one "step" is one full RK3 step (three stages)
 of the fused HIP path.  Prints one JSON line.
 
     python bench.py [--steps K] [--warmup W] [--n 256] [--no-cpu]
@@ -120,7 +120,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"argon two-species {a.n}^3 periodic, CFL 0.2 dt, Philox noise, "
+        "config": {"workload": f"argon two-species {a.n}^3 periodic, CFL 0.1 dt, Philox noise, "
                                f"fused RK3 step", "cells": cells, "nvars": nv,
                    "b_alg_bytes_per_cell_update": b_alg(nv),
                    "step_hbm_frac_b_alg": rate * b_alg(nv) / 1e9 / HBM_PEAK_GBS},
