@@ -283,3 +283,22 @@ def test_four_species_16_cubed_20_steps(dev, orc):
     p, st, cu, prim = _multi_step(torch, fc, L, orc, cfg, 20, 3)
     e = report("cu after 20", valid(p, cu), valid(p, st.cu))
     assert e.max() <= TOL100
+
+
+def test_fused_equals_reference_shaped_path_64_cubed(dev, orc):
+    """Both GPU paths, Philox noise from the same key, 3 steps at 64^3: identical bits."""
+    torch, fc, L = dev
+    cfg = P.argon_gas(64)
+    p = cfg.params
+    say("case fused vs reference-shaped, argon 64^3 x 3 steps")
+    st = perturbed_state(p, cfg, orc, seed=12, amp=0.01, vel=500.0)
+    ctx = L.Context(p)
+    ds = device_state(torch, fc, p, st)
+    cu = to_dev(torch, fc, st.cu, (-p.ngc,) * 3)
+    prim = fc.MultiFab.cell(p, p.nprimvars)
+    for step in range(3):
+        fc.RK3step(ctx, ds, cfg.dt, noise=None, seed=5, step=step)
+        fc.rk3_step(ctx, cu, prim, cfg.dt, 5, step)
+    ctx.check()
+    assert torch.equal(ds.cu.t, cu.t)
+    assert torch.equal(ds.prim.t, prim.t)
