@@ -53,6 +53,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device copy (calibrates HBM byte counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -60,6 +62,21 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if a.copy_probe:
+        n = (1 << 30) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        gbs = 5 * 2 * n * 8 / (time.perf_counter() - t0) / 1e9
+        print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8,
+                          "copy_GBs": gbs}), flush=True)
+        return
 
     from fhdex_amd import compressible as fc
     from fhdex_amd import lib as L
