@@ -502,4 +502,31 @@ int fhdx_rk3_step_injected(fhdx_ctx* c, fhdx_fab cu, fhdx_fab prim, const fhdx_f
                           &rancornB, c->d_err, S(stream));
 }
 
+int fhdx_ctx_timing_enable(fhdx_ctx* c, int enable)
+{
+    if (!c) return FHDX_ERR_ARG;
+    c->fused.timing.reset();
+    c->fused.timing.on = enable != 0;
+    return FHDX_OK;
+}
+
+int fhdx_timing_kinds(void) { return KT_COUNT; }
+
+int fhdx_ctx_timing_read(fhdx_ctx* c, int kind, char* name, int name_len, int64_t* launches, double* total_ms)
+{
+    if (!c || kind < 0 || kind >= KT_COUNT) return FHDX_ERR_ARG;
+    int r = c->fused.timing.resolve();
+    if (r) {
+        set_error("waiting for timing events failed");
+        return r;
+    }
+    if (name && name_len > 0) {
+        std::strncpy(name, kKernelKindNames[kind], (size_t)name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    if (launches) *launches = c->fused.timing.launches[kind];
+    if (total_ms) *total_ms = c->fused.timing.total_ms[kind];
+    return FHDX_OK;
+}
+
 }  // extern "C"

@@ -20,16 +20,48 @@ struct RecLayout {
 };
 
 // Node values: corner velocity gradients [derivative dir][component] (flux.cpp:1684-1703),
-// visccorn (:1705) and the stage-weighted node noise (rancorn).
-constexpr int NODE_VC = 9, NODE_W = 10, NODE_NCOMP = 11;
+// visccorn (:1705) and the four stochastic edge terms sqrt(mu)*rancorn of the node.  The twelve
+// edge viscosities a face sums (flux.cpp:117-135, 481-499, 847-865) use only four distinct
+// node-relative summation orders: x faces all use order 0; y faces 0 (pp, pm) and 1 (mp, mm);
+// z faces 0 (pp), 1 (mp), 2 (pm, the eta(k-2)*T(k) form of :857-858) and 3 (mm).  Computing
+// each order once per node gives the reference's bits with 4 square roots per node instead of
+// 12 per cell.
+constexpr int NODE_VC = 9, NODE_S = 10, NODE_NCOMP = 14;
+
+// Kernel groups the fast path can time with HIP events (fhdx_ctx_timing_*).
+enum KernelKind { KT_NOISE = 0, KT_CELLS, KT_NODES, KT_FACE_X, KT_FACE_Y, KT_FACE_Z, KT_UPDATE, KT_COUNT };
+extern const char* const kKernelKindNames[KT_COUNT];
+
+// When enabled, brackets each kernel group with events recorded on the launch stream; the
+// pairs are resolved (and the events recycled) when the totals are read.
+struct Timing {
+    bool on = false;
+    struct Pair {
+        int kind;
+        hipEvent_t a, b;
+    };
+    std::vector<Pair> pending;
+    std::vector<hipEvent_t> pool;
+    double total_ms[KT_COUNT] = {};
+    long launches[KT_COUNT] = {};
+    int cur = -1;
+    hipEvent_t cur_a = nullptr;
+    hipEvent_t get();
+    void begin(int kind, hipStream_t s);
+    void end(hipStream_t s);
+    int resolve();  // waits for every pending pair
+    void reset();
+    void destroy();
+};
 
 // Device workspace owned by a context for the fast path (allocated on first use).
 struct FusedWorkspace {
     bool ready = false;
     std::vector<void*> blocks;
-    fhdx_fab cup, cup2, rec, node;
+    fhdx_fab cup, cup2, rec, node;  // node and F hold n^3 entries: index n is index 0
     fhdx_fab F[3];
     fhdx_fab sA[3], sB[3], rA, rB;  // Philox noise of the step
+    Timing timing;
 };
 
 void fused_free(FusedWorkspace& w);

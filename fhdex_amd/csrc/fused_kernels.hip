@@ -113,21 +113,65 @@ __global__ void __launch_bounds__(FB) k_cells(DevParams P, Arr U, Arr R, int* er
     for (int q = 0; q < NS * NS; ++q) R(i, j, k, L.D() + q) = D[q];
 }
 
-// ---- K2: nodes (flux.cpp:1627-1713 plus the weighted node noise of timeStep.cpp:125-128) ----
+// ---- tiles: 2-D (x, y) blocks of 256 threads, one z level per block, XCD-aware order ----
+//
+// Workgroup b runs on XCD b % 8.  Tiles are numbered y-tile fastest, then x-tile, then z, and
+// XCD x receives the contiguous range x*nb/8 .. (x+1)*nb/8, so the blocks an XCD holds at once
+// cover whole z planes of one slab and their x/y neighbours share that XCD's L2.
 
-__global__ void __launch_bounds__(FB) k_nodes(DevParams P, Arr R, Arr N, Arr rA, Arr rB, double w)
+struct TileMap {
+    int tx_shift;  // tile width 1 << tx_shift (x), height 256 >> tx_shift (y)
+    int ntx, nty;
+};
+
+static TileMap make_tiles(int n0, int n1)
 {
-    const int nx = P.n[0] + 1, ny = P.n[1] + 1, nz = P.n[2] + 1;
-    long t = (long)blockIdx.x * FB + threadIdx.x;
-    if (t >= (long)nx * ny * nz) return;
+    TileMap m;
+    int s = 0;
+    while ((1 << s) < n0 && s < 6) ++s;  // width min(64, next power of two >= n0)
+    m.tx_shift = s;
+    const int tx = 1 << s, ty = FB >> s;
+    m.ntx = (n0 + tx - 1) / tx;
+    m.nty = (n1 + ty - 1) / ty;
+    return m;
+}
+
+__device__ inline bool tile_cell(const TileMap& m, int n0, int n1, int& i, int& j, int& k)
+{
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int xcd = b & 7, r = b >> 3, per = nb >> 3, rem = nb & 7;
+    const int t = xcd * per + (xcd < rem ? xcd : rem) + r;
+    const int ty = t % m.nty, rest = t / m.nty;
+    const int tx = rest % m.ntx;
+    k = rest / m.ntx;
+    i = (tx << m.tx_shift) + (threadIdx.x & ((1 << m.tx_shift) - 1));
+    j = ty * (FB >> m.tx_shift) + (threadIdx.x >> m.tx_shift);
+    return i < n0 && j < n1;
+}
+
+__device__ inline int wrap_up(int i, int n) { return i + 1 == n ? 0 : i + 1; }
+
+// ---- K2: nodes (flux.cpp:1627-1713) and the stochastic edge terms of each node -------------
+
+// Node-relative summation orders of the edge viscosities: cell (a,b,c) is node - 1 + (a,b,c);
+// the fourth entry shifts the eta cell in z (order 2 reads eta(k-2)*T(k), flux.cpp:857-858).
+__constant__ signed char c_node_order[4][8][4] = {
+    {{1,0,0,0},{0,0,0,0},{1,1,0,0},{0,1,0,0},{1,0,1,0},{0,0,1,0},{1,1,1,0},{0,1,1,0}},
+    {{0,1,0,0},{1,1,0,0},{0,0,0,0},{1,0,0,0},{0,1,1,0},{1,1,1,0},{0,0,1,0},{1,0,1,0}},
+    {{1,1,1,0},{0,1,1,-2},{1,0,1,0},{0,0,1,-2},{1,1,0,0},{0,1,0,0},{1,0,0,0},{0,0,0,0}},
+    {{0,0,1,0},{1,0,1,0},{0,1,1,0},{1,1,1,0},{0,0,0,0},{1,0,0,0},{0,1,0,0},{1,1,0,0}}};
+
+__global__ void __launch_bounds__(FB) k_nodes(DevParams P, TileMap tm, Arr R, Arr N, Arr rA, Arr rB, double w)
+{
     int i, j, k;
-    decode3(t, nx, ny, i, j, k);
+    if (!tile_cell(tm, P.n[0], P.n[1], i, j, k)) return;
     const int E = RecLayout::ETA, Z = RecLayout::ZETA;
+    const bool visc3 = (P.visc_type == 3 || P.visc_type == -3);
     double muxp = 0.125 * (R(i, j - 1, k - 1, E) + R(i - 1, j - 1, k - 1, E) + R(i, j, k - 1, E) +
                            R(i - 1, j, k - 1, E) + R(i, j - 1, k, E) + R(i - 1, j - 1, k, E) + R(i, j, k, E) +
                            R(i - 1, j, k, E));
     double zetaxp = 0.;
-    if (P.visc_type == 3 || P.visc_type == -3)
+    if (visc3)
         zetaxp = 0.125 * (R(i, j - 1, k - 1, Z) + R(i - 1, j - 1, k - 1, Z) + R(i, j, k - 1, Z) +
                           R(i - 1, j, k - 1, Z) + R(i, j - 1, k, Z) + R(i - 1, j - 1, k, Z) + R(i, j, k, Z) +
                           R(i - 1, j, k, Z));
@@ -161,57 +205,70 @@ __global__ void __launch_bounds__(FB) k_nodes(DevParams P, Arr R, Arr N, Arr rA,
          (R(i, j, k, v) - R(i, j - 1, k, v)) / DX1 + (R(i - 1, j - 1, k, ww) - R(i - 1, j - 1, k - 1, ww)) / DX2 +
          (R(i, j - 1, k, ww) - R(i, j - 1, k - 1, ww)) / DX2 + (R(i - 1, j, k, ww) - R(i - 1, j, k - 1, ww)) / DX2 +
          (R(i, j, k, ww) - R(i, j, k - 1, ww)) / DX2);
-    N(i, j, k, NODE_W) = 1.0 * rA(i, j, k) + w * rB(i, j, k);
+    if (P.stoch_on) {
+        // stage-weighted node noise (MultiFab::LinComb, timeStep.cpp:125-128)
+        const double W = 1.0 * rA(i, j, k) + w * rB(i, j, k);
+        const int RT = RecLayout::T;
+#pragma unroll
+        for (int v2 = 0; v2 < 4; ++v2) {
+            double s = 0.;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const signed char* o = c_node_order[v2][q];
+                const int a = i - 1 + o[0], bb = j - 1 + o[1], c = k - 1 + o[2];
+                double term = R(a, bb, c + o[3], E) * R(a, bb, c, RT);
+                s = (q == 0) ? term : s + term;
+            }
+            double mu = 0.25 * s / 3.;
+            if (visc3) {
+                double z = 0.;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const signed char* o = c_node_order[v2][q];
+                    const int a = i - 1 + o[0], bb = j - 1 + o[1], c = k - 1 + o[2];
+                    double term = R(a, bb, c + o[3], Z) * R(a, bb, c, RT);
+                    z = (q == 0) ? term : z + term;
+                }
+                mu += 0.25 * z;
+            }
+            N(i, j, k, NODE_S + v2) = sqrt(mu) * W;
+        }
+    }
 }
 
 // ---- K3: all fluxes of one face direction -------------------------------------------------
 
-// Edge-viscosity summation order (identical to the reference-shaped kernel's table).
-__constant__ signed char c_mu_order_f[3][4][8][6] = {
-    {{{0,0,0,0,0,0},{-1,0,0,-1,0,0},{0,1,0,0,1,0},{-1,1,0,-1,1,0},{0,0,1,0,0,1},{-1,0,1,-1,0,1},{0,1,1,0,1,1},{-1,1,1,-1,1,1}},
-     {{0,-1,0,0,-1,0},{-1,-1,0,-1,-1,0},{0,0,0,0,0,0},{-1,0,0,-1,0,0},{0,-1,1,0,-1,1},{-1,-1,1,-1,-1,1},{0,0,1,0,0,1},{-1,0,1,-1,0,1}},
-     {{0,0,-1,0,0,-1},{-1,0,-1,-1,0,-1},{0,1,-1,0,1,-1},{-1,1,-1,-1,1,-1},{0,0,0,0,0,0},{-1,0,0,-1,0,0},{0,1,0,0,1,0},{-1,1,0,-1,1,0}},
-     {{0,-1,-1,0,-1,-1},{-1,-1,-1,-1,-1,-1},{0,0,-1,0,0,-1},{-1,0,-1,-1,0,-1},{0,-1,0,0,-1,0},{-1,-1,0,-1,-1,0},{0,0,0,0,0,0},{-1,0,0,-1,0,0}}},
-    {{{1,-1,0,1,-1,0},{0,-1,0,0,-1,0},{1,0,0,1,0,0},{0,0,0,0,0,0},{1,-1,1,1,-1,1},{0,-1,1,0,-1,1},{1,0,1,1,0,1},{0,0,1,0,0,1}},
-     {{-1,0,0,-1,0,0},{0,0,0,0,0,0},{-1,-1,0,-1,-1,0},{0,-1,0,0,-1,0},{-1,0,1,-1,0,1},{0,0,1,0,0,1},{-1,-1,1,-1,-1,1},{0,-1,1,0,-1,1}},
-     {{1,-1,-1,1,-1,-1},{0,-1,-1,0,-1,-1},{1,0,-1,1,0,-1},{0,0,-1,0,0,-1},{1,-1,0,1,-1,0},{0,-1,0,0,-1,0},{1,0,0,1,0,0},{0,0,0,0,0,0}},
-     {{-1,0,-1,-1,0,-1},{0,0,-1,0,0,-1},{-1,-1,-1,-1,-1,-1},{0,-1,-1,0,-1,-1},{-1,0,0,-1,0,0},{0,0,0,0,0,0},{-1,-1,0,-1,-1,0},{0,-1,0,0,-1,0}}},
-    {{{1,0,-1,1,0,-1},{0,0,-1,0,0,-1},{1,1,-1,1,1,-1},{0,1,-1,0,1,-1},{1,0,0,1,0,0},{0,0,0,0,0,0},{1,1,0,1,1,0},{0,1,0,0,1,0}},
-     {{-1,1,-1,-1,1,-1},{0,1,-1,0,1,-1},{-1,0,-1,-1,0,-1},{0,0,-1,0,0,-1},{-1,1,0,-1,1,0},{0,1,0,0,1,0},{-1,0,0,-1,0,0},{0,0,0,0,0,0}},
-     {{1,0,0,1,0,0},{0,0,-2,0,0,0},{1,-1,0,1,-1,0},{0,-1,-2,0,-1,0},{1,0,-1,1,0,-1},{0,0,-1,0,0,-1},{1,-1,-1,1,-1,-1},{0,-1,-1,0,-1,-1}},
-     {{-1,-1,0,-1,-1,0},{0,-1,0,0,-1,0},{-1,0,0,-1,0,0},{0,0,0,0,0,0},{-1,-1,-1,-1,-1,-1},{0,-1,-1,0,-1,-1},{-1,0,-1,-1,0,-1},{0,0,-1,0,0,-1}}}};
-
-__constant__ signed char c_rc_node_f[3][4][3] = {{{0, 1, 1}, {0, 0, 1}, {0, 1, 0}, {0, 0, 0}},
-                                                 {{1, 0, 1}, {0, 0, 1}, {1, 0, 0}, {0, 0, 0}},
-                                                 {{1, 1, 0}, {0, 1, 0}, {1, 0, 0}, {0, 0, 0}}};
-
 // Sum over the 4 nodes of face (i,j,k) of component c, in visccorn order (hi,hi),(lo,hi),(hi,lo),
 // (lo,lo) over the two transverse axes, or in the z-face order (hi,hi),(hi,lo),(lo,hi),(lo,lo).
+// Node index n along an axis is node 0 (periodic), hence the wrapped "hi" indices.
 template <int D>
-__device__ inline double node_sum(const Arr& N, int c, int i, int j, int k, bool zorder, double s0, bool first)
+__device__ inline double node_sum(const Arr& N, int c, int i, int j, int k, int ih, int jh, int kh, bool zorder,
+                                  double s0, bool first)
 {
-    constexpr int AI = (D != 0), AJ = (D == 0), BJ = (D == 2), BK = (D != 2);
-    const int oa[4] = {1, zorder ? 1 : 0, zorder ? 0 : 1, 0};
-    const int ob[4] = {1, zorder ? 0 : 1, zorder ? 1 : 0, 0};
+    // first transverse axis: x faces y, y faces x, z faces x; second: x faces z, y faces z, z faces y
+    const bool ahi[4] = {true, zorder ? true : false, zorder ? false : true, false};
+    const bool bhi[4] = {true, zorder ? false : true, zorder ? true : false, false};
     double s = s0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        double v = N(i + oa[q] * AI, j + oa[q] * AJ + ob[q] * BJ, k + ob[q] * BK, c);
+        int a = i, b = j, cc = k;
+        if (D == 0) { b = ahi[q] ? jh : j; cc = bhi[q] ? kh : k; }
+        if (D == 1) { a = ahi[q] ? ih : i; cc = bhi[q] ? kh : k; }
+        if (D == 2) { a = ahi[q] ? ih : i; b = bhi[q] ? jh : j; }
+        double v = N(a, b, cc, c);
         s = (first && q == 0) ? v : s + v;
     }
     return s;
 }
 
 template <int NS, int D>
-__global__ void __launch_bounds__(FB) k_face(DevParams P, Arr Us, Arr R, Arr N, Arr A, Arr B, Arr Fo, double w,
-                                             double volinv, double dtinv, double dt)
+__global__ void __launch_bounds__(FB) k_face(DevParams P, TileMap tm, Arr Us, Arr R, Arr N, Arr A, Arr B, Arr Fo,
+                                             double w, double volinv, double dtinv, double dt)
 {
-    const int fx = P.n[0] + (D == 0), fy = P.n[1] + (D == 1), fz = P.n[2] + (D == 2);
-    long t = (long)blockIdx.x * FB + threadIdx.x;
-    if (t >= (long)fx * fy * fz) return;
     int i, j, k;
-    decode3(t, fx, fy, i, j, k);
+    if (!tile_cell(tm, P.n[0], P.n[1], i, j, k)) return;
     const int il = i - (D == 0), jl = j - (D == 1), kl = k - (D == 2);
+    const int ih = wrap_up(i, P.n[0]), jh = wrap_up(j, P.n[1]), kh = wrap_up(k, P.n[2]);
     constexpr int nv = NS + 5;
     const RecLayout L{NS};
     const int RT = RecLayout::T, RE = RecLayout::ETA;
@@ -241,30 +298,23 @@ __global__ void __launch_bounds__(FB) k_face(DevParams P, Arr Us, Arr R, Arr N, 
         wi[3] = fw1 * wn[3];
         wi[4] = fw4 * wn[4];
         double nweight = sqrt(P.k_B * volinv * dtinv);
-        double sum = 0.;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            double s = 0.;
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const signed char* o = c_mu_order_f[D][c][q];
-                double term = R(i + o[0], j + o[1], k + o[2], RE) * R(i + o[3], j + o[4], k + o[5], RT);
-                s = (q == 0) ? term : s + term;
-            }
-            double mu = 0.25 * s / 3.;
-            if (P.visc_type == 3 || P.visc_type == -3) {
-                double z = 0.;
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const signed char* o = c_mu_order_f[D][c][q];
-                    double term = R(i + o[0], j + o[1], k + o[2], RecLayout::ZETA) * R(i + o[3], j + o[4], k + o[5], RT);
-                    z = (q == 0) ? term : z + term;
-                }
-                mu += 0.25 * z;
-            }
-            const signed char* nd = c_rc_node_f[D][c];
-            double term = sqrt(mu) * N(i + nd[0], j + nd[1], k + nd[2], NODE_W);
-            sum = (c == 0) ? term : sum + term;
+        // edge terms sqrt(mu)*rancorn of the pp, mp, pm, mm edges (flux.cpp:277-280, 644-647, 1011-1014)
+        double sum;
+        if (D == 0) {
+            sum = N(i, jh, kh, NODE_S + 0);
+            sum = sum + N(i, j, kh, NODE_S + 0);
+            sum = sum + N(i, jh, k, NODE_S + 0);
+            sum = sum + N(i, j, k, NODE_S + 0);
+        } else if (D == 1) {
+            sum = N(ih, j, kh, NODE_S + 0);
+            sum = sum + N(i, j, kh, NODE_S + 1);
+            sum = sum + N(ih, j, k, NODE_S + 0);
+            sum = sum + N(i, j, k, NODE_S + 1);
+        } else {
+            sum = N(ih, jh, k, NODE_S + 0);
+            sum = sum + N(i, jh, k, NODE_S + 1);
+            sum = sum + N(ih, j, k, NODE_S + 2);
+            sum = sum + N(i, j, k, NODE_S + 3);
         }
         wi[1 + D] = wi[1 + D] + 0.25 * nweight * sum;
 #pragma unroll
@@ -404,11 +454,11 @@ __global__ void __launch_bounds__(FB) k_face(DevParams P, Arr Us, Arr R, Arr N, 
         constexpr int c2 = (D == 0) ? 8 : (D == 1) ? 8 : 0;
         constexpr int a1 = (D == 0) ? 3 : (D == 1) ? 1 : 2;
         constexpr int a2 = (D == 0) ? 6 : (D == 1) ? 7 : 5;
-        double vc4 = node_sum<D>(N, NODE_VC, i, j, k, false, 0., true);
-        double div8 = node_sum<D>(N, c1, i, j, k, zo, 0., true);
-        div8 = node_sum<D>(N, c2, i, j, k, zo, div8, false);
-        double s1 = node_sum<D>(N, a1, i, j, k, zo, 0., true);
-        double s2 = node_sum<D>(N, a2, i, j, k, zo, 0., true);
+        double vc4 = node_sum<D>(N, NODE_VC, i, j, k, ih, jh, kh, false, 0., true);
+        double div8 = node_sum<D>(N, c1, i, j, k, ih, jh, kh, zo, 0., true);
+        div8 = node_sum<D>(N, c2, i, j, k, ih, jh, kh, zo, div8, false);
+        double s1 = node_sum<D>(N, a1, i, j, k, ih, jh, kh, zo, 0., true);
+        double s2 = node_sum<D>(N, a2, i, j, k, ih, jh, kh, zo, 0., true);
         constexpr int cn = 1 + D;
         constexpr int tc1 = (D == 0) ? 2 : 1, tc2 = (D == 2) ? 2 : 3;
         f[cn] = f[cn] - 0.25 * vc4;
@@ -502,13 +552,15 @@ __global__ void __launch_bounds__(FB) k_update(DevParams P, Arr out, Arr u0, Arr
     if (t >= (long)P.n[0] * P.n[1] * P.n[2]) return;
     int i, j, k;
     decode3(t, P.n[0], P.n[1], i, j, k);
+    const int ih = wrap_up(i, P.n[0]), jh = wrap_up(j, P.n[1]), kh = wrap_up(k, P.n[2]);
     const double src = 0.0;
     double o[NS + 5];
 #pragma unroll
     for (int n = 0; n < NS + 5; ++n) {
-        double div = (fx(i + 1, j, k, n) - fx(i, j, k, n)) / P.dx[0] +
-                     (fy(i, j + 1, k, n) - fy(i, j, k, n)) / P.dx[1] +
-                     (fz(i, j, k + 1, n) - fz(i, j, k, n)) / P.dx[2];
+        // face n is face 0 of the periodic box (same inputs, same bits)
+        double div = (fx(ih, j, k, n) - fx(i, j, k, n)) / P.dx[0] +
+                     (fy(i, jh, k, n) - fy(i, j, k, n)) / P.dx[1] +
+                     (fz(i, j, kh, n) - fz(i, j, k, n)) / P.dx[2];
         if (mode == 0) o[n] = us(i, j, k, n) - dt * div + dt * src;
         else if (mode == 1) o[n] = 0.25 * (3.0 * u0(i, j, k, n) + us(i, j, k, n) - dt * div + dt * src);
         else o[n] = (2. / 3.) * (0.5 * u0(i, j, k, n) + us(i, j, k, n) - dt * div + dt * src);
@@ -542,6 +594,7 @@ __global__ void __launch_bounds__(FB) k_update(DevParams P, Arr out, Arr u0, Arr
 
 void fused_free(FusedWorkspace& w)
 {
+    w.timing.destroy();
     for (void* p : w.blocks) (void)hipFree(p);
     w.blocks.clear();
     w.ready = false;
@@ -577,12 +630,13 @@ static int ensure(const DevParams& P, FusedWorkspace& w)
         if (!r) r = alloc(w, f, 0, 0, 0, n0 - 1 + (d == 0), n1 - 1 + (d == 1), n2 - 1 + (d == 2), nc);
     };
     auto node = [&](fhdx_fab& f, int nc) { if (!r) r = alloc(w, f, 0, 0, 0, n0, n1, n2, nc); };
+    auto per = [&](fhdx_fab& f, int nc) { if (!r) r = alloc(w, f, 0, 0, 0, n0 - 1, n1 - 1, n2 - 1, nc); };
     cell(w.cup, nv);
     cell(w.cup2, nv);
     cell(w.rec, L.ncomp());
-    node(w.node, NODE_NCOMP);
+    per(w.node, NODE_NCOMP);
     for (int d = 0; d < 3; ++d) {
-        face(w.F[d], d, nv);
+        per(w.F[d], nv);
         face(w.sA[d], d, nv);
         face(w.sB[d], d, nv);
     }
@@ -595,26 +649,38 @@ static int ensure(const DevParams& P, FusedWorkspace& w)
 }
 
 template <int NS>
-static void launch_stage(const DevParams& P, const FusedWorkspace& w, Arr cu, Arr us, Arr out, Arr prim,
-                         const Arr A[3], const Arr B[3], Arr rA, Arr rB, double wst, double dt, int st, int* d_err,
-                         hipStream_t s)
+static void launch_stage(const DevParams& P, FusedWorkspace& w, Arr cu, Arr us, Arr out, Arr prim, const Arr A[3],
+                         const Arr B[3], Arr rA, Arr rB, double wst, double dt, int st, int* d_err, hipStream_t s)
 {
     const long ncell = (long)P.n[0] * P.n[1] * P.n[2];
-    const long nnode = (long)(P.n[0] + 1) * (P.n[1] + 1) * (P.n[2] + 1);
     const RecLayout L{NS};
+    const TileMap tm = make_tiles(P.n[0], P.n[1]);
+    const unsigned ntile = (unsigned)(tm.ntx * tm.nty * P.n[2]);
     Arr R = make_arr(w.rec), N = make_arr(w.node);
     Arr F0 = make_arr(w.F[0]), F1 = make_arr(w.F[1]), F2 = make_arr(w.F[2]);
+    Timing& T = w.timing;
+    T.begin(KT_CELLS, s);
     k_cells<NS><<<grid_for(ncell, FB), FB, 0, s>>>(P, us, R, d_err);
     fill_ghosts(P, R, L.ncomp(), s);
-    k_nodes<<<grid_for(nnode, FB), FB, 0, s>>>(P, R, N, rA, rB, wst);
+    T.end(s);
+    T.begin(KT_NODES, s);
+    k_nodes<<<ntile, FB, 0, s>>>(P, tm, R, N, rA, rB, wst);
+    T.end(s);
     const double volinv = 1. / (P.dx[0] * P.dx[1] * P.dx[2]);
     const double dtinv = 1. / dt;
-    auto nf = [&](int d) { return (long)(P.n[0] + (d == 0)) * (P.n[1] + (d == 1)) * (P.n[2] + (d == 2)); };
-    k_face<NS, 0><<<grid_for(nf(0), FB), FB, 0, s>>>(P, us, R, N, A[0], B[0], F0, wst, volinv, dtinv, dt);
-    k_face<NS, 1><<<grid_for(nf(1), FB), FB, 0, s>>>(P, us, R, N, A[1], B[1], F1, wst, volinv, dtinv, dt);
-    k_face<NS, 2><<<grid_for(nf(2), FB), FB, 0, s>>>(P, us, R, N, A[2], B[2], F2, wst, volinv, dtinv, dt);
+    T.begin(KT_FACE_X, s);
+    k_face<NS, 0><<<ntile, FB, 0, s>>>(P, tm, us, R, N, A[0], B[0], F0, wst, volinv, dtinv, dt);
+    T.end(s);
+    T.begin(KT_FACE_Y, s);
+    k_face<NS, 1><<<ntile, FB, 0, s>>>(P, tm, us, R, N, A[1], B[1], F1, wst, volinv, dtinv, dt);
+    T.end(s);
+    T.begin(KT_FACE_Z, s);
+    k_face<NS, 2><<<ntile, FB, 0, s>>>(P, tm, us, R, N, A[2], B[2], F2, wst, volinv, dtinv, dt);
+    T.end(s);
+    T.begin(KT_UPDATE, s);
     k_update<NS><<<grid_for(ncell, FB), FB, 0, s>>>(P, out, cu, us, F0, F1, F2, dt, st, prim, st == 2);
     fill_ghosts(P, out, P.nvars, s);
+    T.end(s);
 }
 
 int fused_rk3_step(const fhdx_params& p, const DevParams& P, FusedWorkspace& w, fhdx_fab cu, fhdx_fab prim,
@@ -639,12 +705,18 @@ int fused_rk3_step(const fhdx_params& p, const DevParams& P, FusedWorkspace& w, 
         }
         RA = make_arr(w.rA);
         RB = make_arr(w.rB);
-        if ((r = ref_fill_noise(P, K, A, B, RA, RB, s))) return r;
+        w.timing.begin(KT_NOISE, s);
+        r = ref_fill_noise(P, K, A, B, RA, RB, s);
+        w.timing.end(s);
+        if (r) return r;
     }
     double ws[3];
     stage_weights(ws);
     Arr U = make_arr(cu), U1 = make_arr(w.cup), U2 = make_arr(w.cup2), PR = make_arr(prim);
-    if ((r = fill_ghosts(P, U, P.nvars, s))) return r;
+    w.timing.begin(KT_UPDATE, s);
+    r = fill_ghosts(P, U, P.nvars, s);
+    w.timing.end(s);
+    if (r) return r;
     Arr in[3] = {U, U1, U2};
     Arr out[3] = {U1, U2, U};
     for (int st = 0; st < 3; ++st) {
@@ -656,8 +728,79 @@ int fused_rk3_step(const fhdx_params& p, const DevParams& P, FusedWorkspace& w, 
         }
         FHDX_HIP_TRY(hipGetLastError());
     }
-    if ((r = fill_ghosts(P, PR, P.nprim, s))) return r;
+    w.timing.begin(KT_UPDATE, s);
+    r = fill_ghosts(P, PR, P.nprim, s);
+    w.timing.end(s);
+    return r;
+}
+
+}  // namespace fhdx
+
+// ---- timing ------------------------------------------------------------------------------
+
+namespace fhdx {
+
+const char* const kKernelKindNames[KT_COUNT] = {"noise", "cells", "nodes", "face_x", "face_y", "face_z",
+                                                "update"};
+
+hipEvent_t Timing::get()
+{
+    if (!pool.empty()) {
+        hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+void Timing::begin(int kind, hipStream_t s)
+{
+    if (!on) return;
+    cur = kind;
+    cur_a = get();
+    (void)hipEventRecord(cur_a, s);
+}
+
+void Timing::end(hipStream_t s)
+{
+    if (!on || cur < 0) return;
+    hipEvent_t b = get();
+    (void)hipEventRecord(b, s);
+    pending.push_back({cur, cur_a, b});
+    cur = -1;
+}
+
+int Timing::resolve()
+{
+    for (const Pair& p : pending) {
+        float ms = 0.f;
+        if (hipEventSynchronize(p.b) != hipSuccess || hipEventElapsedTime(&ms, p.a, p.b) != hipSuccess)
+            return FHDX_ERR_HIP;
+        total_ms[p.kind] += ms;
+        launches[p.kind] += 1;
+        pool.push_back(p.a);
+        pool.push_back(p.b);
+    }
+    pending.clear();
     return FHDX_OK;
+}
+
+void Timing::reset()
+{
+    (void)resolve();
+    for (int k = 0; k < KT_COUNT; ++k) {
+        total_ms[k] = 0.;
+        launches[k] = 0;
+    }
+}
+
+void Timing::destroy()
+{
+    (void)resolve();
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    pool.clear();
 }
 
 }  // namespace fhdx

@@ -19,7 +19,8 @@ EXPORTS = [
     "fhdx_params_finalize", "fhdx_params_check", "fhdx_ctx_create", "fhdx_ctx_destroy",
     "fhdx_ctx_check", "fhdx_cons_to_prim", "fhdx_transport", "fhdx_fill_boundary",
     "fhdx_calculate_flux", "fhdx_fill_noise", "fhdx_rk3_step_ref", "fhdx_rk3_step",
-    "fhdx_rk3_step_injected",
+    "fhdx_rk3_step_injected", "fhdx_ctx_timing_enable", "fhdx_timing_kinds",
+    "fhdx_ctx_timing_read",
 ]
 
 _lib = None
@@ -65,6 +66,9 @@ def lib():
                                         C.c_uint64, vp]
         L.fhdx_rk3_step.argtypes = [vp, Fab, Fab, C.c_double, C.c_uint64, C.c_uint64, vp]
         L.fhdx_rk3_step_injected.argtypes = [vp, Fab, Fab, FA, FA, Fab, Fab, C.c_double, vp]
+        L.fhdx_ctx_timing_enable.argtypes = [vp, C.c_int]
+        L.fhdx_ctx_timing_read.argtypes = [vp, C.c_int, C.c_char_p, C.c_int,
+                                           C.POINTER(C.c_int64), C.POINTER(C.c_double)]
         for name in EXPORTS:
             getattr(L, name).restype = C.c_int
         L.fhdx_params_default.restype = None

@@ -162,6 +162,19 @@ int fhdx_rk3_step_injected(fhdx_ctx* ctx, fhdx_fab cu, fhdx_fab prim,
                            const fhdx_fab stochA[3], const fhdx_fab stochB[3],
                            fhdx_fab rancornA, fhdx_fab rancornB, double dt, void* stream);
 
+/* ---- measurement ----------------------------------------------------------------------- */
+
+/* Per-kernel timing of the fast path.  While enabled, fhdx_rk3_step brackets each of its kernel
+ * groups (noise, cells, nodes, face_x, face_y, face_z, update) with HIP events recorded on the
+ * launch stream; nothing synchronises inside the step.  Enabling (re)sets the totals. */
+int fhdx_ctx_timing_enable(fhdx_ctx* ctx, int enable);
+/* Number of kernel groups that fhdx_ctx_timing_read reports. */
+int fhdx_timing_kinds(void);
+/* Waits for the recorded events, then returns group `kind`'s name, timed launches and summed
+ * milliseconds. */
+int fhdx_ctx_timing_read(fhdx_ctx* ctx, int kind, char* name, int name_len, int64_t* launches,
+                         double* total_ms);
+
 #ifdef __cplusplus
 }
 #endif
