@@ -20,7 +20,10 @@ EXPORTS = [
     "fhdx_ctx_check", "fhdx_cons_to_prim", "fhdx_transport", "fhdx_fill_boundary",
     "fhdx_calculate_flux", "fhdx_fill_noise", "fhdx_rk3_step_ref", "fhdx_rk3_step",
     "fhdx_rk3_step_injected", "fhdx_ctx_timing_enable", "fhdx_timing_kinds",
-    "fhdx_ctx_timing_read",
+    "fhdx_ctx_timing_read", "fhdx_engine_bytes_needed", "fhdx_engine_create",
+    "fhdx_engine_destroy", "fhdx_engine_add", "fhdx_engine_remove", "fhdx_engine_step",
+    "fhdx_engine_sync", "fhdx_engine_state", "fhdx_engine_read", "fhdx_engine_errors",
+    "fhdx_engine_bytes_used",
 ]
 
 _lib = None
@@ -69,8 +72,23 @@ def lib():
         L.fhdx_ctx_timing_enable.argtypes = [vp, C.c_int]
         L.fhdx_ctx_timing_read.argtypes = [vp, C.c_int, C.c_char_p, C.c_int,
                                            C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+        L.fhdx_engine_bytes_needed.argtypes = [P]
+        L.fhdx_engine_create.argtypes = [C.c_int, C.c_uint64, C.POINTER(vp)]
+        L.fhdx_engine_destroy.argtypes = [vp]
+        L.fhdx_engine_add.argtypes = [vp, P, C.c_double, C.c_uint64, C.c_uint64, C.c_uint64, vp,
+                                      C.POINTER(C.c_int)]
+        L.fhdx_engine_remove.argtypes = [vp, C.c_int]
+        L.fhdx_engine_step.argtypes = [vp, C.c_int, C.c_int]
+        L.fhdx_engine_sync.argtypes = [vp]
+        L.fhdx_engine_state.argtypes = [vp, C.c_int, PF, PF, C.POINTER(C.c_uint64),
+                                        C.POINTER(C.c_int)]
+        L.fhdx_engine_read.argtypes = [vp, C.c_int, vp, vp]
+        L.fhdx_engine_errors.argtypes = [vp, C.POINTER(C.c_int)]
+        L.fhdx_engine_bytes_used.argtypes = [vp]
         for name in EXPORTS:
             getattr(L, name).restype = C.c_int
+        L.fhdx_engine_bytes_needed.restype = C.c_uint64
+        L.fhdx_engine_bytes_used.restype = C.c_uint64
         L.fhdx_params_default.restype = None
         L.fhdx_params_finalize.restype = None
         L.fhdx_status_string.restype = C.c_char_p

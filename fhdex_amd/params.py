@@ -167,14 +167,16 @@ def cfl_dt(params: Params, rhobar, T, cfl=0.1) -> float:
 ARGON_CFL = 0.1
 
 
-def argon_gas(n, nsteps=100):
+def argon_gas(n, nsteps=100, shape=None):
     """Hard-sphere argon written as two identical species (the reference's single-gas form,
     exec/compressible/inputs_test_deterministic:81-87): M 39.95, d 3.66e-8 cm, T 273 K,
-    rho 1.78e-3 g/cm^3, dx 8e-6 cm, acoustic CFL 0.1 (see ARGON_CFL)."""
-    p = make_params(2, (n, n, n), 8e-6, [39.95, 39.95], [3.66e-8, 3.66e-8])
+    rho 1.78e-3 g/cm^3, dx 8e-6 cm, acoustic CFL 0.1 (see ARGON_CFL).  The box is n^3 cells,
+    or `shape` = (nx, ny, nz)."""
+    shape = tuple(shape) if shape is not None else (n, n, n)
+    p = make_params(2, shape, 8e-6, [39.95, 39.95], [3.66e-8, 3.66e-8])
     rhobar = [0.5, 0.5]
     dt = cfl_dt(p, rhobar, 273.0, ARGON_CFL)
-    return Config(f"argon2_{n}^3", p, 1.78e-3, rhobar, 273.0, dt, nsteps)
+    return Config(f"argon2_{shape[0]}x{shape[1]}x{shape[2]}", p, 1.78e-3, rhobar, 273.0, dt, nsteps)
 
 
 # The reference's own four-species deck runs at a fixed dt of 1e-12 s

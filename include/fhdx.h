@@ -175,6 +175,48 @@ int fhdx_timing_kinds(void);
 int fhdx_ctx_timing_read(fhdx_ctx* ctx, int kind, char* name, int name_len, int64_t* launches,
                          double* total_ms);
 
+/* ---- ensembles: many independent boxes advanced together -------------------------------- */
+
+/* The reference advances one box per process (main_driver.cpp:574 calls RK3step once per
+ * step); these calls have no reference counterpart.  They batch independent instances of the
+ * same step (any box size, 2-4 species each) into shared launches on one stream: per kernel
+ * kind and species count, one launch covers every admitted instance.  Instance results are
+ * bit-identical to fhdx_rk3_step on the same (state, dt, seed, step): the noise of an instance
+ * is keyed by its own (seed, step, instance 0). */
+typedef struct fhdx_engine fhdx_engine;
+
+/* HBM bytes an instance with these parameters holds while admitted. */
+uint64_t fhdx_engine_bytes_needed(const fhdx_params* p);
+/* At most max_instances slots; admission fails once the admitted instances would hold more
+ * than mem_budget_bytes of HBM (FHDX_ERR_NOMEM) or no slot is free (FHDX_ERR_BUSY). */
+int fhdx_engine_create(int max_instances, uint64_t mem_budget_bytes, fhdx_engine** out);
+int fhdx_engine_destroy(fhdx_engine* e);
+/* Admit an instance: cu_init (host or device pointer) holds its conserved state on the box
+ * grown by ngc, component-major like a FAB.  The instance takes steps step0, step0+1, ... and
+ * idles after nsteps of them (nsteps = 0: no limit).  Returns its slot. */
+int fhdx_engine_add(fhdx_engine* e, const fhdx_params* p, double dt, uint64_t seed,
+                    uint64_t step0, uint64_t nsteps, const double* cu_init, int* slot);
+/* Release a slot and its memory. */
+int fhdx_engine_remove(fhdx_engine* e, int slot);
+/* Queue nsteps steps of every admitted instance on the engine's stream.  use_graph != 0
+ * captures one step of the current membership in a hipGraph (again after each add/remove)
+ * and replays it. Returns without waiting. */
+int fhdx_engine_step(fhdx_engine* e, int nsteps, int use_graph);
+/* Wait for the queued steps. */
+int fhdx_engine_sync(fhdx_engine* e);
+/* Device FABs of a slot's conserved and primitive state (valid after fhdx_engine_sync), its
+ * next step index and whether it has taken all its steps.  Any output may be NULL. */
+int fhdx_engine_state(fhdx_engine* e, int slot, fhdx_fab* cu, fhdx_fab* prim, uint64_t* step,
+                      int* done);
+/* Wait for the queued steps, then copy a slot's grown conserved / primitive state to cu_out /
+ * prim_out (host or device pointers; either may be NULL). */
+int fhdx_engine_read(fhdx_engine* e, int slot, double* cu_out, double* prim_out);
+/* Wait, then copy (and clear) the per-slot device error flags (max_instances ints; bit 0:
+ * mass fraction outside (0,1), as FHDX_ERR_MASSFRAC). */
+int fhdx_engine_errors(fhdx_engine* e, int* flags);
+/* HBM bytes held by the admitted instances. */
+uint64_t fhdx_engine_bytes_used(fhdx_engine* e);
+
 #ifdef __cplusplus
 }
 #endif
