@@ -124,6 +124,45 @@ def ensemble_bench(count: int, steps: int, loop_steps: int):
             "hbm_bytes_held": budget}
 
 
+def serve_bench(rate: float, count: int, max_instances: int = 256, budget_gb: float = 160.0,
+                chunk: int = 8, seed: int = 7):
+    """The served stream: `count` requests with Poisson arrivals at `rate`/s (edge log-uniform
+    in [16,128], 1-4 species, nsteps in [200,2000]), served by the continuous-batching server and,
+    for comparison, by the same server with one slot (requests strictly one after another).
+    Latency is arrival to result in host memory."""
+    import numpy as np
+    from fhdex_amd import serving as S
+
+    reqs = S.request_stream(count, seed)
+    work = sum(c.cells * n for c, _, n, _ in reqs)
+
+    def one(slots):
+        srv = S.Server(slots, int(budget_gb * 2 ** 30), chunk_steps=chunk, graph=True)
+        done, wall, open_n, _ = S.run_stream(srv, reqs, rate, seed=seed)
+        srv.close()
+        ok = [c for c in done if c["status"] == 0]
+        lat = np.array([(c["done_ns"] - c["arrival_ns"]) * 1e-9 for c in ok])
+        queue = np.array([(c["admit_ns"] - c["arrival_ns"]) * 1e-9 for c in ok])
+        t_first = min(c["arrival_ns"] for c in done)
+        t_last = max(c["done_ns"] for c in done)
+        span = (t_last - t_first) * 1e-9
+        return {"completed": len(ok), "failed": len(done) - len(ok), "open": open_n,
+                "p50_latency_s": float(np.percentile(lat, 50)) if len(lat) else None,
+                "p99_latency_s": float(np.percentile(lat, 99)) if len(lat) else None,
+                "mean_queue_s": float(queue.mean()) if len(queue) else None,
+                "value": work / span, "unit": "cell-updates/s", "span_s": span}
+
+    batched = one(max_instances)
+    plain = one(1)
+    return {"workload": f"{count} requests, Poisson {rate}/s, edge log-uniform [16,128], "
+                        f"species 1-4, nsteps 200-2000 (stream seed {seed})",
+            "rate_per_s": rate, "requests": count, "cell_updates": work,
+            "offered_load": work * rate / count,
+            "batched": batched, "one_at_a_time": plain,
+            "speedup_vs_one_at_a_time": batched["value"] / plain["value"],
+            "slots": max_instances, "budget_GiB": budget_gb, "chunk_steps": chunk}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +176,10 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--loop-steps", type=int, default=5)
     ap.add_argument("--ensemble-only", action="store_true")
+    ap.add_argument("--serve-rates", type=str, default="50",
+                    help="comma-separated arrival rates of the served stream ('' to skip)")
+    ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates HBM byte counters)")
     a = ap.parse_args()
@@ -160,6 +203,11 @@ def main():
         gbs = 5 * 2 * n * 8 / (time.perf_counter() - t0) / 1e9
         print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8,
                           "copy_GBs": gbs}), flush=True)
+        return
+
+    if a.serve_only:
+        for r in [float(v) for v in a.serve_rates.split(",") if v]:
+            print(json.dumps({"serving": serve_bench(r, a.serve_requests)}), flush=True)
         return
 
     if a.ensemble_only:
@@ -237,6 +285,9 @@ def main():
     del cu, prim
     if a.ensemble:
         line["ensemble"] = ensemble_bench(a.ensemble, a.ensemble_steps, a.loop_steps)
+    rates = [float(v) for v in a.serve_rates.split(",") if v]
+    if rates:
+        line["serving"] = [serve_bench(r, a.serve_requests) for r in rates]
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(a.cpu_n, a.cpu_steps)
     print(json.dumps(line), flush=True)

@@ -207,9 +207,12 @@ int engine_destroy(Engine* e)
 }
 
 int engine_add(Engine* e, const fhdx_params& p, double dt, uint64_t seed, uint64_t step0, uint64_t nsteps,
-               const double* cu_init, int* slot)
+               const double* cu_init, const double* uniform, int* slot)
 {
-    if (!e || !slot || !cu_init || !(dt > 0.)) return FHDX_ERR_ARG;
+    if (!e || !slot || (!cu_init && !uniform) || !(dt > 0.)) {
+        set_error("engine_add: bad argument (null slot/state or dt <= 0)");
+        return FHDX_ERR_ARG;
+    }
     int r = fhdx_params_check(&p);
     if (r) return r;
     const uint64_t need = engine_bytes_needed(p);
@@ -265,7 +268,13 @@ int engine_add(Engine* e, const fhdx_params& p, double dt, uint64_t seed, uint64
     d.nghost = ghosts_of(p);
     s.bytes = need;
     e->used += need;
-    FHDX_HIP_TRY(hipMemcpyAsync(d.U.p, cu_init, sizeof(double) * grown_of(p) * nv, hipMemcpyDefault, e->stream));
+    if (cu_init) {
+        FHDX_HIP_TRY(
+            hipMemcpyAsync(d.U.p, cu_init, sizeof(double) * grown_of(p) * nv, hipMemcpyDefault, e->stream));
+    } else {
+        for (int c = 0; c < nv && !r; ++c) r = fill_value(d.U.p + c * d.U.sn, d.U.sn, uniform[c], e->stream);
+        if (r) return r;
+    }
     FHDX_HIP_TRY(hipMemcpyAsync(e->d_inst + sid, &d, sizeof(Inst), hipMemcpyHostToDevice, e->stream));
     FHDX_HIP_TRY(hipMemsetAsync(e->d_err + sid, 0, sizeof(int), e->stream));
     FHDX_HIP_TRY(hipStreamSynchronize(e->stream));
@@ -389,6 +398,28 @@ int engine_sync(Engine* e)
     return FHDX_OK;
 }
 
+int engine_errors(Engine* e, int* flags)
+{
+    FHDX_HIP_TRY(hipStreamSynchronize(e->stream));
+    FHDX_HIP_TRY(hipMemcpy(flags, e->d_err, sizeof(int) * e->max_inst, hipMemcpyDeviceToHost));
+    FHDX_HIP_TRY(hipMemset(e->d_err, 0, sizeof(int) * e->max_inst));
+    return FHDX_OK;
+}
+
+int engine_read(Engine* e, int slot, double* cu_out, double* prim_out)
+{
+    if (slot < 0 || slot >= e->max_inst || !e->slots[slot].used) return FHDX_ERR_ARG;
+    const EngineSlot& s = e->slots[slot];
+    const size_t pts = (size_t)grown_of(s.p);
+    FHDX_HIP_TRY(hipStreamSynchronize(e->stream));
+    if (cu_out)
+        FHDX_HIP_TRY(hipMemcpy(cu_out, s.desc.U.p, sizeof(double) * pts * (5 + s.p.nspecies), hipMemcpyDefault));
+    if (prim_out)
+        FHDX_HIP_TRY(
+            hipMemcpy(prim_out, s.desc.prim.p, sizeof(double) * pts * (6 + 2 * s.p.nspecies), hipMemcpyDefault));
+    return FHDX_OK;
+}
+
 }  // namespace fhdx
 
 // ---- C ABI -----------------------------------------------------------------------------
@@ -425,7 +456,14 @@ int fhdx_engine_add(fhdx_engine* h, const fhdx_params* p, double dt, uint64_t se
                     uint64_t nsteps, const double* cu_init, int* slot)
 {
     if (!h || !p) return FHDX_ERR_ARG;
-    return engine_add(h->e, *p, dt, seed, step0, nsteps, cu_init, slot);
+    return engine_add(h->e, *p, dt, seed, step0, nsteps, cu_init, nullptr, slot);
+}
+
+int fhdx_engine_add_uniform(fhdx_engine* h, const fhdx_params* p, double dt, uint64_t seed, uint64_t step0,
+                            uint64_t nsteps, const double* values, int* slot)
+{
+    if (!h || !p || !values) return FHDX_ERR_ARG;
+    return engine_add(h->e, *p, dt, seed, step0, nsteps, nullptr, values, slot);
 }
 
 int fhdx_engine_remove(fhdx_engine* h, int slot) { return h ? engine_remove(h->e, slot) : FHDX_ERR_ARG; }
@@ -462,26 +500,13 @@ int fhdx_engine_state(fhdx_engine* h, int slot, fhdx_fab* cu, fhdx_fab* prim, ui
 
 int fhdx_engine_read(fhdx_engine* h, int slot, double* cu_out, double* prim_out)
 {
-    if (!h || slot < 0 || slot >= h->e->max_inst || !h->e->slots[slot].used) return FHDX_ERR_ARG;
-    Engine* e = h->e;
-    const EngineSlot& s = e->slots[slot];
-    const size_t pts = (size_t)grown_of(s.p);
-    FHDX_HIP_TRY(hipStreamSynchronize(e->stream));
-    if (cu_out) FHDX_HIP_TRY(hipMemcpy(cu_out, s.desc.U.p, sizeof(double) * pts * (5 + s.p.nspecies), hipMemcpyDefault));
-    if (prim_out)
-        FHDX_HIP_TRY(
-            hipMemcpy(prim_out, s.desc.prim.p, sizeof(double) * pts * (6 + 2 * s.p.nspecies), hipMemcpyDefault));
-    return FHDX_OK;
+    return h ? engine_read(h->e, slot, cu_out, prim_out) : FHDX_ERR_ARG;
 }
 
 int fhdx_engine_errors(fhdx_engine* h, int* flags)
 {
     if (!h || !flags) return FHDX_ERR_ARG;
-    Engine* e = h->e;
-    FHDX_HIP_TRY(hipStreamSynchronize(e->stream));
-    FHDX_HIP_TRY(hipMemcpy(flags, e->d_err, sizeof(int) * e->max_inst, hipMemcpyDeviceToHost));
-    FHDX_HIP_TRY(hipMemset(e->d_err, 0, sizeof(int) * e->max_inst));
-    return FHDX_OK;
+    return engine_errors(h->e, flags);
 }
 
 uint64_t fhdx_engine_bytes_used(fhdx_engine* h) { return h ? h->e->used : 0; }

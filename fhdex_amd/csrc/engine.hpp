@@ -61,10 +61,16 @@ struct Engine {
 uint64_t engine_bytes_needed(const fhdx_params& p);
 int engine_create(int max_inst, uint64_t budget, Engine** out);
 int engine_destroy(Engine* e);
+// cu_init: the grown conserved state (host or device); when NULL, component c of every cell is
+// set to uniform[c] instead.
 int engine_add(Engine* e, const fhdx_params& p, double dt, uint64_t seed, uint64_t step0, uint64_t nsteps,
-               const double* cu_init, int* slot);
+               const double* cu_init, const double* uniform, int* slot);
 int engine_remove(Engine* e, int slot);
 int engine_step(Engine* e, int nsteps, bool use_graph);
 int engine_sync(Engine* e);
+// Waits, then copies (and clears) the per-slot error flags into flags[max_inst].
+int engine_errors(Engine* e, int* flags);
+// Waits, then copies a slot's grown conserved / primitive state (either pointer may be NULL).
+int engine_read(Engine* e, int slot, double* cu_out, double* prim_out);
 
 }  // namespace fhdx

@@ -21,17 +21,24 @@ def grown_shape(p: Params, ncomp: int):
     return (ncomp, n2 + 2 * g, n1 + 2 * g, n0 + 2 * g)
 
 
-def uniform_cons(cfg: Config) -> np.ndarray:
-    """Uniform start (InitConsVar prob_type 1, compressible_functions.cpp:163-196) on the box
-    grown by ngc: rho0, zero momentum, rho0 * sum_k Y_k (e0_k + cv_k T), rho0 * Y_k."""
+def uniform_values(cfg: Config):
+    """Per-component values of the uniform start (InitConsVar prob_type 1,
+    compressible_functions.cpp:163-196): rho0, zero momentum, rho0 * sum_k Y_k (e0_k + cv_k T),
+    rho0 * Y_k."""
     p = cfg.params
     ns = p.nspecies
-    cu = np.zeros(grown_shape(p, p.nvars), dtype=np.float64)
-    ie = sum(cfg.rhobar[k] * (p.e0[k] + p.hcv[k] * cfg.T_init) for k in range(ns))
-    cu[0] = cfg.rho0
-    cu[4] = cfg.rho0 * ie
+    ie = 0.0
     for k in range(ns):
-        cu[5 + k] = cfg.rho0 * cfg.rhobar[k]
+        ie += cfg.rhobar[k] * (p.e0[k] + p.hcv[k] * cfg.T_init)
+    return [cfg.rho0, 0.0, 0.0, 0.0, cfg.rho0 * ie] + [cfg.rho0 * cfg.rhobar[k] for k in range(ns)]
+
+
+def uniform_cons(cfg: Config) -> np.ndarray:
+    """The uniform start on the box grown by ngc, as a host array."""
+    p = cfg.params
+    cu = np.zeros(grown_shape(p, p.nvars), dtype=np.float64)
+    for c, v in enumerate(uniform_values(cfg)):
+        cu[c] = v
     return cu
 
 
@@ -74,6 +81,17 @@ class Ensemble:
         slot = C.c_int(-1)
         check(lib().fhdx_engine_add(self.h, C.byref(p), float(dt), int(seed), int(step0),
                                     int(nsteps), C.c_void_p(ptr), C.byref(slot)), "fhdx_engine_add")
+        self.params[slot.value] = p.copy()
+        return slot.value
+
+    def add_uniform(self, cfg: Config, seed: int, step0: int = 0, nsteps: int = 0) -> int:
+        """Admit an instance with the uniform start of `cfg` (filled on the device)."""
+        p = cfg.params
+        vals = uniform_values(cfg)
+        arr = (C.c_double * len(vals))(*vals)
+        slot = C.c_int(-1)
+        check(lib().fhdx_engine_add_uniform(self.h, C.byref(p), float(cfg.dt), int(seed), int(step0),
+                                            int(nsteps), arr, C.byref(slot)), "fhdx_engine_add_uniform")
         self.params[slot.value] = p.copy()
         return slot.value
 

@@ -23,7 +23,9 @@ EXPORTS = [
     "fhdx_ctx_timing_read", "fhdx_engine_bytes_needed", "fhdx_engine_create",
     "fhdx_engine_destroy", "fhdx_engine_add", "fhdx_engine_remove", "fhdx_engine_step",
     "fhdx_engine_sync", "fhdx_engine_state", "fhdx_engine_read", "fhdx_engine_errors",
-    "fhdx_engine_bytes_used",
+    "fhdx_engine_bytes_used", "fhdx_engine_add_uniform", "fhdx_server_create",
+    "fhdx_server_submit", "fhdx_server_poll", "fhdx_server_wait", "fhdx_server_stats",
+    "fhdx_server_destroy",
 ]
 
 _lib = None
@@ -77,6 +79,8 @@ def lib():
         L.fhdx_engine_destroy.argtypes = [vp]
         L.fhdx_engine_add.argtypes = [vp, P, C.c_double, C.c_uint64, C.c_uint64, C.c_uint64, vp,
                                       C.POINTER(C.c_int)]
+        L.fhdx_engine_add_uniform.argtypes = [vp, P, C.c_double, C.c_uint64, C.c_uint64,
+                                              C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.fhdx_engine_remove.argtypes = [vp, C.c_int]
         L.fhdx_engine_step.argtypes = [vp, C.c_int, C.c_int]
         L.fhdx_engine_sync.argtypes = [vp]

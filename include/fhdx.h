@@ -196,6 +196,10 @@ int fhdx_engine_destroy(fhdx_engine* e);
  * idles after nsteps of them (nsteps = 0: no limit).  Returns its slot. */
 int fhdx_engine_add(fhdx_engine* e, const fhdx_params* p, double dt, uint64_t seed,
                     uint64_t step0, uint64_t nsteps, const double* cu_init, int* slot);
+/* As fhdx_engine_add, with a uniform start: component c of every cell (ghosts included) is
+ * values[c], c = 0..nvars-1. */
+int fhdx_engine_add_uniform(fhdx_engine* e, const fhdx_params* p, double dt, uint64_t seed,
+                            uint64_t step0, uint64_t nsteps, const double* values, int* slot);
 /* Release a slot and its memory. */
 int fhdx_engine_remove(fhdx_engine* e, int slot);
 /* Queue nsteps steps of every admitted instance on the engine's stream.  use_graph != 0
@@ -216,6 +220,58 @@ int fhdx_engine_read(fhdx_engine* e, int slot, double* cu_out, double* prim_out)
 int fhdx_engine_errors(fhdx_engine* e, int* flags);
 /* HBM bytes held by the admitted instances. */
 uint64_t fhdx_engine_bytes_used(fhdx_engine* e);
+
+/* ---- serving: continuous batching of arriving ensemble runs ----------------------------- */
+
+/* No reference counterpart.  A worker thread owns an engine: submitted requests are admitted
+ * in arrival order as the memory budget and a slot allow, every admitted instance advances in
+ * the same launches (new arrivals join at the next chunk of at most chunk_steps steps), and a
+ * finished instance's state is copied to host memory before its completion is posted.  A
+ * request whose instance raises an error flag is evicted alone.  Results are bit-identical to a
+ * solo fhdx_rk3_step run with the same (state, dt, seed), steps 0..nsteps-1. */
+typedef struct fhdx_server fhdx_server;
+
+typedef struct fhdx_request {
+    fhdx_params params;       /* box, species, physics (as for fhdx_ctx_create)            */
+    double dt;
+    uint64_t seed;            /* Philox key; steps 0..nsteps-1                              */
+    uint64_t nsteps;          /* >= 1                                                       */
+    const double* cu_init;    /* grown conserved start state (host or device), kept alive by
+                                 the caller until completion; NULL: uniform start below     */
+    double* cu_out;           /* host/device buffer for the grown final cu, or NULL (the
+                                 result is then copied to a server-owned host buffer)       */
+    double* prim_out;         /* grown final prim, or NULL                                  */
+    double rho0, T0;          /* uniform start: density, temperature ...                    */
+    double Y0[FHDX_MAX_SPECIES]; /* ... and mass fractions (InitConsVar prob_type 1)       */
+} fhdx_request;
+
+typedef struct fhdx_completion {
+    int64_t id;               /* as returned by fhdx_server_submit                          */
+    int32_t status;           /* FHDX_OK, or why the request failed / was evicted           */
+    int32_t pad;
+    int64_t arrival_ns;       /* steady-clock times: submission, admission (0: never) ...   */
+    int64_t admit_ns;
+    int64_t done_ns;          /* ... and result on host                                     */
+    uint64_t steps;           /* steps taken                                                */
+    char message[96];
+} fhdx_completion;
+
+/* max_instances slots, mem_budget_bytes of HBM for admitted instances, at most chunk_steps
+ * steps between admission points, use_graph != 0 to replay hipGraph-captured steps. */
+int fhdx_server_create(int max_instances, uint64_t mem_budget_bytes, int chunk_steps,
+                       int use_graph, fhdx_server** out);
+/* Thread-safe.  Records the arrival time and queues the request. */
+int fhdx_server_submit(fhdx_server* s, const fhdx_request* r, int64_t* id);
+/* Moves up to max completions out (n = how many). */
+int fhdx_server_poll(fhdx_server* s, fhdx_completion* out, int max, int* n);
+/* Waits until every submitted request has completed or timeout_s passes (then FHDX_ERR_BUSY);
+ * open = requests not yet completed. */
+int fhdx_server_wait(fhdx_server* s, double timeout_s, int64_t* open);
+/* Steps taken by the worker, cell-updates done (summed over instances), HBM bytes held. */
+int fhdx_server_stats(fhdx_server* s, uint64_t* steps, uint64_t* cell_updates,
+                      uint64_t* bytes_used);
+/* Stops the worker; requests not finished complete as FHDX_ERR_BUSY ("server stopped"). */
+int fhdx_server_destroy(fhdx_server* s);
 
 #ifdef __cplusplus
 }

@@ -59,6 +59,31 @@ def test_struct_layouts_match_header(tmp_path):
         assert getattr(P.Fab, f).offset == offs[len(fields) + i], f
 
 
+def test_serving_struct_layouts_match_header(tmp_path):
+    """fhdx_request / fhdx_completion: ctypes offsets equal offsetof."""
+    from fhdex_amd import serving as S
+    structs = [("fhdx_request", S.Request), ("fhdx_completion", S.Completion)]
+    src = ["#include <stdio.h>", "#include <stddef.h>", '#include "fhdx.h"', "int main(void){"]
+    for cname, py in structs:
+        src.append(f'printf("%zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            src.append(f'printf("%zu\\n", offsetof({cname}, {f}));')
+    src.append("return 0;}")
+    c = tmp_path / "layout2.c"
+    c.write_text("\n".join(src))
+    exe = tmp_path / "layout2"
+    import subprocess
+    subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), str(c), "-o", str(exe)])
+    vals = [int(v) for v in subprocess.check_output([str(exe)]).decode().split()]
+    q = 0
+    for cname, py in structs:
+        assert vals[q] == C.sizeof(py), cname
+        q += 1
+        for f, _ in py._fields_:
+            assert getattr(py, f).offset == vals[q], (cname, f)
+            q += 1
+
+
 def test_params_default_finalize_check():
     lib = L.lib()
     p = P.Params()
