@@ -180,6 +180,8 @@ def main():
                     help="comma-separated arrival rates of the served stream ('' to skip)")
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--serve-only", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--shape", type=str, default="", help="launch shape tile_width,region_tiles,split")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates HBM byte counters)")
     a = ap.parse_args()
@@ -232,6 +234,13 @@ def main():
     cu.t[5] = cfg.rho0 * cfg.rhobar[0]
     cu.t[6] = cfg.rho0 * cfg.rhobar[1]
     seed = 1
+    tune = None
+    if a.shape:
+        tw, rh, split = (int(v) for v in a.shape.split(","))
+        ctx.set_tune(tw, rh, split)
+    elif not a.no_tune:
+        best, trials = ctx.autotune(cu.fab, cfg.dt, trial_steps=2)
+        tune = {"chosen": best.as_dict(), "trials": [t.as_dict() for t in trials]}
     for s in range(a.warmup):
         fc.rk3_step(ctx, cu, prim, cfg.dt, seed, s)
     torch.cuda.synchronize()
@@ -280,6 +289,7 @@ def main():
                    "b_alg_bytes_per_cell_update": b_alg(nv),
                    "step_hbm_frac_b_alg": rate * b_alg(nv) / 1e9 / HBM_PEAK_GBS},
         "roofline": roof,
+        "launch_shape": ctx.get_tune().as_dict() if tune is None else tune,
     }
     ctx.close()
     del cu, prim

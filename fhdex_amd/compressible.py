@@ -136,3 +136,30 @@ def rk3_step_injected(ctx: Context, cu: MultiFab, prim: MultiFab, noise: NoiseSe
                       stream=None):
     check(lib().fhdx_rk3_step_injected(ctx.h, cu.fab, prim.fab, _fa(noise.A), _fa(noise.B),
                                        noise.rA.fab, noise.rB.fab, dt, stream), "rk3_step_injected")
+
+
+class StatsState:
+    """The statistics MultiFabs of main_driver.cpp:205-238 on the device (zero-initialised):
+    cuMeans, cuVars (nvars), primMeans (nprimvars), primVars (nprimvars+5), spatialCross (8),
+    miscStats (10), plus miscVals[20] on the host."""
+
+    def __init__(self, p: Params, device="cuda"):
+        import numpy as np
+        self.consMean = MultiFab.cell(p, p.nvars, device)
+        self.consVar = MultiFab.cell(p, p.nvars, device)
+        self.primMean = MultiFab.cell(p, p.nprimvars, device)
+        self.primVar = MultiFab.cell(p, p.nprimvars + 5, device)
+        self.spatialCross = MultiFab.cell(p, 8, device)
+        self.miscStats = MultiFab.cell(p, 10, device)
+        self.miscVals = np.zeros(20)
+
+
+def evaluateStats(ctx: Context, cons: MultiFab, prim: MultiFab, st: StatsState, steps: int,
+                  cross_cell: int = 0, stream=None):
+    """evaluateStats (stats.cpp:5-363): updates the running statistics in `st`; steps is the
+    running sample count (statsCount, starting at 1)."""
+    check(lib().fhdx_evaluate_stats(ctx.h, cons.fab, st.consMean.fab, st.consVar.fab, prim.fab,
+                                    st.primMean.fab, st.primVar.fab, st.spatialCross.fab,
+                                    st.miscStats.fab,
+                                    st.miscVals.ctypes.data_as(C.POINTER(C.c_double)), int(steps),
+                                    int(cross_cell), stream), "evaluateStats")

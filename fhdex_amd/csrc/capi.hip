@@ -7,7 +7,10 @@
 
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "fused.hpp"
@@ -40,6 +43,8 @@ struct fhdx_ctx {
     int* d_err = nullptr;  // device-side error flags (bit 0: mass fraction outside (0,1))
     RefWorkspace ref;
     FusedWorkspace fused;
+    double *st_yz = nullptr, *st_part = nullptr, *st_misc = nullptr;  // evaluateStats scratch
+    int st_nparts = 0;
 };
 
 extern "C" {
@@ -132,6 +137,14 @@ int fhdx_params_check(const fhdx_params* p)
             set_error("ngc must not exceed n_cells (periodic images wrap once)");
             return FHDX_ERR_UNSUPPORTED;
         }
+    {
+        const int g = p->ngc;
+        const double plane = 8.0 * (p->n_cells[0] + 2.0 * g) * (p->n_cells[1] + 2.0 * g) * (p->n_cells[2] + 2.0 * g);
+        if (plane >= 4294967296.0) {
+            set_error("one component of a cell array must stay below 4 GiB (box too large for one context)");
+            return FHDX_ERR_UNSUPPORTED;
+        }
+    }
     if (p->transport_type != 1) {
         set_error("only transport_type 1 (Giovangigli) is implemented");
         return FHDX_ERR_UNSUPPORTED;
@@ -334,6 +347,9 @@ int fhdx_ctx_destroy(fhdx_ctx* c)
     (void)hipDeviceSynchronize();
     free_ref_workspace(c->ref);
     fused_free(c->fused);
+    if (c->st_yz) (void)hipFree(c->st_yz);
+    if (c->st_part) (void)hipFree(c->st_part);
+    if (c->st_misc) (void)hipFree(c->st_misc);
     if (c->d_err) (void)hipFree(c->d_err);
     delete c;
     return FHDX_OK;
@@ -500,6 +516,192 @@ int fhdx_rk3_step_injected(fhdx_ctx* c, fhdx_fab cu, fhdx_fab prim, const fhdx_f
     if (!ok) return FHDX_ERR_ARG;
     return fused_rk3_step(c->p, c->dp, c->fused, cu, prim, dt, make_noise_key(0, 0, 0), stochA, stochB, &rancornA,
                           &rancornB, c->d_err, S(stream));
+}
+
+// ---- evaluateStats ----------------------------------------------------------------------
+
+int fhdx_evaluate_stats(fhdx_ctx* c, fhdx_fab cons, fhdx_fab consMean, fhdx_fab consVar, fhdx_fab prim,
+                        fhdx_fab primMean, fhdx_fab primVar, fhdx_fab spatialCross, fhdx_fab miscStats,
+                        double* miscVals, int steps, int cross_cell, void* stream)
+{
+    if (!c || !miscVals) return FHDX_ERR_ARG;
+    const int nv = c->dp.nvars, np = c->dp.nprim;
+    if (!check_cell(c, cons, nv, "cons") || !check_cell(c, consMean, nv, "consMean") ||
+        !check_cell(c, consVar, nv, "consVar") || !check_cell(c, prim, np, "prim") ||
+        !check_cell(c, primMean, np, "primMean") || !check_cell(c, primVar, np + 5, "primVar") ||
+        !check_cell(c, spatialCross, 8, "spatialCross") || !check_cell(c, miscStats, 6, "miscStats"))
+        return FHDX_ERR_ARG;
+    if (steps < 1 || cross_cell < 0 || cross_cell >= c->p.n_cells[0]) {
+        set_error("fhdx_evaluate_stats: steps must be >= 1 and 0 <= cross_cell < n_cells[0]");
+        return FHDX_ERR_ARG;
+    }
+    const int n0 = c->p.n_cells[0];
+    if (!c->st_yz) {
+        const long nyz = (long)c->p.n_cells[1] * c->p.n_cells[2];
+        c->st_nparts = (int)(nyz < 256 ? nyz : 256);
+        FHDX_HIP_TRY(hipMalloc((void**)&c->st_yz, sizeof(double) * n0 * 19));
+        FHDX_HIP_TRY(hipMalloc((void**)&c->st_part, sizeof(double) * c->st_nparts * n0 * 19));
+        FHDX_HIP_TRY(hipMalloc((void**)&c->st_misc, sizeof(double) * 20));
+    }
+    StatsArgs a;
+    a.cu = make_arr(cons);
+    a.cm = make_arr(consMean);
+    a.cv = make_arr(consVar);
+    a.prim = make_arr(prim);
+    a.pm = make_arr(primMean);
+    a.pv = make_arr(primVar);
+    a.sc = make_arr(spatialCross);
+    a.ms = make_arr(miscStats);
+    a.yz = c->st_yz;
+    a.part = c->st_part;
+    a.misc = c->st_misc;
+    a.nparts = c->st_nparts;
+    a.steps = steps;
+    a.cross_cell = cross_cell;
+    int r = stats_launch(c->dp, a, S(stream));
+    if (r) return r;
+    FHDX_HIP_TRY(hipMemcpyAsync(miscVals, c->st_misc, sizeof(double) * 20, hipMemcpyDeviceToHost, S(stream)));
+    FHDX_HIP_TRY(hipStreamSynchronize(S(stream)));
+    return FHDX_OK;
+}
+
+// ---- launch-shape tuning ---------------------------------------------------------------
+
+namespace {
+
+struct TuneKey {
+    int ns, n0, n1, n2;
+    bool operator<(const TuneKey& o) const
+    {
+        return std::tie(ns, n0, n1, n2) < std::tie(o.ns, o.n0, o.n1, o.n2);
+    }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, fhdx_tune> g_tune_cache;
+
+void apply_tune(FusedWorkspace& w, const fhdx_tune& t)
+{
+    w.tw_max = t.tile_width;
+    w.rh = t.region_tiles;
+    w.split_faces = t.split_faces != 0;
+}
+
+bool valid_tune(const fhdx_tune& t)
+{
+    return (t.tile_width == 16 || t.tile_width == 32 || t.tile_width == 64) && t.region_tiles >= 1 &&
+           t.region_tiles <= 64 && (t.split_faces == 0 || t.split_faces == 1);
+}
+
+}  // namespace
+
+int fhdx_ctx_set_tune(fhdx_ctx* c, const fhdx_tune* t)
+{
+    if (!c || !t || !valid_tune(*t)) {
+        set_error("fhdx_ctx_set_tune: tile_width must be 16, 32 or 64, region_tiles 1..64, split_faces 0/1");
+        return FHDX_ERR_ARG;
+    }
+    apply_tune(c->fused, *t);
+    return FHDX_OK;
+}
+
+int fhdx_ctx_get_tune(fhdx_ctx* c, fhdx_tune* t)
+{
+    if (!c || !t) return FHDX_ERR_ARG;
+    t->tile_width = c->fused.tw_max;
+    t->region_tiles = c->fused.rh;
+    t->split_faces = c->fused.split_faces ? 1 : 0;
+    t->pad = 0;
+    t->ms_per_step = 0.;
+    return FHDX_OK;
+}
+
+int fhdx_ctx_autotune(fhdx_ctx* c, fhdx_fab cu, double dt, int trial_steps, fhdx_tune* best, fhdx_tune* trials,
+                      int max_trials, int* n_trials, void* stream)
+{
+    if (!c || !best || trial_steps < 1 || !(dt > 0.)) return FHDX_ERR_ARG;
+    if (n_trials) *n_trials = 0;
+    if (!check_cell(c, cu, c->dp.nvars, "cu")) return FHDX_ERR_ARG;
+    const TuneKey key{c->p.nspecies, c->p.n_cells[0], c->p.n_cells[1], c->p.n_cells[2]};
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            apply_tune(c->fused, it->second);
+            *best = it->second;
+            return FHDX_OK;
+        }
+    }
+    hipStream_t s = S(stream);
+    // scratch copies: the caller's state is not advanced
+    const size_t pts = (size_t)(cu.hi[0] - cu.lo[0] + 1) * (cu.hi[1] - cu.lo[1] + 1) * (cu.hi[2] - cu.lo[2] + 1);
+    double *scu = nullptr, *sprim = nullptr;
+    int* serr = nullptr;  // trial steps raise their flags here, not on the context
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int r = FHDX_OK;
+    auto fail = [&](int code, const char* what) {
+        if (!r) {
+            r = code;
+            set_error(std::string("fhdx_ctx_autotune: ") + what);
+        }
+    };
+    if (hipMalloc((void**)&scu, pts * c->dp.nvars * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&sprim, pts * c->dp.nprim * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&serr, sizeof(int)) != hipSuccess)
+        fail(FHDX_ERR_NOMEM, "scratch allocation");
+    if (!r && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) fail(FHDX_ERR_HIP, "events");
+    if (!r && hipMemcpyAsync(scu, cu.p, pts * c->dp.nvars * sizeof(double), hipMemcpyDeviceToDevice, s) != hipSuccess)
+        fail(FHDX_ERR_HIP, "state copy");
+    fhdx_fab fcu = cu, fprim = cu;
+    fcu.p = scu;
+    fprim.p = sprim;
+    fprim.ncomp = c->dp.nprim;
+    fhdx_tune top{};
+    top.ms_per_step = 1e30;
+    const bool timing_was_on = c->fused.timing.on;
+    c->fused.timing.on = false;
+    uint64_t step = 0;
+    for (int tw : {64, 32, 16}) {
+        if (tw > 16 && (tw >> 1) >= c->p.n_cells[0]) continue;  // same tiles as a narrower width
+        for (int rh : {1, 2, 4, 8}) {
+            for (int split : {0, 1}) {
+                if (r) break;
+                fhdx_tune t{tw, rh, split, 0, 0.};
+                apply_tune(c->fused, t);
+                float ms = 0.f;
+                int q = fused_rk3_step(c->p, c->dp, c->fused, fcu, fprim, dt, make_noise_key(0, step++, 0), nullptr,
+                                       nullptr, nullptr, nullptr, serr, s);
+                if (!q && hipEventRecord(e0, s) != hipSuccess) q = FHDX_ERR_HIP;
+                for (int k = 0; !q && k < trial_steps; ++k)
+                    q = fused_rk3_step(c->p, c->dp, c->fused, fcu, fprim, dt, make_noise_key(0, step++, 0), nullptr,
+                                       nullptr, nullptr, nullptr, serr, s);
+                if (!q && hipEventRecord(e1, s) != hipSuccess) q = FHDX_ERR_HIP;
+                if (!q && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                    q = FHDX_ERR_HIP;
+                if (q) {
+                    fail(q, "trial step failed");
+                    break;
+                }
+                t.ms_per_step = ms / trial_steps;
+                if (t.ms_per_step < top.ms_per_step) top = t;
+                if (trials && n_trials && *n_trials < max_trials) trials[(*n_trials)++] = t;
+            }
+        }
+    }
+    c->fused.timing.on = timing_was_on;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(s);
+    if (scu) (void)hipFree(scu);
+    if (sprim) (void)hipFree(sprim);
+    if (serr) (void)hipFree(serr);
+    if (r) return r;
+    apply_tune(c->fused, top);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune_cache[key] = top;
+    }
+    *best = top;
+    return FHDX_OK;
 }
 
 int fhdx_ctx_timing_enable(fhdx_ctx* c, int enable)

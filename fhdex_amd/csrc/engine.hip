@@ -31,8 +31,8 @@ __device__ inline NoiseKey inst_key(const Inst& in)
     return K;
 }
 
-__device__ inline const Arr& stage_in(const Inst& in, int st) { return st == 0 ? in.U : (st == 1 ? in.U1 : in.U2); }
-__device__ inline const Arr& stage_out(const Inst& in, int st) { return st == 0 ? in.U1 : (st == 1 ? in.U2 : in.U); }
+__device__ inline double* stage_in(const Inst& in, int st) { return st == 0 ? in.U : (st == 1 ? in.U1 : in.U2); }
+__device__ inline double* stage_out(const Inst& in, int st) { return st == 0 ? in.U1 : (st == 1 ? in.U2 : in.U); }
 
 #define EB_CELL_PROLOGUE                                       \
     const WorkItem wi = W[blockIdx.x];                         \
@@ -40,44 +40,51 @@ __device__ inline const Arr& stage_out(const Inst& in, int st) { return st == 0 
     const long t = (long)wi.chunk * EB + threadIdx.x;          \
     if (t >= in.ncell || in.step >= in.step_end) return;       \
     int i, j, k;                                               \
-    dec3(t, in.P.n[0], in.P.n[1], i, j, k);
-
-template <int NS>
-__global__ void __launch_bounds__(EB) kb_noise(const Inst* __restrict__ I, const WorkItem* __restrict__ W)
-{
-    EB_CELL_PROLOGUE
-    noise_body<NS + 5>(in.P, inst_key(in), in.A[0], in.A[1], in.A[2], in.B[0], in.B[1], in.B[2], in.rA, in.rB, i, j,
-                       k);
-}
+    dec3(t, in.P.n[0], in.P.n[1], i, j, k);                    \
+    const Grid G = in.G;
 
 template <int NS>
 __global__ void __launch_bounds__(EB) kb_cells(const Inst* __restrict__ I, const WorkItem* __restrict__ W, int st)
 {
     EB_CELL_PROLOGUE
-    cell_body<NS>(in.P, stage_in(in, st), in.R, in.err, i, j, k);
+    cell_body<NS>(in.P, G.view(stage_in(in, st)), G.view(in.R), in.err, i, j, k);
 }
 
 __global__ void __launch_bounds__(EB) kb_nodes(const Inst* __restrict__ I, const WorkItem* __restrict__ W, double w)
 {
     EB_CELL_PROLOGUE
-    node_body(in.P, in.R, in.N, in.rA, in.rB, w, i, j, k);
+    node_body<true>(in.P, G.view(in.R), G.view(in.N), G.view(in.R), G.view(in.R), inst_key(in), w, i, j, k);
 }
 
-template <int NS, int D>
-__global__ void __launch_bounds__(EB) kb_face(const Inst* __restrict__ I, const WorkItem* __restrict__ W, int st,
-                                              double w)
+template <int NS>
+__global__ void __launch_bounds__(EB) kb_faces(const Inst* __restrict__ I, const WorkItem* __restrict__ W, int st,
+                                               double w)
 {
+    constexpr int nv = NS + 5;
     EB_CELL_PROLOGUE
-    face_body<NS, D>(in.P, stage_in(in, st), in.R, in.N, in.A[D], in.B[D], in.F[D], w, in.volinv, in.dtinv, in.dt,
-                     i, j, k);
+    const Arr Us = G.view(stage_in(in, st)), R = G.view(in.R), N = G.view(in.N);
+    const NoiseKey K = inst_key(in);  // noise evaluated in place (no A/B arrays)
+    double f[nv];
+    face_flux<NS, 0, true>(in.P, Us, R, N, R, R, K, w, in.volinv, in.dtinv, in.dt, i, j, k, f);
+    const Arr F0 = G.view(in.F[0]);
+#pragma unroll
+    for (int c = 0; c < nv; ++c) F0(i, j, k, c) = f[c];
+    face_flux<NS, 1, true>(in.P, Us, R, N, R, R, K, w, in.volinv, in.dtinv, in.dt, i, j, k, f);
+    const Arr F1 = G.view(in.F[1]);
+#pragma unroll
+    for (int c = 0; c < nv; ++c) F1(i, j, k, c) = f[c];
+    face_flux<NS, 2, true>(in.P, Us, R, N, R, R, K, w, in.volinv, in.dtinv, in.dt, i, j, k, f);
+    const Arr F2 = G.view(in.F[2]);
+#pragma unroll
+    for (int c = 0; c < nv; ++c) F2(i, j, k, c) = f[c];
 }
 
 template <int NS>
 __global__ void __launch_bounds__(EB) kb_update(const Inst* __restrict__ I, const WorkItem* __restrict__ W, int st)
 {
     EB_CELL_PROLOGUE
-    update_body<NS>(in.P, stage_out(in, st), in.U, stage_in(in, st), in.F[0], in.F[1], in.F[2], in.dt, st, in.prim,
-                    st == 2, i, j, k);
+    update_body<NS>(in.P, G.view(stage_out(in, st)), G.view(in.U), G.view(stage_in(in, st)), G.view(in.F[0]),
+                    G.view(in.F[1]), G.view(in.F[2]), in.dt, st, G.view(in.prim), st == 2, i, j, k);
 }
 
 // which: 0 record, 1 the stage's output state (-1: the step's input state), 2 prim
@@ -89,10 +96,12 @@ __global__ void __launch_bounds__(EB) kb_ghosts(const Inst* __restrict__ I, cons
     const long t = (long)wi.chunk * EB + threadIdx.x;
     if (t >= in.nghost || in.step >= in.step_end) return;
     const RecLayout L{in.P.ns};
-    if (which == 0) ghost_body(in.R, L.ncomp(), in.P.ngc, in.P.n[0], in.P.n[1], in.P.n[2], t);
+    const Grid G = in.G;
+    if (which == 0) ghost_body(G.view(in.R), L.ncomp(), in.P.ngc, in.P.n[0], in.P.n[1], in.P.n[2], t);
     else if (which == 1)
-        ghost_body(st < 0 ? in.U : stage_out(in, st), in.P.nvars, in.P.ngc, in.P.n[0], in.P.n[1], in.P.n[2], t);
-    else ghost_body(in.prim, in.P.nprim, in.P.ngc, in.P.n[0], in.P.n[1], in.P.n[2], t);
+        ghost_body(G.view(st < 0 ? in.U : stage_out(in, st)), in.P.nvars, in.P.ngc, in.P.n[0], in.P.n[1], in.P.n[2],
+                   t);
+    else ghost_body(G.view(in.prim), in.P.nprim, in.P.ngc, in.P.n[0], in.P.n[1], in.P.n[2], t);
 }
 
 __global__ void kb_advance(Inst* I, const int* act, int n)
@@ -114,20 +123,19 @@ static long ghosts_of(const fhdx_params& p)
 }
 static long grown_of(const fhdx_params& p) { return cells_of(p) + ghosts_of(p); }
 
-uint64_t engine_bytes_needed(const fhdx_params& p)
+// Every array of an instance has the grown cell layout (one Grid per instance).
+static int instance_comps(const fhdx_params& p)
 {
     const int NS = p.nspecies, nv = 5 + NS, npv = 6 + 2 * NS;
     const RecLayout L{NS};
-    const uint64_t cell_comps = 3 * nv + npv + L.ncomp();
-    const uint64_t per_comps = NODE_NCOMP + 3 * nv + 6 * nv + 2;
-    return 8ull * (cell_comps * grown_of(p) + per_comps * cells_of(p));
+    return 3 * nv + npv + L.ncomp() + NODE_NCOMP + 3 * nv;
 }
 
-static int alloc_arr(EngineSlot& s, Arr& a, const fhdx_params& p, bool cell, int nc)
+uint64_t engine_bytes_needed(const fhdx_params& p) { return 8ull * instance_comps(p) * grown_of(p); }
+
+static int alloc_comps(EngineSlot& s, const Grid& G, double*& p, int nc)
 {
-    const int g = cell ? p.ngc : 0;
-    const int nx = p.n_cells[0] + 2 * g, ny = p.n_cells[1] + 2 * g, nz = p.n_cells[2] + 2 * g;
-    const size_t n = (size_t)nx * ny * nz * nc;
+    const size_t n = (size_t)G.sn * nc;
     void* ptr = nullptr;
     if (hipMalloc(&ptr, n * sizeof(double)) != hipSuccess) {
         set_error("hipMalloc of an ensemble instance failed");
@@ -135,11 +143,7 @@ static int alloc_arr(EngineSlot& s, Arr& a, const fhdx_params& p, bool cell, int
     }
     if (hipMemset(ptr, 0, n * sizeof(double)) != hipSuccess) return FHDX_ERR_HIP;
     s.blocks.push_back(ptr);
-    a.p = (double*)ptr;
-    a.lo0 = a.lo1 = a.lo2 = -g;
-    a.sy = nx;
-    a.sz = nx * ny;
-    a.sn = (long)nx * ny * nz;
+    p = (double*)ptr;
     return FHDX_OK;
 }
 
@@ -238,21 +242,17 @@ int engine_add(Engine* e, const fhdx_params& p, double dt, uint64_t seed, uint64
     s.p = p;
     Inst& d = s.desc;
     d.P = make_dev_params(p);
+    d.G = grid_of(d.P);
     const int NS = p.nspecies, nv = 5 + NS;
     const RecLayout L{NS};
-    if (!r) r = alloc_arr(s, d.U, p, true, nv);
-    if (!r) r = alloc_arr(s, d.U1, p, true, nv);
-    if (!r) r = alloc_arr(s, d.U2, p, true, nv);
-    if (!r) r = alloc_arr(s, d.prim, p, true, 6 + 2 * NS);
-    if (!r) r = alloc_arr(s, d.R, p, true, L.ncomp());
-    if (!r) r = alloc_arr(s, d.N, p, false, NODE_NCOMP);
-    for (int q = 0; q < 3 && !r; ++q) {
-        r = alloc_arr(s, d.F[q], p, false, nv);
-        if (!r) r = alloc_arr(s, d.A[q], p, false, nv);
-        if (!r) r = alloc_arr(s, d.B[q], p, false, nv);
-    }
-    if (!r) r = alloc_arr(s, d.rA, p, false, 1);
-    if (!r) r = alloc_arr(s, d.rB, p, false, 1);
+    const Grid& G = d.G;
+    if (!r) r = alloc_comps(s, G, d.U, nv);
+    if (!r) r = alloc_comps(s, G, d.U1, nv);
+    if (!r) r = alloc_comps(s, G, d.U2, nv);
+    if (!r) r = alloc_comps(s, G, d.prim, 6 + 2 * NS);
+    if (!r) r = alloc_comps(s, G, d.R, L.ncomp());
+    if (!r) r = alloc_comps(s, G, d.N, NODE_NCOMP);
+    for (int q = 0; q < 3 && !r; ++q) r = alloc_comps(s, G, d.F[q], nv);
     if (r) {
         free_slot(s);
         return r;
@@ -269,10 +269,9 @@ int engine_add(Engine* e, const fhdx_params& p, double dt, uint64_t seed, uint64
     s.bytes = need;
     e->used += need;
     if (cu_init) {
-        FHDX_HIP_TRY(
-            hipMemcpyAsync(d.U.p, cu_init, sizeof(double) * grown_of(p) * nv, hipMemcpyDefault, e->stream));
+        FHDX_HIP_TRY(hipMemcpyAsync(d.U, cu_init, sizeof(double) * grown_of(p) * nv, hipMemcpyDefault, e->stream));
     } else {
-        for (int c = 0; c < nv && !r; ++c) r = fill_value(d.U.p + c * d.U.sn, d.U.sn, uniform[c], e->stream);
+        for (int c = 0; c < nv && !r; ++c) r = fill_value(d.U + c * G.sn, G.sn, uniform[c], e->stream);
         if (r) return r;
     }
     FHDX_HIP_TRY(hipMemcpyAsync(e->d_inst + sid, &d, sizeof(Inst), hipMemcpyHostToDevice, e->stream));
@@ -335,15 +334,12 @@ static void enqueue_group(Engine* e, int g, const double ws[3])
     if (!nc) return;
     const Inst* I = e->d_inst;
     const WorkItem *Wc = e->d_cells[g], *Wg = e->d_ghosts[g];
-    kb_noise<NS><<<nc, EB, 0, s>>>(I, Wc);
     kb_ghosts<<<ng, EB, 0, s>>>(I, Wg, 1, -1);
     for (int st = 0; st < 3; ++st) {
         kb_cells<NS><<<nc, EB, 0, s>>>(I, Wc, st);
         kb_ghosts<<<ng, EB, 0, s>>>(I, Wg, 0, st);
         kb_nodes<<<nc, EB, 0, s>>>(I, Wc, ws[st]);
-        kb_face<NS, 0><<<nc, EB, 0, s>>>(I, Wc, st, ws[st]);
-        kb_face<NS, 1><<<nc, EB, 0, s>>>(I, Wc, st, ws[st]);
-        kb_face<NS, 2><<<nc, EB, 0, s>>>(I, Wc, st, ws[st]);
+        kb_faces<NS><<<nc, EB, 0, s>>>(I, Wc, st, ws[st]);
         kb_update<NS><<<nc, EB, 0, s>>>(I, Wc, st);
         kb_ghosts<<<ng, EB, 0, s>>>(I, Wg, 1, st);
     }
@@ -413,10 +409,10 @@ int engine_read(Engine* e, int slot, double* cu_out, double* prim_out)
     const size_t pts = (size_t)grown_of(s.p);
     FHDX_HIP_TRY(hipStreamSynchronize(e->stream));
     if (cu_out)
-        FHDX_HIP_TRY(hipMemcpy(cu_out, s.desc.U.p, sizeof(double) * pts * (5 + s.p.nspecies), hipMemcpyDefault));
+        FHDX_HIP_TRY(hipMemcpy(cu_out, s.desc.U, sizeof(double) * pts * (5 + s.p.nspecies), hipMemcpyDefault));
     if (prim_out)
         FHDX_HIP_TRY(
-            hipMemcpy(prim_out, s.desc.prim.p, sizeof(double) * pts * (6 + 2 * s.p.nspecies), hipMemcpyDefault));
+            hipMemcpy(prim_out, s.desc.prim, sizeof(double) * pts * (6 + 2 * s.p.nspecies), hipMemcpyDefault));
     return FHDX_OK;
 }
 
@@ -481,9 +477,9 @@ int fhdx_engine_state(fhdx_engine* h, int slot, fhdx_fab* cu, fhdx_fab* prim, ui
     const EngineSlot& s = h->e->slots[slot];
     const fhdx_params& p = s.p;
     const int g = p.ngc;
-    auto fill = [&](fhdx_fab* f, const Arr& a, int nc) {
+    auto fill = [&](fhdx_fab* f, double* a, int nc) {
         if (!f) return;
-        f->p = a.p;
+        f->p = a;
         for (int d = 0; d < 3; ++d) {
             f->lo[d] = -g;
             f->hi[d] = p.n_cells[d] - 1 + g;

@@ -10,15 +10,17 @@
 #include <vector>
 
 #include "fused.hpp"
+#include "stage_bodies.hpp"
 
 namespace fhdx {
 
-// Device descriptor of one instance.
+// Device descriptor of one instance.  Every array has the instance's grown cell layout G; the
+// noise is evaluated in place from (seed, step), so no noise arrays exist.
 struct Inst {
     DevParams P;
-    Arr U, U1, U2, prim, R, N;
-    Arr F[3], A[3], B[3];
-    Arr rA, rB;
+    Grid G;
+    double *U, *U1, *U2, *prim, *R, *N;
+    double* F[3];
     double dt, volinv, dtinv;
     uint64_t seed;
     uint64_t step;      // next step index (noise key); advanced on the device

@@ -29,7 +29,7 @@ struct RecLayout {
 constexpr int NODE_VC = 9, NODE_S = 10, NODE_NCOMP = 14;
 
 // Kernel groups the fast path can time with HIP events (fhdx_ctx_timing_*).
-enum KernelKind { KT_NOISE = 0, KT_CELLS, KT_NODES, KT_FACE_X, KT_FACE_Y, KT_FACE_Z, KT_UPDATE, KT_COUNT };
+enum KernelKind { KT_NOISE = 0, KT_CELLS, KT_NODES, KT_FLUX, KT_GHOSTS, KT_COUNT };
 extern const char* const kKernelKindNames[KT_COUNT];
 
 // When enabled, brackets each kernel group with events recorded on the launch stream; the
@@ -54,13 +54,22 @@ struct Timing {
     void destroy();
 };
 
-// Device workspace owned by a context for the fast path (allocated on first use).
+// Device workspace owned by a context for the fast path (allocated on first use).  Every
+// array has the layout of a cell FAB with ngc ghost layers (component-major); face and node
+// arrays hold the lower face / node of each valid cell there (face n and node n of a periodic
+// box are face 0 and node 0).
 struct FusedWorkspace {
     bool ready = false;
     std::vector<void*> blocks;
-    fhdx_fab cup, cup2, rec, node;  // node and F hold n^3 entries: index n is index 0
-    fhdx_fab F[3];
-    fhdx_fab sA[3], sB[3], rA, rB;  // Philox noise of the step
+    double *cup = nullptr, *cup2 = nullptr, *rec = nullptr, *node = nullptr;
+    double* F[3] = {nullptr, nullptr, nullptr};   // face fluxes (nvars) per direction
+    double* sA[3] = {nullptr, nullptr, nullptr};  // Philox (or copied caller) noise of the step
+    double* sB[3] = {nullptr, nullptr, nullptr};
+    double *rA = nullptr, *rB = nullptr;
+    // launch shape (fhdx_ctx_autotune picks it by timing the candidates on the device)
+    int tw_max = 64;           // tile width limit in x (16, 32, 64); tile height 256 / width
+    int rh = 4;                // tiles per region of the region-major block order
+    bool split_faces = false;  // one face direction per launch instead of k_faces
     Timing timing;
 };
 

@@ -1,16 +1,18 @@
 // fused_kernels.hip -- the fast RK3 step on gfx950.
 //
-// Per step: k_noise writes the A/B noise sets in one pass (or the caller's sets are used).
-// Per stage:
-//   k_cells   conservedToPrimitive + transport coefficients, once per valid cell, into a
-//             ghost-padded record (u, v, w, T, p, eta, kappa, zeta, Y, X, clamped Y, chi, D);
-//   k_nodes   corner velocity gradients, visccorn and the stage-weighted node noise;
-//   k_face<D> all four flux families of one face direction (stochastic, diffusive, corner,
-//             hyperbolic) accumulated in registers in the reference's per-component order,
-//             energy fold included, written once (nvars components);
-//   k_update  divergence + RK3 stage combination (+ primitive variables after stage 3).
-// Only ghost cells are touched by the periodic fills.  Every value is computed with the
-// reference's operation order, so the step is bit-identical to the reference-shaped path.
+// Noise: each face and node evaluates its Philox A/B values in place, per stage (no noise arrays);
+// caller-supplied noise sets are copied into the grid layout and read instead.  Per stage:
+//   k_cells    conservedToPrimitive + transport coefficients, once per valid cell, into a
+//              ghost-padded record (u, v, w, T, p, eta, kappa, zeta, Y, X, clamped Y, chi, D);
+//   k_nodes    corner velocity gradients, visccorn and the stage-weighted node noise;
+//   k_faces    all four flux families (stochastic, diffusive, corner, hyperbolic) of the three
+//              lower faces of each cell, accumulated in registers in the reference's
+//              per-component order with the energy fold;
+//   k_update   divergence + RK3 stage combination (+ primitive variables after stage 3).
+// Every array of the box shares one layout (Grid), so a kernel computes each cell offset once
+// and addresses all arrays as scalar base + 32-bit offset.  Only ghost cells are touched by the
+// periodic fills.  Every value is computed with the reference's operation order, so the step is
+// bit-identical to the reference-shaped path.
 #include "fused.hpp"
 #include "kernels.hpp"
 #include "stage_bodies.hpp"
@@ -29,65 +31,75 @@ __device__ inline void decode3(long t, int nx, int ny, int& i, int& j, int& k)
 
 // ---- periodic ghost fill: one thread per ghost cell --------------------------------------
 
-__global__ void __launch_bounds__(FB) k_fill_ghosts(Arr a, int nc, int g, int n0, int n1, int n2)
+__global__ void __launch_bounds__(FB) k_fill_ghosts(Grid G, double* a, int nc, int g, int n0, int n1, int n2)
 {
-    ghost_body(a, nc, g, n0, n1, n2, (long)blockIdx.x * FB + threadIdx.x);
+    ghost_body(G.view(a), nc, g, n0, n1, n2, (long)blockIdx.x * FB + threadIdx.x);
 }
 
-static int fill_ghosts(const DevParams& P, Arr a, int nc, hipStream_t s)
+static int fill_ghosts(const DevParams& P, const Grid& G, double* a, int nc, hipStream_t s)
 {
     const long G0 = P.n[0] + 2 * P.ngc, G1 = P.n[1] + 2 * P.ngc, g2 = 2L * P.ngc;
     const long n = g2 * G0 * G1 + g2 * G0 * P.n[2] + g2 * P.n[1] * P.n[2];
-    k_fill_ghosts<<<grid_for(n, FB), FB, 0, s>>>(a, nc, P.ngc, P.n[0], P.n[1], P.n[2]);
+    k_fill_ghosts<<<grid_for(n, FB), FB, 0, s>>>(G, a, nc, P.ngc, P.n[0], P.n[1], P.n[2]);
     FHDX_HIP_TRY(hipGetLastError());
     return FHDX_OK;
 }
 
-// ---- K0: the step's noise, one thread per cell (its three lower faces and its node) --------
+// ---- valid-region copy of caller noise (face / node FABs) into the grid layout -------------
 
-template <int NV>
-__global__ void __launch_bounds__(FB) k_noise(DevParams P, NoiseKey K, Arr A0, Arr A1, Arr A2, Arr B0, Arr B1,
-                                              Arr B2, Arr rA, Arr rB)
+__global__ void __launch_bounds__(FB) k_copy_valid(Arr src, Grid G, double* dst, int nc, int n0, int n1, int n2)
 {
     long t = (long)blockIdx.x * FB + threadIdx.x;
-    if (t >= (long)P.n[0] * P.n[1] * P.n[2]) return;
+    if (t >= (long)n0 * n1 * n2) return;
     int i, j, k;
-    decode3(t, P.n[0], P.n[1], i, j, k);
-    noise_body<NV>(P, K, A0, A1, A2, B0, B1, B2, rA, rB, i, j, k);
+    decode3(t, n0, n1, i, j, k);
+    const Arr d = G.view(dst);
+    for (int c = 0; c < nc; ++c) d(i, j, k, c) = src(i, j, k, c);
 }
 
 // ---- K1: cell records ------------------------------------------------------------------
 
 template <int NS>
-__global__ void __launch_bounds__(FB) k_cells(DevParams P, Arr U, Arr R, int* err)
+__global__ void __launch_bounds__(FB) k_cells(DevParams P, Grid G, double* U, double* R, int* err)
 {
     long t = (long)blockIdx.x * FB + threadIdx.x;
     if (t >= (long)P.n[0] * P.n[1] * P.n[2]) return;
     int i, j, k;
     decode3(t, P.n[0], P.n[1], i, j, k);
-    cell_body<NS>(P, U, R, err, i, j, k);
+    cell_body<NS>(P, G.view(U), G.view(R), err, i, j, k);
 }
 
-// ---- tiles: 2-D (x, y) blocks of 256 threads, one z level per block, XCD-aware order ----
+// ---- tiles: 2-D (x, y) blocks of 256 threads, one z level per block, region-major order ----
 //
-// Workgroup b runs on XCD b % 8.  Tiles are numbered y-tile fastest, then x-tile, then z, and
-// XCD x receives the contiguous range x*nb/8 .. (x+1)*nb/8, so the blocks an XCD holds at once
-// cover whole z planes of one slab and their x/y neighbours share that XCD's L2.
+// A tile is tw x th cells (tw = 1 << tx_shift = min(64, next power of two >= nx), th = 256/tw).
+// A region is a column of rh tiles stacked in y.  Block numbers run over the rh tiles of one z
+// level of a region, then up z, then to the next region.  Workgroup b runs on XCD b % 8, and XCD
+// x receives the contiguous number range x*nb/8 .. (x+1)*nb/8: the blocks an XCD holds at once
+// sit on neighbouring z levels of one region, so the z-1 / z+1 and y-1 neighbours a face or node
+// reads were fetched into that XCD's L2 a few blocks earlier.
 
 struct TileMap {
     int tx_shift;  // tile width 1 << tx_shift (x), height 256 >> tx_shift (y)
-    int ntx, nty;
+    int rh;        // tiles per region (y)
+    int nrx, nry;  // regions in x and y
+    int n2;
+    unsigned blocks() const { return (unsigned)((long)nrx * nry * n2 * rh); }
 };
 
-static TileMap make_tiles(int n0, int n1)
+static TileMap make_tiles(int n0, int n1, int n2, int rh, int tw_max)
 {
     TileMap m;
+    int smax = 6;
+    while (smax > 4 && (1 << smax) > tw_max) --smax;
     int s = 0;
-    while ((1 << s) < n0 && s < 6) ++s;  // width min(64, next power of two >= n0)
+    while ((1 << s) < n0 && s < smax) ++s;
     m.tx_shift = s;
     const int tx = 1 << s, ty = FB >> s;
-    m.ntx = (n0 + tx - 1) / tx;
-    m.nty = (n1 + ty - 1) / ty;
+    const int nty = (n1 + ty - 1) / ty;
+    m.rh = rh < 1 ? 1 : (rh > nty ? nty : rh);
+    m.nrx = (n0 + tx - 1) / tx;
+    m.nry = (nty + m.rh - 1) / m.rh;
+    m.n2 = n2;
     return m;
 }
 
@@ -96,45 +108,83 @@ __device__ inline bool tile_cell(const TileMap& m, int n0, int n1, int& i, int& 
     const int nb = gridDim.x, b = blockIdx.x;
     const int xcd = b & 7, r = b >> 3, per = nb >> 3, rem = nb & 7;
     const int t = xcd * per + (xcd < rem ? xcd : rem) + r;
-    const int ty = t % m.nty, rest = t / m.nty;
-    const int tx = rest % m.ntx;
-    k = rest / m.ntx;
-    i = (tx << m.tx_shift) + (threadIdx.x & ((1 << m.tx_shift) - 1));
-    j = ty * (FB >> m.tx_shift) + (threadIdx.x >> m.tx_shift);
+    const int col = m.n2 * m.rh;
+    const int region = t / col, q = t - region * col;
+    k = q / m.rh;
+    const int sub = q - k * m.rh;
+    const int rx = region % m.nrx, ry = region / m.nrx;
+    i = (rx << m.tx_shift) + (threadIdx.x & ((1 << m.tx_shift) - 1));
+    j = (ry * m.rh + sub) * (FB >> m.tx_shift) + (threadIdx.x >> m.tx_shift);
     return i < n0 && j < n1;
 }
 
 // ---- K2: nodes (flux.cpp:1627-1713) and the stochastic edge terms of each node -------------
 
-__global__ void __launch_bounds__(FB) k_nodes(DevParams P, TileMap tm, Arr R, Arr N, Arr rA, Arr rB, double w)
+template <bool INL>
+__global__ void __launch_bounds__(FB) k_nodes(DevParams P, TileMap tm, Grid G, double* R, double* N, double* rA,
+                                              double* rB, NoiseKey K, double w)
 {
     int i, j, k;
     if (!tile_cell(tm, P.n[0], P.n[1], i, j, k)) return;
-    node_body(P, R, N, rA, rB, w, i, j, k);
+    node_body<INL>(P, G.view(R), G.view(N), G.view(rA), G.view(rB), K, w, i, j, k);
 }
 
-// ---- K3: all fluxes of one face direction -------------------------------------------------
+// ---- K3: the three lower faces of each cell (flux.cpp:71-2372) -------------------------------
 
-template <int NS, int D>
-__global__ void __launch_bounds__(FB) k_face(DevParams P, TileMap tm, Arr Us, Arr R, Arr N, Arr A, Arr B, Arr Fo,
-                                             double w, double volinv, double dtinv, double dt)
+struct FaceArgs {
+    double *Us, *R, *N;
+    double *A[3], *B[3], *F[3];
+    NoiseKey K;
+    double w, volinv, dtinv, dt;
+};
+
+template <int NS, bool INL>
+__global__ void __launch_bounds__(FB) k_faces(DevParams P, TileMap tm, Grid G, FaceArgs a)
+{
+    constexpr int nv = NS + 5;
+    int i, j, k;
+    if (!tile_cell(tm, P.n[0], P.n[1], i, j, k)) return;
+    const Arr Us = G.view(a.Us), R = G.view(a.R), N = G.view(a.N);
+    double f[nv];
+    face_flux<NS, 0, INL>(P, Us, R, N, G.view(a.A[0]), G.view(a.B[0]), a.K, a.w, a.volinv, a.dtinv, a.dt, i,
+                          j, k, f);
+    const Arr F0 = G.view(a.F[0]);
+#pragma unroll
+    for (int c = 0; c < nv; ++c) F0(i, j, k, c) = f[c];
+    face_flux<NS, 1, INL>(P, Us, R, N, G.view(a.A[1]), G.view(a.B[1]), a.K, a.w, a.volinv, a.dtinv, a.dt, i,
+                          j, k, f);
+    const Arr F1 = G.view(a.F[1]);
+#pragma unroll
+    for (int c = 0; c < nv; ++c) F1(i, j, k, c) = f[c];
+    face_flux<NS, 2, INL>(P, Us, R, N, G.view(a.A[2]), G.view(a.B[2]), a.K, a.w, a.volinv, a.dtinv, a.dt, i,
+                          j, k, f);
+    const Arr F2 = G.view(a.F[2]);
+#pragma unroll
+    for (int c = 0; c < nv; ++c) F2(i, j, k, c) = f[c];
+}
+
+// One face direction per launch (the same bodies; kept as the tuner's alternative for shapes
+// where the three-face kernel's register footprint costs more than its shared loads save).
+template <int NS, int D, bool INL>
+__global__ void __launch_bounds__(FB) k_face(DevParams P, TileMap tm, Grid G, FaceArgs a)
 {
     int i, j, k;
     if (!tile_cell(tm, P.n[0], P.n[1], i, j, k)) return;
-    face_body<NS, D>(P, Us, R, N, A, B, Fo, w, volinv, dtinv, dt, i, j, k);
+    face_body<NS, D, INL>(P, G.view(a.Us), G.view(a.R), G.view(a.N), G.view(a.A[D]), G.view(a.B[D]), a.K,
+                          G.view(a.F[D]), a.w, a.volinv, a.dtinv, a.dt, i, j, k);
 }
 
 // ---- K4: stage combination (+ primitive variables after the last stage) --------------------
 
 template <int NS>
-__global__ void __launch_bounds__(FB) k_update(DevParams P, Arr out, Arr u0, Arr us, Arr fx, Arr fy, Arr fz,
-                                               double dt, int mode, Arr prim, int write_prim)
+__global__ void __launch_bounds__(FB) k_update(DevParams P, TileMap tm, Grid G, double* out, double* u0,
+                                               double* us, double* fx, double* fy, double* fz, double dt, int mode,
+                                               double* prim, int write_prim)
 {
-    long t = (long)blockIdx.x * FB + threadIdx.x;
-    if (t >= (long)P.n[0] * P.n[1] * P.n[2]) return;
     int i, j, k;
-    decode3(t, P.n[0], P.n[1], i, j, k);
-    update_body<NS>(P, out, u0, us, fx, fy, fz, dt, mode, prim, write_prim, i, j, k);
+    if (!tile_cell(tm, P.n[0], P.n[1], i, j, k)) return;
+    update_body<NS>(P, G.view(out), G.view(u0), G.view(us), G.view(fx), G.view(fy), G.view(fz), dt, mode,
+                    G.view(prim), write_prim, i, j, k);
 }
 
 // ---- host side ---------------------------------------------------------------------------
@@ -147,86 +197,96 @@ void fused_free(FusedWorkspace& w)
     w.ready = false;
 }
 
-static int alloc(FusedWorkspace& w, fhdx_fab& f, int l0, int l1, int l2, int h0, int h1, int h2, int nc)
+static int alloc_grid(FusedWorkspace& w, const Grid& G, double*& p, int nc)
 {
-    f.lo[0] = l0; f.lo[1] = l1; f.lo[2] = l2;
-    f.hi[0] = h0; f.hi[1] = h1; f.hi[2] = h2;
-    f.ncomp = nc;
-    f.pad = 0;
-    size_t n = (size_t)(h0 - l0 + 1) * (h1 - l1 + 1) * (h2 - l2 + 1) * nc;
-    void* p = nullptr;
-    if (hipMalloc(&p, n * sizeof(double)) != hipSuccess) {
+    void* q = nullptr;
+    const size_t n = (size_t)G.sn * nc;
+    if (hipMalloc(&q, n * sizeof(double)) != hipSuccess) {
         set_error("hipMalloc of fast-path workspace failed");
         return FHDX_ERR_NOMEM;
     }
-    (void)hipMemset(p, 0, n * sizeof(double));
-    f.p = (double*)p;
-    w.blocks.push_back(p);
+    (void)hipMemset(q, 0, n * sizeof(double));
+    p = (double*)q;
+    w.blocks.push_back(q);
     return FHDX_OK;
 }
 
 static int ensure(const DevParams& P, FusedWorkspace& w)
 {
     if (w.ready) return FHDX_OK;
-    const int g = P.ngc, nv = P.nvars;
-    const int n0 = P.n[0], n1 = P.n[1], n2 = P.n[2];
+    const Grid G = grid_of(P);
+    const int nv = P.nvars;
     const RecLayout L{P.ns};
     int r = 0;
-    auto cell = [&](fhdx_fab& f, int nc) { if (!r) r = alloc(w, f, -g, -g, -g, n0 - 1 + g, n1 - 1 + g, n2 - 1 + g, nc); };
-    auto face = [&](fhdx_fab& f, int d, int nc) {
-        if (!r) r = alloc(w, f, 0, 0, 0, n0 - 1 + (d == 0), n1 - 1 + (d == 1), n2 - 1 + (d == 2), nc);
-    };
-    auto node = [&](fhdx_fab& f, int nc) { if (!r) r = alloc(w, f, 0, 0, 0, n0, n1, n2, nc); };
-    auto per = [&](fhdx_fab& f, int nc) { if (!r) r = alloc(w, f, 0, 0, 0, n0 - 1, n1 - 1, n2 - 1, nc); };
-    cell(w.cup, nv);
-    cell(w.cup2, nv);
-    cell(w.rec, L.ncomp());
-    per(w.node, NODE_NCOMP);
-    for (int d = 0; d < 3; ++d) {
-        per(w.F[d], nv);
-        face(w.sA[d], d, nv);
-        face(w.sB[d], d, nv);
-    }
-    node(w.rA, 1);
-    node(w.rB, 1);
+    auto a = [&](double*& p, int nc) { if (!r) r = alloc_grid(w, G, p, nc); };
+    a(w.cup, nv);
+    a(w.cup2, nv);
+    a(w.rec, L.ncomp());
+    a(w.node, NODE_NCOMP);
+    for (int d = 0; d < 3; ++d) a(w.F[d], nv);
     if (r) return r;
     FHDX_HIP_TRY(hipDeviceSynchronize());
     w.ready = true;
     return FHDX_OK;
 }
 
-template <int NS>
-static void launch_stage(const DevParams& P, FusedWorkspace& w, Arr cu, Arr us, Arr out, Arr prim, const Arr A[3],
-                         const Arr B[3], Arr rA, Arr rB, double wst, double dt, int st, int* d_err, hipStream_t s)
+// Arrays for caller-supplied noise (the Philox path evaluates its noise in place and needs none).
+static int ensure_noise_arrays(const DevParams& P, FusedWorkspace& w)
+{
+    if (w.sA[0]) return FHDX_OK;
+    const Grid G = grid_of(P);
+    int r = 0;
+    for (int d = 0; d < 3 && !r; ++d) {
+        r = alloc_grid(w, G, w.sA[d], P.nvars);
+        if (!r) r = alloc_grid(w, G, w.sB[d], P.nvars);
+    }
+    if (!r) r = alloc_grid(w, G, w.rA, 1);
+    if (!r) r = alloc_grid(w, G, w.rB, 1);
+    return r;
+}
+
+template <int NS, bool INL>
+static void launch_stage(const DevParams& P, FusedWorkspace& w, const Grid& G, double* cu, double* us, double* out,
+                         double* prim, const NoiseKey& K, double wst, double dt, int st, int* d_err, hipStream_t s)
 {
     const long ncell = (long)P.n[0] * P.n[1] * P.n[2];
     const RecLayout L{NS};
-    const TileMap tm = make_tiles(P.n[0], P.n[1]);
-    const unsigned ntile = (unsigned)(tm.ntx * tm.nty * P.n[2]);
-    Arr R = make_arr(w.rec), N = make_arr(w.node);
-    Arr F0 = make_arr(w.F[0]), F1 = make_arr(w.F[1]), F2 = make_arr(w.F[2]);
+    const TileMap tm = make_tiles(P.n[0], P.n[1], P.n[2], w.rh, w.tw_max);
+    const unsigned ntile = tm.blocks();
     Timing& T = w.timing;
     T.begin(KT_CELLS, s);
-    k_cells<NS><<<grid_for(ncell, FB), FB, 0, s>>>(P, us, R, d_err);
-    fill_ghosts(P, R, L.ncomp(), s);
+    k_cells<NS><<<grid_for(ncell, FB), FB, 0, s>>>(P, G, us, w.rec, d_err);
+    fill_ghosts(P, G, w.rec, L.ncomp(), s);
     T.end(s);
     T.begin(KT_NODES, s);
-    k_nodes<<<ntile, FB, 0, s>>>(P, tm, R, N, rA, rB, wst);
+    k_nodes<INL><<<ntile, FB, 0, s>>>(P, tm, G, w.rec, w.node, w.rA, w.rB, K, wst);
     T.end(s);
-    const double volinv = 1. / (P.dx[0] * P.dx[1] * P.dx[2]);
-    const double dtinv = 1. / dt;
-    T.begin(KT_FACE_X, s);
-    k_face<NS, 0><<<ntile, FB, 0, s>>>(P, tm, us, R, N, A[0], B[0], F0, wst, volinv, dtinv, dt);
+    FaceArgs fa;
+    fa.Us = us;
+    fa.R = w.rec;
+    fa.N = w.node;
+    for (int d = 0; d < 3; ++d) {
+        fa.A[d] = w.sA[d];
+        fa.B[d] = w.sB[d];
+        fa.F[d] = w.F[d];
+    }
+    fa.K = K;
+    fa.w = wst;
+    fa.volinv = 1. / (P.dx[0] * P.dx[1] * P.dx[2]);
+    fa.dtinv = 1. / dt;
+    fa.dt = dt;
+    T.begin(KT_FLUX, s);
+    if (w.split_faces) {
+        k_face<NS, 0, INL><<<ntile, FB, 0, s>>>(P, tm, G, fa);
+        k_face<NS, 1, INL><<<ntile, FB, 0, s>>>(P, tm, G, fa);
+        k_face<NS, 2, INL><<<ntile, FB, 0, s>>>(P, tm, G, fa);
+    } else {
+        k_faces<NS, INL><<<ntile, FB, 0, s>>>(P, tm, G, fa);
+    }
+    k_update<NS><<<ntile, FB, 0, s>>>(P, tm, G, out, cu, us, w.F[0], w.F[1], w.F[2], dt, st, prim, st == 2);
     T.end(s);
-    T.begin(KT_FACE_Y, s);
-    k_face<NS, 1><<<ntile, FB, 0, s>>>(P, tm, us, R, N, A[1], B[1], F1, wst, volinv, dtinv, dt);
-    T.end(s);
-    T.begin(KT_FACE_Z, s);
-    k_face<NS, 2><<<ntile, FB, 0, s>>>(P, tm, us, R, N, A[2], B[2], F2, wst, volinv, dtinv, dt);
-    T.end(s);
-    T.begin(KT_UPDATE, s);
-    k_update<NS><<<grid_for(ncell, FB), FB, 0, s>>>(P, out, cu, us, F0, F1, F2, dt, st, prim, st == 2);
-    fill_ghosts(P, out, P.nvars, s);
+    T.begin(KT_GHOSTS, s);
+    fill_ghosts(P, G, out, P.nvars, s);
     T.end(s);
 }
 
@@ -237,52 +297,50 @@ int fused_rk3_step(const fhdx_params& p, const DevParams& P, FusedWorkspace& w, 
     (void)p;
     int r = ensure(P, w);
     if (r) return r;
-    Arr A[3], B[3], RA, RB;
-    if (sA) {
-        for (int d = 0; d < 3; ++d) {
-            A[d] = make_arr(sA[d]);
-            B[d] = make_arr(sB[d]);
-        }
-        RA = make_arr(*rA);
-        RB = make_arr(*rB);
-    } else {
-        for (int d = 0; d < 3; ++d) {
-            A[d] = make_arr(w.sA[d]);
-            B[d] = make_arr(w.sB[d]);
-        }
-        RA = make_arr(w.rA);
-        RB = make_arr(w.rB);
-        const long ncell = (long)P.n[0] * P.n[1] * P.n[2];
+    const Grid G = grid_of(P);
+    const long ncell = (long)P.n[0] * P.n[1] * P.n[2];
+    const bool inl = (sA == nullptr);
+    if (!inl) {
+        // caller noise: face / node FABs, valid entries copied into the grid layout
+        if ((r = ensure_noise_arrays(P, w))) return r;
         w.timing.begin(KT_NOISE, s);
-        switch (P.ns) {
-            case 2: k_noise<7><<<grid_for(ncell, FB), FB, 0, s>>>(P, K, A[0], A[1], A[2], B[0], B[1], B[2], RA, RB); break;
-            case 3: k_noise<8><<<grid_for(ncell, FB), FB, 0, s>>>(P, K, A[0], A[1], A[2], B[0], B[1], B[2], RA, RB); break;
-            case 4: k_noise<9><<<grid_for(ncell, FB), FB, 0, s>>>(P, K, A[0], A[1], A[2], B[0], B[1], B[2], RA, RB); break;
-            default: return FHDX_ERR_UNSUPPORTED;
+        for (int d = 0; d < 3; ++d) {
+            k_copy_valid<<<grid_for(ncell, FB), FB, 0, s>>>(make_arr(sA[d]), G, w.sA[d], P.nvars, P.n[0], P.n[1],
+                                                            P.n[2]);
+            k_copy_valid<<<grid_for(ncell, FB), FB, 0, s>>>(make_arr(sB[d]), G, w.sB[d], P.nvars, P.n[0], P.n[1],
+                                                            P.n[2]);
         }
+        k_copy_valid<<<grid_for(ncell, FB), FB, 0, s>>>(make_arr(*rA), G, w.rA, 1, P.n[0], P.n[1], P.n[2]);
+        k_copy_valid<<<grid_for(ncell, FB), FB, 0, s>>>(make_arr(*rB), G, w.rB, 1, P.n[0], P.n[1], P.n[2]);
         w.timing.end(s);
         FHDX_HIP_TRY(hipGetLastError());
     }
     double ws[3];
     stage_weights(ws);
-    Arr U = make_arr(cu), U1 = make_arr(w.cup), U2 = make_arr(w.cup2), PR = make_arr(prim);
-    w.timing.begin(KT_UPDATE, s);
-    r = fill_ghosts(P, U, P.nvars, s);
+    double* U = cu.p;
+    w.timing.begin(KT_GHOSTS, s);
+    r = fill_ghosts(P, G, U, P.nvars, s);
     w.timing.end(s);
     if (r) return r;
-    Arr in[3] = {U, U1, U2};
-    Arr out[3] = {U1, U2, U};
+    double* in[3] = {U, w.cup, w.cup2};
+    double* out[3] = {w.cup, w.cup2, U};
+#define FHDX_STAGE(NSV, INLV) \
+    launch_stage<NSV, INLV>(P, w, G, U, in[st], out[st], prim.p, K, ws[st], dt, st, d_err, s)
     for (int st = 0; st < 3; ++st) {
-        switch (P.ns) {
-            case 2: launch_stage<2>(P, w, U, in[st], out[st], PR, A, B, RA, RB, ws[st], dt, st, d_err, s); break;
-            case 3: launch_stage<3>(P, w, U, in[st], out[st], PR, A, B, RA, RB, ws[st], dt, st, d_err, s); break;
-            case 4: launch_stage<4>(P, w, U, in[st], out[st], PR, A, B, RA, RB, ws[st], dt, st, d_err, s); break;
+        switch (P.ns * 2 + (inl ? 1 : 0)) {
+            case 4: FHDX_STAGE(2, false); break;
+            case 5: FHDX_STAGE(2, true); break;
+            case 6: FHDX_STAGE(3, false); break;
+            case 7: FHDX_STAGE(3, true); break;
+            case 8: FHDX_STAGE(4, false); break;
+            case 9: FHDX_STAGE(4, true); break;
             default: return FHDX_ERR_UNSUPPORTED;
         }
         FHDX_HIP_TRY(hipGetLastError());
     }
-    w.timing.begin(KT_UPDATE, s);
-    r = fill_ghosts(P, PR, P.nprim, s);
+#undef FHDX_STAGE
+    w.timing.begin(KT_GHOSTS, s);
+    r = fill_ghosts(P, G, prim.p, P.nprim, s);
     w.timing.end(s);
     return r;
 }
@@ -293,8 +351,7 @@ int fused_rk3_step(const fhdx_params& p, const DevParams& P, FusedWorkspace& w, 
 
 namespace fhdx {
 
-const char* const kKernelKindNames[KT_COUNT] = {"noise", "cells", "nodes", "face_x", "face_y", "face_z",
-                                                "update"};
+const char* const kKernelKindNames[KT_COUNT] = {"noise", "cells", "nodes", "flux_update", "ghosts"};
 
 hipEvent_t Timing::get()
 {

@@ -27,4 +27,12 @@ int ref_lincomb(double* dst, const double* a, const double* b, long npts, int nc
 int ref_fill_noise(const DevParams& P, const NoiseKey& K, const Arr A[3], const Arr B[3], Arr rA, Arr rB,
                    hipStream_t s);
 
+// evaluateStats passes (stats.hip).  yz: n0*19 doubles, part: nparts*n0*19, misc: 20 (device).
+struct StatsArgs {
+    Arr cu, cm, cv, prim, pm, pv, sc, ms;
+    double *yz, *part, *misc;
+    int nparts, steps, cross_cell;
+};
+int stats_launch(const DevParams& P, const StatsArgs& a, hipStream_t s);
+
 }  // namespace fhdx

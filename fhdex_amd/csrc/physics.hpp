@@ -35,9 +35,14 @@ struct Arr {
     int lo0, lo1, lo2;
     int sy, sz;
     long sn;
+    // Component base (wave-uniform) plus a 32-bit byte offset within the component: device
+    // loads and stores then take the scalar-base + 32-bit vector-offset form instead of a
+    // 64-bit vector address per access.  A component of a FAB must stay below 4 GiB (the
+    // C ABI checks it).
     __host__ __device__ inline double& operator()(int i, int j, int k, int c = 0) const
     {
-        return p[(long)((i - lo0) + (j - lo1) * sy + (k - lo2) * sz) + (long)c * sn];
+        const unsigned off = (unsigned)((i - lo0) + (j - lo1) * sy + (k - lo2) * sz);
+        return *(double*)((char*)(p + (long)c * sn) + (off << 3));
     }
 };
 

@@ -11,6 +11,36 @@ namespace fhdx {
 
 __device__ inline int wrap_up(int i, int n) { return i + 1 == n ? 0 : i + 1; }
 
+// One layout for every array of a box (a cell FAB with ngc ghost layers, component-major; face
+// and node arrays use it too, their entry (i,j,k) being the lower face / node of cell (i,j,k)).
+// Views made from one Grid share their index arithmetic, so a kernel computes each cell offset
+// once for all arrays.
+struct Grid {
+    int lo, sy, sz;
+    long sn;
+    __host__ __device__ Arr view(double* p) const
+    {
+        Arr a;
+        a.p = p;
+        a.lo0 = a.lo1 = a.lo2 = lo;
+        a.sy = sy;
+        a.sz = sz;
+        a.sn = sn;
+        return a;
+    }
+};
+
+inline Grid grid_of(const DevParams& P)
+{
+    const int g = P.ngc;
+    Grid G;
+    G.lo = -g;
+    G.sy = P.n[0] + 2 * g;
+    G.sz = G.sy * (P.n[1] + 2 * g);
+    G.sn = (long)G.sz * (P.n[2] + 2 * g);
+    return G;
+}
+
 // ---- periodic ghost fill: thread t of the G^3 - n^3 ghost cells ---------------------------
 
 __device__ inline void ghost_body(const Arr& a, int nc, int g, int n0, int n1, int n2, long t)
@@ -88,84 +118,133 @@ __device__ inline void cell_body(const DevParams& P, const Arr& U, const Arr& R,
 
 // Node-relative summation orders of the edge viscosities: cell (a,b,c) is node - 1 + (a,b,c);
 // the fourth entry shifts the eta cell in z (order 2 reads eta(k-2)*T(k), flux.cpp:857-858).
-__constant__ signed char c_node_order[4][8][4] = {
+struct NodeOrder {
+    signed char o[4][8][4];
+};
+constexpr NodeOrder kNodeOrder = {{
     {{1,0,0,0},{0,0,0,0},{1,1,0,0},{0,1,0,0},{1,0,1,0},{0,0,1,0},{1,1,1,0},{0,1,1,0}},
     {{0,1,0,0},{1,1,0,0},{0,0,0,0},{1,0,0,0},{0,1,1,0},{1,1,1,0},{0,0,1,0},{1,0,1,0}},
     {{1,1,1,0},{0,1,1,-2},{1,0,1,0},{0,0,1,-2},{1,1,0,0},{0,1,0,0},{1,0,0,0},{0,0,0,0}},
-    {{0,0,1,0},{1,0,1,0},{0,1,1,0},{1,1,1,0},{0,0,0,0},{1,0,0,0},{0,1,0,0},{1,1,0,0}}};
+    {{0,0,1,0},{1,0,1,0},{0,1,1,0},{1,1,1,0},{0,0,0,0},{1,0,0,0},{0,1,0,0},{1,1,0,0}}}};
 
+// Cell (a,b,c) of the 2x2x2 block around node (i,j,k) is (i-1+a, j-1+b, k-1+c).
+__device__ constexpr int nc8(int a, int b, int c) { return a + 2 * b + 4 * c; }
+
+// All loads come first (into registers), all stores last: the record and node arrays are
+// distinct, and keeping stores behind the loads lets the compiler load each value once.
+// INL: the node noise is evaluated in place from Philox key K instead of read from rA, rB.
+template <bool INL>
 __device__ inline void node_body(const DevParams& P, const Arr& R, const Arr& N, const Arr& rA, const Arr& rB,
-                                 double w, int i, int j, int k)
+                                 const NoiseKey& K, double w, int i, int j, int k)
 {
-    const int E = RecLayout::ETA, Z = RecLayout::ZETA;
+    const int E = RecLayout::ETA, Z = RecLayout::ZETA, RT = RecLayout::T;
     const bool visc3 = (P.visc_type == 3 || P.visc_type == -3);
-    double muxp = 0.125 * (R(i, j - 1, k - 1, E) + R(i - 1, j - 1, k - 1, E) + R(i, j, k - 1, E) +
-                           R(i - 1, j, k - 1, E) + R(i, j - 1, k, E) + R(i - 1, j - 1, k, E) + R(i, j, k, E) +
-                           R(i - 1, j, k, E));
-    double zetaxp = 0.;
-    if (visc3)
-        zetaxp = 0.125 * (R(i, j - 1, k - 1, Z) + R(i - 1, j - 1, k - 1, Z) + R(i, j, k - 1, Z) +
-                          R(i - 1, j, k - 1, Z) + R(i, j - 1, k, Z) + R(i - 1, j - 1, k, Z) + R(i, j, k, Z) +
-                          R(i - 1, j, k, Z));
-    const double DX0 = P.dx[0], DX1 = P.dx[1], DX2 = P.dx[2];
+    double u[8], v[8], wv[8], e[8], T[8], ek2[4];
 #pragma unroll
-    for (int b = 0; b < 3; ++b) {
-        const int n = RecLayout::U + b;  // u, v, w
-        N(i, j, k, 0 + b) = 0.25 * muxp *
-                            (R(i, j - 1, k - 1, n) - R(i - 1, j - 1, k - 1, n) + R(i, j, k - 1, n) -
-                             R(i - 1, j, k - 1, n) + R(i, j - 1, k, n) - R(i - 1, j - 1, k, n) + R(i, j, k, n) -
-                             R(i - 1, j, k, n)) /
-                            DX0;
-        N(i, j, k, 3 + b) = 0.25 * muxp *
-                            (R(i - 1, j, k - 1, n) - R(i - 1, j - 1, k - 1, n) + R(i, j, k - 1, n) -
-                             R(i, j - 1, k - 1, n) + R(i - 1, j, k, n) - R(i - 1, j - 1, k, n) + R(i, j, k, n) -
-                             R(i, j - 1, k, n)) /
-                            DX1;
-        N(i, j, k, 6 + b) = 0.25 * muxp *
-                            (R(i - 1, j - 1, k, n) - R(i - 1, j - 1, k - 1, n) + R(i, j - 1, k, n) -
-                             R(i, j - 1, k - 1, n) + R(i - 1, j, k, n) - R(i - 1, j, k - 1, n) + R(i, j, k, n) -
-                             R(i, j, k - 1, n)) /
-                            DX2;
-    }
-    const int u = RecLayout::U, v = RecLayout::V, ww = RecLayout::W;
-    N(i, j, k, NODE_VC) =
-        (muxp / 12. + zetaxp / 4.) *
-        ((R(i, j - 1, k - 1, u) - R(i - 1, j - 1, k - 1, u)) / DX0 +
-         (R(i, j, k - 1, u) - R(i - 1, j, k - 1, u)) / DX0 + (R(i, j - 1, k, u) - R(i - 1, j - 1, k, u)) / DX0 +
-         (R(i, j, k, u) - R(i - 1, j, k, u)) / DX0 + (R(i - 1, j, k - 1, v) - R(i - 1, j - 1, k - 1, v)) / DX1 +
-         (R(i, j, k - 1, v) - R(i, j - 1, k - 1, v)) / DX1 + (R(i - 1, j, k, v) - R(i - 1, j - 1, k, v)) / DX1 +
-         (R(i, j, k, v) - R(i, j - 1, k, v)) / DX1 + (R(i - 1, j - 1, k, ww) - R(i - 1, j - 1, k - 1, ww)) / DX2 +
-         (R(i, j - 1, k, ww) - R(i, j - 1, k - 1, ww)) / DX2 + (R(i - 1, j, k, ww) - R(i - 1, j, k - 1, ww)) / DX2 +
-         (R(i, j, k, ww) - R(i, j, k - 1, ww)) / DX2);
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                const int q = nc8(a, b, c);
+                u[q] = R(i - 1 + a, j - 1 + b, k - 1 + c, RecLayout::U);
+                v[q] = R(i - 1 + a, j - 1 + b, k - 1 + c, RecLayout::V);
+                wv[q] = R(i - 1 + a, j - 1 + b, k - 1 + c, RecLayout::W);
+                e[q] = R(i - 1 + a, j - 1 + b, k - 1 + c, E);
+                T[q] = R(i - 1 + a, j - 1 + b, k - 1 + c, RT);
+            }
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int a = 0; a < 2; ++a) ek2[a + 2 * b] = R(i - 1 + a, j - 1 + b, k - 2, E);  // eta(k-2), order 2
+    double rAv = 0., rBv = 0.;
     if (P.stoch_on) {
-        // stage-weighted node noise (MultiFab::LinComb, timeStep.cpp:125-128)
-        const double W = 1.0 * rA(i, j, k) + w * rB(i, j, k);
-        const int RT = RecLayout::T;
-#pragma unroll
-        for (int v2 = 0; v2 < 4; ++v2) {
-            double s = 0.;
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const signed char* o = c_node_order[v2][q];
-                const int a = i - 1 + o[0], bb = j - 1 + o[1], c = k - 1 + o[2];
-                double term = R(a, bb, c + o[3], E) * R(a, bb, c, RT);
-                s = (q == 0) ? term : s + term;
-            }
-            double mu = 0.25 * s / 3.;
-            if (visc3) {
-                double z = 0.;
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const signed char* o = c_node_order[v2][q];
-                    const int a = i - 1 + o[0], bb = j - 1 + o[1], c = k - 1 + o[2];
-                    double term = R(a, bb, c + o[3], Z) * R(a, bb, c, RT);
-                    z = (q == 0) ? term : z + term;
-                }
-                mu += 0.25 * z;
-            }
-            N(i, j, k, NODE_S + v2) = sqrt(mu) * W;
+        if (INL) {
+            double za, zb;
+            noise_ab(K, NODE_FIELD, (uint32_t)(((long)k * P.n[1] + j) * P.n[0] + i), za, zb);
+            rAv = za * P.var_mom;
+            rBv = zb * P.var_mom;
+        } else {
+            rAv = rA(i, j, k);
+            rBv = rB(i, j, k);
         }
     }
+
+    // flux.cpp:1627-1703 (summation orders of the reference kept term by term)
+    double muxp = 0.125 * (e[nc8(1, 0, 0)] + e[nc8(0, 0, 0)] + e[nc8(1, 1, 0)] + e[nc8(0, 1, 0)] +
+                           e[nc8(1, 0, 1)] + e[nc8(0, 0, 1)] + e[nc8(1, 1, 1)] + e[nc8(0, 1, 1)]);
+    double zetaxp = 0.;
+    double z[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
+    if (visc3) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int a = 0; a < 2; ++a) z[nc8(a, b, c)] = R(i - 1 + a, j - 1 + b, k - 1 + c, Z);
+        zetaxp = 0.125 * (z[nc8(1, 0, 0)] + z[nc8(0, 0, 0)] + z[nc8(1, 1, 0)] + z[nc8(0, 1, 0)] +
+                          z[nc8(1, 0, 1)] + z[nc8(0, 0, 1)] + z[nc8(1, 1, 1)] + z[nc8(0, 1, 1)]);
+    }
+    const double DX0 = P.dx[0], DX1 = P.dx[1], DX2 = P.dx[2];
+    double out[NODE_NCOMP];
+#pragma unroll
+    for (int bcomp = 0; bcomp < 3; ++bcomp) {
+        const double* f = bcomp == 0 ? u : (bcomp == 1 ? v : wv);
+        out[0 + bcomp] = 0.25 * muxp *
+                         (f[nc8(1, 0, 0)] - f[nc8(0, 0, 0)] + f[nc8(1, 1, 0)] - f[nc8(0, 1, 0)] + f[nc8(1, 0, 1)] -
+                          f[nc8(0, 0, 1)] + f[nc8(1, 1, 1)] - f[nc8(0, 1, 1)]) /
+                         DX0;
+        out[3 + bcomp] = 0.25 * muxp *
+                         (f[nc8(0, 1, 0)] - f[nc8(0, 0, 0)] + f[nc8(1, 1, 0)] - f[nc8(1, 0, 0)] + f[nc8(0, 1, 1)] -
+                          f[nc8(0, 0, 1)] + f[nc8(1, 1, 1)] - f[nc8(1, 0, 1)]) /
+                         DX1;
+        out[6 + bcomp] = 0.25 * muxp *
+                         (f[nc8(0, 0, 1)] - f[nc8(0, 0, 0)] + f[nc8(1, 0, 1)] - f[nc8(1, 0, 0)] + f[nc8(0, 1, 1)] -
+                          f[nc8(0, 1, 0)] + f[nc8(1, 1, 1)] - f[nc8(1, 1, 0)]) /
+                         DX2;
+    }
+    // flux.cpp:1705-1711
+    out[NODE_VC] =
+        (muxp / 12. + zetaxp / 4.) *
+        ((u[nc8(1, 0, 0)] - u[nc8(0, 0, 0)]) / DX0 + (u[nc8(1, 1, 0)] - u[nc8(0, 1, 0)]) / DX0 +
+         (u[nc8(1, 0, 1)] - u[nc8(0, 0, 1)]) / DX0 + (u[nc8(1, 1, 1)] - u[nc8(0, 1, 1)]) / DX0 +
+         (v[nc8(0, 1, 0)] - v[nc8(0, 0, 0)]) / DX1 + (v[nc8(1, 1, 0)] - v[nc8(1, 0, 0)]) / DX1 +
+         (v[nc8(0, 1, 1)] - v[nc8(0, 0, 1)]) / DX1 + (v[nc8(1, 1, 1)] - v[nc8(1, 0, 1)]) / DX1 +
+         (wv[nc8(0, 0, 1)] - wv[nc8(0, 0, 0)]) / DX2 + (wv[nc8(1, 0, 1)] - wv[nc8(1, 0, 0)]) / DX2 +
+         (wv[nc8(0, 1, 1)] - wv[nc8(0, 1, 0)]) / DX2 + (wv[nc8(1, 1, 1)] - wv[nc8(1, 1, 0)]) / DX2);
+    if (P.stoch_on) {
+        // stage-weighted node noise (MultiFab::LinComb, timeStep.cpp:125-128)
+        const double W = 1.0 * rAv + w * rBv;
+#pragma unroll
+        for (int v2 = 0; v2 < 4; ++v2) {
+            double sm = 0.;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const signed char* o = kNodeOrder.o[v2][q];
+                const int cell = nc8(o[0], o[1], o[2]);
+                const double eta = o[3] == 0 ? e[cell] : ek2[o[0] + 2 * o[1]];
+                const double term = eta * T[cell];
+                sm = (q == 0) ? term : sm + term;
+            }
+            double mu = 0.25 * sm / 3.;
+            if (visc3) {
+                double zs = 0.;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const signed char* o = kNodeOrder.o[v2][q];
+                    const int cell = nc8(o[0], o[1], o[2]);
+                    // the zeta form keeps the reference's k-2 shift as well (flux.cpp:969-970)
+                    const double zt = o[3] == 0 ? z[cell] : R(i - 1 + o[0], j - 1 + o[1], k - 2, Z);
+                    const double term = zt * T[cell];
+                    zs = (q == 0) ? term : zs + term;
+                }
+                mu += 0.25 * zs;
+            }
+            out[NODE_S + v2] = sqrt(mu) * W;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < (P.stoch_on ? NODE_NCOMP : NODE_S); ++c) N(i, j, k, c) = out[c];
 }
 
 // ---- all fluxes of one face direction -------------------------------------------------------
@@ -193,10 +272,13 @@ __device__ inline double node_sum(const Arr& N, int c, int i, int j, int k, int 
     return s;
 }
 
-template <int NS, int D>
-__device__ inline void face_body(const DevParams& P, const Arr& Us, const Arr& R, const Arr& N, const Arr& A,
-                                 const Arr& B, const Arr& Fo, double w, double volinv, double dtinv, double dt, int i,
-                                 int j, int k)
+// The nvars face fluxes of face (i,j,k) of direction D into fo[] (energy fold included).
+// INL: the face's A/B noise is evaluated in place from Philox key K (the values noise_body
+// stores); otherwise it is read from the step's arrays A, B.
+template <int NS, int D, bool INL>
+__device__ inline void face_flux(const DevParams& P, const Arr& Us, const Arr& R, const Arr& N, const Arr& A,
+                                 const Arr& B, const NoiseKey& K, double w, double volinv, double dtinv, double dt,
+                                 int i, int j, int k, double* fo)
 {
     const int il = i - (D == 0), jl = j - (D == 1), kl = k - (D == 2);
     const int ih = wrap_up(i, P.n[0]), jh = wrap_up(j, P.n[1]), kh = wrap_up(k, P.n[2]);
@@ -219,7 +301,18 @@ __device__ inline void face_body(const DevParams& P, const Arr& Us, const Arr& R
         double wn[nv];
         wn[0] = 0.;
 #pragma unroll
-        for (int n = 1; n < nv; ++n) wn[n] = 1.0 * A(i, j, k, n) + w * B(i, j, k, n);
+        for (int n = 1; n < nv; ++n) {
+            if (INL) {
+                const uint32_t lin = (uint32_t)(((long)k * P.n[1] + j) * P.n[0] + i);
+                const double sc = (n <= 3) ? P.var_mom : (n == 4) ? P.var_ener : P.var_mass;
+                double za, zb;
+                noise_ab(K, (uint32_t)(16 * D + n), lin, za, zb);
+                const double a = za * sc, b = zb * sc;
+                wn[n] = 1.0 * a + w * b;
+            } else {
+                wn[n] = 1.0 * A(i, j, k, n) + w * B(i, j, k, n);
+            }
+        }
         double wi[nv];
         double fw1 = sqrt(P.k_B * mup * volinv * dtinv);
         double fw4 = sqrt(P.k_B * kp * volinv * dtinv);
@@ -470,28 +563,35 @@ __device__ inline void face_body(const DevParams& P, const Arr& Us, const Arr& R
         }
     }
 #pragma unroll
-    for (int n = 0; n < nv; ++n) Fo(i, j, k, n) = f[n];
+    for (int n = 0; n < nv; ++n) fo[n] = f[n];
+}
+
+template <int NS, int D, bool INL>
+__device__ inline void face_body(const DevParams& P, const Arr& Us, const Arr& R, const Arr& N, const Arr& A,
+                                 const Arr& B, const NoiseKey& K, const Arr& Fo, double w, double volinv, double dtinv,
+                                 double dt, int i, int j, int k)
+{
+    double fo[NS + 5];
+    face_flux<NS, D, INL>(P, Us, R, N, A, B, K, w, volinv, dtinv, dt, i, j, k, fo);
+#pragma unroll
+    for (int n = 0; n < NS + 5; ++n) Fo(i, j, k, n) = fo[n];
 }
 
 // ---- stage combination (+ primitive variables after the last stage) -----------------------
 
+// div[] = (fx(i+1)-fx(i))/dx + (fy(j+1)-fy(j))/dy + (fz(k+1)-fz(k))/dz, evaluated left to right.
 template <int NS>
-__device__ inline void update_body(const DevParams& P, const Arr& out, const Arr& u0, const Arr& us, const Arr& fx,
-                                   const Arr& fy, const Arr& fz, double dt, int mode, const Arr& prim, int write_prim,
-                                   int i, int j, int k)
+__device__ inline void update_from_div(const DevParams& P, const Arr& out, const Arr& u0, const Arr& us,
+                                       const double* div, double dt, int mode, const Arr& prim, int write_prim, int i,
+                                       int j, int k)
 {
-    const int ih = wrap_up(i, P.n[0]), jh = wrap_up(j, P.n[1]), kh = wrap_up(k, P.n[2]);
     const double src = 0.0;
     double o[NS + 5];
 #pragma unroll
     for (int n = 0; n < NS + 5; ++n) {
-        // face n is face 0 of the periodic box (same inputs, same bits)
-        double div = (fx(ih, j, k, n) - fx(i, j, k, n)) / P.dx[0] +
-                     (fy(i, jh, k, n) - fy(i, j, k, n)) / P.dx[1] +
-                     (fz(i, j, kh, n) - fz(i, j, k, n)) / P.dx[2];
-        if (mode == 0) o[n] = us(i, j, k, n) - dt * div + dt * src;
-        else if (mode == 1) o[n] = 0.25 * (3.0 * u0(i, j, k, n) + us(i, j, k, n) - dt * div + dt * src);
-        else o[n] = (2. / 3.) * (0.5 * u0(i, j, k, n) + us(i, j, k, n) - dt * div + dt * src);
+        if (mode == 0) o[n] = us(i, j, k, n) - dt * div[n] + dt * src;
+        else if (mode == 1) o[n] = 0.25 * (3.0 * u0(i, j, k, n) + us(i, j, k, n) - dt * div[n] + dt * src);
+        else o[n] = (2. / 3.) * (0.5 * u0(i, j, k, n) + us(i, j, k, n) - dt * div[n] + dt * src);
     }
     const double* g = P.grav;
     double r0 = us(i, j, k, 0), r1 = us(i, j, k, 1), r2 = us(i, j, k, 2), r3 = us(i, j, k, 3);
@@ -516,6 +616,21 @@ __device__ inline void update_body(const DevParams& P, const Arr& out, const Arr
 #pragma unroll
         for (int n = 0; n < 2 * NS + 6; ++n) prim(i, j, k, n) = pr[n];
     }
+}
+
+template <int NS>
+__device__ inline void update_body(const DevParams& P, const Arr& out, const Arr& u0, const Arr& us, const Arr& fx,
+                                   const Arr& fy, const Arr& fz, double dt, int mode, const Arr& prim, int write_prim,
+                                   int i, int j, int k)
+{
+    const int ih = wrap_up(i, P.n[0]), jh = wrap_up(j, P.n[1]), kh = wrap_up(k, P.n[2]);
+    double div[NS + 5];
+#pragma unroll
+    for (int n = 0; n < NS + 5; ++n)
+        // face n is face 0 of the periodic box (same inputs, same bits)
+        div[n] = (fx(ih, j, k, n) - fx(i, j, k, n)) / P.dx[0] + (fy(i, jh, k, n) - fy(i, j, k, n)) / P.dx[1] +
+                 (fz(i, j, kh, n) - fz(i, j, k, n)) / P.dx[2];
+    update_from_div<NS>(P, out, u0, us, div, dt, mode, prim, write_prim, i, j, k);
 }
 
 // ---- the noise of one step at cell (i,j,k): its three lower faces and its node ------------

@@ -25,10 +25,22 @@ EXPORTS = [
     "fhdx_engine_sync", "fhdx_engine_state", "fhdx_engine_read", "fhdx_engine_errors",
     "fhdx_engine_bytes_used", "fhdx_engine_add_uniform", "fhdx_server_create",
     "fhdx_server_submit", "fhdx_server_poll", "fhdx_server_wait", "fhdx_server_stats",
-    "fhdx_server_destroy",
+    "fhdx_server_destroy", "fhdx_ctx_autotune", "fhdx_ctx_set_tune", "fhdx_ctx_get_tune",
+    "fhdx_evaluate_stats",
 ]
 
 _lib = None
+
+
+class Tune(C.Structure):
+    """``fhdx_tune`` (include/fhdx.h): launch shape of the fast path."""
+
+    _fields_ = [("tile_width", C.c_int32), ("region_tiles", C.c_int32),
+                ("split_faces", C.c_int32), ("pad", C.c_int32), ("ms_per_step", C.c_double)]
+
+    def as_dict(self):
+        return {"tile_width": self.tile_width, "region_tiles": self.region_tiles,
+                "split_faces": self.split_faces, "ms_per_step": self.ms_per_step}
 
 
 class FhdxError(RuntimeError):
@@ -89,6 +101,12 @@ def lib():
         L.fhdx_engine_read.argtypes = [vp, C.c_int, vp, vp]
         L.fhdx_engine_errors.argtypes = [vp, C.POINTER(C.c_int)]
         L.fhdx_engine_bytes_used.argtypes = [vp]
+        L.fhdx_ctx_autotune.argtypes = [vp, Fab, C.c_double, C.c_int, C.POINTER(Tune), C.POINTER(Tune),
+                                        C.c_int, C.POINTER(C.c_int), vp]
+        L.fhdx_ctx_set_tune.argtypes = [vp, C.POINTER(Tune)]
+        L.fhdx_evaluate_stats.argtypes = [vp, Fab, Fab, Fab, Fab, Fab, Fab, Fab, Fab,
+                                          C.POINTER(C.c_double), C.c_int, C.c_int, vp]
+        L.fhdx_ctx_get_tune.argtypes = [vp, C.POINTER(Tune)]
         for name in EXPORTS:
             getattr(L, name).restype = C.c_int
         L.fhdx_engine_bytes_needed.restype = C.c_uint64
@@ -142,3 +160,22 @@ class Context:
 
     def check(self, stream=None):
         check(lib().fhdx_ctx_check(self.h, stream), "fhdx_ctx_check")
+
+    def autotune(self, cu_fab: Fab, dt: float, trial_steps: int = 2, stream=None):
+        """Pick the fastest launch shape for this box (fhdx_ctx_autotune); cu is not modified.
+        Returns (best, [timed candidates])."""
+        t = Tune()
+        trials = (Tune * 64)()
+        n = C.c_int(0)
+        check(lib().fhdx_ctx_autotune(self.h, cu_fab, float(dt), int(trial_steps), C.byref(t), trials, 64,
+                                      C.byref(n), stream), "fhdx_ctx_autotune")
+        return t, [trials[q] for q in range(n.value)]
+
+    def set_tune(self, tile_width: int, region_tiles: int, split_faces: int):
+        t = Tune(tile_width, region_tiles, split_faces, 0, 0.0)
+        check(lib().fhdx_ctx_set_tune(self.h, C.byref(t)), "fhdx_ctx_set_tune")
+
+    def get_tune(self) -> Tune:
+        t = Tune()
+        check(lib().fhdx_ctx_get_tune(self.h, C.byref(t)), "fhdx_ctx_get_tune")
+        return t

@@ -148,6 +148,19 @@ int fhdx_rk3_step_ref(fhdx_ctx* ctx, fhdx_fab cu, fhdx_fab cup, fhdx_fab cup2, f
                       const fhdx_fab* rancornA, const fhdx_fab* rancornB,
                       double dt, uint64_t seed, uint64_t step, void* stream);
 
+/* evaluateStats (src_compressible/stats.cpp:5-363; called at main_driver.cpp:613): running
+ * means consMean (nvars) and primMean (nprimvars), variances consVar (nvars) and primVar
+ * (nprimvars+5), the cross-correlations against the x = cross_cell slice in spatialCross (8)
+ * and miscStats (>= 6 comps), all cell FABs updated in place on the valid box; miscVals[20]
+ * (host memory) receives the cross_cell slice averages.  steps is the running sample count
+ * (statsCount, >= 1).  Per-cell arithmetic follows the reference's order; the yz-slice sums are
+ * added in a fixed parallel order, so they match the reference's serial sums to rounding.
+ * Waits for `stream` before returning (miscVals is on the host). */
+int fhdx_evaluate_stats(fhdx_ctx* ctx, fhdx_fab cons, fhdx_fab consMean, fhdx_fab consVar,
+                        fhdx_fab prim, fhdx_fab primMean, fhdx_fab primVar,
+                        fhdx_fab spatialCross, fhdx_fab miscStats, double* miscVals, int steps,
+                        int cross_cell, void* stream);
+
 /* ---- fused fast path --------------------------------------------------------------------- */
 
 /* One RK3 step of RK3step's arithmetic with fused stage kernels: cu (nvars comps, ngc ghosts)
@@ -161,6 +174,29 @@ int fhdx_rk3_step(fhdx_ctx* ctx, fhdx_fab cu, fhdx_fab prim, double dt, uint64_t
 int fhdx_rk3_step_injected(fhdx_ctx* ctx, fhdx_fab cu, fhdx_fab prim,
                            const fhdx_fab stochA[3], const fhdx_fab stochB[3],
                            fhdx_fab rancornA, fhdx_fab rancornB, double dt, void* stream);
+
+/* ---- launch-shape tuning ---------------------------------------------------------------- */
+
+/* Launch shape of the fast path: tile width in x (16, 32 or 64 cells; the tile has 256 cells),
+ * tiles per region of the region-major block order, and whether the face fluxes run as one
+ * kernel for the three directions (0) or one kernel per direction (1). */
+typedef struct fhdx_tune {
+    int32_t tile_width;
+    int32_t region_tiles;
+    int32_t split_faces;
+    int32_t pad;
+    double ms_per_step;       /* measured by fhdx_ctx_autotune (0 when set by hand)          */
+} fhdx_tune;
+
+/* Times every candidate launch shape on trial_steps fast-path steps of a scratch copy of cu
+ * (cu is not modified; HIP events on `stream`), keeps the fastest for this context and returns
+ * it in *best; trials (may be NULL) receives up to max_trials timed candidates, n_trials their
+ * count.  The choice is cached per (species count, box shape) for the process, so later
+ * contexts of the same shape reuse it without timing (then n_trials = 0). */
+int fhdx_ctx_autotune(fhdx_ctx* ctx, fhdx_fab cu, double dt, int trial_steps, fhdx_tune* best,
+                      fhdx_tune* trials, int max_trials, int* n_trials, void* stream);
+int fhdx_ctx_set_tune(fhdx_ctx* ctx, const fhdx_tune* cfg);
+int fhdx_ctx_get_tune(fhdx_ctx* ctx, fhdx_tune* cfg);
 
 /* ---- measurement ----------------------------------------------------------------------- */
 

@@ -50,6 +50,8 @@ def lib():
         L.orc_init_cons.argtypes = [P, Fab, C.c_int, C.c_double, C.POINTER(C.c_double),
                                     C.c_double]
         L.orc_num_threads.restype = C.c_int
+        L.orc_evaluate_stats.argtypes = [P, Fab, Fab, Fab, Fab, Fab, Fab, Fab, Fab,
+                                         C.POINTER(C.c_double), C.c_int, C.c_int]
         _lib = L
     return _lib
 
@@ -182,3 +184,29 @@ def normal_pair(words):
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+class StatsState:
+    """The statistics MultiFabs of main_driver.cpp:205-238 (host, zero-initialised)."""
+
+    def __init__(self, p: Params):
+        g = p.ngc
+        n0, n1, n2 = p.n_cells
+        sh = lambda nc: np.zeros((nc, n2 + 2 * g, n1 + 2 * g, n0 + 2 * g))  # noqa: E731
+        self.p = p
+        self.consMean, self.consVar = sh(p.nvars), sh(p.nvars)
+        self.primMean, self.primVar = sh(p.nprimvars), sh(p.nprimvars + 5)
+        self.spatialCross, self.miscStats = sh(8), sh(10)
+        self.miscVals = np.zeros(20)
+        self.lo = (-g, -g, -g)
+
+
+def evaluate_stats(p, cons, prim, ss: StatsState, steps, cross_cell):
+    """evaluateStats (stats.cpp:5-363) on host arrays; updates ss in place."""
+    lo = ss.lo
+    rc = lib().orc_evaluate_stats(C.byref(p), fab_of(cons, lo), fab_of(ss.consMean, lo), fab_of(ss.consVar, lo),
+                                  fab_of(prim, lo), fab_of(ss.primMean, lo), fab_of(ss.primVar, lo),
+                                  fab_of(ss.spatialCross, lo), fab_of(ss.miscStats, lo),
+                                  ss.miscVals.ctypes.data_as(C.POINTER(C.c_double)), int(steps),
+                                  int(cross_cell))
+    assert rc == 0, rc

@@ -1134,6 +1134,168 @@ int orc_init_cons(const fhdx_params* P, fhdx_fab cu_, int prob_type, double rho0
     return FHDX_OK;
 }
 
+/* evaluateStats (src_compressible/stats.cpp:5-363): running means and variances of cons and
+ * prim, the yz-slice averages at every x, the values of the cross_cell slice (miscVals) and the
+ * spatial cross-correlations against that slice.  Serial loops in the reference's k, j, i order,
+ * so every sum is accumulated in the reference's order. */
+int orc_evaluate_stats(const fhdx_params* P, fhdx_fab cons_, fhdx_fab consMean_, fhdx_fab consVar_,
+                       fhdx_fab prim_, fhdx_fab primMean_, fhdx_fab primVar_, fhdx_fab spatialCross_,
+                       fhdx_fab miscStats_, double* miscVals, int steps, int cross_cell)
+{
+    fab_t cu = F(cons_), cumeans = F(consMean_), cuvars = F(consVar_), prim = F(prim_);
+    fab_t primmeans = F(primMean_), primvars = F(primVar_), spatialcross = F(spatialCross_);
+    fab_t miscstats = F(miscStats_);
+    const int NS = P->nspecies, nvars = nvars_of(P), nprimvars = 6 + 2 * NS;
+    const int n0 = P->n_cells[0], n1 = P->n_cells[1], n2 = P->n_cells[2];
+    if (steps < 1 || cross_cell < 0 || cross_cell >= n0) return FHDX_ERR_ARG;
+    const double stepsminusone = steps - 1.;
+    const double stepsinv = 1. / steps;
+    const int n_cells_yz = n1 * n2;
+    double fracvec[MS];
+
+    /* evaluate_means (stats.cpp:45-88) */
+    for (int k = 0; k < n2; ++k)
+        for (int j = 0; j < n1; ++j)
+            for (int i = 0; i < n0; ++i) {
+                for (int l = 0; l < nvars; ++l)
+                    AT(cumeans, i, j, k, l) = (AT(cumeans, i, j, k, l) * stepsminusone + AT(cu, i, j, k, l)) * stepsinv;
+                double densitymeaninv = 1.0 / AT(cumeans, i, j, k, 0);
+                for (int l = 5; l < nvars; ++l) fracvec[l - 5] = AT(cumeans, i, j, k, l) * densitymeaninv;
+                AT(primmeans, i, j, k, 0) = AT(cumeans, i, j, k, 0);
+                AT(primmeans, i, j, k, 1) = AT(cumeans, i, j, k, 1) * densitymeaninv;
+                AT(primmeans, i, j, k, 2) = AT(cumeans, i, j, k, 2) * densitymeaninv;
+                AT(primmeans, i, j, k, 3) = AT(cumeans, i, j, k, 3) * densitymeaninv;
+                double vsqr = AT(primmeans, i, j, k, 1) * AT(primmeans, i, j, k, 1) +
+                              AT(primmeans, i, j, k, 2) * AT(primmeans, i, j, k, 2) +
+                              AT(primmeans, i, j, k, 3) * AT(primmeans, i, j, k, 3);
+                double intenergy = AT(cumeans, i, j, k, 4) / AT(cumeans, i, j, k, 0) - 0.5 * vsqr;
+                AT(primmeans, i, j, k, 4) = get_temperature(P, intenergy, fracvec);
+                AT(primmeans, i, j, k, 5) = get_pressure(P, fracvec, AT(cumeans, i, j, k, 0), AT(primmeans, i, j, k, 4));
+            }
+
+    /* the cross_cell slice (stats.cpp:96-150) */
+    for (int q = 0; q < 20; ++q) miscVals[q] = 0.;
+    {
+        const int x = cross_cell;
+        for (int k = 0; k < n2; ++k)
+            for (int j = 0; j < n1; ++j) {
+                miscVals[0] += AT(cumeans, x, j, k, 1);
+                miscVals[1] += AT(cu, x, j, k, 1);
+                miscVals[2] += AT(primmeans, x, j, k, 1);
+                miscVals[3] += AT(cumeans, x, j, k, 0);
+                miscVals[4] += AT(cu, x, j, k, 0);
+                miscVals[5] += AT(prim, x, j, k, 1);
+                miscVals[6] += AT(cu, x, j, k, 4);
+                miscVals[7] += AT(cumeans, x, j, k, 4);
+                miscVals[8] += AT(cu, x, j, k, 2);
+                miscVals[9] += AT(cumeans, x, j, k, 2);
+                miscVals[10] += AT(cu, x, j, k, 3);
+                miscVals[11] += AT(cumeans, x, j, k, 3);
+                double cv = 0;
+                for (int l = 0; l < NS; ++l) cv = cv + P->hcv[l] * AT(cumeans, x, j, k, 5 + l) / AT(cumeans, x, j, k, 0);
+                miscVals[12] += cv;
+                miscVals[13] += AT(primmeans, x, j, k, 4);
+                miscVals[14] += AT(primmeans, x, j, k, 2);
+                miscVals[15] += AT(primmeans, x, j, k, 3);
+                miscVals[16] += AT(prim, x, j, k, 4);
+            }
+    }
+    for (int q = 0; q < 20; ++q) miscVals[q] /= n_cells_yz;
+
+    /* yz averages at every x (stats.cpp:156-210) */
+    const int nstats = 19;
+    double* yz = (double*)calloc((size_t)n0 * nstats, sizeof(double));
+    if (!yz) return FHDX_ERR_NOMEM;
+    for (int k = 0; k < n2; ++k)
+        for (int j = 0; j < n1; ++j)
+            for (int i = 0; i < n0; ++i) {
+                double* Y = yz + (long)i * nstats;
+                Y[0] += AT(cu, i, j, k, 0);
+                Y[1] += AT(cumeans, i, j, k, 0);
+                Y[2] += AT(cu, i, j, k, 4);
+                Y[3] += AT(cumeans, i, j, k, 4);
+                Y[4] += AT(cu, i, j, k, 1);
+                Y[5] += AT(cumeans, i, j, k, 1);
+                Y[6] += AT(cu, i, j, k, 2);
+                Y[7] += AT(cumeans, i, j, k, 2);
+                Y[8] += AT(cu, i, j, k, 3);
+                Y[9] += AT(cumeans, i, j, k, 3);
+                Y[10] += AT(prim, i, j, k, 1);
+                Y[11] += AT(primmeans, i, j, k, 1);
+                Y[12] += AT(prim, i, j, k, 2);
+                Y[13] += AT(primmeans, i, j, k, 2);
+                Y[14] += AT(prim, i, j, k, 3);
+                Y[15] += AT(primmeans, i, j, k, 3);
+                double cv = 0;
+                for (int l = 0; l < NS; ++l) cv = cv + P->hcv[l] * AT(cumeans, i, j, k, 5 + l) / AT(cumeans, i, j, k, 0);
+                Y[16] += cv;
+                Y[17] += AT(prim, i, j, k, 4);
+                Y[18] += AT(primmeans, i, j, k, 4);
+            }
+    for (long q = 0; q < (long)n0 * nstats; ++q) yz[q] /= n_cells_yz;
+
+    /* variances and correlations (stats.cpp:215-360) */
+    for (int k = 0; k < n2; ++k)
+        for (int j = 0; j < n1; ++j)
+            for (int i = 0; i < n0; ++i) {
+                const double* Y = yz + (long)i * nstats;
+                double densitymeaninv = 1.0 / AT(cumeans, i, j, k, 0);
+                double delrho = AT(cu, i, j, k, 0) - AT(cumeans, i, j, k, 0);
+                double delpx = AT(cu, i, j, k, 1) - AT(cumeans, i, j, k, 1);
+                double delpy = AT(cu, i, j, k, 2) - AT(cumeans, i, j, k, 2);
+                double delpz = AT(cu, i, j, k, 3) - AT(cumeans, i, j, k, 3);
+                double delenergy = AT(cu, i, j, k, 4) - AT(cumeans, i, j, k, 4);
+                double delrhoS = Y[0] - Y[1];
+                double delES = Y[3] - Y[2];
+                double delrhoSstar = miscVals[4] - miscVals[3];
+                double delpxSstar = miscVals[1] - miscVals[0];
+                AT(cuvars, i, j, k, 0) = (AT(cuvars, i, j, k, 0) * stepsminusone + delrho * delrho) * stepsinv;
+                AT(cuvars, i, j, k, 1) = (AT(cuvars, i, j, k, 1) * stepsminusone + delpx * delpx) * stepsinv;
+                AT(cuvars, i, j, k, 2) = (AT(cuvars, i, j, k, 2) * stepsminusone + delpy * delpy) * stepsinv;
+                AT(cuvars, i, j, k, 3) = (AT(cuvars, i, j, k, 3) * stepsminusone + delpz * delpz) * stepsinv;
+                AT(cuvars, i, j, k, 4) = (AT(cuvars, i, j, k, 4) * stepsminusone + delenergy * delenergy) * stepsinv;
+                double delvelx = (delpx - AT(primmeans, i, j, k, 1) * delrho) * densitymeaninv;
+                double delvely = (delpy - AT(primmeans, i, j, k, 2) * delrho) * densitymeaninv;
+                double delvelz = (delpz - AT(primmeans, i, j, k, 3) * delrho) * densitymeaninv;
+                AT(primvars, i, j, k, 0) = AT(cuvars, i, j, k, 0);
+                AT(primvars, i, j, k, 1) = (AT(primvars, i, j, k, 1) * stepsminusone + delvelx * delvelx) * stepsinv;
+                AT(primvars, i, j, k, 2) = (AT(primvars, i, j, k, 2) * stepsminusone + delvely * delvely) * stepsinv;
+                AT(primvars, i, j, k, 3) = (AT(primvars, i, j, k, 3) * stepsminusone + delvelz * delvelz) * stepsinv;
+                double delg = AT(primmeans, i, j, k, 1) * delpx + AT(primmeans, i, j, k, 2) * delpy +
+                              AT(primmeans, i, j, k, 3) * delpz;
+                AT(primvars, i, j, k, nprimvars) =
+                    (AT(primvars, i, j, k, nprimvars) * stepsminusone + delg * delg) * stepsinv;
+                AT(primvars, i, j, k, nprimvars + 1) =
+                    (AT(primvars, i, j, k, nprimvars + 1) * stepsminusone + delg * delenergy) * stepsinv;
+                AT(primvars, i, j, k, nprimvars + 2) =
+                    (AT(primvars, i, j, k, nprimvars + 2) * stepsminusone + delrho * delenergy) * stepsinv;
+                AT(primvars, i, j, k, nprimvars + 3) =
+                    (AT(primvars, i, j, k, nprimvars + 3) * stepsminusone + delrho * delg) * stepsinv;
+                double delT = AT(prim, i, j, k, 4) - AT(primmeans, i, j, k, 4);
+                AT(primvars, i, j, k, 4) = (AT(primvars, i, j, k, 4) * stepsminusone + delT * delT) * stepsinv;
+                AT(miscstats, i, j, k, 0) = (AT(miscstats, i, j, k, 0) * stepsminusone + miscVals[1] * Y[0]) * stepsinv;
+                double delpdelrho = AT(miscstats, i, j, k, 0) - miscVals[0] * AT(cumeans, i, j, k, 0);
+                AT(miscstats, i, j, k, 1) = (AT(miscstats, i, j, k, 1) * stepsminusone + delrhoS * delrhoSstar) * stepsinv;
+                AT(miscstats, i, j, k, 2) = (AT(miscstats, i, j, k, 2) * stepsminusone + miscVals[16] * Y[17]) * stepsinv;
+                AT(miscstats, i, j, k, 3) = (AT(miscstats, i, j, k, 3) * stepsminusone + miscVals[16] * Y[0]) * stepsinv;
+                AT(miscstats, i, j, k, 4) = (AT(miscstats, i, j, k, 4) * stepsminusone + delrhoS * delpxSstar) * stepsinv;
+                AT(miscstats, i, j, k, 5) = (AT(miscstats, i, j, k, 5) * stepsminusone + delES * delrhoSstar) * stepsinv;
+                AT(spatialcross, i, j, k, 0) = miscVals[13];
+                AT(spatialcross, i, j, k, 1) = Y[18];
+                AT(spatialcross, i, j, k, 2) = AT(miscstats, i, j, k, 2);
+                AT(spatialcross, i, j, k, 3) = AT(miscstats, i, j, k, 2) - Y[18] * miscVals[13];
+                AT(spatialcross, i, j, k, 4) = AT(miscstats, i, j, k, 3) - Y[1] * miscVals[13];
+                if (miscVals[3] == 0.)
+                    AT(spatialcross, i, j, k, 5) = 0.;
+                else
+                    AT(spatialcross, i, j, k, 5) = (delpdelrho - miscVals[2] * AT(miscstats, i, j, k, 1)) / miscVals[3];
+                AT(spatialcross, i, j, k, 6) = AT(miscstats, i, j, k, 4);
+                AT(spatialcross, i, j, k, 7) = AT(miscstats, i, j, k, 5);
+            }
+    free(yz);
+    return FHDX_OK;
+}
+
 long orc_noise_doubles_per_step(const fhdx_params* P)
 {
     const long n0 = P->n_cells[0], n1 = P->n_cells[1], n2 = P->n_cells[2];

@@ -1,0 +1,89 @@
+"""Equilibrium fluctuations of the noisy RK3 step against fluctuation theory.
+
+A periodic argon box started uniform reaches thermal equilibrium; the variances of the cell
+values of rho, j_x, j_y, j_z and rho E (over cells and snapshots) must match
+
+    Var rho   = rho m / dV
+    Var j_d   = rho k_B T / dV
+    Var rhoE  = (15/4) rho (k_B T)^2 / (m dV)       (monatomic, e0 = 0)
+
+with m = M k_B / R the molecular mass and T the mean temperature of the sampled states (the
+thermal kinetic energy the fluctuations take up lowers T by ~1/N of a cell's molecules).  These are the linearised (Gaussian) values; they hold
+when a cell holds many molecules.  At dx = 8e-5 cm a cell holds ~1.4e4 argon atoms and the
+bar is 1%.  At the reference deck's dx = 8e-6 cm (~14 atoms per cell, 27% density
+fluctuations) nonlinear corrections of order 1/N reach several percent; that run is reported
+and held to 5%.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from fhdex_amd import params as P
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def theory(cfg, T):
+    p = cfg.params
+    kB, R = p.k_B, p.Runiv
+    m = p.molmass[0] * kB / R
+    dV = p.dx[0] * p.dx[1] * p.dx[2]
+    rho = cfg.rho0
+    return np.array([rho * m / dV, rho * kB * T / dV, rho * kB * T / dV, rho * kB * T / dV,
+                     3.75 * rho * (kB * T) ** 2 / (m * dV)])
+
+
+def sample(cfg, seed, burn, nsnap, every):
+    import torch
+    from fhdex_amd import compressible as fc
+    from fhdex_amd import ensemble as E
+    from fhdex_amd import lib as L
+    p = cfg.params
+    ctx = L.Context(p)
+    cu = fc.MultiFab(torch.from_numpy(E.uniform_cons(cfg)).cuda(), (-p.ngc,) * 3)
+    prim = fc.MultiFab.cell(p, p.nprimvars)
+    g = p.ngc
+    n0, n1, n2 = p.n_cells
+    step = 0
+    for _ in range(burn):
+        fc.rk3_step(ctx, cu, prim, cfg.dt, seed, step)
+        step += 1
+    acc = torch.zeros(5, dtype=torch.float64, device="cuda")
+    tsum = torch.zeros((), dtype=torch.float64, device="cuda")
+    for _ in range(nsnap):
+        for _ in range(every):
+            fc.rk3_step(ctx, cu, prim, cfg.dt, seed, step)
+            step += 1
+        v = cu.t[:5, g:g + n2, g:g + n1, g:g + n0].reshape(5, -1)
+        acc += v.var(dim=1)
+        tsum += prim.t[4, g:g + n2, g:g + n1, g:g + n0].mean()
+    ctx.check()
+    return (acc / nsnap).cpu().numpy(), float(tsum.cpu()) / nsnap
+
+
+def argon(dx, n):
+    cfg = P.argon_gas(n)
+    p = cfg.params
+    for d in range(3):
+        p.dx[d] = dx
+    cfg.dt = P.cfl_dt(p, cfg.rhobar, cfg.T_init, P.ARGON_CFL)
+    return cfg
+
+
+@pytest.mark.parametrize("dx,n,tol", [(8e-5, 32, 0.01), (8e-6, 32, 0.05)])
+def test_equilibrium_variances(dx, n, tol):
+    cfg = argon(dx, n)
+    say(f"case equilibrium argon {n}^3 dx={dx} dt={cfg.dt:.3e}")
+    got, T = sample(cfg, seed=2, burn=2000, nsnap=100, every=20)
+    want = theory(cfg, T)
+    say(f"  mean T {T:.4f} K (start {cfg.T_init} K)")
+    ratio = got / want
+    for name, g, w, r in zip(["rho", "jx", "jy", "jz", "rhoE"], got, want, ratio):
+        say(f"  Var {name}: {g:.5e} theory {w:.5e} ratio {r:.4f}")
+    assert np.all(np.abs(ratio - 1) < tol), ratio

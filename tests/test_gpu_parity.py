@@ -302,3 +302,31 @@ def test_fused_equals_reference_shaped_path_64_cubed(dev, orc):
     ctx.check()
     assert torch.equal(ds.cu.t, cu.t)
     assert torch.equal(ds.prim.t, prim.t)
+
+
+def test_launch_shapes_give_identical_bits(dev, orc):
+    """Every launch shape the tuner can pick computes the same bits; the tuner leaves cu alone."""
+    torch, fc, L = dev
+    cfg = P.argon_gas(0, shape=(40, 24, 20))
+    p = cfg.params
+    say("case launch shapes, argon 40x24x20 x 2 steps")
+    st = perturbed_state(p, cfg, orc, seed=21, amp=0.01, vel=500.0)
+    ctx = L.Context(p)
+    ref = None
+    for tw in (16, 32, 64):
+        for rh in (1, 3):
+            for split in (0, 1):
+                ctx.set_tune(tw, rh, split)
+                cu = to_dev(torch, fc, st.cu, (-p.ngc,) * 3)
+                prim = fc.MultiFab.cell(p, p.nprimvars)
+                for step in range(2):
+                    fc.rk3_step(ctx, cu, prim, cfg.dt, 9, step)
+                got = (host(cu), host(prim))
+                if ref is None:
+                    ref = got
+                assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), (tw, rh, split)
+    cu = to_dev(torch, fc, st.cu, (-p.ngc,) * 3)
+    best, trials = ctx.autotune(cu.fab, cfg.dt, trial_steps=1)
+    assert np.array_equal(host(cu), st.cu)
+    assert len(trials) >= 2 and best.ms_per_step == min(t.ms_per_step for t in trials)
+    ctx.check()

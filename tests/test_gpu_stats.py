@@ -1,0 +1,66 @@
+"""fhdx_evaluate_stats on the GPU against the oracle's evaluateStats (stats.cpp:5-363).
+
+Per-cell running means and variances keep the reference's operation order: bit-identical.
+The yz-slice sums are added in a fixed parallel order instead of the serial k-j loop, so the
+slice averages (miscVals) match to rounding, <= 1e-13 relative; the correlation fields built
+from differences of slice averages (spatialCross 3-7, miscStats 0-5) are held to 1e-9 of each
+field's magnitude (their operands cancel to ~1e-4 relative, which scales the rounding).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from fhdex_amd import params as P
+from tests.helpers import perturbed_state, valid
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("name,mk", [("argon", lambda: P.argon_gas(0, shape=(20, 12, 9))),
+                                     ("species4", lambda: P.four_species(10))])
+def test_evaluate_stats_matches_oracle(orc, name, mk):
+    import torch
+    from fhdex_amd import compressible as fc
+    from fhdex_amd import lib as L
+    cfg = mk()
+    p = cfg.params
+    say(f"case evaluateStats {name}")
+    ctx = L.Context(p)
+    gs = fc.StatsState(p)
+    hs = orc.StatsState(p)
+    lo = (-p.ngc,) * 3
+    xc = 3
+    for s in range(1, 4):
+        st = perturbed_state(p, cfg, orc, seed=70 + s, amp=0.01, vel=300.0)
+        cu = fc.MultiFab(torch.from_numpy(st.cu.copy()).cuda(), lo)
+        prim = fc.MultiFab(torch.from_numpy(st.prim.copy()).cuda(), lo)
+        fc.evaluateStats(ctx, cu, prim, gs, s, xc)
+        orc.evaluate_stats(p, st.cu, st.prim, hs, s, xc)
+    exact = [("consMean", None), ("consVar", None), ("primMean", None)]
+    for f, _ in exact:
+        a = valid(p, getattr(gs, f).t.cpu().numpy())
+        b = valid(p, getattr(hs, f))
+        say(f"  {f}: exact={np.array_equal(a, b)}")
+        assert np.array_equal(a, b), f
+    pv_g = valid(p, gs.primVar.t.cpu().numpy())
+    pv_h = valid(p, hs.primVar)
+    np_ = p.nprimvars
+    for c in list(range(5)) + [np_, np_ + 1, np_ + 2, np_ + 3]:
+        assert np.array_equal(pv_g[c], pv_h[c]), c
+    rel = np.abs(gs.miscVals - hs.miscVals) / np.maximum(np.abs(hs.miscVals), 1e-300)
+    say(f"  miscVals max rel {rel[:17].max():.2e}")
+    assert np.all((rel[:17] <= 1e-13) | (np.abs(hs.miscVals[:17]) < 1e-300))
+    for f, comps in (("spatialCross", range(8)), ("miscStats", range(6))):
+        a = valid(p, getattr(gs, f).t.cpu().numpy())
+        b = valid(p, getattr(hs, f))
+        for c in comps:
+            scale = max(np.abs(b[c]).max(), 1e-300)
+            e = np.abs(a[c] - b[c]).max() / scale
+            say(f"  {f}[{c}] rel {e:.2e}")
+            assert e <= 1e-9, (f, c)
