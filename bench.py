@@ -36,22 +36,71 @@ def measured_traffic(n: int):
         return json.load(f)["bytes_per_step"]
 
 
-def cpu_baseline(n: int, steps: int):
-    """The CPU oracle (the reference's arithmetic, OpenMP over z-planes) on the host cores."""
+def cpu_baseline(n: int, steps: int, n1: int = 32, steps1: int = 2):
+    """The CPU oracle (the reference's arithmetic in the reference's operation order, OpenMP over
+    z planes) on the host cores: all threads on argon n^3 x steps, and one thread on argon
+    n1^3 x steps1."""
     from fhdex_amd import params as P
     from oracle import ffi as orc
-    cfg = P.argon_gas(n)
+
+    def run(nn, ss):
+        cfg = P.argon_gas(nn)
+        p = cfg.params
+        st = orc.HostState(p)
+        orc.init_cons(p, st.cu, cfg.rho0, cfg.rhobar, cfg.T_init)
+        t0 = time.perf_counter()
+        rc = orc.advance(p, st.cu, st.prim, ss, cfg.dt, None, 1, 0, 0)
+        dt = time.perf_counter() - t0
+        assert rc == 0
+        return nn ** 3 * ss / dt, dt
+
+    threads = orc.num_threads()
+    v, t = run(n, steps)
+    orc.set_num_threads(1)
+    v1, t1 = run(n1, steps1)
+    orc.set_num_threads(threads)
+    return {"value": v, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (CPU restatement of the reference step, gcc -O3, OpenMP), argon {n}^3 x "
+                      f"{steps} RK3 steps on {threads} threads, {t:.1f} s",
+            "value_1thread": v1,
+            "sample_1thread": f"argon {n1}^3 x {steps1} RK3 steps on 1 thread, {t1:.1f} s"}
+
+
+def config_rate(cfg, steps: int, warmup: int = 2):
+    """cell-updates/s of the fused step on one configuration (uniform start, tuned shape)."""
+    import torch
+    from fhdex_amd import compressible as fc
+    from fhdex_amd import ensemble as E
+    from fhdex_amd import lib as L
     p = cfg.params
-    st = orc.HostState(p)
-    orc.init_cons(p, st.cu, cfg.rho0, cfg.rhobar, cfg.T_init)
+    ctx = L.Context(p)
+    cu = fc.MultiFab(torch.from_numpy(E.uniform_cons(cfg)).cuda(), (-p.ngc,) * 3)
+    prim = fc.MultiFab.cell(p, p.nprimvars)
+    best, _ = ctx.autotune(cu.fab, cfg.dt, trial_steps=2)
+    for s in range(warmup):
+        fc.rk3_step(ctx, cu, prim, cfg.dt, 1, s)
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    rc = orc.advance(p, st.cu, st.prim, steps, cfg.dt, None, 1, 0, 0)
+    for s in range(steps):
+        fc.rk3_step(ctx, cu, prim, cfg.dt, 1, warmup + s)
+    torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    assert rc == 0
-    return {"value": n ** 3 * steps / dt, "unit": "cell-updates/s", "cores": orc.num_threads(),
-            "kind": "port",
-            "sample": f"oracle (CPU restatement of the reference step), argon {n}^3 x {steps} "
-                      f"RK3 steps, {dt:.1f} s"}
+    ctx.check()
+    ctx.close()
+    cells = p.n_cells[0] * p.n_cells[1] * p.n_cells[2]
+    return {"workload": cfg.name, "value": cells * steps / dt, "unit": "cell-updates/s",
+            "ms_per_step": dt / steps * 1e3, "steps": steps, "nvars": p.nvars,
+            "launch_shape": best.as_dict()}
+
+
+def measured_flops(n: int):
+    """FP64 flops per cell-update from the committed rocprofv3 counter summary of this workload
+    (profiles/fp64_argon<n>.json), or None."""
+    path = os.path.join(ROOT, "profiles", f"fp64_argon{n}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)["flops_per_cell_update"]
 
 
 def ensemble_members(count: int, seed: int = 2024, lo: int = 16, hi: int = 64):
@@ -171,7 +220,7 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=128)
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--ensemble", type=int, default=512, help="ensemble members (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--loop-steps", type=int, default=5)
@@ -181,7 +230,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
-    ap.add_argument("--shape", type=str, default="", help="launch shape tile_width,region_tiles,split")
+    ap.add_argument("--no-configs", action="store_true", help="skip the 32^3 argon and 128^3 four-species rates")
+    ap.add_argument("--shape", type=str, default="", help="launch shape tile_width,region_tiles,z_levels,noise_inline")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates HBM byte counters)")
     a = ap.parse_args()
@@ -192,7 +242,8 @@ def main():
 
     import torch
 
-    if a.copy_probe:
+    def copy_probe():
+        """A plain 1 GiB device-to-device copy, 5 times: read + written bytes per second."""
         n = (1 << 30) // 8
         src = torch.ones(n, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
@@ -203,8 +254,13 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         gbs = 5 * 2 * n * 8 / (time.perf_counter() - t0) / 1e9
+        del src, dst
+        return gbs
+
+    if a.copy_probe:
+        n = (1 << 30) // 8
         print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8,
-                          "copy_GBs": gbs}), flush=True)
+                          "copy_GBs": copy_probe()}), flush=True)
         return
 
     if a.serve_only:
@@ -236,8 +292,8 @@ def main():
     seed = 1
     tune = None
     if a.shape:
-        tw, rh, split = (int(v) for v in a.shape.split(","))
-        ctx.set_tune(tw, rh, split)
+        tw, rh, kz, inl = (int(v) for v in a.shape.split(","))
+        ctx.set_tune(tw, rh, kz, inl)
     elif not a.no_tune:
         best, trials = ctx.autotune(cu.fab, cfg.dt, trial_steps=2)
         tune = {"chosen": best.as_dict(), "trials": [t.as_dict() for t in trials]}
@@ -264,13 +320,21 @@ def main():
     step_ms_events = sum(t for _, t in groups.values()) / a.steps
     achieved = b_alg(nv) * cells / (step_ms_events * 1e-3) / 1e9
     dom = max(groups.items(), key=lambda kv: kv[1][1])
-    roof = {"bound": "hbm", "kernel": "rk3_step (all kernel groups of one step, HIP-event timed)",
+    roof = {"bound": "hbm",
+            "kernel": "rk3_step: one full RK3 step of the box (every kernel of the step, HIP events on "
+                      "the launch stream); algorithmic bytes = B_alg x cells",
             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(a.n),
             "avg_launch_ms": step_ms_events,
-            "dominant_group": {"name": dom[0], "launches_per_step": dom[1][0] / a.steps,
-                               "ms_per_step": dom[1][1] / a.steps},
-            "groups_ms_per_step": {k: v[1] / a.steps for k, v in groups.items()}}
+            "dominant_kernel": {"name": "k_faces" if dom[0] == "faces" else dom[0],
+                                "launches_per_step": dom[1][0] / a.steps,
+                                "avg_launch_ms": dom[1][1] / max(dom[1][0], 1)},
+            "groups_ms_per_step": {k: v[1] / a.steps for k, v in groups.items()},
+            "copy_probe_GBs": copy_probe()}
+    fl = measured_flops(a.n)
+    if fl is not None:
+        roof["fp64"] = {"achieved": fl * rate / 1e12, "peak": 78.6, "unit": "TFLOP/s",
+                        "frac": fl * rate / 1e12 / 78.6, "flops_per_cell_update": fl}
     line = {
         "metric": "fp64_cell_updates_per_s",
         "value": rate,
@@ -293,6 +357,8 @@ def main():
     }
     ctx.close()
     del cu, prim
+    if not a.no_configs:
+        line["configs"] = [config_rate(P.argon_gas(32), 50), config_rate(P.four_species(128), 10)]
     if a.ensemble:
         line["ensemble"] = ensemble_bench(a.ensemble, a.ensemble_steps, a.loop_steps)
     rates = [float(v) for v in a.serve_rates.split(",") if v]

@@ -583,13 +583,15 @@ void apply_tune(FusedWorkspace& w, const fhdx_tune& t)
 {
     w.tw_max = t.tile_width;
     w.rh = t.region_tiles;
-    w.split_faces = t.split_faces != 0;
+    w.noise_inline = t.noise_inline != 0;
+    w.kz = t.z_levels;
 }
 
 bool valid_tune(const fhdx_tune& t)
 {
     return (t.tile_width == 16 || t.tile_width == 32 || t.tile_width == 64) && t.region_tiles >= 1 &&
-           t.region_tiles <= 64 && (t.split_faces == 0 || t.split_faces == 1);
+           t.region_tiles <= 64 && (t.noise_inline == 0 || t.noise_inline == 1) && t.z_levels >= 1 &&
+           t.z_levels <= 64;
 }
 
 }  // namespace
@@ -597,7 +599,8 @@ bool valid_tune(const fhdx_tune& t)
 int fhdx_ctx_set_tune(fhdx_ctx* c, const fhdx_tune* t)
 {
     if (!c || !t || !valid_tune(*t)) {
-        set_error("fhdx_ctx_set_tune: tile_width must be 16, 32 or 64, region_tiles 1..64, split_faces 0/1");
+        set_error("fhdx_ctx_set_tune: tile_width must be 16, 32 or 64, region_tiles and z_levels 1..64, "
+                  "noise_inline 0/1");
         return FHDX_ERR_ARG;
     }
     apply_tune(c->fused, *t);
@@ -609,8 +612,8 @@ int fhdx_ctx_get_tune(fhdx_ctx* c, fhdx_tune* t)
     if (!c || !t) return FHDX_ERR_ARG;
     t->tile_width = c->fused.tw_max;
     t->region_tiles = c->fused.rh;
-    t->split_faces = c->fused.split_faces ? 1 : 0;
-    t->pad = 0;
+    t->noise_inline = c->fused.noise_inline ? 1 : 0;
+    t->z_levels = c->fused.kz;
     t->ms_per_step = 0.;
     return FHDX_OK;
 }
@@ -656,35 +659,53 @@ int fhdx_ctx_autotune(fhdx_ctx* c, fhdx_fab cu, double dt, int trial_steps, fhdx
     fprim.p = sprim;
     fprim.ncomp = c->dp.nprim;
     fhdx_tune top{};
-    top.ms_per_step = 1e30;
     const bool timing_was_on = c->fused.timing.on;
     c->fused.timing.on = false;
     uint64_t step = 0;
-    for (int tw : {64, 32, 16}) {
-        if (tw > 16 && (tw >> 1) >= c->p.n_cells[0]) continue;  // same tiles as a narrower width
-        for (int rh : {1, 2, 4, 8}) {
-            for (int split : {0, 1}) {
-                if (r) break;
-                fhdx_tune t{tw, rh, split, 0, 0.};
-                apply_tune(c->fused, t);
-                float ms = 0.f;
-                int q = fused_rk3_step(c->p, c->dp, c->fused, fcu, fprim, dt, make_noise_key(0, step++, 0), nullptr,
-                                       nullptr, nullptr, nullptr, serr, s);
-                if (!q && hipEventRecord(e0, s) != hipSuccess) q = FHDX_ERR_HIP;
-                for (int k = 0; !q && k < trial_steps; ++k)
-                    q = fused_rk3_step(c->p, c->dp, c->fused, fcu, fprim, dt, make_noise_key(0, step++, 0), nullptr,
-                                       nullptr, nullptr, nullptr, serr, s);
-                if (!q && hipEventRecord(e1, s) != hipSuccess) q = FHDX_ERR_HIP;
-                if (!q && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-                    q = FHDX_ERR_HIP;
-                if (q) {
-                    fail(q, "trial step failed");
-                    break;
-                }
-                t.ms_per_step = ms / trial_steps;
-                if (t.ms_per_step < top.ms_per_step) top = t;
-                if (trials && n_trials && *n_trials < max_trials) trials[(*n_trials)++] = t;
+    auto trial = [&](fhdx_tune& t) -> int {
+        apply_tune(c->fused, t);
+        float ms = 0.f;
+        int q = fused_rk3_step(c->p, c->dp, c->fused, fcu, fprim, dt, make_noise_key(0, step++, 0), nullptr, nullptr,
+                               nullptr, nullptr, serr, s);
+        if (!q && hipEventRecord(e0, s) != hipSuccess) q = FHDX_ERR_HIP;
+        for (int k = 0; !q && k < trial_steps; ++k)
+            q = fused_rk3_step(c->p, c->dp, c->fused, fcu, fprim, dt, make_noise_key(0, step++, 0), nullptr, nullptr,
+                               nullptr, nullptr, serr, s);
+        if (!q && hipEventRecord(e1, s) != hipSuccess) q = FHDX_ERR_HIP;
+        if (!q && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            q = FHDX_ERR_HIP;
+        t.ms_per_step = ms / trial_steps;
+        if (!q && trials && n_trials && *n_trials < max_trials) trials[(*n_trials)++] = t;
+        return q;
+    };
+    // coordinate descent from the default shape: each parameter swept with the others fixed
+    fhdx_tune cur{64, 1, 1, 1, 0.};
+    if (!r) {
+        int q = trial(cur);
+        if (q) fail(q, "trial step failed");
+    }
+    top = cur;
+    const int nz = c->p.n_cells[2];
+    for (int dim = 0; dim < 4 && !r; ++dim) {
+        std::vector<int> vals;
+        if (dim == 0) vals = {64, 32, 16};
+        if (dim == 1) vals = {1, 2, 4, 8};
+        if (dim == 2) vals = {1, 2, 4, 8, 16};
+        if (dim == 3) vals = {1, 0};
+        for (int v : vals) {
+            fhdx_tune t = top;
+            int32_t* field = dim == 0 ? &t.tile_width : dim == 1 ? &t.region_tiles : dim == 2 ? &t.z_levels
+                                                                                        : &t.noise_inline;
+            if (*field == v) continue;
+            if (dim == 0 && v > 16 && (v >> 1) >= c->p.n_cells[0]) continue;  // same tiles as a narrower one
+            if (dim == 2 && v > nz) continue;
+            *field = v;
+            int q = trial(t);
+            if (q) {
+                fail(q, "trial step failed");
+                break;
             }
+            if (t.ms_per_step < top.ms_per_step) top = t;
         }
     }
     c->fused.timing.on = timing_was_on;

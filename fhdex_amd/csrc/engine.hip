@@ -47,13 +47,14 @@ template <int NS>
 __global__ void __launch_bounds__(EB) kb_cells(const Inst* __restrict__ I, const WorkItem* __restrict__ W, int st)
 {
     EB_CELL_PROLOGUE
-    cell_body<NS>(in.P, G.view(stage_in(in, st)), G.view(in.R), in.err, i, j, k);
+    cell_body<NS>(in.P, G.fld(stage_in(in, st)), G.fld(in.R), in.err, G.off(i, j, k));
 }
 
 __global__ void __launch_bounds__(EB) kb_nodes(const Inst* __restrict__ I, const WorkItem* __restrict__ W, double w)
 {
     EB_CELL_PROLOGUE
-    node_body<true>(in.P, G.view(in.R), G.view(in.N), G.view(in.R), G.view(in.R), inst_key(in), w, i, j, k);
+    node_body<true>(in.P, G.fld(in.R), G.fld(in.N), G.fld(in.R), G.fld(in.R), inst_key(in), w, i, j, k,
+                    G.off(i, j, k));
 }
 
 template <int NS>
@@ -62,29 +63,24 @@ __global__ void __launch_bounds__(EB) kb_faces(const Inst* __restrict__ I, const
 {
     constexpr int nv = NS + 5;
     EB_CELL_PROLOGUE
-    const Arr Us = G.view(stage_in(in, st)), R = G.view(in.R), N = G.view(in.N);
+    const unsigned cb = G.off(i, j, k);
+    const Fld Us = G.fld(stage_in(in, st)), R = G.fld(in.R), N = G.fld(in.N);
     const NoiseKey K = inst_key(in);  // noise evaluated in place (no A/B arrays)
     double f[nv];
-    face_flux<NS, 0, true>(in.P, Us, R, N, R, R, K, w, in.volinv, in.dtinv, in.dt, i, j, k, f);
-    const Arr F0 = G.view(in.F[0]);
-#pragma unroll
-    for (int c = 0; c < nv; ++c) F0(i, j, k, c) = f[c];
-    face_flux<NS, 1, true>(in.P, Us, R, N, R, R, K, w, in.volinv, in.dtinv, in.dt, i, j, k, f);
-    const Arr F1 = G.view(in.F[1]);
-#pragma unroll
-    for (int c = 0; c < nv; ++c) F1(i, j, k, c) = f[c];
-    face_flux<NS, 2, true>(in.P, Us, R, N, R, R, K, w, in.volinv, in.dtinv, in.dt, i, j, k, f);
-    const Arr F2 = G.view(in.F[2]);
-#pragma unroll
-    for (int c = 0; c < nv; ++c) F2(i, j, k, c) = f[c];
+    face_flux<NS, 0, true>(in.P, Us, R, N, R, R, K, w, in.volinv, in.dtinv, in.dt, i, j, k, cb, f);
+    store_face<NS, 0>(in.P, G.fld(in.F[0]), f, i, j, k, cb);
+    face_flux<NS, 1, true>(in.P, Us, R, N, R, R, K, w, in.volinv, in.dtinv, in.dt, i, j, k, cb, f);
+    store_face<NS, 1>(in.P, G.fld(in.F[1]), f, i, j, k, cb);
+    face_flux<NS, 2, true>(in.P, Us, R, N, R, R, K, w, in.volinv, in.dtinv, in.dt, i, j, k, cb, f);
+    store_face<NS, 2>(in.P, G.fld(in.F[2]), f, i, j, k, cb);
 }
 
 template <int NS>
 __global__ void __launch_bounds__(EB) kb_update(const Inst* __restrict__ I, const WorkItem* __restrict__ W, int st)
 {
     EB_CELL_PROLOGUE
-    update_body<NS>(in.P, G.view(stage_out(in, st)), G.view(in.U), G.view(stage_in(in, st)), G.view(in.F[0]),
-                    G.view(in.F[1]), G.view(in.F[2]), in.dt, st, G.view(in.prim), st == 2, i, j, k);
+    update_body<NS>(in.P, G.fld(stage_out(in, st)), G.fld(in.U), G.fld(stage_in(in, st)), G.fld(in.F[0]),
+                    G.fld(in.F[1]), G.fld(in.F[2]), in.dt, st, G.fld(in.prim), st == 2, G.off(i, j, k));
 }
 
 // which: 0 record, 1 the stage's output state (-1: the step's input state), 2 prim

@@ -29,7 +29,7 @@ struct RecLayout {
 constexpr int NODE_VC = 9, NODE_S = 10, NODE_NCOMP = 14;
 
 // Kernel groups the fast path can time with HIP events (fhdx_ctx_timing_*).
-enum KernelKind { KT_NOISE = 0, KT_CELLS, KT_NODES, KT_FLUX, KT_GHOSTS, KT_COUNT };
+enum KernelKind { KT_NOISE = 0, KT_CELLS, KT_NODES, KT_FACES, KT_UPDATE, KT_GHOSTS, KT_COUNT };
 extern const char* const kKernelKindNames[KT_COUNT];
 
 // When enabled, brackets each kernel group with events recorded on the launch stream; the
@@ -68,8 +68,9 @@ struct FusedWorkspace {
     double *rA = nullptr, *rB = nullptr;
     // launch shape (fhdx_ctx_autotune picks it by timing the candidates on the device)
     int tw_max = 64;           // tile width limit in x (16, 32, 64); tile height 256 / width
-    int rh = 4;                // tiles per region of the region-major block order
-    bool split_faces = false;  // one face direction per launch instead of k_faces
+    int rh = 1;                // tiles per region of the region-major block order
+    int kz = 1;                // z levels each tile block marches through
+    bool noise_inline = true;  // Philox noise evaluated in the faces/nodes, or once into arrays
     Timing timing;
 };
 

@@ -1,7 +1,8 @@
 // fused_kernels.hip -- the fast RK3 step on gfx950.
 //
-// Noise: each face and node evaluates its Philox A/B values in place, per stage (no noise arrays);
-// caller-supplied noise sets are copied into the grid layout and read instead.  Per stage:
+// Noise: each face and node evaluates its Philox A/B values in place, per stage, or (a tunable)
+// k_noise writes them to arrays once per step; caller-supplied noise sets are copied into the
+// grid layout and read.  Per stage:
 //   k_cells    conservedToPrimitive + transport coefficients, once per valid cell, into a
 //              ghost-padded record (u, v, w, T, p, eta, kappa, zeta, Y, X, clamped Y, chi, D);
 //   k_nodes    corner velocity gradients, visccorn and the stage-weighted node noise;
@@ -57,6 +58,56 @@ __global__ void __launch_bounds__(FB) k_copy_valid(Arr src, Grid G, double* dst,
     for (int c = 0; c < nc; ++c) d(i, j, k, c) = src(i, j, k, c);
 }
 
+// The z levels of the block that owns tile cell (i, j): k runs over [k0, min(k0 + kz, nz)).  k is
+// wave-uniform; taking it through readfirstlane keeps every address scalar.
+// ZM false: one level (kz == 1), no loop, so the compiler keeps nothing live across levels.
+#define TILE_LEVELS(TM, ...)                                             \
+    int i, j, k0;                                                        \
+    if (!tile_cell(TM, P.n[0], P.n[1], i, j, k0)) return;                \
+    if (!ZM) {                                                           \
+        const int k = k0;                                                \
+        __VA_ARGS__                                                      \
+    } else {                                                             \
+        const int k1 = min(k0 + (TM).kz, P.n[2]);                        \
+        for (int kl = k0; kl < k1; ++kl) {                               \
+            const int k = __builtin_amdgcn_readfirstlane(kl);            \
+            __VA_ARGS__                                                  \
+        }                                                                \
+    }
+
+// ---- K0: the step's noise into arrays (when not evaluated in place): one thread per cell, its
+// three lower faces and its node; the same values the faces and nodes evaluate in place.
+
+template <int NV>
+__global__ void __launch_bounds__(FB) k_noise(DevParams P, NoiseKey K, Grid G, double* A0, double* A1, double* A2,
+                                              double* B0, double* B1, double* B2, double* rA, double* rB)
+{
+    long t = (long)blockIdx.x * FB + threadIdx.x;
+    if (t >= (long)P.n[0] * P.n[1] * P.n[2]) return;
+    int i, j, k;
+    decode3(t, P.n[0], P.n[1], i, j, k);
+    const unsigned cb = G.off(i, j, k);
+    const uint32_t lin = (uint32_t)t;
+    double* As[3] = {A0, A1, A2};
+    double* Bs[3] = {B0, B1, B2};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const Fld A = G.fld(As[d]), B = G.fld(Bs[d]);
+#pragma unroll
+        for (int c = 1; c < NV; ++c) {
+            const double sc = (c <= 3) ? P.var_mom : (c == 4) ? P.var_ener : P.var_mass;
+            double za, zb;
+            noise_ab(K, (uint32_t)(16 * d + c), lin, za, zb);
+            A(cb, c) = za * sc;
+            B(cb, c) = zb * sc;
+        }
+    }
+    double za, zb;
+    noise_ab(K, NODE_FIELD, lin, za, zb);
+    G.fld(rA)(cb, 0) = za * P.var_mom;
+    G.fld(rB)(cb, 0) = zb * P.var_mom;
+}
+
 // ---- K1: cell records ------------------------------------------------------------------
 
 template <int NS>
@@ -66,7 +117,7 @@ __global__ void __launch_bounds__(FB) k_cells(DevParams P, Grid G, double* U, do
     if (t >= (long)P.n[0] * P.n[1] * P.n[2]) return;
     int i, j, k;
     decode3(t, P.n[0], P.n[1], i, j, k);
-    cell_body<NS>(P, G.view(U), G.view(R), err, i, j, k);
+    cell_body<NS>(P, G.fld(U), G.fld(R), err, G.off(i, j, k));
 }
 
 // ---- tiles: 2-D (x, y) blocks of 256 threads, one z level per block, region-major order ----
@@ -76,19 +127,23 @@ __global__ void __launch_bounds__(FB) k_cells(DevParams P, Grid G, double* U, do
 // level of a region, then up z, then to the next region.  Workgroup b runs on XCD b % 8, and XCD
 // x receives the contiguous number range x*nb/8 .. (x+1)*nb/8: the blocks an XCD holds at once
 // sit on neighbouring z levels of one region, so the z-1 / z+1 and y-1 neighbours a face or node
-// reads were fetched into that XCD's L2 a few blocks earlier.
+// reads were fetched into that XCD's L2 a few blocks earlier.  With kz > 1 a block marches
+// through kz consecutive z levels (TILE_LEVELS), reusing the level below from cache.
 
 struct TileMap {
     int tx_shift;  // tile width 1 << tx_shift (x), height 256 >> tx_shift (y)
     int rh;        // tiles per region (y)
     int nrx, nry;  // regions in x and y
-    int n2;
+    int n2;        // z chunks of kz levels
+    int kz;        // z levels a block marches through
     unsigned blocks() const { return (unsigned)((long)nrx * nry * n2 * rh); }
 };
 
-static TileMap make_tiles(int n0, int n1, int n2, int rh, int tw_max)
+static TileMap make_tiles(int n0, int n1, int n2, int rh, int tw_max, int kz)
 {
     TileMap m;
+    m.kz = kz < 1 ? 1 : kz;
+    n2 = (n2 + m.kz - 1) / m.kz;
     int smax = 6;
     while (smax > 4 && (1 << smax) > tw_max) --smax;
     int s = 0;
@@ -112,6 +167,7 @@ __device__ inline bool tile_cell(const TileMap& m, int n0, int n1, int& i, int& 
     const int region = t / col, q = t - region * col;
     k = q / m.rh;
     const int sub = q - k * m.rh;
+    k *= m.kz;  // first z level of the block
     const int rx = region % m.nrx, ry = region / m.nrx;
     i = (rx << m.tx_shift) + (threadIdx.x & ((1 << m.tx_shift) - 1));
     j = (ry * m.rh + sub) * (FB >> m.tx_shift) + (threadIdx.x >> m.tx_shift);
@@ -120,13 +176,11 @@ __device__ inline bool tile_cell(const TileMap& m, int n0, int n1, int& i, int& 
 
 // ---- K2: nodes (flux.cpp:1627-1713) and the stochastic edge terms of each node -------------
 
-template <bool INL>
+template <bool INL, bool ZM>
 __global__ void __launch_bounds__(FB) k_nodes(DevParams P, TileMap tm, Grid G, double* R, double* N, double* rA,
                                               double* rB, NoiseKey K, double w)
 {
-    int i, j, k;
-    if (!tile_cell(tm, P.n[0], P.n[1], i, j, k)) return;
-    node_body<INL>(P, G.view(R), G.view(N), G.view(rA), G.view(rB), K, w, i, j, k);
+    TILE_LEVELS(tm, node_body<INL>(P, G.fld(R), G.fld(N), G.fld(rA), G.fld(rB), K, w, i, j, k, G.off(i, j, k));)
 }
 
 // ---- K3: the three lower faces of each cell (flux.cpp:71-2372) -------------------------------
@@ -138,53 +192,35 @@ struct FaceArgs {
     double w, volinv, dtinv, dt;
 };
 
-template <int NS, bool INL>
+template <int NS, bool INL, bool ZM>
 __global__ void __launch_bounds__(FB) k_faces(DevParams P, TileMap tm, Grid G, FaceArgs a)
 {
     constexpr int nv = NS + 5;
-    int i, j, k;
-    if (!tile_cell(tm, P.n[0], P.n[1], i, j, k)) return;
-    const Arr Us = G.view(a.Us), R = G.view(a.R), N = G.view(a.N);
-    double f[nv];
-    face_flux<NS, 0, INL>(P, Us, R, N, G.view(a.A[0]), G.view(a.B[0]), a.K, a.w, a.volinv, a.dtinv, a.dt, i,
-                          j, k, f);
-    const Arr F0 = G.view(a.F[0]);
-#pragma unroll
-    for (int c = 0; c < nv; ++c) F0(i, j, k, c) = f[c];
-    face_flux<NS, 1, INL>(P, Us, R, N, G.view(a.A[1]), G.view(a.B[1]), a.K, a.w, a.volinv, a.dtinv, a.dt, i,
-                          j, k, f);
-    const Arr F1 = G.view(a.F[1]);
-#pragma unroll
-    for (int c = 0; c < nv; ++c) F1(i, j, k, c) = f[c];
-    face_flux<NS, 2, INL>(P, Us, R, N, G.view(a.A[2]), G.view(a.B[2]), a.K, a.w, a.volinv, a.dtinv, a.dt, i,
-                          j, k, f);
-    const Arr F2 = G.view(a.F[2]);
-#pragma unroll
-    for (int c = 0; c < nv; ++c) F2(i, j, k, c) = f[c];
-}
-
-// One face direction per launch (the same bodies; kept as the tuner's alternative for shapes
-// where the three-face kernel's register footprint costs more than its shared loads save).
-template <int NS, int D, bool INL>
-__global__ void __launch_bounds__(FB) k_face(DevParams P, TileMap tm, Grid G, FaceArgs a)
-{
-    int i, j, k;
-    if (!tile_cell(tm, P.n[0], P.n[1], i, j, k)) return;
-    face_body<NS, D, INL>(P, G.view(a.Us), G.view(a.R), G.view(a.N), G.view(a.A[D]), G.view(a.B[D]), a.K,
-                          G.view(a.F[D]), a.w, a.volinv, a.dtinv, a.dt, i, j, k);
+    const Fld Us = G.fld(a.Us), R = G.fld(a.R), N = G.fld(a.N);
+    TILE_LEVELS(tm, {
+        const unsigned cb = G.off(i, j, k);
+        double f[nv];
+        face_flux<NS, 0, INL>(P, Us, R, N, G.fld(a.A[0]), G.fld(a.B[0]), a.K, a.w, a.volinv, a.dtinv, a.dt, i, j, k,
+                              cb, f);
+        store_face<NS, 0>(P, G.fld(a.F[0]), f, i, j, k, cb);
+        face_flux<NS, 1, INL>(P, Us, R, N, G.fld(a.A[1]), G.fld(a.B[1]), a.K, a.w, a.volinv, a.dtinv, a.dt, i, j, k,
+                              cb, f);
+        store_face<NS, 1>(P, G.fld(a.F[1]), f, i, j, k, cb);
+        face_flux<NS, 2, INL>(P, Us, R, N, G.fld(a.A[2]), G.fld(a.B[2]), a.K, a.w, a.volinv, a.dtinv, a.dt, i, j, k,
+                              cb, f);
+        store_face<NS, 2>(P, G.fld(a.F[2]), f, i, j, k, cb);
+    })
 }
 
 // ---- K4: stage combination (+ primitive variables after the last stage) --------------------
 
-template <int NS>
+template <int NS, bool ZM>
 __global__ void __launch_bounds__(FB) k_update(DevParams P, TileMap tm, Grid G, double* out, double* u0,
                                                double* us, double* fx, double* fy, double* fz, double dt, int mode,
                                                double* prim, int write_prim)
 {
-    int i, j, k;
-    if (!tile_cell(tm, P.n[0], P.n[1], i, j, k)) return;
-    update_body<NS>(P, G.view(out), G.view(u0), G.view(us), G.view(fx), G.view(fy), G.view(fz), dt, mode,
-                    G.view(prim), write_prim, i, j, k);
+    TILE_LEVELS(tm, update_body<NS>(P, G.fld(out), G.fld(u0), G.fld(us), G.fld(fx), G.fld(fy), G.fld(fz), dt, mode,
+                                    G.fld(prim), write_prim, G.off(i, j, k));)
 }
 
 // ---- host side ---------------------------------------------------------------------------
@@ -251,15 +287,17 @@ static void launch_stage(const DevParams& P, FusedWorkspace& w, const Grid& G, d
 {
     const long ncell = (long)P.n[0] * P.n[1] * P.n[2];
     const RecLayout L{NS};
-    const TileMap tm = make_tiles(P.n[0], P.n[1], P.n[2], w.rh, w.tw_max);
+    const TileMap tm = make_tiles(P.n[0], P.n[1], P.n[2], w.rh, w.tw_max, w.kz);
     const unsigned ntile = tm.blocks();
     Timing& T = w.timing;
     T.begin(KT_CELLS, s);
     k_cells<NS><<<grid_for(ncell, FB), FB, 0, s>>>(P, G, us, w.rec, d_err);
     fill_ghosts(P, G, w.rec, L.ncomp(), s);
     T.end(s);
+    const bool zm = tm.kz > 1;
     T.begin(KT_NODES, s);
-    k_nodes<INL><<<ntile, FB, 0, s>>>(P, tm, G, w.rec, w.node, w.rA, w.rB, K, wst);
+    if (zm) k_nodes<INL, true><<<ntile, FB, 0, s>>>(P, tm, G, w.rec, w.node, w.rA, w.rB, K, wst);
+    else k_nodes<INL, false><<<ntile, FB, 0, s>>>(P, tm, G, w.rec, w.node, w.rA, w.rB, K, wst);
     T.end(s);
     FaceArgs fa;
     fa.Us = us;
@@ -275,15 +313,16 @@ static void launch_stage(const DevParams& P, FusedWorkspace& w, const Grid& G, d
     fa.volinv = 1. / (P.dx[0] * P.dx[1] * P.dx[2]);
     fa.dtinv = 1. / dt;
     fa.dt = dt;
-    T.begin(KT_FLUX, s);
-    if (w.split_faces) {
-        k_face<NS, 0, INL><<<ntile, FB, 0, s>>>(P, tm, G, fa);
-        k_face<NS, 1, INL><<<ntile, FB, 0, s>>>(P, tm, G, fa);
-        k_face<NS, 2, INL><<<ntile, FB, 0, s>>>(P, tm, G, fa);
-    } else {
-        k_faces<NS, INL><<<ntile, FB, 0, s>>>(P, tm, G, fa);
-    }
-    k_update<NS><<<ntile, FB, 0, s>>>(P, tm, G, out, cu, us, w.F[0], w.F[1], w.F[2], dt, st, prim, st == 2);
+    T.begin(KT_FACES, s);
+    if (zm) k_faces<NS, INL, true><<<ntile, FB, 0, s>>>(P, tm, G, fa);
+    else k_faces<NS, INL, false><<<ntile, FB, 0, s>>>(P, tm, G, fa);
+    T.end(s);
+    T.begin(KT_UPDATE, s);
+    if (zm)
+        k_update<NS, true><<<ntile, FB, 0, s>>>(P, tm, G, out, cu, us, w.F[0], w.F[1], w.F[2], dt, st, prim, st == 2);
+    else
+        k_update<NS, false><<<ntile, FB, 0, s>>>(P, tm, G, out, cu, us, w.F[0], w.F[1], w.F[2], dt, st, prim,
+                                                 st == 2);
     T.end(s);
     T.begin(KT_GHOSTS, s);
     fill_ghosts(P, G, out, P.nvars, s);
@@ -299,8 +338,21 @@ int fused_rk3_step(const fhdx_params& p, const DevParams& P, FusedWorkspace& w, 
     if (r) return r;
     const Grid G = grid_of(P);
     const long ncell = (long)P.n[0] * P.n[1] * P.n[2];
-    const bool inl = (sA == nullptr);
-    if (!inl) {
+    const bool philox = (sA == nullptr);
+    const bool inl = philox && w.noise_inline;
+    if (philox && !inl) {
+        if ((r = ensure_noise_arrays(P, w))) return r;
+        w.timing.begin(KT_NOISE, s);
+        switch (P.ns) {
+            case 2: k_noise<7><<<grid_for(ncell, FB), FB, 0, s>>>(P, K, G, w.sA[0], w.sA[1], w.sA[2], w.sB[0], w.sB[1], w.sB[2], w.rA, w.rB); break;
+            case 3: k_noise<8><<<grid_for(ncell, FB), FB, 0, s>>>(P, K, G, w.sA[0], w.sA[1], w.sA[2], w.sB[0], w.sB[1], w.sB[2], w.rA, w.rB); break;
+            case 4: k_noise<9><<<grid_for(ncell, FB), FB, 0, s>>>(P, K, G, w.sA[0], w.sA[1], w.sA[2], w.sB[0], w.sB[1], w.sB[2], w.rA, w.rB); break;
+            default: return FHDX_ERR_UNSUPPORTED;
+        }
+        w.timing.end(s);
+        FHDX_HIP_TRY(hipGetLastError());
+    }
+    if (!philox) {
         // caller noise: face / node FABs, valid entries copied into the grid layout
         if ((r = ensure_noise_arrays(P, w))) return r;
         w.timing.begin(KT_NOISE, s);
@@ -351,7 +403,7 @@ int fused_rk3_step(const fhdx_params& p, const DevParams& P, FusedWorkspace& w, 
 
 namespace fhdx {
 
-const char* const kKernelKindNames[KT_COUNT] = {"noise", "cells", "nodes", "flux_update", "ghosts"};
+const char* const kKernelKindNames[KT_COUNT] = {"noise", "cells", "nodes", "faces", "update", "ghosts"};
 
 hipEvent_t Timing::get()
 {

@@ -9,12 +9,25 @@
 
 namespace fhdx {
 
-__device__ inline int wrap_up(int i, int n) { return i + 1 == n ? 0 : i + 1; }
 
 // One layout for every array of a box (a cell FAB with ngc ghost layers, component-major; face
 // and node arrays use it too, their entry (i,j,k) being the lower face / node of cell (i,j,k)).
 // Views made from one Grid share their index arithmetic, so a kernel computes each cell offset
 // once for all arrays.
+// A field of one box in its Grid layout.  F(cb, c, di, dj, dk) is component c of the cell
+// (di,dj,dk) away from the cell at byte offset cb: the shift goes into the wave-uniform base
+// pointer, so every access of a thread uses the same 32-bit vector offset (one VGPR) and the
+// address arithmetic runs on the scalar unit.
+struct Fld {
+    double* p;
+    long sn;
+    int sy, sz;
+    __device__ inline double& operator()(unsigned cb, int c, int di = 0, int dj = 0, int dk = 0) const
+    {
+        return *(double*)((char*)(p + (long)c * sn + di + (long)dj * sy + (long)dk * sz) + cb);
+    }
+};
+
 struct Grid {
     int lo, sy, sz;
     long sn;
@@ -27,6 +40,12 @@ struct Grid {
         a.sz = sz;
         a.sn = sn;
         return a;
+    }
+    __host__ __device__ Fld fld(double* p) const { return Fld{p, sn, sy, sz}; }
+    // byte offset of cell (i,j,k) within a component (below 4 GiB: checked by the C ABI)
+    __device__ unsigned off(int i, int j, int k) const
+    {
+        return (unsigned)((i - lo) + (j - lo) * sy + (k - lo) * sz) << 3;
     }
 };
 
@@ -76,12 +95,12 @@ __device__ inline void ghost_body(const Arr& a, int nc, int g, int n0, int n1, i
 // ---- conservedToPrimitive + transport coefficients into the cell record -------------------
 
 template <int NS>
-__device__ inline void cell_body(const DevParams& P, const Arr& U, const Arr& R, int* err, int i, int j, int k)
+__device__ inline void cell_body(const DevParams& P, const Fld& U, const Fld& R, int* err, unsigned cb)
 {
     const RecLayout L{NS};
     double c[NS + 5], pr[2 * NS + 6];
 #pragma unroll
-    for (int n = 0; n < NS + 5; ++n) c[n] = U(i, j, k, n);
+    for (int n = 0; n < NS + 5; ++n) c[n] = U(cb, n);
     cons_to_prim_cell<NS>(P, c, pr);
     Transport tr;
     double chi[NS], D[NS * NS];
@@ -95,23 +114,23 @@ __device__ inline void cell_body(const DevParams& P, const Arr& U, const Arr& R,
     for (int s = 0; s < NS; ++s) sumy += yn[s];
 #pragma unroll
     for (int s = 0; s < NS; ++s) yn[s] /= sumy;
-    R(i, j, k, RecLayout::U) = pr[1];
-    R(i, j, k, RecLayout::V) = pr[2];
-    R(i, j, k, RecLayout::W) = pr[3];
-    R(i, j, k, RecLayout::T) = pr[4];
-    R(i, j, k, RecLayout::P) = pr[5];
-    R(i, j, k, RecLayout::ETA) = tr.eta;
-    R(i, j, k, RecLayout::KAP) = tr.kappa;
-    R(i, j, k, RecLayout::ZETA) = tr.zeta;
+    R(cb, RecLayout::U) = pr[1];
+    R(cb, RecLayout::V) = pr[2];
+    R(cb, RecLayout::W) = pr[3];
+    R(cb, RecLayout::T) = pr[4];
+    R(cb, RecLayout::P) = pr[5];
+    R(cb, RecLayout::ETA) = tr.eta;
+    R(cb, RecLayout::KAP) = tr.kappa;
+    R(cb, RecLayout::ZETA) = tr.zeta;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-        R(i, j, k, RecLayout::Y + s) = pr[6 + s];
-        R(i, j, k, L.X() + s) = pr[6 + NS + s];
-        R(i, j, k, L.YN() + s) = yn[s];
-        R(i, j, k, L.CHI() + s) = chi[s];
+        R(cb, RecLayout::Y + s) = pr[6 + s];
+        R(cb, L.X() + s) = pr[6 + NS + s];
+        R(cb, L.YN() + s) = yn[s];
+        R(cb, L.CHI() + s) = chi[s];
     }
 #pragma unroll
-    for (int q = 0; q < NS * NS; ++q) R(i, j, k, L.D() + q) = D[q];
+    for (int q = 0; q < NS * NS; ++q) R(cb, L.D() + q) = D[q];
 }
 
 // ---- nodes (flux.cpp:1627-1713) and the stochastic edge terms of each node -----------------
@@ -133,9 +152,11 @@ __device__ constexpr int nc8(int a, int b, int c) { return a + 2 * b + 4 * c; }
 // All loads come first (into registers), all stores last: the record and node arrays are
 // distinct, and keeping stores behind the loads lets the compiler load each value once.
 // INL: the node noise is evaluated in place from Philox key K instead of read from rA, rB.
+// Node (i,j,k) of a periodic box equals node (i,j,k) + n along each axis: the values of nodes with
+// a zero index are stored at those images too, so faces read node i+1 without wrapping.
 template <bool INL>
-__device__ inline void node_body(const DevParams& P, const Arr& R, const Arr& N, const Arr& rA, const Arr& rB,
-                                 const NoiseKey& K, double w, int i, int j, int k)
+__device__ inline void node_body(const DevParams& P, const Fld& R, const Fld& N, const Fld& rA, const Fld& rB,
+                                 const NoiseKey& K, double w, int i, int j, int k, unsigned cb)
 {
     const int E = RecLayout::ETA, Z = RecLayout::ZETA, RT = RecLayout::T;
     const bool visc3 = (P.visc_type == 3 || P.visc_type == -3);
@@ -147,16 +168,16 @@ __device__ inline void node_body(const DevParams& P, const Arr& R, const Arr& N,
 #pragma unroll
             for (int a = 0; a < 2; ++a) {
                 const int q = nc8(a, b, c);
-                u[q] = R(i - 1 + a, j - 1 + b, k - 1 + c, RecLayout::U);
-                v[q] = R(i - 1 + a, j - 1 + b, k - 1 + c, RecLayout::V);
-                wv[q] = R(i - 1 + a, j - 1 + b, k - 1 + c, RecLayout::W);
-                e[q] = R(i - 1 + a, j - 1 + b, k - 1 + c, E);
-                T[q] = R(i - 1 + a, j - 1 + b, k - 1 + c, RT);
+                u[q] = R(cb, RecLayout::U, a - 1, b - 1, c - 1);
+                v[q] = R(cb, RecLayout::V, a - 1, b - 1, c - 1);
+                wv[q] = R(cb, RecLayout::W, a - 1, b - 1, c - 1);
+                e[q] = R(cb, E, a - 1, b - 1, c - 1);
+                T[q] = R(cb, RT, a - 1, b - 1, c - 1);
             }
 #pragma unroll
     for (int b = 0; b < 2; ++b)
 #pragma unroll
-        for (int a = 0; a < 2; ++a) ek2[a + 2 * b] = R(i - 1 + a, j - 1 + b, k - 2, E);  // eta(k-2), order 2
+        for (int a = 0; a < 2; ++a) ek2[a + 2 * b] = R(cb, E, a - 1, b - 1, -2);  // eta(k-2), order 2
     double rAv = 0., rBv = 0.;
     if (P.stoch_on) {
         if (INL) {
@@ -165,8 +186,8 @@ __device__ inline void node_body(const DevParams& P, const Arr& R, const Arr& N,
             rAv = za * P.var_mom;
             rBv = zb * P.var_mom;
         } else {
-            rAv = rA(i, j, k);
-            rBv = rB(i, j, k);
+            rAv = rA(cb, 0);
+            rBv = rB(cb, 0);
         }
     }
 
@@ -181,7 +202,7 @@ __device__ inline void node_body(const DevParams& P, const Arr& R, const Arr& N,
 #pragma unroll
             for (int b = 0; b < 2; ++b)
 #pragma unroll
-                for (int a = 0; a < 2; ++a) z[nc8(a, b, c)] = R(i - 1 + a, j - 1 + b, k - 1 + c, Z);
+                for (int a = 0; a < 2; ++a) z[nc8(a, b, c)] = R(cb, Z, a - 1, b - 1, c - 1);
         zetaxp = 0.125 * (z[nc8(1, 0, 0)] + z[nc8(0, 0, 0)] + z[nc8(1, 1, 0)] + z[nc8(0, 1, 0)] +
                           z[nc8(1, 0, 1)] + z[nc8(0, 0, 1)] + z[nc8(1, 1, 1)] + z[nc8(0, 1, 1)]);
     }
@@ -234,7 +255,7 @@ __device__ inline void node_body(const DevParams& P, const Arr& R, const Arr& N,
                     const signed char* o = kNodeOrder.o[v2][q];
                     const int cell = nc8(o[0], o[1], o[2]);
                     // the zeta form keeps the reference's k-2 shift as well (flux.cpp:969-970)
-                    const double zt = o[3] == 0 ? z[cell] : R(i - 1 + o[0], j - 1 + o[1], k - 2, Z);
+                    const double zt = o[3] == 0 ? z[cell] : R(cb, Z, o[0] - 1, o[1] - 1, -2);
                     const double term = zt * T[cell];
                     zs = (q == 0) ? term : zs + term;
                 }
@@ -243,8 +264,15 @@ __device__ inline void node_body(const DevParams& P, const Arr& R, const Arr& N,
             out[NODE_S + v2] = sqrt(mu) * W;
         }
     }
+    const int nout = P.stoch_on ? NODE_NCOMP : NODE_S;
 #pragma unroll
-    for (int c = 0; c < (P.stoch_on ? NODE_NCOMP : NODE_S); ++c) N(i, j, k, c) = out[c];
+    for (int m = 0; m < 8; ++m) {
+        if (((m & 1) && i != 0) || ((m & 2) && j != 0) || ((m & 4) && k != 0)) continue;
+        const int sx = (m & 1) ? P.n[0] : 0, sy = (m & 2) ? P.n[1] : 0, sz = (m & 4) ? P.n[2] : 0;
+#pragma unroll
+        for (int c = 0; c < NODE_NCOMP; ++c)
+            if (c < nout) N(cb, c, sx, sy, sz) = out[c];
+    }
 }
 
 // ---- all fluxes of one face direction -------------------------------------------------------
@@ -253,8 +281,7 @@ __device__ inline void node_body(const DevParams& P, const Arr& R, const Arr& N,
 // (lo,lo) over the two transverse axes, or in the z-face order (hi,hi),(hi,lo),(lo,hi),(lo,lo).
 // Node index n along an axis is node 0 (periodic), hence the wrapped "hi" indices.
 template <int D>
-__device__ inline double node_sum(const Arr& N, int c, int i, int j, int k, int ih, int jh, int kh, bool zorder,
-                                  double s0, bool first)
+__device__ inline double node_sum(const Fld& N, int c, unsigned cb, bool zorder, double s0, bool first)
 {
     // first transverse axis: x faces y, y faces x, z faces x; second: x faces z, y faces z, z faces y
     const bool ahi[4] = {true, zorder ? true : false, zorder ? false : true, false};
@@ -262,31 +289,29 @@ __device__ inline double node_sum(const Arr& N, int c, int i, int j, int k, int 
     double s = s0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        int a = i, b = j, cc = k;
-        if (D == 0) { b = ahi[q] ? jh : j; cc = bhi[q] ? kh : k; }
-        if (D == 1) { a = ahi[q] ? ih : i; cc = bhi[q] ? kh : k; }
-        if (D == 2) { a = ahi[q] ? ih : i; b = bhi[q] ? jh : j; }
-        double v = N(a, b, cc, c);
+        const int a = ahi[q] ? 1 : 0, b = bhi[q] ? 1 : 0;
+        double v = D == 0 ? N(cb, c, 0, a, b) : (D == 1 ? N(cb, c, a, 0, b) : N(cb, c, a, b, 0));
         s = (first && q == 0) ? v : s + v;
     }
     return s;
 }
 
 // The nvars face fluxes of face (i,j,k) of direction D into fo[] (energy fold included).
-// INL: the face's A/B noise is evaluated in place from Philox key K (the values noise_body
-// stores); otherwise it is read from the step's arrays A, B.
+// INL: the face's A/B noise is evaluated in place from Philox key K (the values fhdx_fill_noise
+// writes for that key); otherwise it is read from the step's arrays A, B.
 template <int NS, int D, bool INL>
-__device__ inline void face_flux(const DevParams& P, const Arr& Us, const Arr& R, const Arr& N, const Arr& A,
-                                 const Arr& B, const NoiseKey& K, double w, double volinv, double dtinv, double dt,
-                                 int i, int j, int k, double* fo)
+__device__ inline void face_flux(const DevParams& P, const Fld& Us, const Fld& R, const Fld& N, const Fld& A,
+                                 const Fld& B, const NoiseKey& K, double w, double volinv, double dtinv, double dt,
+                                 int i, int j, int k, unsigned cb, double* fo)
 {
-    const int il = i - (D == 0), jl = j - (D == 1), kl = k - (D == 2);
-    const int ih = wrap_up(i, P.n[0]), jh = wrap_up(j, P.n[1]), kh = wrap_up(k, P.n[2]);
+    // the face lies between cell (i,j,k) ("U", offset cb) and its lower neighbour ("L", -e)
+    constexpr int ex = (D == 0), ey = (D == 1), ez = (D == 2);
     constexpr int nv = NS + 5;
     const RecLayout L{NS};
     const int RT = RecLayout::T, RE = RecLayout::ETA;
     // velocity component c (1..3) lives in record slot c-1
-    auto vel = [&](int a, int b, int cc, int c) { return R(a, b, cc, RecLayout::U + c - 1); };
+    auto velU = [&](int c) { return R(cb, RecLayout::U + c - 1); };
+    auto velL = [&](int c) { return R(cb, RecLayout::U + c - 1, -ex, -ey, -ez); };
 
     double f[nv + 4];
 #pragma unroll
@@ -294,9 +319,9 @@ __device__ inline void face_flux(const DevParams& P, const Arr& Us, const Arr& R
 
     // ---- stochastic (flux.cpp:71-1145) ----
     if (P.stoch_on) {
-        double Tu = R(i, j, k, RT), Tl = R(il, jl, kl, RT);
-        double mup = R(i, j, k, RE) * Tu + R(il, jl, kl, RE) * Tl;
-        double kp = R(i, j, k, RecLayout::KAP) * Tu * Tu + R(il, jl, kl, RecLayout::KAP) * Tl * Tl;
+        double Tu = R(cb, RT), Tl = R(cb, RT, -ex, -ey, -ez);
+        double mup = R(cb, RE) * Tu + R(cb, RE, -ex, -ey, -ez) * Tl;
+        double kp = R(cb, RecLayout::KAP) * Tu * Tu + R(cb, RecLayout::KAP, -ex, -ey, -ez) * Tl * Tl;
         double meanT = 0.5 * (Tu + Tl);
         double wn[nv];
         wn[0] = 0.;
@@ -310,7 +335,7 @@ __device__ inline void face_flux(const DevParams& P, const Arr& Us, const Arr& R
                 const double a = za * sc, b = zb * sc;
                 wn[n] = 1.0 * a + w * b;
             } else {
-                wn[n] = 1.0 * A(i, j, k, n) + w * B(i, j, k, n);
+                wn[n] = 1.0 * A(cb, n) + w * B(cb, n);
             }
         }
         double wi[nv];
@@ -325,29 +350,29 @@ __device__ inline void face_flux(const DevParams& P, const Arr& Us, const Arr& R
         // edge terms sqrt(mu)*rancorn of the pp, mp, pm, mm edges (flux.cpp:277-280, 644-647, 1011-1014)
         double sum;
         if (D == 0) {
-            sum = N(i, jh, kh, NODE_S + 0);
-            sum = sum + N(i, j, kh, NODE_S + 0);
-            sum = sum + N(i, jh, k, NODE_S + 0);
-            sum = sum + N(i, j, k, NODE_S + 0);
+            sum = N(cb, NODE_S + 0, 0, 1, 1);
+            sum = sum + N(cb, NODE_S + 0, 0, 0, 1);
+            sum = sum + N(cb, NODE_S + 0, 0, 1, 0);
+            sum = sum + N(cb, NODE_S + 0);
         } else if (D == 1) {
-            sum = N(ih, j, kh, NODE_S + 0);
-            sum = sum + N(i, j, kh, NODE_S + 1);
-            sum = sum + N(ih, j, k, NODE_S + 0);
-            sum = sum + N(i, j, k, NODE_S + 1);
+            sum = N(cb, NODE_S + 0, 1, 0, 1);
+            sum = sum + N(cb, NODE_S + 1, 0, 0, 1);
+            sum = sum + N(cb, NODE_S + 0, 1, 0, 0);
+            sum = sum + N(cb, NODE_S + 1);
         } else {
-            sum = N(ih, jh, k, NODE_S + 0);
-            sum = sum + N(i, jh, k, NODE_S + 1);
-            sum = sum + N(ih, j, k, NODE_S + 2);
-            sum = sum + N(i, j, k, NODE_S + 3);
+            sum = N(cb, NODE_S + 0, 1, 1, 0);
+            sum = sum + N(cb, NODE_S + 1, 0, 1, 0);
+            sum = sum + N(cb, NODE_S + 2, 1, 0, 0);
+            sum = sum + N(cb, NODE_S + 3);
         }
         wi[1 + D] = wi[1 + D] + 0.25 * nweight * sum;
 #pragma unroll
         for (int n = 1; n < 4; ++n) f[n] = f[n] + wi[n];
         f[nv] = f[nv] + wi[4];
         constexpr int t1 = (D == 0) ? 2 : 1, t2 = (D == 2) ? 2 : 3;
-        double phidiag = wi[1 + D] * (vel(il, jl, kl, 1 + D) + vel(i, j, k, 1 + D));
-        double phishear = wi[t1] * (vel(il, jl, kl, t1) + vel(i, j, k, t1)) +
-                          wi[t2] * (vel(il, jl, kl, t2) + vel(i, j, k, t2));
+        double phidiag = wi[1 + D] * (velL(1 + D) + velU(1 + D));
+        double phishear = wi[t1] * (velL(t1) + velU(t1)) +
+                          wi[t2] * (velL(t2) + velU(t2));
         phidiag = -0.5 * phidiag;
         phishear = -0.5 * phishear;
         f[nv + 1] = f[nv + 1] - phidiag;
@@ -357,8 +382,8 @@ __device__ inline void face_flux(const DevParams& P, const Arr& Us, const Arr& R
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
                 wi[5 + s] = 0.;
-                yy[s] = R(il, jl, kl, L.YN() + s);
-                yyp[s] = R(i, j, k, L.YN() + s);
+                yy[s] = R(cb, L.YN() + s, -ex, -ey, -ez);
+                yyp[s] = R(cb, L.YN() + s);
             }
             double MWmix = 0.;
 #pragma unroll
@@ -366,10 +391,10 @@ __device__ inline void face_flux(const DevParams& P, const Arr& Us, const Arr& R
                 MWmix = MWmix + 0.5 * (yy[s] + yyp[s]) / P.molmass[s];
 #pragma unroll
                 for (int l = 0; l < NS; ++l)
-                    DijY[s * NS + l] = 0.5 * (R(il, jl, kl, L.D() + l * NS + s) * yy[l] +
-                                              R(i, j, k, L.D() + l * NS + s) * yyp[l] +
-                                              (R(il, jl, kl, L.D() + s * NS + l) * yy[s] +
-                                               R(i, j, k, L.D() + s * NS + l) * yyp[s]));
+                    DijY[s * NS + l] = 0.5 * (R(cb, L.D() + l * NS + s, -ex, -ey, -ez) * yy[l] +
+                                              R(cb, L.D() + l * NS + s) * yyp[l] +
+                                              (R(cb, L.D() + s * NS + l, -ex, -ey, -ez) * yy[s] +
+                                               R(cb, L.D() + s * NS + l) * yyp[s]));
             }
 #pragma unroll
             for (int s = 0; s < NS; ++s)
@@ -393,7 +418,7 @@ __device__ inline void face_flux(const DevParams& P, const Arr& Us, const Arr& R
             for (int s = 0; s < NS; ++s) {
                 double hk = P.e0[s] + P.hcp[s] * meanT;
                 double ss = (hk + P.Runiv * meanT / P.molmass[s] * 0.5 *
-                                      (R(il, jl, kl, L.CHI() + s) + R(i, j, k, L.CHI() + s))) *
+                                      (R(cb, L.CHI() + s, -ex, -ey, -ez) + R(cb, L.CHI() + s))) *
                             wi[5 + s];
                 soret = soret + ss;
             }
@@ -404,20 +429,20 @@ __device__ inline void face_flux(const DevParams& P, const Arr& Us, const Arr& R
     // ---- diffusive (flux.cpp:1193-1623) ----
     {
         const double half = 0.5, h = P.dx[D];
-        double mup = half * (R(i, j, k, RE) + R(il, jl, kl, RE));
-        double kp = half * (R(i, j, k, RecLayout::KAP) + R(il, jl, kl, RecLayout::KAP));
-        double tau1 = mup * (vel(i, j, k, 1) - vel(il, jl, kl, 1)) / h;
-        double tau2 = mup * (vel(i, j, k, 2) - vel(il, jl, kl, 2)) / h;
-        double tau3 = mup * (vel(i, j, k, 3) - vel(il, jl, kl, 3)) / h;
+        double mup = half * (R(cb, RE) + R(cb, RE, -ex, -ey, -ez));
+        double kp = half * (R(cb, RecLayout::KAP) + R(cb, RecLayout::KAP, -ex, -ey, -ez));
+        double tau1 = mup * (velU(1) - velL(1)) / h;
+        double tau2 = mup * (velU(2) - velL(2)) / h;
+        double tau3 = mup * (velU(3) - velL(3)) / h;
         double divp = 0.;
-        double s1 = vel(il, jl, kl, 1) + vel(i, j, k, 1);
-        double s2 = vel(il, jl, kl, 2) + vel(i, j, k, 2);
-        double s3 = vel(il, jl, kl, 3) + vel(i, j, k, 3);
+        double s1 = velL(1) + velU(1);
+        double s2 = velL(2) + velU(2);
+        double s3 = velL(3) + velU(3);
         double phiflx;
         if (D == 0) phiflx = tau1 * s1 + divp * s1 + tau2 * s2 + tau3 * s3;
         else if (D == 1) phiflx = tau1 * s1 + tau2 * s2 + divp * s2 + tau3 * s3;
         else phiflx = tau1 * s1 + tau2 * s2 + tau3 * s3 + divp * s3;
-        double TU = R(i, j, k, RT), TL = R(il, jl, kl, RT);
+        double TU = R(cb, RT), TL = R(cb, RT, -ex, -ey, -ez);
         double Qflux = kp * (TU - TL) / h;
         f[1] = f[1] - (D == 0 ? tau1 + divp : tau1);
         f[2] = f[2] - (D == 1 ? tau2 + divp : tau2);
@@ -425,23 +450,23 @@ __device__ inline void face_flux(const DevParams& P, const Arr& Us, const Arr& R
         f[nv] = f[nv] - Qflux;
         f[nv + 1] = f[nv + 1] - (half * phiflx);
         double meanT = 0.5 * (TL + TU);
-        double PU = R(i, j, k, RecLayout::P), PL = R(il, jl, kl, RecLayout::P);
+        double PU = R(cb, RecLayout::P), PL = R(cb, RecLayout::P, -ex, -ey, -ez);
         double meanP = 0.5 * (PL + PU);
         if (P.algorithm_type == 2) {
             double dk[NS], soret[NS], Fk[NS];
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
-                double XU = R(i, j, k, L.X() + s), XL = R(il, jl, kl, L.X() + s);
+                double XU = R(cb, L.X() + s), XL = R(cb, L.X() + s, -ex, -ey, -ez);
                 double term1 = (XU - XL) / h;
                 double mX = 0.5 * (XL + XU);
-                double mY = 0.5 * (R(il, jl, kl, RecLayout::Y + s) + R(i, j, k, RecLayout::Y + s));
+                double mY = 0.5 * (R(cb, RecLayout::Y + s, -ex, -ey, -ez) + R(cb, RecLayout::Y + s));
                 double term2 = (mX - mY) * (PU - PL) / h / meanP;
                 dk[s] = term1 + term2;
                 double ChiX;
                 if (D == 2)  // flux.cpp:1561 reads chi at k and k+1
-                    ChiX = 0.5 * (R(i, j, k, L.CHI() + s) * XL + R(i, j, k + 1, L.CHI() + s) * XU);
+                    ChiX = 0.5 * (R(cb, L.CHI() + s) * XL + R(cb, L.CHI() + s, 0, 0, 1) * XU);
                 else
-                    ChiX = 0.5 * (R(il, jl, kl, L.CHI() + s) * XL + R(i, j, k, L.CHI() + s) * XU);
+                    ChiX = 0.5 * (R(cb, L.CHI() + s, -ex, -ey, -ez) * XL + R(cb, L.CHI() + s) * XU);
                 soret[s] = ChiX * (TU - TL) / h / meanT;
             }
 #pragma unroll
@@ -449,7 +474,7 @@ __device__ inline void face_flux(const DevParams& P, const Arr& Us, const Arr& R
                 double fk = 0.;
 #pragma unroll
                 for (int ll = 0; ll < NS; ++ll) {
-                    double Fks = half * (R(il, jl, kl, L.D() + ll * NS + kk) + R(i, j, k, L.D() + ll * NS + kk)) *
+                    double Fks = half * (R(cb, L.D() + ll * NS + kk, -ex, -ey, -ez) + R(cb, L.D() + ll * NS + kk)) *
                                  (dk[ll] + soret[ll]);
                     fk = fk - Fks;
                 }
@@ -459,8 +484,8 @@ __device__ inline void face_flux(const DevParams& P, const Arr& Us, const Arr& R
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
                 double hk = P.e0[s] + P.hcp[s] * meanT;
-                double csum = (D == 2) ? R(i, j, k, L.CHI() + s) + R(i, j, k, L.CHI() + s)  // flux.cpp:1604
-                                       : R(il, jl, kl, L.CHI() + s) + R(i, j, k, L.CHI() + s);
+                double csum = (D == 2) ? R(cb, L.CHI() + s) + R(cb, L.CHI() + s)  // flux.cpp:1604
+                                       : R(cb, L.CHI() + s, -ex, -ey, -ez) + R(cb, L.CHI() + s);
                 double Q5s = (hk + 0.5 * P.Runiv * meanT * csum / P.molmass[s]) * Fk[s];
                 Q5 = Q5 + Q5s;
             }
@@ -478,42 +503,41 @@ __device__ inline void face_flux(const DevParams& P, const Arr& Us, const Arr& R
         constexpr int c2 = (D == 0) ? 8 : (D == 1) ? 8 : 0;
         constexpr int a1 = (D == 0) ? 3 : (D == 1) ? 1 : 2;
         constexpr int a2 = (D == 0) ? 6 : (D == 1) ? 7 : 5;
-        double vc4 = node_sum<D>(N, NODE_VC, i, j, k, ih, jh, kh, false, 0., true);
-        double div8 = node_sum<D>(N, c1, i, j, k, ih, jh, kh, zo, 0., true);
-        div8 = node_sum<D>(N, c2, i, j, k, ih, jh, kh, zo, div8, false);
-        double s1 = node_sum<D>(N, a1, i, j, k, ih, jh, kh, zo, 0., true);
-        double s2 = node_sum<D>(N, a2, i, j, k, ih, jh, kh, zo, 0., true);
+        double vc4 = node_sum<D>(N, NODE_VC, cb, false, 0., true);
+        double div8 = node_sum<D>(N, c1, cb, zo, 0., true);
+        div8 = node_sum<D>(N, c2, cb, zo, div8, false);
+        double s1 = node_sum<D>(N, a1, cb, zo, 0., true);
+        double s2 = node_sum<D>(N, a2, cb, zo, 0., true);
         constexpr int cn = 1 + D;
         constexpr int tc1 = (D == 0) ? 2 : 1, tc2 = (D == 2) ? 2 : 3;
         f[cn] = f[cn] - 0.25 * vc4;
         f[cn] = f[cn] + .25 * div8;
         f[tc1] = f[tc1] - .25 * s1;
         f[tc2] = f[tc2] - .25 * s2;
-        double phiflx = 0.25 * (vc4 - div8) * (vel(il, jl, kl, cn) + vel(i, j, k, cn));
-        phiflx = phiflx + .25 * s1 * (vel(il, jl, kl, tc1) + vel(i, j, k, tc1));
-        phiflx = phiflx + .25 * s2 * (vel(il, jl, kl, tc2) + vel(i, j, k, tc2));
+        double phiflx = 0.25 * (vc4 - div8) * (velL(cn) + velU(cn));
+        phiflx = phiflx + .25 * s1 * (velL(tc1) + velU(tc1));
+        phiflx = phiflx + .25 * s2 * (velL(tc2) + velU(tc2));
         f[nv + 1] = f[nv + 1] - 0.5 * phiflx;
     }
 
     // ---- hyperbolic (flux.cpp:2016-2372) ----
     if (P.advection_type == 1 || P.advection_type == 2) {
-        constexpr int ex = (D == 0), ey = (D == 1), ez = (D == 2);
         const double wgt2 = 1. / 12.;
         const double wgt1 = 0.5 + wgt2;
         double cv[NS + 5], pv[NS + 6], Yk[NS];
         if (P.advection_type == 1) {
             // primitive interpolation: rho from cons, the rest from the record
-            auto pr = [&](int a, int b, int cc, int l) {
-                if (l == 0) return Us(a, b, cc, 0);
-                if (l <= 3) return R(a, b, cc, RecLayout::U + l - 1);
-                if (l == 4) return R(a, b, cc, RecLayout::T);
-                if (l == 5) return R(a, b, cc, RecLayout::P);
-                return R(a, b, cc, RecLayout::Y + l - 6);
+            // value l of the primitive vector at cell (i,j,k) + m*e
+            auto pr = [&](int m, int l) {
+                if (l == 0) return Us(cb, 0, m * ex, m * ey, m * ez);
+                if (l <= 3) return R(cb, RecLayout::U + l - 1, m * ex, m * ey, m * ez);
+                if (l == 4) return R(cb, RecLayout::T, m * ex, m * ey, m * ez);
+                if (l == 5) return R(cb, RecLayout::P, m * ex, m * ey, m * ez);
+                return R(cb, RecLayout::Y + l - 6, m * ex, m * ey, m * ez);
             };
 #pragma unroll
             for (int l = 0; l < NS + 6; ++l)
-                pv[l] = wgt1 * (pr(i, j, k, l) + pr(i - ex, j - ey, k - ez, l)) -
-                        wgt2 * (pr(i - 2 * ex, j - 2 * ey, k - 2 * ez, l) + pr(i + ex, j + ey, k + ez, l));
+                pv[l] = wgt1 * (pr(0, l) + pr(-1, l)) - wgt2 * (pr(-2, l) + pr(1, l));
             double rho = pv[0];
             cv[0] = rho;
 #pragma unroll
@@ -524,8 +548,8 @@ __device__ inline void face_flux(const DevParams& P, const Arr& Us, const Arr& R
         } else {
 #pragma unroll
             for (int l = 0; l < NS + 5; ++l)
-                cv[l] = wgt1 * (Us(i, j, k, l) + Us(i - ex, j - ey, k - ez, l)) -
-                        wgt2 * (Us(i - 2 * ex, j - 2 * ey, k - 2 * ez, l) + Us(i + ex, j + ey, k + ez, l));
+                cv[l] = wgt1 * (Us(cb, l) + Us(cb, l, -ex, -ey, -ez)) -
+                        wgt2 * (Us(cb, l, -2 * ex, -2 * ey, -2 * ez) + Us(cb, l, ex, ey, ez));
 #pragma unroll
             for (int l = 1; l < 4; ++l) pv[l] = cv[l] / cv[0];
 #pragma unroll
@@ -566,35 +590,51 @@ __device__ inline void face_flux(const DevParams& P, const Arr& Us, const Arr& R
     for (int n = 0; n < nv; ++n) fo[n] = f[n];
 }
 
+// Store the fluxes of face (i,j,k) of direction D; face 0 is also face n of the periodic box, so
+// the update reads face i+1 without wrapping.
+template <int NS, int D>
+__device__ inline void store_face(const DevParams& P, const Fld& Fo, const double* fo, int i, int j, int k,
+                                  unsigned cb)
+{
+    const int idx = D == 0 ? i : (D == 1 ? j : k);
+    constexpr int ex = (D == 0), ey = (D == 1), ez = (D == 2);
+    const int n = P.n[D];
+#pragma unroll
+    for (int c = 0; c < NS + 5; ++c) Fo(cb, c) = fo[c];
+    if (idx == 0) {
+#pragma unroll
+        for (int c = 0; c < NS + 5; ++c) Fo(cb, c, ex * n, ey * n, ez * n) = fo[c];
+    }
+}
+
 template <int NS, int D, bool INL>
-__device__ inline void face_body(const DevParams& P, const Arr& Us, const Arr& R, const Arr& N, const Arr& A,
-                                 const Arr& B, const NoiseKey& K, const Arr& Fo, double w, double volinv, double dtinv,
-                                 double dt, int i, int j, int k)
+__device__ inline void face_body(const DevParams& P, const Fld& Us, const Fld& R, const Fld& N, const Fld& A,
+                                 const Fld& B, const NoiseKey& K, const Fld& Fo, double w, double volinv, double dtinv,
+                                 double dt, int i, int j, int k, unsigned cb)
 {
     double fo[NS + 5];
-    face_flux<NS, D, INL>(P, Us, R, N, A, B, K, w, volinv, dtinv, dt, i, j, k, fo);
-#pragma unroll
-    for (int n = 0; n < NS + 5; ++n) Fo(i, j, k, n) = fo[n];
+    face_flux<NS, D, INL>(P, Us, R, N, A, B, K, w, volinv, dtinv, dt, i, j, k, cb, fo);
+    store_face<NS, D>(P, Fo, fo, i, j, k, cb);
 }
 
 // ---- stage combination (+ primitive variables after the last stage) -----------------------
 
 // div[] = (fx(i+1)-fx(i))/dx + (fy(j+1)-fy(j))/dy + (fz(k+1)-fz(k))/dz, evaluated left to right.
 template <int NS>
-__device__ inline void update_from_div(const DevParams& P, const Arr& out, const Arr& u0, const Arr& us,
-                                       const double* div, double dt, int mode, const Arr& prim, int write_prim, int i,
-                                       int j, int k)
+__device__ inline void update_from_div(const DevParams& P, const Fld& out, const Fld& u0, const Fld& us,
+                                       const double* div, double dt, int mode, const Fld& prim, int write_prim,
+                                       unsigned cb)
 {
     const double src = 0.0;
     double o[NS + 5];
 #pragma unroll
     for (int n = 0; n < NS + 5; ++n) {
-        if (mode == 0) o[n] = us(i, j, k, n) - dt * div[n] + dt * src;
-        else if (mode == 1) o[n] = 0.25 * (3.0 * u0(i, j, k, n) + us(i, j, k, n) - dt * div[n] + dt * src);
-        else o[n] = (2. / 3.) * (0.5 * u0(i, j, k, n) + us(i, j, k, n) - dt * div[n] + dt * src);
+        if (mode == 0) o[n] = us(cb, n) - dt * div[n] + dt * src;
+        else if (mode == 1) o[n] = 0.25 * (3.0 * u0(cb, n) + us(cb, n) - dt * div[n] + dt * src);
+        else o[n] = (2. / 3.) * (0.5 * u0(cb, n) + us(cb, n) - dt * div[n] + dt * src);
     }
     const double* g = P.grav;
-    double r0 = us(i, j, k, 0), r1 = us(i, j, k, 1), r2 = us(i, j, k, 2), r3 = us(i, j, k, 3);
+    double r0 = us(cb, 0), r1 = us(cb, 1), r2 = us(cb, 2), r3 = us(cb, 3);
     if (mode == 0) {
 #pragma unroll
         for (int n = 0; n < 3; ++n) o[n + 1] += dt * r0 * g[n];
@@ -609,56 +649,27 @@ __device__ inline void update_from_div(const DevParams& P, const Arr& out, const
         o[4] += 2. / 3. * dt * (g[0] * r1 + g[1] * r2 + g[2] * r3);
     }
 #pragma unroll
-    for (int n = 0; n < NS + 5; ++n) out(i, j, k, n) = o[n];
+    for (int n = 0; n < NS + 5; ++n) out(cb, n) = o[n];
     if (write_prim) {
         double pr[2 * NS + 6];
         cons_to_prim_cell<NS>(P, o, pr);
 #pragma unroll
-        for (int n = 0; n < 2 * NS + 6; ++n) prim(i, j, k, n) = pr[n];
+        for (int n = 0; n < 2 * NS + 6; ++n) prim(cb, n) = pr[n];
     }
 }
 
 template <int NS>
-__device__ inline void update_body(const DevParams& P, const Arr& out, const Arr& u0, const Arr& us, const Arr& fx,
-                                   const Arr& fy, const Arr& fz, double dt, int mode, const Arr& prim, int write_prim,
-                                   int i, int j, int k)
+__device__ inline void update_body(const DevParams& P, const Fld& out, const Fld& u0, const Fld& us, const Fld& fx,
+                                   const Fld& fy, const Fld& fz, double dt, int mode, const Fld& prim, int write_prim,
+                                   unsigned cb)
 {
-    const int ih = wrap_up(i, P.n[0]), jh = wrap_up(j, P.n[1]), kh = wrap_up(k, P.n[2]);
     double div[NS + 5];
 #pragma unroll
     for (int n = 0; n < NS + 5; ++n)
-        // face n is face 0 of the periodic box (same inputs, same bits)
-        div[n] = (fx(ih, j, k, n) - fx(i, j, k, n)) / P.dx[0] + (fy(i, jh, k, n) - fy(i, j, k, n)) / P.dx[1] +
-                 (fz(i, j, kh, n) - fz(i, j, k, n)) / P.dx[2];
-    update_from_div<NS>(P, out, u0, us, div, dt, mode, prim, write_prim, i, j, k);
-}
-
-// ---- the noise of one step at cell (i,j,k): its three lower faces and its node ------------
-// (face n and node n of a direction are face/node 0 of the periodic box and are never read)
-
-template <int NV>
-__device__ inline void noise_body(const DevParams& P, const NoiseKey& K, const Arr& A0, const Arr& A1, const Arr& A2,
-                                  const Arr& B0, const Arr& B1, const Arr& B2, const Arr& rA, const Arr& rB, int i,
-                                  int j, int k)
-{
-    const uint32_t lin = (uint32_t)(((long)k * P.n[1] + j) * P.n[0] + i);
-    const Arr* As[3] = {&A0, &A1, &A2};
-    const Arr* Bs[3] = {&B0, &B1, &B2};
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-#pragma unroll
-        for (int c = 1; c < NV; ++c) {
-            const double sc = (c <= 3) ? P.var_mom : (c == 4) ? P.var_ener : P.var_mass;
-            double za, zb;
-            noise_ab(K, (uint32_t)(16 * d + c), lin, za, zb);
-            (*As[d])(i, j, k, c) = za * sc;
-            (*Bs[d])(i, j, k, c) = zb * sc;
-        }
-    }
-    double za, zb;
-    noise_ab(K, NODE_FIELD, lin, za, zb);
-    rA(i, j, k) = za * P.var_mom;
-    rB(i, j, k) = zb * P.var_mom;
+        // face n holds face 0 of the periodic box (store_face)
+        div[n] = (fx(cb, n, 1, 0, 0) - fx(cb, n)) / P.dx[0] + (fy(cb, n, 0, 1, 0) - fy(cb, n)) / P.dx[1] +
+                 (fz(cb, n, 0, 0, 1) - fz(cb, n)) / P.dx[2];
+    update_from_div<NS>(P, out, u0, us, div, dt, mode, prim, write_prim, cb);
 }
 
 }  // namespace fhdx

@@ -35,12 +35,13 @@ _lib = None
 class Tune(C.Structure):
     """``fhdx_tune`` (include/fhdx.h): launch shape of the fast path."""
 
-    _fields_ = [("tile_width", C.c_int32), ("region_tiles", C.c_int32),
-                ("split_faces", C.c_int32), ("pad", C.c_int32), ("ms_per_step", C.c_double)]
+    _fields_ = [("tile_width", C.c_int32), ("region_tiles", C.c_int32), ("z_levels", C.c_int32),
+                ("noise_inline", C.c_int32), ("ms_per_step", C.c_double)]
 
     def as_dict(self):
         return {"tile_width": self.tile_width, "region_tiles": self.region_tiles,
-                "split_faces": self.split_faces, "ms_per_step": self.ms_per_step}
+                "z_levels": self.z_levels, "noise_inline": self.noise_inline,
+                "ms_per_step": self.ms_per_step}
 
 
 class FhdxError(RuntimeError):
@@ -171,8 +172,8 @@ class Context:
                                       C.byref(n), stream), "fhdx_ctx_autotune")
         return t, [trials[q] for q in range(n.value)]
 
-    def set_tune(self, tile_width: int, region_tiles: int, split_faces: int):
-        t = Tune(tile_width, region_tiles, split_faces, 0, 0.0)
+    def set_tune(self, tile_width: int, region_tiles: int = 1, z_levels: int = 1, noise_inline: int = 1):
+        t = Tune(tile_width, region_tiles, z_levels, noise_inline, 0.0)
         check(lib().fhdx_ctx_set_tune(self.h, C.byref(t)), "fhdx_ctx_set_tune")
 
     def get_tune(self) -> Tune:

@@ -178,19 +178,21 @@ int fhdx_rk3_step_injected(fhdx_ctx* ctx, fhdx_fab cu, fhdx_fab prim,
 /* ---- launch-shape tuning ---------------------------------------------------------------- */
 
 /* Launch shape of the fast path: tile width in x (16, 32 or 64 cells; the tile has 256 cells),
- * tiles per region of the region-major block order, and whether the face fluxes run as one
- * kernel for the three directions (0) or one kernel per direction (1). */
+ * tiles per region of the region-major block order, how many z levels a tile block marches
+ * through, and whether the Philox noise is evaluated in place by every stage (1) or written once
+ * per step to arrays (0). */
 typedef struct fhdx_tune {
     int32_t tile_width;
-    int32_t region_tiles;
-    int32_t split_faces;
-    int32_t pad;
+    int32_t region_tiles;     /* 1..64                                                       */
+    int32_t z_levels;         /* 1..64                                                       */
+    int32_t noise_inline;     /* 0 or 1                                                      */
     double ms_per_step;       /* measured by fhdx_ctx_autotune (0 when set by hand)          */
 } fhdx_tune;
 
-/* Times every candidate launch shape on trial_steps fast-path steps of a scratch copy of cu
- * (cu is not modified; HIP events on `stream`), keeps the fastest for this context and returns
- * it in *best; trials (may be NULL) receives up to max_trials timed candidates, n_trials their
+/* Times candidate launch shapes on trial_steps fast-path steps of a scratch copy of cu (cu is
+ * not modified; HIP events on `stream`): one coordinate at a time (tile width, region height,
+ * z levels, noise), each swept with the others at their best so far.  Keeps the
+ * fastest for this context and returns it in *best; trials (may be NULL) receives up to max_trials timed candidates, n_trials their
  * count.  The choice is cached per (species count, box shape) for the process, so later
  * contexts of the same shape reuse it without timing (then n_trials = 0). */
 int fhdx_ctx_autotune(fhdx_ctx* ctx, fhdx_fab cu, double dt, int trial_steps, fhdx_tune* best,
@@ -201,8 +203,8 @@ int fhdx_ctx_get_tune(fhdx_ctx* ctx, fhdx_tune* cfg);
 /* ---- measurement ----------------------------------------------------------------------- */
 
 /* Per-kernel timing of the fast path.  While enabled, fhdx_rk3_step brackets each of its kernel
- * groups (noise, cells, nodes, face_x, face_y, face_z, update) with HIP events recorded on the
- * launch stream; nothing synchronises inside the step.  Enabling (re)sets the totals. */
+ * groups (noise, cells, nodes, faces, update, ghosts) with HIP events recorded on the launch
+ * stream; nothing synchronises inside the step.  Enabling (re)sets the totals. */
 int fhdx_ctx_timing_enable(fhdx_ctx* ctx, int enable);
 /* Number of kernel groups that fhdx_ctx_timing_read reports. */
 int fhdx_timing_kinds(void);

@@ -50,6 +50,8 @@ def lib():
         L.orc_init_cons.argtypes = [P, Fab, C.c_int, C.c_double, C.POINTER(C.c_double),
                                     C.c_double]
         L.orc_num_threads.restype = C.c_int
+        L.orc_set_num_threads.argtypes = [C.c_int]
+        L.orc_set_num_threads.restype = None
         L.orc_evaluate_stats.argtypes = [P, Fab, Fab, Fab, Fab, Fab, Fab, Fab, Fab,
                                          C.POINTER(C.c_double), C.c_int, C.c_int]
         _lib = L
@@ -180,6 +182,10 @@ def normal_pair(words):
     a, b = C.c_double(), C.c_double()
     lib().orc_normal_pair(w, C.byref(a), C.byref(b))
     return a.value, b.value
+
+
+def set_num_threads(n):
+    lib().orc_set_num_threads(int(n))
 
 
 def num_threads():

@@ -1381,6 +1381,15 @@ int orc_advance(const fhdx_params* P, fhdx_fab cu, fhdx_fab prim, int nsteps, do
     return err;
 }
 
+void orc_set_num_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
 int orc_num_threads(void)
 {
 #ifdef _OPENMP

@@ -313,18 +313,18 @@ def test_launch_shapes_give_identical_bits(dev, orc):
     st = perturbed_state(p, cfg, orc, seed=21, amp=0.01, vel=500.0)
     ctx = L.Context(p)
     ref = None
-    for tw in (16, 32, 64):
-        for rh in (1, 3):
-            for split in (0, 1):
-                ctx.set_tune(tw, rh, split)
-                cu = to_dev(torch, fc, st.cu, (-p.ngc,) * 3)
-                prim = fc.MultiFab.cell(p, p.nprimvars)
-                for step in range(2):
-                    fc.rk3_step(ctx, cu, prim, cfg.dt, 9, step)
-                got = (host(cu), host(prim))
-                if ref is None:
-                    ref = got
-                assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), (tw, rh, split)
+    shapes = [(tw, rh, kz, inl) for tw in (16, 32, 64) for rh in (1, 3) for kz in (1, 3)
+              for inl in (0, 1)]
+    for tw, rh, kz, inl in shapes:
+        ctx.set_tune(tw, rh, kz, inl)
+        cu = to_dev(torch, fc, st.cu, (-p.ngc,) * 3)
+        prim = fc.MultiFab.cell(p, p.nprimvars)
+        for step in range(2):
+            fc.rk3_step(ctx, cu, prim, cfg.dt, 9, step)
+        got = (host(cu), host(prim))
+        if ref is None:
+            ref = got
+        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), (tw, rh, kz, inl)
     cu = to_dev(torch, fc, st.cu, (-p.ngc,) * 3)
     best, trials = ctx.autotune(cu.fab, cfg.dt, trial_steps=1)
     assert np.array_equal(host(cu), st.cu)
