@@ -231,7 +231,7 @@ def main():
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--no-configs", action="store_true", help="skip the 32^3 argon and 128^3 four-species rates")
-    ap.add_argument("--shape", type=str, default="", help="launch shape tile_width,region_tiles,z_levels,noise_inline")
+    ap.add_argument("--shape", type=str, default="", help="launch shape tile_width,region_tiles,z_levels,noise_inline[,face_blocks_per_cu]")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates HBM byte counters)")
     a = ap.parse_args()
@@ -292,8 +292,7 @@ def main():
     seed = 1
     tune = None
     if a.shape:
-        tw, rh, kz, inl = (int(v) for v in a.shape.split(","))
-        ctx.set_tune(tw, rh, kz, inl)
+        ctx.set_tune(*(int(v) for v in a.shape.split(",")))
     elif not a.no_tune:
         best, trials = ctx.autotune(cu.fab, cfg.dt, trial_steps=2)
         tune = {"chosen": best.as_dict(), "trials": [t.as_dict() for t in trials]}

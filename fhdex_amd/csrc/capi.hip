@@ -585,13 +585,14 @@ void apply_tune(FusedWorkspace& w, const fhdx_tune& t)
     w.rh = t.region_tiles;
     w.noise_inline = t.noise_inline != 0;
     w.kz = t.z_levels;
+    w.face_bpc = t.face_blocks_per_cu;
 }
 
 bool valid_tune(const fhdx_tune& t)
 {
     return (t.tile_width == 16 || t.tile_width == 32 || t.tile_width == 64) && t.region_tiles >= 1 &&
            t.region_tiles <= 64 && (t.noise_inline == 0 || t.noise_inline == 1) && t.z_levels >= 1 &&
-           t.z_levels <= 64;
+           t.z_levels <= 64 && (t.face_blocks_per_cu == 0 || t.face_blocks_per_cu == 2 || t.face_blocks_per_cu == 3);
 }
 
 }  // namespace
@@ -600,7 +601,7 @@ int fhdx_ctx_set_tune(fhdx_ctx* c, const fhdx_tune* t)
 {
     if (!c || !t || !valid_tune(*t)) {
         set_error("fhdx_ctx_set_tune: tile_width must be 16, 32 or 64, region_tiles and z_levels 1..64, "
-                  "noise_inline 0/1");
+                  "noise_inline 0/1, face_blocks_per_cu 0, 2 or 3");
         return FHDX_ERR_ARG;
     }
     apply_tune(c->fused, *t);
@@ -614,6 +615,8 @@ int fhdx_ctx_get_tune(fhdx_ctx* c, fhdx_tune* t)
     t->region_tiles = c->fused.rh;
     t->noise_inline = c->fused.noise_inline ? 1 : 0;
     t->z_levels = c->fused.kz;
+    t->face_blocks_per_cu = c->fused.face_bpc;
+    t->pad = 0;
     t->ms_per_step = 0.;
     return FHDX_OK;
 }
@@ -679,23 +682,24 @@ int fhdx_ctx_autotune(fhdx_ctx* c, fhdx_fab cu, double dt, int trial_steps, fhdx
         return q;
     };
     // coordinate descent from the default shape: each parameter swept with the others fixed
-    fhdx_tune cur{64, 1, 1, 1, 0.};
+    fhdx_tune cur{64, 1, 1, 1, 0, 0, 0.};
     if (!r) {
         int q = trial(cur);
         if (q) fail(q, "trial step failed");
     }
     top = cur;
     const int nz = c->p.n_cells[2];
-    for (int dim = 0; dim < 4 && !r; ++dim) {
+    for (int dim = 0; dim < 5 && !r; ++dim) {
         std::vector<int> vals;
         if (dim == 0) vals = {64, 32, 16};
         if (dim == 1) vals = {1, 2, 4, 8};
         if (dim == 2) vals = {1, 2, 4, 8, 16};
         if (dim == 3) vals = {1, 0};
+        if (dim == 4) vals = {0, 3, 2};
         for (int v : vals) {
             fhdx_tune t = top;
             int32_t* field = dim == 0 ? &t.tile_width : dim == 1 ? &t.region_tiles : dim == 2 ? &t.z_levels
-                                                                                        : &t.noise_inline;
+                           : dim == 3 ? &t.noise_inline : &t.face_blocks_per_cu;
             if (*field == v) continue;
             if (dim == 0 && v > 16 && (v >> 1) >= c->p.n_cells[0]) continue;  // same tiles as a narrower one
             if (dim == 2 && v > nz) continue;

@@ -71,6 +71,7 @@ struct FusedWorkspace {
     int rh = 1;                // tiles per region of the region-major block order
     int kz = 1;                // z levels each tile block marches through
     bool noise_inline = true;  // Philox noise evaluated in the faces/nodes, or once into arrays
+    int face_bpc = 0;          // cap on resident k_faces blocks per CU (0: none)
     Timing timing;
 };
 

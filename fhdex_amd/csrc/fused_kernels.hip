@@ -314,8 +314,10 @@ static void launch_stage(const DevParams& P, FusedWorkspace& w, const Grid& G, d
     fa.dtinv = 1. / dt;
     fa.dt = dt;
     T.begin(KT_FACES, s);
-    if (zm) k_faces<NS, INL, true><<<ntile, FB, 0, s>>>(P, tm, G, fa);
-    else k_faces<NS, INL, false><<<ntile, FB, 0, s>>>(P, tm, G, fa);
+    // a block cap per CU is imposed by reserving LDS the kernel does not use (160 KiB per CU)
+    const unsigned lds = w.face_bpc == 2 ? 64u << 10 : (w.face_bpc == 3 ? 48u << 10 : 0u);
+    if (zm) k_faces<NS, INL, true><<<ntile, FB, lds, s>>>(P, tm, G, fa);
+    else k_faces<NS, INL, false><<<ntile, FB, lds, s>>>(P, tm, G, fa);
     T.end(s);
     T.begin(KT_UPDATE, s);
     if (zm)

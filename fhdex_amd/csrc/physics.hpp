@@ -432,7 +432,77 @@ __host__ __device__ inline U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1)
     return c;
 }
 
-// Two N(0,1) values from one Philox block: 53-bit u1 in (0,1), u2 in [0,1).
+// Box-Muller with its own elementary functions, written with IEEE +, -, *, /, fma and sqrt only,
+// so that the oracle (oracle.c:orc_normal_pair) evaluates the identical operation sequence and
+// gets the same bits on the host.  Accuracy: both series are truncated below 2e-18 relative.
+
+// ln x for x in (0, 1]: x = m 2^e with m in [sqrt(1/2), sqrt(2)); ln x = e ln2 + 2 atanh(s),
+// s = (m-1)/(m+1) (|s| <= 0.1716), atanh(s)/s = sum_k s^(2k)/(2k+1), k = 0..11.
+__device__ inline double bm_log(double x)
+{
+    const uint64_t b = (uint64_t)__double_as_longlong(x);
+    int e = (int)((b >> 52) & 0x7ff) - 1023;
+    double m = __longlong_as_double((long long)((b & 0x000fffffffffffffull) | 0x3ff0000000000000ull));
+    if (m > 1.4142135623730951) {
+        m = m * 0.5;
+        e = e + 1;
+    }
+    const double sr = (m - 1.0) / (m + 1.0);
+    const double z = sr * sr;
+    double p = 1.0 / 23.0;
+    p = fma(p, z, 1.0 / 21.0);
+    p = fma(p, z, 1.0 / 19.0);
+    p = fma(p, z, 1.0 / 17.0);
+    p = fma(p, z, 1.0 / 15.0);
+    p = fma(p, z, 1.0 / 13.0);
+    p = fma(p, z, 1.0 / 11.0);
+    p = fma(p, z, 1.0 / 9.0);
+    p = fma(p, z, 1.0 / 7.0);
+    p = fma(p, z, 1.0 / 5.0);
+    p = fma(p, z, 1.0 / 3.0);
+    p = fma(p, z, 1.0);
+    const double lm = 2.0 * sr * p;
+    const double de = (double)e;
+    // ln2 = LN2_HI + LN2_LO; LN2_HI has 21 significant bits, so de * LN2_HI is exact
+    return de * 0.6931467056274414 + fma(de, 4.7493250390316726e-07, lm);
+}
+
+// sin(pi t) and cos(pi t) for t in [0, 2): t = n/2 + r with |r| <= 1/4 (exact), Taylor series of
+// sin(pi r) to r^19 and cos(pi r) to r^18, then the quadrant n mod 4.
+__device__ inline void bm_sincospi(double t, double& sn, double& cs)
+{
+    const double n = rint(2.0 * t);
+    const double r = t - 0.5 * n;
+    const double r2 = r * r;
+    double ps = -2.2948428997269873e-08;
+    ps = fma(ps, r2, 7.952054001475513e-07);
+    ps = fma(ps, r2, -2.1915353447830217e-05);
+    ps = fma(ps, r2, 0.00046630280576761255);
+    ps = fma(ps, r2, -0.0073704309457143504);
+    ps = fma(ps, r2, 0.08214588661112823);
+    ps = fma(ps, r2, -0.5992645293207921);
+    ps = fma(ps, r2, 2.5501640398773455);
+    ps = fma(ps, r2, -5.16771278004997);
+    ps = fma(ps, r2, 3.141592653589793);
+    const double S = r * ps;
+    double pc = -1.3878952462213771e-07;
+    pc = fma(pc, r2, 4.303069587032947e-06);
+    pc = fma(pc, r2, -0.0001046381049248457);
+    pc = fma(pc, r2, 0.0019295743094039231);
+    pc = fma(pc, r2, -0.02580689139001406);
+    pc = fma(pc, r2, 0.2353306303588932);
+    pc = fma(pc, r2, -1.3352627688545895);
+    pc = fma(pc, r2, 4.0587121264167685);
+    pc = fma(pc, r2, -4.934802200544679);
+    const double C = fma(pc, r2, 1.0);
+    // quadrant q = n mod 4: (sin, cos) = (S, C), (C, -S), (-S, -C), (-C, S)
+    const int q = ((int)n) & 3;
+    const double a = (q & 1) ? C : S, b = (q & 1) ? S : C;
+    sn = (q & 2) ? -a : a;
+    cs = ((q + 1) & 2) ? -b : b;
+}
+
+// Two N(0,1) values from one Philox block: 53-bit u1 in (0,1], u2 in [0,1).
 __device__ inline void normal_pair(U4 r, double& z0, double& z1)
 {
     const double two_m53 = 1.1102230246251565e-16;
@@ -440,9 +510,9 @@ __device__ inline void normal_pair(U4 r, double& z0, double& z1)
     uint64_t b = ((((uint64_t)r.z) << 32) | r.w) >> 11;
     double u1 = ((double)a + 0.5) * two_m53;
     double u2 = (double)b * two_m53;
-    double rad = sqrt(-2.0 * log(u1));
+    double rad = sqrt(-2.0 * bm_log(u1));
     double s, c;
-    sincospi(2.0 * u2, &s, &c);
+    bm_sincospi(2.0 * u2, s, c);
     z0 = rad * c;
     z1 = rad * s;
 }

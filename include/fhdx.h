@@ -179,19 +179,22 @@ int fhdx_rk3_step_injected(fhdx_ctx* ctx, fhdx_fab cu, fhdx_fab prim,
 
 /* Launch shape of the fast path: tile width in x (16, 32 or 64 cells; the tile has 256 cells),
  * tiles per region of the region-major block order, how many z levels a tile block marches
- * through, and whether the Philox noise is evaluated in place by every stage (1) or written once
- * per step to arrays (0). */
+ * through, whether the Philox noise is evaluated in place by every stage (1) or written once per
+ * step to arrays (0), and an optional cap on resident face-kernel blocks per CU (fewer blocks
+ * in flight keep more of their neighbours' data in L2). */
 typedef struct fhdx_tune {
     int32_t tile_width;
     int32_t region_tiles;     /* 1..64                                                       */
     int32_t z_levels;         /* 1..64                                                       */
     int32_t noise_inline;     /* 0 or 1                                                      */
+    int32_t face_blocks_per_cu; /* 0: as many as registers allow; 2 or 3: cap (reserves LDS) */
+    int32_t pad;
     double ms_per_step;       /* measured by fhdx_ctx_autotune (0 when set by hand)          */
 } fhdx_tune;
 
 /* Times candidate launch shapes on trial_steps fast-path steps of a scratch copy of cu (cu is
  * not modified; HIP events on `stream`): one coordinate at a time (tile width, region height,
- * z levels, noise), each swept with the others at their best so far.  Keeps the
+ * z levels, noise, face blocks per CU), each swept with the others at their best so far.  Keeps the
  * fastest for this context and returns it in *best; trials (may be NULL) receives up to max_trials timed candidates, n_trials their
  * count.  The choice is cached per (species count, box shape) for the process, so later
  * contexts of the same shape reuse it without timing (then n_trials = 0). */

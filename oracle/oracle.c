@@ -1026,7 +1026,76 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
-/* u1 in (0,1) from words 0,1; u2 in [0,1) from words 2,3 (53 bits each);
+/* Box-Muller with elementary functions built from IEEE +, -, *, /, fma and sqrt, in the same
+ * operation order as the device generator (fhdex_amd/csrc/physics.hpp), so host and device
+ * noise agree bit for bit.  ln x, x in (0,1]: x = m 2^e, m in [sqrt(1/2), sqrt(2)),
+ * ln x = e ln2 + 2 atanh(s), s = (m-1)/(m+1), atanh series to s^23. */
+static double bm_log(double x)
+{
+    uint64_t b;
+    memcpy(&b, &x, sizeof b);
+    int e = (int)((b >> 52) & 0x7ff) - 1023;
+    uint64_t mb = (b & 0x000fffffffffffffull) | 0x3ff0000000000000ull;
+    double m;
+    memcpy(&m, &mb, sizeof m);
+    if (m > 1.4142135623730951) {
+        m = m * 0.5;
+        e = e + 1;
+    }
+    const double sr = (m - 1.0) / (m + 1.0);
+    const double z = sr * sr;
+    double p = 1.0 / 23.0;
+    p = fma(p, z, 1.0 / 21.0);
+    p = fma(p, z, 1.0 / 19.0);
+    p = fma(p, z, 1.0 / 17.0);
+    p = fma(p, z, 1.0 / 15.0);
+    p = fma(p, z, 1.0 / 13.0);
+    p = fma(p, z, 1.0 / 11.0);
+    p = fma(p, z, 1.0 / 9.0);
+    p = fma(p, z, 1.0 / 7.0);
+    p = fma(p, z, 1.0 / 5.0);
+    p = fma(p, z, 1.0 / 3.0);
+    p = fma(p, z, 1.0);
+    const double lm = 2.0 * sr * p;
+    const double de = (double)e;
+    return de * 0.6931467056274414 + fma(de, 4.7493250390316726e-07, lm);
+}
+
+/* sin(pi t), cos(pi t), t in [0,2): t = n/2 + r, |r| <= 1/4; Taylor series in r, quadrant n mod 4. */
+static void bm_sincospi(double t, double* sn, double* cs)
+{
+    const double n = rint(2.0 * t);
+    const double r = t - 0.5 * n;
+    const double r2 = r * r;
+    double ps = -2.2948428997269873e-08;
+    ps = fma(ps, r2, 7.952054001475513e-07);
+    ps = fma(ps, r2, -2.1915353447830217e-05);
+    ps = fma(ps, r2, 0.00046630280576761255);
+    ps = fma(ps, r2, -0.0073704309457143504);
+    ps = fma(ps, r2, 0.08214588661112823);
+    ps = fma(ps, r2, -0.5992645293207921);
+    ps = fma(ps, r2, 2.5501640398773455);
+    ps = fma(ps, r2, -5.16771278004997);
+    ps = fma(ps, r2, 3.141592653589793);
+    const double S = r * ps;
+    double pc = -1.3878952462213771e-07;
+    pc = fma(pc, r2, 4.303069587032947e-06);
+    pc = fma(pc, r2, -0.0001046381049248457);
+    pc = fma(pc, r2, 0.0019295743094039231);
+    pc = fma(pc, r2, -0.02580689139001406);
+    pc = fma(pc, r2, 0.2353306303588932);
+    pc = fma(pc, r2, -1.3352627688545895);
+    pc = fma(pc, r2, 4.0587121264167685);
+    pc = fma(pc, r2, -4.934802200544679);
+    const double C = fma(pc, r2, 1.0);
+    /* quadrant q = n mod 4: (sin, cos) = (S, C), (C, -S), (-S, -C), (-C, S) */
+    const int q = ((int)n) & 3;
+    const double a = (q & 1) ? C : S, b = (q & 1) ? S : C;
+    *sn = (q & 2) ? -a : a;
+    *cs = ((q + 1) & 2) ? -b : b;
+}
+
+/* u1 in (0,1] from words 0,1; u2 in [0,1) from words 2,3 (53 bits each);
  * z0 = sqrt(-2 ln u1) cos(2 pi u2), z1 = sqrt(-2 ln u1) sin(2 pi u2). */
 void orc_normal_pair(const uint32_t r[4], double* z0, double* z1)
 {
@@ -1035,10 +1104,11 @@ void orc_normal_pair(const uint32_t r[4], double* z0, double* z1)
     uint64_t b = (((uint64_t)r[2] << 32) | r[3]) >> 11;
     double u1 = ((double)a + 0.5) * two_m53;
     double u2 = (double)b * two_m53;
-    double rad = sqrt(-2.0 * log(u1));
-    double th = 6.283185307179586 * u2;
-    *z0 = rad * cos(th);
-    *z1 = rad * sin(th);
+    double rad = sqrt(-2.0 * bm_log(u1));
+    double s, c;
+    bm_sincospi(2.0 * u2, &s, &c);
+    *z0 = rad * c;
+    *z1 = rad * s;
 }
 
 int orc_fill_noise(const fhdx_params* P, uint64_t seed, uint64_t step, uint32_t instance,

@@ -59,10 +59,10 @@ def test_struct_layouts_match_header(tmp_path):
         assert getattr(P.Fab, f).offset == offs[len(fields) + i], f
 
 
-def test_serving_struct_layouts_match_header(tmp_path):
-    """fhdx_request / fhdx_completion: ctypes offsets equal offsetof."""
+def test_serving_and_tune_struct_layouts_match_header(tmp_path):
+    """fhdx_request / fhdx_completion / fhdx_tune: ctypes offsets equal offsetof."""
     from fhdex_amd import serving as S
-    structs = [("fhdx_request", S.Request), ("fhdx_completion", S.Completion)]
+    structs = [("fhdx_request", S.Request), ("fhdx_completion", S.Completion), ("fhdx_tune", L.Tune)]
     src = ["#include <stdio.h>", "#include <stddef.h>", '#include "fhdx.h"', "int main(void){"]
     for cname, py in structs:
         src.append(f'printf("%zu\\n", sizeof({cname}));')

@@ -42,6 +42,28 @@ def test_normal_pair_moments(orc):
     assert abs(np.corrcoef(z[0::2], z[1::2])[0, 1]) < 5 / math.sqrt(n)
 
 
+def test_normal_transform_matches_libm_box_muller(orc):
+    """The generator's own ln / sincospi (IEEE ops only, shared bit for bit with the device)
+    against Box-Muller evaluated with the C library's log, cos and sin."""
+    rng = np.random.default_rng(1)
+    worst = 0.0
+    for _ in range(20000):
+        w = [int(x) for x in rng.integers(0, 2 ** 32, size=4)]
+        z0, z1 = orc.normal_pair(w)
+        u1 = ((((w[0] << 32) | w[1]) >> 11) + 0.5) * 2.0 ** -53
+        u2 = (((w[2] << 32) | w[3]) >> 11) * 2.0 ** -53
+        r = math.sqrt(-2 * math.log(u1))
+        worst = max(worst, abs(z0 - r * math.cos(2 * math.pi * u2)), abs(z1 - r * math.sin(2 * math.pi * u2)))
+    assert worst < 5e-15
+    # u1 = 1 gives zero; quarter turns of u2 put the radius on one axis
+    assert orc.normal_pair([0xFFFFFFFF] * 4) == (0.0, 0.0)
+    r0 = math.sqrt(-2 * math.log(0.5 * 2.0 ** -53))
+    z0, z1 = orc.normal_pair([0, 0, 0x40000000, 0])
+    assert z0 == 0.0 and abs(z1 - r0) < 1e-14
+    z0, z1 = orc.normal_pair([0, 0, 0x80000000, 0])
+    assert abs(z0 + r0) < 1e-14 and z1 == 0.0
+
+
 def test_noise_duplicate_faces_equal(orc):
     """Face n_d and face 0 (and node n and node 0) carry the same value, as after
     OverrideSync (MultiFabFillRandom.cpp:25); A and B sets differ."""
