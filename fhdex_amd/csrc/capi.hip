@@ -133,8 +133,8 @@ int fhdx_params_check(const fhdx_params* p)
         return FHDX_ERR_UNSUPPORTED;
     }
     for (int d = 0; d < 3; ++d)
-        if (p->ngc > p->n_cells[d]) {
-            set_error("ngc must not exceed n_cells (periodic images wrap once)");
+        if (2 * p->ngc > p->n_cells[d]) {
+            set_error("n_cells must be at least 2 ngc (each cell has at most one periodic image per direction)");
             return FHDX_ERR_UNSUPPORTED;
         }
     {

@@ -47,7 +47,7 @@ template <int NS>
 __global__ void __launch_bounds__(EB) kb_cells(const Inst* __restrict__ I, const WorkItem* __restrict__ W, int st)
 {
     EB_CELL_PROLOGUE
-    cell_body<NS>(in.P, G.fld(stage_in(in, st)), G.fld(in.R), in.err, G.off(i, j, k));
+    cell_body<NS>(in.P, G.fld(stage_in(in, st)), G.fld(in.R), in.err, i, j, k, G.off(i, j, k));
 }
 
 __global__ void __launch_bounds__(EB) kb_nodes(const Inst* __restrict__ I, const WorkItem* __restrict__ W, double w)
@@ -80,24 +80,16 @@ __global__ void __launch_bounds__(EB) kb_update(const Inst* __restrict__ I, cons
 {
     EB_CELL_PROLOGUE
     update_body<NS>(in.P, G.fld(stage_out(in, st)), G.fld(in.U), G.fld(stage_in(in, st)), G.fld(in.F[0]),
-                    G.fld(in.F[1]), G.fld(in.F[2]), in.dt, st, G.fld(in.prim), st == 2, G.off(i, j, k));
+                    G.fld(in.F[1]), G.fld(in.F[2]), in.dt, st, G.fld(in.prim), st == 2, i, j, k, G.off(i, j, k));
 }
 
-// which: 0 record, 1 the stage's output state (-1: the step's input state), 2 prim
-__global__ void __launch_bounds__(EB) kb_ghosts(const Inst* __restrict__ I, const WorkItem* __restrict__ W,
-                                                int which, int st)
+// Periodic ghost fill of one instance's state at admission; from then on every kernel that
+// writes a cell array also writes the cell's ghost images (store_cell).
+__global__ void __launch_bounds__(EB) kb_fill_state(Grid G, double* U, int nc, int g, int n0, int n1, int n2,
+                                                    long nghost)
 {
-    const WorkItem wi = W[blockIdx.x];
-    const Inst& in = I[wi.inst];
-    const long t = (long)wi.chunk * EB + threadIdx.x;
-    if (t >= in.nghost || in.step >= in.step_end) return;
-    const RecLayout L{in.P.ns};
-    const Grid G = in.G;
-    if (which == 0) ghost_body(G.view(in.R), L.ncomp(), in.P.ngc, in.P.n[0], in.P.n[1], in.P.n[2], t);
-    else if (which == 1)
-        ghost_body(G.view(st < 0 ? in.U : stage_out(in, st)), in.P.nvars, in.P.ngc, in.P.n[0], in.P.n[1], in.P.n[2],
-                   t);
-    else ghost_body(G.view(in.prim), in.P.nprim, in.P.ngc, in.P.n[0], in.P.n[1], in.P.n[2], t);
+    const long t = (long)blockIdx.x * EB + threadIdx.x;
+    if (t < nghost) ghost_body(G.view(U), nc, g, n0, n1, n2, t);
 }
 
 __global__ void kb_advance(Inst* I, const int* act, int n)
@@ -185,9 +177,8 @@ static void drop_tables(Engine* e)
 {
     for (int g = 0; g < 3; ++g) {
         if (e->d_cells[g]) (void)hipFree(e->d_cells[g]);
-        if (e->d_ghosts[g]) (void)hipFree(e->d_ghosts[g]);
-        e->d_cells[g] = e->d_ghosts[g] = nullptr;
-        e->ncells_items[g] = e->nghost_items[g] = 0;
+        e->d_cells[g] = nullptr;
+        e->ncells_items[g] = 0;
     }
 }
 
@@ -270,6 +261,9 @@ int engine_add(Engine* e, const fhdx_params& p, double dt, uint64_t seed, uint64
         for (int c = 0; c < nv && !r; ++c) r = fill_value(d.U + c * G.sn, G.sn, uniform[c], e->stream);
         if (r) return r;
     }
+    kb_fill_state<<<(unsigned)((d.nghost + EB - 1) / EB), EB, 0, e->stream>>>(G, d.U, nv, p.ngc, p.n_cells[0],
+                                                                               p.n_cells[1], p.n_cells[2], d.nghost);
+    FHDX_HIP_TRY(hipGetLastError());
     FHDX_HIP_TRY(hipMemcpyAsync(e->d_inst + sid, &d, sizeof(Inst), hipMemcpyHostToDevice, e->stream));
     FHDX_HIP_TRY(hipMemsetAsync(e->d_err + sid, 0, sizeof(int), e->stream));
     FHDX_HIP_TRY(hipStreamSynchronize(e->stream));
@@ -294,25 +288,18 @@ static int rebuild(Engine* e)
     drop_tables(e);
     std::vector<int> act;
     for (int g = 0; g < 3; ++g) {
-        std::vector<WorkItem> cells, ghosts;
+        std::vector<WorkItem> cells;
         for (int q = 0; q < e->max_inst; ++q) {
             const EngineSlot& s = e->slots[q];
             if (!s.used || s.p.nspecies != g + 2) continue;
-            const long nc = (s.desc.ncell + EB - 1) / EB, ng = (s.desc.nghost + EB - 1) / EB;
+            const long nc = (s.desc.ncell + EB - 1) / EB;
             for (long c = 0; c < nc; ++c) cells.push_back({q, (int)c});
-            for (long c = 0; c < ng; ++c) ghosts.push_back({q, (int)c});
             act.push_back(q);
         }
         e->ncells_items[g] = (int)cells.size();
-        e->nghost_items[g] = (int)ghosts.size();
         if (!cells.empty()) {
             FHDX_HIP_TRY(hipMalloc((void**)&e->d_cells[g], sizeof(WorkItem) * cells.size()));
             FHDX_HIP_TRY(hipMemcpy(e->d_cells[g], cells.data(), sizeof(WorkItem) * cells.size(), hipMemcpyHostToDevice));
-        }
-        if (!ghosts.empty()) {
-            FHDX_HIP_TRY(hipMalloc((void**)&e->d_ghosts[g], sizeof(WorkItem) * ghosts.size()));
-            FHDX_HIP_TRY(
-                hipMemcpy(e->d_ghosts[g], ghosts.data(), sizeof(WorkItem) * ghosts.size(), hipMemcpyHostToDevice));
         }
     }
     e->nactive = (int)act.size();
@@ -326,20 +313,17 @@ template <int NS>
 static void enqueue_group(Engine* e, int g, const double ws[3])
 {
     hipStream_t s = e->stream;
-    const unsigned nc = (unsigned)e->ncells_items[g], ng = (unsigned)e->nghost_items[g];
+    const unsigned nc = (unsigned)e->ncells_items[g];
     if (!nc) return;
     const Inst* I = e->d_inst;
-    const WorkItem *Wc = e->d_cells[g], *Wg = e->d_ghosts[g];
-    kb_ghosts<<<ng, EB, 0, s>>>(I, Wg, 1, -1);
+    const WorkItem* Wc = e->d_cells[g];
+    // ghost images come with every store (store_cell): no fill launches inside the step
     for (int st = 0; st < 3; ++st) {
         kb_cells<NS><<<nc, EB, 0, s>>>(I, Wc, st);
-        kb_ghosts<<<ng, EB, 0, s>>>(I, Wg, 0, st);
         kb_nodes<<<nc, EB, 0, s>>>(I, Wc, ws[st]);
         kb_faces<NS><<<nc, EB, 0, s>>>(I, Wc, st, ws[st]);
         kb_update<NS><<<nc, EB, 0, s>>>(I, Wc, st);
-        kb_ghosts<<<ng, EB, 0, s>>>(I, Wg, 1, st);
     }
-    kb_ghosts<<<ng, EB, 0, s>>>(I, Wg, 2, 0);
 }
 
 static int enqueue_step(Engine* e)

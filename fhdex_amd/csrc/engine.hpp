@@ -51,8 +51,7 @@ struct Engine {
     hipStream_t stream = nullptr;
     // work tables per species count (index ns-2): cell chunks and ghost chunks
     WorkItem* d_cells[3] = {nullptr, nullptr, nullptr};
-    WorkItem* d_ghosts[3] = {nullptr, nullptr, nullptr};
-    int ncells_items[3] = {0, 0, 0}, nghost_items[3] = {0, 0, 0};
+    int ncells_items[3] = {0, 0, 0};
     int* d_active = nullptr;  // active slot list for the step counter kernel
     int nactive = 0;
     bool dirty = true;        // membership changed: rebuild tables, recapture the graph

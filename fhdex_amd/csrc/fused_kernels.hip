@@ -11,8 +11,9 @@
 //              per-component order with the energy fold;
 //   k_update   divergence + RK3 stage combination (+ primitive variables after stage 3).
 // Every array of the box shares one layout (Grid), so a kernel computes each cell offset once
-// and addresses all arrays as scalar base + 32-bit offset.  Only ghost cells are touched by the
-// periodic fills.  Every value is computed with the reference's operation order, so the step is
+// and addresses all arrays as scalar base + 32-bit offset.  Records, stage states and prim are
+// written with their periodic ghost images by the producing kernel; only the caller's cu gets
+// a ghost fill, once per step.  Every value is computed with the reference's operation order, so the step is
 // bit-identical to the reference-shaped path.
 #include "fused.hpp"
 #include "kernels.hpp"
@@ -117,7 +118,7 @@ __global__ void __launch_bounds__(FB) k_cells(DevParams P, Grid G, double* U, do
     if (t >= (long)P.n[0] * P.n[1] * P.n[2]) return;
     int i, j, k;
     decode3(t, P.n[0], P.n[1], i, j, k);
-    cell_body<NS>(P, G.fld(U), G.fld(R), err, G.off(i, j, k));
+    cell_body<NS>(P, G.fld(U), G.fld(R), err, i, j, k, G.off(i, j, k));
 }
 
 // ---- tiles: 2-D (x, y) blocks of 256 threads, one z level per block, region-major order ----
@@ -220,7 +221,7 @@ __global__ void __launch_bounds__(FB) k_update(DevParams P, TileMap tm, Grid G, 
                                                double* prim, int write_prim)
 {
     TILE_LEVELS(tm, update_body<NS>(P, G.fld(out), G.fld(u0), G.fld(us), G.fld(fx), G.fld(fy), G.fld(fz), dt, mode,
-                                    G.fld(prim), write_prim, G.off(i, j, k));)
+                                    G.fld(prim), write_prim, i, j, k, G.off(i, j, k));)
 }
 
 // ---- host side ---------------------------------------------------------------------------
@@ -291,8 +292,7 @@ static void launch_stage(const DevParams& P, FusedWorkspace& w, const Grid& G, d
     const unsigned ntile = tm.blocks();
     Timing& T = w.timing;
     T.begin(KT_CELLS, s);
-    k_cells<NS><<<grid_for(ncell, FB), FB, 0, s>>>(P, G, us, w.rec, d_err);
-    fill_ghosts(P, G, w.rec, L.ncomp(), s);
+    k_cells<NS><<<grid_for(ncell, FB), FB, 0, s>>>(P, G, us, w.rec, d_err);  // records + ghost images
     T.end(s);
     const bool zm = tm.kz > 1;
     T.begin(KT_NODES, s);
@@ -325,9 +325,6 @@ static void launch_stage(const DevParams& P, FusedWorkspace& w, const Grid& G, d
     else
         k_update<NS, false><<<ntile, FB, 0, s>>>(P, tm, G, out, cu, us, w.F[0], w.F[1], w.F[2], dt, st, prim,
                                                  st == 2);
-    T.end(s);
-    T.begin(KT_GHOSTS, s);
-    fill_ghosts(P, G, out, P.nvars, s);
     T.end(s);
 }
 
@@ -393,10 +390,7 @@ int fused_rk3_step(const fhdx_params& p, const DevParams& P, FusedWorkspace& w, 
         FHDX_HIP_TRY(hipGetLastError());
     }
 #undef FHDX_STAGE
-    w.timing.begin(KT_GHOSTS, s);
-    r = fill_ghosts(P, G, prim.p, P.nprim, s);
-    w.timing.end(s);
-    return r;
+    return FHDX_OK;
 }
 
 }  // namespace fhdx

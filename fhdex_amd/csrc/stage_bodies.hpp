@@ -60,6 +60,30 @@ inline Grid grid_of(const DevParams& P)
     return G;
 }
 
+// ---- stores with periodic images ------------------------------------------------------------
+
+// Stores value(c), c < NC, for valid cell (i,j,k) at byte offset cb and at each periodic image of
+// the cell that lies in the ngc ghost layers (up to 7), so no separate ghost-fill pass is needed.
+// Requires n >= 2 ngc on every axis (a cell then has at most one image per axis).
+template <int NC, typename V>
+__device__ inline void store_cell(const DevParams& P, const Fld& F, V value, int i, int j, int k, unsigned cb)
+{
+#pragma unroll
+    for (int c = 0; c < NC; ++c) F(cb, c) = value(c);
+    const int g = P.ngc;
+    const int ax = i < g ? P.n[0] : (i >= P.n[0] - g ? -P.n[0] : 0);
+    const int ay = j < g ? P.n[1] : (j >= P.n[1] - g ? -P.n[1] : 0);
+    const int az = k < g ? P.n[2] : (k >= P.n[2] - g ? -P.n[2] : 0);
+    if ((ax | ay | az) == 0) return;
+    for (int m = 1; m < 8; ++m) {
+        if (((m & 1) && !ax) || ((m & 2) && !ay) || ((m & 4) && !az)) continue;
+        const int d = ((m & 1) ? ax : 0) + ((m & 2) ? ay : 0) * F.sy + ((m & 4) ? az : 0) * F.sz;
+        const unsigned cbi = cb + (unsigned)(d * 8);  // modulo 2^32: negative shifts wrap correctly
+#pragma unroll
+        for (int c = 0; c < NC; ++c) F(cbi, c) = value(c);
+    }
+}
+
 // ---- periodic ghost fill: thread t of the G^3 - n^3 ghost cells ---------------------------
 
 __device__ inline void ghost_body(const Arr& a, int nc, int g, int n0, int n1, int n2, long t)
@@ -95,7 +119,8 @@ __device__ inline void ghost_body(const Arr& a, int nc, int g, int n0, int n1, i
 // ---- conservedToPrimitive + transport coefficients into the cell record -------------------
 
 template <int NS>
-__device__ inline void cell_body(const DevParams& P, const Fld& U, const Fld& R, int* err, unsigned cb)
+__device__ inline void cell_body(const DevParams& P, const Fld& U, const Fld& R, int* err, int i, int j, int k,
+                                 unsigned cb)
 {
     const RecLayout L{NS};
     double c[NS + 5], pr[2 * NS + 6];
@@ -114,23 +139,25 @@ __device__ inline void cell_body(const DevParams& P, const Fld& U, const Fld& R,
     for (int s = 0; s < NS; ++s) sumy += yn[s];
 #pragma unroll
     for (int s = 0; s < NS; ++s) yn[s] /= sumy;
-    R(cb, RecLayout::U) = pr[1];
-    R(cb, RecLayout::V) = pr[2];
-    R(cb, RecLayout::W) = pr[3];
-    R(cb, RecLayout::T) = pr[4];
-    R(cb, RecLayout::P) = pr[5];
-    R(cb, RecLayout::ETA) = tr.eta;
-    R(cb, RecLayout::KAP) = tr.kappa;
-    R(cb, RecLayout::ZETA) = tr.zeta;
+    double rec[8 + 4 * NS + NS * NS];
+    rec[RecLayout::U] = pr[1];
+    rec[RecLayout::V] = pr[2];
+    rec[RecLayout::W] = pr[3];
+    rec[RecLayout::T] = pr[4];
+    rec[RecLayout::P] = pr[5];
+    rec[RecLayout::ETA] = tr.eta;
+    rec[RecLayout::KAP] = tr.kappa;
+    rec[RecLayout::ZETA] = tr.zeta;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-        R(cb, RecLayout::Y + s) = pr[6 + s];
-        R(cb, L.X() + s) = pr[6 + NS + s];
-        R(cb, L.YN() + s) = yn[s];
-        R(cb, L.CHI() + s) = chi[s];
+        rec[RecLayout::Y + s] = pr[6 + s];
+        rec[L.X() + s] = pr[6 + NS + s];
+        rec[L.YN() + s] = yn[s];
+        rec[L.CHI() + s] = chi[s];
     }
 #pragma unroll
-    for (int q = 0; q < NS * NS; ++q) R(cb, L.D() + q) = D[q];
+    for (int q = 0; q < NS * NS; ++q) rec[L.D() + q] = D[q];
+    store_cell<8 + 4 * NS + NS * NS>(P, R, [&](int q) { return rec[q]; }, i, j, k, cb);
 }
 
 // ---- nodes (flux.cpp:1627-1713) and the stochastic edge terms of each node -----------------
@@ -622,8 +649,8 @@ __device__ inline void face_body(const DevParams& P, const Fld& Us, const Fld& R
 // div[] = (fx(i+1)-fx(i))/dx + (fy(j+1)-fy(j))/dy + (fz(k+1)-fz(k))/dz, evaluated left to right.
 template <int NS>
 __device__ inline void update_from_div(const DevParams& P, const Fld& out, const Fld& u0, const Fld& us,
-                                       const double* div, double dt, int mode, const Fld& prim, int write_prim,
-                                       unsigned cb)
+                                       const double* div, double dt, int mode, const Fld& prim, int write_prim, int i,
+                                       int j, int k, unsigned cb)
 {
     const double src = 0.0;
     double o[NS + 5];
@@ -648,20 +675,18 @@ __device__ inline void update_from_div(const DevParams& P, const Fld& out, const
         for (int n = 0; n < 3; ++n) o[n + 1] += 2. / 3. * dt * r0 * g[n];
         o[4] += 2. / 3. * dt * (g[0] * r1 + g[1] * r2 + g[2] * r3);
     }
-#pragma unroll
-    for (int n = 0; n < NS + 5; ++n) out(cb, n) = o[n];
+    store_cell<NS + 5>(P, out, [&](int n) { return o[n]; }, i, j, k, cb);
     if (write_prim) {
         double pr[2 * NS + 6];
         cons_to_prim_cell<NS>(P, o, pr);
-#pragma unroll
-        for (int n = 0; n < 2 * NS + 6; ++n) prim(cb, n) = pr[n];
+        store_cell<2 * NS + 6>(P, prim, [&](int n) { return pr[n]; }, i, j, k, cb);
     }
 }
 
 template <int NS>
 __device__ inline void update_body(const DevParams& P, const Fld& out, const Fld& u0, const Fld& us, const Fld& fx,
                                    const Fld& fy, const Fld& fz, double dt, int mode, const Fld& prim, int write_prim,
-                                   unsigned cb)
+                                   int i, int j, int k, unsigned cb)
 {
     double div[NS + 5];
 #pragma unroll
@@ -669,7 +694,7 @@ __device__ inline void update_body(const DevParams& P, const Fld& out, const Fld
         // face n holds face 0 of the periodic box (store_face)
         div[n] = (fx(cb, n, 1, 0, 0) - fx(cb, n)) / P.dx[0] + (fy(cb, n, 0, 1, 0) - fy(cb, n)) / P.dx[1] +
                  (fz(cb, n, 0, 0, 1) - fz(cb, n)) / P.dx[2];
-    update_from_div<NS>(P, out, u0, us, div, dt, mode, prim, write_prim, cb);
+    update_from_div<NS>(P, out, u0, us, div, dt, mode, prim, write_prim, i, j, k, cb);
 }
 
 }  // namespace fhdx

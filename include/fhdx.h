@@ -45,7 +45,7 @@ enum fhdx_status {
 typedef struct fhdx_params {
     int32_t nspecies;          /* NS, 1..FHDX_MAX_SPECIES                                   */
     int32_t n_cells[3];        /* domain cells per direction                                 */
-    int32_t ngc;               /* ghost layers of cell data (>= 2)                           */
+    int32_t ngc;               /* ghost layers of cell data (>= 2, <= n_cells/2)             */
     int32_t algorithm_type;    /* 2: multispecies (species fluxes on), else species off      */
     int32_t advection_type;    /* 0: none, 1: interpolate prim, 2: interpolate cons          */
     int32_t visc_type;         /* |3| adds bulk viscosity zeta = zeta_ratio*eta              */
