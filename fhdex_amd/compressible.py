@@ -163,3 +163,57 @@ def evaluateStats(ctx: Context, cons: MultiFab, prim: MultiFab, st: StatsState, 
                                     st.miscStats.fab,
                                     st.miscVals.ctypes.data_as(C.POINTER(C.c_double)), int(steps),
                                     int(cross_cell), stream), "evaluateStats")
+
+
+# ---- on-disk formats (writePlotFile.cpp, checkpoint.cpp) -------------------------------------
+
+def _fields(cu: MultiFab, prim: MultiFab, st: StatsState, eta: MultiFab, kappa: MultiFab):
+    from .lib import Fields
+    mfs = (cu, st.consMean, st.consVar, prim, st.primMean, st.primVar, st.spatialCross,
+           st.miscStats, eta, kappa)
+    host = {not m.t.is_cuda for m in mfs}
+    if len(host) != 1:
+        raise ValueError("the fields must all be device arrays or all host arrays")
+    f = Fields(*[m.fab for m in mfs])
+    return f, (1 if host.pop() else 0)
+
+
+def WritePlotFile(p: Params, step: int, time: float, prob_lo, cu: MultiFab, prim: MultiFab,
+                  st: StatsState, eta: MultiFab, kappa: MultiFab, plot_base_name: str = "plt",
+                  plot_means: int = 1, plot_vars: int = 1, stream=None) -> str:
+    """WritePlotFile (writePlotFile.cpp:7-190): an AMReX plotfile named
+    <plot_base_name><step, 9 digits>; returns that directory name."""
+    f, host = _fields(cu, prim, st, eta, kappa)
+    lo = (C.c_double * 3)(*prob_lo)
+    check(lib().fhdx_write_plotfile(C.byref(p), plot_base_name.encode(), int(step), float(time), lo,
+                                    C.byref(f), int(plot_means), int(plot_vars), host, stream),
+          "WritePlotFile")
+    return f"{plot_base_name}{int(step):09d}"
+
+
+def WriteCheckPoint(p: Params, step: int, time: float, statsCount: int, seed: int, cu: MultiFab,
+                    prim: MultiFab, st: StatsState, eta: MultiFab, kappa: MultiFab,
+                    chk_base_name: str = "chk", stream=None) -> str:
+    """WriteCheckPoint (checkpoint.cpp:22-133) plus the Philox seed; returns the directory."""
+    f, host = _fields(cu, prim, st, eta, kappa)
+    check(lib().fhdx_write_checkpoint(C.byref(p), chk_base_name.encode(), int(step), float(time),
+                                      int(statsCount), int(seed), C.byref(f), host, stream),
+          "WriteCheckPoint")
+    return f"{chk_base_name}{int(step):09d}"
+
+
+def ReadCheckPoint(p: Params, restart: int, cu: MultiFab, prim: MultiFab, st: StatsState,
+                   eta: MultiFab, kappa: MultiFab, chk_base_name: str = "chk", reset_stats: int = 0,
+                   stream=None):
+    """ReadCheckPoint (checkpoint.cpp:135-255) into existing arrays.  Returns (step, time,
+    statsCount, seed) with step = the checkpoint's step + 1, as the reference does; seed is None
+    for a checkpoint that records none."""
+    f, host = _fields(cu, prim, st, eta, kappa)
+    step, sc, has = C.c_int(0), C.c_int(0), C.c_int(0)
+    t = C.c_double(0.0)
+    seed = C.c_uint64(0)
+    check(lib().fhdx_read_checkpoint(C.byref(p), chk_base_name.encode(), int(restart),
+                                     int(reset_stats), C.byref(step), C.byref(t), C.byref(sc),
+                                     C.byref(seed), C.byref(has), C.byref(f), host, stream),
+          "ReadCheckPoint")
+    return step.value, t.value, sc.value, (seed.value if has.value else None)

@@ -26,10 +26,20 @@ EXPORTS = [
     "fhdx_engine_bytes_used", "fhdx_engine_add_uniform", "fhdx_server_create",
     "fhdx_server_submit", "fhdx_server_poll", "fhdx_server_wait", "fhdx_server_stats",
     "fhdx_server_destroy", "fhdx_ctx_autotune", "fhdx_ctx_set_tune", "fhdx_ctx_get_tune",
-    "fhdx_evaluate_stats",
+    "fhdx_evaluate_stats", "fhdx_write_plotfile", "fhdx_write_checkpoint", "fhdx_read_checkpoint",
 ]
 
 _lib = None
+
+
+class Fields(C.Structure):
+    """``fhdx_fields`` (include/fhdx.h): the MultiFabs of WritePlotFile / WriteCheckPoint."""
+
+    _fields_ = [(n, Fab) for n in ("cu", "cuMeans", "cuVars", "prim", "primMeans", "primVars",
+                                   "spatialCross", "miscStats", "eta", "kappa")]
+
+
+IO_HOST_ARRAYS = 1
 
 
 class Tune(C.Structure):
@@ -109,6 +119,15 @@ def lib():
         L.fhdx_evaluate_stats.argtypes = [vp, Fab, Fab, Fab, Fab, Fab, Fab, Fab, Fab,
                                           C.POINTER(C.c_double), C.c_int, C.c_int, vp]
         L.fhdx_ctx_get_tune.argtypes = [vp, C.POINTER(Tune)]
+        PFd = C.POINTER(Fields)
+        L.fhdx_write_plotfile.argtypes = [P, C.c_char_p, C.c_int, C.c_double, C.POINTER(C.c_double),
+                                          PFd, C.c_int, C.c_int, C.c_int, vp]
+        L.fhdx_write_checkpoint.argtypes = [P, C.c_char_p, C.c_int, C.c_double, C.c_int, C.c_uint64,
+                                            PFd, C.c_int, vp]
+        L.fhdx_read_checkpoint.argtypes = [P, C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                           C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                           C.POINTER(C.c_uint64), C.POINTER(C.c_int), PFd, C.c_int,
+                                           vp]
         for name in EXPORTS:
             getattr(L, name).restype = C.c_int
         L.fhdx_engine_bytes_needed.restype = C.c_uint64

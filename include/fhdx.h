@@ -314,6 +314,45 @@ int fhdx_server_stats(fhdx_server* s, uint64_t* steps, uint64_t* cell_updates,
 /* Stops the worker; requests not finished complete as FHDX_ERR_BUSY ("server stopped"). */
 int fhdx_server_destroy(fhdx_server* s);
 
+/* ---- on-disk formats: AMReX plotfile and checkpoint ------------------------------------ */
+/* The MultiFabs WritePlotFile / WriteCheckPoint / ReadCheckPoint take
+ * (src_compressible/writePlotFile.cpp:7-18, checkpoint.cpp:22-36 and 135-149), each a cell
+ * array of the run's box with ngc ghosts.  Component counts: cu, cuMeans, cuVars nvars; prim,
+ * primMeans nprimvars; primVars nprimvars+5; spatialCross 8; miscStats 10 (checkpoint only);
+ * eta, kappa 1. */
+typedef struct fhdx_fields {
+    fhdx_fab cu, cuMeans, cuVars, prim, primMeans, primVars, spatialCross, miscStats, eta, kappa;
+} fhdx_fields;
+
+/* flags: the fabs are host memory (default: device memory, copied on `stream`) */
+#define FHDX_IO_HOST_ARRAYS 1
+
+/* WritePlotFile (writePlotFile.cpp:7-190): directory <plot_base_name><step, 9 digits> holding an
+ * AMReX plotfile (Header, Level_0/Cell_H, Level_0/Cell_D_00000) of cu, prim, [cuMeans 0-4,
+ * primMeans 0-5 if plot_means == 1], [cuVars 0-4, primVars 0-4 if plot_vars == 1],
+ * spatialCross, eta, kappa with the reference's variable names; valid cells only.  prob_lo is
+ * the domain's lower corner (Geometry::ProbLo); the upper one is prob_lo + n_cells * dx. */
+int fhdx_write_plotfile(const fhdx_params* p, const char* plot_base_name, int step, double time,
+                        const double prob_lo[3], const fhdx_fields* f, int plot_means,
+                        int plot_vars, int flags, void* stream);
+
+/* WriteCheckPoint (checkpoint.cpp:22-133): directory <chk_base_name><step, 9 digits> with the
+ * reference's Header (title, step, time, statsCount, BoxArray) plus one trailing line with the
+ * Philox seed, and one VisMF MultiFab per field (ghost cells included) under Level_0/. */
+int fhdx_write_checkpoint(const fhdx_params* p, const char* chk_base_name, int step, double time,
+                          int statsCount, uint64_t seed, const fhdx_fields* f, int flags,
+                          void* stream);
+
+/* ReadCheckPoint (checkpoint.cpp:135-255) of checkpoint <chk_base_name><restart, 9 digits>:
+ * cu, prim, eta, kappa always; the statistics fields if reset_stats == 0, zeroed with
+ * statsCount = 1 if reset_stats == 1.  *step is the checkpoint's step + 1 (the next step to
+ * take).  *has_seed (may be NULL) is 1 and *seed the recorded Philox seed if the checkpoint
+ * records one, else 0 with *seed unchanged. */
+int fhdx_read_checkpoint(const fhdx_params* p, const char* chk_base_name, int restart,
+                         int reset_stats, int* step, double* time, int* statsCount,
+                         uint64_t* seed, int* has_seed, const fhdx_fields* f, int flags,
+                         void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,77 @@
+"""Restart from a checkpoint continues the run bit for bit (checkpoint.cpp:22-255 with the
+main_driver.cpp:557-562, 608-616, 677-684 loop).
+
+Run A takes steps 1..6 with evaluateStats after each.  Run B takes steps 1..3, writes a
+checkpoint from device memory, reads it into fresh device arrays and takes steps 4..6 with the
+seed and statsCount the checkpoint recorded.  Noise is keyed by (seed, step), so the state and
+every statistics field of B must equal A's exactly.
+"""
+import sys
+
+import pytest
+
+from fhdex_amd import params as P
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("name,mk", [("argon", lambda: P.argon_gas(0, shape=(12, 8, 16))),
+                                     ("species4", lambda: P.four_species(8))])
+def test_checkpoint_restart_is_exact(tmp_path, orc, name, mk):
+    import torch
+    from fhdex_amd import compressible as fc
+    from fhdex_amd import lib as L
+    from tests.helpers import perturbed_state
+    cfg = mk()
+    p = cfg.params
+    say(f"case restart {name} {tuple(p.n_cells)}")
+    ctx = L.Context(p)
+    st0 = perturbed_state(p, cfg, orc, seed=3, amp=0.01, vel=300.0)
+    seed, xc = 4242, 2
+
+    def fresh():
+        cu = fc.MultiFab(torch.from_numpy(st0.cu.copy()).cuda(), (-p.ngc,) * 3)
+        return cu, fc.MultiFab.cell(p, p.nprimvars), fc.StatsState(p)
+
+    def run(cu, prim, stats, first, last, sc, sd):
+        for step in range(first, last + 1):
+            fc.rk3_step(ctx, cu, prim, cfg.dt, sd, step)
+            fc.evaluateStats(ctx, cu, prim, stats, sc, xc)
+            sc += 1
+        return sc
+
+    cuA, primA, stA = fresh()
+    run(cuA, primA, stA, 1, 6, 1, seed)
+
+    cuB, primB, stB = fresh()
+    sc = run(cuB, primB, stB, 1, 3, 1, seed)
+    eta, zeta, kappa = (fc.MultiFab.cell(p, 1) for _ in range(3))
+    chi = fc.MultiFab.cell(p, p.nspecies)
+    D = fc.MultiFab.cell(p, p.nspecies * p.nspecies)
+    fc.calculateTransportCoeffs(ctx, primB, eta, zeta, kappa, chi, D)
+    torch.cuda.synchronize()
+    base = str(tmp_path / "chk")
+    fc.WriteCheckPoint(p, 3, 3 * cfg.dt, sc, seed, cuB, primB, stB, eta, kappa, chk_base_name=base)
+
+    cuC = fc.MultiFab.cell(p, p.nvars)
+    primC = fc.MultiFab.cell(p, p.nprimvars)
+    stC = fc.StatsState(p)
+    etaC, kappaC = fc.MultiFab.cell(p, 1), fc.MultiFab.cell(p, 1)
+    step, t, sc2, sd = fc.ReadCheckPoint(p, 3, cuC, primC, stC, etaC, kappaC, chk_base_name=base)
+    say(f"  resumed at step {step}, statsCount {sc2}, seed {sd}")
+    assert (step, t, sc2, sd) == (4, 3 * cfg.dt, sc, seed)
+    assert torch.equal(etaC.t, eta.t) and torch.equal(kappaC.t, kappa.t)
+    run(cuC, primC, stC, step, 6, sc2, sd)
+    torch.cuda.synchronize()
+    pairs = [("cu", cuA, cuC), ("prim", primA, primC)]
+    for f in ("consMean", "consVar", "primMean", "primVar", "spatialCross", "miscStats"):
+        pairs.append((f, getattr(stA, f), getattr(stC, f)))
+    for f, a, c in pairs:
+        same = torch.equal(a.t, c.t)
+        say(f"  {f}: exact={same}")
+        assert same, f
