@@ -335,10 +335,34 @@ __device__ inline void face_flux(const DevParams& P, const Fld& Us, const Fld& R
     constexpr int ex = (D == 0), ey = (D == 1), ez = (D == 2);
     constexpr int nv = NS + 5;
     const RecLayout L{NS};
-    const int RT = RecLayout::T, RE = RecLayout::ETA;
-    // velocity component c (1..3) lives in record slot c-1
-    auto velU = [&](int c) { return R(cb, RecLayout::U + c - 1); };
-    auto velL = [&](int c) { return R(cb, RecLayout::U + c - 1, -ex, -ey, -ez); };
+    // record values of the two cells, loaded once and shared by the flux families below
+    const double TU = R(cb, RecLayout::T), TL = R(cb, RecLayout::T, -ex, -ey, -ez);
+    const double etaU = R(cb, RecLayout::ETA), etaL = R(cb, RecLayout::ETA, -ex, -ey, -ez);
+    const double kapU = R(cb, RecLayout::KAP), kapL = R(cb, RecLayout::KAP, -ex, -ey, -ez);
+    const double PU = R(cb, RecLayout::P), PL = R(cb, RecLayout::P, -ex, -ey, -ez);
+    double vU[3], vL[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        vU[c] = R(cb, RecLayout::U + c);
+        vL[c] = R(cb, RecLayout::U + c, -ex, -ey, -ez);
+    }
+    // velocity component c (1..3)
+    auto velU = [&](int c) { return vU[c - 1]; };
+    auto velL = [&](int c) { return vL[c - 1]; };
+    const bool species = P.algorithm_type == 2;
+    double chiU[NS], chiL[NS], DU[NS * NS], DL[NS * NS];
+    if (species) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            chiU[s] = R(cb, L.CHI() + s);
+            chiL[s] = R(cb, L.CHI() + s, -ex, -ey, -ez);
+        }
+#pragma unroll
+        for (int q = 0; q < NS * NS; ++q) {
+            DU[q] = R(cb, L.D() + q);
+            DL[q] = R(cb, L.D() + q, -ex, -ey, -ez);
+        }
+    }
 
     double f[nv + 4];
 #pragma unroll
@@ -346,9 +370,9 @@ __device__ inline void face_flux(const DevParams& P, const Fld& Us, const Fld& R
 
     // ---- stochastic (flux.cpp:71-1145) ----
     if (P.stoch_on) {
-        double Tu = R(cb, RT), Tl = R(cb, RT, -ex, -ey, -ez);
-        double mup = R(cb, RE) * Tu + R(cb, RE, -ex, -ey, -ez) * Tl;
-        double kp = R(cb, RecLayout::KAP) * Tu * Tu + R(cb, RecLayout::KAP, -ex, -ey, -ez) * Tl * Tl;
+        double Tu = TU, Tl = TL;
+        double mup = etaU * Tu + etaL * Tl;
+        double kp = kapU * Tu * Tu + kapL * Tl * Tl;
         double meanT = 0.5 * (Tu + Tl);
         double wn[nv];
         wn[0] = 0.;
@@ -404,7 +428,7 @@ __device__ inline void face_flux(const DevParams& P, const Fld& Us, const Fld& R
         phishear = -0.5 * phishear;
         f[nv + 1] = f[nv + 1] - phidiag;
         f[nv + 2] = f[nv + 2] - phishear;
-        if (P.algorithm_type == 2) {
+        if (species) {
             double yy[NS], yyp[NS], DijY[NS * NS], sqD[NS * NS];
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
@@ -418,10 +442,8 @@ __device__ inline void face_flux(const DevParams& P, const Fld& Us, const Fld& R
                 MWmix = MWmix + 0.5 * (yy[s] + yyp[s]) / P.molmass[s];
 #pragma unroll
                 for (int l = 0; l < NS; ++l)
-                    DijY[s * NS + l] = 0.5 * (R(cb, L.D() + l * NS + s, -ex, -ey, -ez) * yy[l] +
-                                              R(cb, L.D() + l * NS + s) * yyp[l] +
-                                              (R(cb, L.D() + s * NS + l, -ex, -ey, -ez) * yy[s] +
-                                               R(cb, L.D() + s * NS + l) * yyp[s]));
+                    DijY[s * NS + l] = 0.5 * (DL[l * NS + s] * yy[l] + DU[l * NS + s] * yyp[l] +
+                                              (DL[s * NS + l] * yy[s] + DU[s * NS + l] * yyp[s]));
             }
 #pragma unroll
             for (int s = 0; s < NS; ++s)
@@ -444,9 +466,7 @@ __device__ inline void face_flux(const DevParams& P, const Fld& Us, const Fld& R
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
                 double hk = P.e0[s] + P.hcp[s] * meanT;
-                double ss = (hk + P.Runiv * meanT / P.molmass[s] * 0.5 *
-                                      (R(cb, L.CHI() + s, -ex, -ey, -ez) + R(cb, L.CHI() + s))) *
-                            wi[5 + s];
+                double ss = (hk + P.Runiv * meanT / P.molmass[s] * 0.5 * (chiL[s] + chiU[s])) * wi[5 + s];
                 soret = soret + ss;
             }
             f[nv + 3] = f[nv + 3] + soret;
@@ -456,8 +476,8 @@ __device__ inline void face_flux(const DevParams& P, const Fld& Us, const Fld& R
     // ---- diffusive (flux.cpp:1193-1623) ----
     {
         const double half = 0.5, h = P.dx[D];
-        double mup = half * (R(cb, RE) + R(cb, RE, -ex, -ey, -ez));
-        double kp = half * (R(cb, RecLayout::KAP) + R(cb, RecLayout::KAP, -ex, -ey, -ez));
+        double mup = half * (etaU + etaL);
+        double kp = half * (kapU + kapL);
         double tau1 = mup * (velU(1) - velL(1)) / h;
         double tau2 = mup * (velU(2) - velL(2)) / h;
         double tau3 = mup * (velU(3) - velL(3)) / h;
@@ -469,7 +489,6 @@ __device__ inline void face_flux(const DevParams& P, const Fld& Us, const Fld& R
         if (D == 0) phiflx = tau1 * s1 + divp * s1 + tau2 * s2 + tau3 * s3;
         else if (D == 1) phiflx = tau1 * s1 + tau2 * s2 + divp * s2 + tau3 * s3;
         else phiflx = tau1 * s1 + tau2 * s2 + tau3 * s3 + divp * s3;
-        double TU = R(cb, RT), TL = R(cb, RT, -ex, -ey, -ez);
         double Qflux = kp * (TU - TL) / h;
         f[1] = f[1] - (D == 0 ? tau1 + divp : tau1);
         f[2] = f[2] - (D == 1 ? tau2 + divp : tau2);
@@ -477,9 +496,8 @@ __device__ inline void face_flux(const DevParams& P, const Fld& Us, const Fld& R
         f[nv] = f[nv] - Qflux;
         f[nv + 1] = f[nv + 1] - (half * phiflx);
         double meanT = 0.5 * (TL + TU);
-        double PU = R(cb, RecLayout::P), PL = R(cb, RecLayout::P, -ex, -ey, -ez);
         double meanP = 0.5 * (PL + PU);
-        if (P.algorithm_type == 2) {
+        if (species) {
             double dk[NS], soret[NS], Fk[NS];
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
@@ -491,9 +509,9 @@ __device__ inline void face_flux(const DevParams& P, const Fld& Us, const Fld& R
                 dk[s] = term1 + term2;
                 double ChiX;
                 if (D == 2)  // flux.cpp:1561 reads chi at k and k+1
-                    ChiX = 0.5 * (R(cb, L.CHI() + s) * XL + R(cb, L.CHI() + s, 0, 0, 1) * XU);
+                    ChiX = 0.5 * (chiU[s] * XL + R(cb, L.CHI() + s, 0, 0, 1) * XU);
                 else
-                    ChiX = 0.5 * (R(cb, L.CHI() + s, -ex, -ey, -ez) * XL + R(cb, L.CHI() + s) * XU);
+                    ChiX = 0.5 * (chiL[s] * XL + chiU[s] * XU);
                 soret[s] = ChiX * (TU - TL) / h / meanT;
             }
 #pragma unroll
@@ -501,8 +519,7 @@ __device__ inline void face_flux(const DevParams& P, const Fld& Us, const Fld& R
                 double fk = 0.;
 #pragma unroll
                 for (int ll = 0; ll < NS; ++ll) {
-                    double Fks = half * (R(cb, L.D() + ll * NS + kk, -ex, -ey, -ez) + R(cb, L.D() + ll * NS + kk)) *
-                                 (dk[ll] + soret[ll]);
+                    double Fks = half * (DL[ll * NS + kk] + DU[ll * NS + kk]) * (dk[ll] + soret[ll]);
                     fk = fk - Fks;
                 }
                 Fk[kk] = fk;
@@ -511,8 +528,8 @@ __device__ inline void face_flux(const DevParams& P, const Fld& Us, const Fld& R
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
                 double hk = P.e0[s] + P.hcp[s] * meanT;
-                double csum = (D == 2) ? R(cb, L.CHI() + s) + R(cb, L.CHI() + s)  // flux.cpp:1604
-                                       : R(cb, L.CHI() + s, -ex, -ey, -ez) + R(cb, L.CHI() + s);
+                double csum = (D == 2) ? chiU[s] + chiU[s]  // flux.cpp:1604
+                                       : chiL[s] + chiU[s];
                 double Q5s = (hk + 0.5 * P.Runiv * meanT * csum / P.molmass[s]) * Fk[s];
                 Q5 = Q5 + Q5s;
             }
