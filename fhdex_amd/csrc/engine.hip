@@ -34,8 +34,17 @@ __device__ inline NoiseKey inst_key(const Inst& in)
 __device__ inline double* stage_in(const Inst& in, int st) { return st == 0 ? in.U : (st == 1 ? in.U1 : in.U2); }
 __device__ inline double* stage_out(const Inst& in, int st) { return st == 0 ? in.U1 : (st == 1 ? in.U2 : in.U); }
 
+// Workgroup b runs on XCD b % 8.  XCD x takes the contiguous work-item range
+// [x*nb/8, (x+1)*nb/8), so the chunks of one instance (and the z neighbours a chunk reads) stay
+// on one XCD and its L2.
+__device__ inline unsigned xcd_item(unsigned b, unsigned nb)
+{
+    const unsigned xcd = b & 7, r = b >> 3, per = nb >> 3, rem = nb & 7;
+    return xcd * per + (xcd < rem ? xcd : rem) + r;
+}
+
 #define EB_CELL_PROLOGUE                                       \
-    const WorkItem wi = W[blockIdx.x];                         \
+    const WorkItem wi = W[xcd_item(blockIdx.x, gridDim.x)];    \
     const Inst& in = I[wi.inst];                               \
     const long t = (long)wi.chunk * EB + threadIdx.x;          \
     if (t >= in.ncell || in.step >= in.step_end) return;       \
