@@ -1,9 +1,11 @@
 // fhdx_amrex.H -- header-only AMReX adapter: the reference's RK3step / calculateFlux /
-// conservedToPrimitive / calculateTransportCoeffs signatures (src_compressible/
-// compressible_functions.H:23-55) implemented over the C ABI of fhdx.h.
+// conservedToPrimitive / calculateTransportCoeffs / evaluateStats / WritePlotFile /
+// WriteCheckPoint / ReadCheckPoint signatures (src_compressible/compressible_functions.H:23-135)
+// implemented over the C ABI of fhdx.h.
 //
-// Include it from src_compressible instead of linking timeStep.cpp, flux.cpp, transCoeffs.cpp
-// and conservedPrimitiveConversions.cpp, build AMReX with USE_HIP=TRUE for gfx950, and link
+// Include it from src_compressible instead of linking timeStep.cpp, flux.cpp, transCoeffs.cpp,
+// conservedPrimitiveConversions.cpp, stats.cpp, writePlotFile.cpp and checkpoint.cpp, build
+// AMReX with USE_HIP=TRUE for gfx950, and link
 // fhdex_amd/libfhdx.so.  Requirements: a fully periodic domain and one box per rank
 // (max_grid_size >= n_cells), which is what the library's index checks accept.  Compiled only
 // where AMREX_HOME exists; it is not part of the library build.
@@ -96,6 +98,40 @@ inline void check(int status)
 
 inline void* stream() { return (void*)amrex::Gpu::gpuStream(); }
 
+// Philox key of the next RK3step: the seed and the number of steps taken.  ReadCheckPoint sets
+// both from the checkpoint, so a restarted run continues the noise sequence exactly.
+inline uint64_t& philox_seed()
+{
+    static uint64_t s = (uint64_t)common::seed;
+    return s;
+}
+inline uint64_t& philox_step()
+{
+    static uint64_t s = 0;
+    return s;
+}
+
+// The MultiFabs of WritePlotFile / WriteCheckPoint at iterator mfi (miscStats may be absent).
+inline fhdx_fields fields(amrex::MFIter const& mfi, const amrex::MultiFab& cu, const amrex::MultiFab& cuMeans,
+                          const amrex::MultiFab& cuVars, const amrex::MultiFab& prim,
+                          const amrex::MultiFab& primMeans, const amrex::MultiFab& primVars,
+                          const amrex::MultiFab& spatialCross, const amrex::MultiFab* miscStats,
+                          const amrex::MultiFab& eta, const amrex::MultiFab& kappa)
+{
+    fhdx_fields f;
+    f.cu = fab(cu, mfi);
+    f.cuMeans = fab(cuMeans, mfi);
+    f.cuVars = fab(cuVars, mfi);
+    f.prim = fab(prim, mfi);
+    f.primMeans = fab(primMeans, mfi);
+    f.primVars = fab(primVars, mfi);
+    f.spatialCross = fab(spatialCross, mfi);
+    f.miscStats = miscStats ? fab(*miscStats, mfi) : fhdx_fab{};
+    f.eta = fab(eta, mfi);
+    f.kappa = fab(kappa, mfi);
+    return f;
+}
+
 }  // namespace fhdx_amrex
 
 // ---- the reference's entry points ----------------------------------------------------------
@@ -142,7 +178,7 @@ inline void calculateFlux(const amrex::MultiFab& cons, const amrex::MultiFab& pr
     }
 }
 
-// RK3step with Philox noise keyed by (common::seed, a step counter kept here).
+// RK3step with Philox noise keyed by (philox_seed(), philox_step()).
 inline void RK3step(amrex::MultiFab& cu, amrex::MultiFab& cup, amrex::MultiFab& cup2, amrex::MultiFab& /*cup3*/,
                     amrex::MultiFab& prim, amrex::MultiFab& /*source*/, amrex::MultiFab& eta, amrex::MultiFab& zeta,
                     amrex::MultiFab& kappa, amrex::MultiFab& chi, amrex::MultiFab& D,
@@ -152,7 +188,6 @@ inline void RK3step(amrex::MultiFab& cu, amrex::MultiFab& cup, amrex::MultiFab& 
                     amrex::MultiFab& /*ranchem*/, const amrex::Geometry& geom, const amrex::Real dt)
 {
     using namespace fhdx_amrex;
-    static uint64_t step = 0;
     for (amrex::MFIter mfi(cu); mfi.isValid(); ++mfi) {
         fhdx_fab fl[3], sf[3], cx[3], cy[3], cz[3];
         for (int d = 0; d < 3; ++d) {
@@ -165,7 +200,91 @@ inline void RK3step(amrex::MultiFab& cu, amrex::MultiFab& cup, amrex::MultiFab& 
         check(fhdx_rk3_step_ref(context(geom), fab(cu, mfi), fab(cup, mfi), fab(cup2, mfi), fab(prim, mfi),
                                 fab(eta, mfi), fab(zeta, mfi), fab(kappa, mfi), fab(chi, mfi), fab(D, mfi), fl, sf,
                                 cx, cy, cz, fab(visccorn, mfi), fab(rancorn, mfi), nullptr, nullptr, nullptr,
-                                nullptr, dt, (uint64_t)common::seed, step, stream()));
+                                nullptr, dt, philox_seed(), philox_step(), stream()));
     }
-    ++step;
+    ++philox_step();
+}
+
+// evaluateStats (stats.cpp:5-363); cross_cell from common_namespace.H:35.
+inline void evaluateStats(const amrex::MultiFab& cons, amrex::MultiFab& consMean, amrex::MultiFab& consVar,
+                          const amrex::MultiFab& prim, amrex::MultiFab& primMean, amrex::MultiFab& primVar,
+                          amrex::MultiFab& spatialCross, amrex::MultiFab& miscStats, amrex::Real* miscVals,
+                          const int steps, const amrex::Real* /*dx*/)
+{
+    using namespace fhdx_amrex;
+    for (amrex::MFIter mfi(cons); mfi.isValid(); ++mfi)
+        check(fhdx_evaluate_stats(context(amrex::DefaultGeometry()), fab(cons, mfi), fab(consMean, mfi),
+                                  fab(consVar, mfi), fab(prim, mfi), fab(primMean, mfi), fab(primVar, mfi),
+                                  fab(spatialCross, mfi), fab(miscStats, mfi), miscVals, steps, common::cross_cell,
+                                  stream()));
+}
+
+// WritePlotFile (writePlotFile.cpp:7-190): <plot_base_name><step, 9 digits>.
+inline void WritePlotFile(int step, const amrex::Real time, const amrex::Geometry& geom, const amrex::MultiFab& cu,
+                          const amrex::MultiFab& cuMeans, const amrex::MultiFab& cuVars, const amrex::MultiFab& prim,
+                          const amrex::MultiFab& primMeans, const amrex::MultiFab& primVars,
+                          const amrex::MultiFab& spatialCross, const amrex::MultiFab& eta,
+                          const amrex::MultiFab& kappa)
+{
+    using namespace fhdx_amrex;
+    const fhdx_params p = params_from_namespaces(geom);
+    const double lo[3] = {geom.ProbLo(0), geom.ProbLo(1), geom.ProbLo(2)};
+    for (amrex::MFIter mfi(cu); mfi.isValid(); ++mfi) {
+        const fhdx_fields f = fields(mfi, cu, cuMeans, cuVars, prim, primMeans, primVars, spatialCross, nullptr,
+                                     eta, kappa);
+        check(fhdx_write_plotfile(&p, common::plot_base_name.c_str(), step, time, lo, &f, common::plot_means,
+                                  common::plot_vars, 0, stream()));
+    }
+}
+
+// WriteCheckPoint (checkpoint.cpp:22-133): chk<step, 9 digits>, plus the Philox seed.
+inline void WriteCheckPoint(int step, const amrex::Real time, int statsCount, const amrex::Geometry& geom,
+                            const amrex::MultiFab& cu, const amrex::MultiFab& cuMeans, const amrex::MultiFab& cuVars,
+                            const amrex::MultiFab& prim, const amrex::MultiFab& primMeans,
+                            const amrex::MultiFab& primVars, const amrex::MultiFab& spatialCross,
+                            const amrex::MultiFab& miscStats, const amrex::MultiFab& eta,
+                            const amrex::MultiFab& kappa)
+{
+    using namespace fhdx_amrex;
+    const fhdx_params p = params_from_namespaces(geom);
+    for (amrex::MFIter mfi(cu); mfi.isValid(); ++mfi) {
+        const fhdx_fields f = fields(mfi, cu, cuMeans, cuVars, prim, primMeans, primVars, spatialCross, &miscStats,
+                                     eta, kappa);
+        check(fhdx_write_checkpoint(&p, "chk", step, time, statsCount, philox_seed(), &f, 0, stream()));
+    }
+}
+
+// ReadCheckPoint (checkpoint.cpp:135-255): defines the MultiFabs on the one-box domain, reads
+// checkpoint chk<restart>, returns step + 1, and resumes the Philox key at that step.
+inline void ReadCheckPoint(int& step, amrex::Real& time, int& statsCount, amrex::Geometry& geom,
+                           amrex::MultiFab& cu, amrex::MultiFab& cuMeans, amrex::MultiFab& cuVars,
+                           amrex::MultiFab& prim, amrex::MultiFab& primMeans, amrex::MultiFab& primVars,
+                           amrex::MultiFab& spatialCross, amrex::MultiFab& miscStats, amrex::MultiFab& eta,
+                           amrex::MultiFab& kappa)
+{
+    using namespace fhdx_amrex;
+    const fhdx_params p = params_from_namespaces(geom);
+    const int nv = 5 + p.nspecies, np = 6 + 2 * p.nspecies, g = p.ngc;
+    amrex::BoxArray ba(geom.Domain());
+    amrex::DistributionMapping dm(ba, amrex::ParallelDescriptor::NProcs());
+    cu.define(ba, dm, nv, g);
+    cuMeans.define(ba, dm, nv, g);
+    cuVars.define(ba, dm, nv, g);
+    prim.define(ba, dm, np, g);
+    primMeans.define(ba, dm, np, g);
+    primVars.define(ba, dm, np + 5, g);
+    spatialCross.define(ba, dm, 8, g);
+    miscStats.define(ba, dm, 10, g);
+    eta.define(ba, dm, 1, g);
+    kappa.define(ba, dm, 1, g);
+    for (amrex::MFIter mfi(cu); mfi.isValid(); ++mfi) {
+        const fhdx_fields f = fields(mfi, cu, cuMeans, cuVars, prim, primMeans, primVars, spatialCross, &miscStats,
+                                     eta, kappa);
+        uint64_t seed = philox_seed();
+        int has_seed = 0;
+        check(fhdx_read_checkpoint(&p, "chk", common::restart, common::reset_stats, &step, &time, &statsCount, &seed,
+                                   &has_seed, &f, 0, stream()));
+        if (has_seed) philox_seed() = seed;
+        philox_step() = (uint64_t)(step - 1);
+    }
 }
