@@ -586,13 +586,15 @@ void apply_tune(FusedWorkspace& w, const fhdx_tune& t)
     w.noise_inline = t.noise_inline != 0;
     w.kz = t.z_levels;
     w.face_bpc = t.face_blocks_per_cu;
+    w.rec_in_update = t.records_in_update != 0;
 }
 
 bool valid_tune(const fhdx_tune& t)
 {
     return (t.tile_width == 16 || t.tile_width == 32 || t.tile_width == 64) && t.region_tiles >= 1 &&
            t.region_tiles <= 64 && (t.noise_inline == 0 || t.noise_inline == 1) && t.z_levels >= 1 &&
-           t.z_levels <= 64 && (t.face_blocks_per_cu == 0 || t.face_blocks_per_cu == 2 || t.face_blocks_per_cu == 3);
+           t.z_levels <= 64 && (t.face_blocks_per_cu == 0 || t.face_blocks_per_cu == 2 || t.face_blocks_per_cu == 3) &&
+           (t.records_in_update == 0 || t.records_in_update == 1);
 }
 
 }  // namespace
@@ -601,7 +603,7 @@ int fhdx_ctx_set_tune(fhdx_ctx* c, const fhdx_tune* t)
 {
     if (!c || !t || !valid_tune(*t)) {
         set_error("fhdx_ctx_set_tune: tile_width must be 16, 32 or 64, region_tiles and z_levels 1..64, "
-                  "noise_inline 0/1, face_blocks_per_cu 0, 2 or 3");
+                  "noise_inline 0/1, face_blocks_per_cu 0, 2 or 3, records_in_update 0/1");
         return FHDX_ERR_ARG;
     }
     apply_tune(c->fused, *t);
@@ -616,7 +618,7 @@ int fhdx_ctx_get_tune(fhdx_ctx* c, fhdx_tune* t)
     t->noise_inline = c->fused.noise_inline ? 1 : 0;
     t->z_levels = c->fused.kz;
     t->face_blocks_per_cu = c->fused.face_bpc;
-    t->pad = 0;
+    t->records_in_update = c->fused.rec_in_update ? 1 : 0;
     t->ms_per_step = 0.;
     return FHDX_OK;
 }
@@ -689,17 +691,19 @@ int fhdx_ctx_autotune(fhdx_ctx* c, fhdx_fab cu, double dt, int trial_steps, fhdx
     }
     top = cur;
     const int nz = c->p.n_cells[2];
-    for (int dim = 0; dim < 5 && !r; ++dim) {
+    for (int dim = 0; dim < 6 && !r; ++dim) {
         std::vector<int> vals;
         if (dim == 0) vals = {64, 32, 16};
         if (dim == 1) vals = {1, 2, 4, 8};
         if (dim == 2) vals = {1, 2, 4, 8, 16};
         if (dim == 3) vals = {1, 0};
         if (dim == 4) vals = {0, 3, 2};
+        if (dim == 5) vals = {0, 1};
         for (int v : vals) {
             fhdx_tune t = top;
             int32_t* field = dim == 0 ? &t.tile_width : dim == 1 ? &t.region_tiles : dim == 2 ? &t.z_levels
-                           : dim == 3 ? &t.noise_inline : &t.face_blocks_per_cu;
+                           : dim == 3 ? &t.noise_inline : dim == 4 ? &t.face_blocks_per_cu
+                                                                  : &t.records_in_update;
             if (*field == v) continue;
             if (dim == 0 && v > 16 && (v >> 1) >= c->p.n_cells[0]) continue;  // same tiles as a narrower one
             if (dim == 2 && v > nz) continue;

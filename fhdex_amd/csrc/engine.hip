@@ -84,12 +84,15 @@ __global__ void __launch_bounds__(EB) kb_faces(const Inst* __restrict__ I, const
     store_face<NS, 2>(in.P, G.fld(in.F[2]), f, i, j, k, cb);
 }
 
-template <int NS>
+// REC: also write the next stage's cell records (stages 1 and 2), so kb_cells runs once a step.
+template <int NS, bool REC>
 __global__ void __launch_bounds__(EB) kb_update(const Inst* __restrict__ I, const WorkItem* __restrict__ W, int st)
 {
     EB_CELL_PROLOGUE
+    const Fld rn = G.fld(in.R);
     update_body<NS>(in.P, G.fld(stage_out(in, st)), G.fld(in.U), G.fld(stage_in(in, st)), G.fld(in.F[0]),
-                    G.fld(in.F[1]), G.fld(in.F[2]), in.dt, st, G.fld(in.prim), st == 2, i, j, k, G.off(i, j, k));
+                    G.fld(in.F[1]), G.fld(in.F[2]), in.dt, st, G.fld(in.prim), st == 2, i, j, k, G.off(i, j, k),
+                    REC ? &rn : nullptr, in.err);
 }
 
 // Periodic ghost fill of one instance's state at admission; from then on every kernel that
@@ -327,11 +330,12 @@ static void enqueue_group(Engine* e, int g, const double ws[3])
     const Inst* I = e->d_inst;
     const WorkItem* Wc = e->d_cells[g];
     // ghost images come with every store (store_cell): no fill launches inside the step
+    kb_cells<NS><<<nc, EB, 0, s>>>(I, Wc, 0);
     for (int st = 0; st < 3; ++st) {
-        kb_cells<NS><<<nc, EB, 0, s>>>(I, Wc, st);
         kb_nodes<<<nc, EB, 0, s>>>(I, Wc, ws[st]);
         kb_faces<NS><<<nc, EB, 0, s>>>(I, Wc, st, ws[st]);
-        kb_update<NS><<<nc, EB, 0, s>>>(I, Wc, st);
+        if (st < 2) kb_update<NS, true><<<nc, EB, 0, s>>>(I, Wc, st);
+        else kb_update<NS, false><<<nc, EB, 0, s>>>(I, Wc, st);
     }
 }
 

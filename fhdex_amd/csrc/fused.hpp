@@ -72,6 +72,7 @@ struct FusedWorkspace {
     int kz = 1;                // z levels each tile block marches through
     bool noise_inline = true;  // Philox noise evaluated in the faces/nodes, or once into arrays
     int face_bpc = 0;          // cap on resident k_faces blocks per CU (0: none)
+    bool rec_in_update = false;  // stages 1-2: k_update writes the next stage's records
     Timing timing;
 };
 

@@ -9,7 +9,8 @@
 //   k_faces    all four flux families (stochastic, diffusive, corner, hyperbolic) of the three
 //              lower faces of each cell, accumulated in registers in the reference's
 //              per-component order with the energy fold;
-//   k_update   divergence + RK3 stage combination (+ primitive variables after stage 3).
+//   k_update   divergence + RK3 stage combination (+ the next stage's cell records after stages 1
+//              and 2, primitive variables after stage 3).
 // Every array of the box shares one layout (Grid), so a kernel computes each cell offset once
 // and addresses all arrays as scalar base + 32-bit offset.  Records, stage states and prim are
 // written with their periodic ghost images by the producing kernel; only the caller's cu gets
@@ -215,13 +216,15 @@ __global__ void __launch_bounds__(FB) k_faces(DevParams P, TileMap tm, Grid G, F
 
 // ---- K4: stage combination (+ primitive variables after the last stage) --------------------
 
-template <int NS, bool ZM>
+// REC: also write the next stage's cell records (k_cells of that stage is then skipped).
+template <int NS, bool ZM, bool REC>
 __global__ void __launch_bounds__(FB) k_update(DevParams P, TileMap tm, Grid G, double* out, double* u0,
                                                double* us, double* fx, double* fy, double* fz, double dt, int mode,
-                                               double* prim, int write_prim)
+                                               double* prim, int write_prim, double* rec, int* err)
 {
+    const Fld rn = G.fld(rec);
     TILE_LEVELS(tm, update_body<NS>(P, G.fld(out), G.fld(u0), G.fld(us), G.fld(fx), G.fld(fy), G.fld(fz), dt, mode,
-                                    G.fld(prim), write_prim, i, j, k, G.off(i, j, k));)
+                                    G.fld(prim), write_prim, i, j, k, G.off(i, j, k), REC ? &rn : nullptr, err);)
 }
 
 // ---- host side ---------------------------------------------------------------------------
@@ -291,9 +294,11 @@ static void launch_stage(const DevParams& P, FusedWorkspace& w, const Grid& G, d
     const TileMap tm = make_tiles(P.n[0], P.n[1], P.n[2], w.rh, w.tw_max, w.kz);
     const unsigned ntile = tm.blocks();
     Timing& T = w.timing;
-    T.begin(KT_CELLS, s);
-    k_cells<NS><<<grid_for(ncell, FB), FB, 0, s>>>(P, G, us, w.rec, d_err);  // records + ghost images
-    T.end(s);
+    if (st == 0 || !w.rec_in_update) {  // else the previous stage's k_update wrote them
+        T.begin(KT_CELLS, s);
+        k_cells<NS><<<grid_for(ncell, FB), FB, 0, s>>>(P, G, us, w.rec, d_err);  // records + ghost images
+        T.end(s);
+    }
     const bool zm = tm.kz > 1;
     T.begin(KT_NODES, s);
     if (zm) k_nodes<INL, true><<<ntile, FB, 0, s>>>(P, tm, G, w.rec, w.node, w.rA, w.rB, K, wst);
@@ -320,11 +325,18 @@ static void launch_stage(const DevParams& P, FusedWorkspace& w, const Grid& G, d
     else k_faces<NS, INL, false><<<ntile, FB, lds, s>>>(P, tm, G, fa);
     T.end(s);
     T.begin(KT_UPDATE, s);
-    if (zm)
-        k_update<NS, true><<<ntile, FB, 0, s>>>(P, tm, G, out, cu, us, w.F[0], w.F[1], w.F[2], dt, st, prim, st == 2);
-    else
-        k_update<NS, false><<<ntile, FB, 0, s>>>(P, tm, G, out, cu, us, w.F[0], w.F[1], w.F[2], dt, st, prim,
-                                                 st == 2);
+#define FHDX_UPD(ZMV, RECV)                                                                                    \
+    k_update<NS, ZMV, RECV><<<ntile, FB, 0, s>>>(P, tm, G, out, cu, us, w.F[0], w.F[1], w.F[2], dt, st, prim, \
+                                                 st == 2, w.rec, d_err)
+    const bool rec_next = w.rec_in_update && st < 2;
+    if (zm) {
+        if (rec_next) FHDX_UPD(true, true);
+        else FHDX_UPD(true, false);
+    } else {
+        if (rec_next) FHDX_UPD(false, true);
+        else FHDX_UPD(false, false);
+    }
+#undef FHDX_UPD
     T.end(s);
 }
 

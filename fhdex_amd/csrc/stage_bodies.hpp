@@ -118,14 +118,13 @@ __device__ inline void ghost_body(const Arr& a, int nc, int g, int n0, int n1, i
 
 // ---- conservedToPrimitive + transport coefficients into the cell record -------------------
 
+// The record of cell (i,j,k) from its conserved values c[] (held in registers by the caller).
 template <int NS>
-__device__ inline void cell_body(const DevParams& P, const Fld& U, const Fld& R, int* err, int i, int j, int k,
-                                 unsigned cb)
+__device__ inline void cell_record(const DevParams& P, const double* c, const Fld& R, int* err, int i, int j, int k,
+                                   unsigned cb)
 {
     const RecLayout L{NS};
-    double c[NS + 5], pr[2 * NS + 6];
-#pragma unroll
-    for (int n = 0; n < NS + 5; ++n) c[n] = U(cb, n);
+    double pr[2 * NS + 6];
     cons_to_prim_cell<NS>(P, c, pr);
     Transport tr;
     double chi[NS], D[NS * NS];
@@ -158,6 +157,16 @@ __device__ inline void cell_body(const DevParams& P, const Fld& U, const Fld& R,
 #pragma unroll
     for (int q = 0; q < NS * NS; ++q) rec[L.D() + q] = D[q];
     store_cell<8 + 4 * NS + NS * NS>(P, R, [&](int q) { return rec[q]; }, i, j, k, cb);
+}
+
+template <int NS>
+__device__ inline void cell_body(const DevParams& P, const Fld& U, const Fld& R, int* err, int i, int j, int k,
+                                 unsigned cb)
+{
+    double c[NS + 5];
+#pragma unroll
+    for (int n = 0; n < NS + 5; ++n) c[n] = U(cb, n);
+    cell_record<NS>(P, c, R, err, i, j, k, cb);
 }
 
 // ---- nodes (flux.cpp:1627-1713) and the stochastic edge terms of each node -----------------
@@ -667,7 +676,7 @@ __device__ inline void face_body(const DevParams& P, const Fld& Us, const Fld& R
 template <int NS>
 __device__ inline void update_from_div(const DevParams& P, const Fld& out, const Fld& u0, const Fld& us,
                                        const double* div, double dt, int mode, const Fld& prim, int write_prim, int i,
-                                       int j, int k, unsigned cb)
+                                       int j, int k, unsigned cb, const Fld* rec_next = nullptr, int* err = nullptr)
 {
     const double src = 0.0;
     double o[NS + 5];
@@ -693,6 +702,8 @@ __device__ inline void update_from_div(const DevParams& P, const Fld& out, const
         o[4] += 2. / 3. * dt * (g[0] * r1 + g[1] * r2 + g[2] * r3);
     }
     store_cell<NS + 5>(P, out, [&](int n) { return o[n]; }, i, j, k, cb);
+    // the next stage's record depends on this cell's new state only: computed here from registers
+    if (rec_next) cell_record<NS>(P, o, *rec_next, err, i, j, k, cb);
     if (write_prim) {
         double pr[2 * NS + 6];
         cons_to_prim_cell<NS>(P, o, pr);
@@ -703,7 +714,8 @@ __device__ inline void update_from_div(const DevParams& P, const Fld& out, const
 template <int NS>
 __device__ inline void update_body(const DevParams& P, const Fld& out, const Fld& u0, const Fld& us, const Fld& fx,
                                    const Fld& fy, const Fld& fz, double dt, int mode, const Fld& prim, int write_prim,
-                                   int i, int j, int k, unsigned cb)
+                                   int i, int j, int k, unsigned cb, const Fld* rec_next = nullptr,
+                                   int* err = nullptr)
 {
     double div[NS + 5];
 #pragma unroll
@@ -711,7 +723,7 @@ __device__ inline void update_body(const DevParams& P, const Fld& out, const Fld
         // face n holds face 0 of the periodic box (store_face)
         div[n] = (fx(cb, n, 1, 0, 0) - fx(cb, n)) / P.dx[0] + (fy(cb, n, 0, 1, 0) - fy(cb, n)) / P.dx[1] +
                  (fz(cb, n, 0, 0, 1) - fz(cb, n)) / P.dx[2];
-    update_from_div<NS>(P, out, u0, us, div, dt, mode, prim, write_prim, i, j, k, cb);
+    update_from_div<NS>(P, out, u0, us, div, dt, mode, prim, write_prim, i, j, k, cb, rec_next, err);
 }
 
 }  // namespace fhdx

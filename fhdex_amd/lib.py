@@ -46,13 +46,15 @@ class Tune(C.Structure):
     """``fhdx_tune`` (include/fhdx.h): launch shape of the fast path."""
 
     _fields_ = [("tile_width", C.c_int32), ("region_tiles", C.c_int32), ("z_levels", C.c_int32),
-                ("noise_inline", C.c_int32), ("face_blocks_per_cu", C.c_int32), ("pad", C.c_int32),
+                ("noise_inline", C.c_int32), ("face_blocks_per_cu", C.c_int32),
+                ("records_in_update", C.c_int32),
                 ("ms_per_step", C.c_double)]
 
     def as_dict(self):
         return {"tile_width": self.tile_width, "region_tiles": self.region_tiles,
                 "z_levels": self.z_levels, "noise_inline": self.noise_inline,
-                "face_blocks_per_cu": self.face_blocks_per_cu, "ms_per_step": self.ms_per_step}
+                "face_blocks_per_cu": self.face_blocks_per_cu,
+                "records_in_update": self.records_in_update, "ms_per_step": self.ms_per_step}
 
 
 class FhdxError(RuntimeError):
@@ -193,8 +195,9 @@ class Context:
         return t, [trials[q] for q in range(n.value)]
 
     def set_tune(self, tile_width: int, region_tiles: int = 1, z_levels: int = 1, noise_inline: int = 1,
-                 face_blocks_per_cu: int = 0):
-        t = Tune(tile_width, region_tiles, z_levels, noise_inline, face_blocks_per_cu, 0, 0.0)
+                 face_blocks_per_cu: int = 0, records_in_update: int = 0):
+        t = Tune(tile_width, region_tiles, z_levels, noise_inline, face_blocks_per_cu,
+                 records_in_update, 0.0)
         check(lib().fhdx_ctx_set_tune(self.h, C.byref(t)), "fhdx_ctx_set_tune")
 
     def get_tune(self) -> Tune:

@@ -188,13 +188,15 @@ typedef struct fhdx_tune {
     int32_t z_levels;         /* 1..64                                                       */
     int32_t noise_inline;     /* 0 or 1                                                      */
     int32_t face_blocks_per_cu; /* 0: as many as registers allow; 2 or 3: cap (reserves LDS) */
-    int32_t pad;
+    int32_t records_in_update;  /* 1: stages 1-2 write the next stage's cell records in the
+                                   update kernel (one cell-kernel launch per step, not 3)    */
     double ms_per_step;       /* measured by fhdx_ctx_autotune (0 when set by hand)          */
 } fhdx_tune;
 
 /* Times candidate launch shapes on trial_steps fast-path steps of a scratch copy of cu (cu is
  * not modified; HIP events on `stream`): one coordinate at a time (tile width, region height,
- * z levels, noise, face blocks per CU), each swept with the others at their best so far.  Keeps the
+ * z levels, noise, face blocks per CU, records in update), each swept with the others at
+ * their best so far.  Keeps the
  * fastest for this context and returns it in *best; trials (may be NULL) receives up to max_trials timed candidates, n_trials their
  * count.  The choice is cached per (species count, box shape) for the process, so later
  * contexts of the same shape reuse it without timing (then n_trials = 0). */

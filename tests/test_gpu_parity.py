@@ -313,8 +313,9 @@ def test_launch_shapes_give_identical_bits(dev, orc):
     st = perturbed_state(p, cfg, orc, seed=21, amp=0.01, vel=500.0)
     ctx = L.Context(p)
     ref = None
-    shapes = [(tw, rh, kz, inl, 0) for tw in (16, 32, 64) for rh in (1, 3) for kz in (1, 3)
-              for inl in (0, 1)] + [(64, 1, 1, 1, 2), (32, 2, 3, 0, 3)]
+    shapes = [(tw, rh, kz, inl, 0, 0) for tw in (16, 32, 64) for rh in (1, 3) for kz in (1, 3)
+              for inl in (0, 1)] + [(64, 1, 1, 1, 2, 0), (32, 2, 3, 0, 3, 0), (64, 1, 1, 1, 0, 1),
+                                    (32, 2, 3, 0, 0, 1), (16, 1, 1, 1, 3, 1)]
     for shape in shapes:
         ctx.set_tune(*shape)
         cu = to_dev(torch, fc, st.cu, (-p.ngc,) * 3)
