@@ -74,6 +74,8 @@ def config_rate(cfg, steps: int, warmup: int = 2):
     from fhdex_amd import lib as L
     p = cfg.params
     ctx = L.Context(p)
+    if cfg.chem is not None:
+        ctx.set_chemistry(cfg.chem)
     cu = fc.MultiFab(torch.from_numpy(E.uniform_cons(cfg)).cuda(), (-p.ngc,) * 3)
     prim = fc.MultiFab.cell(p, p.nprimvars)
     best, _ = ctx.autotune(cu.fab, cfg.dt, trial_steps=2)
@@ -369,7 +371,8 @@ def main():
     ctx.close()
     del cu, prim
     if not a.no_configs:
-        line["configs"] = [config_rate(P.argon_gas(32), 50), config_rate(P.four_species(128), 10)]
+        line["configs"] = [config_rate(P.argon_gas(32), 50), config_rate(P.four_species(128), 10),
+                           config_rate(P.no2_n2o4(128), 10)]
     if a.ensemble:
         line["ensemble"] = ensemble_bench(a.ensemble, a.ensemble_steps, a.loop_steps)
     rates = [float(v) for v in a.serve_rates.split(",") if v]

@@ -217,3 +217,20 @@ def ReadCheckPoint(p: Params, restart: int, cu: MultiFab, prim: MultiFab, st: St
                                      C.byref(seed), C.byref(has), C.byref(f), host, stream),
           "ReadCheckPoint")
     return step.value, t.value, sc.value, (seed.value if has.value else None)
+
+
+# ---- chemistry (chemistry_functions.cpp:128-193) ------------------------------------------------
+
+def compute_compressible_chemistry_source_CLE(ctx: Context, dt: float, prim: MultiFab,
+                                              source: MultiFab, ranchem: MultiFab, stream=None):
+    """The species source (components 5..) of the valid cells from prim and the weighted
+    reaction noise ranchem; the context's reactions come from Context.set_chemistry."""
+    check(lib().fhdx_chemistry_source(ctx.h, prim.fab, source.fab, ranchem.fab, float(dt), stream),
+          "compute_compressible_chemistry_source_CLE")
+
+
+def fill_chem_noise(ctx: Context, seed: int, step: int, A: MultiFab, B: MultiFab, instance: int = 0,
+                    stream=None):
+    """The reaction noise sets of key (seed, step, instance), as the steps draw them."""
+    check(lib().fhdx_fill_chem_noise(ctx.h, seed, step, instance, A.fab, B.fab, stream),
+          "fill_chem_noise")

@@ -34,6 +34,7 @@ struct RefWorkspace {
     fhdx_fab cup, cup2, prim, eta, zeta, kappa, chi, D;
     fhdx_fab flux[3], stoch[3], sA[3], sB[3], corn[3][3];
     fhdx_fab visccorn, rancorn, rA, rB;
+    fhdx_fab source, ranA, ranB, ran;  // chemistry (nreaction > 0): source, reaction noise A, B, A + w B
 };
 
 struct fhdx_ctx {
@@ -45,6 +46,7 @@ struct fhdx_ctx {
     FusedWorkspace fused;
     double *st_yz = nullptr, *st_part = nullptr, *st_misc = nullptr;  // evaluateStats scratch
     int st_nparts = 0;
+    bool chem_set = false;  // fhdx_ctx_set_chemistry called (required when p.nreaction > 0)
 };
 
 extern "C" {
@@ -149,8 +151,8 @@ int fhdx_params_check(const fhdx_params* p)
         set_error("only transport_type 1 (Giovangigli) is implemented");
         return FHDX_ERR_UNSUPPORTED;
     }
-    if (p->nreaction != 0) {
-        set_error("chemistry (nreaction > 0) is outside this path");
+    if (p->nreaction < 0 || p->nreaction > FHDX_MAX_REACTION) {
+        set_error("nreaction must be 0..FHDX_MAX_REACTION");
         return FHDX_ERR_UNSUPPORTED;
     }
     if (p->advection_type < 0 || p->advection_type > 2) return FHDX_ERR_ARG;
@@ -250,6 +252,12 @@ int ensure_ref_workspace(fhdx_ctx* c)
     node(w.rancorn);
     node(w.rA);
     node(w.rB);
+    if (c->p.nreaction > 0) {
+        cell(w.source, nv);  // components 0..4 stay zero (main_driver.cpp:241-242)
+        cell(w.ranA, c->p.nreaction);
+        cell(w.ranB, c->p.nreaction);
+        cell(w.ran, c->p.nreaction);
+    }
     if (r) return r;
     FHDX_HIP_TRY(hipDeviceSynchronize());
     w.ready = true;
@@ -275,6 +283,12 @@ int rk3_ref(fhdx_ctx* c, fhdx_fab cu, fhdx_fab cup, fhdx_fab cup2, fhdx_fab prim
             const fhdx_fab cornz[3], fhdx_fab visccorn, fhdx_fab rancorn, const fhdx_fab sA[3],
             const fhdx_fab sB[3], fhdx_fab rA, fhdx_fab rB, double dt, hipStream_t s)
 {
+    const bool chem = c->p.nreaction > 0;
+    int r0 = chem ? ensure_ref_workspace(c) : FHDX_OK;
+    if (r0) return r0;
+    const RefWorkspace& rw = c->ref;
+    Arr src{};
+    if (chem) src = make_arr(rw.source);
     const DevParams& P = c->dp;
     const int nv = P.nvars;
     double ws[3];
@@ -307,7 +321,12 @@ int rk3_ref(fhdx_ctx* c, fhdx_fab cu, fhdx_fab cup, fhdx_fab cup2, fhdx_fab prim
         if ((r = ref_lincomb(rancorn.p, rA.p, rB.p, npts(rancorn), 1, 0, ws[st], s))) return r;
         a.cons = make_arr(in[st]);
         if ((r = ref_calculate_flux(P, a, s))) return r;
-        if ((r = ref_stage(P, make_arr(out[st]), make_arr(cu), make_arr(in[st]), fl, dt, st, s))) return r;
+        if (chem) {  // timeStep.cpp:135-143: ranchem = A + w B, then the source from this stage's prim
+            if ((r = ref_lincomb(rw.ran.p, rw.ranA.p, rw.ranB.p, npts(rw.ran), c->p.nreaction, 0, ws[st], s)))
+                return r;
+            if ((r = ref_chem_source(P, a.prim, src, make_arr(rw.ran), dt, s))) return r;
+        }
+        if ((r = ref_stage(P, make_arr(out[st]), make_arr(cu), make_arr(in[st]), fl, src, dt, st, s))) return r;
         if ((r = ref_c2p(P, a.prim, make_arr(out[st]), s))) return r;
         if ((r = ref_fill_boundary(P, make_arr(out[st]), nv, s))) return r;
         if ((r = ref_fill_boundary(P, a.prim, P.nprim, s))) return r;
@@ -478,7 +497,15 @@ int fhdx_rk3_step_ref(fhdx_ctx* c, fhdx_fab cu, fhdx_fab cup, fhdx_fab cup2, fhd
     if (!ok || !(dt > 0.)) return FHDX_ERR_ARG;
     hipStream_t s = S(stream);
     const bool injected = stochA != nullptr;
+    if (c->p.nreaction > 0 && !c->chem_set) {
+        set_error("nreaction > 0: call fhdx_ctx_set_chemistry before stepping");
+        return FHDX_ERR_ARG;
+    }
     if (injected) {
+        if (c->p.nreaction > 0) {
+            set_error("caller noise with chemistry: the reaction noise has no array argument");
+            return FHDX_ERR_UNSUPPORTED;
+        }
         if (!stochB || !rancornA || !rancornB) return FHDX_ERR_ARG;
         for (int d = 0; d < 3; ++d)
             if (!check_face(c, stochA[d], d, nv, "stochA") || !check_face(c, stochB[d], d, nv, "stochB"))
@@ -491,6 +518,9 @@ int fhdx_rk3_step_ref(fhdx_ctx* c, fhdx_fab cu, fhdx_fab cup, fhdx_fab cup2, fhd
     if (r) return r;
     RefWorkspace& w = c->ref;
     if ((r = fhdx_fill_noise(c, seed, step, 0, w.sA, w.sB, w.rA, w.rB, stream))) return r;
+    if (c->p.nreaction > 0 &&
+        (r = ref_fill_chem_noise(c->dp, make_noise_key(seed, step, 0), make_arr(w.ranA), make_arr(w.ranB), s)))
+        return r;
     return rk3_ref(c, cu, cup, cup2, prim, eta, zeta, kappa, chi, D, flux, stochFlux, cornx, corny, cornz,
                    visccorn, rancorn, w.sA, w.sB, w.rA, w.rB, dt, s);
 }
@@ -500,6 +530,10 @@ int fhdx_rk3_step(fhdx_ctx* c, fhdx_fab cu, fhdx_fab prim, double dt, uint64_t s
     if (!c) return FHDX_ERR_ARG;
     if (!check_cell(c, cu, c->dp.nvars, "cu") || !check_cell(c, prim, c->dp.nprim, "prim") || !(dt > 0.))
         return FHDX_ERR_ARG;
+    if (c->p.nreaction > 0 && !c->chem_set) {
+        set_error("nreaction > 0: call fhdx_ctx_set_chemistry before stepping");
+        return FHDX_ERR_ARG;
+    }
     return fused_rk3_step(c->p, c->dp, c->fused, cu, prim, dt, make_noise_key(seed, step, 0), nullptr, nullptr,
                           nullptr, nullptr, c->d_err, S(stream));
 }
@@ -508,6 +542,10 @@ int fhdx_rk3_step_injected(fhdx_ctx* c, fhdx_fab cu, fhdx_fab prim, const fhdx_f
                            const fhdx_fab stochB[3], fhdx_fab rancornA, fhdx_fab rancornB, double dt, void* stream)
 {
     if (!c || !stochA || !stochB) return FHDX_ERR_ARG;
+    if (c->p.nreaction > 0) {
+        set_error("caller noise with chemistry: the reaction noise has no array argument");
+        return FHDX_ERR_UNSUPPORTED;
+    }
     const int nv = c->dp.nvars;
     bool ok = check_cell(c, cu, nv, "cu") && check_cell(c, prim, c->dp.nprim, "prim") &&
               check_node(c, rancornA, "rancornA") && check_node(c, rancornB, "rancornB") && dt > 0.;
@@ -516,6 +554,75 @@ int fhdx_rk3_step_injected(fhdx_ctx* c, fhdx_fab cu, fhdx_fab prim, const fhdx_f
     if (!ok) return FHDX_ERR_ARG;
     return fused_rk3_step(c->p, c->dp, c->fused, cu, prim, dt, make_noise_key(0, 0, 0), stochA, stochB, &rancornA,
                           &rancornB, c->d_err, S(stream));
+}
+
+// ---- chemistry ------------------------------------------------------------------------------
+
+int fhdx_ctx_set_chemistry(fhdx_ctx* c, const fhdx_chemistry* ch)
+{
+    if (!c || !ch) return FHDX_ERR_ARG;
+    const int nr = ch->nreaction, NS = c->p.nspecies;
+    if (nr < 1 || nr > FHDX_MAX_REACTION || nr != c->p.nreaction) {
+        set_error("fhdx_ctx_set_chemistry: nreaction must be 1..FHDX_MAX_REACTION and equal fhdx_params.nreaction");
+        return FHDX_ERR_ARG;
+    }
+    if (ch->reaction_type != 0 && ch->reaction_type != 1) {  // chemistry_functions.cpp:131-133
+        set_error("fhdx_ctx_set_chemistry: reaction_type must be 0 (deterministic) or 1 (CLE)");
+        return FHDX_ERR_UNSUPPORTED;
+    }
+    if (!(ch->T0_chem > 0.)) {  // chemistry_functions.cpp:135
+        set_error("fhdx_ctx_set_chemistry: T0_chem > 0 expected");
+        return FHDX_ERR_ARG;
+    }
+    DevChem d{};
+    d.nreaction = nr;
+    d.reaction_type = ch->reaction_type;
+    for (int m = 0; m < nr; ++m) {
+        for (int n = 0; n < NS; ++n) {
+            const int R = ch->stoich_R[m][n], Pp = ch->stoich_P[m][n];
+            if (R < 0 || Pp < 0 || R > 100 || Pp > 100) {
+                set_error("fhdx_ctx_set_chemistry: stoichiometric coefficients must be 0..100");
+                return FHDX_ERR_ARG;
+            }
+            d.R[m][n] = (signed char)R;
+            d.PR[m][n] = (signed char)(Pp - R);
+        }
+        d.rate[m] = ch->rate_const[m];
+        d.alpha[m] = ch->alpha_param[m];
+        d.beta[m] = ch->beta_param[m];
+    }
+    d.T0 = ch->T0_chem;
+    d.Navo = c->p.Runiv / c->p.k_B;
+    c->dp.chem = d;
+    c->chem_set = true;
+    return FHDX_OK;
+}
+
+int fhdx_chemistry_source(fhdx_ctx* c, fhdx_fab prim, fhdx_fab source, fhdx_fab ranchem, double dt, void* stream)
+{
+    if (!c || !(dt > 0.)) return FHDX_ERR_ARG;
+    if (!c->chem_set) {
+        set_error("fhdx_chemistry_source: call fhdx_ctx_set_chemistry first");
+        return FHDX_ERR_ARG;
+    }
+    if (!check_cell(c, prim, c->dp.nprim, "prim") || !check_cell(c, source, c->dp.nvars, "source") ||
+        !check_cell(c, ranchem, c->p.nreaction, "ranchem"))
+        return FHDX_ERR_ARG;
+    return ref_chem_source(c->dp, make_arr(prim), make_arr(source), make_arr(ranchem), dt, S(stream));
+}
+
+int fhdx_fill_chem_noise(fhdx_ctx* c, uint64_t seed, uint64_t step, uint32_t instance, fhdx_fab ranchemA,
+                         fhdx_fab ranchemB, void* stream)
+{
+    if (!c) return FHDX_ERR_ARG;
+    if (!c->chem_set) {
+        set_error("fhdx_fill_chem_noise: call fhdx_ctx_set_chemistry first");
+        return FHDX_ERR_ARG;
+    }
+    if (!check_cell(c, ranchemA, c->p.nreaction, "ranchemA") || !check_cell(c, ranchemB, c->p.nreaction, "ranchemB"))
+        return FHDX_ERR_ARG;
+    return ref_fill_chem_noise(c->dp, make_noise_key(seed, step, instance), make_arr(ranchemA), make_arr(ranchemB),
+                               S(stream));
 }
 
 // ---- evaluateStats ----------------------------------------------------------------------

@@ -218,6 +218,10 @@ int engine_add(Engine* e, const fhdx_params& p, double dt, uint64_t seed, uint64
     }
     int r = fhdx_params_check(&p);
     if (r) return r;
+    if (p.nreaction > 0) {
+        set_error("ensemble instances with chemistry (nreaction > 0) are not supported");
+        return FHDX_ERR_UNSUPPORTED;
+    }
     const uint64_t need = engine_bytes_needed(p);
     if (e->used + need > e->budget) {
         set_error("ensemble memory budget exhausted");

@@ -217,14 +217,19 @@ __global__ void __launch_bounds__(FB) k_faces(DevParams P, TileMap tm, Grid G, F
 // ---- K4: stage combination (+ primitive variables after the last stage) --------------------
 
 // REC: also write the next stage's cell records (k_cells of that stage is then skipped).
-template <int NS, bool ZM, bool REC>
+// With chemistry (P.chem.nreaction > 0) the species source of the stage is evaluated from the
+// stage's records (read before REC overwrites a cell's record) and in-place noise of key K.
+template <int NS, bool ZM, bool REC, bool CHEM>
 __global__ void __launch_bounds__(FB) k_update(DevParams P, TileMap tm, Grid G, double* out, double* u0,
                                                double* us, double* fx, double* fy, double* fz, double dt, int mode,
-                                               double* prim, int write_prim, double* rec, int* err)
+                                               double* prim, int write_prim, double* rec, int* err, NoiseKey K,
+                                               double w, double dV)
 {
     const Fld rn = G.fld(rec);
+    const ChemStage ch{rn, K, w, dV};
     TILE_LEVELS(tm, update_body<NS>(P, G.fld(out), G.fld(u0), G.fld(us), G.fld(fx), G.fld(fy), G.fld(fz), dt, mode,
-                                    G.fld(prim), write_prim, i, j, k, G.off(i, j, k), REC ? &rn : nullptr, err);)
+                                    G.fld(prim), write_prim, i, j, k, G.off(i, j, k), REC ? &rn : nullptr, err,
+                                    CHEM ? &ch : nullptr);)
 }
 
 // ---- host side ---------------------------------------------------------------------------
@@ -325,16 +330,21 @@ static void launch_stage(const DevParams& P, FusedWorkspace& w, const Grid& G, d
     else k_faces<NS, INL, false><<<ntile, FB, lds, s>>>(P, tm, G, fa);
     T.end(s);
     T.begin(KT_UPDATE, s);
-#define FHDX_UPD(ZMV, RECV)                                                                                    \
-    k_update<NS, ZMV, RECV><<<ntile, FB, 0, s>>>(P, tm, G, out, cu, us, w.F[0], w.F[1], w.F[2], dt, st, prim, \
-                                                 st == 2, w.rec, d_err)
+#define FHDX_UPD(ZMV, RECV, CHEMV)                                                                     \
+    k_update<NS, ZMV, RECV, CHEMV><<<ntile, FB, 0, s>>>(P, tm, G, out, cu, us, w.F[0], w.F[1], w.F[2], dt, st, \
+                                                        prim, st == 2, w.rec, d_err, K, wst,                   \
+                                                        P.dx[0] * P.dx[1] * P.dx[2])
     const bool rec_next = w.rec_in_update && st < 2;
-    if (zm) {
-        if (rec_next) FHDX_UPD(true, true);
-        else FHDX_UPD(true, false);
-    } else {
-        if (rec_next) FHDX_UPD(false, true);
-        else FHDX_UPD(false, false);
+    const int variant = (zm ? 4 : 0) + (rec_next ? 2 : 0) + (P.chem.nreaction > 0 ? 1 : 0);
+    switch (variant) {
+        case 0: FHDX_UPD(false, false, false); break;
+        case 1: FHDX_UPD(false, false, true); break;
+        case 2: FHDX_UPD(false, true, false); break;
+        case 3: FHDX_UPD(false, true, true); break;
+        case 4: FHDX_UPD(true, false, false); break;
+        case 5: FHDX_UPD(true, false, true); break;
+        case 6: FHDX_UPD(true, true, false); break;
+        default: FHDX_UPD(true, true, true); break;
     }
 #undef FHDX_UPD
     T.end(s);

@@ -20,8 +20,12 @@ int ref_transport(const DevParams& P, Arr prim, Arr eta, Arr zeta, Arr kappa, Ar
 int ref_fill_boundary(const DevParams& P, Arr a, int nc, hipStream_t s);
 int fill_value(double* p, long n, double v, hipStream_t s);
 int ref_calculate_flux(const DevParams& P, const FluxArgs& a, hipStream_t s);
-int ref_stage(const DevParams& P, Arr out, Arr u0, Arr us, const Arr flux[3], double dt, int mode,
+// source: the source MultiFab (p == nullptr: zero)
+int ref_stage(const DevParams& P, Arr out, Arr u0, Arr us, const Arr flux[3], Arr source, double dt, int mode,
               hipStream_t s);
+// compute_compressible_chemistry_source_CLE over the valid cells; reaction noise sets A, B.
+int ref_chem_source(const DevParams& P, Arr prim, Arr source, Arr ranchem, double dt, hipStream_t s);
+int ref_fill_chem_noise(const DevParams& P, const NoiseKey& K, Arr A, Arr B, hipStream_t s);
 int ref_lincomb(double* dst, const double* a, const double* b, long npts, int nc, int c0, double w,
                 hipStream_t s);
 int ref_fill_noise(const DevParams& P, const NoiseKey& K, const Arr A[3], const Arr B[3], Arr rA, Arr rB,

@@ -18,6 +18,16 @@ namespace fhdx {
 constexpr int MS = FHDX_MAX_SPECIES;
 
 // Kernel-argument copy of the run parameters plus per-call constants.
+constexpr int MR = 8;  // reactions (FHDX_MAX_REACTION)
+
+// Reaction data of the chemistry source (chemistry_namespace.H:1-39); nreaction 0: none.
+struct DevChem {
+    int nreaction, reaction_type;
+    signed char R[MR][8], PR[MR][8];  // stoich_coeffs_R and P - R (chemistry_functions.cpp:86-88)
+    double rate[MR], alpha[MR], beta[MR];
+    double T0, Navo;                  // T0_chem; avogadro = Runiv / k_B (common_functions.cpp:775)
+};
+
 struct DevParams {
     int ns, nvars, nprim, ngc;
     int n[3];
@@ -27,6 +37,7 @@ struct DevParams {
     double k_B, Runiv, zeta_ratio;
     double grav[3];
     double var_mom, var_ener, var_mass;  // sqrt(coef*coef), the MultiFabFillRandom scale
+    DevChem chem;
 };
 
 // Strided view of one FAB (AMReX Array4 layout: x fastest, then y, z, component).
@@ -517,8 +528,10 @@ __device__ inline void normal_pair(U4 r, double& z0, double& z1)
     z1 = rad * s;
 }
 
-// Noise keys: field = 16*dir + comp for face components 1..nvars-1, 48 for the node field.
+// Noise keys: field = 16*dir + comp for face components 1..nvars-1, 48 for the node field,
+// 64 + m for reaction m.
 constexpr uint32_t NODE_FIELD = 48u;
+constexpr uint32_t CHEM_FIELD = 64u;
 
 struct NoiseKey {
     uint32_t k0, k1;     // seed
@@ -533,6 +546,100 @@ __device__ inline void noise_ab(const NoiseKey& K, uint32_t field, uint32_t lin,
 {
     U4 c{lin, K.step_lo, field | K.field_hi, K.instance};
     normal_pair(philox4x32_10(c, K.k0, K.k1), a, b);
+}
+
+
+// ---- chemistry source (chemistry_functions.cpp:128-193) -------------------------------------
+//
+// exp and pow are evaluated here (and identically in the oracle) with IEEE operations only, so
+// host and device agree bit for bit.  e^x: x = k ln2 + r with |r| <= ln2/2, Taylor series of e^r
+// to r^14 (truncation below 1e-19 relative), times 2^k.  x^y = e^(y ln x) with the Box-Muller
+// logarithm above (valid for any positive normal x); y = 0 gives 1 and y = 1 gives x exactly.
+__device__ inline double chem_exp(double x)
+{
+    if (x == 0.0) return 1.0;
+    if (x > 709.0) return __longlong_as_double(0x7ff0000000000000ll);
+    if (x < -745.0) return 0.0;
+    const double k = rint(x * 1.4426950408889634);
+    const double r = fma(-k, 4.7493250390316726e-07, fma(-k, 0.6931467056274414, x));
+    double p = 1.1470745597729725e-11;
+    p = fma(p, r, 1.6059043836821613e-10);
+    p = fma(p, r, 2.08767569878681e-09);
+    p = fma(p, r, 2.505210838544172e-08);
+    p = fma(p, r, 2.755731922398589e-07);
+    p = fma(p, r, 2.7557319223985893e-06);
+    p = fma(p, r, 2.48015873015873e-05);
+    p = fma(p, r, 0.0001984126984126984);
+    p = fma(p, r, 0.001388888888888889);
+    p = fma(p, r, 0.008333333333333333);
+    p = fma(p, r, 0.041666666666666664);
+    p = fma(p, r, 0.16666666666666666);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    // times 2^k as two exact power-of-two factors (each within the normal range)
+    const int ki = (int)k, k1 = ki / 2, k2 = ki - k1;
+    const double s1 = __longlong_as_double((long long)(k1 + 1023) << 52);
+    const double s2 = __longlong_as_double((long long)(k2 + 1023) << 52);
+    return p * s1 * s2;
+}
+
+__device__ inline double chem_pow(double x, double y)
+{
+    if (y == 0.0) return 1.0;
+    if (y == 1.0) return x;
+    if (x == 0.0) return y > 0.0 ? 0.0 : __longlong_as_double(0x7ff0000000000000ll);
+    if (x < 0.0) return __longlong_as_double(0x7ff8000000000000ll);
+    return chem_exp(y * bm_log(x));
+}
+
+// c^s for an integer stoichiometric coefficient s >= 0 (pow(ck, stoich) of :167).
+__device__ inline double chem_ipow(double c, int s)
+{
+    if (s == 0) return 1.0;
+    double r = c;
+    for (int q = 1; q < s; ++q) r = r * c;
+    return r;
+}
+
+// Species source of one cell (chemistry_functions.cpp:147-190), operation order kept: T, p
+// and X_n of the cell, ranchem(m) the weighted noise of reaction m, dV the cell volume.
+template <int NS, typename Noise>
+__device__ inline void chem_source_cell(const DevParams& P, double T, double pres, const double* X,
+                                        Noise ranchem, double dt, double dV, double* src)
+{
+    const DevChem& C = P.chem;
+    const double T0 = C.T0;
+    const double ctot = pres / P.Runiv / T;
+    const double Navo = C.Navo;
+    double ck[NS];
+#pragma unroll
+    for (int n = 0; n < NS; ++n) ck[n] = X[n] * ctot;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) src[n] = 0.;
+    for (int m = 0; m < C.nreaction; ++m) {
+        double rate = C.rate[m];
+        rate *= chem_exp(-C.alpha[m] / P.Runiv * (1 / T - 1 / T0));
+        rate *= chem_pow(T / T0, C.beta[m]);
+#pragma unroll
+        for (int n = 0; n < NS; ++n) rate *= chem_ipow(ck[n], C.R[m][n]);
+        rate = (0. < rate) ? rate : 0.;  // std::max(0., rate)
+        const double W = ranchem(m) / sqrt(dt * dV);
+#pragma unroll
+        for (int n = 0; n < NS; ++n) {
+            src[n] += P.molmass[n] * C.PR[m][n] * rate;
+            if (C.reaction_type == 1) src[n] += P.molmass[n] * C.PR[m][n] * sqrt(rate / Navo) * W;
+        }
+    }
+}
+
+// The weighted reaction noise A + w B of reaction m at linear cell index lin (as the stochastic
+// fluxes: MultiFabFillRandom with variance 1, then LinComb, timeStep.cpp:93-97, 135-140).
+__device__ inline double chem_noise(const NoiseKey& K, int m, uint32_t lin, double w)
+{
+    double za, zb;
+    noise_ab(K, CHEM_FIELD + (uint32_t)m, lin, za, zb);
+    return 1.0 * za + w * zb;
 }
 
 }  // namespace fhdx

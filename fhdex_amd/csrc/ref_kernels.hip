@@ -497,18 +497,19 @@ __global__ void __launch_bounds__(BLK) k_hyp(DevParams P, Arr cons, Arr prim, Ar
 
 // ---- stage combination (timeStep.cpp:158-201, 266-306, 370-413) ----------------------------
 
+// src: the source MultiFab (null: zero, main_driver.cpp:241-242)
 template <int NS>
 __global__ void __launch_bounds__(BLK) k_stage(DevParams P, Arr out, Arr u0, Arr us, Arr fx, Arr fy,
-                                               Arr fz, double dt, int mode)
+                                               Arr fz, Arr srcf, double dt, int mode)
 {
     long t = (long)blockIdx.x * BLK + threadIdx.x;
     if (t >= (long)P.n[0] * P.n[1] * P.n[2]) return;
     int i, j, k;
     face_index(t, P.n[0], P.n[1], i, j, k);
-    const double src = 0.0;
     double o[NS + 5];
 #pragma unroll
     for (int n = 0; n < NS + 5; ++n) {
+        const double src = srcf.p ? srcf(i, j, k, n) : 0.0;
         double div = (fx(i + 1, j, k, n) - fx(i, j, k, n)) / P.dx[0] +
                      (fy(i, j + 1, k, n) - fy(i, j, k, n)) / P.dx[1] +
                      (fz(i, j, k + 1, n) - fz(i, j, k, n)) / P.dx[2];
@@ -533,6 +534,41 @@ __global__ void __launch_bounds__(BLK) k_stage(DevParams P, Arr out, Arr u0, Arr
     }
 #pragma unroll
     for (int n = 0; n < NS + 5; ++n) out(i, j, k, n) = o[n];
+}
+
+// ---- chemistry source (chemistry_functions.cpp:128-193) ------------------------------------
+
+template <int NS>
+__global__ void __launch_bounds__(BLK) k_chem_source(DevParams P, Arr prim, Arr source, Arr ranchem, double dt,
+                                                     double dV)
+{
+    long t = (long)blockIdx.x * BLK + threadIdx.x;
+    if (t >= (long)P.n[0] * P.n[1] * P.n[2]) return;
+    int i, j, k;
+    face_index(t, P.n[0], P.n[1], i, j, k);
+    double X[NS], src[NS];
+#pragma unroll
+    for (int n = 0; n < NS; ++n) X[n] = prim(i, j, k, 6 + NS + n);
+    chem_source_cell<NS>(P, prim(i, j, k, 4), prim(i, j, k, 5), X, [&](int m) { return ranchem(i, j, k, m); }, dt,
+                         dV, src);
+#pragma unroll
+    for (int n = 0; n < NS; ++n) source(i, j, k, 5 + n) = src[n];
+}
+
+// Reaction noise sets A, B (MultiFabFillRandom(ranchem_A/B, m, 1.0), timeStep.cpp:93-97).
+__global__ void __launch_bounds__(BLK) k_chem_noise(DevParams P, NoiseKey K, Arr A, Arr B)
+{
+    long t = (long)blockIdx.x * BLK + threadIdx.x;
+    if (t >= (long)P.n[0] * P.n[1] * P.n[2]) return;
+    int i, j, k;
+    face_index(t, P.n[0], P.n[1], i, j, k);
+    const uint32_t lin = (uint32_t)(((long)k * P.n[1] + j) * P.n[0] + i);
+    for (int m = 0; m < P.chem.nreaction; ++m) {
+        double za, zb;
+        noise_ab(K, CHEM_FIELD + (uint32_t)m, lin, za, zb);
+        A(i, j, k, m) = za;
+        B(i, j, k, m) = zb;
+    }
 }
 
 // ---- noise --------------------------------------------------------------------------------
@@ -684,11 +720,27 @@ int ref_calculate_flux(const DevParams& P, const FluxArgs& a, hipStream_t s)
     return FHDX_OK;
 }
 
-int ref_stage(const DevParams& P, Arr out, Arr u0, Arr us, const Arr flux[3], double dt, int mode,
+int ref_stage(const DevParams& P, Arr out, Arr u0, Arr us, const Arr flux[3], Arr source, double dt, int mode,
               hipStream_t s)
 {
     NS_DISPATCH(P.ns, (k_stage<NS><<<grid_for(ncell(P), BLK), BLK, 0, s>>>(P, out, u0, us, flux[0], flux[1],
-                                                                             flux[2], dt, mode)));
+                                                                             flux[2], source, dt, mode)));
+    FHDX_HIP_TRY(hipGetLastError());
+    return FHDX_OK;
+}
+
+int ref_chem_source(const DevParams& P, Arr prim, Arr source, Arr ranchem, double dt, hipStream_t s)
+{
+    const double dV = P.dx[0] * P.dx[1] * P.dx[2];
+    NS_DISPATCH(P.ns, (k_chem_source<NS><<<grid_for(ncell(P), BLK), BLK, 0, s>>>(P, prim, source, ranchem, dt,
+                                                                                   dV)));
+    FHDX_HIP_TRY(hipGetLastError());
+    return FHDX_OK;
+}
+
+int ref_fill_chem_noise(const DevParams& P, const NoiseKey& K, Arr A, Arr B, hipStream_t s)
+{
+    k_chem_noise<<<grid_for(ncell(P), BLK), BLK, 0, s>>>(P, K, A, B);
     FHDX_HIP_TRY(hipGetLastError());
     return FHDX_OK;
 }

@@ -672,19 +672,40 @@ __device__ inline void face_body(const DevParams& P, const Fld& Us, const Fld& R
 
 // ---- stage combination (+ primitive variables after the last stage) -----------------------
 
+// What a stage update needs for the chemistry source: the stage's cell records (T, p, X of the
+// stage input), the noise key and weight of the stage, and the cell volume.
+struct ChemStage {
+    Fld R;
+    NoiseKey K;
+    double w, dV;
+};
+
 // div[] = (fx(i+1)-fx(i))/dx + (fy(j+1)-fy(j))/dy + (fz(k+1)-fz(k))/dz, evaluated left to right.
 template <int NS>
 __device__ inline void update_from_div(const DevParams& P, const Fld& out, const Fld& u0, const Fld& us,
                                        const double* div, double dt, int mode, const Fld& prim, int write_prim, int i,
-                                       int j, int k, unsigned cb, const Fld* rec_next = nullptr, int* err = nullptr)
+                                       int j, int k, unsigned cb, const Fld* rec_next = nullptr, int* err = nullptr,
+                                       const ChemStage* chem = nullptr)
 {
-    const double src = 0.0;
+    // source S: zero (main_driver.cpp:241-242) but for the species equations under chemistry
+    double src[NS + 5];
+#pragma unroll
+    for (int n = 0; n < NS + 5; ++n) src[n] = 0.0;
+    if (chem && P.chem.nreaction > 0) {
+        const RecLayout L{NS};
+        double X[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) X[s] = chem->R(cb, L.X() + s);
+        const uint32_t lin = (uint32_t)(((long)k * P.n[1] + j) * P.n[0] + i);
+        chem_source_cell<NS>(P, chem->R(cb, RecLayout::T), chem->R(cb, RecLayout::P), X,
+                             [&](int m) { return chem_noise(chem->K, m, lin, chem->w); }, dt, chem->dV, &src[5]);
+    }
     double o[NS + 5];
 #pragma unroll
     for (int n = 0; n < NS + 5; ++n) {
-        if (mode == 0) o[n] = us(cb, n) - dt * div[n] + dt * src;
-        else if (mode == 1) o[n] = 0.25 * (3.0 * u0(cb, n) + us(cb, n) - dt * div[n] + dt * src);
-        else o[n] = (2. / 3.) * (0.5 * u0(cb, n) + us(cb, n) - dt * div[n] + dt * src);
+        if (mode == 0) o[n] = us(cb, n) - dt * div[n] + dt * src[n];
+        else if (mode == 1) o[n] = 0.25 * (3.0 * u0(cb, n) + us(cb, n) - dt * div[n] + dt * src[n]);
+        else o[n] = (2. / 3.) * (0.5 * u0(cb, n) + us(cb, n) - dt * div[n] + dt * src[n]);
     }
     const double* g = P.grav;
     double r0 = us(cb, 0), r1 = us(cb, 1), r2 = us(cb, 2), r3 = us(cb, 3);
@@ -715,7 +736,7 @@ template <int NS>
 __device__ inline void update_body(const DevParams& P, const Fld& out, const Fld& u0, const Fld& us, const Fld& fx,
                                    const Fld& fy, const Fld& fz, double dt, int mode, const Fld& prim, int write_prim,
                                    int i, int j, int k, unsigned cb, const Fld* rec_next = nullptr,
-                                   int* err = nullptr)
+                                   int* err = nullptr, const ChemStage* chem = nullptr)
 {
     double div[NS + 5];
 #pragma unroll
@@ -723,7 +744,7 @@ __device__ inline void update_body(const DevParams& P, const Fld& out, const Fld
         // face n holds face 0 of the periodic box (store_face)
         div[n] = (fx(cb, n, 1, 0, 0) - fx(cb, n)) / P.dx[0] + (fy(cb, n, 0, 1, 0) - fy(cb, n)) / P.dx[1] +
                  (fz(cb, n, 0, 0, 1) - fz(cb, n)) / P.dx[2];
-    update_from_div<NS>(P, out, u0, us, div, dt, mode, prim, write_prim, i, j, k, cb, rec_next, err);
+    update_from_div<NS>(P, out, u0, us, div, dt, mode, prim, write_prim, i, j, k, cb, rec_next, err, chem);
 }
 
 }  // namespace fhdx

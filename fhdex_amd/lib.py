@@ -8,7 +8,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-from .params import Fab, Params
+from .params import Chemistry, Fab, Params
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfhdx.so")
@@ -27,6 +27,7 @@ EXPORTS = [
     "fhdx_server_submit", "fhdx_server_poll", "fhdx_server_wait", "fhdx_server_stats",
     "fhdx_server_destroy", "fhdx_ctx_autotune", "fhdx_ctx_set_tune", "fhdx_ctx_get_tune",
     "fhdx_evaluate_stats", "fhdx_write_plotfile", "fhdx_write_checkpoint", "fhdx_read_checkpoint",
+    "fhdx_ctx_set_chemistry", "fhdx_chemistry_source", "fhdx_fill_chem_noise",
 ]
 
 _lib = None
@@ -121,6 +122,9 @@ def lib():
         L.fhdx_evaluate_stats.argtypes = [vp, Fab, Fab, Fab, Fab, Fab, Fab, Fab, Fab,
                                           C.POINTER(C.c_double), C.c_int, C.c_int, vp]
         L.fhdx_ctx_get_tune.argtypes = [vp, C.POINTER(Tune)]
+        L.fhdx_ctx_set_chemistry.argtypes = [vp, C.POINTER(Chemistry)]
+        L.fhdx_chemistry_source.argtypes = [vp, Fab, Fab, Fab, C.c_double, vp]
+        L.fhdx_fill_chem_noise.argtypes = [vp, C.c_uint64, C.c_uint64, C.c_uint32, Fab, Fab, vp]
         PFd = C.POINTER(Fields)
         L.fhdx_write_plotfile.argtypes = [P, C.c_char_p, C.c_int, C.c_double, C.POINTER(C.c_double),
                                           PFd, C.c_int, C.c_int, C.c_int, vp]
@@ -199,6 +203,10 @@ class Context:
         t = Tune(tile_width, region_tiles, z_levels, noise_inline, face_blocks_per_cu,
                  records_in_update, 0.0)
         check(lib().fhdx_ctx_set_tune(self.h, C.byref(t)), "fhdx_ctx_set_tune")
+
+    def set_chemistry(self, chem: Chemistry):
+        """fhdx_ctx_set_chemistry: the reactions of a context with params.nreaction > 0."""
+        check(lib().fhdx_ctx_set_chemistry(self.h, C.byref(chem)), "fhdx_ctx_set_chemistry")
 
     def get_tune(self) -> Tune:
         t = Tune()

@@ -11,6 +11,7 @@ import ctypes as C
 import math
 
 MAX_SPECIES = 8
+MAX_REACTION = 8
 
 # physical constants, src_common/common_functions.cpp:431,433
 K_B = 1.38064852e-16
@@ -69,6 +70,39 @@ class Params(C.Structure):
         q = Params()
         C.memmove(C.byref(q), C.byref(self), C.sizeof(Params))
         return q
+
+
+class Chemistry(C.Structure):
+    """``fhdx_chemistry`` (include/fhdx.h): reactions of chemistry_namespace.H:1-39."""
+
+    _fields_ = [
+        ("nreaction", C.c_int32),
+        ("reaction_type", C.c_int32),
+        ("stoich_R", (C.c_int32 * MAX_SPECIES) * MAX_REACTION),
+        ("stoich_P", (C.c_int32 * MAX_SPECIES) * MAX_REACTION),
+        ("rate_const", C.c_double * MAX_REACTION),
+        ("alpha_param", C.c_double * MAX_REACTION),
+        ("beta_param", C.c_double * MAX_REACTION),
+        ("T0_chem", C.c_double),
+    ]
+
+
+def make_chemistry(stoich_R, stoich_P, rate_const, T0_chem, reaction_type=1, alpha=None,
+                   beta=None) -> Chemistry:
+    """Reaction data as InitializeChemistryNamespace reads it (chemistry_functions.cpp:42-127):
+    stoich_R[m][n], stoich_P[m][n] per reaction m and species n."""
+    c = Chemistry()
+    c.nreaction = len(rate_const)
+    c.reaction_type = reaction_type
+    for m in range(c.nreaction):
+        for n in range(len(stoich_R[m])):
+            c.stoich_R[m][n] = int(stoich_R[m][n])
+            c.stoich_P[m][n] = int(stoich_P[m][n])
+        c.rate_const[m] = float(rate_const[m])
+        c.alpha_param[m] = float(alpha[m]) if alpha is not None else 0.0
+        c.beta_param[m] = float(beta[m]) if beta is not None else 0.0
+    c.T0_chem = float(T0_chem)
+    return c
 
 
 class Fab(C.Structure):
@@ -130,8 +164,9 @@ def make_params(nspecies: int, n_cells, dx, molmass, diameter, *, ngc: int = 2,
 class Config:
     """A run: parameters plus the uniform initial state and time step."""
 
-    def __init__(self, name, params: Params, rho0, rhobar, T_init, dt, nsteps):
+    def __init__(self, name, params: Params, rho0, rhobar, T_init, dt, nsteps, chem=None):
         self.name = name
+        self.chem = chem
         self.params = params
         self.rho0 = rho0
         self.rhobar = list(rhobar)
@@ -207,3 +242,28 @@ def species_prefix(n_species, edge, nsteps):
     p = make_params(n_species, (edge, edge, edge), 8e-6, M, d)
     rhobar = [1.0 / n_species] * n_species
     return Config(f"species{n_species}_{edge}^3", p, 4.83e-4, rhobar, 300.0, MIXTURE_DT, nsteps)
+
+
+def no2_n2o4(n, nsteps=100, reaction_type=1):
+    """2 NO2 <-> N2O4 with the chemical Langevin source, from the reference's deck
+    exec/compressible_stag/test_NO2_dimerization_eq/inputs_NO2_dimerization_eq (the same
+    compute_compressible_chemistry_source_CLE the collocated scheme calls): M 46.0055 / 92.0110,
+    d 3.765e-8 / 4.621e-8 cm, dof 7.279586 / 18.049072, e0 4.685624e9 / -1.730782e9, Y 0.723969
+    / 0.276031, T 350 K, rho 1.858338e-3, k_B 1.38065e-16, Runiv 8.31446e7, dx 8e-6 cm, dt 1e-12
+    s; rate constants 4.065753e11 / 6.236899e7, alpha -5.337990e10 / 5.37e11, beta 0.644950 /
+    -1.1, T0_chem 350.  (The collocated deck exec/compressible/chemistry_inputs_files/
+    inputs_NO2_N2O4_eq_SF gives rate constants in other units: its source is ~1e24 g/cm^3/s at
+    its own composition.)"""
+    p = make_params(2, (n, n, n), 1.28e-4 / 16, [46.0055, 92.0110], [3.765e-08, 4.621e-08],
+                    dof=[7.279586, 18.049072], e0=[4.685624e+09, -1.730782e+09])
+    p.k_B = 1.38065e-16
+    p.Runiv = 8.31446e7
+    for q in range(2):
+        p.hcv[q] = 0.5 * p.dof[q] * p.Runiv / p.molmass[q]
+        p.hcp[q] = 0.5 * (2.0 + p.dof[q]) * p.Runiv / p.molmass[q]
+    p.nreaction = 2
+    chem = make_chemistry([[2, 0], [0, 1]], [[0, 1], [2, 0]], [4.065753e+11, 6.236899e+07],
+                          350.0, reaction_type=reaction_type, alpha=[-5.337990e+10, 5.370000e+11],
+                          beta=[0.644950, -1.100000])
+    return Config(f"no2_n2o4_{n}^3", p, 1.858338e-03, [0.723969, 0.276031], 350.0, 1e-12,
+                  nsteps, chem=chem)

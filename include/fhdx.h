@@ -26,6 +26,7 @@ extern "C" {
 #endif
 
 #define FHDX_MAX_SPECIES 8
+#define FHDX_MAX_REACTION 8
 #define FHDX_ABI_VERSION 1
 
 enum fhdx_status {
@@ -51,7 +52,7 @@ typedef struct fhdx_params {
     int32_t visc_type;         /* |3| adds bulk viscosity zeta = zeta_ratio*eta              */
     int32_t transport_type;    /* 1: Giovangigli (the only one implemented)                  */
     int32_t stoch_stress_form; /* 1: stochastic fluxes on, 0: deterministic                  */
-    int32_t nreaction;         /* must be 0 (chemistry source is outside this path)          */
+    int32_t nreaction;         /* 0, or the reaction count given to fhdx_ctx_set_chemistry   */
     int32_t bc_vel_lo[3], bc_vel_hi[3];   /* -1 periodic (the only supported value)           */
     int32_t bc_mass_lo[3], bc_mass_hi[3];
     int32_t bc_therm_lo[3], bc_therm_hi[3];
@@ -170,10 +171,46 @@ int fhdx_evaluate_stats(fhdx_ctx* ctx, fhdx_fab cons, fhdx_fab consMean, fhdx_fa
 int fhdx_rk3_step(fhdx_ctx* ctx, fhdx_fab cu, fhdx_fab prim, double dt, uint64_t seed,
                   uint64_t step, void* stream);
 
-/* As fhdx_rk3_step but the A/B noise is read from caller arrays (parity runs). */
+/* As fhdx_rk3_step but the A/B noise is read from caller arrays (parity runs).  Not with
+ * chemistry (its noise has no array here): FHDX_ERR_UNSUPPORTED. */
 int fhdx_rk3_step_injected(fhdx_ctx* ctx, fhdx_fab cu, fhdx_fab prim,
                            const fhdx_fab stochA[3], const fhdx_fab stochB[3],
                            fhdx_fab rancornA, fhdx_fab rancornB, double dt, void* stream);
+
+/* ---- chemistry: the +dt*S source of the RK3 stages ------------------------------------ */
+/* Reaction data of src_chemistry/chemistry_namespace.H:1-39 as compute_compressible_chemistry_
+ * source_CLE (chemistry_functions.cpp:128-193) uses it: law-of-mass-action rates
+ * k_m exp(-alpha_m/Runiv (1/T - 1/T0)) (T/T0)^beta_m prod_n c_n^R(m,n) with molar
+ * concentrations c_n = X_n p/(Runiv T), and for reaction_type 1 the chemical-Langevin noise
+ * sqrt(rate/N_A) W, W = ranchem/sqrt(dt dV), ranchem = A + w_s B per stage like the flux noise. */
+typedef struct fhdx_chemistry {
+    int32_t nreaction;       /* 1..FHDX_MAX_REACTION; must equal fhdx_params.nreaction       */
+    int32_t reaction_type;   /* 0: deterministic rates, 1: chemical Langevin (CLE)           */
+    int32_t stoich_R[FHDX_MAX_REACTION][FHDX_MAX_SPECIES]; /* reactants (stoich_<m>R), >= 0   */
+    int32_t stoich_P[FHDX_MAX_REACTION][FHDX_MAX_SPECIES]; /* products  (stoich_<m>P), >= 0   */
+    double rate_const[FHDX_MAX_REACTION];
+    double alpha_param[FHDX_MAX_REACTION];
+    double beta_param[FHDX_MAX_REACTION];
+    double T0_chem;          /* > 0                                                          */
+} fhdx_chemistry;
+
+/* Sets the reactions of a context whose fhdx_params.nreaction > 0 (required before its first
+ * step).  The steps then add dt * S to the species equations of every stage, S evaluated from
+ * the stage's input state, and draw the reaction noise from the same Philox key as the flux
+ * noise (field 64 + m).  exp and pow are evaluated with IEEE operations only (identical on host
+ * and device); integer powers of concentrations by repeated multiplication. */
+int fhdx_ctx_set_chemistry(fhdx_ctx* ctx, const fhdx_chemistry* chem);
+
+/* compute_compressible_chemistry_source_CLE (chemistry_functions.cpp:128-193): source
+ * components 5..5+NS-1 of the valid cells from prim (nprimvars) and ranchem (nreaction comps,
+ * the already weighted noise); other components are not touched. */
+int fhdx_chemistry_source(fhdx_ctx* ctx, fhdx_fab prim, fhdx_fab source, fhdx_fab ranchem,
+                          double dt, void* stream);
+
+/* The reaction noise sets A and B (nreaction comps, valid cells) of Philox key
+ * (seed, step, instance): the values the steps evaluate in place. */
+int fhdx_fill_chem_noise(fhdx_ctx* ctx, uint64_t seed, uint64_t step, uint32_t instance,
+                         fhdx_fab ranchemA, fhdx_fab ranchemB, void* stream);
 
 /* ---- launch-shape tuning ---------------------------------------------------------------- */
 

@@ -1,6 +1,6 @@
 // fhdx_amrex.H -- header-only AMReX adapter: the reference's RK3step / calculateFlux /
 // conservedToPrimitive / calculateTransportCoeffs / evaluateStats / WritePlotFile /
-// WriteCheckPoint / ReadCheckPoint signatures (src_compressible/compressible_functions.H:23-135)
+// WriteCheckPoint / ReadCheckPoint / compute_compressible_chemistry_source_CLE signatures (src_compressible/compressible_functions.H:23-135)
 // implemented over the C ABI of fhdx.h.
 //
 // Include it from src_compressible instead of linking timeStep.cpp, flux.cpp, transCoeffs.cpp,
@@ -16,6 +16,7 @@
 
 #include <array>
 
+#include "chemistry_namespace.H"
 #include "common_functions.H"
 #include "compressible_namespace.H"
 #include "fhdx.h"
@@ -62,7 +63,7 @@ inline fhdx_params params_from_namespaces(amrex::Geometry const& geom)
     p.visc_type = visc_type;
     p.transport_type = compressible::transport_type;
     p.stoch_stress_form = stoch_stress_form;
-    p.nreaction = 0;
+    p.nreaction = chemistry::nreaction;
     for (int n = 0; n < nspecies; ++n) {
         p.molmass[n] = molmass[n];
         p.diameter[n] = diameter[n];
@@ -87,6 +88,22 @@ inline fhdx_ctx* context(amrex::Geometry const& geom)
     if (!ctx) {
         fhdx_params p = params_from_namespaces(geom);
         if (fhdx_ctx_create(&p, &ctx) != FHDX_OK) amrex::Abort(fhdx_last_error());
+        if (p.nreaction > 0) {  // reactions of chemistry_namespace.H
+            fhdx_chemistry c{};
+            c.nreaction = chemistry::nreaction;
+            c.reaction_type = chemistry::reaction_type;
+            for (int m = 0; m < c.nreaction; ++m) {
+                for (int n = 0; n < p.nspecies; ++n) {
+                    c.stoich_R[m][n] = chemistry::stoich_coeffs_R(m, n);
+                    c.stoich_P[m][n] = chemistry::stoich_coeffs_P(m, n);
+                }
+                c.rate_const[m] = chemistry::rate_const[m];
+                c.alpha_param[m] = chemistry::alpha_param[m];
+                c.beta_param[m] = chemistry::beta_param[m];
+            }
+            c.T0_chem = chemistry::T0_chem;
+            if (fhdx_ctx_set_chemistry(ctx, &c) != FHDX_OK) amrex::Abort(fhdx_last_error());
+        }
     }
     return ctx;
 }
@@ -203,6 +220,16 @@ inline void RK3step(amrex::MultiFab& cu, amrex::MultiFab& cup, amrex::MultiFab& 
                                 nullptr, dt, philox_seed(), philox_step(), stream()));
     }
     ++philox_step();
+}
+
+// compute_compressible_chemistry_source_CLE (chemistry_functions.cpp:128-193).
+inline void compute_compressible_chemistry_source_CLE(amrex::Real dt, amrex::Real /*dV*/, amrex::MultiFab& prim,
+                                                      amrex::MultiFab& source, amrex::MultiFab& ranchem)
+{
+    using namespace fhdx_amrex;
+    for (amrex::MFIter mfi(prim); mfi.isValid(); ++mfi)
+        check(fhdx_chemistry_source(context(amrex::DefaultGeometry()), fab(prim, mfi), fab(source, mfi),
+                                    fab(ranchem, mfi), dt, stream()));
 }
 
 // evaluateStats (stats.cpp:5-363); cross_cell from common_namespace.H:35.

@@ -11,7 +11,7 @@ import subprocess
 
 import numpy as np
 
-from fhdex_amd.params import Fab, Params
+from fhdex_amd.params import Chemistry, Fab, Params
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liborc.so")
@@ -54,6 +54,15 @@ def lib():
         L.orc_set_num_threads.restype = None
         L.orc_evaluate_stats.argtypes = [P, Fab, Fab, Fab, Fab, Fab, Fab, Fab, Fab,
                                          C.POINTER(C.c_double), C.c_int, C.c_int]
+        CH = C.POINTER(Chemistry)
+        L.orc_chem_exp.argtypes = [C.c_double]
+        L.orc_chem_exp.restype = C.c_double
+        L.orc_chem_pow.argtypes = [C.c_double, C.c_double]
+        L.orc_chem_pow.restype = C.c_double
+        L.orc_chem_source.argtypes = [P, CH, Fab, Fab, Fab, C.c_double]
+        L.orc_fill_chem_noise.argtypes = [P, C.c_uint64, C.c_uint64, C.c_uint32, Fab, Fab]
+        L.orc_advance_chem.argtypes = [P, CH, Fab, Fab, C.c_int, C.c_double, C.c_uint64,
+                                       C.c_uint64, C.c_uint32]
         _lib = L
     return _lib
 
@@ -167,6 +176,34 @@ def advance(p, cu, prim, nsteps, dt, noise=None, seed=1, step0=0, instance=0):
     ptr = None if noise is None else noise.ctypes.data
     return lib().orc_advance(C.byref(p), fab_of(cu, (-g,) * 3), fab_of(prim, (-g,) * 3),
                              nsteps, dt, ptr, seed, step0, instance)
+
+
+def chem_exp(x):
+    return lib().orc_chem_exp(float(x))
+
+
+def chem_pow(x, y):
+    return lib().orc_chem_pow(float(x), float(y))
+
+
+def chem_source(p, chem, prim, source, ranchem, dt):
+    """compute_compressible_chemistry_source_CLE on host cell arrays (ngc ghosts)."""
+    g = p.ngc
+    return lib().orc_chem_source(C.byref(p), C.byref(chem), fab_of(prim, (-g,) * 3),
+                                 fab_of(source, (-g,) * 3), fab_of(ranchem, (-g,) * 3), dt)
+
+
+def fill_chem_noise(p, A, B, seed, step, instance=0):
+    g = p.ngc
+    return lib().orc_fill_chem_noise(C.byref(p), seed, step, instance, fab_of(A, (-g,) * 3),
+                                     fab_of(B, (-g,) * 3))
+
+
+def advance_chem(p, chem, cu, prim, nsteps, dt, seed=1, step0=0, instance=0):
+    """orc_advance with the chemistry source and Philox noise (flux and reaction noise)."""
+    g = p.ngc
+    return lib().orc_advance_chem(C.byref(p), C.byref(chem), fab_of(cu, (-g,) * 3),
+                                  fab_of(prim, (-g,) * 3), nsteps, dt, seed, step0, instance)
 
 
 def philox(ctr, key):

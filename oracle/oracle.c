@@ -930,20 +930,22 @@ static void weight_noise(const fhdx_params* P, const fhdx_fab stochFlux[3], fhdx
 /* stage combination: mode 0 -> out = U - dt*div + dt*S;
  * mode 1 -> out = 0.25*(3*U0 + U - dt*div + dt*S); mode 2 -> out = 2/3*(0.5*U0 + U - ...). */
 static void stage_update(const fhdx_params* P, fhdx_fab out_, fhdx_fab u0_, fhdx_fab us_,
-                         const fhdx_fab flux[3], double dt, int mode)
+                         const fhdx_fab flux[3], const fhdx_fab* source, double dt, int mode)
 {
     fab_t out = F(out_), u0 = F(u0_), us = F(us_);
+    fab_t sf = source ? F(*source) : us;
     fab_t fx = F(flux[0]), fy = F(flux[1]), fz = F(flux[2]);
     const int nv = nvars_of(P);
     const int n0 = P->n_cells[0], n1 = P->n_cells[1], n2 = P->n_cells[2];
     const double dx0 = P->dx[0], dx1 = P->dx[1], dx2 = P->dx[2];
-    const double src = 0.0; /* nreaction == 0: source stays zero (main_driver.cpp:241-242) */
 #pragma omp parallel for schedule(static)
     for (int k = 0; k < n2; ++k)
         for (int j = 0; j < n1; ++j)
             for (int i = 0; i < n0; ++i) {
                 double o[MS + 5];
                 for (int n = 0; n < nv; ++n) {
+                    /* the source MultiFab, zero without chemistry (main_driver.cpp:241-242) */
+                    const double src = source ? AT(sf, i, j, k, n) : 0.0;
                     double div = (AT(fx, i + 1, j, k, n) - AT(fx, i, j, k, n)) / dx0 +
                                  (AT(fy, i, j + 1, k, n) - AT(fy, i, j, k, n)) / dx1 +
                                  (AT(fz, i, j, k + 1, n) - AT(fz, i, j, k, n)) / dx2;
@@ -974,12 +976,15 @@ static void stage_update(const fhdx_params* P, fhdx_fab out_, fhdx_fab u0_, fhdx
             }
 }
 
-int orc_rk3_step(const fhdx_params* P, fhdx_fab cu, fhdx_fab cup, fhdx_fab cup2,
-                 fhdx_fab prim, fhdx_fab eta, fhdx_fab zeta, fhdx_fab kappa, fhdx_fab chi,
-                 fhdx_fab D, const fhdx_fab flux[3], const fhdx_fab stochFlux[3],
-                 const fhdx_fab cornx[3], const fhdx_fab corny[3], const fhdx_fab cornz[3],
-                 fhdx_fab visccorn, fhdx_fab rancorn, const fhdx_fab stochA[3],
-                 const fhdx_fab stochB[3], fhdx_fab rancornA, fhdx_fab rancornB, double dt)
+static int rk3_step_impl(const fhdx_params* P, const fhdx_chemistry* chem, fhdx_fab cu,
+                         fhdx_fab cup, fhdx_fab cup2, fhdx_fab prim, fhdx_fab eta, fhdx_fab zeta,
+                         fhdx_fab kappa, fhdx_fab chi, fhdx_fab D, const fhdx_fab flux[3],
+                         const fhdx_fab stochFlux[3], const fhdx_fab cornx[3],
+                         const fhdx_fab corny[3], const fhdx_fab cornz[3], fhdx_fab visccorn,
+                         fhdx_fab rancorn, const fhdx_fab stochA[3], const fhdx_fab stochB[3],
+                         fhdx_fab rancornA, fhdx_fab rancornB, const fhdx_fab* ranchemA,
+                         const fhdx_fab* ranchemB, const fhdx_fab* ranchem, const fhdx_fab* source,
+                         double dt)
 {
     const double sq2 = sqrt(2.0), sq3 = sqrt(3.0);
     const double w1 = (2.0 * sq2 + 1.0 * sq3) / 5.0;
@@ -995,12 +1000,46 @@ int orc_rk3_step(const fhdx_params* P, fhdx_fab cu, fhdx_fab cup, fhdx_fab cup2,
         weight_noise(P, stochFlux, rancorn, stochA, stochB, rancornA, rancornB, ws[s]);
         orc_calculate_flux(P, in[s], prim, eta, zeta, kappa, chi, D, flux, stochFlux, cornx,
                            corny, cornz, visccorn, rancorn, dt);
-        stage_update(P, outs[s], cu, in[s], flux, dt, s);
+        if (chem) { /* timeStep.cpp:135-143: ranchem = 1*A + w*B, source from this stage's prim */
+            fab_t r = F(*ranchem), a = F(*ranchemA), b = F(*ranchemB);
+            for (int m = 0; m < chem->nreaction; ++m)
+                for (long q = 0; q < r.sn; ++q)
+                    r.p[q + m * r.sn] = 1.0 * a.p[q + m * a.sn] + ws[s] * b.p[q + m * b.sn];
+            orc_chem_source(P, chem, prim, *source, *ranchem, dt);
+        }
+        stage_update(P, outs[s], cu, in[s], flux, chem ? source : NULL, dt, s);
         orc_cons_to_prim(P, prim, outs[s]);
         orc_fill_boundary(P, outs[s]);
         orc_fill_boundary(P, prim);
     }
     return err;
+}
+
+int orc_rk3_step(const fhdx_params* P, fhdx_fab cu, fhdx_fab cup, fhdx_fab cup2,
+                 fhdx_fab prim, fhdx_fab eta, fhdx_fab zeta, fhdx_fab kappa, fhdx_fab chi,
+                 fhdx_fab D, const fhdx_fab flux[3], const fhdx_fab stochFlux[3],
+                 const fhdx_fab cornx[3], const fhdx_fab corny[3], const fhdx_fab cornz[3],
+                 fhdx_fab visccorn, fhdx_fab rancorn, const fhdx_fab stochA[3],
+                 const fhdx_fab stochB[3], fhdx_fab rancornA, fhdx_fab rancornB, double dt)
+{
+    return rk3_step_impl(P, NULL, cu, cup, cup2, prim, eta, zeta, kappa, chi, D, flux, stochFlux,
+                         cornx, corny, cornz, visccorn, rancorn, stochA, stochB, rancornA,
+                         rancornB, NULL, NULL, NULL, NULL, dt);
+}
+
+int orc_rk3_step_chem(const fhdx_params* P, const fhdx_chemistry* chem, fhdx_fab cu,
+                      fhdx_fab cup, fhdx_fab cup2, fhdx_fab prim, fhdx_fab eta, fhdx_fab zeta,
+                      fhdx_fab kappa, fhdx_fab chi, fhdx_fab D, const fhdx_fab flux[3],
+                      const fhdx_fab stochFlux[3], const fhdx_fab cornx[3],
+                      const fhdx_fab corny[3], const fhdx_fab cornz[3], fhdx_fab visccorn,
+                      fhdx_fab rancorn, const fhdx_fab stochA[3], const fhdx_fab stochB[3],
+                      fhdx_fab rancornA, fhdx_fab rancornB, fhdx_fab ranchemA, fhdx_fab ranchemB,
+                      fhdx_fab ranchem, fhdx_fab source, double dt)
+{
+    if (!chem || chem->nreaction != P->nreaction || chem->nreaction < 1) return FHDX_ERR_ARG;
+    return rk3_step_impl(P, chem, cu, cup, cup2, prim, eta, zeta, kappa, chi, D, flux, stochFlux,
+                         cornx, corny, cornz, visccorn, rancorn, stochA, stochB, rancornA,
+                         rancornB, &ranchemA, &ranchemB, &ranchem, &source, dt);
 }
 
 /* ------------------------------------------------------------------------------------- */
@@ -1204,6 +1243,129 @@ int orc_init_cons(const fhdx_params* P, fhdx_fab cu_, int prob_type, double rho0
     return FHDX_OK;
 }
 
+/* ------------------------------------------------------------------------------------- */
+/* Chemistry source (src_chemistry/chemistry_functions.cpp:128-193)                          */
+/* ------------------------------------------------------------------------------------- */
+
+/* e^x and x^y from IEEE operations only, the same sequence as the device (physics.hpp:
+ * chem_exp, chem_pow): x = k ln2 + r, |r| <= ln2/2, Taylor series of e^r to r^14, times 2^k;
+ * x^y = e^(y ln x) with the Box-Muller logarithm; y = 0 gives 1, y = 1 gives x. */
+double orc_chem_exp(double x)
+{
+    if (x == 0.0) return 1.0;
+    if (x > 709.0) return INFINITY;
+    if (x < -745.0) return 0.0;
+    const double k = rint(x * 1.4426950408889634);
+    const double r = fma(-k, 4.7493250390316726e-07, fma(-k, 0.6931467056274414, x));
+    double p = 1.1470745597729725e-11;
+    p = fma(p, r, 1.6059043836821613e-10);
+    p = fma(p, r, 2.08767569878681e-09);
+    p = fma(p, r, 2.505210838544172e-08);
+    p = fma(p, r, 2.755731922398589e-07);
+    p = fma(p, r, 2.7557319223985893e-06);
+    p = fma(p, r, 2.48015873015873e-05);
+    p = fma(p, r, 0.0001984126984126984);
+    p = fma(p, r, 0.001388888888888889);
+    p = fma(p, r, 0.008333333333333333);
+    p = fma(p, r, 0.041666666666666664);
+    p = fma(p, r, 0.16666666666666666);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    const int ki = (int)k, k1 = ki / 2, k2 = ki - k1;
+    const uint64_t b1 = (uint64_t)(k1 + 1023) << 52, b2 = (uint64_t)(k2 + 1023) << 52;
+    double s1, s2;
+    memcpy(&s1, &b1, sizeof s1);
+    memcpy(&s2, &b2, sizeof s2);
+    return p * s1 * s2;
+}
+
+double orc_chem_pow(double x, double y)
+{
+    if (y == 0.0) return 1.0;
+    if (y == 1.0) return x;
+    if (x == 0.0) return y > 0.0 ? 0.0 : INFINITY;
+    if (x < 0.0) return NAN;
+    return orc_chem_exp(y * bm_log(x));
+}
+
+/* pow(c, s) for an integer stoichiometric coefficient s >= 0, by repeated multiplication */
+static double chem_ipow(double c, int s)
+{
+    if (s == 0) return 1.0;
+    double r = c;
+    for (int q = 1; q < s; ++q) r = r * c;
+    return r;
+}
+
+int orc_chem_source(const fhdx_params* P, const fhdx_chemistry* C, fhdx_fab prim_,
+                    fhdx_fab source_, fhdx_fab ranchem_, double dt)
+{
+    if (C->reaction_type != 0 && C->reaction_type != 1) return FHDX_ERR_UNSUPPORTED;
+    if (!(C->T0_chem > 0.)) return FHDX_ERR_ARG;
+    fab_t prim = F(prim_), source = F(source_), ranchem = F(ranchem_);
+    const int NS = P->nspecies, nr = C->nreaction;
+    const double dV = P->dx[0] * P->dx[1] * P->dx[2];
+    const double Runiv = P->Runiv;
+    const double Navo = P->Runiv / P->k_B; /* avogadro (common_functions.cpp:775) */
+    const int n0 = P->n_cells[0], n1 = P->n_cells[1], n2 = P->n_cells[2];
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < n2; ++k)
+        for (int j = 0; j < n1; ++j)
+            for (int i = 0; i < n0; ++i) {
+                const double pres = AT(prim, i, j, k, 5);
+                const double T = AT(prim, i, j, k, 4);
+                const double T0 = C->T0_chem;
+                const double ctot = pres / Runiv / T;
+                double ck[MS], rate[FHDX_MAX_REACTION], src[MS];
+                for (int n = 0; n < NS; ++n) ck[n] = AT(prim, i, j, k, 6 + NS + n) * ctot;
+                for (int m = 0; m < nr; ++m) {
+                    rate[m] = C->rate_const[m];
+                    rate[m] *= orc_chem_exp(-C->alpha_param[m] / Runiv * (1 / T - 1 / T0));
+                    rate[m] *= orc_chem_pow(T / T0, C->beta_param[m]);
+                    for (int n = 0; n < NS; ++n) rate[m] *= chem_ipow(ck[n], C->stoich_R[m][n]);
+                }
+                for (int n = 0; n < NS; ++n) src[n] = 0.;
+                for (int m = 0; m < nr; ++m) {
+                    rate[m] = (0. < rate[m]) ? rate[m] : 0.; /* std::max(0., rate) */
+                    const double W = AT(ranchem, i, j, k, m) / sqrt(dt * dV);
+                    for (int n = 0; n < NS; ++n) {
+                        const int pr = C->stoich_P[m][n] - C->stoich_R[m][n];
+                        src[n] += P->molmass[n] * pr * rate[m];
+                        if (C->reaction_type == 1)
+                            src[n] += P->molmass[n] * pr * sqrt(rate[m] / Navo) * W;
+                    }
+                }
+                for (int n = 0; n < NS; ++n) AT(source, i, j, k, 5 + n) = src[n];
+            }
+    return FHDX_OK;
+}
+
+/* Reaction noise sets A and B (nreaction comps, valid cells) of key (seed, step, instance):
+ * field 64 + m, the linear cell index as the counter, variance 1 (timeStep.cpp:93-97). */
+int orc_fill_chem_noise(const fhdx_params* P, uint64_t seed, uint64_t step, uint32_t instance,
+                        fhdx_fab A_, fhdx_fab B_)
+{
+    fab_t A = F(A_), B = F(B_);
+    const int n0 = P->n_cells[0], n1 = P->n_cells[1], n2 = P->n_cells[2];
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    const uint32_t fhi = ((uint32_t)(step >> 32)) << 8;
+    for (int m = 0; m < P->nreaction; ++m)
+        for (int k = 0; k < n2; ++k)
+            for (int j = 0; j < n1; ++j)
+                for (int i = 0; i < n0; ++i) {
+                    const uint32_t lin = (uint32_t)(((long)k * n1 + j) * n0 + i);
+                    const uint32_t ctr[4] = {lin, (uint32_t)step, (64u + (uint32_t)m) | fhi, instance};
+                    uint32_t r[4];
+                    double z0, z1;
+                    orc_philox4x32_10(ctr, key, r);
+                    orc_normal_pair(r, &z0, &z1);
+                    AT(A, i, j, k, m) = z0;
+                    AT(B, i, j, k, m) = z1;
+                }
+    return FHDX_OK;
+}
+
 /* evaluateStats (src_compressible/stats.cpp:5-363): running means and variances of cons and
  * prim, the yz-slice averages at every x, the values of the cross_cell slice (miscVals) and the
  * spatial cross-correlations against that slice.  Serial loops in the reference's k, j, i order,
@@ -1387,8 +1549,9 @@ static fhdx_fab mkfab(int l0, int l1, int l2, int h0, int h1, int h2, int nc)
     return f;
 }
 
-int orc_advance(const fhdx_params* P, fhdx_fab cu, fhdx_fab prim, int nsteps, double dt,
-                const double* noise, uint64_t seed, uint64_t step0, uint32_t instance)
+static int advance_impl(const fhdx_params* P, const fhdx_chemistry* chem, fhdx_fab cu,
+                        fhdx_fab prim, int nsteps, double dt, const double* noise, uint64_t seed,
+                        uint64_t step0, uint32_t instance)
 {
     const int NS = P->nspecies, nv = nvars_of(P), g = P->ngc;
     const int n0 = P->n_cells[0], n1 = P->n_cells[1], n2 = P->n_cells[2];
@@ -1414,6 +1577,11 @@ int orc_advance(const fhdx_params* P, fhdx_fab cu, fhdx_fab prim, int nsteps, do
     fhdx_fab rc = mkfab(0, 0, 0, n0, n1, n2, 1);
     fhdx_fab rA = mkfab(0, 0, 0, n0, n1, n2, 1);
     fhdx_fab rB = mkfab(0, 0, 0, n0, n1, n2, 1);
+    const int nr = chem ? chem->nreaction : 1;
+    fhdx_fab cA = mkfab(-g, -g, -g, n0 + g - 1, n1 + g - 1, n2 + g - 1, nr);
+    fhdx_fab cB = mkfab(-g, -g, -g, n0 + g - 1, n1 + g - 1, n2 + g - 1, nr);
+    fhdx_fab cR = mkfab(-g, -g, -g, n0 + g - 1, n1 + g - 1, n2 + g - 1, nr);
+    fhdx_fab src = mkfab(-g, -g, -g, n0 + g - 1, n1 + g - 1, n2 + g - 1, nv);
 
     orc_fill_boundary(P, cu);
     orc_cons_to_prim(P, prim, cu);
@@ -1438,10 +1606,18 @@ int orc_advance(const fhdx_params* P, fhdx_fab cu, fhdx_fab prim, int nsteps, do
         } else {
             orc_fill_noise(P, seed, step0 + (uint64_t)s, instance, sA, sB, rA, rB);
         }
-        int e = orc_rk3_step(P, cu, cup, cup2, prim, eta, zeta, kappa, chi, D, flux, sf, cx,
-                             cy, cz, vcorn, rc, sA, sB, rA, rB, dt);
+        int e;
+        if (chem) {
+            orc_fill_chem_noise(P, seed, step0 + (uint64_t)s, instance, cA, cB);
+            e = orc_rk3_step_chem(P, chem, cu, cup, cup2, prim, eta, zeta, kappa, chi, D, flux, sf,
+                                  cx, cy, cz, vcorn, rc, sA, sB, rA, rB, cA, cB, cR, src, dt);
+        } else {
+            e = orc_rk3_step(P, cu, cup, cup2, prim, eta, zeta, kappa, chi, D, flux, sf, cx, cy,
+                             cz, vcorn, rc, sA, sB, rA, rB, dt);
+        }
         if (e) err = e;
     }
+    free(cA.p); free(cB.p); free(cR.p); free(src.p);
     free(cup.p); free(cup2.p); free(eta.p); free(zeta.p); free(kappa.p); free(chi.p); free(D.p);
     for (int d = 0; d < 3; ++d) {
         free(flux[d].p); free(sf[d].p); free(sA[d].p); free(sB[d].p);
@@ -1449,6 +1625,19 @@ int orc_advance(const fhdx_params* P, fhdx_fab cu, fhdx_fab prim, int nsteps, do
     }
     free(vcorn.p); free(rc.p); free(rA.p); free(rB.p);
     return err;
+}
+
+int orc_advance(const fhdx_params* P, fhdx_fab cu, fhdx_fab prim, int nsteps, double dt,
+                const double* noise, uint64_t seed, uint64_t step0, uint32_t instance)
+{
+    return advance_impl(P, NULL, cu, prim, nsteps, dt, noise, seed, step0, instance);
+}
+
+int orc_advance_chem(const fhdx_params* P, const fhdx_chemistry* chem, fhdx_fab cu, fhdx_fab prim,
+                     int nsteps, double dt, uint64_t seed, uint64_t step0, uint32_t instance)
+{
+    if (!chem || chem->nreaction != P->nreaction || chem->nreaction < 1) return FHDX_ERR_ARG;
+    return advance_impl(P, chem, cu, prim, nsteps, dt, NULL, seed, step0, instance);
 }
 
 void orc_set_num_threads(int n)
