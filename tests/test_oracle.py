@@ -336,14 +336,20 @@ def test_shear_wave_decays_at_kinematic_viscosity(orc):
     assert rate == pytest.approx(eta / cfg.rho0 * keff2, rel=5e-3)
 
 
-def test_sound_wave_speed_and_damping(orc):
+@pytest.mark.parametrize("adv,visc,zr", [(2, 2, -1.0), (1, 2, -1.0), (2, 3, 1.5), (2, 1, -1.0)])
+def test_sound_wave_speed_and_damping(orc, adv, visc, zr):
     """A standing adiabatic sound wave oscillates at c_s k and decays at
-    Gamma = k^2/(2 rho) (4/3 eta + kappa (1/c_v - 1/c_p)): pins the hyperbolic flux, the
-    corner (bulk) viscous terms and heat conduction together."""
+    Gamma = k^2/(2 rho) (4/3 eta + zeta + kappa (1/c_v - 1/c_p)): pins the hyperbolic flux, the
+    corner viscous terms and heat conduction together, for conserved (advection_type 2) and
+    primitive (1) interpolation, which agree at linear order, with bulk viscosity
+    zeta = zeta_ratio * eta (visc_type 3, transCoeffs.cpp:168-173).  visc_type 1 damps like 2:
+    the collocated flux only tests |visc_type| == 3 (flux.cpp:182, 546, 912, 1635), so the
+    inputs files' "not-symmetric" option is the symmetric stress in this scheme."""
     from scipy.optimize import curve_fit
     cfg = P.argon_gas(4)
     n = 32
-    p = P.make_params(2, (4, n, 4), 8e-6, [39.95, 39.95], [3.66e-8, 3.66e-8], stoch_stress_form=0)
+    p = P.make_params(2, (4, n, 4), 8e-6, [39.95, 39.95], [3.66e-8, 3.66e-8], stoch_stress_form=0,
+                      advection_type=adv, visc_type=visc, zeta_ratio=zr)
     g = p.ngc
     dy = p.dx[1]
     k = 2 * math.pi / (n * dy)
@@ -374,11 +380,12 @@ def test_sound_wave_speed_and_damping(orc):
     k1 = (8 * math.sin(k * h) - math.sin(2 * k * h)) / (6 * h)
     k2sq = (2 - 2 * math.cos(k * h)) / h ** 2
     eta, kappa = 2.0812e-4, 1624.3
+    zeta = zr * eta if visc == 3 else 0.0
     rho0, T0 = cfg.rho0, cfg.T_init
     Rm = P.R_UNIV / 39.95
     p0 = rho0 * Rm * T0
     A = np.array([[0, -1j * k1 * rho0, 0],
-                  [-1j * k1 * Rm * T0 / rho0, -4 / 3 * eta / rho0 * k2sq, -1j * k1 * Rm],
+                  [-1j * k1 * Rm * T0 / rho0, -(4 / 3 * eta + zeta) / rho0 * k2sq, -1j * k1 * Rm],
                   [0, -1j * k1 * p0 / (rho0 * cv), -kappa * k2sq / (rho0 * cv)]])
     ev = np.linalg.eigvals(A)
     sound = ev[np.argmax(np.abs(ev.imag))]
@@ -391,3 +398,69 @@ def test_sound_wave_speed_and_damping(orc):
     popt, _ = curve_fit(model, t, a, p0=[eps, G0, w0, 0.0, 0.0])
     assert popt[2] == pytest.approx(w0, rel=1e-3)
     assert popt[1] == pytest.approx(G0, rel=1e-2)
+
+
+def test_gravity_accelerates_a_uniform_gas(orc):
+    """Uniform gas, deterministic fluxes, gravity g (timeStep.cpp:190-201 and its stage-2/3
+    forms): the fluxes vanish, so j = rho g t and rho E = rho E0 + rho |g|^2 t^2 / 2 after every
+    step -- RK3 integrates the constant force and the linear-in-time work exactly."""
+    cfg = P.argon_gas(4)
+    grav = (3.0e11, -1.0e12, 0.5e12)
+    p = P.make_params(2, (4, 4, 4), 8e-6, [39.95, 39.95], [3.66e-8, 3.66e-8], stoch_stress_form=0,
+                      grav=grav)
+    st = orc.HostState(p)
+    orc.init_cons(p, st.cu, cfg.rho0, cfg.rhobar, cfg.T_init)
+    orc.fill_boundary(p, st.cu)
+    orc.cons_to_prim(p, st.prim, st.cu)
+    orc.fill_boundary(p, st.prim)
+    g = p.ngc
+    E0 = st.cu[4, g, g, g]
+    rho = cfg.rho0
+    gg = sum(x * x for x in grav)
+    for n in range(1, 11):
+        assert orc.rk3_step(p, st, cfg.dt) == 0
+        t = n * cfg.dt
+        core = st.cu[:, g:-g, g:-g, g:-g]
+        assert np.ptp(core, axis=(1, 2, 3)).max() <= 1e-14 * np.abs(core).max()
+        assert core[0, 0, 0, 0] == pytest.approx(rho, rel=1e-15)   # the stage sums round
+        for d in range(3):
+            assert core[1 + d, 0, 0, 0] == pytest.approx(rho * grav[d] * t, rel=1e-13, abs=1e-30)
+        dE = core[4, 0, 0, 0] - E0
+        assert dE == pytest.approx(0.5 * rho * gg * t * t, rel=1e-8)
+
+
+@pytest.mark.parametrize("alg", [2, 1])
+def test_label_diffusion_of_identical_species(orc, alg):
+    """Two identical species at rest with a sinusoidal mass fraction: the mixture is uniform, so
+    only the labels move.  With species fluxes (algorithm_type 2) the label profile decays at
+    D_12 k_eff^2 (D_12 the hard-sphere binary diffusivity, k_eff^2 = (2 - 2 cos kh)/h^2); with
+    algorithm_type 1 (no species fluxes) it stays put."""
+    cfg = P.argon_gas(4)
+    n = 32
+    p = P.make_params(2, (4, n, 4), 8e-6, [39.95, 39.95], [3.66e-8, 3.66e-8], stoch_stress_form=0,
+                      algorithm_type=alg)
+    g = p.ngc
+    h = p.dx[1]
+    k = 2 * math.pi / (n * h)
+    s = np.sin(k * (np.arange(n) + 0.5) * h)[None, :, None]
+    eps = 0.1
+
+    def init(st):
+        st.cu[5, g:-g, g:-g, g:-g] = cfg.rho0 * (0.5 + eps * s)
+        st.cu[6, g:-g, g:-g, g:-g] = cfg.rho0 * (0.5 - eps * s)
+
+    def probe(st):
+        y = st.cu[5, g, g:-g, g] / cfg.rho0 - 0.5
+        return 2 * (y * s[0, :, 0]).mean()
+
+    amp = _wave_run(orc, cfg, p, init, 40, probe)
+    if alg == 1:
+        assert np.abs(amp - amp[0]).max() <= 1e-14
+        return
+    t = cfg.dt * np.arange(len(amp))
+    rate = -np.polyfit(t, np.log(amp), 1)[0]
+    kB, T, d = P.K_B, cfg.T_init, 3.66e-8
+    m = 39.95 * P.K_B / P.R_UNIV
+    pres = cfg.rho0 / m * kB * T
+    D12 = 3 / 16 * math.sqrt(2 * math.pi * (kB * T) ** 3 / (m / 2)) / (pres * math.pi * d * d)
+    assert rate == pytest.approx(D12 * (2 - 2 * math.cos(k * h)) / h ** 2, rel=5e-3)
