@@ -187,19 +187,21 @@ def ensemble_bench(count: int, steps: int, loop_steps: int):
 
 
 def serve_bench(rate: float, count: int, max_instances: int = 256, budget_gb: float = 160.0,
-                chunk: int = 8, seed: int = 7):
+                chunk: int = 8, seed: int = 7, cap_cells: int = 2 << 20):
     """The served stream: `count` requests with Poisson arrivals at `rate`/s (edge log-uniform
-    in [16,128], 1-4 species, nsteps in [200,2000]), served by the continuous-batching server and,
-    for comparison, by the same server with one slot (requests strictly one after another).
-    Latency is arrival to result in host memory."""
+    in [16,128], 1-4 species, nsteps in [200,2000]), served by the continuous-batching server
+    (admission in arrival order, and smallest work first with at most `cap_cells` cells in
+    flight) and, for comparison, by the same server with one slot (requests strictly one after
+    another).  Latency is arrival to result in host memory."""
     import numpy as np
     from fhdex_amd import serving as S
 
     reqs = S.request_stream(count, seed)
     work = sum(c.cells * n for c, _, n, _ in reqs)
 
-    def one(slots):
+    def one(slots, policy=0, cap=0):
         srv = S.Server(slots, int(budget_gb * 2 ** 30), chunk_steps=chunk, graph=True)
+        srv.set_policy(policy, cap)
         done, wall, open_n, _ = S.run_stream(srv, reqs, rate, seed=seed)
         srv.close()
         ok = [c for c in done if c["status"] == 0]
@@ -215,12 +217,14 @@ def serve_bench(rate: float, count: int, max_instances: int = 256, budget_gb: fl
                 "value": work / span, "unit": "cell-updates/s", "span_s": span}
 
     batched = one(max_instances)
+    smallest = one(max_instances, S.Server.SMALLEST_FIRST, cap_cells)
     plain = one(1)
     return {"workload": f"{count} requests, Poisson {rate}/s, edge log-uniform [16,128], "
                         f"species 1-4, nsteps 200-2000 (stream seed {seed})",
             "rate_per_s": rate, "requests": count, "cell_updates": work,
             "offered_load": work * rate / count,
-            "batched": batched, "one_at_a_time": plain,
+            "batched": batched, "batched_smallest_first": dict(smallest, max_active_cells=cap_cells),
+            "one_at_a_time": plain,
             "speedup_vs_one_at_a_time": batched["value"] / plain["value"],
             "slots": max_instances, "budget_GiB": budget_gb, "chunk_steps": chunk}
 
@@ -241,6 +245,8 @@ def main():
     ap.add_argument("--serve-rates", type=str, default="50",
                     help="comma-separated arrival rates of the served stream ('' to skip)")
     ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--serve-cap", type=int, default=2 << 20,
+                    help="cells in flight of the smallest-first served run")
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--no-configs", action="store_true", help="skip the 32^3 argon and 128^3 four-species rates")
@@ -279,7 +285,7 @@ def main():
 
     if a.serve_only:
         for r in [float(v) for v in a.serve_rates.split(",") if v]:
-            print(json.dumps({"serving": serve_bench(r, a.serve_requests)}), flush=True)
+            print(json.dumps({"serving": serve_bench(r, a.serve_requests, cap_cells=a.serve_cap)}), flush=True)
         return
 
     if a.ensemble_only:
@@ -377,7 +383,7 @@ def main():
         line["ensemble"] = ensemble_bench(a.ensemble, a.ensemble_steps, a.loop_steps)
     rates = [float(v) for v in a.serve_rates.split(",") if v]
     if rates:
-        line["serving"] = [serve_bench(r, a.serve_requests) for r in rates]
+        line["serving"] = [serve_bench(r, a.serve_requests, cap_cells=a.serve_cap) for r in rates]
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(a.cpu_n, a.cpu_steps)
     print(json.dumps(line), flush=True)

@@ -1,8 +1,9 @@
 // server.hip -- continuous-batching request loop over the ensemble engine.
 //
 // Callers submit ensemble runs (parameters, dt, seed, nsteps, start state) from any host
-// thread.  One worker thread owns the engine: it admits queued requests in arrival order while
-// the memory budget and a slot allow, advances every admitted instance together in chunks of
+// thread.  One worker thread owns the engine: it admits queued requests -- in arrival order, or
+// smallest work (cells x steps) first -- while the memory budget, a slot and an optional cap on
+// the cells in flight allow, advances every admitted instance together in chunks of
 // steps (new arrivals join at the next chunk boundary), copies each finished instance's result
 // to host memory, frees its slot and records a completion with arrival, admission and
 // completion times.  A request whose instance raises an error flag is evicted alone; the others
@@ -38,6 +39,11 @@ struct Pending {
     fhdx_request r;
     double uniform[FHDX_MAX_SPECIES + 5];
     int64_t t_arrival;
+    uint64_t cells() const
+    {
+        return (uint64_t)r.params.n_cells[0] * r.params.n_cells[1] * r.params.n_cells[2];
+    }
+    double work() const { return (double)cells() * (double)r.nsteps; }
 };
 
 struct Running {
@@ -61,6 +67,8 @@ struct fhdx_server {
     int64_t next_id = 0;                   // guarded by mu
     int64_t open = 0;                      // submitted, not yet completed; guarded by mu
     bool stop = false;                     // guarded by mu
+    int policy = 0;                        // FHDX_ADMIT_*; guarded by mu
+    uint64_t max_active_cells = 0;         // 0: no cap; guarded by mu
 
     // worker thread only
     std::thread worker;
@@ -87,8 +95,13 @@ struct fhdx_server {
         cv_done.notify_all();
     }
 
-    void admit()
+    void admit(int pol, uint64_t cap)
     {
+        if (pol == FHDX_ADMIT_SMALLEST_FIRST)
+            std::stable_sort(pending.begin(), pending.end(),
+                             [](const Pending& a, const Pending& b) { return a.work() < b.work(); });
+        uint64_t active = 0;
+        for (const Running& r : running) active += r.q.cells();
         while (!pending.empty()) {
             Pending& q = pending.front();
             const fhdx_params& p = q.r.params;
@@ -107,6 +120,7 @@ struct fhdx_server {
             bool free_slot = false;
             for (const EngineSlot& s : e->slots) free_slot = free_slot || !s.used;
             if (e->used + need > e->budget || !free_slot) break;  // wait for memory / a slot
+            if (cap && active > 0 && active + q.cells() > cap) break;  // the GPU is full enough
             int slot = -1;
             rc = engine_add(e, p, q.r.dt, q.r.seed, 0, q.r.nsteps, q.r.cu_init, q.r.cu_init ? nullptr : q.uniform,
                             &slot);
@@ -116,6 +130,7 @@ struct fhdx_server {
                 continue;
             }
             running.push_back({q, slot, now_ns()});
+            active += q.cells();
             pending.pop_front();
             bytes_used = e->used;
         }
@@ -165,6 +180,8 @@ struct fhdx_server {
         (void)hipSetDevice(device);
         flags.assign(e->max_inst, 0);
         for (;;) {
+            int pol;
+            uint64_t cap;
             {
                 std::unique_lock<std::mutex> lk(mu);
                 if (pending.empty() && running.empty())
@@ -174,8 +191,10 @@ struct fhdx_server {
                     pending.push_back(incoming.front());
                     incoming.pop_front();
                 }
+                pol = policy;
+                cap = max_active_cells;
             }
-            admit();
+            admit(pol, cap);
             if (running.empty()) continue;
             uint64_t chunk_steps = (uint64_t)chunk;
             for (const Running& r : running) {
@@ -235,6 +254,18 @@ int fhdx_server_create(int max_instances, uint64_t mem_budget_bytes, int chunk_s
     s->use_graph = use_graph != 0;
     s->worker = std::thread([s] { s->run(); });
     *out = s;
+    return FHDX_OK;
+}
+
+int fhdx_server_set_policy(fhdx_server* s, int policy, uint64_t max_active_cells)
+{
+    if (!s || (policy != FHDX_ADMIT_ARRIVAL && policy != FHDX_ADMIT_SMALLEST_FIRST)) return FHDX_ERR_ARG;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->policy = policy;
+        s->max_active_cells = max_active_cells;
+    }
+    s->cv.notify_one();
     return FHDX_OK;
 }
 

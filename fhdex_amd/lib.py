@@ -25,6 +25,7 @@ EXPORTS = [
     "fhdx_engine_sync", "fhdx_engine_state", "fhdx_engine_read", "fhdx_engine_errors",
     "fhdx_engine_bytes_used", "fhdx_engine_add_uniform", "fhdx_server_create",
     "fhdx_server_submit", "fhdx_server_poll", "fhdx_server_wait", "fhdx_server_stats",
+    "fhdx_server_set_policy",
     "fhdx_server_destroy", "fhdx_ctx_autotune", "fhdx_ctx_set_tune", "fhdx_ctx_get_tune",
     "fhdx_evaluate_stats", "fhdx_write_plotfile", "fhdx_write_checkpoint", "fhdx_read_checkpoint",
     "fhdx_ctx_set_chemistry", "fhdx_chemistry_source", "fhdx_fill_chem_noise",

@@ -60,8 +60,9 @@ def _bind():
     L.fhdx_server_stats.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                     C.POINTER(C.c_uint64)]
     L.fhdx_server_destroy.argtypes = [vp]
+    L.fhdx_server_set_policy.argtypes = [vp, C.c_int, C.c_uint64]
     for n in ("fhdx_server_create", "fhdx_server_submit", "fhdx_server_poll", "fhdx_server_wait",
-              "fhdx_server_stats", "fhdx_server_destroy"):
+              "fhdx_server_stats", "fhdx_server_destroy", "fhdx_server_set_policy"):
         getattr(L, n).restype = C.c_int
     L._serving_bound = True
     return L
@@ -78,6 +79,14 @@ class Server:
                                    int(bool(graph)), C.byref(h)), "fhdx_server_create")
         self.h = h
         self._keep = {}   # request id -> buffers that must outlive the request
+
+    ARRIVAL, SMALLEST_FIRST = 0, 1
+
+    def set_policy(self, policy: int, max_active_cells: int = 0):
+        """fhdx_server_set_policy: admission in arrival order (0) or smallest work first (1),
+        with an optional cap on the cells in flight."""
+        check(_bind().fhdx_server_set_policy(self.h, int(policy), int(max_active_cells)),
+              "fhdx_server_set_policy")
 
     def close(self):
         if self.h:

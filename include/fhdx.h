@@ -342,6 +342,13 @@ int fhdx_server_create(int max_instances, uint64_t mem_budget_bytes, int chunk_s
                        int use_graph, fhdx_server** out);
 /* Thread-safe.  Records the arrival time and queues the request. */
 int fhdx_server_submit(fhdx_server* s, const fhdx_request* r, int64_t* id);
+/* Admission order of waiting requests: arrival (FIFO, the default) or smallest work
+ * (cells x nsteps) first; max_active_cells > 0 also keeps new admissions waiting while the
+ * admitted instances already hold that many cells (one is always admitted onto an idle GPU).
+ * Smallest-first can delay a large request for as long as smaller ones keep arriving. */
+#define FHDX_ADMIT_ARRIVAL 0
+#define FHDX_ADMIT_SMALLEST_FIRST 1
+int fhdx_server_set_policy(fhdx_server* s, int policy, uint64_t max_active_cells);
 /* Moves up to max completions out (n = how many). */
 int fhdx_server_poll(fhdx_server* s, fhdx_completion* out, int max, int* n);
 /* Waits until every submitted request has completed or timeout_s passes (then FHDX_ERR_BUSY);

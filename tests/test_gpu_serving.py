@@ -55,14 +55,18 @@ def mix(orc):
             for q, (cfg, n) in enumerate(cfgs)]
 
 
-def test_served_results_equal_solo_runs(dev, orc):
+@pytest.mark.parametrize("policy,cap", [(0, 0), (1, 3000)])
+def test_served_results_equal_solo_runs(dev, orc, policy, cap):
     """Six requests of uneven size, 2-4 species and 3-9 steps, arriving in two waves into a
-    server with room for about three at a time and chunks of 2 steps."""
+    server with room for about three at a time and chunks of 2 steps; admission in arrival
+    order, or smallest work first with a cap on the cells in flight below one request's size
+    (one request is still admitted onto an idle GPU)."""
     torch, fc, L, E, S = dev
-    say("case serving: 6 requests vs solo runs")
+    say(f"case serving: 6 requests vs solo runs, policy {policy}, cap {cap}")
     reqs = mix(orc)
     need = sorted(E.bytes_needed(c.params) for c, _, _ in reqs)
     srv = S.Server(4, sum(need[-3:]), chunk_steps=2)
+    srv.set_policy(policy, cap)
     outs = {}
     for q, (cfg, n, cu0) in enumerate(reqs):
         cu_out = np.zeros_like(cu0)
