@@ -26,6 +26,12 @@ struct Fld {
     {
         return *(double*)((char*)(p + (long)c * sn + di + (long)dj * sy + (long)dk * sz) + cb);
     }
+    // Streaming (nontemporal) store, for arrays the next kernel reads back from HBM anyway: the
+    // store does not displace the lines the writing kernel re-reads from L2.
+    __device__ inline void put_nt(unsigned cb, int c, double v, int di = 0, int dj = 0, int dk = 0) const
+    {
+        __builtin_nontemporal_store(v, &(*this)(cb, c, di, dj, dk));
+    }
 };
 
 struct Grid {
@@ -65,11 +71,15 @@ inline Grid grid_of(const DevParams& P)
 // Stores value(c), c < NC, for valid cell (i,j,k) at byte offset cb and at each periodic image of
 // the cell that lies in the ngc ghost layers (up to 7), so no separate ghost-fill pass is needed.
 // Requires n >= 2 ngc on every axis (a cell then has at most one image per axis).
-template <int NC, typename V>
+// NT: streaming stores for the cell itself (the images are few and stay ordinary stores).
+template <int NC, bool NT = false, typename V>
 __device__ inline void store_cell(const DevParams& P, const Fld& F, V value, int i, int j, int k, unsigned cb)
 {
 #pragma unroll
-    for (int c = 0; c < NC; ++c) F(cb, c) = value(c);
+    for (int c = 0; c < NC; ++c) {
+        if (NT) F.put_nt(cb, c, value(c));
+        else F(cb, c) = value(c);
+    }
     const int g = P.ngc;
     const int ax = i < g ? P.n[0] : (i >= P.n[0] - g ? -P.n[0] : 0);
     const int ay = j < g ? P.n[1] : (j >= P.n[1] - g ? -P.n[1] : 0);
@@ -653,10 +663,10 @@ __device__ inline void store_face(const DevParams& P, const Fld& Fo, const doubl
     constexpr int ex = (D == 0), ey = (D == 1), ez = (D == 2);
     const int n = P.n[D];
 #pragma unroll
-    for (int c = 0; c < NS + 5; ++c) Fo(cb, c) = fo[c];
+    for (int c = 0; c < NS + 5; ++c) Fo.put_nt(cb, c, fo[c]);
     if (idx == 0) {
 #pragma unroll
-        for (int c = 0; c < NS + 5; ++c) Fo(cb, c, ex * n, ey * n, ez * n) = fo[c];
+        for (int c = 0; c < NS + 5; ++c) Fo.put_nt(cb, c, fo[c], ex * n, ey * n, ez * n);
     }
 }
 
@@ -722,13 +732,13 @@ __device__ inline void update_from_div(const DevParams& P, const Fld& out, const
         for (int n = 0; n < 3; ++n) o[n + 1] += 2. / 3. * dt * r0 * g[n];
         o[4] += 2. / 3. * dt * (g[0] * r1 + g[1] * r2 + g[2] * r3);
     }
-    store_cell<NS + 5>(P, out, [&](int n) { return o[n]; }, i, j, k, cb);
+    store_cell<NS + 5, true>(P, out, [&](int n) { return o[n]; }, i, j, k, cb);
     // the next stage's record depends on this cell's new state only: computed here from registers
     if (rec_next) cell_record<NS>(P, o, *rec_next, err, i, j, k, cb);
     if (write_prim) {
         double pr[2 * NS + 6];
         cons_to_prim_cell<NS>(P, o, pr);
-        store_cell<2 * NS + 6>(P, prim, [&](int n) { return pr[n]; }, i, j, k, cb);
+        store_cell<2 * NS + 6, true>(P, prim, [&](int n) { return pr[n]; }, i, j, k, cb);
     }
 }
 
