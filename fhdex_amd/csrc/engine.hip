@@ -90,9 +90,43 @@ __global__ void __launch_bounds__(EB) kb_update(const Inst* __restrict__ I, cons
 {
     EB_CELL_PROLOGUE
     const Fld rn = G.fld(in.R);
+    // prim is not written here: engine_read / fhdx_engine_state compute it from U on request
     update_body<NS>(in.P, G.fld(stage_out(in, st)), G.fld(in.U), G.fld(stage_in(in, st)), G.fld(in.F[0]),
-                    G.fld(in.F[1]), G.fld(in.F[2]), in.dt, st, G.fld(in.prim), st == 2, i, j, k, G.off(i, j, k),
+                    G.fld(in.F[1]), G.fld(in.F[2]), in.dt, st, G.fld(in.prim), 0, i, j, k, G.off(i, j, k),
                     REC ? &rn : nullptr, in.err);
+}
+
+// prim (valid cells and ghost images) of one instance from its state U: conservedToPrimitive
+// as the single-box step applies it after its last stage, evaluated only when prim is read.
+template <int NS>
+__global__ void __launch_bounds__(EB) kb_prim(DevParams P, Grid G, const double* U, double* prim, long ncell)
+{
+    const long t = (long)blockIdx.x * EB + threadIdx.x;
+    if (t >= ncell) return;
+    int i, j, k;
+    dec3(t, P.n[0], P.n[1], i, j, k);
+    const unsigned cb = G.off(i, j, k);
+    const Fld u = G.fld(const_cast<double*>(U));
+    double c[NS + 5], pr[2 * NS + 6];
+#pragma unroll
+    for (int n = 0; n < NS + 5; ++n) c[n] = u(cb, n);
+    cons_to_prim_cell<NS>(P, c, pr);
+    store_cell<2 * NS + 6, true>(P, G.fld(prim), [&](int n) { return pr[n]; }, i, j, k, cb);
+}
+
+static int compute_prim(Engine* e, const EngineSlot& s)
+{
+    const Inst& d = s.desc;
+    const unsigned nb = (unsigned)((d.ncell + EB - 1) / EB);
+    switch (s.p.nspecies) {
+        case 2: kb_prim<2><<<nb, EB, 0, e->stream>>>(d.P, d.G, d.U, d.prim, d.ncell); break;
+        case 3: kb_prim<3><<<nb, EB, 0, e->stream>>>(d.P, d.G, d.U, d.prim, d.ncell); break;
+        case 4: kb_prim<4><<<nb, EB, 0, e->stream>>>(d.P, d.G, d.U, d.prim, d.ncell); break;
+        default: return FHDX_ERR_UNSUPPORTED;
+    }
+    FHDX_HIP_TRY(hipGetLastError());
+    FHDX_HIP_TRY(hipStreamSynchronize(e->stream));
+    return FHDX_OK;
 }
 
 // Periodic ghost fill of one instance's state at admission; from then on every kernel that
@@ -405,6 +439,10 @@ int engine_read(Engine* e, int slot, double* cu_out, double* prim_out)
     const EngineSlot& s = e->slots[slot];
     const size_t pts = (size_t)grown_of(s.p);
     FHDX_HIP_TRY(hipStreamSynchronize(e->stream));
+    if (prim_out) {
+        int r = compute_prim(e, s);
+        if (r) return r;
+    }
     if (cu_out)
         FHDX_HIP_TRY(hipMemcpy(cu_out, s.desc.U, sizeof(double) * pts * (5 + s.p.nspecies), hipMemcpyDefault));
     if (prim_out)
@@ -486,6 +524,11 @@ int fhdx_engine_state(fhdx_engine* h, int slot, fhdx_fab* cu, fhdx_fab* prim, ui
     };
     fill(cu, s.desc.U, 5 + p.nspecies);
     fill(prim, s.desc.prim, 6 + 2 * p.nspecies);
+    if (prim) {
+        FHDX_HIP_TRY(hipStreamSynchronize(h->e->stream));
+        int r = compute_prim(h->e, s);
+        if (r) return r;
+    }
     if (step) *step = s.desc.step;
     if (done) *done = s.desc.step >= s.desc.step_end;
     return FHDX_OK;

@@ -289,7 +289,9 @@ int fhdx_engine_step(fhdx_engine* e, int nsteps, int use_graph);
 /* Wait for the queued steps. */
 int fhdx_engine_sync(fhdx_engine* e);
 /* Device FABs of a slot's conserved and primitive state (valid after fhdx_engine_sync), its
- * next step index and whether it has taken all its steps.  Any output may be NULL. */
+ * next step index and whether it has taken all its steps.  Any output may be NULL.  The engine
+ * keeps only the conserved state between steps; asking for prim computes it from that state
+ * (conservedToPrimitive, ghost cells included) before returning. */
 int fhdx_engine_state(fhdx_engine* e, int slot, fhdx_fab* cu, fhdx_fab* prim, uint64_t* step,
                       int* done);
 /* Wait for the queued steps, then copy a slot's grown conserved / primitive state to cu_out /
