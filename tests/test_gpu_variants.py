@@ -85,3 +85,51 @@ def test_variant_one_step(orc, name, kw):
     check("reference-shaped cu", valid(p, ds.cu.t.cpu().numpy()), valid(p, st.cu))
     for d in range(3):
         check(f"reference-shaped flux[{d}]", ds.flux[d].t.cpu().numpy(), st.flux[d])
+
+
+def test_trace_species_branch(orc):
+    """A third species at mass fraction ~2e-14 takes the zeroed-row branch of the species noise
+    (flux.cpp:395-401) on every face: calculateFlux and one fused step against the oracle."""
+    import torch
+    from fhdex_amd import compressible as fc
+    from fhdex_amd import lib as L
+    cfg = variant({})
+    cfg.rhobar = [0.6, 0.4 - 2e-14, 2e-14]
+    p = cfg.params
+    g = p.ngc
+    say(f"case trace species {tuple(p.n_cells)}")
+    st = perturbed_state(p, cfg, orc, seed=17, amp=0.02, vel=400.0)
+    assert st.prim[6 + 2, g:-g, g:-g, g:-g].max() < 1e-13
+    assert orc.transport(p, st) == 0
+    orc.fill_noise(p, st, 5, 9)
+    w = 0.37
+    for d in range(3):
+        st.stoch[d][:] = st.sA[d] + w * st.sB[d]
+        st.stoch[d][0] = 0
+    st.rancorn[:] = st.rA + w * st.rB
+    ctx = L.Context(p)
+    ds = fc.DeviceState(p)
+    for name in ("cu", "prim", "eta", "zeta", "kappa", "chi", "D"):
+        getattr(ds, name).t.copy_(torch.from_numpy(getattr(st, name)))
+    for d in range(3):
+        ds.stochFlux[d].t.copy_(torch.from_numpy(st.stoch[d]))
+    ds.rancorn.t.copy_(torch.from_numpy(st.rancorn))
+    fc.calculateFlux(ctx, ds.cu, ds.prim, ds.eta, ds.zeta, ds.kappa, ds.chi, ds.D, ds.flux,
+                     ds.stochFlux, ds.cornx, ds.corny, ds.cornz, ds.visccorn, ds.rancorn, cfg.dt)
+    torch.cuda.synchronize()
+    assert orc.calculate_flux(p, st, st.cu, cfg.dt) == 0
+    for d in range(3):
+        check(f"flux[{d}]", ds.flux[d].t.cpu().numpy(), st.flux[d])
+    # one fused step with the same (injected) noise
+    cu = fc.MultiFab(torch.from_numpy(st.cu.copy()).cuda(), (-g,) * 3)
+    prim = fc.MultiFab.cell(p, p.nprimvars)
+    ns = fc.NoiseSets(p)
+    for d in range(3):
+        ns.A[d].t.copy_(torch.from_numpy(st.sA[d]))
+        ns.B[d].t.copy_(torch.from_numpy(st.sB[d]))
+    ns.rA.t.copy_(torch.from_numpy(st.rA))
+    ns.rB.t.copy_(torch.from_numpy(st.rB))
+    fc.rk3_step_injected(ctx, cu, prim, ns, cfg.dt)
+    ctx.check()
+    assert orc.rk3_step(p, st, cfg.dt) == 0
+    check("fused cu", valid(p, cu.t.cpu().numpy()), valid(p, st.cu))

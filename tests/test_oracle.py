@@ -464,3 +464,34 @@ def test_label_diffusion_of_identical_species(orc, alg):
     pres = cfg.rho0 / m * kB * T
     D12 = 3 / 16 * math.sqrt(2 * math.pi * (kB * T) ** 3 / (m / 2)) / (pres * math.pi * d * d)
     assert rate == pytest.approx(D12 * (2 - 2 * math.cos(k * h)) / h ** 2, rel=5e-3)
+
+
+def test_trace_species_gets_no_species_noise(orc):
+    """A species whose clamped mass fraction is at most 1e-12 on both sides of a face has its row
+    and column of the face's species-noise matrix zeroed (flux.cpp:395-401), and the Cholesky
+    factor then skips it: its stochastic flux is exactly zero, the other species' is not."""
+    base = P.species_prefix(3, 6, 1)
+    Y = [0.6, 0.4 - 2e-14, 2e-14]
+
+    def fluxes(stoch):
+        p = P.make_params(3, (6, 6, 6), 8e-6, [4.0026, 20.1797, 39.9480], [2.18e-8, 2.58e-8, 3.63e-8],
+                          stoch_stress_form=stoch)
+        st = orc.HostState(p)
+        orc.init_cons(p, st.cu, base.rho0, Y, base.T_init)
+        orc.fill_boundary(p, st.cu)
+        orc.cons_to_prim(p, st.prim, st.cu)
+        orc.fill_boundary(p, st.prim)
+        assert orc.transport(p, st) == 0
+        orc.fill_noise(p, st, 5, 9)
+        for d in range(3):
+            st.stoch[d][:] = st.sA[d] + 0.37 * st.sB[d]
+            st.stoch[d][0] = 0
+        st.rancorn[:] = st.rA + 0.37 * st.rB
+        assert orc.calculate_flux(p, st, st.cu, base.dt) == 0
+        return [f.copy() for f in st.flux]
+
+    noisy, quiet = fluxes(1), fluxes(0)
+    for d in range(3):
+        assert np.array_equal(noisy[d][7], quiet[d][7]), d          # trace species: no noise
+        for s in (5, 6):
+            assert np.abs(noisy[d][s] - quiet[d][s]).max() > 0, (d, s)
