@@ -92,6 +92,11 @@ def FillBoundary(ctx: Context, mf: MultiFab, stream=None):
     check(lib().fhdx_fill_boundary(ctx.h, mf.fab, stream), "FillBoundary")
 
 
+def setBC(ctx: Context, prim: MultiFab, cons: MultiFab, stream=None):
+    """setBC (boundary.cpp:172-1209): the ghost cells outside walls, after FillBoundary."""
+    check(lib().fhdx_set_bc(ctx.h, prim.fab, cons.fab, stream), "setBC")
+
+
 def calculateTransportCoeffs(ctx: Context, prim, eta, zeta, kappa, chi, D, stream=None):
     check(lib().fhdx_transport(ctx.h, prim.fab, eta.fab, zeta.fab, kappa.fab, chi.fab, D.fab, stream),
           "calculateTransportCoeffs")

@@ -107,6 +107,34 @@ void fhdx_params_finalize(fhdx_params* p)
         if (p->hcv[i] < 0.) p->hcv[i] = 0.5 * p->dof[i] * p->Runiv / p->molmass[i];
         if (p->hcp[i] < 0.) p->hcp[i] = 0.5 * (2. + p->dof[i]) * p->Runiv / p->molmass[i];
     }
+    // SetupBC (boundary.cpp:3-12): a periodic direction has periodic mass and thermal codes
+    for (int d = 0; d < 3; ++d)
+        if (p->bc_vel_lo[d] == -1) {
+            p->bc_mass_lo[d] = p->bc_mass_hi[d] = -1;
+            p->bc_therm_lo[d] = p->bc_therm_hi[d] = -1;
+        }
+    // SetupCWall (boundary.cpp:14-170): the wall composition of bc_mass 2 faces from whichever
+    // of X or Y sums to 1 (X first); GetMassfrac / GetMolfrac, compressible_functions.H:139-167
+    const int NS = p->nspecies < FHDX_MAX_SPECIES ? p->nspecies : FHDX_MAX_SPECIES;
+    for (int f = 0; f < 6; ++f) {
+        const int code = (f & 1) ? p->bc_mass_hi[f >> 1] : p->bc_mass_lo[f >> 1];
+        if (code != 2 && code != 3) continue;
+        double sumx = 0., sumy = 0.;
+        for (int n = 0; n < NS; ++n) {
+            sumx = sumx + p->bc_Xk[f][n];
+            sumy = sumy + p->bc_Yk[f][n];
+        }
+        if (std::fabs(sumx - 1) < 1.e-10) {
+            double molmix = 0.;
+            for (int n = 0; n < NS; ++n) molmix += p->bc_Xk[f][n] * p->molmass[n];
+            for (int n = 0; n < NS; ++n) p->bc_Yk[f][n] = p->bc_Xk[f][n] * p->molmass[n] / molmix;
+        } else if (std::fabs(sumy - 1) < 1.e-10) {
+            double molmix = 0.;
+            for (int n = 0; n < NS; ++n) molmix += p->bc_Yk[f][n] / p->molmass[n];
+            molmix = 1. / molmix;
+            for (int n = 0; n < NS; ++n) p->bc_Xk[f][n] = p->bc_Yk[f][n] * molmix / p->molmass[n];
+        }
+    }
 }
 
 int fhdx_params_check(const fhdx_params* p)
@@ -125,8 +153,60 @@ int fhdx_params_check(const fhdx_params* p)
             set_error("dx must be positive");
             return FHDX_ERR_ARG;
         }
-        if (p->bc_vel_lo[d] != -1 || p->bc_vel_hi[d] != -1) {
-            set_error("only fully periodic domains are implemented (bc_vel = -1)");
+        const bool per_lo = p->bc_vel_lo[d] == -1, per_hi = p->bc_vel_hi[d] == -1;
+        if (per_lo != per_hi) {
+            set_error("a direction is periodic on both sides or on neither (bc_vel_lo/hi = -1 together)");
+            return FHDX_ERR_ARG;
+        }
+        if (per_lo) {
+            if (p->bc_mass_lo[d] != -1 || p->bc_mass_hi[d] != -1 || p->bc_therm_lo[d] != -1 ||
+                p->bc_therm_hi[d] != -1) {
+                set_error("a periodic direction needs bc_mass and bc_therm -1 (fhdx_params_finalize sets them)");
+                return FHDX_ERR_ARG;
+            }
+            continue;
+        }
+        for (int side = 0; side < 2; ++side) {
+            const int v = side ? p->bc_vel_hi[d] : p->bc_vel_lo[d];
+            const int m = side ? p->bc_mass_hi[d] : p->bc_mass_lo[d];
+            const int t = side ? p->bc_therm_hi[d] : p->bc_therm_lo[d];
+            if (v != 1 && v != 2) {
+                set_error("bc_vel of a wall must be 1 (slip) or 2 (no slip)");
+                return FHDX_ERR_UNSUPPORTED;
+            }
+            if (m == 3) {
+                set_error("bc_mass 3 (reservoir) is not implemented; walls take bc_mass 1 or 2");
+                return FHDX_ERR_UNSUPPORTED;
+            }
+            if (m != 1 && m != 2) {
+                set_error("bc_mass of a wall must be 1 (wall) or 2 (fixed composition)");
+                return FHDX_ERR_UNSUPPORTED;
+            }
+            if (t != 1 && t != 2) {
+                set_error("bc_therm of a wall must be 1 (adiabatic) or 2 (isothermal)");
+                return FHDX_ERR_UNSUPPORTED;
+            }
+            const double tw = side ? p->t_hi[d] : p->t_lo[d];
+            if (t == 2 && !(tw > 0.)) {
+                set_error("an isothermal wall needs t_lo/t_hi > 0");
+                return FHDX_ERR_ARG;
+            }
+            if (m == 2) {
+                const int f = 2 * d + side;
+                double sy = 0.;
+                bool pos = true;
+                for (int n = 0; n < p->nspecies && n < FHDX_MAX_SPECIES; ++n) {
+                    sy += p->bc_Yk[f][n];
+                    pos = pos && p->bc_Yk[f][n] > 0. && p->bc_Xk[f][n] > 0.;
+                }
+                if (!pos || std::fabs(sy - 1.) > 1e-10) {
+                    set_error("a bc_mass 2 face needs positive bc_Yk/bc_Xk summing to 1 (fhdx_params_finalize)");
+                    return FHDX_ERR_ARG;
+                }
+            }
+        }
+        if (p->algorithm_type != 2) {
+            set_error("walls need algorithm_type 2 (setBC writes the species ghosts only then)");
             return FHDX_ERR_UNSUPPORTED;
         }
     }
@@ -330,8 +410,17 @@ int rk3_ref(fhdx_ctx* c, fhdx_fab cu, fhdx_fab cup, fhdx_fab cup2, fhdx_fab prim
         if ((r = ref_c2p(P, a.prim, make_arr(out[st]), s))) return r;
         if ((r = ref_fill_boundary(P, make_arr(out[st]), nv, s))) return r;
         if ((r = ref_fill_boundary(P, a.prim, P.nprim, s))) return r;
+        if ((r = ref_set_bc(P, a.prim, make_arr(out[st]), s))) return r;  // timeStep.cpp:209, 314, 425
     }
     return FHDX_OK;
+}
+
+// The fused step, the ensemble engine and the server keep the periodic fast path.
+int reject_walls(const fhdx_ctx* c, const char* who)
+{
+    if (!c->dp.any_wall) return FHDX_OK;
+    set_error(std::string(who) + ": walls are served by the reference-shaped step (fhdx_rk3_step_ref)");
+    return FHDX_ERR_UNSUPPORTED;
 }
 
 }  // namespace
@@ -416,6 +505,14 @@ int fhdx_fill_boundary(fhdx_ctx* c, fhdx_fab mf, void* stream)
     if (!c) return FHDX_ERR_ARG;
     if (!check_cell(c, mf, 1, "mf")) return FHDX_ERR_ARG;
     return ref_fill_boundary(c->dp, make_arr(mf), mf.ncomp, S(stream));
+}
+
+int fhdx_set_bc(fhdx_ctx* c, fhdx_fab prim, fhdx_fab cons, void* stream)
+{
+    if (!c) return FHDX_ERR_ARG;
+    if (!check_cell(c, prim, c->dp.nprim, "prim") || !check_cell(c, cons, c->dp.nvars, "cons"))
+        return FHDX_ERR_ARG;
+    return ref_set_bc(c->dp, make_arr(prim), make_arr(cons), S(stream));
 }
 
 int fhdx_calculate_flux(fhdx_ctx* c, fhdx_fab cons, fhdx_fab prim, fhdx_fab eta, fhdx_fab zeta,
@@ -534,6 +631,7 @@ int fhdx_rk3_step(fhdx_ctx* c, fhdx_fab cu, fhdx_fab prim, double dt, uint64_t s
         set_error("nreaction > 0: call fhdx_ctx_set_chemistry before stepping");
         return FHDX_ERR_ARG;
     }
+    if (int r = reject_walls(c, "fhdx_rk3_step")) return r;
     return fused_rk3_step(c->p, c->dp, c->fused, cu, prim, dt, make_noise_key(seed, step, 0), nullptr, nullptr,
                           nullptr, nullptr, c->d_err, S(stream));
 }
@@ -546,6 +644,7 @@ int fhdx_rk3_step_injected(fhdx_ctx* c, fhdx_fab cu, fhdx_fab prim, const fhdx_f
         set_error("caller noise with chemistry: the reaction noise has no array argument");
         return FHDX_ERR_UNSUPPORTED;
     }
+    if (int r = reject_walls(c, "fhdx_rk3_step_injected")) return r;
     const int nv = c->dp.nvars;
     bool ok = check_cell(c, cu, nv, "cu") && check_cell(c, prim, c->dp.nprim, "prim") &&
               check_node(c, rancornA, "rancornA") && check_node(c, rancornB, "rancornB") && dt > 0.;
@@ -736,6 +835,7 @@ int fhdx_ctx_autotune(fhdx_ctx* c, fhdx_fab cu, double dt, int trial_steps, fhdx
     if (!c || !best || trial_steps < 1 || !(dt > 0.)) return FHDX_ERR_ARG;
     if (n_trials) *n_trials = 0;
     if (!check_cell(c, cu, c->dp.nvars, "cu")) return FHDX_ERR_ARG;
+    if (int r = reject_walls(c, "fhdx_ctx_autotune")) return r;
     const TuneKey key{c->p.nspecies, c->p.n_cells[0], c->p.n_cells[1], c->p.n_cells[2]};
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);

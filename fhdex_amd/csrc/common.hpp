@@ -65,6 +65,23 @@ inline DevParams make_dev_params(const fhdx_params& p)
     d.var_mom = std::sqrt(p.variance_coef_mom * p.variance_coef_mom);
     d.var_ener = std::sqrt(p.variance_coef_ener * p.variance_coef_ener);
     d.var_mass = std::sqrt(p.variance_coef_mass * p.variance_coef_mass);
+    d.any_wall = 0;
+    for (int i = 0; i < 3; ++i) {
+        d.periodic[i] = (p.bc_vel_lo[i] == -1);
+        if (!d.periodic[i]) d.any_wall = 1;
+        for (int side = 0; side < 2; ++side) {
+            const int f = 2 * i + side;
+            d.bcv[f] = side ? p.bc_vel_hi[i] : p.bc_vel_lo[i];
+            d.bct[f] = side ? p.bc_therm_hi[i] : p.bc_therm_lo[i];
+            d.bcm[f] = side ? p.bc_mass_hi[i] : p.bc_mass_lo[i];
+            d.wall[f] = (d.bcm[f] != 3 && d.bcm[f] != -1);
+            d.tw[f] = side ? p.t_hi[i] : p.t_lo[i];
+            for (int n = 0; n < MS; ++n) {
+                d.Yw[f][n] = p.bc_Yk[f][n];
+                d.Xw[f][n] = p.bc_Xk[f][n];
+            }
+        }
+    }
     return d;
 }
 

@@ -38,6 +38,13 @@ struct DevParams {
     double grav[3];
     double var_mom, var_ener, var_mass;  // sqrt(coef*coef), the MultiFabFillRandom scale
     DevChem chem;
+    // Walls (face f = 2*direction + side).  periodic[d]: bc_vel -1; wall[f]: bc_mass 1 or 2
+    // (is_*_dirichlet_mass, flux.cpp:29-34); bcv/bct/bcm: bc_vel/therm/mass codes.
+    int any_wall;
+    int periodic[3];
+    int wall[6], bcv[6], bct[6], bcm[6];
+    double tw[6];
+    double Yw[6][MS], Xw[6][MS];
 };
 
 // Strided view of one FAB (AMReX Array4 layout: x fastest, then y, z, component).

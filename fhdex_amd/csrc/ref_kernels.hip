@@ -33,6 +33,39 @@ __constant__ signed char c_rc_node[3][4][3] = {{{0, 1, 1}, {0, 0, 1}, {0, 1, 0},
                                                {{1, 0, 1}, {0, 0, 1}, {1, 0, 0}, {0, 0, 0}},
                                                {{1, 1, 0}, {0, 1, 0}, {1, 0, 0}, {0, 0, 0}}};
 
+// Wall faces: the four edge-viscosity cells in the ghost layer (normal offset -1 at a lo face,
+// 0 at a hi face), transverse offsets per face direction and edge in the reference's order
+// (flux.cpp:137-156 x, 501-520 y, 867-886 z).
+__constant__ signed char c_mu_wall[3][4][4][3] = {
+    {{{0, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 1, 1}}, {{0, -1, 0}, {0, 0, 0}, {0, -1, 1}, {0, 0, 1}},
+     {{0, 0, -1}, {0, 1, -1}, {0, 0, 0}, {0, 1, 0}}, {{0, -1, -1}, {0, 0, -1}, {0, -1, 0}, {0, 0, 0}}},
+    {{{1, 0, 0}, {0, 0, 0}, {1, 0, 1}, {0, 0, 1}}, {{-1, 0, 0}, {0, 0, 0}, {-1, 0, 1}, {0, 0, 1}},
+     {{1, 0, -1}, {0, 0, -1}, {1, 0, 0}, {0, 0, 0}}, {{-1, 0, -1}, {0, 0, -1}, {-1, 0, 0}, {0, 0, 0}}},
+    {{{1, 0, 0}, {0, 0, 0}, {1, 1, 0}, {0, 1, 0}}, {{-1, 1, 0}, {0, 1, 0}, {-1, 0, 0}, {0, 0, 0}},
+     {{1, 0, 0}, {0, 0, 0}, {1, -1, 0}, {0, -1, 0}}, {{-1, -1, 0}, {0, -1, 0}, {-1, 0, 0}, {0, 0, 0}}}};
+
+// Where a face of direction D stands relative to the walls: lo/hi are set on a bc_mass 1/2
+// face at index 0 / n_D (the is_*_dirichlet_mass branches of flux.cpp); (wi, wj, wk) is the
+// wall-side cell (the ghost at a lo face, the boundary cell at a hi face).
+struct WallFace {
+    bool lo, hi;
+    int wi, wj, wk;
+    __device__ bool any() const { return lo || hi; }
+};
+
+template <int D>
+__device__ inline WallFace wall_face(const DevParams& P, int i, int j, int k)
+{
+    WallFace w;
+    const int c = (D == 0) ? i : (D == 1) ? j : k;
+    w.lo = P.any_wall && c == 0 && P.wall[2 * D];
+    w.hi = P.any_wall && c == P.n[D] && P.wall[2 * D + 1];
+    w.wi = i - (w.lo && D == 0);
+    w.wj = j - (w.lo && D == 1);
+    w.wk = k - (w.lo && D == 2);
+    return w;
+}
+
 __device__ inline void face_index(long t, int fx, int fy, int& i, int& j, int& k)
 {
     i = (int)(t % fx);
@@ -87,9 +120,11 @@ __global__ void __launch_bounds__(BLK) k_transport(DevParams P, Arr prim, Arr et
     for (int n = 0; n < NS * NS; ++n) Dm(i, j, k, n) = Dv[n];
 }
 
-// Periodic ghost fill of a cell FAB: ghost (i,j,k) <- valid (i mod n, ...).
-__global__ void __launch_bounds__(BLK) k_fill_boundary(Arr a, int nc, int g, int n0, int n1, int n2)
+// Periodic ghost fill of a cell FAB: ghost (i,j,k) <- valid (i mod n, ...) along periodic
+// directions; a ghost cell outside a wall has no periodic image and is left to setBC.
+__global__ void __launch_bounds__(BLK) k_fill_boundary(DevParams P, Arr a, int nc)
 {
+    const int g = P.ngc, n0 = P.n[0], n1 = P.n[1], n2 = P.n[2];
     const int gx = n0 + 2 * g, gy = n1 + 2 * g, gz = n2 + 2 * g;
     long t = (long)blockIdx.x * BLK + threadIdx.x;
     if (t >= (long)gx * gy * gz) return;
@@ -98,9 +133,92 @@ __global__ void __launch_bounds__(BLK) k_fill_boundary(Arr a, int nc, int g, int
     i -= g;
     j -= g;
     k -= g;
+    if ((!P.periodic[0] && (i < 0 || i >= n0)) || (!P.periodic[1] && (j < 0 || j >= n1)) ||
+        (!P.periodic[2] && (k < 0 || k >= n2)))
+        return;
     int iv = (i + n0) % n0, jv = (j + n1) % n1, kv = (k + n2) % n2;
     if (iv == i && jv == j && kv == k) return;
     for (int n = 0; n < nc; ++n) a(i, j, k, n) = a(iv, jv, kv, n);
+}
+
+// One face of setBC (boundary.cpp:180-1036): every cell of the grown box outside face
+// (D, side) takes its value from the mirror cell (-c-1 at lo, 2n-c-1 at hi) and the wall data:
+// mass fractions mirrored (bc_mass 1) or fixed (2); T and p mirrored (adiabatic) or T = t_wall
+// with p mirrored (isothermal); velocity: slip (normal 0, tangential mirrored) or no slip (0);
+// then rho from GetDensity and the conserved state from it.
+template <int NS>
+__global__ void __launch_bounds__(BLK) k_set_bc(DevParams P, Arr prim, Arr cons, int D, int side)
+{
+    const int g = P.ngc, n = P.n[D];
+    const int gx = P.n[0] + 2 * g, gy = P.n[1] + 2 * g, gz = P.n[2] + 2 * g;
+    long t = (long)blockIdx.x * BLK + threadIdx.x;
+    if (t >= (long)gx * gy * gz) return;
+    int c[3];
+    face_index(t, gx, gy, c[0], c[1], c[2]);
+    c[0] -= g;
+    c[1] -= g;
+    c[2] -= g;
+    if (side == 0 ? c[D] >= 0 : c[D] < n) return;
+    int m[3] = {c[0], c[1], c[2]};
+    m[D] = side ? 2 * (n - 1) - c[D] + 1 : -c[D] - 1;
+    const int f = 2 * D + side;
+    const int bm = P.bcm[f], bt = P.bct[f], bv = P.bcv[f];
+#define PR(a, q) prim(a[0], a[1], a[2], q)
+#define CO(a, q) cons(a[0], a[1], a[2], q)
+    if (bm == 1 && P.algorithm_type == 2)
+#pragma unroll
+        for (int q = 6; q < 6 + 2 * NS; ++q) PR(c, q) = PR(m, q);
+    if (bm == 2 && P.algorithm_type == 2)
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            PR(c, 6 + s) = P.Yw[f][s];
+            PR(c, 6 + NS + s) = P.Xw[f][s];
+        }
+    if (bt == 1) {
+        PR(c, 4) = PR(m, 4);
+        PR(c, 5) = PR(m, 5);
+    } else if (bt == 2) {
+        PR(c, 4) = P.tw[f];
+        PR(c, 5) = PR(m, 5);
+    }
+    if (bv != 1 && bv != 2) return;
+    double v[4];
+    if (bv == 1) {
+#pragma unroll
+        for (int q = 1; q < 4; ++q) {
+            const bool normal = (q == 1 + D);
+            CO(c, q) = normal ? 0.0 : CO(m, q);
+            v[q] = normal ? 0.0 : PR(m, q);
+            PR(c, q) = v[q];
+        }
+    } else {
+#pragma unroll
+        for (int q = 1; q < 4; ++q) {
+            v[q] = 0.0;
+            PR(c, q) = 0.0;
+        }
+    }
+    double Y[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) Y[s] = PR(c, 6 + s);
+    const double T = PR(c, 4), pt = PR(c, 5);
+    double molmix = 0.;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) molmix += Y[s] / P.molmass[s];
+    molmix = 1. / molmix;
+    const double rho = pt / (P.Runiv / molmix) / T;  // GetDensity, compressible_functions.H:1481-1495
+    const double ie = eos_energy<NS>(P, Y, T);
+    PR(c, 0) = rho;
+    CO(c, 0) = rho;
+    if (P.algorithm_type == 2)
+#pragma unroll
+        for (int s = 0; s < NS; ++s) CO(c, 5 + s) = rho * Y[s];
+    if (bv == 2)
+#pragma unroll
+        for (int q = 1; q < 4; ++q) CO(c, q) = rho * v[q];
+    CO(c, 4) = rho * ie + 0.5 * rho * (v[1] * v[1] + v[2] * v[2] + v[3] * v[3]);
+#undef PR
+#undef CO
 }
 
 __global__ void __launch_bounds__(BLK) k_fill(double* p, long n, double v)
@@ -110,6 +228,12 @@ __global__ void __launch_bounds__(BLK) k_fill(double* p, long n, double v)
 }
 
 // ---- stochastic face flux (flux.cpp:71-1145) -----------------------------------------------
+// At a wall face (bc_mass 1/2) the face-local averages read the wall-side cell for both sides:
+// 2*eta*T, 2*kappa*T^2, T, 2*w*u and the species terms of flux.cpp:88-97, 137-202, 318-327,
+// 343-427 are then reproduced bit for bit (every rewrite is an exact doubling or halving).
+// The edge viscosities take the four ghost-layer cells, and edges touching a tangential wall
+// are scaled by sqrt(2) (slip) or 0 (no slip) (:250-280).  StochFlux (boundary.cpp:1392-1970)
+// then scales the face's stochastic fluxes, applied here to the values this thread wrote.
 
 template <int NS, int D>
 __global__ void __launch_bounds__(BLK) k_stoch(DevParams P, Arr prim, Arr eta, Arr zeta, Arr kappa,
@@ -121,12 +245,16 @@ __global__ void __launch_bounds__(BLK) k_stoch(DevParams P, Arr prim, Arr eta, A
     if (t >= (long)fx * fy * fz) return;
     int i, j, k;
     face_index(t, fx, fy, i, j, k);
-    const int il = i - (D == 0), jl = j - (D == 1), kl = k - (D == 2);
+    const WallFace W = wall_face<D>(P, i, j, k);
+    // lower and upper cells of the face averages; both the wall-side cell at a wall
+    const int il = W.any() ? W.wi : i - (D == 0), jl = W.any() ? W.wj : j - (D == 1),
+              kl = W.any() ? W.wk : k - (D == 2);
+    const int iu = W.any() ? W.wi : i, ju = W.any() ? W.wj : j, ku = W.any() ? W.wk : k;
     const int nv = NS + 5;
 
-    double Tu = prim(i, j, k, 4), Tl = prim(il, jl, kl, 4);
-    double mup = eta(i, j, k) * Tu + eta(il, jl, kl) * Tl;
-    double kp = kappa(i, j, k) * Tu * Tu + kappa(il, jl, kl) * Tl * Tl;
+    double Tu = prim(iu, ju, ku, 4), Tl = prim(il, jl, kl, 4);
+    double mup = eta(iu, ju, ku) * Tu + eta(il, jl, kl) * Tl;
+    double kp = kappa(iu, ju, ku) * Tu * Tu + kappa(il, jl, kl) * Tl * Tl;
     double meanT = 0.5 * (Tu + Tl);
     double w[NS + 5];
     double fw1 = sqrt(P.k_B * mup * volinv * dtinv);
@@ -138,45 +266,87 @@ __global__ void __launch_bounds__(BLK) k_stoch(DevParams P, Arr prim, Arr eta, A
     w[4] = fw4 * ran(i, j, k, 4);
     double nweight = sqrt(P.k_B * volinv * dtinv);
 
+    // tangential-wall factors of the four edges (pp, mp, pm, mm) over transverse axes (A, B)
+    constexpr int TA = (D == 0) ? 1 : 0, TB = (D == 2) ? 1 : 2;
+    double fac[4] = {1., 1., 1., 1.};
+    if (P.any_wall) {
+        const int idx[3] = {i, j, k};
+        double flo[2] = {1., 1.}, fhi[2] = {1., 1.};
+        const int ax[2] = {TA, TB};
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int a = ax[q];
+            const int vl = P.bcv[2 * a], vh = P.bcv[2 * a + 1];
+            if ((vl == 1 || vl == 2) && idx[a] == 0) flo[q] = (vl == 1) ? sqrt(2.0) : 0.;
+            if ((vh == 1 || vh == 2) && idx[a] == P.n[a] - 1) fhi[q] = (vh == 1) ? sqrt(2.0) : 0.;
+        }
+        fac[0] = fhi[0] * fhi[1];
+        fac[1] = flo[0] * fhi[1];
+        fac[2] = fhi[0] * flo[1];
+        fac[3] = flo[0] * flo[1];
+    }
+
     double sum = 0.;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        double s = 0.;
+        double mu;
+        if (W.any()) {
+            const int oi = W.wi - i, oj = W.wj - j, ok = W.wk - k;  // ghost-layer shift
+            double s4 = 0., z4 = 0.;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const signed char* o = c_mu_order[D][c][q];
-            double term = eta(i + o[0], j + o[1], k + o[2]) * prim(i + o[3], j + o[4], k + o[5], 4);
-            s = (q == 0) ? term : s + term;
-        }
-        double mu = 0.25 * s / 3.;
-        if (P.visc_type == 3 || P.visc_type == -3) {
-            double z = 0.;
+            for (int q = 0; q < 4; ++q) {
+                const signed char* o = c_mu_wall[D][c][q];
+                const int ci = i + oi + o[0], cj = j + oj + o[1], ck = k + ok + o[2];
+                const double T = prim(ci, cj, ck, 4);
+                const double e = eta(ci, cj, ck) * T;
+                s4 = (q == 0) ? e : s4 + e;
+                if (P.visc_type == 3 || P.visc_type == -3) {
+                    const double zz = zeta(ci, cj, ck) * T;
+                    z4 = (q == 0) ? zz : z4 + zz;
+                }
+            }
+            mu = 0.5 * s4 / 3.;
+            if (P.visc_type == 3 || P.visc_type == -3) mu += (D == 0) ? 0.5 * z4 : 0.5 * z4 / 3.;
+        } else {
+            double s8 = 0.;
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
                 const signed char* o = c_mu_order[D][c][q];
-                double term = zeta(i + o[0], j + o[1], k + o[2]) * prim(i + o[3], j + o[4], k + o[5], 4);
-                z = (q == 0) ? term : z + term;
+                double term = eta(i + o[0], j + o[1], k + o[2]) * prim(i + o[3], j + o[4], k + o[5], 4);
+                s8 = (q == 0) ? term : s8 + term;
             }
-            mu += 0.25 * z;
+            mu = 0.25 * s8 / 3.;
+            if (P.visc_type == 3 || P.visc_type == -3) {
+                double z = 0.;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const signed char* o = c_mu_order[D][c][q];
+                    double term = zeta(i + o[0], j + o[1], k + o[2]) * prim(i + o[3], j + o[4], k + o[5], 4);
+                    z = (q == 0) ? term : z + term;
+                }
+                mu += 0.25 * z;
+            }
         }
         const signed char* nd = c_rc_node[D][c];
-        double term = sqrt(mu) * rc(i + nd[0], j + nd[1], k + nd[2]);
+        double term = fac[c] * sqrt(mu) * rc(i + nd[0], j + nd[1], k + nd[2]);
         sum = (c == 0) ? term : sum + term;
     }
     w[1 + D] = w[1 + D] + 0.25 * nweight * sum;
 
+    double o[NS + 9];  // this face's stochastic flux, nvars + 4 components
 #pragma unroll
-    for (int n = 1; n < 4; ++n) fl(i, j, k, n) = fl(i, j, k, n) + w[n];
-    fl(i, j, k, nv) = fl(i, j, k, nv) + w[4];
+    for (int n = 1; n < 4; ++n) o[n] = fl(i, j, k, n) + w[n];
+    o[nv] = fl(i, j, k, nv) + w[4];
 
     constexpr int t1 = (D == 0) ? 2 : 1, t2 = (D == 2) ? 2 : 3;
-    double phidiag = w[1 + D] * (prim(il, jl, kl, 1 + D) + prim(i, j, k, 1 + D));
-    double phishear = w[t1] * (prim(il, jl, kl, t1) + prim(i, j, k, t1)) +
-                      w[t2] * (prim(il, jl, kl, t2) + prim(i, j, k, t2));
+    double phidiag = w[1 + D] * (prim(il, jl, kl, 1 + D) + prim(iu, ju, ku, 1 + D));
+    double phishear = w[t1] * (prim(il, jl, kl, t1) + prim(iu, ju, ku, t1)) +
+                      w[t2] * (prim(il, jl, kl, t2) + prim(iu, ju, ku, t2));
     phidiag = -0.5 * phidiag;
     phishear = -0.5 * phishear;
-    fl(i, j, k, nv + 1) = fl(i, j, k, nv + 1) - phidiag;
-    fl(i, j, k, nv + 2) = fl(i, j, k, nv + 2) - phishear;
+    o[nv + 1] = fl(i, j, k, nv + 1) - phidiag;
+    o[nv + 2] = fl(i, j, k, nv + 2) - phishear;
+    o[nv + 3] = fl(i, j, k, nv + 3);
 
     if (P.algorithm_type == 2) {
         double yy[NS], yyp[NS], DijY[NS * NS], sqD[NS * NS];
@@ -184,7 +354,7 @@ __global__ void __launch_bounds__(BLK) k_stoch(DevParams P, Arr prim, Arr eta, A
         for (int s = 0; s < NS; ++s) {
             w[5 + s] = 0.;
             yy[s] = clamp01(prim(il, jl, kl, 6 + s));
-            yyp[s] = clamp01(prim(i, j, k, 6 + s));
+            yyp[s] = clamp01(prim(iu, ju, ku, 6 + s));
         }
         double sumy = 0., sumyp = 0.;
 #pragma unroll
@@ -203,8 +373,8 @@ __global__ void __launch_bounds__(BLK) k_stoch(DevParams P, Arr prim, Arr eta, A
             MWmix = MWmix + 0.5 * (yy[s] + yyp[s]) / P.molmass[s];
 #pragma unroll
             for (int l = 0; l < NS; ++l)
-                DijY[s * NS + l] = 0.5 * (Dm(il, jl, kl, l * NS + s) * yy[l] + Dm(i, j, k, l * NS + s) * yyp[l] +
-                                          (Dm(il, jl, kl, s * NS + l) * yy[s] + Dm(i, j, k, s * NS + l) * yyp[s]));
+                DijY[s * NS + l] = 0.5 * (Dm(il, jl, kl, l * NS + s) * yy[l] + Dm(iu, ju, ku, l * NS + s) * yyp[l] +
+                                          (Dm(il, jl, kl, s * NS + l) * yy[s] + Dm(iu, ju, ku, s * NS + l) * yyp[s]));
         }
 #pragma unroll
         for (int s = 0; s < NS; ++s)
@@ -221,21 +391,72 @@ __global__ void __launch_bounds__(BLK) k_stoch(DevParams P, Arr prim, Arr eta, A
         for (int s = 0; s < NS; ++s) {
 #pragma unroll
             for (int l = 0; l <= s; ++l) w[5 + s] = w[5 + s] + (wsp * sqD[s * NS + l]) * ran(i, j, k, 5 + l);
-            fl(i, j, k, 5 + s) = w[5 + s];
+            o[5 + s] = w[5 + s];
         }
         double soret = 0.;
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
             double hk = P.e0[s] + P.hcp[s] * meanT;
-            double ss = (hk + P.Runiv * meanT / P.molmass[s] * 0.5 * (chi(il, jl, kl, s) + chi(i, j, k, s))) *
+            double ss = (hk + P.Runiv * meanT / P.molmass[s] * 0.5 * (chi(il, jl, kl, s) + chi(iu, ju, ku, s))) *
                         w[5 + s];
             soret = soret + ss;
         }
-        fl(i, j, k, nv + 3) = fl(i, j, k, nv + 3) + soret;
+        o[nv + 3] = o[nv + 3] + soret;
+    }
+
+    if (P.any_wall) {  // StochFlux on the wall planes of direction D
+        const int c = (D == 0) ? i : (D == 1) ? j : k;
+        const int f = (c == 0) ? 2 * D : (c == P.n[D]) ? 2 * D + 1 : -1;
+        if (f >= 0) {
+            const int bm = P.bcm[f], bt = P.bct[f], bv = P.bcv[f];
+            if (bm == 1 || bm == 2) {
+                const double factor = (bm == 1) ? 0. : sqrt(2.);
+                if (P.algorithm_type == 2)
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) o[5 + s] *= factor;
+                o[nv + 3] *= factor;
+            }
+            if (bt == 1 || bt == 2) {
+                double factor = (bt == 1) ? 0. : sqrt(2.);
+                if (D == 0 && bm == 3) factor = 1.0;
+                o[nv] *= factor;
+            }
+            if (bv == 1 || bv == 2) {
+                const double factor = (bv == 1) ? 0. : sqrt(2.);
+                const double sqrtTwo = sqrt(2.);
+#pragma unroll
+                for (int n = 0; n < 3; ++n) {
+                    if (n == D) {
+                        o[1 + n] *= sqrtTwo;
+                        o[nv + 1] *= sqrtTwo;
+                        o[nv + 2] *= factor;
+                    } else {
+                        o[1 + n] *= factor;
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int n = 1; n < 4; ++n) fl(i, j, k, n) = o[n];
+    fl(i, j, k, nv) = o[nv];
+    fl(i, j, k, nv + 1) = o[nv + 1];
+    fl(i, j, k, nv + 2) = o[nv + 2];
+    if (P.algorithm_type == 2) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) fl(i, j, k, 5 + s) = o[5 + s];
+        fl(i, j, k, nv + 3) = o[nv + 3];
+    } else if (P.any_wall) {
+        fl(i, j, k, nv + 3) = o[nv + 3];
     }
 }
 
 // ---- diffusive face flux (flux.cpp:1193-1623) ----------------------------------------------
+// At a wall face the gradients span half a cell and the coefficients, means and the Soret
+// chi*X read the wall-side cell (flux.cpp:1218-1290 and the y/z images; the z-face chi quirks
+// of :1561, :1604 are not in those branches).  BCWallSpeciesFlux (boundary.cpp:1212-1390) then
+// zeroes species, density and Dufour fluxes on bc_mass 1 faces; the corner pass in between
+// touches none of those components, so it is applied here.
 
 template <int NS, int D>
 __global__ void __launch_bounds__(BLK) k_diff(DevParams P, Arr prim, Arr eta, Arr kappa, Arr chi,
@@ -247,18 +468,23 @@ __global__ void __launch_bounds__(BLK) k_diff(DevParams P, Arr prim, Arr eta, Ar
     int i, j, k;
     face_index(t, fx, fy, i, j, k);
     const int il = i - (D == 0), jl = j - (D == 1), kl = k - (D == 2);
+    const WallFace W = wall_face<D>(P, i, j, k);
+    const bool wall = W.any();
+    // coefficient / mean cells: the two neighbours, or the wall-side cell twice
+    const int ai = wall ? W.wi : il, aj = wall ? W.wj : jl, ak = wall ? W.wk : kl;
+    const int bi = wall ? W.wi : i, bj = wall ? W.wj : j, bk = wall ? W.wk : k;
     const int nv = NS + 5;
-    const double half = 0.5, h = P.dx[D];
+    const double half = 0.5, h = wall ? 0.5 * P.dx[D] : P.dx[D];
 
-    double mup = half * (eta(i, j, k) + eta(il, jl, kl));
-    double kp = half * (kappa(i, j, k) + kappa(il, jl, kl));
+    double mup = half * (eta(bi, bj, bk) + eta(ai, aj, ak));
+    double kp = half * (kappa(bi, bj, bk) + kappa(ai, aj, ak));
     double tau1 = mup * (prim(i, j, k, 1) - prim(il, jl, kl, 1)) / h;
     double tau2 = mup * (prim(i, j, k, 2) - prim(il, jl, kl, 2)) / h;
     double tau3 = mup * (prim(i, j, k, 3) - prim(il, jl, kl, 3)) / h;
     double divp = 0.;
-    double s1 = prim(il, jl, kl, 1) + prim(i, j, k, 1);
-    double s2 = prim(il, jl, kl, 2) + prim(i, j, k, 2);
-    double s3 = prim(il, jl, kl, 3) + prim(i, j, k, 3);
+    double s1 = prim(ai, aj, ak, 1) + prim(bi, bj, bk, 1);
+    double s2 = prim(ai, aj, ak, 2) + prim(bi, bj, bk, 2);
+    double s3 = prim(ai, aj, ak, 3) + prim(bi, bj, bk, 3);
     double phiflx;
     if (D == 0) phiflx = tau1 * s1 + divp * s1 + tau2 * s2 + tau3 * s3;
     else if (D == 1) phiflx = tau1 * s1 + tau2 * s2 + divp * s2 + tau3 * s3;
@@ -270,23 +496,24 @@ __global__ void __launch_bounds__(BLK) k_diff(DevParams P, Arr prim, Arr eta, Ar
     fl(i, j, k, nv) = fl(i, j, k, nv) - Qflux;
     fl(i, j, k, nv + 1) = fl(i, j, k, nv + 1) - (half * phiflx);
 
-    double meanT = 0.5 * (prim(il, jl, kl, 4) + prim(i, j, k, 4));
-    double meanP = 0.5 * (prim(il, jl, kl, 5) + prim(i, j, k, 5));
+    double meanT = 0.5 * (prim(ai, aj, ak, 4) + prim(bi, bj, bk, 4));
+    double meanP = 0.5 * (prim(ai, aj, ak, 5) + prim(bi, bj, bk, 5));
     if (P.algorithm_type == 2) {
         double dk[NS], soret[NS], Fk[NS];
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
             double XU = prim(i, j, k, 6 + NS + s), XL = prim(il, jl, kl, 6 + NS + s);
             double term1 = (XU - XL) / h;
-            double mX = 0.5 * (XL + XU);
-            double mY = 0.5 * (prim(il, jl, kl, 6 + s) + prim(i, j, k, 6 + s));
+            double mX = 0.5 * (prim(ai, aj, ak, 6 + NS + s) + prim(bi, bj, bk, 6 + NS + s));
+            double mY = 0.5 * (prim(ai, aj, ak, 6 + s) + prim(bi, bj, bk, 6 + s));
             double term2 = (mX - mY) * (prim(i, j, k, 5) - prim(il, jl, kl, 5)) / h / meanP;
             dk[s] = term1 + term2;
             double ChiX;
-            if (D == 2)  // flux.cpp:1561 reads chi at k and k+1
+            if (D == 2 && !wall)  // flux.cpp:1561 reads chi at k and k+1
                 ChiX = 0.5 * (chi(i, j, k, s) * XL + chi(i, j, k + 1, s) * XU);
             else
-                ChiX = 0.5 * (chi(il, jl, kl, s) * XL + chi(i, j, k, s) * XU);
+                ChiX = 0.5 * (chi(ai, aj, ak, s) * prim(ai, aj, ak, 6 + NS + s) +
+                              chi(bi, bj, bk, s) * prim(bi, bj, bk, 6 + NS + s));
             soret[s] = ChiX * (prim(i, j, k, 4) - prim(il, jl, kl, 4)) / h / meanT;
         }
 #pragma unroll
@@ -294,7 +521,7 @@ __global__ void __launch_bounds__(BLK) k_diff(DevParams P, Arr prim, Arr eta, Ar
             double f = 0.;
 #pragma unroll
             for (int ll = 0; ll < NS; ++ll) {
-                double Fks = half * (Dm(il, jl, kl, ll * NS + kk) + Dm(i, j, k, ll * NS + kk)) * (dk[ll] + soret[ll]);
+                double Fks = half * (Dm(ai, aj, ak, ll * NS + kk) + Dm(bi, bj, bk, ll * NS + kk)) * (dk[ll] + soret[ll]);
                 f = f - Fks;
             }
             Fk[kk] = f;
@@ -303,14 +530,20 @@ __global__ void __launch_bounds__(BLK) k_diff(DevParams P, Arr prim, Arr eta, Ar
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
             double hk = P.e0[s] + P.hcp[s] * meanT;
-            double csum = (D == 2) ? chi(i, j, k, s) + chi(i, j, k, s)  // flux.cpp:1604
-                                   : chi(il, jl, kl, s) + chi(i, j, k, s);
+            double csum = (D == 2 && !wall) ? chi(i, j, k, s) + chi(i, j, k, s)  // flux.cpp:1604
+                                            : chi(ai, aj, ak, s) + chi(bi, bj, bk, s);
             double Q5s = (hk + 0.5 * P.Runiv * meanT * csum / P.molmass[s]) * Fk[s];
             Q5 = Q5 + Q5s;
         }
         fl(i, j, k, nv + 3) = fl(i, j, k, nv + 3) + Q5;
 #pragma unroll
         for (int s = 0; s < NS; ++s) fl(i, j, k, 5 + s) = fl(i, j, k, 5 + s) + Fk[s];
+    }
+    if (wall && P.bcm[2 * D + (W.hi ? 1 : 0)] == 1) {  // BCWallSpeciesFlux
+#pragma unroll
+        for (int s = 0; s < NS; ++s) fl(i, j, k, 5 + s) = 0.;
+        fl(i, j, k, 0) = 0.;
+        fl(i, j, k, nv + 3) = 0.;
     }
 }
 
@@ -333,7 +566,32 @@ __global__ void __launch_bounds__(BLK) k_corner_node(DevParams P, Arr prim, Arr 
         zetaxp = 0.125 * (zeta(i, j - 1, k - 1) + zeta(i - 1, j - 1, k - 1) + zeta(i, j, k - 1) +
                           zeta(i - 1, j, k - 1) + zeta(i, j - 1, k) + zeta(i - 1, j - 1, k) + zeta(i, j, k) +
                           zeta(i - 1, j, k));
-    const double DX0 = P.dx[0], DX1 = P.dx[1], DX2 = P.dx[2];
+    double DXa[3] = {P.dx[0], P.dx[1], P.dx[2]};
+    if (P.any_wall) {
+        // wall nodes (flux.cpp:1647-1682): half spacing across each wall met, viscosities from
+        // the four ghost-layer cells of the last wall met (the checks run lo x .. hi z)
+        const int nd[3] = {i, j, k};
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+            const int d = f >> 1, side = f & 1;
+            if (!P.wall[f] || nd[d] != (side ? P.n[d] : 0)) continue;
+            DXa[d] = 0.5 * P.dx[d];
+            const int a = (d == 0) ? 1 : 0, b = (d == 2) ? 1 : 2;
+            double e4 = 0., z4 = 0.;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                int c[3];
+                c[d] = side ? nd[d] : nd[d] - 1;
+                c[a] = nd[a] - 1 + (q >> 1);
+                c[b] = nd[b] - 1 + (q & 1);
+                e4 = (q == 0) ? eta(c[0], c[1], c[2]) : e4 + eta(c[0], c[1], c[2]);
+                z4 = (q == 0) ? zeta(c[0], c[1], c[2]) : z4 + zeta(c[0], c[1], c[2]);
+            }
+            muxp = 0.25 * e4;
+            zetaxp = (P.visc_type == 3 || P.visc_type == -3) ? 0.25 * z4 : 0.;
+        }
+    }
+    const double DX0 = DXa[0], DX1 = DXa[1], DX2 = DXa[2];
     Arr gx[3] = {cux, cvx, cwx}, gy[3] = {cuy, cvy, cwy}, gz[3] = {cuz, cvz, cwz};
 #pragma unroll
     for (int n = 1; n < 4; ++n) {
@@ -404,7 +662,12 @@ __global__ void __launch_bounds__(BLK) k_corner_face(DevParams P, Arr prim, Arr 
     if (t >= (long)fx * fy * fz) return;
     int i, j, k;
     face_index(t, fx, fy, i, j, k);
-    const int il = i - (D == 0), jl = j - (D == 1), kl = k - (D == 2);
+    const WallFace W = wall_face<D>(P, i, j, k);
+    // velocity cells of the heating term: the two neighbours, or the wall-side cell twice
+    // (flux.cpp:1791-1822: 0.5*x*u_wall equals 0.25*x*(u_wall + u_wall) exactly)
+    const int il = W.any() ? W.wi : i - (D == 0), jl = W.any() ? W.wj : j - (D == 1),
+              kl = W.any() ? W.wk : k - (D == 2);
+    const int iu = W.any() ? W.wi : i, ju = W.any() ? W.wj : j, ku = W.any() ? W.wk : k;
     const int nv = NS + 5;
     const bool zo = (D == 2);
     double vc4 = node4<D>(vcorn, i, j, k, false);
@@ -418,9 +681,9 @@ __global__ void __launch_bounds__(BLK) k_corner_face(DevParams P, Arr prim, Arr 
     fl(i, j, k, cn) = fl(i, j, k, cn) + .25 * div8;
     fl(i, j, k, tc1) = fl(i, j, k, tc1) - .25 * s1;
     fl(i, j, k, tc2) = fl(i, j, k, tc2) - .25 * s2;
-    double phiflx = 0.25 * (vc4 - div8) * (prim(il, jl, kl, cn) + prim(i, j, k, cn));
-    phiflx = phiflx + .25 * s1 * (prim(il, jl, kl, tc1) + prim(i, j, k, tc1));
-    phiflx = phiflx + .25 * s2 * (prim(il, jl, kl, tc2) + prim(i, j, k, tc2));
+    double phiflx = 0.25 * (vc4 - div8) * (prim(il, jl, kl, cn) + prim(iu, ju, ku, cn));
+    phiflx = phiflx + .25 * s1 * (prim(il, jl, kl, tc1) + prim(iu, ju, ku, tc1));
+    phiflx = phiflx + .25 * s2 * (prim(il, jl, kl, tc2) + prim(iu, ju, ku, tc2));
     fl(i, j, k, nv + 1) = fl(i, j, k, nv + 1) - 0.5 * phiflx;
 }
 
@@ -439,11 +702,18 @@ __global__ void __launch_bounds__(BLK) k_hyp(DevParams P, Arr cons, Arr prim, Ar
     const double wgt2 = 1. / 12.;
     const double wgt1 = 0.5 + wgt2;
     double cv[NS + 5], pv[NS + 6], Yk[NS];
+    // walls: the wall face takes the wall-side cell (flux.cpp:2028-2037); with conserved
+    // interpolation the next face in uses the one-sided weights of :1994-1997 (:2190-2206)
+    const WallFace W = wall_face<D>(P, i, j, k);
+    const int c = (D == 0) ? i : (D == 1) ? j : k;
+    const bool near_lo = P.any_wall && c == 1 && P.wall[2 * D];
+    const bool near_hi = P.any_wall && c == P.n[D] - 1 && P.wall[2 * D + 1];
     if (P.advection_type == 1) {
 #pragma unroll
         for (int l = 0; l < NS + 6; ++l)
-            pv[l] = wgt1 * (prim(i, j, k, l) + prim(i - ex, j - ey, k - ez, l)) -
-                    wgt2 * (prim(i - 2 * ex, j - 2 * ey, k - 2 * ez, l) + prim(i + ex, j + ey, k + ez, l));
+            pv[l] = W.any() ? prim(W.wi, W.wj, W.wk, l)
+                            : wgt1 * (prim(i, j, k, l) + prim(i - ex, j - ey, k - ez, l)) -
+                                  wgt2 * (prim(i - 2 * ex, j - 2 * ey, k - 2 * ez, l) + prim(i + ex, j + ey, k + ez, l));
         double rho = pv[0];
         cv[0] = rho;
 #pragma unroll
@@ -454,10 +724,17 @@ __global__ void __launch_bounds__(BLK) k_hyp(DevParams P, Arr cons, Arr prim, Ar
 #pragma unroll
         for (int n = 0; n < NS; ++n) cv[5 + n] = cv[0] * pv[6 + n];
     } else {
+        const double wgta = -0.2, wgtb = 0.75, wgtc = 0.5, wgtd = -0.05;
 #pragma unroll
-        for (int l = 0; l < NS + 5; ++l)
-            cv[l] = wgt1 * (cons(i, j, k, l) + cons(i - ex, j - ey, k - ez, l)) -
-                    wgt2 * (cons(i - 2 * ex, j - 2 * ey, k - 2 * ez, l) + cons(i + ex, j + ey, k + ez, l));
+        for (int l = 0; l < NS + 5; ++l) {
+            const double cm2 = cons(i - 2 * ex, j - 2 * ey, k - 2 * ez, l), cm1 = cons(i - ex, j - ey, k - ez, l);
+            const double c0 = cons(i, j, k, l), cp1 = cons(i + ex, j + ey, k + ez, l);
+            if (W.lo) cv[l] = cm1;
+            else if (W.hi) cv[l] = c0;
+            else if (near_lo) cv[l] = wgta * cm2 + wgtb * cm1 + wgtc * c0 + wgtd * cp1;
+            else if (near_hi) cv[l] = wgta * cp1 + wgtb * c0 + wgtc * cm1 + wgtd * cm2;
+            else cv[l] = wgt1 * (c0 + cm1) - wgt2 * (cm2 + cp1);
+        }
 #pragma unroll
         for (int l = 1; l < 4; ++l) pv[l] = cv[l] / cv[0];
 #pragma unroll
@@ -599,8 +876,11 @@ __global__ void __launch_bounds__(BLK) k_noise_face(DevParams P, NoiseKey K, Arr
     const int jw = (D == 1 && j == P.n[1]) ? 0 : j;
     const int kw = (D == 2 && k == P.n[2]) ? 0 : k;
     uint32_t lin = (uint32_t)(((long)kw * P.n[1] + jw) * P.n[0] + iw);
+    // in a wall direction face n is a face of its own, drawn from its own field
+    const int c = (D == 0) ? i : (D == 1) ? j : k;
+    const uint32_t wall_hi = (c == P.n[D] && !P.periodic[D]) ? 128u : 0u;
     double za, zb;
-    noise_ab(K, (uint32_t)(16 * D + comp), lin, za, zb);
+    noise_ab(K, (uint32_t)(16 * D + comp) + wall_hi, lin, za, zb);
     A(i, j, k, comp) = za * scale;
     B(i, j, k, comp) = zb * scale;
 }
@@ -614,8 +894,12 @@ __global__ void __launch_bounds__(BLK) k_noise_node(DevParams P, NoiseKey K, Arr
     face_index(t, nx, ny, i, j, k);
     const int iw = (i == P.n[0]) ? 0 : i, jw = (j == P.n[1]) ? 0 : j, kw = (k == P.n[2]) ? 0 : k;
     uint32_t lin = (uint32_t)(((long)kw * P.n[1] + jw) * P.n[0] + iw);
+    // nodes on the hi plane of a wall direction are distinct: field NODE_FIELD + mask
+    const uint32_t mask = ((i == P.n[0] && !P.periodic[0]) ? 1u : 0u) |
+                          ((j == P.n[1] && !P.periodic[1]) ? 2u : 0u) |
+                          ((k == P.n[2] && !P.periodic[2]) ? 4u : 0u);
     double za, zb;
-    noise_ab(K, NODE_FIELD, lin, za, zb);
+    noise_ab(K, NODE_FIELD + mask, lin, za, zb);
     A(i, j, k) = za * scale;
     B(i, j, k) = zb * scale;
 }
@@ -660,7 +944,19 @@ int ref_transport(const DevParams& P, Arr prim, Arr eta, Arr zeta, Arr kappa, Ar
 
 int ref_fill_boundary(const DevParams& P, Arr a, int nc, hipStream_t s)
 {
-    k_fill_boundary<<<grid_for(ngrown(P), BLK), BLK, 0, s>>>(a, nc, P.ngc, P.n[0], P.n[1], P.n[2]);
+    k_fill_boundary<<<grid_for(ngrown(P), BLK), BLK, 0, s>>>(P, a, nc);
+    FHDX_HIP_TRY(hipGetLastError());
+    return FHDX_OK;
+}
+
+int ref_set_bc(const DevParams& P, Arr prim, Arr cons, hipStream_t s)
+{
+    if (!P.any_wall) return FHDX_OK;
+    for (int d = 0; d < 3; ++d) {
+        if (P.periodic[d]) continue;
+        for (int side = 0; side < 2; ++side)
+            NS_DISPATCH(P.ns, (k_set_bc<NS><<<grid_for(ngrown(P), BLK), BLK, 0, s>>>(P, prim, cons, d, side)));
+    }
     FHDX_HIP_TRY(hipGetLastError());
     return FHDX_OK;
 }

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "libfhdx.so")
 EXPORTS = [
     "fhdx_abi_version", "fhdx_status_string", "fhdx_last_error", "fhdx_params_default",
     "fhdx_params_finalize", "fhdx_params_check", "fhdx_ctx_create", "fhdx_ctx_destroy",
-    "fhdx_ctx_check", "fhdx_cons_to_prim", "fhdx_transport", "fhdx_fill_boundary",
+    "fhdx_ctx_check", "fhdx_cons_to_prim", "fhdx_transport", "fhdx_fill_boundary", "fhdx_set_bc",
     "fhdx_calculate_flux", "fhdx_fill_noise", "fhdx_rk3_step_ref", "fhdx_rk3_step",
     "fhdx_rk3_step_injected", "fhdx_ctx_timing_enable", "fhdx_timing_kinds",
     "fhdx_ctx_timing_read", "fhdx_engine_bytes_needed", "fhdx_engine_create",
@@ -91,6 +91,7 @@ def lib():
         L.fhdx_cons_to_prim.argtypes = [vp, Fab, Fab, vp]
         L.fhdx_transport.argtypes = [vp, Fab, Fab, Fab, Fab, Fab, Fab, vp]
         L.fhdx_fill_boundary.argtypes = [vp, Fab, vp]
+        L.fhdx_set_bc.argtypes = [vp, Fab, Fab, vp]
         L.fhdx_calculate_flux.argtypes = [vp, Fab, Fab, Fab, Fab, Fab, Fab, Fab, FA, FA, FA, FA, FA,
                                           Fab, Fab, C.c_double, vp]
         L.fhdx_fill_noise.argtypes = [vp, C.c_uint64, C.c_uint64, C.c_uint32, FA, FA, Fab, Fab, vp]

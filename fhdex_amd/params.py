@@ -52,7 +52,19 @@ class Params(C.Structure):
         ("variance_coef_mass", C.c_double),
         ("grav", C.c_double * 3),
         ("zeta_ratio", C.c_double),
+        ("t_lo", C.c_double * 3),
+        ("t_hi", C.c_double * 3),
+        ("bc_Yk", (C.c_double * MAX_SPECIES) * 6),
+        ("bc_Xk", (C.c_double * MAX_SPECIES) * 6),
     ]
+
+    @property
+    def periodic(self):
+        return tuple(self.bc_vel_lo[d] == -1 for d in range(3))
+
+    @property
+    def has_walls(self) -> bool:
+        return not all(self.periodic)
 
     @property
     def nvars(self) -> int:
@@ -267,3 +279,84 @@ def no2_n2o4(n, nsteps=100, reaction_type=1):
                           beta=[0.644950, -1.100000])
     return Config(f"no2_n2o4_{n}^3", p, 1.858338e-03, [0.723969, 0.276031], 350.0, 1e-12,
                   nsteps, chem=chem)
+
+
+def set_walls(p: Params, bc_vel, bc_therm, bc_mass, t_lo=(0.0, 0.0, 0.0), t_hi=(0.0, 0.0, 0.0),
+              bc_Yk=None) -> Params:
+    """Boundary codes of the reference's namelist (common_namespace.H): bc_vel, bc_therm and
+    bc_mass are ((lo, hi) per direction) or a single (lo, hi) pair or code for all three.
+    A direction with bc_vel -1 is periodic and every code of it becomes -1 (SetupBC,
+    boundary.cpp:3-12).  bc_Yk is one composition for every bc_mass 2 face or a list of six
+    (face 2*direction + side); the mole fractions follow from GetMolfrac as SetupCWall does
+    (boundary.cpp:14-170, compressible_functions.H:139-152)."""
+
+    def per_dir(v):
+        if isinstance(v, int):
+            return [(v, v)] * 3
+        if len(v) == 2 and isinstance(v[0], int):
+            return [tuple(v)] * 3
+        return [tuple(x) if not isinstance(x, int) else (x, x) for x in v]
+
+    vel, therm, mass = per_dir(bc_vel), per_dir(bc_therm), per_dir(bc_mass)
+    for d in range(3):
+        if vel[d][0] == -1:
+            vel[d] = therm[d] = mass[d] = (-1, -1)
+        p.bc_vel_lo[d], p.bc_vel_hi[d] = vel[d]
+        p.bc_therm_lo[d], p.bc_therm_hi[d] = therm[d]
+        p.bc_mass_lo[d], p.bc_mass_hi[d] = mass[d]
+        p.t_lo[d] = float(t_lo[d])
+        p.t_hi[d] = float(t_hi[d])
+    ns = p.nspecies
+    for f in range(6):
+        code = mass[f // 2][f % 2]
+        for n in range(MAX_SPECIES):
+            p.bc_Yk[f][n] = 0.0
+            p.bc_Xk[f][n] = 0.0
+        if code != 2:
+            continue
+        Y = bc_Yk if (bc_Yk is not None and not hasattr(bc_Yk[0], "__len__")) else bc_Yk[f]
+        for n in range(ns):
+            p.bc_Yk[f][n] = float(Y[n])
+        molmix = 0.0
+        for n in range(ns):
+            molmix += p.bc_Yk[f][n] / p.molmass[n]
+        molmix = 1.0 / molmix
+        for n in range(ns):
+            p.bc_Xk[f][n] = p.bc_Yk[f][n] * molmix / p.molmass[n]
+    return p
+
+
+def wall_deck(name, n=None, nsteps=3):
+    """The reference's wall regression decks (exec/compressible/inputs_regression_*): four
+    identical argon species (M 39.948, d 3.66e-8 cm, Y 1/4 each), rho 1.78e-3, T_init 400 K.
+
+    * slip_wall_adiabatic: 32^3 cells of 6.25e-5/32 cm, dt 2e-12 s, every face a slip,
+      adiabatic, species-wall face (codes 1).
+    * noslip_reservoir_isothermal: 32^3 cells of 1e-4/32 cm, dt 2e-12 s, no-slip isothermal
+      walls at 390 K (lo) / 410 K (hi) with a fixed composition Y = 0.22/0.24/0.26/0.28 (codes 2).
+    * temp_grad: 80 x 10 x 10 cells of 2e-6 cm, dt 1e-12 s, advection_type 1; x: no-slip
+      isothermal walls at 280.8 / 508.5 K, y and z: slip adiabatic walls; species walls.
+
+    n rescales the box edge (cells per direction, the x count of temp_grad) at the deck's cell
+    size, for runs small enough for the oracle."""
+    M = [39.9480] * 4
+    dmol = [3.66e-8] * 4
+    if name == "slip_wall_adiabatic":
+        cells = (n or 32,) * 3
+        p = make_params(4, cells, 6.25e-5 / 32, M, dmol)
+        set_walls(p, 1, 1, 1, t_lo=(300.0,) * 3, t_hi=(300.0,) * 3)
+        dt = 2e-12
+    elif name == "noslip_reservoir_isothermal":
+        cells = (n or 32,) * 3
+        p = make_params(4, cells, 1e-4 / 32, M, dmol)
+        set_walls(p, 2, 2, 2, t_lo=(390.0,) * 3, t_hi=(410.0,) * 3, bc_Yk=[0.22, 0.24, 0.26, 0.28])
+        dt = 2e-12
+    elif name == "temp_grad":
+        cells = (n or 80, 10, 10)
+        p = make_params(4, cells, 2e-6, M, dmol, advection_type=1)
+        set_walls(p, [(2, 2), (1, 1), (1, 1)], [(2, 2), (1, 1), (1, 1)], 1,
+                  t_lo=(280.8, 300.0, 300.0), t_hi=(508.5, 400.0, 400.0))
+        dt = 1e-12
+    else:
+        raise ValueError(name)
+    return Config(f"{name}_{'x'.join(map(str, cells))}", p, 1.78e-3, [0.25] * 4, 400.0, dt, nsteps)

@@ -27,7 +27,7 @@ extern "C" {
 
 #define FHDX_MAX_SPECIES 8
 #define FHDX_MAX_REACTION 8
-#define FHDX_ABI_VERSION 1
+#define FHDX_ABI_VERSION 2
 
 enum fhdx_status {
     FHDX_OK = 0,
@@ -53,7 +53,13 @@ typedef struct fhdx_params {
     int32_t transport_type;    /* 1: Giovangigli (the only one implemented)                  */
     int32_t stoch_stress_form; /* 1: stochastic fluxes on, 0: deterministic                  */
     int32_t nreaction;         /* 0, or the reaction count given to fhdx_ctx_set_chemistry   */
-    int32_t bc_vel_lo[3], bc_vel_hi[3];   /* -1 periodic (the only supported value)           */
+    /* Boundary codes per direction (common_namespace.H, boundary.cpp:172-2001).  bc_vel -1:
+     * periodic (then every code of that direction is -1, SetupBC boundary.cpp:3-12); 1: slip
+     * wall, 2: no-slip wall.  bc_mass 1: wall (no species flux), 2: fixed composition bc_Yk;
+     * 3 (reservoir) is not implemented.  bc_therm 1: adiabatic, 2: isothermal at t_lo/t_hi.
+     * Walls are served by the reference-shaped entry points; the fused step, the ensemble
+     * engine and the server take periodic boxes only. */
+    int32_t bc_vel_lo[3], bc_vel_hi[3];
     int32_t bc_mass_lo[3], bc_mass_hi[3];
     int32_t bc_therm_lo[3], bc_therm_hi[3];
     int32_t pad0;
@@ -69,6 +75,12 @@ typedef struct fhdx_params {
     double variance_coef_mom, variance_coef_ener, variance_coef_mass;
     double grav[3];
     double zeta_ratio;
+    double t_lo[3], t_hi[3];   /* isothermal wall temperatures [K]                           */
+    /* Wall composition of bc_mass 2 faces, face f = 2*direction + (0 lo, 1 hi): the reference's
+     * bc_Yk_x_lo .. bc_Yk_z_hi and bc_Xk_*.  fhdx_params_finalize fills whichever of the two sums
+     * to 0 from the other (SetupCWall, boundary.cpp:14-170). */
+    double bc_Yk[6][FHDX_MAX_SPECIES];
+    double bc_Xk[6][FHDX_MAX_SPECIES];
 } fhdx_params;
 
 /* One FAB: data pointer plus the inclusive index box it covers (ghosts included). */
@@ -91,7 +103,8 @@ const char* fhdx_last_error(void);
 /* Reference defaults of src_common/common_functions.cpp:320-490 with the compressible
  * periodic settings every reference input uses (ngc 2, algorithm/visc/advection type 2). */
 void fhdx_params_default(fhdx_params* p);
-/* GetHcGas (src_compressible/compressible_functions.cpp:91-100): hcv/hcp < 0 -> from dof. */
+/* GetHcGas (src_compressible/compressible_functions.cpp:91-100): hcv/hcp < 0 -> from dof;
+ * SetupBC (boundary.cpp:3-12) and the wall compositions of SetupCWall (boundary.cpp:14-170). */
 void fhdx_params_finalize(fhdx_params* p);
 /* 0 if the parameters are inside the implemented scope, else an fhdx_status. */
 int fhdx_params_check(const fhdx_params* p);
@@ -114,12 +127,19 @@ int fhdx_cons_to_prim(fhdx_ctx* ctx, fhdx_fab prim, fhdx_fab cons, void* stream)
 int fhdx_transport(fhdx_ctx* ctx, fhdx_fab prim, fhdx_fab eta, fhdx_fab zeta, fhdx_fab kappa,
                    fhdx_fab chi, fhdx_fab D, void* stream);
 
-/* MultiFab::FillBoundary(periodicity) for one periodic cell FAB (all components). */
+/* MultiFab::FillBoundary(periodicity) for one cell FAB (all components): every ghost cell whose
+ * periodic image is a valid cell gets that image; ghost cells outside a wall are left alone. */
 int fhdx_fill_boundary(fhdx_ctx* ctx, fhdx_fab mf, void* stream);
+
+/* setBC (src_compressible/boundary.cpp:172-1209): the physical ghost cells of prim and cons at
+ * walls, face by face in the order lo x, hi x, lo y, hi y, lo z, hi z.  No-op when periodic. */
+int fhdx_set_bc(fhdx_ctx* ctx, fhdx_fab prim, fhdx_fab cons, void* stream);
 
 /* calculateFlux (src_compressible/flux.cpp:4-2376; signature compressible_functions.H:23-35):
  * stochastic, diffusive, corner and hyperbolic face fluxes into flux[3] (nvars+4 comps),
- * the corner arrays and visccorn.  stochFlux/rancorn hold the already weighted noise. */
+ * the corner arrays and visccorn.  stochFlux/rancorn hold the already weighted noise.  At walls
+ * the one-sided face branches of flux.cpp, StochFlux (boundary.cpp:1392-1970) and
+ * BCWallSpeciesFlux (boundary.cpp:1212-1390) apply. */
 int fhdx_calculate_flux(fhdx_ctx* ctx, fhdx_fab cons, fhdx_fab prim, fhdx_fab eta,
                         fhdx_fab zeta, fhdx_fab kappa, fhdx_fab chi, fhdx_fab D,
                         const fhdx_fab flux[3], const fhdx_fab stochFlux[3],
@@ -130,7 +150,8 @@ int fhdx_calculate_flux(fhdx_ctx* ctx, fhdx_fab cons, fhdx_fab prim, fhdx_fab et
 /* The noise of one step (replaces the MultiFabFillRandom calls of timeStep.cpp:75-91):
  * N(0,1)*|variance_coef| from a counter-based Philox4x32-10 keyed by (seed, step, instance),
  * set A and set B, face components 1..nvars-1 and the nodal field.  Duplicate periodic
- * faces/nodes (index n_d == index 0) carry the same value, as after OverrideSync. */
+ * faces/nodes (index n_d == index 0) carry the same value, as after OverrideSync; in a wall
+ * direction the faces and nodes at index 0 and n_d are distinct and draw their own values. */
 int fhdx_fill_noise(fhdx_ctx* ctx, uint64_t seed, uint64_t step, uint32_t instance,
                     const fhdx_fab stochA[3], const fhdx_fab stochB[3], fhdx_fab rancornA,
                     fhdx_fab rancornB, void* stream);
@@ -139,7 +160,9 @@ int fhdx_fill_noise(fhdx_ctx* ctx, uint64_t seed, uint64_t step, uint32_t instan
  * with the reference's own arrays: cu is advanced in place, cup/cup2 are stage scratch, prim,
  * eta..D, flux, stochFlux, corners, visccorn and rancorn end as the reference leaves them.
  * Noise: if stochA is non-NULL the caller's A/B arrays are used (injected noise), otherwise
- * Philox noise for (seed, step, instance 0) is generated into ctx workspace. */
+ * Philox noise for (seed, step, instance 0) is generated into ctx workspace.  At walls the
+ * ghost cells after each stage come from setBC (timeStep.cpp:207-209, 312-314, 420-425); the
+ * caller's cu/prim enter with their ghosts set (main_driver.cpp:535-542). */
 int fhdx_rk3_step_ref(fhdx_ctx* ctx, fhdx_fab cu, fhdx_fab cup, fhdx_fab cup2, fhdx_fab prim,
                       fhdx_fab eta, fhdx_fab zeta, fhdx_fab kappa, fhdx_fab chi, fhdx_fab D,
                       const fhdx_fab flux[3], const fhdx_fab stochFlux[3],
@@ -167,7 +190,8 @@ int fhdx_evaluate_stats(fhdx_ctx* ctx, fhdx_fab cons, fhdx_fab consMean, fhdx_fa
 /* One RK3 step of RK3step's arithmetic with fused stage kernels: cu (nvars comps, ngc ghosts)
  * advanced in place, prim (nprimvars comps) recomputed from the new cu; ghost layers of both
  * are filled periodically, as after timeStep.cpp:417-425.  Noise is generated in-kernel from
- * Philox (seed, step, instance 0), bit-identical to fhdx_fill_noise for the same key. */
+ * Philox (seed, step, instance 0), bit-identical to fhdx_fill_noise for the same key.
+ * Periodic boxes only (walls: FHDX_ERR_UNSUPPORTED; use fhdx_rk3_step_ref). */
 int fhdx_rk3_step(fhdx_ctx* ctx, fhdx_fab cu, fhdx_fab prim, double dt, uint64_t seed,
                   uint64_t step, void* stream);
 

@@ -56,7 +56,19 @@ inline fhdx_params params_from_namespaces(amrex::Geometry const& geom)
         p.bc_therm_lo[d] = bc_therm_lo[d];
         p.bc_therm_hi[d] = bc_therm_hi[d];
         p.grav[d] = grav[d];
+        p.t_lo[d] = t_lo[d];
+        p.t_hi[d] = t_hi[d];
     }
+    // wall compositions (common_namespace.H:173-184), face 2*direction + side
+    const amrex::GpuArray<amrex::Real, MAX_SPECIES>* Yw[6] = {&bc_Yk_x_lo, &bc_Yk_x_hi, &bc_Yk_y_lo,
+                                                              &bc_Yk_y_hi, &bc_Yk_z_lo, &bc_Yk_z_hi};
+    const amrex::GpuArray<amrex::Real, MAX_SPECIES>* Xw[6] = {&bc_Xk_x_lo, &bc_Xk_x_hi, &bc_Xk_y_lo,
+                                                              &bc_Xk_y_hi, &bc_Xk_z_lo, &bc_Xk_z_hi};
+    for (int f = 0; f < 6; ++f)
+        for (int n = 0; n < nspecies; ++n) {
+            p.bc_Yk[f][n] = (*Yw[f])[n];
+            p.bc_Xk[f][n] = (*Xw[f])[n];
+        }
     p.ngc = ngc[0];
     p.algorithm_type = algorithm_type;
     p.advection_type = advection_type;
@@ -159,6 +171,14 @@ inline void conservedToPrimitive(amrex::MultiFab& prim, const amrex::MultiFab& c
         fhdx_amrex::check(fhdx_cons_to_prim(fhdx_amrex::context(amrex::DefaultGeometry()),
                                             fhdx_amrex::fab(prim, mfi), fhdx_amrex::fab(cons, mfi),
                                             fhdx_amrex::stream()));
+}
+
+// setBC (boundary.cpp:172-1209), after prim.FillBoundary / cons.FillBoundary.
+inline void setBC(amrex::MultiFab& prim, amrex::MultiFab& cons)
+{
+    for (amrex::MFIter mfi(prim); mfi.isValid(); ++mfi)
+        fhdx_amrex::check(fhdx_set_bc(fhdx_amrex::context(amrex::DefaultGeometry()), fhdx_amrex::fab(prim, mfi),
+                                      fhdx_amrex::fab(cons, mfi), fhdx_amrex::stream()));
 }
 
 inline void calculateTransportCoeffs(const amrex::MultiFab& prim, amrex::MultiFab& eta, amrex::MultiFab& zeta,

@@ -34,6 +34,7 @@ def lib():
         L.orc_cons_to_prim.argtypes = [P, Fab, Fab]
         L.orc_transport.argtypes = [P, Fab, Fab, Fab, Fab, Fab, Fab]
         L.orc_fill_boundary.argtypes = [P, Fab]
+        L.orc_set_bc.argtypes = [P, Fab, Fab]
         L.orc_calculate_flux.argtypes = [P, Fab, Fab, Fab, Fab, Fab, Fab, Fab, FA, FA, FA, FA,
                                          FA, Fab, Fab, C.c_double]
         L.orc_rk3_step.argtypes = [P, Fab, Fab, Fab, Fab, Fab, Fab, Fab, Fab, Fab, FA, FA, FA,
@@ -141,6 +142,12 @@ def cons_to_prim(p, prim, cons):
 def fill_boundary(p, a):
     g = p.ngc
     return lib().orc_fill_boundary(C.byref(p), fab_of(a, (-g,) * 3))
+
+
+def set_bc(p, prim, cons):
+    """setBC on host cell arrays with ngc ghosts."""
+    g = p.ngc
+    return lib().orc_set_bc(C.byref(p), fab_of(prim, (-g,) * 3), fab_of(cons, (-g,) * 3))
 
 
 def transport(p, st: HostState):

@@ -396,18 +396,125 @@ int orc_transport(const fhdx_params* P, fhdx_fab prim_, fhdx_fab eta_, fhdx_fab 
 
 static int wrap(int i, int n) { return ((i % n) + n) % n; }
 
+/* A direction is periodic when its bc_vel is -1 (SetupBC, boundary.cpp:3-12). */
+static int is_periodic(const fhdx_params* P, int d) { return P->bc_vel_lo[d] == -1; }
+/* is_{lo,hi}_*_dirichlet_mass (flux.cpp:29-34): a wall face unless periodic or reservoir. */
+static int dir_mass(int code) { return code != 3 && code != -1; }
+static int wall_lo(const fhdx_params* P, int d) { return dir_mass(P->bc_mass_lo[d]); }
+static int wall_hi(const fhdx_params* P, int d) { return dir_mass(P->bc_mass_hi[d]); }
+
+/* FillBoundary(periodicity): a ghost cell whose periodic image is a valid cell takes that
+ * image; ghost cells outside a wall keep what they hold (setBC writes them). */
 int orc_fill_boundary(const fhdx_params* P, fhdx_fab mf_)
 {
     fab_t f = F(mf_);
-    const int nx = P->n_cells[0], ny = P->n_cells[1], nz = P->n_cells[2];
+    const int n[3] = {P->n_cells[0], P->n_cells[1], P->n_cells[2]};
+    const int per[3] = {is_periodic(P, 0), is_periodic(P, 1), is_periodic(P, 2)};
 #pragma omp parallel for schedule(static)
     for (int k = f.lo2; k <= f.hi2; ++k)
         for (int j = f.lo1; j <= f.hi1; ++j)
             for (int i = f.lo0; i <= f.hi0; ++i) {
-                int iv = wrap(i, nx), jv = wrap(j, ny), kv = wrap(k, nz);
-                if (iv == i && jv == j && kv == k) continue;
-                for (int n = 0; n < f.nc; ++n) AT(f, i, j, k, n) = AT(f, iv, jv, kv, n);
+                const int c[3] = {i, j, k};
+                int v[3], inside = 1;
+                for (int d = 0; d < 3; ++d) {
+                    if (per[d]) v[d] = wrap(c[d], n[d]);
+                    else {
+                        v[d] = c[d];
+                        if (c[d] < 0 || c[d] >= n[d]) inside = 0;
+                    }
+                }
+                if (!inside || (v[0] == i && v[1] == j && v[2] == k)) continue;
+                for (int q = 0; q < f.nc; ++q) AT(f, i, j, k, q) = AT(f, v[0], v[1], v[2], q);
             }
+    return FHDX_OK;
+}
+
+/* GetDensity (compressible_functions.H:1481-1495). */
+static double get_density(const fhdx_params* P, double pres, double T, const double* Y)
+{
+    double molmix = 0.;
+    for (int n = 0; n < P->nspecies; ++n) molmix += Y[n] / P->molmass[n];
+    molmix = 1. / molmix;
+    return pres / (P->Runiv / molmix) / T;
+}
+
+/* One ghost cell of setBC's face (d, side) (boundary.cpp:180-1036; mass-fraction branches
+ * :185-209, thermal :230-254, slip :283-339, no slip :340-378 for lo x and their mirror images
+ * for the other faces).  The mirror of ghost index c is -c-1 at a lo face and 2n-c-1 at a hi
+ * face.  bc_mass 3 (reservoir) is rejected by the parameter check before this is reached. */
+static void set_bc_cell(const fhdx_params* P, fab_t* prim, fab_t* cons, int d, int side, int i,
+                        int j, int k)
+{
+    const int NS = P->nspecies, np = 6 + 2 * NS;
+    const int n = P->n_cells[d], f = 2 * d + side;
+    int c[3] = {i, j, k}, m[3] = {i, j, k};
+    m[d] = side ? 2 * (n - 1) - c[d] + 1 : -c[d] - 1;
+    const int bm = side ? P->bc_mass_hi[d] : P->bc_mass_lo[d];
+    const int bt = side ? P->bc_therm_hi[d] : P->bc_therm_lo[d];
+    const int bv = side ? P->bc_vel_hi[d] : P->bc_vel_lo[d];
+    const double tw = side ? P->t_hi[d] : P->t_lo[d];
+#define PR(a, q) AT(*prim, a[0], a[1], a[2], q)
+#define CO(a, q) AT(*cons, a[0], a[1], a[2], q)
+    if (bm == 1 && P->algorithm_type == 2)
+        for (int q = 6; q < np; ++q) PR(c, q) = PR(m, q);
+    if (bm == 2 && P->algorithm_type == 2)
+        for (int s = 0; s < NS; ++s) {
+            PR(c, 6 + s) = P->bc_Yk[f][s];
+            PR(c, 6 + NS + s) = P->bc_Xk[f][s];
+        }
+    if (bt == 1) {
+        PR(c, 4) = PR(m, 4);
+        PR(c, 5) = PR(m, 5);
+    } else if (bt == 2) {
+        PR(c, 4) = tw;
+        PR(c, 5) = PR(m, 5);
+    }
+    if (bv != 1 && bv != 2) return;
+    if (bv == 1) { /* slip: normal velocity 0, tangential mirrored (cons and prim) */
+        for (int q = 1; q < 4; ++q) {
+            CO(c, q) = (q == 1 + d) ? 0.0 : CO(m, q);
+            PR(c, q) = (q == 1 + d) ? 0.0 : PR(m, q);
+        }
+    } else {
+        PR(c, 1) = 0.0;
+        PR(c, 2) = 0.0;
+        PR(c, 3) = 0.0;
+    }
+    double Y[MS];
+    for (int s = 0; s < NS; ++s) Y[s] = PR(c, 6 + s);
+    const double T = PR(c, 4), pt = PR(c, 5);
+    const double rho = get_density(P, pt, T, Y);
+    const double ie = get_energy(P, Y, T);
+    PR(c, 0) = rho;
+    CO(c, 0) = rho;
+    if (P->algorithm_type == 2)
+        for (int s = 0; s < NS; ++s) CO(c, 5 + s) = rho * PR(c, 6 + s);
+    if (bv == 2)
+        for (int q = 1; q < 4; ++q) CO(c, q) = rho * PR(c, q);
+    CO(c, 4) = rho * ie + 0.5 * rho * (PR(c, 1) * PR(c, 1) + PR(c, 2) * PR(c, 2) + PR(c, 3) * PR(c, 3));
+#undef PR
+#undef CO
+}
+
+/* setBC (boundary.cpp:172-1209): faces lo x, hi x, lo y, hi y, lo z, hi z in turn, each over
+ * every cell of the grown box outside that face; a periodic direction is skipped. */
+int orc_set_bc(const fhdx_params* P, fhdx_fab prim_, fhdx_fab cons_)
+{
+    fab_t prim = F(prim_), cons = F(cons_);
+    for (int d = 0; d < 3; ++d) {
+        if (is_periodic(P, d)) continue;
+        const int n = P->n_cells[d];
+        for (int side = 0; side < 2; ++side) {
+#pragma omp parallel for schedule(static)
+            for (int k = prim.lo2; k <= prim.hi2; ++k)
+                for (int j = prim.lo1; j <= prim.hi1; ++j)
+                    for (int i = prim.lo0; i <= prim.hi0; ++i) {
+                        const int c = (d == 0) ? i : (d == 1) ? j : k;
+                        if (side == 0 ? c >= 0 : c < n) continue;
+                        set_bc_cell(P, &prim, &cons, d, side, i, j, k);
+                    }
+        }
+    }
     return FHDX_OK;
 }
 
@@ -480,7 +587,28 @@ static void cholesky(int NS, double* a, double* sqda)
     }
 }
 
-/* Stochastic flux of one face (flux.cpp:71-1145, periodic branches). */
+/* Cells of the four edge viscosities at a wall face, all in the ghost layer (normal offset -1
+ * at a lo face, 0 at a hi face), in the reference's summation order: transverse offsets per
+ * face direction and edge (pp, mp, pm, mm), flux.cpp:137-156 (x), 501-520 (y), 867-886 (z). */
+static const int MU_WALL[3][4][4][3] = {
+    { {{0,0,0},{0,1,0},{0,0,1},{0,1,1}}, {{0,-1,0},{0,0,0},{0,-1,1},{0,0,1}},
+      {{0,0,-1},{0,1,-1},{0,0,0},{0,1,0}}, {{0,-1,-1},{0,0,-1},{0,-1,0},{0,0,0}} },
+    { {{1,0,0},{0,0,0},{1,0,1},{0,0,1}}, {{-1,0,0},{0,0,0},{-1,0,1},{0,0,1}},
+      {{1,0,-1},{0,0,-1},{1,0,0},{0,0,0}}, {{-1,0,-1},{0,0,-1},{-1,0,0},{0,0,0}} },
+    { {{1,0,0},{0,0,0},{1,1,0},{0,1,0}}, {{-1,1,0},{0,1,0},{-1,0,0},{0,0,0}},
+      {{1,0,0},{0,0,0},{1,-1,0},{0,-1,0}}, {{-1,-1,0},{0,-1,0},{-1,0,0},{0,0,0}} }
+};
+
+/* The two transverse axes of a face direction: x faces (y, z), y faces (x, z), z faces (x, y). */
+static void transverse_axes(int d, int* a, int* b)
+{
+    *a = (d == 0) ? 1 : 0;
+    *b = (d == 2) ? 1 : 2;
+}
+
+/* Stochastic flux of one face (flux.cpp:71-1145), with the wall branches of a bc_mass 1/2 face
+ * (:88-97, :137-202 for x and their y/z images) and the tangential-wall factors of the
+ * divergence noise (:250-280). */
 static void stoch_face(const flux_args* A, int d, int i, int j, int k)
 {
     const fhdx_params* P = A->P;
@@ -492,11 +620,24 @@ static void stoch_face(const flux_args* A, int d, int i, int j, int k)
     fab_t* fl = (fab_t*)&A->flux[d];
     const double volinv = 1. / (P->dx[0] * P->dx[1] * P->dx[2]);
     const double dtinv = 1. / A->dt;
+    const int idx[3] = {i, j, k};
+    const int lo_w = idx[d] == 0 && wall_lo(P, d);
+    const int hi_w = idx[d] == P->n_cells[d] && wall_hi(P, d);
 
     double Tu = AT(*prim, i, j, k, 4), Tl = AT(*prim, il, jl, kl, 4);
     double mup = AT(*eta, i, j, k, 0) * Tu + AT(*eta, il, jl, kl, 0) * Tl;
     double kp = AT(*kappa, i, j, k, 0) * Tu * Tu + AT(*kappa, il, jl, kl, 0) * Tl * Tl;
     double meanT = 0.5 * (Tu + Tl);
+    if (lo_w) {
+        mup = 2.0 * AT(*eta, il, jl, kl, 0) * Tl;
+        kp = 2.0 * AT(*kappa, il, jl, kl, 0) * Tl * Tl;
+        meanT = Tl;
+    }
+    if (hi_w) {
+        mup = 2.0 * AT(*eta, i, j, k, 0) * Tu;
+        kp = 2.0 * AT(*kappa, i, j, k, 0) * Tu * Tu;
+        meanT = Tu;
+    }
     double fw[MS + 5], w[MS + 5];
     fw[0] = 0;
     fw[1] = sqrt(P->k_B * mup * volinv * dtinv);
@@ -508,6 +649,22 @@ static void stoch_face(const flux_args* A, int d, int i, int j, int k)
 
     double mus[4];
     for (int c = 0; c < 4; ++c) {
+        if (lo_w || hi_w) {
+            const int off = lo_w ? -1 : 0;
+            double s = 0., z = 0.;
+            for (int t = 0; t < 4; ++t) {
+                const int* o = MU_WALL[d][c][t];
+                const int ci = i + o[0] + off * di, cj = j + o[1] + off * dj, ck = k + o[2] + off * dk;
+                double e = AT(*eta, ci, cj, ck, 0) * AT(*prim, ci, cj, ck, 4);
+                double zz = AT(*zeta, ci, cj, ck, 0) * AT(*prim, ci, cj, ck, 4);
+                s = (t == 0) ? e : s + e;
+                z = (t == 0) ? zz : z + zz;
+            }
+            mus[c] = 0.5 * s / 3.;
+            if (abs(P->visc_type) == 3) /* x walls do not divide by 3, y and z walls do */
+                mus[c] += (d == 0) ? 0.5 * z : 0.5 * z / 3.;
+            continue;
+        }
         const int(*o)[6] = MU_ORDER[d][c];
         double s = AT(*eta, i + o[0][0], j + o[0][1], k + o[0][2], 0) *
                    AT(*prim, i + o[0][3], j + o[0][4], k + o[0][5], 4);
@@ -524,10 +681,23 @@ static void stoch_face(const flux_args* A, int d, int i, int j, int k)
             mus[c] += 0.25 * z;
         }
     }
+    /* tangential walls: sqrt(2) (slip) or 0 (no slip) on the edges that touch them */
+    int ta, tb;
+    transverse_axes(d, &ta, &tb);
+    double f_lo[2] = {1., 1.}, f_hi[2] = {1., 1.};
+    const int ax[2] = {ta, tb};
+    for (int q = 0; q < 2; ++q) {
+        const int t = ax[q];
+        if ((P->bc_vel_lo[t] == 1 || P->bc_vel_lo[t] == 2) && idx[t] == 0)
+            f_lo[q] = (P->bc_vel_lo[t] == 1) ? sqrt(2.0) : 0.;
+        if ((P->bc_vel_hi[t] == 1 || P->bc_vel_hi[t] == 2) && idx[t] == P->n_cells[t] - 1)
+            f_hi[q] = (P->bc_vel_hi[t] == 1) ? sqrt(2.0) : 0.;
+    }
+    const double fac[4] = {f_hi[0] * f_hi[1], f_lo[0] * f_hi[1], f_hi[0] * f_lo[1], f_lo[0] * f_lo[1]};
     double sum = 0.;
     for (int c = 0; c < 4; ++c) {
         const int* q = RC_NODE[d][c];
-        double term = sqrt(mus[c]) * AT(*rc, i + q[0], j + q[1], k + q[2], 0);
+        double term = fac[c] * sqrt(mus[c]) * AT(*rc, i + q[0], j + q[1], k + q[2], 0);
         sum = (c == 0) ? term : sum + term;
     }
     w[1 + d] = w[1 + d] + 0.25 * nweight * sum;
@@ -540,17 +710,25 @@ static void stoch_face(const flux_args* A, int d, int i, int j, int k)
     double phidiag = w[1 + d] * (AT(*prim, il, jl, kl, 1 + d) + AT(*prim, i, j, k, 1 + d));
     double phishear = w[t1] * (AT(*prim, il, jl, kl, t1) + AT(*prim, i, j, k, t1)) +
                       w[t2] * (AT(*prim, il, jl, kl, t2) + AT(*prim, i, j, k, t2));
+    if (lo_w || hi_w) {
+        const int wi = lo_w ? il : i, wj = lo_w ? jl : j, wk = lo_w ? kl : k;
+        phidiag = 2.0 * w[1 + d] * AT(*prim, wi, wj, wk, 1 + d);
+        phishear = 2.0 * w[t1] * AT(*prim, wi, wj, wk, t1) + 2.0 * w[t2] * AT(*prim, wi, wj, wk, t2);
+    }
     phidiag = -0.5 * phidiag;
     phishear = -0.5 * phishear;
     AT(*fl, i, j, k, nv + 1) = AT(*fl, i, j, k, nv + 1) - phidiag;
     AT(*fl, i, j, k, nv + 2) = AT(*fl, i, j, k, nv + 2) - phishear;
 
     if (P->algorithm_type == 2) {
+        /* at a wall both sides read the ghost cell (lo) or the boundary cell (hi) */
+        const int Li = hi_w ? i : il, Lj = hi_w ? j : jl, Lk = hi_w ? k : kl;
+        const int Ui = lo_w ? il : i, Uj = lo_w ? jl : j, Uk = lo_w ? kl : k;
         double yy[MS], yyp[MS], DijY[MS * MS], sqD[MS * MS], hk[MS];
         for (int n = 5; n < 5 + NS; ++n) w[n] = 0.;
         for (int s = 0; s < NS; ++s) {
-            yy[s] = clamp01(AT(*prim, il, jl, kl, 6 + s));
-            yyp[s] = clamp01(AT(*prim, i, j, k, 6 + s));
+            yy[s] = clamp01(AT(*prim, Li, Lj, Lk, 6 + s));
+            yyp[s] = clamp01(AT(*prim, Ui, Uj, Uk, 6 + s));
         }
         double sumy = 0., sumyp = 0.;
         for (int n = 0; n < NS; ++n) {
@@ -565,10 +743,10 @@ static void stoch_face(const flux_args* A, int d, int i, int j, int k)
         for (int s = 0; s < NS; ++s) {
             MWmix = MWmix + 0.5 * (yy[s] + yyp[s]) / P->molmass[s];
             for (int l = 0; l < NS; ++l)
-                DijY[s * NS + l] = 0.5 * (AT(A->D, il, jl, kl, l * NS + s) * yy[l] +
-                                          AT(A->D, i, j, k, l * NS + s) * yyp[l] +
-                                          (AT(A->D, il, jl, kl, s * NS + l) * yy[s] +
-                                           AT(A->D, i, j, k, s * NS + l) * yyp[s]));
+                DijY[s * NS + l] = 0.5 * (AT(A->D, Li, Lj, Lk, l * NS + s) * yy[l] +
+                                          AT(A->D, Ui, Uj, Uk, l * NS + s) * yyp[l] +
+                                          (AT(A->D, Li, Lj, Lk, s * NS + l) * yy[s] +
+                                           AT(A->D, Ui, Uj, Uk, s * NS + l) * yyp[s]));
         }
         for (int s = 0; s < NS; ++s)
             if (fabs(yy[s]) + fabs(yyp[s]) <= 1.e-12)
@@ -588,16 +766,21 @@ static void stoch_face(const flux_args* A, int d, int i, int j, int k)
         for (int s = 0; s < NS; ++s) hk[s] = P->e0[s] + P->hcp[s] * meanT;
         double soret = 0.;
         for (int s = 0; s < NS; ++s) {
-            double ss = (hk[s] + P->Runiv * meanT / P->molmass[s] * 0.5 *
-                                     (AT(A->chi, il, jl, kl, s) + AT(A->chi, i, j, k, s))) *
-                        w[5 + s];
+            double ss;
+            if (lo_w || hi_w) /* flux.cpp:422-427 */
+                ss = (hk[s] + P->Runiv * meanT / P->molmass[s] * AT(A->chi, Li, Lj, Lk, s)) * w[5 + s];
+            else
+                ss = (hk[s] + P->Runiv * meanT / P->molmass[s] * 0.5 *
+                                  (AT(A->chi, il, jl, kl, s) + AT(A->chi, i, j, k, s))) *
+                     w[5 + s];
             soret = soret + ss;
         }
         AT(*fl, i, j, k, nv + 3) = AT(*fl, i, j, k, nv + 3) + soret;
     }
 }
 
-/* Diffusive flux of one face (flux.cpp:1193-1623, periodic branches). */
+/* Diffusive flux of one face (flux.cpp:1193-1623), with the one-sided branches of a bc_mass
+ * 1/2 wall face. */
 static void diff_face(const flux_args* A, int d, int i, int j, int k)
 {
     const fhdx_params* P = A->P;
@@ -606,8 +789,15 @@ static void diff_face(const flux_args* A, int d, int i, int j, int k)
     const fab_t *prim = &A->prim, *eta = &A->eta, *kappa = &A->kappa, *chi = &A->chi;
     fab_t* fl = (fab_t*)&A->flux[d];
     const double half = 0.5, h = P->dx[d];
+    const int idx[3] = {i, j, k};
+    const int lo_w = idx[d] == 0 && wall_lo(P, d);
+    const int hi_w = idx[d] == P->n_cells[d] && wall_hi(P, d);
+    const int wall = lo_w || hi_w;
+    /* the wall-side cell: the ghost at a lo face, the boundary cell at a hi face */
+    const int wi = lo_w ? il : i, wj = lo_w ? jl : j, wk = lo_w ? kl : k;
 #define PU(n) AT(*prim, i, j, k, n)
 #define PL(n) AT(*prim, il, jl, kl, n)
+#define PW(n) AT(*prim, wi, wj, wk, n)
     double mup = half * (AT(*eta, i, j, k, 0) + AT(*eta, il, jl, kl, 0));
     double kp = half * (AT(*kappa, i, j, k, 0) + AT(*kappa, il, jl, kl, 0));
     double tau[4];
@@ -619,6 +809,15 @@ static void diff_face(const flux_args* A, int d, int i, int j, int k)
     else if (d == 1) phiflx = tau[1] * s1 + tau[2] * s2 + divp * s2 + tau[3] * s3;
     else phiflx = tau[1] * s1 + tau[2] * s2 + tau[3] * s3 + divp * s3;
     double Qflux = kp * (PU(4) - PL(4)) / h;
+    if (wall) { /* flux.cpp:1218-1241 (x), 1361-1384 (y), 1506-1529 (z) */
+        mup = AT(*eta, wi, wj, wk, 0);
+        kp = AT(*kappa, wi, wj, wk, 0);
+        for (int c = 1; c < 4; ++c) tau[c] = mup * (PU(c) - PL(c)) / (0.5 * h);
+        if (d == 0) phiflx = 2.0 * (tau[1] * PW(1) + divp * PW(1) + tau[2] * PW(2) + tau[3] * PW(3));
+        else if (d == 1) phiflx = 2.0 * (tau[1] * PW(1) + tau[2] * PW(2) + divp * PW(2) + tau[3] * PW(3));
+        else phiflx = 2.0 * (tau[1] * PW(1) + tau[2] * PW(2) + tau[3] * PW(3) + divp * PW(3));
+        Qflux = kp * (PU(4) - PL(4)) / (0.5 * h);
+    }
     for (int c = 1; c < 4; ++c) {
         double t = (c == 1 + d) ? (tau[c] + divp) : tau[c];
         AT(*fl, i, j, k, c) = AT(*fl, i, j, k, c) - t;
@@ -628,6 +827,10 @@ static void diff_face(const flux_args* A, int d, int i, int j, int k)
 
     double meanT = 0.5 * (PL(4) + PU(4));
     double meanP = 0.5 * (PL(5) + PU(5));
+    if (wall) {
+        meanT = PW(4);
+        meanP = PW(5);
+    }
     if (P->algorithm_type == 2) {
         double dk[MS], soret[MS], Fk[MS], hk[MS];
         for (int s = 0; s < NS; ++s) {
@@ -644,6 +847,15 @@ static void diff_face(const flux_args* A, int d, int i, int j, int k)
                 ChiX = 0.5 * (AT(*chi, il, jl, kl, s) * PL(6 + NS + s) +
                               AT(*chi, i, j, k, s) * PU(6 + NS + s));
             soret[s] = ChiX * (PU(4) - PL(4)) / h / meanT;
+            if (wall) { /* flux.cpp:1271-1290: half-cell gradients, wall-side values */
+                term1 = (PU(6 + NS + s) - PL(6 + NS + s)) / (0.5 * h);
+                mX = PW(6 + NS + s);
+                mY = PW(6 + s);
+                term2 = (mX - mY) * (PU(5) - PL(5)) / (0.5 * h) / meanP;
+                dk[s] = term1 + term2;
+                ChiX = AT(*chi, wi, wj, wk, s) * PW(6 + NS + s);
+                soret[s] = ChiX * (PU(4) - PL(4)) / (0.5 * h) / meanT;
+            }
         }
         for (int kk = 0; kk < NS; ++kk) {
             Fk[kk] = 0.;
@@ -651,6 +863,7 @@ static void diff_face(const flux_args* A, int d, int i, int j, int k)
                 double Fks = half * (AT(A->D, il, jl, kl, ll * NS + kk) +
                                      AT(A->D, i, j, k, ll * NS + kk)) *
                              (dk[ll] + soret[ll]);
+                if (wall) Fks = AT(A->D, wi, wj, wk, ll * NS + kk) * (dk[ll] + soret[ll]);
                 Fk[kk] = Fk[kk] - Fks;
             }
         }
@@ -661,6 +874,7 @@ static void diff_face(const flux_args* A, int d, int i, int j, int k)
                               ? AT(*chi, i, j, k, s) + AT(*chi, i, j, k, s)
                               : AT(*chi, il, jl, kl, s) + AT(*chi, i, j, k, s);
             double Q5s = (hk[s] + 0.5 * P->Runiv * meanT * csum / P->molmass[s]) * Fk[s];
+            if (wall) Q5s = (hk[s] + P->Runiv * meanT * AT(*chi, wi, wj, wk, s) / P->molmass[s]) * Fk[s];
             Q5 = Q5 + Q5s;
         }
         AT(*fl, i, j, k, nv + 3) = AT(*fl, i, j, k, nv + 3) + Q5;
@@ -668,6 +882,7 @@ static void diff_face(const flux_args* A, int d, int i, int j, int k)
     }
 #undef PU
 #undef PL
+#undef PW
 }
 
 /* Corner (nodal) viscous terms at node (i,j,k) (flux.cpp:1627-1713). */
@@ -688,7 +903,33 @@ static void corner_node(const flux_args* A, int i, int j, int k)
                           Z(i - 1, j, k));
     else
         zetaxp = 0.;
-    const double DX0 = P->dx[0], DX1 = P->dx[1], DX2 = P->dx[2];
+    double DXa[3] = {P->dx[0], P->dx[1], P->dx[2]};
+    /* wall nodes (flux.cpp:1647-1682): half spacing across the wall and the four ghost-layer
+     * cells for the viscosities; the checks run in turn, so the last wall met sets mu */
+    {
+        const int nd[3] = {i, j, k};
+        for (int d = 0; d < 3; ++d)
+            for (int side = 0; side < 2; ++side) {
+                if (!(side ? (nd[d] == P->n_cells[d] && wall_hi(P, d)) : (nd[d] == 0 && wall_lo(P, d))))
+                    continue;
+                DXa[d] = 0.5 * P->dx[d];
+                /* the other two axes, in (i, j, k) order; ghost layer index g on axis d */
+                const int g = side ? nd[d] : nd[d] - 1;
+                const int a = (d == 0) ? 1 : 0, b = (d == 2) ? 1 : 2;
+                double e4 = 0., z4 = 0.;
+                for (int t = 0; t < 4; ++t) {
+                    int c[3];
+                    c[d] = g;
+                    c[a] = nd[a] - 1 + (t >> 1);
+                    c[b] = nd[b] - 1 + (t & 1);
+                    e4 = (t == 0) ? E(c[0], c[1], c[2]) : e4 + E(c[0], c[1], c[2]);
+                    z4 = (t == 0) ? Z(c[0], c[1], c[2]) : z4 + Z(c[0], c[1], c[2]);
+                }
+                muxp = 0.25 * e4;
+                zetaxp = (abs(P->visc_type) == 3) ? 0.25 * z4 : 0.;
+            }
+    }
+    const double DX0 = DXa[0], DX1 = DXa[1], DX2 = DXa[2];
     double gx[4], gy[4], gz[4]; /* index 1..3: u, v, w */
     for (int n = 1; n < 4; ++n) {
         gx[n] = 0.25 * muxp *
@@ -775,10 +1016,19 @@ static void corner_face(const flux_args* A, int d, int i, int j, int k)
     double phiflx = 0.25 * (vc4 - div8) * (AT(*pr, il, jl, kl, cn) + AT(*pr, i, j, k, cn));
     phiflx = phiflx + .25 * s1 * (AT(*pr, il, jl, kl, tc1) + AT(*pr, i, j, k, tc1));
     phiflx = phiflx + .25 * s2 * (AT(*pr, il, jl, kl, tc2) + AT(*pr, i, j, k, tc2));
+    const int idx[3] = {i, j, k};
+    const int lo_w = idx[d] == 0 && wall_lo(A->P, d);
+    const int hi_w = idx[d] == A->P->n_cells[d] && wall_hi(A->P, d);
+    if (lo_w || hi_w) { /* flux.cpp:1791-1822: the wall-side velocities only */
+        const int wi = lo_w ? il : i, wj = lo_w ? jl : j, wk = lo_w ? kl : k;
+        phiflx = 0.5 * (vc4 - div8) * AT(*pr, wi, wj, wk, cn);
+        phiflx = phiflx + .5 * s1 * AT(*pr, wi, wj, wk, tc1);
+        phiflx = phiflx + .5 * s2 * AT(*pr, wi, wj, wk, tc2);
+    }
     AT(*fl, i, j, k, nv + 1) = AT(*fl, i, j, k, nv + 1) - 0.5 * phiflx;
 }
 
-/* Hyperbolic flux of one face (flux.cpp:2016-2372). */
+/* Hyperbolic flux of one face (flux.cpp:2016-2372), with the wall-face branches. */
 static void hyp_face(const flux_args* A, int d, int i, int j, int k)
 {
     const fhdx_params* P = A->P;
@@ -787,12 +1037,20 @@ static void hyp_face(const flux_args* A, int d, int i, int j, int k)
     fab_t* fl = (fab_t*)&A->flux[d];
     const double wgt2 = 1. / 12.;
     const double wgt1 = 0.5 + wgt2;
+    /* boundary interpolation weights (flux.cpp:1991-1997) */
+    const double wgta = -0.2, wgtb = 0.75, wgtc = 0.5, wgtd = -0.05;
+    const int idx[3] = {i, j, k}, n = P->n_cells[d];
+    const int wl = wall_lo(P, d), wh = wall_hi(P, d);
     double cv[MS + 5], pv[MS + 6], Yk[MS];
     if (P->advection_type == 1) {
         for (int l = 0; l < NS + 6; ++l)
             pv[l] = wgt1 * (AT(A->prim, i, j, k, l) + AT(A->prim, i - ex, j - ey, k - ez, l)) -
                     wgt2 * (AT(A->prim, i - 2 * ex, j - 2 * ey, k - 2 * ez, l) +
                             AT(A->prim, i + ex, j + ey, k + ez, l));
+        if (idx[d] == 0 && wl) /* flux.cpp:2028-2037: the wall face takes the ghost value */
+            for (int l = 0; l < NS + 6; ++l) pv[l] = AT(A->prim, i - ex, j - ey, k - ez, l);
+        if (idx[d] == n && wh)
+            for (int l = 0; l < NS + 6; ++l) pv[l] = AT(A->prim, i, j, k, l);
         double temp = pv[4], rho = pv[0];
         cv[0] = rho;
         for (int n = 0; n < NS; ++n) Yk[n] = pv[6 + n];
@@ -804,6 +1062,20 @@ static void hyp_face(const flux_args* A, int d, int i, int j, int k)
             cv[l] = wgt1 * (AT(A->cons, i, j, k, l) + AT(A->cons, i - ex, j - ey, k - ez, l)) -
                     wgt2 * (AT(A->cons, i - 2 * ex, j - 2 * ey, k - 2 * ez, l) +
                             AT(A->cons, i + ex, j + ey, k + ez, l));
+        /* flux.cpp:2190-2206: the wall face takes the ghost value, the next face in a one-sided
+         * four-point interpolation */
+#define CS(o, l) AT(A->cons, i + (o) * ex, j + (o) * ey, k + (o) * ez, l)
+        if (idx[d] == 0 && wl)
+            for (int l = 0; l < NS + 5; ++l) cv[l] = CS(-1, l);
+        else if (idx[d] == 1 && wl)
+            for (int l = 0; l < NS + 5; ++l)
+                cv[l] = wgta * CS(-2, l) + wgtb * CS(-1, l) + wgtc * CS(0, l) + wgtd * CS(1, l);
+        if (idx[d] == n && wh)
+            for (int l = 0; l < NS + 5; ++l) cv[l] = CS(0, l);
+        else if (idx[d] == n - 1 && wh)
+            for (int l = 0; l < NS + 5; ++l)
+                cv[l] = wgta * CS(1, l) + wgtb * CS(0, l) + wgtc * CS(-1, l) + wgtd * CS(-2, l);
+#undef CS
         for (int l = 1; l < 4; ++l) pv[l] = cv[l] / cv[0];
         for (int n = 0; n < NS; ++n) Yk[n] = cv[5 + n] / cv[0];
         double vsqr = pv[1] * pv[1] + pv[2] * pv[2] + pv[3] * pv[3];
@@ -835,6 +1107,87 @@ static void hyp_face(const flux_args* A, int d, int i, int j, int k)
         else
             for (int n = 0; n < NS; ++n) AT(*fl, i, j, k, 5 + n) += cv[5 + n] * un;
     }
+}
+
+/* Visits every face of the wall plane (d, side): index 0 (lo) or n_d (hi) along d. */
+#define FOR_WALL_FACES(P, d, side, BODY)                                                     \
+    do {                                                                                     \
+        const int n0_ = (P)->n_cells[0], n1_ = (P)->n_cells[1], n2_ = (P)->n_cells[2];       \
+        const int fix_ = (side) ? (P)->n_cells[d] : 0;                                       \
+        for (int k = ((d) == 2 ? fix_ : 0); k <= ((d) == 2 ? fix_ : n2_ - 1); ++k)           \
+            for (int j = ((d) == 1 ? fix_ : 0); j <= ((d) == 1 ? fix_ : n1_ - 1); ++j)       \
+                for (int i = ((d) == 0 ? fix_ : 0); i <= ((d) == 0 ? fix_ : n0_ - 1); ++i) { \
+                    BODY;                                                                    \
+                }                                                                            \
+    } while (0)
+
+/* StochFlux (boundary.cpp:1392-1970): on wall faces the stochastic fluxes are scaled.  Mass
+ * walls: species and Dufour by 0 (bc_mass 1) or sqrt(2) (bc_mass 2); thermal walls: the heat
+ * flux by 0 (adiabatic) or sqrt(2) (isothermal); velocity walls: normal momentum and the
+ * diagonal viscous heating by sqrt(2), tangential momentum and the shear heating by 0 (slip)
+ * or sqrt(2) (no slip). */
+static void stoch_flux_walls(const flux_args* A)
+{
+    const fhdx_params* P = A->P;
+    const int NS = A->NS, nv = A->nv;
+    for (int d = 0; d < 3; ++d)
+        for (int side = 0; side < 2; ++side) {
+            const int bm = side ? P->bc_mass_hi[d] : P->bc_mass_lo[d];
+            if (bm != 1 && bm != 2) continue;
+            const double factor = (bm == 1) ? 0. : sqrt(2.);
+            fab_t* fl = (fab_t*)&A->flux[d];
+            FOR_WALL_FACES(P, d, side, {
+                for (int n = 0; n < NS; ++n) AT(*fl, i, j, k, n + 5) *= factor;
+                AT(*fl, i, j, k, nv + 3) *= factor;
+            });
+        }
+    for (int d = 0; d < 3; ++d)
+        for (int side = 0; side < 2; ++side) {
+            const int bt = side ? P->bc_therm_hi[d] : P->bc_therm_lo[d];
+            if (bt != 1 && bt != 2) continue;
+            double factor = (bt == 1) ? 0. : sqrt(2.);
+            /* a reservoir leaves the heat flux alone, written for x faces only (:1593, :1620) */
+            if (d == 0 && (side ? P->bc_mass_hi[0] : P->bc_mass_lo[0]) == 3) factor = 1.0;
+            fab_t* fl = (fab_t*)&A->flux[d];
+            FOR_WALL_FACES(P, d, side, { AT(*fl, i, j, k, nv) *= factor; });
+        }
+    const double sqrtTwo = sqrt(2.);
+    for (int d = 0; d < 3; ++d)
+        for (int side = 0; side < 2; ++side) {
+            const int bv = side ? P->bc_vel_hi[d] : P->bc_vel_lo[d];
+            if (bv != 1 && bv != 2) continue;
+            const double factor = (bv == 1) ? 0. : sqrt(2.);
+            fab_t* fl = (fab_t*)&A->flux[d];
+            FOR_WALL_FACES(P, d, side, {
+                for (int n = 0; n < 3; ++n) {
+                    if (n == d) {
+                        AT(*fl, i, j, k, 1 + n) *= sqrtTwo;
+                        AT(*fl, i, j, k, nv + 1) *= sqrtTwo;
+                        AT(*fl, i, j, k, nv + 2) *= factor;
+                    } else {
+                        AT(*fl, i, j, k, 1 + n) *= factor;
+                    }
+                }
+            });
+        }
+}
+
+/* BCWallSpeciesFlux (boundary.cpp:1212-1390): species, density and Dufour fluxes are zero on
+ * bc_mass 1 faces. */
+static void wall_species_flux(const flux_args* A)
+{
+    const fhdx_params* P = A->P;
+    const int NS = A->NS, nv = A->nv;
+    for (int d = 0; d < 3; ++d)
+        for (int side = 0; side < 2; ++side) {
+            if ((side ? P->bc_mass_hi[d] : P->bc_mass_lo[d]) != 1) continue;
+            fab_t* fl = (fab_t*)&A->flux[d];
+            FOR_WALL_FACES(P, d, side, {
+                for (int n = 0; n < NS; ++n) AT(*fl, i, j, k, n + 5) = 0.;
+                AT(*fl, i, j, k, 0) = 0.;
+                AT(*fl, i, j, k, nv + 3) = 0.;
+            });
+        }
 }
 
 int orc_calculate_flux(const fhdx_params* P, fhdx_fab cons, fhdx_fab prim, fhdx_fab eta,
@@ -877,6 +1230,7 @@ int orc_calculate_flux(const fhdx_params* P, fhdx_fab cons, fhdx_fab prim, fhdx_
                 for (int j = 0; j < ky; ++j)
                     for (int i = 0; i < kx; ++i) stoch_face(&A, d, i, j, k);
         }
+    if (P->stoch_stress_form == 1) stoch_flux_walls(&A); /* flux.cpp:1149 */
     for (int d = 0; d < 3; ++d) {
         const int kx = n0 + (d == 0), ky = n1 + (d == 1), kz = n2 + (d == 2);
 #pragma omp parallel for schedule(static)
@@ -895,6 +1249,7 @@ int orc_calculate_flux(const fhdx_params* P, fhdx_fab cons, fhdx_fab prim, fhdx_
             for (int j = 0; j < ky; ++j)
                 for (int i = 0; i < kx; ++i) corner_face(&A, d, i, j, k);
     }
+    wall_species_flux(&A); /* flux.cpp:1985 */
     if (P->advection_type == 1 || P->advection_type == 2)
         for (int d = 0; d < 3; ++d) {
             const int kx = n0 + (d == 0), ky = n1 + (d == 1), kz = n2 + (d == 2);
@@ -1011,6 +1366,7 @@ static int rk3_step_impl(const fhdx_params* P, const fhdx_chemistry* chem, fhdx_
         orc_cons_to_prim(P, prim, outs[s]);
         orc_fill_boundary(P, outs[s]);
         orc_fill_boundary(P, prim);
+        orc_set_bc(P, prim, outs[s]); /* timeStep.cpp:209, 314, 425 */
     }
     return err;
 }
@@ -1173,7 +1529,10 @@ int orc_fill_noise(const fhdx_params* P, uint64_t seed, uint64_t step, uint32_t 
                     for (int i = 0; i <= a.hi0; ++i) {
                         uint32_t lin = (uint32_t)(((long)(k % n[2]) * n[1] + (j % n[1])) * n[0] +
                                                   (i % n[0]));
-                        uint32_t ctr[4] = {lin, step_lo, field | (step_hi << 8), instance};
+                        /* a wall direction's face n is its own face, with its own stream */
+                        const int fi = (d == 0) ? i : (d == 1) ? j : k;
+                        const uint32_t fld = (fi == n[d] && !is_periodic(P, d)) ? field + 128u : field;
+                        uint32_t ctr[4] = {lin, step_lo, fld | (step_hi << 8), instance};
                         uint32_t r[4];
                         double z0, z1;
                         orc_philox4x32_10(ctr, key, r);
@@ -1191,7 +1550,12 @@ int orc_fill_noise(const fhdx_params* P, uint64_t seed, uint64_t step, uint32_t 
             for (int i = 0; i <= a.hi0; ++i) {
                 uint32_t lin =
                     (uint32_t)(((long)(k % n[2]) * n[1] + (j % n[1])) * n[0] + (i % n[0]));
-                uint32_t ctr[4] = {lin, step_lo, 48u | (step_hi << 8), instance};
+                /* nodes on the hi plane of a wall direction are distinct: field 48 + mask */
+                const int nd[3] = {i, j, k};
+                uint32_t mask = 0;
+                for (int d = 0; d < 3; ++d)
+                    if (nd[d] == n[d] && !is_periodic(P, d)) mask |= 1u << d;
+                uint32_t ctr[4] = {lin, step_lo, (48u + mask) | (step_hi << 8), instance};
                 uint32_t r[4];
                 double z0, z1;
                 orc_philox4x32_10(ctr, key, r);
@@ -1586,6 +1950,7 @@ static int advance_impl(const fhdx_params* P, const fhdx_chemistry* chem, fhdx_f
     orc_fill_boundary(P, cu);
     orc_cons_to_prim(P, prim, cu);
     orc_fill_boundary(P, prim);
+    orc_set_bc(P, prim, cu); /* main_driver.cpp:535-542 */
     int err = FHDX_OK;
     const long per = orc_noise_doubles_per_step(P);
     for (int s = 0; s < nsteps; ++s) {

@@ -7,8 +7,9 @@ from fhdex_amd.params import Params
 
 
 def perturbed_state(p: Params, cfg, orc, seed=0, amp=1e-2, vel=2e3):
-    """A periodic conserved state near the uniform start of `cfg`: density, velocity,
-    temperature and composition perturbed cell by cell (all mass fractions stay in (0,1))."""
+    """A conserved state near the uniform start of `cfg`: density, velocity, temperature and
+    composition perturbed cell by cell (all mass fractions stay in (0,1)); ghost cells are
+    periodic images, or set by setBC outside walls."""
     rng = np.random.default_rng(seed)
     st = orc.HostState(p)
     orc.init_cons(p, st.cu, cfg.rho0, cfg.rhobar, cfg.T_init)
@@ -33,6 +34,8 @@ def perturbed_state(p: Params, cfg, orc, seed=0, amp=1e-2, vel=2e3):
     orc.fill_boundary(p, st.cu)
     orc.cons_to_prim(p, st.prim, st.cu)
     orc.fill_boundary(p, st.prim)
+    if p.has_walls:
+        orc.set_bc(p, st.prim, st.cu)
     return st
 
 

@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_status_strings():
     lib = L.lib()
-    assert lib.fhdx_abi_version() == 1
+    assert lib.fhdx_abi_version() == 2
     assert lib.fhdx_status_string(0) == b"ok"
     assert b"mass fraction" in lib.fhdx_status_string(4)
 
@@ -98,8 +98,26 @@ def test_params_default_finalize_check():
     assert lib.fhdx_params_check(C.byref(p)) == 0
     q = p.copy()
     q.bc_vel_lo[0] = 1
-    assert lib.fhdx_params_check(C.byref(q)) == 2          # walls are out of scope
+    assert lib.fhdx_params_check(C.byref(q)) == 1          # one side periodic, the other not
     assert b"periodic" in lib.fhdx_last_error()
+    q = p.copy()
+    P.set_walls(q, [(1, 1), (-1, -1), (-1, -1)], 1, 1)
+    assert lib.fhdx_params_check(C.byref(q)) == 0          # slip adiabatic species walls in x
+    r = q.copy()
+    r.bc_mass_hi[0] = 3
+    assert lib.fhdx_params_check(C.byref(r)) == 2          # reservoirs are not implemented
+    assert b"reservoir" in lib.fhdx_last_error()
+    r = q.copy()
+    r.bc_therm_lo[0] = 2                                   # isothermal without a wall temperature
+    assert lib.fhdx_params_check(C.byref(r)) == 1
+    r = q.copy()
+    P.set_walls(r, 2, 2, 2, t_lo=(390.0,) * 3, t_hi=(410.0,) * 3, bc_Yk=[0.6, 0.4])
+    assert lib.fhdx_params_check(C.byref(r)) == 0
+    r.bc_Xk[3][1] = 0.0                                    # the finalize step fills X from Y
+    lib.fhdx_params_finalize(C.byref(r))
+    assert r.bc_Xk[3][1] == pytest.approx(0.4) and lib.fhdx_params_check(C.byref(r)) == 0
+    r.algorithm_type = 1
+    assert lib.fhdx_params_check(C.byref(r)) == 2          # walls need the species ghosts
     q = p.copy()
     q.nspecies = 1
     assert lib.fhdx_params_check(C.byref(q)) == 2

@@ -1,0 +1,181 @@
+"""Walls on the GPU against the oracle, through the C ABI: setBC ghosts, the wall-aware noise,
+calculateFlux (one-sided face branches, StochFlux, BCWallSpeciesFlux) with injected noise, and
+RK3 steps of the reference-shaped path with Philox noise, on the reference's wall regression
+decks (exec/compressible/inputs_regression_{slip_wall_adiabatic, noslip_reservoir_isothermal,
+temp_grad}) at reduced size, a bulk-viscosity variant and a box with walls in x only.  The GPU
+reads the wall-side cell for both sides of a wall face where the oracle spells out the doubled
+terms; every comparison is exact.  The fused step refuses walls with FHDX_ERR_UNSUPPORTED.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from fhdex_amd import params as P
+from tests.helpers import perturbed_state, valid
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def x_walls():
+    base = P.species_prefix(3, 10, 1)
+    p = P.make_params(3, (10, 8, 9), 8e-6, [4.0026, 20.1797, 39.9480], [2.18e-8, 2.58e-8, 3.63e-8])
+    P.set_walls(p, [(1, 2), (-1, -1), (-1, -1)], [(2, 1), (-1, -1), (-1, -1)], [(2, 1), (-1, -1), (-1, -1)],
+                t_lo=(310.0, 0.0, 0.0), bc_Yk=[0.5, 0.3, 0.2])
+    return P.Config("x_walls", p, base.rho0, base.rhobar, base.T_init, base.dt, 1)
+
+
+def visc3_walls():
+    cfg = P.wall_deck("slip_wall_adiabatic", 8)
+    cfg.params.visc_type = 3
+    cfg.params.zeta_ratio = 1.5
+    return cfg
+
+
+CASES = [
+    ("slip_wall_adiabatic", lambda: P.wall_deck("slip_wall_adiabatic", 8)),
+    ("noslip_reservoir_isothermal", lambda: P.wall_deck("noslip_reservoir_isothermal", 8)),
+    ("temp_grad", lambda: P.wall_deck("temp_grad", 16)),
+    ("visc_type_3", visc3_walls),
+    ("x_walls", x_walls),
+]
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    from fhdex_amd import compressible as fc
+    from fhdex_amd import lib as L
+    L.lib()
+    return torch, fc, L
+
+
+def to_dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+@pytest.mark.parametrize("name,mk", CASES)
+def test_set_bc_and_noise_match_oracle(dev, orc, name, mk):
+    torch, fc, L = dev
+    cfg = mk()
+    p = cfg.params
+    g = p.ngc
+    say(f"case walls setBC + noise {name} {tuple(p.n_cells)}")
+    st = perturbed_state(p, cfg, orc, seed=3, amp=0.02, vel=300.0)
+    ctx = L.Context(p)
+
+    def scrambled(a):  # valid cells kept, every ghost cell garbage
+        b = np.full_like(a, 7.0)
+        b[:, g:-g, g:-g, g:-g] = a[:, g:-g, g:-g, g:-g]
+        return b
+
+    prim = fc.MultiFab(to_dev(torch, scrambled(st.prim)), (-g,) * 3)
+    cu = fc.MultiFab(to_dev(torch, scrambled(st.cu)), (-g,) * 3)
+    # rebuild the ghosts: periodic fill, then setBC (timeStep.cpp:207-209)
+    fc.FillBoundary(ctx, cu)
+    fc.FillBoundary(ctx, prim)
+    fc.setBC(ctx, prim, cu)
+    torch.cuda.synchronize()
+    assert np.array_equal(prim.t.cpu().numpy(), st.prim)
+    assert np.array_equal(cu.t.cpu().numpy(), st.cu)
+    ns = fc.NoiseSets(p)
+    fc.fill_noise(ctx, ns, 7, 3)
+    orc.fill_noise(p, st, 7, 3)
+    for d in range(3):
+        assert np.array_equal(ns.A[d].t.cpu().numpy(), st.sA[d]), d
+        assert np.array_equal(ns.B[d].t.cpu().numpy(), st.sB[d]), d
+    assert np.array_equal(ns.rA.t.cpu().numpy(), st.rA) and np.array_equal(ns.rB.t.cpu().numpy(), st.rB)
+    # faces 0 and n of a wall direction draw different values
+    for d in range(3):
+        if not p.periodic[d]:
+            a = st.sA[d]
+            lo, hi = np.take(a, 0, axis=3 - d), np.take(a, -1, axis=3 - d)
+            assert not np.array_equal(lo[1:], hi[1:])
+
+
+@pytest.mark.parametrize("name,mk", CASES)
+def test_calculate_flux_matches_oracle(dev, orc, name, mk):
+    torch, fc, L = dev
+    cfg = mk()
+    p = cfg.params
+    say(f"case walls calculateFlux {name} {tuple(p.n_cells)} adv={p.advection_type} visc={p.visc_type}")
+    st = perturbed_state(p, cfg, orc, seed=5, amp=0.02, vel=300.0)
+    assert orc.transport(p, st) == 0
+    orc.fill_noise(p, st, 5, 2)
+    w = 0.41
+    for d in range(3):
+        st.stoch[d][:] = st.sA[d] + w * st.sB[d]
+        st.stoch[d][0] = 0
+    st.rancorn[:] = st.rA + w * st.rB
+    ctx = L.Context(p)
+    ds = fc.DeviceState(p)
+    for nm in ("cu", "prim", "eta", "zeta", "kappa", "chi", "D"):
+        getattr(ds, nm).t.copy_(torch.from_numpy(getattr(st, nm)))
+    for d in range(3):
+        ds.stochFlux[d].t.copy_(torch.from_numpy(st.stoch[d]))
+    ds.rancorn.t.copy_(torch.from_numpy(st.rancorn))
+    fc.calculateFlux(ctx, ds.cu, ds.prim, ds.eta, ds.zeta, ds.kappa, ds.chi, ds.D, ds.flux,
+                     ds.stochFlux, ds.cornx, ds.corny, ds.cornz, ds.visccorn, ds.rancorn, cfg.dt)
+    torch.cuda.synchronize()
+    assert orc.calculate_flux(p, st, st.cu, cfg.dt) == 0
+    for d in range(3):
+        got = ds.flux[d].t.cpu().numpy()
+        same = np.array_equal(got, st.flux[d])
+        if not same:
+            bad = np.argwhere(got != st.flux[d])
+            say(f"  flux[{d}] differs at {len(bad)} entries, first {bad[:4].tolist()}")
+        assert same, d
+    for a in range(3):
+        for b, arr in enumerate((ds.cornx, ds.corny, ds.cornz)[a]):
+            assert np.array_equal(arr.t.cpu().numpy(), st.corn[a][b]), (a, b)
+    assert np.array_equal(ds.visccorn.t.cpu().numpy(), st.visccorn)
+
+
+@pytest.mark.parametrize("name,mk", CASES)
+def test_rk3_steps_match_oracle(dev, orc, name, mk):
+    """Three reference-shaped steps with Philox noise (setBC after every stage)."""
+    torch, fc, L = dev
+    cfg = mk()
+    p = cfg.params
+    say(f"case walls RK3step x3 {name} {tuple(p.n_cells)}")
+    st = perturbed_state(p, cfg, orc, seed=9, amp=0.02, vel=300.0)
+    ctx = L.Context(p)
+    ds = fc.DeviceState(p)
+    ds.cu.t.copy_(torch.from_numpy(st.cu))
+    ds.prim.t.copy_(torch.from_numpy(st.prim))
+    for step in range(3):
+        fc.RK3step(ctx, ds, cfg.dt, noise=None, seed=21, step=step)
+    ctx.check()
+    assert orc.advance(p, st.cu, st.prim, 3, cfg.dt, seed=21, step0=0) == 0
+    got_cu, got_prim = ds.cu.t.cpu().numpy(), ds.prim.t.cpu().numpy()
+    say(f"  cu exact={np.array_equal(got_cu, st.cu)} prim exact={np.array_equal(got_prim, st.prim)}")
+    assert np.array_equal(got_cu, st.cu)            # ghosts included: setBC after the last stage
+    assert np.array_equal(got_prim, st.prim)
+
+
+def test_fused_paths_refuse_walls(dev, orc):
+    torch, fc, L = dev
+    cfg = P.wall_deck("slip_wall_adiabatic", 8)
+    p = cfg.params
+    st = perturbed_state(p, cfg, orc, seed=1, amp=0.01, vel=100.0)
+    ctx = L.Context(p)
+    cu = fc.MultiFab(to_dev(torch, st.cu), (-p.ngc,) * 3)
+    prim = fc.MultiFab.cell(p, p.nprimvars)
+    with pytest.raises(L.FhdxError) as e:
+        fc.rk3_step(ctx, cu, prim, cfg.dt, 1, 0)
+    assert e.value.status == 2
+    with pytest.raises(L.FhdxError) as e:
+        fc.rk3_step_injected(ctx, cu, prim, fc.NoiseSets(p), cfg.dt)
+    assert e.value.status == 2
+    from fhdex_amd import ensemble as E
+    ens = E.Ensemble(2, 1 << 30)
+    with pytest.raises(L.FhdxError) as e:
+        ens.add(p, cfg.dt, 1, st.cu)
+    assert e.value.status == 2
+    ens.close()
