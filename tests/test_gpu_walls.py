@@ -179,3 +179,67 @@ def test_fused_paths_refuse_walls(dev, orc):
         ens.add(p, cfg.dt, 1, st.cu)
     assert e.value.status == 2
     ens.close()
+
+
+@pytest.mark.parametrize("family", ["slip", "noslip"])
+def test_wall_layers_fluctuate_like_the_bulk(dev, family):
+    """Argon (two identical species) at 8e-5 cm cells between x walls, periodic y and z,
+    primitive interpolation, reference-shaped noisy steps: the variances of rho, j and rho*E
+    in the two cell layers touching the walls equal fluctuation theory (rho m/dV,
+    rho k_B T/dV, e^2 rho m/dV + rho c_v k_B T^2/dV) as in the bulk.  This is what the sqrt(2)
+    and 0 factors of StochFlux are for.  (With conserved interpolation the one-sided weights
+    -0.2/0.75/0.5/-0.05 at the face next to a wall lower the normal-momentum variance of the
+    wall layer to about 0.85 of theory, a property of the reference's scheme; it is printed,
+    not asserted.)"""
+    torch, fc, L = dev
+    base = P.argon_gas(8)
+    for adv in (1, 2):
+        p = P.make_params(2, (16, 16, 16), 8e-5, [39.948] * 2, [3.66e-8] * 2, advection_type=adv)
+        if family == "slip":
+            P.set_walls(p, [(1, 1), (-1, -1), (-1, -1)], [(1, 1), (-1, -1), (-1, -1)], [(1, 1), (-1, -1), (-1, -1)])
+        else:
+            P.set_walls(p, [(2, 2), (-1, -1), (-1, -1)], [(2, 2), (-1, -1), (-1, -1)], [(1, 1), (-1, -1), (-1, -1)],
+                        t_lo=(base.T_init,) * 3, t_hi=(base.T_init,) * 3)
+        cfg = P.Config("argon_walls", p, base.rho0, base.rhobar, base.T_init,
+                       P.cfl_dt(p, base.rhobar, base.T_init, 0.1), 1)
+        from fhdex_amd import ensemble as E
+        say(f"case wall-layer equilibrium {family} advection_type {adv} {tuple(p.n_cells)}")
+        ctx = L.Context(p)
+        ds = fc.DeviceState(p)
+        ds.cu.t.copy_(torch.from_numpy(E.uniform_cons(cfg)))
+        fc.conservedToPrimitive(ctx, ds.prim, ds.cu)
+        fc.FillBoundary(ctx, ds.cu)
+        fc.FillBoundary(ctx, ds.prim)
+        fc.setBC(ctx, ds.prim, ds.cu)
+        g = p.ngc
+        step = 0
+        for _ in range(1000):
+            fc.RK3step(ctx, ds, cfg.dt, noise=None, seed=13, step=step)
+            step += 1
+        acc = torch.zeros((5, 16), dtype=torch.float64, device="cuda")
+        tsum = 0.0
+        nsnap = 300
+        for _ in range(nsnap):
+            for _ in range(10):
+                fc.RK3step(ctx, ds, cfg.dt, noise=None, seed=13, step=step)
+                step += 1
+            v = ds.cu.t[:5, g:-g, g:-g, g:-g]
+            m = v.mean(dim=(1, 2, 3), keepdim=True)
+            acc += ((v - m) ** 2).mean(dim=(1, 2))
+            tsum += float(ds.prim.t[4, g:-g, g:-g, g:-g].mean())
+        ctx.check()
+        var = (acc / nsnap).cpu().numpy()
+        T = tsum / nsnap
+        kB, dV, rho = p.k_B, p.dx[0] ** 3, cfg.rho0
+        mm = p.molmass[0] * kB / p.Runiv
+        cv = p.hcv[0]
+        th = np.array([rho * mm / dV] + [rho * kB * T / dV] * 3 +
+                      [(cv * T) ** 2 * rho * mm / dV + rho * cv * kB * T * T / dV])
+        ratio = var / th[:, None]
+        for c, nm in enumerate(["rho", "jx", "jy", "jz", "rhoE"]):
+            say(f"  {nm:5s} " + " ".join(f"{x:.3f}" for x in ratio[c]))
+        bulk = ratio[:, 3:-3].mean(axis=1)
+        assert np.all(np.abs(bulk - 1) < 0.04), bulk
+        if adv == 1:
+            walls = ratio[:, [0, -1]]
+            assert np.all(np.abs(walls - 1) < 0.08), walls
