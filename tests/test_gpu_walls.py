@@ -243,3 +243,82 @@ def test_wall_layers_fluctuate_like_the_bulk(dev, family):
         if adv == 1:
             walls = ratio[:, [0, -1]]
             assert np.all(np.abs(walls - 1) < 0.08), walls
+
+
+@pytest.mark.parametrize("name", ["slip_wall_adiabatic", "noslip_reservoir_isothermal", "temp_grad"])
+def test_wall_decks_at_full_size_match_oracle(dev, orc, name):
+    """The regression decks at their own sizes (32^3; 80x10x10) and step counts (max_step 3),
+    from the uniform start (prob_type 1), Philox noise: GPU equals the oracle bit for bit, and
+    closed species walls keep every species total."""
+    torch, fc, L = dev
+    from fhdex_amd import ensemble as E
+    cfg = P.wall_deck(name)
+    p = cfg.params
+    g = p.ngc
+    say(f"case wall deck {name} {tuple(p.n_cells)} x {cfg.nsteps}")
+    cu0 = E.uniform_cons(cfg)
+    ctx = L.Context(p)
+    ds = fc.DeviceState(p)
+    ds.cu.t.copy_(torch.from_numpy(cu0))
+    fc.conservedToPrimitive(ctx, ds.prim, ds.cu)       # main_driver.cpp:535-542
+    fc.FillBoundary(ctx, ds.cu)
+    fc.FillBoundary(ctx, ds.prim)
+    fc.setBC(ctx, ds.prim, ds.cu)
+    for step in range(cfg.nsteps):
+        fc.RK3step(ctx, ds, cfg.dt, noise=None, seed=1, step=step)
+    ctx.check()
+    cu = cu0.copy()
+    prim = np.zeros((p.nprimvars,) + cu.shape[1:])
+    assert orc.advance(p, cu, prim, cfg.nsteps, cfg.dt, seed=1, step0=0) == 0
+    got = ds.cu.t.cpu().numpy()
+    say(f"  exact={np.array_equal(got, cu)}")
+    assert np.array_equal(got, cu) and np.array_equal(ds.prim.t.cpu().numpy(), prim)
+    s0, s1 = valid(p, cu0).sum(axis=(1, 2, 3)), valid(p, got).sum(axis=(1, 2, 3))
+    assert abs(s1[0] - s0[0]) <= 1e-14 * s0[0]
+    if p.bc_mass_lo[0] == 1:
+        for s in range(p.nspecies):
+            assert abs(s1[5 + s] - s0[5 + s]) <= 1e-14 * s0[5 + s]
+
+
+def test_temp_grad_reaches_steady_conduction(dev):
+    """temp_grad without noise on a 20-cell gap (x: no-slip walls at 280.8 K and 508.5 K; y, z:
+    slip adiabatic walls), run to steady state (12000 steps of 1e-12 s, ~25 thermal and 15
+    acoustic relaxation times).  At rest the heat flux through every x face is the same, T rises
+    monotonically, and the wall faces' half-cell gradients place the wall temperatures on the
+    profile: the linear extrapolation of the two cells next to each wall to the wall plane
+    lands within 0.5 K of t_lo / t_hi (a full-cell gradient there would miss by half a cell's
+    temperature step, ~5.7 K)."""
+    torch, fc, L = dev
+    from fhdex_amd import ensemble as E
+    cfg = P.wall_deck("temp_grad", 20)
+    p = cfg.params
+    p.stoch_stress_form = 0
+    g = p.ngc
+    nv = p.nvars
+    say(f"case temp_grad steady conduction {tuple(p.n_cells)}")
+    ctx = L.Context(p)
+    ds = fc.DeviceState(p)
+    ds.cu.t.copy_(torch.from_numpy(E.uniform_cons(cfg)))
+    fc.conservedToPrimitive(ctx, ds.prim, ds.cu)
+    fc.FillBoundary(ctx, ds.cu)
+    fc.FillBoundary(ctx, ds.prim)
+    fc.setBC(ctx, ds.prim, ds.cu)
+    nsteps = 12000
+    for step in range(nsteps):
+        fc.RK3step(ctx, ds, cfg.dt, noise=None, seed=1, step=step)
+        if step == nsteps - 2:
+            before = ds.cu.t.clone()
+    ctx.check()
+    T = ds.prim.t[4, g:-g, g:-g, g:-g].mean(dim=(0, 1)).cpu().numpy()
+    q = -ds.flux[0].t[nv].mean(dim=(0, 1)).cpu().numpy()       # kappa dT/dx on the x faces
+    u = ds.prim.t[1:4, g:-g, g:-g, g:-g].abs().max().item()
+    t_lo = T[0] - 0.5 * (T[1] - T[0])
+    t_hi = T[-1] + 0.5 * (T[-1] - T[-2])
+    change = (ds.cu.t - before).abs().max().item() / ds.cu.t.abs().max().item()
+    say(f"  T: {T[0]:.3f} .. {T[-1]:.3f} K, extrapolated to the walls {t_lo:.3f} / {t_hi:.3f} K")
+    say(f"  q(x faces) {q.min():.6e} .. {q.max():.6e}; max |u| {u:.2e} cm/s; last-step change {change:.2e}")
+    assert np.all(np.diff(T) > 0)
+    assert q.min() > 0 and (q.max() - q.min()) <= 1e-3 * q.max()
+    assert abs(t_lo - 280.8) < 0.5 and abs(t_hi - 508.5) < 0.5
+    # at rest up to a residual acoustic ring-down far below the sound speed (3.7e4 cm/s)
+    assert u < 5.0 and change < 1e-6
