@@ -95,6 +95,50 @@ def config_rate(cfg, steps: int, warmup: int = 2):
             "launch_shape": best.as_dict()}
 
 
+def wall_rate(cfg, reps: int = 20):
+    """cell-updates/s of the reference-shaped step (the path that serves walls) on a wall deck,
+    run as the reference runs it: max_step (3) noisy steps from the uniform start with ghosts
+    set by setBC, repeated `reps` times with seeds 1.. ; only the steps are timed.  The decks
+    hold ~50 molecules per species per cell, so now and then a stage leaves a mass fraction
+    outside (0,1), where the reference aborts (transCoeffs.cpp:45-46): such runs are counted
+    and reported, not raised."""
+    import torch
+    from fhdex_amd import compressible as fc
+    from fhdex_amd import ensemble as E
+    from fhdex_amd import lib as L
+    p = cfg.params
+    ctx = L.Context(p)
+    ds = fc.DeviceState(p)
+    cu0 = torch.from_numpy(E.uniform_cons(cfg)).cuda()
+    total = 0.0
+    flagged = 0
+    for r in range(reps + 1):
+        ds.cu.t.copy_(cu0)
+        fc.conservedToPrimitive(ctx, ds.prim, ds.cu)
+        fc.FillBoundary(ctx, ds.cu)
+        fc.FillBoundary(ctx, ds.prim)
+        fc.setBC(ctx, ds.prim, ds.cu)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for s in range(cfg.nsteps):
+            fc.RK3step(ctx, ds, cfg.dt, noise=None, seed=1 + r, step=s)
+        torch.cuda.synchronize()
+        if r > 0:  # the first run is warm-up
+            total += time.perf_counter() - t0
+        try:
+            ctx.check()
+        except L.FhdxError as e:
+            if e.status != 4:  # FHDX_ERR_MASSFRAC: a stage left Y in (0,1), where the reference aborts
+                raise
+            flagged += 1
+    ctx.close()
+    steps = reps * cfg.nsteps
+    cells = p.n_cells[0] * p.n_cells[1] * p.n_cells[2]
+    return {"workload": cfg.name + " (walls, reference-shaped step)", "value": cells * steps / total,
+            "unit": "cell-updates/s", "ms_per_step": total / steps * 1e3, "steps": steps, "nvars": p.nvars,
+            "runs": reps + 1, "runs_with_massfrac_flag": flagged}
+
+
 def measured_flops(n: int):
     """FP64 flops per cell-update from the committed rocprofv3 counter summary of this workload
     (profiles/fp64_argon<n>.json), or None."""
@@ -378,7 +422,9 @@ def main():
     del cu, prim
     if not a.no_configs:
         line["configs"] = [config_rate(P.argon_gas(32), 50), config_rate(P.four_species(128), 10),
-                           config_rate(P.no2_n2o4(128), 10)]
+                           config_rate(P.no2_n2o4(128), 10),
+                           wall_rate(P.wall_deck("slip_wall_adiabatic")),
+                           wall_rate(P.wall_deck("temp_grad"))]
     if a.ensemble:
         line["ensemble"] = ensemble_bench(a.ensemble, a.ensemble_steps, a.loop_steps)
     rates = [float(v) for v in a.serve_rates.split(",") if v]

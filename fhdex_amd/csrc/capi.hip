@@ -566,8 +566,10 @@ int fhdx_fill_noise(fhdx_ctx* c, uint64_t seed, uint64_t step, uint32_t instance
     for (int d = 0; d < 3; ++d) {
         A[d] = make_arr(stochA[d]);
         B[d] = make_arr(stochB[d]);
-        int r = fill_value(stochA[d].p, A[d].sn * stochA[d].ncomp, 0.0, s);
-        if (!r) r = fill_value(stochB[d].p, B[d].sn * stochB[d].ncomp, 0.0, s);
+        // the noise kernel writes components 0..nvars-1; zero any further ones first
+        int r = FHDX_OK;
+        if (stochA[d].ncomp > c->dp.nvars) r = fill_value(stochA[d].p, A[d].sn * stochA[d].ncomp, 0.0, s);
+        if (!r && stochB[d].ncomp > c->dp.nvars) r = fill_value(stochB[d].p, B[d].sn * stochB[d].ncomp, 0.0, s);
         if (r) return r;
     }
     return ref_fill_noise(c->dp, make_noise_key(seed, step, instance), A, B, make_arr(rancornA),

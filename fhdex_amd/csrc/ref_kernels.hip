@@ -862,10 +862,10 @@ __global__ void __launch_bounds__(BLK) k_lincomb(double* dst, const double* a, c
     }
 }
 
-// Fills face component `comp` of direction D for sets A and B.
+// Fills every component of direction D's face noise for sets A and B: component 0 (density)
+// is zero (timeStep.cpp:74-76 skips it), components 1..nvars-1 are N(0,1) * their scale.
 template <int D>
-__global__ void __launch_bounds__(BLK) k_noise_face(DevParams P, NoiseKey K, Arr A, Arr B, int comp,
-                                                    double scale)
+__global__ void __launch_bounds__(BLK) k_noise_face(DevParams P, NoiseKey K, Arr A, Arr B)
 {
     const int fx = P.n[0] + (D == 0), fy = P.n[1] + (D == 1), fz = P.n[2] + (D == 2);
     long t = (long)blockIdx.x * BLK + threadIdx.x;
@@ -879,10 +879,15 @@ __global__ void __launch_bounds__(BLK) k_noise_face(DevParams P, NoiseKey K, Arr
     // in a wall direction face n is a face of its own, drawn from its own field
     const int c = (D == 0) ? i : (D == 1) ? j : k;
     const uint32_t wall_hi = (c == P.n[D] && !P.periodic[D]) ? 128u : 0u;
-    double za, zb;
-    noise_ab(K, (uint32_t)(16 * D + comp) + wall_hi, lin, za, zb);
-    A(i, j, k, comp) = za * scale;
-    B(i, j, k, comp) = zb * scale;
+    A(i, j, k, 0) = 0.0;
+    B(i, j, k, 0) = 0.0;
+    for (int comp = 1; comp < P.nvars; ++comp) {
+        const double scale = (comp <= 3) ? P.var_mom : (comp == 4) ? P.var_ener : P.var_mass;
+        double za, zb;
+        noise_ab(K, (uint32_t)(16 * D + comp) + wall_hi, lin, za, zb);
+        A(i, j, k, comp) = za * scale;
+        B(i, j, k, comp) = zb * scale;
+    }
 }
 
 __global__ void __launch_bounds__(BLK) k_noise_node(DevParams P, NoiseKey K, Arr A, Arr B, double scale)
@@ -1052,12 +1057,9 @@ int ref_lincomb(double* dst, const double* a, const double* b, long npts, int nc
 int ref_fill_noise(const DevParams& P, const NoiseKey& K, const Arr A[3], const Arr B[3], Arr rA, Arr rB,
                    hipStream_t s)
 {
-    for (int c = 1; c < P.nvars; ++c) {
-        double sc = (c <= 3) ? P.var_mom : (c == 4) ? P.var_ener : P.var_mass;
-        k_noise_face<0><<<grid_for(nface(P, 0), BLK), BLK, 0, s>>>(P, K, A[0], B[0], c, sc);
-        k_noise_face<1><<<grid_for(nface(P, 1), BLK), BLK, 0, s>>>(P, K, A[1], B[1], c, sc);
-        k_noise_face<2><<<grid_for(nface(P, 2), BLK), BLK, 0, s>>>(P, K, A[2], B[2], c, sc);
-    }
+    k_noise_face<0><<<grid_for(nface(P, 0), BLK), BLK, 0, s>>>(P, K, A[0], B[0]);
+    k_noise_face<1><<<grid_for(nface(P, 1), BLK), BLK, 0, s>>>(P, K, A[1], B[1]);
+    k_noise_face<2><<<grid_for(nface(P, 2), BLK), BLK, 0, s>>>(P, K, A[2], B[2]);
     k_noise_node<<<grid_for(nnode(P), BLK), BLK, 0, s>>>(P, K, rA, rB, P.var_mom);
     FHDX_HIP_TRY(hipGetLastError());
     return FHDX_OK;
