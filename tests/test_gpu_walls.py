@@ -322,3 +322,27 @@ def test_temp_grad_reaches_steady_conduction(dev):
     assert abs(t_lo - 280.8) < 0.5 and abs(t_hi - 508.5) < 0.5
     # at rest up to a residual acoustic ring-down far below the sound speed (3.7e4 cm/s)
     assert u < 5.0 and change < 1e-6
+
+
+def test_chemistry_between_walls_matches_oracle(dev, orc):
+    """The NO2/N2O4 chemical Langevin deck between no-slip isothermal walls in x (periodic y, z):
+    three reference-shaped Philox steps, reaction noise included, equal the oracle bit for bit."""
+    torch, fc, L = dev
+    cfg = P.no2_n2o4(10)
+    p = cfg.params
+    P.set_walls(p, [(2, 2), (-1, -1), (-1, -1)], [(2, 2), (-1, -1), (-1, -1)], [(1, 1), (-1, -1), (-1, -1)],
+                t_lo=(340.0, 0.0, 0.0), t_hi=(360.0, 0.0, 0.0))
+    say(f"case chemistry between walls {tuple(p.n_cells)} x 3")
+    st = perturbed_state(p, cfg, orc, seed=12, amp=0.02, vel=200.0)
+    ctx = L.Context(p)
+    ctx.set_chemistry(cfg.chem)
+    ds = fc.DeviceState(p)
+    ds.cu.t.copy_(torch.from_numpy(st.cu))
+    ds.prim.t.copy_(torch.from_numpy(st.prim))
+    for step in range(3):
+        fc.RK3step(ctx, ds, cfg.dt, noise=None, seed=31, step=step)
+    ctx.check()
+    assert orc.advance_chem(p, cfg.chem, st.cu, st.prim, 3, cfg.dt, seed=31, step0=0) == 0
+    same = np.array_equal(ds.cu.t.cpu().numpy(), st.cu) and np.array_equal(ds.prim.t.cpu().numpy(), st.prim)
+    say(f"  exact={same}")
+    assert same
