@@ -134,6 +134,17 @@ void fhdx_params_finalize(fhdx_params* p)
             molmix = 1. / molmix;
             for (int n = 0; n < NS; ++n) p->bc_Xk[f][n] = p->bc_Yk[f][n] * molmix / p->molmass[n];
         }
+        if (code != 3) continue;
+        // reservoir gas: density from p and T (GetDensity), or T from p and rho (boundary.cpp:49-57)
+        const int d = f >> 1;
+        double& pw = (f & 1) ? p->p_hi[d] : p->p_lo[d];
+        double& rw = (f & 1) ? p->rho_hi[d] : p->rho_lo[d];
+        double& tw = (f & 1) ? p->t_hi[d] : p->t_lo[d];
+        double molmix = 0.;
+        for (int n = 0; n < NS; ++n) molmix += p->bc_Yk[f][n] / p->molmass[n];
+        molmix = 1. / molmix;
+        if (pw > 0. && rw <= 0. && tw > 0.) rw = pw / (p->Runiv / molmix) / tw;
+        else if (pw > 0. && rw > 0. && tw <= 0.) tw = pw * (molmix / p->Runiv) / rw;
     }
 }
 
@@ -174,13 +185,22 @@ int fhdx_params_check(const fhdx_params* p)
                 set_error("bc_vel of a wall must be 1 (slip) or 2 (no slip)");
                 return FHDX_ERR_UNSUPPORTED;
             }
-            if (m == 3) {
-                set_error("bc_mass 3 (reservoir) is not implemented; walls take bc_mass 1 or 2");
+            if (m == 3 && d != 0) {
+                set_error("reservoirs (bc_mass 3) are implemented on x faces only: the reference's y and z "
+                          "reservoir ghosts keep stale velocity and energy (boundary.cpp:648-649, 806-807)");
                 return FHDX_ERR_UNSUPPORTED;
             }
-            if (m != 1 && m != 2) {
-                set_error("bc_mass of a wall must be 1 (wall) or 2 (fixed composition)");
+            if (m != 1 && m != 2 && m != 3) {
+                set_error("bc_mass of a wall must be 1 (wall), 2 (fixed composition) or 3 (x reservoir)");
                 return FHDX_ERR_UNSUPPORTED;
+            }
+            if (m == 3) {
+                const double pw = side ? p->p_hi[d] : p->p_lo[d], rw = side ? p->rho_hi[d] : p->rho_lo[d];
+                const double tr = side ? p->t_hi[d] : p->t_lo[d];
+                if (!(pw > 0.) || !(rw > 0.) || !(tr > 0.)) {
+                    set_error("a reservoir needs p_lo/p_hi > 0 and, after fhdx_params_finalize, rho and t > 0");
+                    return FHDX_ERR_ARG;
+                }
             }
             if (t != 1 && t != 2) {
                 set_error("bc_therm of a wall must be 1 (adiabatic) or 2 (isothermal)");
@@ -191,7 +211,7 @@ int fhdx_params_check(const fhdx_params* p)
                 set_error("an isothermal wall needs t_lo/t_hi > 0");
                 return FHDX_ERR_ARG;
             }
-            if (m == 2) {
+            if (m == 2 || m == 3) {
                 const int f = 2 * d + side;
                 double sy = 0.;
                 bool pos = true;
@@ -200,7 +220,7 @@ int fhdx_params_check(const fhdx_params* p)
                     pos = pos && p->bc_Yk[f][n] > 0. && p->bc_Xk[f][n] > 0.;
                 }
                 if (!pos || std::fabs(sy - 1.) > 1e-10) {
-                    set_error("a bc_mass 2 face needs positive bc_Yk/bc_Xk summing to 1 (fhdx_params_finalize)");
+                    set_error("a bc_mass 2/3 face needs positive bc_Yk/bc_Xk summing to 1 (fhdx_params_finalize)");
                     return FHDX_ERR_ARG;
                 }
             }

@@ -76,6 +76,8 @@ inline DevParams make_dev_params(const fhdx_params& p)
             d.bcm[f] = side ? p.bc_mass_hi[i] : p.bc_mass_lo[i];
             d.wall[f] = (d.bcm[f] != 3 && d.bcm[f] != -1);
             d.tw[f] = side ? p.t_hi[i] : p.t_lo[i];
+            d.pw[f] = side ? p.p_hi[i] : p.p_lo[i];
+            d.rhow[f] = side ? p.rho_hi[i] : p.rho_lo[i];
             for (int n = 0; n < MS; ++n) {
                 d.Yw[f][n] = p.bc_Yk[f][n];
                 d.Xw[f][n] = p.bc_Xk[f][n];

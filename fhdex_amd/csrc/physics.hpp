@@ -43,7 +43,7 @@ struct DevParams {
     int any_wall;
     int periodic[3];
     int wall[6], bcv[6], bct[6], bcm[6];
-    double tw[6];
+    double tw[6], pw[6], rhow[6];  // wall / reservoir temperature; reservoir pressure, density
     double Yw[6][MS], Xw[6][MS];
 };
 

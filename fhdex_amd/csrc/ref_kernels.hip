@@ -174,6 +174,33 @@ __global__ void __launch_bounds__(BLK) k_set_bc(DevParams P, Arr prim, Arr cons,
             PR(c, 6 + s) = P.Yw[f][s];
             PR(c, 6 + NS + s) = P.Xw[f][s];
         }
+    if (bm == 3 && P.algorithm_type == 2) {
+        // x reservoir (boundary.cpp:221-238, 260-288): the reservoir gas at rest; no thermal or velocity
+        // branch follows (the x chain is else-if)
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            PR(c, 6 + s) = P.Yw[f][s];
+            PR(c, 6 + NS + s) = P.Xw[f][s];
+        }
+        const double rho = P.rhow[f];
+        PR(c, 0) = rho;
+        CO(c, 0) = rho;
+        PR(c, 4) = P.tw[f];
+        PR(c, 5) = P.pw[f];
+#pragma unroll
+        for (int q = 1; q < 4; ++q) {
+            CO(c, q) = 0.0;
+            PR(c, q) = 0.0;
+        }
+        double Yr[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) Yr[s] = P.Yw[f][s];
+        const double ie = eos_energy<NS>(P, Yr, P.tw[f]);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) CO(c, 5 + s) = rho * Yr[s];
+        CO(c, 4) = rho * ie + 0.5 * rho * (0.0 * 0.0 + 0.0 * 0.0 + 0.0 * 0.0);
+        return;
+    }
     if (bt == 1) {
         PR(c, 4) = PR(m, 4);
         PR(c, 5) = PR(m, 5);

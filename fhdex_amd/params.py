@@ -56,6 +56,10 @@ class Params(C.Structure):
         ("t_hi", C.c_double * 3),
         ("bc_Yk", (C.c_double * MAX_SPECIES) * 6),
         ("bc_Xk", (C.c_double * MAX_SPECIES) * 6),
+        ("p_lo", C.c_double * 3),
+        ("p_hi", C.c_double * 3),
+        ("rho_lo", C.c_double * 3),
+        ("rho_hi", C.c_double * 3),
     ]
 
     @property
@@ -282,7 +286,8 @@ def no2_n2o4(n, nsteps=100, reaction_type=1):
 
 
 def set_walls(p: Params, bc_vel, bc_therm, bc_mass, t_lo=(0.0, 0.0, 0.0), t_hi=(0.0, 0.0, 0.0),
-              bc_Yk=None) -> Params:
+              bc_Yk=None, p_lo=(0.0, 0.0, 0.0), p_hi=(0.0, 0.0, 0.0), rho_lo=(0.0, 0.0, 0.0),
+              rho_hi=(0.0, 0.0, 0.0)) -> Params:
     """Boundary codes of the reference's namelist (common_namespace.H): bc_vel, bc_therm and
     bc_mass are ((lo, hi) per direction) or a single (lo, hi) pair or code for all three.
     A direction with bc_vel -1 is periodic and every code of it becomes -1 (SetupBC,
@@ -306,13 +311,15 @@ def set_walls(p: Params, bc_vel, bc_therm, bc_mass, t_lo=(0.0, 0.0, 0.0), t_hi=(
         p.bc_mass_lo[d], p.bc_mass_hi[d] = mass[d]
         p.t_lo[d] = float(t_lo[d])
         p.t_hi[d] = float(t_hi[d])
+        p.p_lo[d], p.p_hi[d] = float(p_lo[d]), float(p_hi[d])
+        p.rho_lo[d], p.rho_hi[d] = float(rho_lo[d]), float(rho_hi[d])
     ns = p.nspecies
     for f in range(6):
         code = mass[f // 2][f % 2]
         for n in range(MAX_SPECIES):
             p.bc_Yk[f][n] = 0.0
             p.bc_Xk[f][n] = 0.0
-        if code != 2:
+        if code != 2 and code != 3:
             continue
         Y = bc_Yk if (bc_Yk is not None and not hasattr(bc_Yk[0], "__len__")) else bc_Yk[f]
         for n in range(ns):
@@ -323,7 +330,26 @@ def set_walls(p: Params, bc_vel, bc_therm, bc_mass, t_lo=(0.0, 0.0, 0.0), t_hi=(
         molmix = 1.0 / molmix
         for n in range(ns):
             p.bc_Xk[f][n] = p.bc_Yk[f][n] * molmix / p.molmass[n]
+        if code == 3:  # reservoir density from p, T and the composition (GetDensity), if not given
+            tw = p.t_hi[f // 2] if f % 2 else p.t_lo[f // 2]
+            pw = p.p_hi[f // 2] if f % 2 else p.p_lo[f // 2]
+            if (p.rho_hi[f // 2] if f % 2 else p.rho_lo[f // 2]) <= 0.0 and pw > 0.0 and tw > 0.0:
+                rho = pw / (p.Runiv / molmix) / tw
+                if f % 2:
+                    p.rho_hi[f // 2] = rho
+                else:
+                    p.rho_lo[f // 2] = rho
     return p
+
+
+def ambient_pressure(p: Params, rho0, rhobar, T) -> float:
+    """GetPressureGas of the initial state (compressible_functions.H:186-199): the reference's
+    default reservoir pressure (boundary.cpp:43-47)."""
+    molmix = 0.0
+    for n in range(p.nspecies):
+        molmix += rhobar[n] / p.molmass[n]
+    molmix = 1.0 / molmix
+    return rho0 * (p.Runiv / molmix) * T
 
 
 def wall_deck(name, n=None, nsteps=3):

@@ -55,8 +55,9 @@ typedef struct fhdx_params {
     int32_t nreaction;         /* 0, or the reaction count given to fhdx_ctx_set_chemistry   */
     /* Boundary codes per direction (common_namespace.H, boundary.cpp:172-2001).  bc_vel -1:
      * periodic (then every code of that direction is -1, SetupBC boundary.cpp:3-12); 1: slip
-     * wall, 2: no-slip wall.  bc_mass 1: wall (no species flux), 2: fixed composition bc_Yk;
-     * 3 (reservoir) is not implemented.  bc_therm 1: adiabatic, 2: isothermal at t_lo/t_hi.
+     * wall, 2: no-slip wall.  bc_mass 1: wall (no species flux), 2: fixed composition bc_Yk,
+     * 3: reservoir (x faces only; the reference's y/z reservoir ghosts keep stale velocity and
+     * energy, boundary.cpp:648-649, 806-807).  bc_therm 1: adiabatic, 2: isothermal at t_lo/t_hi.
      * Walls are served by the reference-shaped entry points; the fused step, the ensemble
      * engine and the server take periodic boxes only. */
     int32_t bc_vel_lo[3], bc_vel_hi[3];
@@ -81,6 +82,13 @@ typedef struct fhdx_params {
      * to 0 from the other (SetupCWall, boundary.cpp:14-170). */
     double bc_Yk[6][FHDX_MAX_SPECIES];
     double bc_Xk[6][FHDX_MAX_SPECIES];
+    /* Reservoirs (bc_mass 3, x faces only): pressure, density and temperature (t_lo/t_hi) of the
+     * reservoir gas.  p must be given (the reference's default, the ambient pressure of the
+     * initial state, is computed by the AMReX adapter); fhdx_params_finalize fills rho <= 0
+     * from p, T and the wall composition, or T <= 0 from p and rho (SetupCWall,
+     * boundary.cpp:38-58, 76-96). */
+    double p_lo[3], p_hi[3];
+    double rho_lo[3], rho_hi[3];
 } fhdx_params;
 
 /* One FAB: data pointer plus the inclusive index box it covers (ghosts included). */

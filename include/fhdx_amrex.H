@@ -58,6 +58,11 @@ inline fhdx_params params_from_namespaces(amrex::Geometry const& geom)
         p.grav[d] = grav[d];
         p.t_lo[d] = t_lo[d];
         p.t_hi[d] = t_hi[d];
+        // reservoir gas, as SetupCWall (boundary.cpp:14-170, run by main_driver) left it
+        p.p_lo[d] = p_lo[d];
+        p.p_hi[d] = p_hi[d];
+        p.rho_lo[d] = rho_lo[d];
+        p.rho_hi[d] = rho_hi[d];
     }
     // wall compositions (common_namespace.H:173-184), face 2*direction + side
     const amrex::GpuArray<amrex::Real, MAX_SPECIES>* Yw[6] = {&bc_Yk_x_lo, &bc_Yk_x_hi, &bc_Yk_y_lo,

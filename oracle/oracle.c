@@ -441,7 +441,7 @@ static double get_density(const fhdx_params* P, double pres, double T, const dou
 /* One ghost cell of setBC's face (d, side) (boundary.cpp:180-1036; mass-fraction branches
  * :185-209, thermal :230-254, slip :283-339, no slip :340-378 for lo x and their mirror images
  * for the other faces).  The mirror of ghost index c is -c-1 at a lo face and 2n-c-1 at a hi
- * face.  bc_mass 3 (reservoir) is rejected by the parameter check before this is reached. */
+ * face.  bc_mass 3 (reservoir) is accepted on x faces only (the parameter check). */
 static void set_bc_cell(const fhdx_params* P, fab_t* prim, fab_t* cons, int d, int side, int i,
                         int j, int k)
 {
@@ -462,6 +462,29 @@ static void set_bc_cell(const fhdx_params* P, fab_t* prim, fab_t* cons, int d, i
             PR(c, 6 + s) = P->bc_Yk[f][s];
             PR(c, 6 + NS + s) = P->bc_Xk[f][s];
         }
+    if (bm == 3 && P->algorithm_type == 2) { /* x reservoir, boundary.cpp:221-238 and 260-288 */
+        const double rw = side ? P->rho_hi[d] : P->rho_lo[d], pw = side ? P->p_hi[d] : P->p_lo[d];
+        for (int s = 0; s < NS; ++s) {
+            PR(c, 6 + s) = P->bc_Yk[f][s];
+            PR(c, 6 + NS + s) = P->bc_Xk[f][s];
+            PR(c, 0) = rw;
+            CO(c, 0) = rw;
+            PR(c, 4) = tw;
+            PR(c, 5) = pw;
+        }
+        /* the momentum block of a reservoir: gas at rest, energy from rho and T */
+        for (int q = 1; q < 4; ++q) {
+            CO(c, q) = 0.0;
+            PR(c, q) = 0.0;
+        }
+        double Yr[MS];
+        for (int s = 0; s < NS; ++s) Yr[s] = PR(c, 6 + s);
+        const double temp = PR(c, 4), rho = PR(c, 0);
+        const double ie = get_energy(P, Yr, temp);
+        for (int s = 0; s < NS; ++s) CO(c, 5 + s) = rho * PR(c, 6 + s);
+        CO(c, 4) = rho * ie + 0.5 * rho * (PR(c, 1) * PR(c, 1) + PR(c, 2) * PR(c, 2) + PR(c, 3) * PR(c, 3));
+        return;
+    }
     if (bt == 1) {
         PR(c, 4) = PR(m, 4);
         PR(c, 5) = PR(m, 5);

@@ -104,9 +104,22 @@ def test_params_default_finalize_check():
     P.set_walls(q, [(1, 1), (-1, -1), (-1, -1)], 1, 1)
     assert lib.fhdx_params_check(C.byref(q)) == 0          # slip adiabatic species walls in x
     r = q.copy()
-    r.bc_mass_hi[0] = 3
-    assert lib.fhdx_params_check(C.byref(r)) == 2          # reservoirs are not implemented
+    r.bc_mass_hi[0] = 3                                    # x reservoir without its gas state
+    assert lib.fhdx_params_check(C.byref(r)) == 1
     assert b"reservoir" in lib.fhdx_last_error()
+    r = q.copy()                                           # x reservoir: rho follows from p, T, Y
+    P.set_walls(r, [(2, 2), (-1, -1), (-1, -1)], [(2, 2), (-1, -1), (-1, -1)], [(3, 1), (-1, -1), (-1, -1)],
+                t_lo=(300.0, 0.0, 0.0), t_hi=(300.0, 0.0, 0.0), bc_Yk=[0.5, 0.5], p_lo=(1.0e6, 0.0, 0.0))
+    r.rho_lo[0] = 0.0
+    lib.fhdx_params_finalize(C.byref(r))
+    assert r.rho_lo[0] == pytest.approx(1.0e6 * 39.95 / (P.R_UNIV * 300.0))
+    assert lib.fhdx_params_check(C.byref(r)) == 0
+    r = q.copy()                                           # y reservoirs are not
+    P.set_walls(r, [(-1, -1), (2, 2), (-1, -1)], [(-1, -1), (2, 2), (-1, -1)], [(-1, -1), (3, 3), (-1, -1)],
+                t_lo=(0.0, 300.0, 0.0), t_hi=(0.0, 300.0, 0.0), bc_Yk=[0.5, 0.5], p_lo=(0.0, 1e6, 0.0),
+                p_hi=(0.0, 1e6, 0.0))
+    assert lib.fhdx_params_check(C.byref(r)) == 2
+    assert b"x faces only" in lib.fhdx_last_error()
     r = q.copy()
     r.bc_therm_lo[0] = 2                                   # isothermal without a wall temperature
     assert lib.fhdx_params_check(C.byref(r)) == 1

@@ -30,6 +30,18 @@ def x_walls():
     return P.Config("x_walls", p, base.rho0, base.rhobar, base.T_init, base.dt, 1)
 
 
+def x_reservoirs():
+    """x reservoirs (bc_mass 3, no slip, isothermal) with different gas at each end; y walls."""
+    base = P.species_prefix(3, 10, 1)
+    p = P.make_params(3, (10, 8, 9), 8e-6, [4.0026, 20.1797, 39.9480], [2.18e-8, 2.58e-8, 3.63e-8])
+    pa = P.ambient_pressure(p, base.rho0, base.rhobar, base.T_init)
+    P.set_walls(p, [(2, 2), (1, 1), (-1, -1)], [(2, 2), (1, 1), (-1, -1)], [(3, 3), (1, 1), (-1, -1)],
+                t_lo=(310.0, 0.0, 0.0), t_hi=(295.0, 0.0, 0.0),
+                bc_Yk=[[0.5, 0.3, 0.2], list(base.rhobar), [0] * 3, [0] * 3, [0] * 3, [0] * 3],
+                p_lo=(1.03 * pa, 0.0, 0.0), p_hi=(pa, 0.0, 0.0))
+    return P.Config("x_reservoirs", p, base.rho0, base.rhobar, base.T_init, base.dt, 1)
+
+
 def visc3_walls():
     cfg = P.wall_deck("slip_wall_adiabatic", 8)
     cfg.params.visc_type = 3
@@ -43,6 +55,7 @@ CASES = [
     ("temp_grad", lambda: P.wall_deck("temp_grad", 16)),
     ("visc_type_3", visc3_walls),
     ("x_walls", x_walls),
+    ("x_reservoirs", x_reservoirs),
 ]
 
 

@@ -190,3 +190,58 @@ def test_no_mass_species_or_heat_through_closed_adiabatic_walls(orc):
             assert np.all(face[nv] == 0.0) and np.all(face[nv + 3] == 0.0)
         inner = np.take(f, 1, axis=3 - dd)
         assert np.abs(inner[nv]).max() > 0 and np.abs(inner[5]).max() > 0
+
+
+def reservoir_box(dp_frac=0.0, t_res=300.0):
+    """Three species between x reservoirs (no slip, isothermal, bc_mass 3) at the gas's own
+    composition; the reservoir pressure is the ambient one times (1 + dp_frac)."""
+    cfg = box((0,), "noslip", n=(10, 8, 8))
+    p = cfg.params
+    pa = P.ambient_pressure(p, cfg.rho0, cfg.rhobar, cfg.T_init) * (1.0 + dp_frac)
+    P.set_walls(p, [(2, 2), (-1, -1), (-1, -1)], [(2, 2), (-1, -1), (-1, -1)], [(3, 3), (-1, -1), (-1, -1)],
+                t_lo=(t_res, 0.0, 0.0), t_hi=(t_res, 0.0, 0.0), bc_Yk=list(cfg.rhobar),
+                p_lo=(pa, 0.0, 0.0), p_hi=(pa, 0.0, 0.0))
+    return cfg
+
+
+def test_reservoir_ghosts_hold_the_reservoir_gas_at_rest(orc):
+    cfg = reservoir_box(dp_frac=0.05, t_res=320.0)
+    p = cfg.params
+    st = perturbed_state(p, cfg, orc, seed=4, amp=0.02, vel=300.0)
+    g, NS = p.ngc, p.nspecies
+    for gi in (0, 1, -1, -2):                            # the two ghost layers on each side
+        a = st.prim[:, g:-g, g:-g, gi]
+        c = st.cu[:, g:-g, g:-g, gi]
+        assert np.all(a[0] == p.rho_lo[0]) and np.all(a[4] == 320.0) and np.all(a[5] == p.p_lo[0])
+        assert np.all(a[1:4] == 0.0) and np.all(c[1:4] == 0.0)
+        for s in range(NS):
+            assert np.all(a[6 + s] == p.bc_Yk[0][s])
+        # the reservoir's density is the equation of state's at (p, T, Y)
+        molmix = 1.0 / sum(p.bc_Yk[0][s] / p.molmass[s] for s in range(NS))
+        assert a[0].flat[0] * p.Runiv / molmix * 320.0 == pytest.approx(p.p_lo[0], rel=1e-14)
+        e = sum(p.bc_Yk[0][s] * (p.e0[s] + p.hcv[s] * 320.0) for s in range(NS))
+        assert np.allclose(c[4], a[0] * e, rtol=1e-15, atol=0)
+
+
+def test_gas_at_the_reservoir_state_stays_uniform_and_higher_pressure_fills_the_box(orc):
+    cfg = reservoir_box(dp_frac=0.0, t_res=300.0)
+    p = cfg.params
+    p.stoch_stress_form = 0
+    cu = E.uniform_cons(cfg)
+    prim = np.zeros((p.nprimvars,) + cu.shape[1:])
+    u0 = valid(p, cu).copy()
+    assert orc.advance(p, cu, prim, 5, cfg.dt) == 0
+    u1 = valid(p, cu)
+    for c in (0, 4, 5, 6, 7):
+        assert np.abs(u1[c] - u0[c]).max() <= 1e-12 * np.abs(u0[c]).max(), c
+    cfg = reservoir_box(dp_frac=0.05, t_res=300.0)       # 5 % over-pressure at both ends
+    p = cfg.params
+    p.stoch_stress_form = 0
+    cu = E.uniform_cons(cfg)
+    prim = np.zeros((p.nprimvars,) + cu.shape[1:])
+    m0 = valid(p, cu)[0].sum()
+    assert orc.advance(p, cu, prim, 20, cfg.dt) == 0
+    assert valid(p, cu)[0].sum() > m0 * (1 + 1e-5)
+    # the inflow is symmetric: x momentum is antisymmetric about the centre
+    jx = valid(p, cu)[1]
+    assert np.abs(jx + np.flip(jx, axis=2)).max() <= 1e-12 * np.abs(jx).max()
