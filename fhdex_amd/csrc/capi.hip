@@ -416,11 +416,17 @@ int rk3_ref(fhdx_ctx* c, fhdx_fab cu, fhdx_fab cup, fhdx_fab cup2, fhdx_fab prim
     for (int st = 0; st < 3; ++st) {
         int r;
         if ((r = ref_transport(P, a.prim, a.eta, a.zeta, a.kappa, a.chi, a.D, c->d_err, s))) return r;
-        for (int d = 0; d < 3; ++d)
-            if ((r = ref_lincomb(stochFlux[d].p, sA[d].p, sB[d].p, npts(stochFlux[d]), nv, 1, ws[st], s))) return r;
-        if ((r = ref_lincomb(rancorn.p, rA.p, rB.p, npts(rancorn), 1, 0, ws[st], s))) return r;
+        {
+            double* const dst[4] = {stochFlux[0].p, stochFlux[1].p, stochFlux[2].p, rancorn.p};
+            const double* const la[4] = {sA[0].p, sA[1].p, sA[2].p, rA.p};
+            const double* const lb[4] = {sB[0].p, sB[1].p, sB[2].p, rB.p};
+            const long np[4] = {npts(stochFlux[0]), npts(stochFlux[1]), npts(stochFlux[2]), npts(rancorn)};
+            const int lnc[4] = {nv, nv, nv, 1}, lc0[4] = {1, 1, 1, 0};
+            if ((r = ref_lincomb4(dst, la, lb, np, lnc, lc0, ws[st], s))) return r;
+        }
         a.cons = make_arr(in[st]);
-        if ((r = ref_calculate_flux(P, a, s))) return r;
+        int fnc[3] = {flux[0].ncomp, flux[1].ncomp, flux[2].ncomp};
+        if ((r = ref_calculate_flux(P, a, fnc, s))) return r;
         if (chem) {  // timeStep.cpp:135-143: ranchem = A + w B, then the source from this stage's prim
             if ((r = ref_lincomb(rw.ran.p, rw.ranA.p, rw.ranB.p, npts(rw.ran), c->p.nreaction, 0, ws[st], s)))
                 return r;
@@ -569,7 +575,8 @@ int fhdx_calculate_flux(fhdx_ctx* c, fhdx_fab cons, fhdx_fab prim, fhdx_fab eta,
     a.visccorn = make_arr(visccorn);
     a.rancorn = make_arr(rancorn);
     a.dt = dt;
-    return ref_calculate_flux(c->dp, a, S(stream));
+    int fnc[3] = {flux[0].ncomp, flux[1].ncomp, flux[2].ncomp};
+    return ref_calculate_flux(c->dp, a, fnc, S(stream));
 }
 
 int fhdx_fill_noise(fhdx_ctx* c, uint64_t seed, uint64_t step, uint32_t instance, const fhdx_fab stochA[3],

@@ -21,7 +21,8 @@ int ref_fill_boundary(const DevParams& P, Arr a, int nc, hipStream_t s);
 // setBC: the ghost cells outside walls, faces lo x .. hi z in turn (no-op when periodic).
 int ref_set_bc(const DevParams& P, Arr prim, Arr cons, hipStream_t s);
 int fill_value(double* p, long n, double v, hipStream_t s);
-int ref_calculate_flux(const DevParams& P, const FluxArgs& a, hipStream_t s);
+// flux_ncomp: the components of each flux FAB (>= nvars + 4)
+int ref_calculate_flux(const DevParams& P, const FluxArgs& a, int flux_ncomp[3], hipStream_t s);
 // source: the source MultiFab (p == nullptr: zero)
 int ref_stage(const DevParams& P, Arr out, Arr u0, Arr us, const Arr flux[3], Arr source, double dt, int mode,
               hipStream_t s);
@@ -30,6 +31,9 @@ int ref_chem_source(const DevParams& P, Arr prim, Arr source, Arr ranchem, doubl
 int ref_fill_chem_noise(const DevParams& P, const NoiseKey& K, Arr A, Arr B, hipStream_t s);
 int ref_lincomb(double* dst, const double* a, const double* b, long npts, int nc, int c0, double w,
                 hipStream_t s);
+// The same for four arrays at once (a stage's three face noise arrays and the node array).
+int ref_lincomb4(double* const dst[4], const double* const a[4], const double* const b[4], const long npts[4],
+                 const int nc[4], const int c0[4], double w, hipStream_t s);
 int ref_fill_noise(const DevParams& P, const NoiseKey& K, const Arr A[3], const Arr B[3], Arr rA, Arr rB,
                    hipStream_t s);
 

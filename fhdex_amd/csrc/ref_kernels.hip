@@ -146,19 +146,23 @@ __global__ void __launch_bounds__(BLK) k_fill_boundary(DevParams P, Arr a, int n
 // mass fractions mirrored (bc_mass 1) or fixed (2); T and p mirrored (adiabatic) or T = t_wall
 // with p mirrored (isothermal); velocity: slip (normal 0, tangential mirrored) or no slip (0);
 // then rho from GetDensity and the conserved state from it.
+// One launch per direction covers the ghost slabs of both faces (2*ngc layers across D, the
+// grown box along the other two): the lo and hi passes write disjoint cells and read only
+// mirror cells inside the box, so running them together equals running them in turn.
 template <int NS>
-__global__ void __launch_bounds__(BLK) k_set_bc(DevParams P, Arr prim, Arr cons, int D, int side)
+__global__ void __launch_bounds__(BLK) k_set_bc(DevParams P, Arr prim, Arr cons, int D)
 {
     const int g = P.ngc, n = P.n[D];
-    const int gx = P.n[0] + 2 * g, gy = P.n[1] + 2 * g, gz = P.n[2] + 2 * g;
+    int ext[3] = {P.n[0] + 2 * g, P.n[1] + 2 * g, P.n[2] + 2 * g};
+    ext[D] = 2 * g;
     long t = (long)blockIdx.x * BLK + threadIdx.x;
-    if (t >= (long)gx * gy * gz) return;
+    if (t >= (long)ext[0] * ext[1] * ext[2]) return;
     int c[3];
-    face_index(t, gx, gy, c[0], c[1], c[2]);
-    c[0] -= g;
-    c[1] -= g;
-    c[2] -= g;
-    if (side == 0 ? c[D] >= 0 : c[D] < n) return;
+    face_index(t, ext[0], ext[1], c[0], c[1], c[2]);
+    for (int d = 0; d < 3; ++d)
+        if (d != D) c[d] -= g;
+    const int side = c[D] >= g;  // layers 0..g-1: lo ghosts -g..-1; g..2g-1: hi ghosts n..n+g-1
+    c[D] = side ? n + (c[D] - g) : c[D] - g;
     int m[3] = {c[0], c[1], c[2]};
     m[D] = side ? 2 * (n - 1) - c[D] + 1 : -c[D] - 1;
     const int f = 2 * D + side;
@@ -360,10 +364,14 @@ __global__ void __launch_bounds__(BLK) k_stoch(DevParams P, Arr prim, Arr eta, A
     }
     w[1 + D] = w[1 + D] + 0.25 * nweight * sum;
 
-    double o[NS + 9];  // this face's stochastic flux, nvars + 4 components
+    // this face's flux after the stochastic pass, every one of the nvars + 4 components: the
+    // reference adds to a zeroed array (flux.cpp:24-26), so each term is 0.0 + term
+    double o[NS + 9];
 #pragma unroll
-    for (int n = 1; n < 4; ++n) o[n] = fl(i, j, k, n) + w[n];
-    o[nv] = fl(i, j, k, nv) + w[4];
+    for (int n = 0; n < NS + 9; ++n) o[n] = 0.0;
+#pragma unroll
+    for (int n = 1; n < 4; ++n) o[n] = 0.0 + w[n];
+    o[nv] = 0.0 + w[4];
 
     constexpr int t1 = (D == 0) ? 2 : 1, t2 = (D == 2) ? 2 : 3;
     double phidiag = w[1 + D] * (prim(il, jl, kl, 1 + D) + prim(iu, ju, ku, 1 + D));
@@ -371,9 +379,8 @@ __global__ void __launch_bounds__(BLK) k_stoch(DevParams P, Arr prim, Arr eta, A
                       w[t2] * (prim(il, jl, kl, t2) + prim(iu, ju, ku, t2));
     phidiag = -0.5 * phidiag;
     phishear = -0.5 * phishear;
-    o[nv + 1] = fl(i, j, k, nv + 1) - phidiag;
-    o[nv + 2] = fl(i, j, k, nv + 2) - phishear;
-    o[nv + 3] = fl(i, j, k, nv + 3);
+    o[nv + 1] = 0.0 - phidiag;
+    o[nv + 2] = 0.0 - phishear;
 
     if (P.algorithm_type == 2) {
         double yy[NS], yyp[NS], DijY[NS * NS], sqD[NS * NS];
@@ -465,17 +472,7 @@ __global__ void __launch_bounds__(BLK) k_stoch(DevParams P, Arr prim, Arr eta, A
         }
     }
 #pragma unroll
-    for (int n = 1; n < 4; ++n) fl(i, j, k, n) = o[n];
-    fl(i, j, k, nv) = o[nv];
-    fl(i, j, k, nv + 1) = o[nv + 1];
-    fl(i, j, k, nv + 2) = o[nv + 2];
-    if (P.algorithm_type == 2) {
-#pragma unroll
-        for (int s = 0; s < NS; ++s) fl(i, j, k, 5 + s) = o[5 + s];
-        fl(i, j, k, nv + 3) = o[nv + 3];
-    } else if (P.any_wall) {
-        fl(i, j, k, nv + 3) = o[nv + 3];
-    }
+    for (int n = 0; n < NS + 9; ++n) fl(i, j, k, n) = o[n];
 }
 
 // ---- diffusive face flux (flux.cpp:1193-1623) ----------------------------------------------
@@ -877,6 +874,29 @@ __global__ void __launch_bounds__(BLK) k_chem_noise(DevParams P, NoiseKey K, Arr
 
 // ---- noise --------------------------------------------------------------------------------
 
+// The four weighted-noise arrays of a stage in one launch: stochFlux[d] = 1*A + w*B on comps
+// 1..nvars-1 (comp 0 zeroed), rancorn likewise on its one component (MultiFab::LinComb).
+struct LinSet {
+    double* dst[4];
+    const double* a[4];
+    const double* b[4];
+    long npts[4];
+    int nc[4], c0[4];
+};
+
+__global__ void __launch_bounds__(BLK) k_lincomb4(LinSet L, double w)
+{
+    long t = (long)blockIdx.x * BLK + threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (t >= L.npts[q]) continue;
+        for (int n = 0; n < L.nc[q]; ++n) {
+            const long x = t + (long)n * L.npts[q];
+            L.dst[q][x] = (n < L.c0[q]) ? 0.0 : 1.0 * L.a[q][x] + w * L.b[q][x];
+        }
+    }
+}
+
 // stochFlux = 1*A + w*B on comps 1..nvars-1 (comp 0 zeroed), MultiFab::LinComb.
 __global__ void __launch_bounds__(BLK) k_lincomb(double* dst, const double* a, const double* b, long npts,
                                                  int nc, int c0, double w)
@@ -984,10 +1004,13 @@ int ref_fill_boundary(const DevParams& P, Arr a, int nc, hipStream_t s)
 int ref_set_bc(const DevParams& P, Arr prim, Arr cons, hipStream_t s)
 {
     if (!P.any_wall) return FHDX_OK;
+    const int g = P.ngc;
     for (int d = 0; d < 3; ++d) {
         if (P.periodic[d]) continue;
-        for (int side = 0; side < 2; ++side)
-            NS_DISPATCH(P.ns, (k_set_bc<NS><<<grid_for(ngrown(P), BLK), BLK, 0, s>>>(P, prim, cons, d, side)));
+        long slab = 2L * g;
+        for (int e = 0; e < 3; ++e)
+            if (e != d) slab *= P.n[e] + 2 * g;
+        NS_DISPATCH(P.ns, (k_set_bc<NS><<<grid_for(slab, BLK), BLK, 0, s>>>(P, prim, cons, d)));
     }
     FHDX_HIP_TRY(hipGetLastError());
     return FHDX_OK;
@@ -1036,11 +1059,14 @@ static void flux_passes(const DevParams& P, const FluxArgs& a, hipStream_t s)
     }
 }
 
-int ref_calculate_flux(const DevParams& P, const FluxArgs& a, hipStream_t s)
+int ref_calculate_flux(const DevParams& P, const FluxArgs& a, int flux_ncomp[3], hipStream_t s)
 {
     const int nc = P.nvars + 4;
     for (int d = 0; d < 3; ++d) {
-        int r = fill_value(a.flux[d].p, a.flux[d].sn * nc, 0.0, s);
+        // flux_in.setVal(0) (flux.cpp:24-26): the stochastic pass writes all nvars + 4
+        // components itself; zero first only without it or for extra components
+        if (P.stoch_on && flux_ncomp[d] == nc) continue;
+        int r = fill_value(a.flux[d].p, a.flux[d].sn * flux_ncomp[d], 0.0, s);
         if (r) return r;
     }
     NS_DISPATCH(P.ns, flux_passes<NS>(P, a, s));
@@ -1069,6 +1095,25 @@ int ref_chem_source(const DevParams& P, Arr prim, Arr source, Arr ranchem, doubl
 int ref_fill_chem_noise(const DevParams& P, const NoiseKey& K, Arr A, Arr B, hipStream_t s)
 {
     k_chem_noise<<<grid_for(ncell(P), BLK), BLK, 0, s>>>(P, K, A, B);
+    FHDX_HIP_TRY(hipGetLastError());
+    return FHDX_OK;
+}
+
+int ref_lincomb4(double* const dst[4], const double* const a[4], const double* const b[4], const long npts[4],
+                 const int nc[4], const int c0[4], double w, hipStream_t s)
+{
+    LinSet L;
+    long mx = 0;
+    for (int q = 0; q < 4; ++q) {
+        L.dst[q] = dst[q];
+        L.a[q] = a[q];
+        L.b[q] = b[q];
+        L.npts[q] = npts[q];
+        L.nc[q] = nc[q];
+        L.c0[q] = c0[q];
+        mx = npts[q] > mx ? npts[q] : mx;
+    }
+    k_lincomb4<<<grid_for(mx, BLK), BLK, 0, s>>>(L, w);
     FHDX_HIP_TRY(hipGetLastError());
     return FHDX_OK;
 }
