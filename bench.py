@@ -300,8 +300,9 @@ def main():
                     help="only run a plain 1 GiB device copy (calibrates HBM byte counters)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path "
+              "(boxes would shard by spatial slabs with 2-layer halo exchange); run with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch

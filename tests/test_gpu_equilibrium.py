@@ -10,7 +10,7 @@ values of rho, j_x, j_y, j_z and rho E (over cells and snapshots) must match
 with m = M k_B / R the molecular mass and T the mean temperature of the sampled states (the
 thermal kinetic energy the fluctuations take up lowers T by ~1/N of a cell's molecules).  These
 are the linearised (Gaussian) values; they hold when a cell holds many molecules: 1.4e7 argon
-atoms at dx = 8e-5 cm, 1.4e4 at the C1 cell size dx = 8e-6 cm.  At 8e-5 cm the bar is 1%.  At
+atoms at dx = 8e-5 cm, 1.4e4 at the 32^3 argon deck's dx = 8e-6 cm.  At 8e-5 cm the bar is 1%.  At
 8e-6 cm the CFL-0.1 step has a 10x larger diffusive number, and Var(rho E) then carries a
 time-step bias of the scheme, second order in dt: +1.1% at dt, +0.3% at dt/2, 0.0% at dt/4.
 That case runs at dt (held to 1.5%) and at dt/2 (held to 1%, with the rho E excess at least

@@ -264,7 +264,7 @@ def _multi_step(torch, fc, L, orc, cfg, nsteps, seed):
 
 
 def test_argon_32_cubed_100_steps(dev, orc):
-    """C1 shape: 32^3 two-identical-species argon, CFL 0.2, 100 steps, Philox noise generated
+    """The 100-step parity shape: 32^3 two-identical-species argon, CFL 0.2, Philox noise generated
     in-kernel on the GPU and injected into the oracle step by step."""
     torch, fc, L = dev
     cfg = P.argon_gas(32)

@@ -251,7 +251,7 @@ def test_deterministic_fluxes_are_cyclic_symmetric(orc, maker):
         got = st2.flux[new_d]
         diff = np.abs(got - expect).reshape(nv + 4, -1).max(axis=1)
         scale = np.abs(expect).reshape(nv + 4, -1).max(axis=1)
-        # Dufour sums cancel to rounding for identical species: judge them on the heat scale
+        # Dufour sums cancel to rounding for identical species: compare them on the heat scale
         scale[nv + 3] = max(scale[nv + 3], 1e-2 * scale[nv])
         quirk = (new_d == 0 or new_d == 2) and maker is P.four_species   # z faces before or after
         for c in range(nv + 4):
