@@ -139,6 +139,38 @@ def wall_rate(cfg, reps: int = 20):
             "runs": reps + 1, "runs_with_massfrac_flag": flagged}
 
 
+def ref_rate(n: int, steps: int, warmup: int = 1):
+    """cell-updates/s of the reference-shaped step (fhdx_rk3_step_ref, the literal RK3step
+    drop-in with the reference's own arrays) on periodic argon n^3 from the uniform start,
+    Philox noise."""
+    import torch
+    from fhdex_amd import compressible as fc
+    from fhdex_amd import ensemble as E
+    from fhdex_amd import lib as L
+    from fhdex_amd import params as P
+    cfg = P.argon_gas(n)
+    p = cfg.params
+    ctx = L.Context(p)
+    ds = fc.DeviceState(p)
+    ds.cu.t.copy_(torch.from_numpy(E.uniform_cons(cfg)))
+    fc.conservedToPrimitive(ctx, ds.prim, ds.cu)
+    fc.FillBoundary(ctx, ds.prim)
+    for s in range(warmup):
+        fc.RK3step(ctx, ds, cfg.dt, noise=None, seed=1, step=s)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        fc.RK3step(ctx, ds, cfg.dt, noise=None, seed=1, step=warmup + s)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ctx.check()
+    ctx.close()
+    del ds
+    return {"workload": f"argon two-species {n}^3 periodic, reference-shaped step (fhdx_rk3_step_ref)",
+            "value": n ** 3 * steps / dt, "unit": "cell-updates/s", "ms_per_step": dt / steps * 1e3,
+            "steps": steps}
+
+
 def measured_flops(n: int):
     """FP64 flops per cell-update from the committed rocprofv3 counter summary of this workload
     (profiles/fp64_argon<n>.json), or None."""
@@ -286,6 +318,9 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--loop-steps", type=int, default=5)
     ap.add_argument("--ensemble-only", action="store_true")
+    ap.add_argument("--ref-steps", type=int, default=3,
+                    help="steps of the reference-shaped step at --n (0: skip)")
+    ap.add_argument("--ref-only", action="store_true", help="only time the reference-shaped step")
     ap.add_argument("--serve-rates", type=str, default="50",
                     help="comma-separated arrival rates of the served stream ('' to skip)")
     ap.add_argument("--serve-requests", type=int, default=16)
@@ -336,6 +371,10 @@ def main():
     if a.ensemble_only:
         print(json.dumps({"ensemble": ensemble_bench(a.ensemble, a.ensemble_steps, a.loop_steps)}),
               flush=True)
+        return
+
+    if a.ref_only:
+        print(json.dumps({"reference_shaped": ref_rate(a.n, a.ref_steps, a.warmup)}), flush=True)
         return
 
     from fhdex_amd import compressible as fc
@@ -426,6 +465,8 @@ def main():
                            config_rate(P.no2_n2o4(128), 10),
                            wall_rate(P.wall_deck("slip_wall_adiabatic")),
                            wall_rate(P.wall_deck("temp_grad"))]
+    if a.ref_steps:
+        line["reference_shaped"] = ref_rate(a.n, a.ref_steps)
     if a.ensemble:
         line["ensemble"] = ensemble_bench(a.ensemble, a.ensemble_steps, a.loop_steps)
     rates = [float(v) for v in a.serve_rates.split(",") if v]
