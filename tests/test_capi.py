@@ -141,5 +141,7 @@ def test_params_default_finalize_check():
 
 def test_config_params_pass_check():
     lib = L.lib()
-    for cfg in (P.argon_gas(32), P.four_species(16), P.species_prefix(3, 16, 10)):
-        assert lib.fhdx_params_check(C.byref(cfg.params)) == 0
+    for cfg in (P.argon_gas(32), P.four_species(16), P.species_prefix(3, 16, 10), P.no2_n2o4(16),
+                P.wall_deck("slip_wall_adiabatic"), P.wall_deck("noslip_reservoir_isothermal"),
+                P.wall_deck("temp_grad")):
+        assert lib.fhdx_params_check(C.byref(cfg.params)) == 0, cfg.name

@@ -172,6 +172,18 @@ def test_rk3_steps_match_oracle(dev, orc, name, mk):
     assert np.array_equal(got_prim, st.prim)
 
 
+def test_set_bc_rejects_bad_shapes(dev):
+    torch, fc, L = dev
+    p = P.wall_deck("slip_wall_adiabatic", 8).params
+    ctx = L.Context(p)
+    prim = fc.MultiFab.cell(p, p.nprimvars)
+    cu = fc.MultiFab.cell(p, p.nvars)
+    with pytest.raises(L.FhdxError) as e:                # prim and cons swapped
+        fc.setBC(ctx, cu, prim)
+    assert e.value.status == 1
+    fc.setBC(ctx, prim, cu)                               # the right shapes pass
+
+
 def test_fused_paths_refuse_walls(dev, orc):
     torch, fc, L = dev
     cfg = P.wall_deck("slip_wall_adiabatic", 8)
