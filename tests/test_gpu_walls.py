@@ -371,3 +371,38 @@ def test_chemistry_between_walls_matches_oracle(dev, orc):
     same = np.array_equal(ds.cu.t.cpu().numpy(), st.cu) and np.array_equal(ds.prim.t.cpu().numpy(), st.prim)
     say(f"  exact={same}")
     assert same
+
+
+def test_closed_walls_keep_totals_at_128_cubed(dev):
+    """Argon (four identical species, 8e-6 cm cells, ~3400 atoms of each per cell) at 128^3
+    between slip adiabatic species walls on every face, 50 noisy reference-shaped steps at CFL
+    0.1: no mass or species crosses a closed wall, so Sigma rho and Sigma rho Y_k stay at their
+    start values to rounding (<= 1e-13 of Sigma|.|)."""
+    torch, fc, L = dev
+    from fhdex_amd import ensemble as E
+    cfg = P.wall_deck("slip_wall_adiabatic")
+    cfg.params = P.make_params(4, (128, 128, 128), 8e-6, [39.948] * 4, [3.66e-8] * 4)
+    P.set_walls(cfg.params, 1, 1, 1)
+    cfg.dt = P.cfl_dt(cfg.params, cfg.rhobar, cfg.T_init, 0.1)
+    p = cfg.params
+    g = p.ngc
+    say("case closed walls 128^3 x 50")
+    ctx = L.Context(p)
+    ds = fc.DeviceState(p)
+    ds.cu.t.copy_(torch.from_numpy(E.uniform_cons(cfg)))
+    fc.conservedToPrimitive(ctx, ds.prim, ds.cu)
+    fc.FillBoundary(ctx, ds.cu)
+    fc.FillBoundary(ctx, ds.prim)
+    fc.setBC(ctx, ds.prim, ds.cu)
+    v = lambda: ds.cu.t[:, g:-g, g:-g, g:-g]
+    s0 = v().sum(dim=(1, 2, 3)).cpu().numpy()
+    for step in range(50):
+        fc.RK3step(ctx, ds, cfg.dt, noise=None, seed=3, step=step)
+    ctx.check()
+    s1 = v().sum(dim=(1, 2, 3)).cpu().numpy()
+    a1 = v().abs().sum(dim=(1, 2, 3)).cpu().numpy()
+    assert bool(torch.isfinite(ds.cu.t).all())
+    for c in [0] + list(range(5, 5 + p.nspecies)):
+        drift = abs(s1[c] - s0[c]) / a1[c]
+        say(f"  component {c}: drift {drift:.2e} of sum|.|")
+        assert drift <= 1e-13, c
