@@ -145,3 +145,35 @@ def test_config_params_pass_check():
                 P.wall_deck("slip_wall_adiabatic"), P.wall_deck("noslip_reservoir_isothermal"),
                 P.wall_deck("temp_grad")):
         assert lib.fhdx_params_check(C.byref(cfg.params)) == 0, cfg.name
+
+
+def test_wall_helpers_match_the_c_finalize_bit_for_bit():
+    """params.set_walls computes the wall mole fractions (GetMolfrac) and the reservoir density
+    (GetDensity) with the C library's operation order: fhdx_params_finalize, given only the
+    mass fractions and p, T, reproduces them exactly."""
+    lib = L.lib()
+    p = P.make_params(3, (8, 8, 8), 8e-6, [4.0026, 20.1797, 39.9480], [2.18e-8, 2.58e-8, 3.63e-8])
+    P.set_walls(p, [(2, 2), (1, 1), (-1, -1)], [(2, 2), (1, 1), (-1, -1)], [(3, 2), (2, 1), (-1, -1)],
+                t_lo=(310.0, 305.0, 0.0), t_hi=(295.0, 0.0, 0.0),
+                bc_Yk=[[0.5, 0.3, 0.2], [0.2, 0.3, 0.5], [0.1, 0.6, 0.3], [0] * 3, [0] * 3, [0] * 3],
+                p_lo=(1.1e6, 0.0, 0.0))
+    q = p.copy()
+    for f in range(6):
+        for n in range(P.MAX_SPECIES):
+            q.bc_Xk[f][n] = 0.0
+    q.rho_lo[0] = 0.0
+    lib.fhdx_params_finalize(C.byref(q))
+    for f in range(3):
+        assert list(q.bc_Xk[f]) == list(p.bc_Xk[f]), f
+    assert q.rho_lo[0] == p.rho_lo[0] > 0
+    assert lib.fhdx_params_check(C.byref(p)) == 0
+
+
+def test_ambient_pressure_is_the_equation_of_state(orc):
+    cfg = P.four_species(4)
+    p = cfg.params
+    st = orc.HostState(p)
+    orc.init_cons(p, st.cu, cfg.rho0, cfg.rhobar, cfg.T_init)
+    orc.cons_to_prim(p, st.prim, st.cu)
+    g = p.ngc
+    assert P.ambient_pressure(p, cfg.rho0, cfg.rhobar, cfg.T_init) == pytest.approx(st.prim[5, g, g, g], rel=1e-14)
