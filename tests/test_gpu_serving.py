@@ -118,13 +118,16 @@ def test_requests_that_cannot_run_are_rejected(dev, orc):
     srv = S.Server(2, 2 * E.bytes_needed(small.params), chunk_steps=4)
     odd = P.argon_gas(8)
     odd.params.ngc = 1            # stencils need two ghost layers
+    walled = P.argon_gas(8)       # walls are served by the reference-shaped step, not batched
+    P.set_walls(walled.params, 1, 1, 1)
     ids = {srv.submit(big, 1, 3): "big", srv.submit(odd, 2, 3): "odd",
-           srv.submit(small, 3, 3): "ok"}
+           srv.submit(walled, 4, 3): "walls", srv.submit(small, 3, 3): "ok"}
     assert srv.wait(60.0) == 0
     done = {ids[c["id"]]: c for c in srv.poll()}
     st = srv.stats()
     srv.close()
     assert done["big"]["status"] == 5 and done["big"]["admit_ns"] == 0
     assert done["odd"]["status"] == 2 and "ngc" in done["odd"]["message"]
+    assert done["walls"]["status"] == 2 and "periodic" in done["walls"]["message"]
     assert done["ok"]["status"] == 0 and done["ok"]["steps"] == 3
     assert st["steps"] == 3 and st["cell_updates"] == 3 * 8 ** 3 and st["bytes_used"] == 0
