@@ -97,11 +97,11 @@ def config_rate(cfg, steps: int, warmup: int = 2):
 
 def wall_rate(cfg, reps: int = 20):
     """cell-updates/s of the reference-shaped step (the path that serves walls) on a wall deck,
-    run as the reference runs it: max_step (3) noisy steps from the uniform start with ghosts
-    set by setBC, repeated `reps` times with seeds 1.. ; only the steps are timed.  The decks
-    hold ~50 molecules per species per cell, so now and then a stage leaves a mass fraction
-    outside (0,1), where the reference aborts (transCoeffs.cpp:45-46): such runs are counted
-    and reported, not raised."""
+    run as the reference runs it: max_step (3) noisy steps from InitConsVar's start with ghosts
+    set by setBC, repeated `reps` times with seeds 1.. ; only the steps are timed.  The argon
+    decks hold ~50 molecules per species per cell, so now and then a stage leaves a mass
+    fraction outside (0,1), where the reference aborts (transCoeffs.cpp:45-46): such runs are
+    counted and reported, not raised."""
     import torch
     from fhdex_amd import compressible as fc
     from fhdex_amd import ensemble as E
@@ -109,7 +109,7 @@ def wall_rate(cfg, reps: int = 20):
     p = cfg.params
     ctx = L.Context(p)
     ds = fc.DeviceState(p)
-    cu0 = torch.from_numpy(E.uniform_cons(cfg)).cuda()
+    cu0 = torch.from_numpy(E.initial_cons(cfg)).cuda()
     total = 0.0
     flagged = 0
     for r in range(reps + 1):
@@ -464,7 +464,8 @@ def main():
         line["configs"] = [config_rate(P.argon_gas(32), 50), config_rate(P.four_species(128), 10),
                            config_rate(P.no2_n2o4(128), 10),
                            wall_rate(P.wall_deck("slip_wall_adiabatic")),
-                           wall_rate(P.wall_deck("temp_grad"))]
+                           wall_rate(P.wall_deck("temp_grad")), wall_rate(P.wall_deck("RT_3d")),
+                           wall_rate(P.wall_deck("giantfluct_3d"))]
     if a.ref_steps:
         line["reference_shaped"] = ref_rate(a.n, a.ref_steps)
     if a.ensemble:

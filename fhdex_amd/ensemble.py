@@ -42,6 +42,54 @@ def uniform_cons(cfg: Config) -> np.ndarray:
     return cu
 
 
+def initial_cons(cfg: Config) -> np.ndarray:
+    """InitConsVar's conserved start (compressible_functions.cpp:103-413) on the grown box, as a
+    host array, for cfg.prob_type 1 (uniform), 2 (Rayleigh-Taylor layers, :172-214: four
+    species, mass fractions 0.4/0.4/0.1/0.1 above the mid-plane in z and 0.1/0.1/0.4/0.4 below,
+    density rho0 exp(g_z z / (R_mix T)) at hydrostatic equilibrium) or 3 (diffusion barrier,
+    :216-227: mass fractions linear in y between the y walls' compositions).  The domain is
+    [0, n dx].  For types 2 and 3 the ghost cells keep the uniform values: fill them with
+    fill_boundary / setBC, as the reference does after InitConsVar (main_driver.cpp:532-542)."""
+    cu = uniform_cons(cfg)
+    if cfg.prob_type == 1:
+        return cu
+    p = cfg.params
+    ns = p.nspecies
+    g = p.ngc
+    n0, n1, n2 = p.n_cells
+    v = (slice(None), slice(g, g + n2), slice(g, g + n1), slice(g, g + n0))
+    rho = np.full((n2, n1, n0), cfg.rho0)
+    if cfg.prob_type == 2:
+        if ns != 4:
+            raise ValueError("prob_type 2 (Rayleigh-Taylor) is a four-species start")
+        posz = ((np.arange(n2) + 0.5) * p.dx[2])[:, None, None]
+        top = posz - n2 * p.dx[2] / 2.0 >= 0.0
+        Y = np.empty((ns, n2, 1, 1))
+        for l, (hi, lo) in enumerate(((0.4, 0.1), (0.4, 0.1), (0.1, 0.4), (0.1, 0.4))):
+            Y[l] = np.where(top, hi, lo)
+        molmix = 1.0 / sum(Y[l] / p.molmass[l] for l in range(ns))
+        pamb = rho * (p.Runiv / molmix) * cfg.T_init
+        rgasmix = p.Runiv / molmix
+        alpha = p.grav[2] / (rgasmix * cfg.T_init)
+        rho = pamb * np.exp(alpha * posz) / (rgasmix * cfg.T_init)
+    elif cfg.prob_type == 3:
+        posy = ((np.arange(n1) + 0.5) * p.dx[1])[None, :, None]
+        Ly = n1 * p.dx[1]
+        Y = np.empty((ns, 1, n1, 1))
+        for l in range(ns):
+            Y[l] = (p.bc_Yk[3][l] - p.bc_Yk[2][l]) / Ly * posy + p.bc_Yk[2][l]
+    else:
+        raise ValueError(f"prob_type {cfg.prob_type} is not built (types 1, 2 and 3 are)")
+    cv = sum(Y[l] * p.hcv[l] for l in range(ns))
+    e0 = sum(Y[l] * p.e0[l] for l in range(ns))
+    cu[v][0] = rho
+    cu[v][1:4] = 0.0
+    cu[v][4] = rho * (e0 + cfg.T_init * cv)
+    for l in range(ns):
+        cu[v][5 + l] = rho * Y[l]
+    return cu
+
+
 def bytes_needed(p: Params) -> int:
     return int(lib().fhdx_engine_bytes_needed(C.byref(p)))
 

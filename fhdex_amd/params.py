@@ -178,11 +178,13 @@ def make_params(nspecies: int, n_cells, dx, molmass, diameter, *, ngc: int = 2,
 
 
 class Config:
-    """A run: parameters plus the uniform initial state and time step."""
+    """A run: parameters plus the initial state (rho0, mass fractions rhobar, T_init and the
+    InitConsVar prob_type) and time step."""
 
-    def __init__(self, name, params: Params, rho0, rhobar, T_init, dt, nsteps, chem=None):
+    def __init__(self, name, params: Params, rho0, rhobar, T_init, dt, nsteps, chem=None, prob_type=1):
         self.name = name
         self.chem = chem
+        self.prob_type = prob_type  # InitConsVar's initial state (compressible_functions.cpp:103-413)
         self.params = params
         self.rho0 = rho0
         self.rhobar = list(rhobar)
@@ -353,8 +355,10 @@ def ambient_pressure(p: Params, rho0, rhobar, T) -> float:
 
 
 def wall_deck(name, n=None, nsteps=3):
-    """The reference's wall regression decks (exec/compressible/inputs_regression_*): four
-    identical argon species (M 39.948, d 3.66e-8 cm, Y 1/4 each), rho 1.78e-3, T_init 400 K.
+    """The reference's wall decks (exec/compressible/inputs_regression_*, inputs_giantfluct_3d).
+
+    The first three hold four identical argon species (M 39.948, d 3.66e-8 cm, Y 1/4 each),
+    rho 1.78e-3, T_init 400 K, and start uniform (prob_type 1):
 
     * slip_wall_adiabatic: 32^3 cells of 6.25e-5/32 cm, dt 2e-12 s, every face a slip,
       adiabatic, species-wall face (codes 1).
@@ -363,8 +367,18 @@ def wall_deck(name, n=None, nsteps=3):
     * temp_grad: 80 x 10 x 10 cells of 2e-6 cm, dt 1e-12 s, advection_type 1; x: no-slip
       isothermal walls at 280.8 / 508.5 K, y and z: slip adiabatic walls; species walls.
 
-    n rescales the box edge (cells per direction, the x count of temp_grad) at the deck's cell
-    size, for runs small enough for the oracle."""
+    * RT_3d (inputs_regression_RT_3d): 64 x 32 x 32 cells of 1.2e-5 cm, dt 5e-12 s, four
+      species (M 2/20/2/20, d 2/20/20/2e-8 cm), rho0 4.83e-4, T 300 K, gravity -4e12 cm/s^2 in z,
+      slip adiabatic species walls in z; prob_type 2 (two layers at hydrostatic equilibrium).
+    * giantfluct_3d (inputs_giantfluct_3d): 256 x 128 cells of 2.7e-5 cm, dt 1e-10 s, three
+      argon-mass species (d 3.63/3.63/36.3e-8 cm), rho0 4.83e-4, T 300 K, no-slip isothermal
+      (300 K) walls of fixed composition in y; prob_type 3 (mass fractions linear between the
+      two wall compositions).  The deck is one cell deep in z; this build needs at least 4
+      cells per direction, so the box is 4 cells deep.
+
+    n rescales the box (cells per direction; the x count of temp_grad; the y count of RT_3d
+    and giantfluct_3d, whose x count is 2 n) at the deck's cell size, for runs small enough
+    for the oracle."""
     M = [39.9480] * 4
     dmol = [3.66e-8] * 4
     if name == "slip_wall_adiabatic":
@@ -383,6 +397,24 @@ def wall_deck(name, n=None, nsteps=3):
         set_walls(p, [(2, 2), (1, 1), (1, 1)], [(2, 2), (1, 1), (1, 1)], 1,
                   t_lo=(280.8, 300.0, 300.0), t_hi=(508.5, 400.0, 400.0))
         dt = 1e-12
+    elif name == "RT_3d":
+        m = n or 32
+        cells = (2 * m, m, m)
+        p = make_params(4, cells, 7.68e-4 / 64, [2.0, 20.0, 2.0, 20.0], [2.0e-8, 20.0e-8, 20.0e-8, 2.0e-8],
+                        grav=(0.0, 0.0, -4.0e12))
+        set_walls(p, [(-1, -1), (-1, -1), (1, 1)], 1, 1, t_lo=(300.0,) * 3, t_hi=(300.0,) * 3)
+        return Config(f"{name}_{'x'.join(map(str, cells))}", p, 4.83e-4, [0.25] * 4, 300.0, 5.0e-12, nsteps,
+                      prob_type=2)
+    elif name == "giantfluct_3d":
+        m = n or 128
+        cells = (2 * m, m, 4)
+        p = make_params(3, cells, (6.912e-3 / 256, 3.456e-3 / 128, 2.7e-5), [39.9480] * 3,
+                        [3.63e-8, 3.63e-8, 36.3e-8])
+        set_walls(p, [(-1, -1), (2, 2), (-1, -1)], 2, 2, t_lo=(300.0,) * 3, t_hi=(300.0,) * 3,
+                  bc_Yk=[[0.0] * 3, [0.0] * 3, [0.2, 0.09316672, 0.70683328],
+                         [0.3, 0.40683328, 0.29316672], [0.0] * 3, [0.0] * 3])
+        return Config(f"{name}_{'x'.join(map(str, cells))}", p, 4.83e-4, [0.25, 0.25, 0.5], 300.0, 1.0e-10,
+                      nsteps, prob_type=3)
     else:
         raise ValueError(name)
     return Config(f"{name}_{'x'.join(map(str, cells))}", p, 1.78e-3, [0.25] * 4, 400.0, dt, nsteps)

@@ -1623,6 +1623,51 @@ int orc_init_cons(const fhdx_params* P, fhdx_fab cu_, int prob_type, double rho0
                     for (int l = 0; l < NS; ++l) AT(cu, i, j, k, 5 + l) = AT(cu, i, j, k, 0) * Yk[l];
                 }
         orc_fill_boundary(P, cu_);
+    } else if (prob_type == 2 || prob_type == 3) {
+        /* Valid cells only (the ParallelFor over the tile box); ghost cells keep the uniform
+         * values until FillBoundary / setBC.  The domain is [0, n dx] (prob_lo = 0). */
+        if (prob_type == 2 && NS != 4) return FHDX_ERR_UNSUPPORTED;
+        const double zc = P->n_cells[2] * P->dx[2] / 2.;
+        const double Ly = P->n_cells[1] * P->dx[1];
+        for (int k = 0; k < P->n_cells[2]; ++k)
+            for (int j = 0; j < P->n_cells[1]; ++j)
+                for (int i = 0; i < P->n_cells[0]; ++i) {
+                    const double posy = (j + 0.5) * P->dx[1];
+                    const double posz = (k + 0.5) * P->dx[2];
+                    double mv[MS] = {0};
+                    if (prob_type == 2) {
+                        /* Rayleigh-Taylor: two layers of equal mean molar mass, density
+                         * exponential in z at hydrostatic equilibrium (:172-214) */
+                        const int top = posz - zc >= 0.;
+                        mv[0] = top ? 0.4 : 0.1;
+                        mv[1] = top ? 0.4 : 0.1;
+                        mv[2] = top ? 0.1 : 0.4;
+                        mv[3] = top ? 0.1 : 0.4;
+                        double molmix = 0.;
+                        for (int l = 0; l < NS; ++l) molmix += mv[l] / P->molmass[l];
+                        molmix = 1. / molmix;
+                        const double pamb = AT(cu, i, j, k, 0) * (P->Runiv / molmix) * T_init;
+                        molmix = 0.;
+                        for (int l = 0; l < NS; ++l) molmix = molmix + mv[l] / P->molmass[l];
+                        molmix = 1.0 / molmix;
+                        const double rgasmix = P->Runiv / molmix;
+                        const double alpha = P->grav[2] / (rgasmix * T_init);
+                        AT(cu, i, j, k, 0) = pamb * exp(alpha * posz) / (rgasmix * T_init);
+                        for (int l = 0; l < NS; ++l) AT(cu, i, j, k, 5 + l) = AT(cu, i, j, k, 0) * mv[l];
+                    } else {
+                        /* diffusion barrier: mass fractions linear in y between the y walls'
+                         * compositions (:216-227) */
+                        for (int l = 0; l < NS; ++l) {
+                            const double Ygrad = (P->bc_Yk[3][l] - P->bc_Yk[2][l]) / Ly;
+                            mv[l] = Ygrad * posy + P->bc_Yk[2][l];
+                            AT(cu, i, j, k, 5 + l) = AT(cu, i, j, k, 0) * mv[l];
+                        }
+                    }
+                    const double e = get_energy(P, mv, T_init);
+                    const double r = AT(cu, i, j, k, 0), m1 = AT(cu, i, j, k, 1),
+                                 m2 = AT(cu, i, j, k, 2), m3 = AT(cu, i, j, k, 3);
+                    AT(cu, i, j, k, 4) = r * e + 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) / r;
+                }
     } else if (prob_type != 1) {
         return FHDX_ERR_UNSUPPORTED;
     }

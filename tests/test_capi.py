@@ -143,7 +143,7 @@ def test_config_params_pass_check():
     lib = L.lib()
     for cfg in (P.argon_gas(32), P.four_species(16), P.species_prefix(3, 16, 10), P.no2_n2o4(16),
                 P.wall_deck("slip_wall_adiabatic"), P.wall_deck("noslip_reservoir_isothermal"),
-                P.wall_deck("temp_grad")):
+                P.wall_deck("temp_grad"), P.wall_deck("RT_3d"), P.wall_deck("giantfluct_3d")):
         assert lib.fhdx_params_check(C.byref(cfg.params)) == 0, cfg.name
 
 

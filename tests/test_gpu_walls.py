@@ -1,8 +1,9 @@
 """Walls on the GPU against the oracle, through the C ABI: setBC ghosts, the wall-aware noise,
 calculateFlux (one-sided face branches, StochFlux, BCWallSpeciesFlux) with injected noise, and
-RK3 steps of the reference-shaped path with Philox noise, on the reference's wall regression
-decks (exec/compressible/inputs_regression_{slip_wall_adiabatic, noslip_reservoir_isothermal,
-temp_grad}) at reduced size, a bulk-viscosity variant and a box with walls in x only.  The GPU
+RK3 steps of the reference-shaped path with Philox noise, on the reference's wall decks
+(exec/compressible/inputs_regression_{slip_wall_adiabatic, noslip_reservoir_isothermal,
+temp_grad, RT_3d}, inputs_giantfluct_3d) at reduced and at full size, a bulk-viscosity
+variant, a box with walls in x only and one with x reservoirs.  The GPU
 reads the wall-side cell for both sides of a wall face where the oracle spells out the doubled
 terms; every comparison is exact.  The fused step refuses walls with FHDX_ERR_UNSUPPORTED.
 """
@@ -53,6 +54,8 @@ CASES = [
     ("slip_wall_adiabatic", lambda: P.wall_deck("slip_wall_adiabatic", 8)),
     ("noslip_reservoir_isothermal", lambda: P.wall_deck("noslip_reservoir_isothermal", 8)),
     ("temp_grad", lambda: P.wall_deck("temp_grad", 16)),
+    ("RT_3d", lambda: P.wall_deck("RT_3d", 8)),
+    ("giantfluct_3d", lambda: P.wall_deck("giantfluct_3d", 8)),
     ("visc_type_3", visc3_walls),
     ("x_walls", x_walls),
     ("x_reservoirs", x_reservoirs),
@@ -270,18 +273,21 @@ def test_wall_layers_fluctuate_like_the_bulk(dev, family):
             assert np.all(np.abs(walls - 1) < 0.08), walls
 
 
-@pytest.mark.parametrize("name", ["slip_wall_adiabatic", "noslip_reservoir_isothermal", "temp_grad"])
+@pytest.mark.parametrize("name", ["slip_wall_adiabatic", "noslip_reservoir_isothermal", "temp_grad", "RT_3d",
+                                  "giantfluct_3d"])
 def test_wall_decks_at_full_size_match_oracle(dev, orc, name):
-    """The regression decks at their own sizes (32^3; 80x10x10) and step counts (max_step 3),
-    from the uniform start (prob_type 1), Philox noise: GPU equals the oracle bit for bit, and
-    closed species walls keep every species total."""
+    """The wall decks at their own sizes (32^3; 80x10x10; 64x32x32; 256x128 cells, 4 deep)
+    for 3 steps (the regression decks' max_step), from InitConsVar's start (uniform; the
+    Rayleigh-Taylor layers with gravity; the diffusion barrier), Philox noise: GPU equals the
+    oracle bit for bit, total mass is kept, and where every wall is a species wall so is every
+    species total."""
     torch, fc, L = dev
     from fhdex_amd import ensemble as E
     cfg = P.wall_deck(name)
     p = cfg.params
     g = p.ngc
     say(f"case wall deck {name} {tuple(p.n_cells)} x {cfg.nsteps}")
-    cu0 = E.uniform_cons(cfg)
+    cu0 = E.initial_cons(cfg)
     ctx = L.Context(p)
     ds = fc.DeviceState(p)
     ds.cu.t.copy_(torch.from_numpy(cu0))
@@ -300,7 +306,7 @@ def test_wall_decks_at_full_size_match_oracle(dev, orc, name):
     assert np.array_equal(got, cu) and np.array_equal(ds.prim.t.cpu().numpy(), prim)
     s0, s1 = valid(p, cu0).sum(axis=(1, 2, 3)), valid(p, got).sum(axis=(1, 2, 3))
     assert abs(s1[0] - s0[0]) <= 1e-14 * s0[0]
-    if p.bc_mass_lo[0] == 1:
+    if all(c in (-1, 1) for c in list(p.bc_mass_lo) + list(p.bc_mass_hi)):
         for s in range(p.nspecies):
             assert abs(s1[5 + s] - s0[5 + s]) <= 1e-14 * s0[5 + s]
 

@@ -245,3 +245,105 @@ def test_gas_at_the_reservoir_state_stays_uniform_and_higher_pressure_fills_the_
     # the inflow is symmetric: x momentum is antisymmetric about the centre
     jx = valid(p, cu)[1]
     assert np.abs(jx + np.flip(jx, axis=2)).max() <= 1e-12 * np.abs(jx).max()
+
+
+def deck_state(orc, cfg):
+    """InitConsVar's start of a deck, then conservedToPrimitive, FillBoundary and setBC
+    (main_driver.cpp:532-542)."""
+    p = cfg.params
+    cu = E.initial_cons(cfg)
+    prim = np.zeros((p.nprimvars,) + cu.shape[1:])
+    orc.fill_boundary(p, cu)
+    orc.cons_to_prim(p, prim, cu)
+    orc.fill_boundary(p, prim)
+    orc.set_bc(p, prim, cu)
+    return cu, prim
+
+
+@pytest.mark.parametrize("name", ["RT_3d", "giantfluct_3d"])
+def test_initial_states_of_the_rt_and_diffusion_barrier_decks(orc, name):
+    """The host start (ensemble.initial_cons) equals the oracle's InitConsVar bit for bit, and
+    each is the state its deck describes, checked from its definition:
+
+    * RT_3d: the discrete hydrostatic balance (p_{k+1} - p_{k-1}) / (2 dz) = rho_k g_z holds to
+      the central difference's O((dz/H)^2) (~1e-5; a wrong sign of g or a wrong mixture molar
+      mass would leave O(1)); 0.4/0.4/0.1/0.1 above the mid-plane, 0.1/0.1/0.4/0.4 below;
+    * giantfluct_3d: the mass fractions sum to 1, are linear in y, and their linear
+      extrapolation to y = 0 and y = L_y is the lo and hi walls' composition;
+    * both: at rest at T_init (the primitive temperature recovered from the conserved state)."""
+    cfg = P.wall_deck(name, 8)
+    p = cfg.params
+    NS, g = p.nspecies, p.ngc
+    st = orc.HostState(p)
+    assert orc.init_cons(p, st.cu, cfg.rho0, cfg.rhobar, cfg.T_init, cfg.prob_type) == 0
+    assert np.array_equal(E.initial_cons(cfg), st.cu)
+    cu, prim = deck_state(orc, cfg)
+    pr = valid(p, prim)
+    assert np.all(pr[1:4] == 0.0)
+    assert np.allclose(pr[4], cfg.T_init, rtol=1e-13, atol=0)
+    Y = pr[6:6 + NS]
+    assert np.allclose(Y.sum(axis=0), 1.0, rtol=0, atol=1e-15)
+    if name == "RT_3d":
+        bal = (pr[5][2:] - pr[5][:-2]) / (2 * p.dx[2]) - pr[0][1:-1] * p.grav[2]
+        assert np.abs(bal).max() <= 2e-5 * np.abs(pr[0] * p.grav[2]).max()
+        nz = p.n_cells[2]
+        assert np.allclose(Y[:, nz // 2:], np.array([0.4, 0.4, 0.1, 0.1])[:, None, None, None], rtol=1e-15)
+        assert np.allclose(Y[:, :nz // 2], np.array([0.1, 0.1, 0.4, 0.4])[:, None, None, None], rtol=1e-15)
+    else:
+        n1, L = p.n_cells[1], p.n_cells[1] * p.dx[1]
+        y = (np.arange(n1) + 0.5) * p.dx[1]
+        for s in range(NS):
+            prof = Y[s, 0, :, 0]
+            slope, icpt = np.polyfit(y, prof, 1)
+            assert np.abs(np.polyval([slope, icpt], y) - prof).max() <= 1e-14
+            assert icpt == pytest.approx(p.bc_Yk[2][s], abs=1e-13)
+            assert slope * L + icpt == pytest.approx(p.bc_Yk[3][s], abs=1e-13)
+
+
+def test_rt_layers_hold_hydrostatic_rest_under_gravity(orc):
+    """RT_3d without noise, one step from its start: away from the walls and the mid-plane the
+    pressure gradient and gravity cancel, so the z momentum changes by at most 1e-3 of
+    rho g dt (about 6e-5 is seen; gravity of the wrong sign, or missing, gives about 2 or 1).
+    Next to the walls the half-cell mirror pressure leaves half of rho g dt, and at the
+    mid-plane the species begin to interdiffuse: both do move."""
+    cfg = P.wall_deck("RT_3d", 16)
+    p = cfg.params
+    p.stoch_stress_form = 0
+    cu, prim = deck_state(orc, cfg)
+    u0 = valid(p, cu).copy()
+    assert orc.advance(p, cu, prim, 1, cfg.dt) == 0
+    u1 = valid(p, cu)
+    ref = np.abs(u0[0] * p.grav[2] * cfg.dt).max()
+    by_layer = np.abs(u1[3] - u0[3]).max(axis=(1, 2)) / ref
+    nz = p.n_cells[2]
+    quiet = [k for k in range(nz) if 2 <= k < nz - 2 and abs(k + 0.5 - nz / 2) > 3]
+    assert by_layer[quiet].max() <= 1e-3, by_layer
+    assert by_layer[0] > 0.2 and by_layer[-1] > 0.2
+    assert by_layer[nz // 2] > 1e-3
+    assert np.all(u1[1:3] == 0.0)                       # nothing drives x or y momentum
+
+
+def test_linear_composition_between_fixed_composition_walls_is_steady(orc):
+    """giantfluct_3d with three identical species (one diffusion coefficient) and no noise:
+    the linear mass-fraction profile between the two wall compositions carries a uniform
+    diffusive flux, into the box at one wall and out at the other, so nothing changes, wall
+    cells included (the fixed-composition ghost is the wall value, the wall gradient spans half
+    a cell).  Control: with species walls (mirrored composition, no flux) the wall cells do
+    change."""
+    for code, steady in ((2, True), (1, False)):
+        cfg = P.wall_deck("giantfluct_3d", 8)
+        p = cfg.params
+        p.diameter[2] = p.diameter[0]
+        p.stoch_stress_form = 0
+        p.bc_mass_lo[1] = p.bc_mass_hi[1] = code
+        cu, prim = deck_state(orc, cfg)
+        u0 = valid(p, cu).copy()
+        assert orc.advance(p, cu, prim, 3, cfg.dt) == 0
+        u1 = valid(p, cu)
+        worst = max(np.abs(u1[c] - u0[c]).max() / np.abs(u0[c]).max() for c in (0, 4, 5, 6, 7))
+        if steady:
+            assert worst <= 1e-14, worst
+            assert np.abs(u1[1:4]).max() <= 1e-12
+        else:
+            wall = np.abs(u1[5, :, [0, -1], :] - u0[5, :, [0, -1], :]).max() / np.abs(u0[5]).max()
+            assert wall > 1e-6, wall
