@@ -36,15 +36,15 @@ def measured_traffic(n: int):
         return json.load(f)["bytes_per_step"]
 
 
-def cpu_baseline(n: int, steps: int, n1: int = 32, steps1: int = 2):
+def cpu_baseline(n: int, steps: int, n1: int = 32, steps1: int = 2, n4: int = 64, steps4: int = 4):
     """The CPU oracle (the reference's arithmetic in the reference's operation order, OpenMP over
-    z planes) on the host cores: all threads on argon n^3 x steps, and one thread on argon
-    n1^3 x steps1."""
+    z planes) on the host cores: all threads on argon n^3 x steps, one thread on argon
+    n1^3 x steps1, and all threads on the four-species gas n4^3 x steps4."""
     from fhdex_amd import params as P
     from oracle import ffi as orc
 
-    def run(nn, ss):
-        cfg = P.argon_gas(nn)
+    def run(nn, ss, gas=P.argon_gas):
+        cfg = gas(nn)
         p = cfg.params
         st = orc.HostState(p)
         orc.init_cons(p, st.cu, cfg.rho0, cfg.rhobar, cfg.T_init)
@@ -59,11 +59,14 @@ def cpu_baseline(n: int, steps: int, n1: int = 32, steps1: int = 2):
     orc.set_num_threads(1)
     v1, t1 = run(n1, steps1)
     orc.set_num_threads(threads)
+    v4, t4 = run(n4, steps4, P.four_species)
     return {"value": v, "unit": "cell-updates/s", "cores": threads, "kind": "port",
             "sample": f"oracle (CPU restatement of the reference step, gcc -O3, OpenMP), argon {n}^3 x "
                       f"{steps} RK3 steps on {threads} threads, {t:.1f} s",
             "value_1thread": v1,
-            "sample_1thread": f"argon {n1}^3 x {steps1} RK3 steps on 1 thread, {t1:.1f} s"}
+            "sample_1thread": f"argon {n1}^3 x {steps1} RK3 steps on 1 thread, {t1:.1f} s",
+            "value_species4": v4,
+            "sample_species4": f"four species {n4}^3 x {steps4} RK3 steps on {threads} threads, {t4:.1f} s"}
 
 
 def config_rate(cfg, steps: int, warmup: int = 2):
