@@ -156,9 +156,9 @@ int fhdx_params_check(const fhdx_params* p)
         return FHDX_ERR_UNSUPPORTED;
     }
     for (int d = 0; d < 3; ++d) {
-        if (p->n_cells[d] < 4) {
-            set_error("n_cells must be >= 4 in every direction (3-D only)");
-            return FHDX_ERR_UNSUPPORTED;
+        if (p->n_cells[d] < 1) {
+            set_error("n_cells must be >= 1 in every direction");
+            return FHDX_ERR_ARG;
         }
         if (!(p->dx[d] > 0.)) {
             set_error("dx must be positive");
@@ -235,8 +235,8 @@ int fhdx_params_check(const fhdx_params* p)
         return FHDX_ERR_UNSUPPORTED;
     }
     for (int d = 0; d < 3; ++d)
-        if (2 * p->ngc > p->n_cells[d]) {
-            set_error("n_cells must be at least 2 ngc (each cell has at most one periodic image per direction)");
+        if (p->bc_vel_lo[d] != -1 && 2 * p->ngc > p->n_cells[d]) {
+            set_error("a direction with walls needs at least 2 ngc cells (setBC mirrors ngc valid cells at each wall)");
             return FHDX_ERR_UNSUPPORTED;
         }
     {
@@ -441,12 +441,21 @@ int rk3_ref(fhdx_ctx* c, fhdx_fab cu, fhdx_fab cup, fhdx_fab cup2, fhdx_fab prim
     return FHDX_OK;
 }
 
-// The fused step, the ensemble engine and the server keep the periodic fast path.
-int reject_walls(const fhdx_ctx* c, const char* who)
+// The fused step, the ensemble engine and the server serve periodic boxes of at least
+// max(4, 2 ngc) cells per direction: their producing kernels write each ghost cell's single
+// periodic image.  Walls and thinner (quasi-2-D) boxes take the reference-shaped step.
+int reject_unfused(const fhdx_ctx* c, const char* who)
 {
-    if (!c->dp.any_wall) return FHDX_OK;
-    set_error(std::string(who) + ": walls are served by the reference-shaped step (fhdx_rk3_step_ref)");
-    return FHDX_ERR_UNSUPPORTED;
+    if (c->dp.any_wall) {
+        set_error(std::string(who) + ": walls are served by the reference-shaped step (fhdx_rk3_step_ref)");
+        return FHDX_ERR_UNSUPPORTED;
+    }
+    if (!fused_box_ok(c->p)) {
+        set_error(std::string(who) + ": boxes thinner than max(4, 2 ngc) cells in a direction are served by "
+                                     "the reference-shaped step (fhdx_rk3_step_ref)");
+        return FHDX_ERR_UNSUPPORTED;
+    }
+    return FHDX_OK;
 }
 
 }  // namespace
@@ -660,7 +669,7 @@ int fhdx_rk3_step(fhdx_ctx* c, fhdx_fab cu, fhdx_fab prim, double dt, uint64_t s
         set_error("nreaction > 0: call fhdx_ctx_set_chemistry before stepping");
         return FHDX_ERR_ARG;
     }
-    if (int r = reject_walls(c, "fhdx_rk3_step")) return r;
+    if (int r = reject_unfused(c, "fhdx_rk3_step")) return r;
     return fused_rk3_step(c->p, c->dp, c->fused, cu, prim, dt, make_noise_key(seed, step, 0), nullptr, nullptr,
                           nullptr, nullptr, c->d_err, S(stream));
 }
@@ -673,7 +682,7 @@ int fhdx_rk3_step_injected(fhdx_ctx* c, fhdx_fab cu, fhdx_fab prim, const fhdx_f
         set_error("caller noise with chemistry: the reaction noise has no array argument");
         return FHDX_ERR_UNSUPPORTED;
     }
-    if (int r = reject_walls(c, "fhdx_rk3_step_injected")) return r;
+    if (int r = reject_unfused(c, "fhdx_rk3_step_injected")) return r;
     const int nv = c->dp.nvars;
     bool ok = check_cell(c, cu, nv, "cu") && check_cell(c, prim, c->dp.nprim, "prim") &&
               check_node(c, rancornA, "rancornA") && check_node(c, rancornB, "rancornB") && dt > 0.;
@@ -864,7 +873,7 @@ int fhdx_ctx_autotune(fhdx_ctx* c, fhdx_fab cu, double dt, int trial_steps, fhdx
     if (!c || !best || trial_steps < 1 || !(dt > 0.)) return FHDX_ERR_ARG;
     if (n_trials) *n_trials = 0;
     if (!check_cell(c, cu, c->dp.nvars, "cu")) return FHDX_ERR_ARG;
-    if (int r = reject_walls(c, "fhdx_ctx_autotune")) return r;
+    if (int r = reject_unfused(c, "fhdx_ctx_autotune")) return r;
     const TuneKey key{c->p.nspecies, c->p.n_cells[0], c->p.n_cells[1], c->p.n_cells[2]};
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);

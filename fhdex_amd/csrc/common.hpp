@@ -109,4 +109,13 @@ inline void stage_weights(double w[3])
 
 inline unsigned grid_for(long n, int block) { return (unsigned)((n + block - 1) / block); }
 
+// Boxes the fused kernels serve: at least 4 and at least 2 ngc cells per direction, so every
+// ghost cell is the image of exactly one valid cell and the tiles cover whole planes.
+inline bool fused_box_ok(const fhdx_params& p)
+{
+    for (int d = 0; d < 3; ++d)
+        if (p.n_cells[d] < 4 || 2 * p.ngc > p.n_cells[d]) return false;
+    return true;
+}
+
 }  // namespace fhdx

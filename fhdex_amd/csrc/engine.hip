@@ -260,6 +260,11 @@ int engine_add(Engine* e, const fhdx_params& p, double dt, uint64_t seed, uint64
         set_error("ensemble instances are periodic boxes (walls: fhdx_rk3_step_ref)");
         return FHDX_ERR_UNSUPPORTED;
     }
+    if (!fused_box_ok(p)) {
+        set_error("ensemble instances need at least max(4, 2 ngc) cells per direction (thinner boxes: "
+                  "fhdx_rk3_step_ref)");
+        return FHDX_ERR_UNSUPPORTED;
+    }
     const uint64_t need = engine_bytes_needed(p);
     if (e->used + need > e->budget) {
         set_error("ensemble memory budget exhausted");

@@ -136,7 +136,8 @@ __global__ void __launch_bounds__(BLK) k_fill_boundary(DevParams P, Arr a, int n
     if ((!P.periodic[0] && (i < 0 || i >= n0)) || (!P.periodic[1] && (j < 0 || j >= n1)) ||
         (!P.periodic[2] && (k < 0 || k >= n2)))
         return;
-    int iv = (i + n0) % n0, jv = (j + n1) % n1, kv = (k + n2) % n2;
+    // the image of a ghost cell, any number of periods away (boxes thinner than ngc included)
+    int iv = (i % n0 + n0) % n0, jv = (j % n1 + n1) % n1, kv = (k % n2 + n2) % n2;
     if (iv == i && jv == j && kv == k) return;
     for (int n = 0; n < nc; ++n) a(i, j, k, n) = a(iv, jv, kv, n);
 }

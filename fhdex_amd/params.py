@@ -370,11 +370,11 @@ def wall_deck(name, n=None, nsteps=3):
     * RT_3d (inputs_regression_RT_3d): 64 x 32 x 32 cells of 1.2e-5 cm, dt 5e-12 s, four
       species (M 2/20/2/20, d 2/20/20/2e-8 cm), rho0 4.83e-4, T 300 K, gravity -4e12 cm/s^2 in z,
       slip adiabatic species walls in z; prob_type 2 (two layers at hydrostatic equilibrium).
-    * giantfluct_3d (inputs_giantfluct_3d): 256 x 128 cells of 2.7e-5 cm, dt 1e-10 s, three
+    * giantfluct_3d (inputs_giantfluct_3d): 256 x 128 x 1 cells of 2.7e-5 cm, dt 1e-10 s, three
       argon-mass species (d 3.63/3.63/36.3e-8 cm), rho0 4.83e-4, T 300 K, no-slip isothermal
       (300 K) walls of fixed composition in y; prob_type 3 (mass fractions linear between the
-      two wall compositions).  The deck is one cell deep in z; this build needs at least 4
-      cells per direction, so the box is 4 cells deep.
+      two wall compositions).  The box is one cell deep in z (quasi-2-D), which the
+      reference-shaped step serves; the fused step needs at least 4 cells per direction.
 
     n rescales the box (cells per direction; the x count of temp_grad; the y count of RT_3d
     and giantfluct_3d, whose x count is 2 n) at the deck's cell size, for runs small enough
@@ -407,7 +407,7 @@ def wall_deck(name, n=None, nsteps=3):
                       prob_type=2)
     elif name == "giantfluct_3d":
         m = n or 128
-        cells = (2 * m, m, 4)
+        cells = (2 * m, m, 1)
         p = make_params(3, cells, (6.912e-3 / 256, 3.456e-3 / 128, 2.7e-5), [39.9480] * 3,
                         [3.63e-8, 3.63e-8, 36.3e-8])
         set_walls(p, [(-1, -1), (2, 2), (-1, -1)], 2, 2, t_lo=(300.0,) * 3, t_hi=(300.0,) * 3,

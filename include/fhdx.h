@@ -46,7 +46,7 @@ enum fhdx_status {
 typedef struct fhdx_params {
     int32_t nspecies;          /* NS, 1..FHDX_MAX_SPECIES                                   */
     int32_t n_cells[3];        /* domain cells per direction                                 */
-    int32_t ngc;               /* ghost layers of cell data (>= 2, <= n_cells/2)             */
+    int32_t ngc;               /* ghost layers of cell data (>= 2; <= n_cells/2 at walls)    */
     int32_t algorithm_type;    /* 2: multispecies (species fluxes on), else species off      */
     int32_t advection_type;    /* 0: none, 1: interpolate prim, 2: interpolate cons          */
     int32_t visc_type;         /* |3| adds bulk viscosity zeta = zeta_ratio*eta              */
@@ -199,7 +199,9 @@ int fhdx_evaluate_stats(fhdx_ctx* ctx, fhdx_fab cons, fhdx_fab consMean, fhdx_fa
  * advanced in place, prim (nprimvars comps) recomputed from the new cu; ghost layers of both
  * are filled periodically, as after timeStep.cpp:417-425.  Noise is generated in-kernel from
  * Philox (seed, step, instance 0), bit-identical to fhdx_fill_noise for the same key.
- * Periodic boxes only (walls: FHDX_ERR_UNSUPPORTED; use fhdx_rk3_step_ref). */
+ * Periodic boxes of at least max(4, 2 ngc) cells per direction only (walls, and thinner
+ * boxes such as the quasi-2-D n_cells = nx ny 1 decks: FHDX_ERR_UNSUPPORTED; use
+ * fhdx_rk3_step_ref, whose periodic ghosts are images any number of periods away). */
 int fhdx_rk3_step(fhdx_ctx* ctx, fhdx_fab cu, fhdx_fab prim, double dt, uint64_t seed,
                   uint64_t step, void* stream);
 

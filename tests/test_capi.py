@@ -132,6 +132,18 @@ def test_params_default_finalize_check():
     r.algorithm_type = 1
     assert lib.fhdx_params_check(C.byref(r)) == 2          # walls need the species ghosts
     q = p.copy()
+    q.n_cells[2] = 1                                       # quasi-2-D: one periodic cell deep
+    assert lib.fhdx_params_check(C.byref(q)) == 0
+    q.n_cells[2] = 0
+    assert lib.fhdx_params_check(C.byref(q)) == 1
+    q = p.copy()
+    P.set_walls(q, [(-1, -1), (-1, -1), (1, 1)], 1, 1)
+    q.n_cells[2] = 3                                       # setBC mirrors ngc = 2 cells per wall
+    assert lib.fhdx_params_check(C.byref(q)) == 2
+    assert b"2 ngc" in lib.fhdx_last_error()
+    q.n_cells[2] = 4
+    assert lib.fhdx_params_check(C.byref(q)) == 0
+    q = p.copy()
     q.nspecies = 1
     assert lib.fhdx_params_check(C.byref(q)) == 2
     q = p.copy()

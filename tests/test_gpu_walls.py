@@ -3,7 +3,7 @@ calculateFlux (one-sided face branches, StochFlux, BCWallSpeciesFlux) with injec
 RK3 steps of the reference-shaped path with Philox noise, on the reference's wall decks
 (exec/compressible/inputs_regression_{slip_wall_adiabatic, noslip_reservoir_isothermal,
 temp_grad, RT_3d}, inputs_giantfluct_3d) at reduced and at full size, a bulk-viscosity
-variant, a box with walls in x only and one with x reservoirs.  The GPU
+variant, a box with walls in x only, one with x reservoirs and a periodic box one cell deep.  The GPU
 reads the wall-side cell for both sides of a wall face where the oracle spells out the doubled
 terms; every comparison is exact.  The fused step refuses walls with FHDX_ERR_UNSUPPORTED.
 """
@@ -43,6 +43,14 @@ def x_reservoirs():
     return P.Config("x_reservoirs", p, base.rho0, base.rhobar, base.T_init, base.dt, 1)
 
 
+def one_cell_deep():
+    """A periodic three-species box one cell deep in z (the quasi-2-D shape of
+    exec/compressible/inputs_test_3d): ghosts are images two periods away."""
+    cfg = P.species_prefix(3, 8, 1)
+    cfg.params.n_cells[0], cfg.params.n_cells[2] = 9, 1
+    return cfg
+
+
 def visc3_walls():
     cfg = P.wall_deck("slip_wall_adiabatic", 8)
     cfg.params.visc_type = 3
@@ -56,6 +64,7 @@ CASES = [
     ("temp_grad", lambda: P.wall_deck("temp_grad", 16)),
     ("RT_3d", lambda: P.wall_deck("RT_3d", 8)),
     ("giantfluct_3d", lambda: P.wall_deck("giantfluct_3d", 8)),
+    ("one_cell_deep_periodic", lambda: one_cell_deep()),
     ("visc_type_3", visc3_walls),
     ("x_walls", x_walls),
     ("x_reservoirs", x_reservoirs),
@@ -187,10 +196,15 @@ def test_set_bc_rejects_bad_shapes(dev):
     fc.setBC(ctx, prim, cu)                               # the right shapes pass
 
 
-def test_fused_paths_refuse_walls(dev, orc):
+@pytest.mark.parametrize("case", ["walls", "one_cell_deep"])
+def test_fused_paths_refuse_walls_and_thin_boxes(dev, orc, case):
     torch, fc, L = dev
     cfg = P.wall_deck("slip_wall_adiabatic", 8)
+    if case == "one_cell_deep":
+        cfg = P.species_prefix(3, 8, 1)
+        cfg.params.n_cells[2] = 1
     p = cfg.params
+    say(f"case fused refusal {case} {tuple(p.n_cells)}")
     st = perturbed_state(p, cfg, orc, seed=1, amp=0.01, vel=100.0)
     ctx = L.Context(p)
     cu = fc.MultiFab(to_dev(torch, st.cu), (-p.ngc,) * 3)

@@ -347,3 +347,61 @@ def test_linear_composition_between_fixed_composition_walls_is_steady(orc):
         else:
             wall = np.abs(u1[5, :, [0, -1], :] - u0[5, :, [0, -1], :]).max() / np.abs(u0[5]).max()
             assert wall > 1e-6, wall
+
+
+def plane_state(orc, cfg, depth, seed):
+    """A state uniform in z, perturbed in x and y (the same plane at every depth), with its
+    ghosts filled and set."""
+    p = cfg.params
+    p.n_cells[2] = depth
+    cu = E.initial_cons(cfg)
+    g, NS = p.ngc, p.nspecies
+    n0, n1 = p.n_cells[0], p.n_cells[1]
+    rng = np.random.default_rng(seed)
+    rho = cfg.rho0 * (1 + 0.02 * rng.standard_normal((n1, n0)))
+    u = 300.0 * rng.standard_normal((3, n1, n0))
+    T = cfg.T_init * (1 + 0.01 * rng.standard_normal((n1, n0)))
+    Y = valid(p, cu)[5:5 + NS, 0] / valid(p, cu)[0, 0]
+    Y = Y * (1 + 0.02 * rng.standard_normal((NS, n1, n0)))
+    Y /= Y.sum(axis=0)
+    e = sum(Y[s] * (p.e0[s] + p.hcv[s] * T) for s in range(NS))
+    v = (slice(g, g + depth), slice(g, g + n1), slice(g, g + n0))
+    cu[(0,) + v] = rho
+    for d in range(3):
+        cu[(1 + d,) + v] = rho * u[d]
+    cu[(4,) + v] = rho * (e + 0.5 * (u ** 2).sum(axis=0))
+    for s in range(NS):
+        cu[(5 + s,) + v] = rho * Y[s]
+    prim = np.zeros((p.nprimvars,) + cu.shape[1:])
+    orc.fill_boundary(p, cu)
+    orc.cons_to_prim(p, prim, cu)
+    orc.fill_boundary(p, prim)
+    orc.set_bc(p, prim, cu)
+    return cu, prim
+
+
+@pytest.mark.parametrize("walls", [True, False])
+def test_one_cell_deep_box_evolves_like_every_layer_of_a_deep_one(orc, walls):
+    """A box one cell deep in z (the quasi-2-D decks, n_cells = nx ny 1) fills its ghosts with
+    the images of that one cell, any number of periods away.  Without noise a z-uniform state
+    has no z derivatives, so the 1-deep box must evolve exactly as each layer of a 4-deep box
+    started from the same plane: every flux through the z faces cancels, and the x and y
+    faces see the same neighbours at every depth.  With and without the y walls of
+    giantfluct_3d (perturbed in x and y so that every flux term is active)."""
+    out = {}
+    for depth in (1, 4):
+        cfg = P.wall_deck("giantfluct_3d", 8)
+        p = cfg.params
+        if not walls:
+            P.set_walls(p, -1, -1, -1)
+            cfg.prob_type = 1                            # the diffusion barrier needs the y walls
+        p.stoch_stress_form = 0
+        cu, prim = plane_state(orc, cfg, depth, seed=17)
+        start = valid(p, cu).copy()
+        assert orc.advance(p, cu, prim, 3, cfg.dt) == 0
+        out[depth] = valid(p, cu).copy()
+    a, b = out[1][:, 0], out[4]
+    assert np.all(np.isfinite(a))
+    for k in range(4):
+        assert np.array_equal(b[:, k], a), k
+    assert np.abs(a - start[:, 0]).max(axis=(1, 2)).min() > 0   # every component moved
