@@ -70,7 +70,7 @@ def cpu_baseline(n: int, steps: int, n1: int = 32, steps1: int = 2, n4: int = 64
 
 
 def config_rate(cfg, steps: int, warmup: int = 2):
-    """cell-updates/s of the fused step on one configuration (uniform start, tuned shape)."""
+    """cell-updates/s of the fused step on one configuration (InitConsVar's start, tuned shape)."""
     import torch
     from fhdex_amd import compressible as fc
     from fhdex_amd import ensemble as E
@@ -79,7 +79,7 @@ def config_rate(cfg, steps: int, warmup: int = 2):
     ctx = L.Context(p)
     if cfg.chem is not None:
         ctx.set_chemistry(cfg.chem)
-    cu = fc.MultiFab(torch.from_numpy(E.uniform_cons(cfg)).cuda(), (-p.ngc,) * 3)
+    cu = fc.MultiFab(torch.from_numpy(E.initial_cons(cfg)).cuda(), (-p.ngc,) * 3)
     prim = fc.MultiFab.cell(p, p.nprimvars)
     best, _ = ctx.autotune(cu.fab, cfg.dt, trial_steps=2)
     for s in range(warmup):
@@ -465,7 +465,7 @@ def main():
     del cu, prim
     if not a.no_configs:
         line["configs"] = [config_rate(P.argon_gas(32), 50), config_rate(P.four_species(128), 10),
-                           config_rate(P.no2_n2o4(128), 10),
+                           config_rate(P.no2_n2o4(128), 10), config_rate(P.taylor_green(64), 50),
                            wall_rate(P.wall_deck("slip_wall_adiabatic")),
                            wall_rate(P.wall_deck("temp_grad")), wall_rate(P.wall_deck("RT_3d")),
                            wall_rate(P.wall_deck("giantfluct_3d"))]

@@ -44,12 +44,17 @@ def uniform_cons(cfg: Config) -> np.ndarray:
 
 def initial_cons(cfg: Config) -> np.ndarray:
     """InitConsVar's conserved start (compressible_functions.cpp:103-413) on the grown box, as a
-    host array, for cfg.prob_type 1 (uniform), 2 (Rayleigh-Taylor layers, :172-214: four
-    species, mass fractions 0.4/0.4/0.1/0.1 above the mid-plane in z and 0.1/0.1/0.4/0.4 below,
-    density rho0 exp(g_z z / (R_mix T)) at hydrostatic equilibrium) or 3 (diffusion barrier,
-    :216-227: mass fractions linear in y between the y walls' compositions).  The domain is
-    [0, n dx].  For types 2 and 3 the ghost cells keep the uniform values: fill them with
-    fill_boundary / setBC, as the reference does after InitConsVar (main_driver.cpp:532-542)."""
+    host array, for cfg.prob_type
+      1 uniform;
+      2 Rayleigh-Taylor layers (:172-214): four species, mass fractions 0.4/0.4/0.1/0.1 above
+        the mid-plane in z and 0.1/0.1/0.4/0.4 below, density rho0 exp(g_z z / (R_mix T)) at
+        hydrostatic equilibrium;
+      3 diffusion barrier (:216-227): mass fractions linear in y between the y walls'
+        compositions;
+      4 Taylor-Green vortex (:228-262): two species.
+    The domain is [0, n dx].  For types 2, 3 and 4 the ghost cells keep the uniform values:
+    fill them with fill_boundary / setBC, as the reference does after InitConsVar
+    (main_driver.cpp:532-542)."""
     cu = uniform_cons(cfg)
     if cfg.prob_type == 1:
         return cu
@@ -59,6 +64,32 @@ def initial_cons(cfg: Config) -> np.ndarray:
     n0, n1, n2 = p.n_cells
     v = (slice(None), slice(g, g + n2), slice(g, g + n1), slice(g, g + n0))
     rho = np.full((n2, n1, n0), cfg.rho0)
+    if cfg.prob_type == 4:
+        # Taylor-Green vortex (:228-262): positions from the lower corner, all scaled by the
+        # x extent; rho from the pressure at T_init; rho E = p / (5/3 - 1) + kinetic energy
+        if ns != 2:
+            raise ValueError("prob_type 4 (Taylor-Green) is a two-species start")
+        pi = np.arccos(-1.0)
+        Lf = n0 * p.dx[0]
+        avogadro = p.Runiv / p.k_B
+        velscale, pscale = 9237.0, 884.147e3
+        rhoscale = p.molmass[0] / avogadro * pscale / (p.k_B * cfg.T_init)
+        x = ((np.arange(n0) + 0.5) * p.dx[0])[None, None, :]
+        y = ((np.arange(n1) + 0.5) * p.dx[1])[None, :, None]
+        z = ((np.arange(n2) + 0.5) * p.dx[2])[:, None, None]
+        pres = pscale + rhoscale * velscale * velscale * np.cos(2. * pi * x / Lf) * np.cos(4. * pi * y / Lf) \
+            * (np.cos(4. * pi * z / Lf) + 2.)
+        r = (p.molmass[0] / avogadro) * pres / (p.k_B * cfg.T_init)
+        m1 = velscale * r * np.sin(2. * pi * x / Lf) * np.cos(2. * pi * y / Lf) * np.cos(2. * pi * z / Lf)
+        m2 = -velscale * r * np.cos(2. * pi * x / Lf) * np.sin(2. * pi * y / Lf) * np.cos(2. * pi * z / Lf)
+        cu[v][0] = r
+        cu[v][1] = m1
+        cu[v][2] = m2
+        cu[v][3] = 0.0
+        cu[v][4] = pres / (5. / 3. - 1.) + 0.5 * (m1 * m1 + m2 * m2) / r
+        cu[v][5] = 0.5 * r
+        cu[v][6] = 0.5 * r
+        return cu
     if cfg.prob_type == 2:
         if ns != 4:
             raise ValueError("prob_type 2 (Rayleigh-Taylor) is a four-species start")
@@ -79,7 +110,7 @@ def initial_cons(cfg: Config) -> np.ndarray:
         for l in range(ns):
             Y[l] = (p.bc_Yk[3][l] - p.bc_Yk[2][l]) / Ly * posy + p.bc_Yk[2][l]
     else:
-        raise ValueError(f"prob_type {cfg.prob_type} is not built (types 1, 2 and 3 are)")
+        raise ValueError(f"prob_type {cfg.prob_type} is not built (types 1 to 4 are)")
     cv = sum(Y[l] * p.hcv[l] for l in range(ns))
     e0 = sum(Y[l] * p.e0[l] for l in range(ns))
     cu[v][0] = rho

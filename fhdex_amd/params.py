@@ -250,6 +250,17 @@ def four_species(n, nsteps=200):
     return Config(f"species4_{n}^3", p, 4.83e-4, rhobar, 300.0, MIXTURE_DT, nsteps)
 
 
+def taylor_green(n=64, nsteps=100):
+    """The Taylor-Green vortex deck (exec/compressible/inputs_tg, its final block): two
+    argon species (M 39.948, d 3.63e-8 cm, Y 1/2 each), T 273 K, an n^3 periodic box of edge
+    7.85e-3 cm (64^3 cells), dt 1e-9 s, 100 steps, prob_type 4 (velocity amplitude 9237 cm/s,
+    mean pressure 884147 dyn/cm^2; the deck's turbForcing only adds energy-spectrum
+    diagnostics in main_driver.cpp)."""
+    p = make_params(2, (n, n, n), 0.00785 / 64, [39.948, 39.948], [3.63e-8, 3.63e-8])
+    rho = 39.948 / (R_UNIV / K_B) * 884.147e3 / (K_B * 273.0)
+    return Config(f"taylor_green_{n}^3", p, rho, [0.5, 0.5], 273.0, 1.0e-9, nsteps, prob_type=4)
+
+
 def species_prefix(n_species, edge, nsteps):
     """Species 1..n_species of the four-species list with equal mass fractions (the served
     request mix); n_species == 1 is written as two identical species (argon form)."""

@@ -1668,6 +1668,39 @@ int orc_init_cons(const fhdx_params* P, fhdx_fab cu_, int prob_type, double rho0
                                  m2 = AT(cu, i, j, k, 2), m3 = AT(cu, i, j, k, 3);
                     AT(cu, i, j, k, 4) = r * e + 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) / r;
                 }
+    } else if (prob_type == 4) {
+        /* Taylor-Green vortex (:228-262): two species, 50/50; x, y, z measured from the
+         * domain's lower corner, every direction scaled by the x extent Lf; rho from the
+         * pressure field at T_init, rho E = p/(5/3 - 1) + kinetic energy. */
+        if (NS != 2) return FHDX_ERR_UNSUPPORTED;
+        const double pi = acos(-1.);
+        const double Lf = P->n_cells[0] * P->dx[0];
+        const double avogadro = P->Runiv / P->k_B;
+        const double velscale = 9237.;
+        const double pscale = 884.147e3;
+        const double rhoscale = P->molmass[0] / avogadro * pscale / (P->k_B * T_init);
+        for (int k = 0; k < P->n_cells[2]; ++k)
+            for (int j = 0; j < P->n_cells[1]; ++j)
+                for (int i = 0; i < P->n_cells[0]; ++i) {
+                    const double x = (i + 0.5) * P->dx[0];
+                    const double y = (j + 0.5) * P->dx[1];
+                    const double z = (k + 0.5) * P->dx[2];
+                    const double pres = pscale + rhoscale * velscale * velscale * cos(2. * pi * x / Lf) *
+                                                     cos(4. * pi * y / Lf) * (cos(4. * pi * z / Lf) + 2.);
+                    const double r = (P->molmass[0] / avogadro) * pres / (P->k_B * T_init);
+                    const double m1 = velscale * r * sin(2. * pi * x / Lf) * cos(2. * pi * y / Lf) *
+                                      cos(2. * pi * z / Lf);
+                    const double m2 = -velscale * r * cos(2. * pi * x / Lf) * sin(2. * pi * y / Lf) *
+                                      cos(2. * pi * z / Lf);
+                    const double m3 = 0.;
+                    AT(cu, i, j, k, 0) = r;
+                    AT(cu, i, j, k, 1) = m1;
+                    AT(cu, i, j, k, 2) = m2;
+                    AT(cu, i, j, k, 3) = m3;
+                    AT(cu, i, j, k, 4) = pres / (5. / 3. - 1.) + 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) / r;
+                    AT(cu, i, j, k, 5) = 0.5 * r;
+                    AT(cu, i, j, k, 6) = 0.5 * r;
+                }
     } else if (prob_type != 1) {
         return FHDX_ERR_UNSUPPORTED;
     }

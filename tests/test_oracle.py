@@ -495,3 +495,37 @@ def test_trace_species_gets_no_species_noise(orc):
         assert np.array_equal(noisy[d][7], quiet[d][7]), d          # trace species: no noise
         for s in (5, 6):
             assert np.abs(noisy[d][s] - quiet[d][s]).max() > 0, (d, s)
+
+
+def test_taylor_green_start(orc):
+    """prob_type 4 (compressible_functions.cpp:228-262, exec/compressible/inputs_tg): the host
+    start (ensemble.initial_cons) equals the oracle's bit for bit, and it is the deck's vortex,
+    checked from properties rather than the formulas: the gas is at T_init everywhere (density
+    and energy both follow the pressure, so T = p / (rho R/M) is exact up to rounding); the
+    pressure averages to the deck's 884147 dyn/cm^2 over the periodic box (every modulation is
+    a whole cosine period); the velocity is the 2-D vortex, w = 0, with zero discrete
+    divergence (the two centred differences cancel term by term); its amplitude is 9237 cm/s;
+    the two species are half and half."""
+    from fhdex_amd import ensemble as E
+    cfg = P.taylor_green(16)
+    p = cfg.params
+    st = orc.HostState(p)
+    assert orc.init_cons(p, st.cu, cfg.rho0, cfg.rhobar, cfg.T_init, cfg.prob_type) == 0
+    assert np.array_equal(E.initial_cons(cfg), st.cu)
+    orc.fill_boundary(p, st.cu)
+    orc.cons_to_prim(p, st.prim, st.cu)
+    orc.fill_boundary(p, st.prim)
+    g = p.ngc
+    pr = st.prim[:, g:-g, g:-g, g:-g]
+    assert np.allclose(pr[4], 273.0, rtol=1e-13, atol=0)
+    assert pr[5].mean() == pytest.approx(884.147e3, rel=1e-13)
+    assert pr[5].std() > 1e-3 * 884.147e3
+    assert np.all(pr[3] == 0.0)
+    u = st.prim[1:3]
+    dudx = (u[0][g:-g, g:-g, g + 1:-g + 1] - u[0][g:-g, g:-g, g - 1:-g - 1]) / (2 * p.dx[0])
+    dvdy = (u[1][g:-g, g + 1:-g + 1, g:-g] - u[1][g:-g, g - 1:-g - 1, g:-g]) / (2 * p.dx[1])
+    assert np.abs(dudx + dvdy).max() <= 1e-12 * 9237.0 / p.dx[0]
+    assert np.abs(dudx).max() > 1e-3 * 9237.0 / p.dx[0]
+    # on 16 cells per period the crest nearest the centres is half a cell off in x, y and z
+    assert np.abs(pr[1]).max() == pytest.approx(9237.0 * np.cos(np.pi / 16) ** 3, rel=1e-13)
+    assert np.allclose(pr[6:8], 0.5, rtol=1e-15)
