@@ -51,7 +51,8 @@ def initial_cons(cfg: Config) -> np.ndarray:
         hydrostatic equilibrium;
       3 diffusion barrier (:216-227): mass fractions linear in y between the y walls'
         compositions;
-      4 Taylor-Green vortex (:228-262): two species.
+      4 Taylor-Green vortex (:228-262): two species;
+      100 sinusoidal density in y at the uniform energy density (:292-302), ghosts filled.
     The domain is [0, n dx].  For types 2, 3 and 4 the ghost cells keep the uniform values:
     fill them with fill_boundary / setBC, as the reference does after InitConsVar
     (main_driver.cpp:532-542)."""
@@ -64,6 +65,20 @@ def initial_cons(cfg: Config) -> np.ndarray:
     n0, n1, n2 = p.n_cells
     v = (slice(None), slice(g, g + n2), slice(g, g + n1), slice(g, g + n0))
     rho = np.full((n2, n1, n0), cfg.rho0)
+    if cfg.prob_type == 100:
+        # sinusoidal density in y at the uniform energy density (:292-302), mass fractions kept;
+        # the ghost cells are the periodic images
+        y = ((np.arange(n1) + 0.5) * p.dx[1])[None, :, None]
+        Ly = p.dx[1] * n1
+        Yk = [cu[5 + l][v[1:]] / cu[0][v[1:]] for l in range(ns)]
+        r = cfg.rho0 + 0.1 * cfg.rho0 * np.sin(2. * np.arccos(-1.) * y / Ly)
+        cu[v][0] = r
+        for l in range(ns):
+            cu[v][5 + l] = cu[v][0] * Yk[l]
+        for ax, n in ((3, n0), (2, n1), (1, n2)):           # periodic images, x, then y, then z
+            src = (np.arange(-g, n + g) % n) + g
+            cu = np.take(cu, src, axis=ax)
+        return np.ascontiguousarray(cu)
     if cfg.prob_type == 4:
         # Taylor-Green vortex (:228-262): positions from the lower corner, all scaled by the
         # x extent; rho from the pressure at T_init; rho E = p / (5/3 - 1) + kinetic energy
@@ -110,7 +125,7 @@ def initial_cons(cfg: Config) -> np.ndarray:
         for l in range(ns):
             Y[l] = (p.bc_Yk[3][l] - p.bc_Yk[2][l]) / Ly * posy + p.bc_Yk[2][l]
     else:
-        raise ValueError(f"prob_type {cfg.prob_type} is not built (types 1 to 4 are)")
+        raise ValueError(f"prob_type {cfg.prob_type} is not built (types 1 to 4 and 100 are)")
     cv = sum(Y[l] * p.hcv[l] for l in range(ns))
     e0 = sum(Y[l] * p.e0[l] for l in range(ns))
     cu[v][0] = rho

@@ -250,6 +250,17 @@ def four_species(n, nsteps=200):
     return Config(f"species4_{n}^3", p, 4.83e-4, rhobar, 300.0, MIXTURE_DT, nsteps)
 
 
+def deterministic_wave(nsteps=100000):
+    """exec/compressible/inputs_test_deterministic: two identical argon species (M 39.928...,
+    d 3.66e-8 cm), T 300 K, rho0 1.78e-3, a 4 x 80 x 4 periodic box of 3.125e-6 cm cells, dt
+    1e-12 s, deterministic fluxes (stoch_stress_form 0), prob_type 100: density
+    rho0 (1 + 0.1 sin(2 pi y / L_y)) at the uniform energy density, an isobaric (entropy) wave."""
+    M = 39.92820856716215701081
+    p = make_params(2, (4, 80, 4), (1.25e-5 / 4, 2.5e-4 / 80, 1.25e-5 / 4), [M, M], [3.66e-8, 3.66e-8],
+                    stoch_stress_form=0)
+    return Config("deterministic_wave_4x80x4", p, 1.78e-3, [0.5, 0.5], 300.0, 1.0e-12, nsteps, prob_type=100)
+
+
 def taylor_green(n=64, nsteps=100):
     """The Taylor-Green vortex deck (exec/compressible/inputs_tg, its final block): two
     argon species (M 39.948, d 3.63e-8 cm, Y 1/2 each), T 273 K, an n^3 periodic box of edge

@@ -111,3 +111,60 @@ def test_taylor_green_deck_matches_oracle_and_keeps_totals(orc):
     s1, a1 = totals(torch, cu, p)
     for c in range(p.nvars):
         assert abs(s1[c] - s0[c]) <= 1e-12 * a1[c], c
+
+
+def test_deterministic_wave_decays_at_the_thermal_diffusion_rate(orc):
+    """exec/compressible/inputs_test_deterministic (argon 4 x 80 x 4, noise off, prob_type 100):
+    an isobaric density wave.  100 fused steps equal the oracle bit for bit.  Its entropy
+    part q = rho' - p'/c^2 (the density not carried by sound) relaxes by heat conduction at
+    lambda = kappa k^2 / (rho c_p), with kappa the hard-sphere closed form
+    (75/64)(k_B/d^2) sqrt(k_B T / (pi m)) at 300 K (GIO gives it for two identical species),
+    c_p = 5R/2M and k = 2 pi / L_y: lambda = 1.161e8 /s.  After 16000 steps (1.6e-8 s, 1.9
+    e-folds) the measured mean rate is within 3 % of lambda (the oracle gives 0.5 %; the
+    residual sound the wave sheds makes the running rate swing by +-4 % before that).  A
+    wrong factor in kappa, c_p or the heat-flux stencil moves it by far more."""
+    import math
+    import torch
+    from fhdex_amd import compressible as fc
+    from fhdex_amd import ensemble as E
+    from fhdex_amd import lib as L
+    cfg = P.deterministic_wave()
+    p = cfg.params
+    g = p.ngc
+    say(f"case deterministic wave {tuple(p.n_cells)} x 100 vs oracle, then x 16000")
+    cu0 = E.initial_cons(cfg)
+    ctx = L.Context(p)
+    cu = fc.MultiFab(torch.from_numpy(cu0.copy()).cuda(), (-g,) * 3)
+    prim = fc.MultiFab.cell(p, p.nprimvars)
+    for step in range(100):
+        fc.rk3_step(ctx, cu, prim, cfg.dt, 1, step)
+    ctx.check()
+    ref = cu0.copy()
+    rprim = np.zeros((p.nprimvars,) + ref.shape[1:])
+    assert orc.advance(p, ref, rprim, 100, cfg.dt) == 0
+    assert np.array_equal(cu.t.cpu().numpy(), ref)
+
+    n1 = p.n_cells[1]
+    k = 2 * math.pi / (n1 * p.dx[1])
+    s = np.sin(k * (np.arange(n1) + 0.5) * p.dx[1])
+
+    def q_amp(pr):
+        r = pr[0, g:-g, g:-g, g:-g].mean(axis=(0, 2))
+        pp = pr[5, g:-g, g:-g, g:-g].mean(axis=(0, 2))
+        c2 = 5.0 / 3.0 * pp.mean() / r.mean()
+        return 2 * (((r - r.mean()) - (pp - pp.mean()) / c2) * s).mean()
+
+    cu = fc.MultiFab(torch.from_numpy(cu0.copy()).cuda(), (-g,) * 3)
+    fc.conservedToPrimitive(ctx, prim, cu)
+    q0 = q_amp(prim.t.cpu().numpy())
+    nsteps = 16000
+    for step in range(nsteps):
+        fc.rk3_step(ctx, cu, prim, cfg.dt, 1, step)
+    ctx.check()
+    q1 = q_amp(prim.t.cpu().numpy())
+    M, kB, R, d, T = p.molmass[0], p.k_B, p.Runiv, p.diameter[0], cfg.T_init
+    kappa = 75.0 / 64.0 * kB / d ** 2 * math.sqrt(kB * T / (math.pi * M * kB / R))
+    lam = kappa / (cfg.rho0 * 2.5 * R / M) * k * k
+    rate = -math.log(q1 / q0) / (nsteps * cfg.dt)
+    say(f"  q/q0 = {q1 / q0:.5f}, rate {rate:.4e} /s, closed form {lam:.4e} /s, ratio {rate / lam:.4f}")
+    assert abs(rate / lam - 1) < 0.03

@@ -529,3 +529,25 @@ def test_taylor_green_start(orc):
     # on 16 cells per period the crest nearest the centres is half a cell off in x, y and z
     assert np.abs(pr[1]).max() == pytest.approx(9237.0 * np.cos(np.pi / 16) ** 3, rel=1e-13)
     assert np.allclose(pr[6:8], 0.5, rtol=1e-15)
+
+
+def test_deterministic_wave_start(orc):
+    """prob_type 100 (compressible_functions.cpp:292-302, exec/compressible/inputs_test_deterministic):
+    the host start equals the oracle's bit for bit, ghosts included; it is an isobaric wave:
+    density rho0 (1 + 0.1 sin(2 pi y / L_y)), the pressure uniform (the energy density is the
+    uniform start's), the temperature in antiphase, the gas at rest, Y = 1/2."""
+    from fhdex_amd import ensemble as E
+    cfg = P.deterministic_wave()
+    p = cfg.params
+    st = orc.HostState(p)
+    assert orc.init_cons(p, st.cu, cfg.rho0, cfg.rhobar, cfg.T_init, cfg.prob_type) == 0
+    assert np.array_equal(E.initial_cons(cfg), st.cu)
+    orc.cons_to_prim(p, st.prim, st.cu)
+    g = p.ngc
+    pr = st.prim[:, g:-g, g:-g, g:-g]
+    assert np.allclose(pr[5], pr[5].mean(), rtol=1e-14, atol=0)
+    y = (np.arange(p.n_cells[1]) + 0.5) * p.dx[1]
+    wave = cfg.rho0 * (1 + 0.1 * np.sin(2 * np.pi * y / (p.n_cells[1] * p.dx[1])))
+    assert np.allclose(pr[0], wave[None, :, None], rtol=1e-14, atol=0)
+    assert np.allclose(pr[0] * pr[4], (pr[0] * pr[4]).mean(), rtol=1e-14, atol=0)   # p = rho R T / M
+    assert np.all(pr[1:4] == 0.0) and np.allclose(pr[6:8], 0.5, rtol=1e-15)
