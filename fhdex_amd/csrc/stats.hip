@@ -1,6 +1,6 @@
 // stats.hip -- evaluateStats (src_compressible/stats.cpp:5-363) on the device.
 //
-// Three passes over the valid box of one periodic FAB:
+// Three passes over the valid box of one FAB (valid cells only, so any boundary conditions):
 //   k_stats_means  running means of cons and the primitive variables of the mean state
 //                  (stats.cpp:45-88), per cell;
 //   k_stats_yz_*   the 19 yz-slice sums at every x (stats.cpp:156-210): each thread sums one
