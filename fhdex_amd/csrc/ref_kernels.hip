@@ -268,12 +268,11 @@ __global__ void __launch_bounds__(BLK) k_fill(double* p, long n, double v)
 // then scales the face's stochastic fluxes, applied here to the values this thread wrote.
 
 template <int NS, int D>
-__global__ void __launch_bounds__(BLK) k_stoch(DevParams P, Arr prim, Arr eta, Arr zeta, Arr kappa,
-                                               Arr chi, Arr Dm, Arr fl, Arr ran, Arr rc,
-                                               double volinv, double dtinv, double dt)
+__device__ __forceinline__ void stoch_body(const DevParams& P, Arr prim, Arr eta, Arr zeta, Arr kappa, Arr chi,
+                                           Arr Dm, Arr fl, Arr ran, Arr rc, double volinv, double dtinv,
+                                           double dt, long t)
 {
     const int fx = P.n[0] + (D == 0), fy = P.n[1] + (D == 1), fz = P.n[2] + (D == 2);
-    long t = (long)blockIdx.x * BLK + threadIdx.x;
     if (t >= (long)fx * fy * fz) return;
     int i, j, k;
     face_index(t, fx, fy, i, j, k);
@@ -484,11 +483,10 @@ __global__ void __launch_bounds__(BLK) k_stoch(DevParams P, Arr prim, Arr eta, A
 // touches none of those components, so it is applied here.
 
 template <int NS, int D>
-__global__ void __launch_bounds__(BLK) k_diff(DevParams P, Arr prim, Arr eta, Arr kappa, Arr chi,
-                                              Arr Dm, Arr fl)
+__device__ __forceinline__ void diff_body(const DevParams& P, Arr prim, Arr eta, Arr kappa, Arr chi, Arr Dm,
+                                          Arr fl, long t)
 {
     const int fx = P.n[0] + (D == 0), fy = P.n[1] + (D == 1), fz = P.n[2] + (D == 2);
-    long t = (long)blockIdx.x * BLK + threadIdx.x;
     if (t >= (long)fx * fy * fz) return;
     int i, j, k;
     face_index(t, fx, fy, i, j, k);
@@ -679,11 +677,10 @@ __device__ inline double node4(const Arr& a, int i, int j, int k, bool zorder, d
 }
 
 template <int NS, int D>
-__global__ void __launch_bounds__(BLK) k_corner_face(DevParams P, Arr prim, Arr fl, Arr vcorn, Arr c1,
-                                                     Arr c2, Arr ta1, Arr ta2)
+__device__ __forceinline__ void corner_face_body(const DevParams& P, Arr prim, Arr fl, Arr vcorn, Arr c1,
+                                                 Arr c2, Arr ta1, Arr ta2, long t)
 {
     const int fx = P.n[0] + (D == 0), fy = P.n[1] + (D == 1), fz = P.n[2] + (D == 2);
-    long t = (long)blockIdx.x * BLK + threadIdx.x;
     if (t >= (long)fx * fy * fz) return;
     int i, j, k;
     face_index(t, fx, fy, i, j, k);
@@ -715,10 +712,9 @@ __global__ void __launch_bounds__(BLK) k_corner_face(DevParams P, Arr prim, Arr 
 // ---- hyperbolic face flux (flux.cpp:2016-2372) ---------------------------------------------
 
 template <int NS, int D>
-__global__ void __launch_bounds__(BLK) k_hyp(DevParams P, Arr cons, Arr prim, Arr fl)
+__device__ __forceinline__ void hyp_body(const DevParams& P, Arr cons, Arr prim, Arr fl, long t)
 {
     const int fx = P.n[0] + (D == 0), fy = P.n[1] + (D == 1), fz = P.n[2] + (D == 2);
-    long t = (long)blockIdx.x * BLK + threadIdx.x;
     if (t >= (long)fx * fy * fz) return;
     int i, j, k;
     face_index(t, fx, fy, i, j, k);
@@ -957,6 +953,69 @@ __global__ void __launch_bounds__(BLK) k_noise_node(DevParams P, NoiseKey K, Arr
     B(i, j, k) = zb * scale;
 }
 
+// ---- the face passes, one launch for all three directions ------------------------------------
+// blockIdx.y is the face direction; each thread runs the same per-face body as a launch of that
+// direction alone would, so the arithmetic and its order are unchanged.  A pass writes only
+// the flux of its own direction and reads shared inputs, so the directions are independent.
+// On small boxes (the wall decks), three directions' faces in one launch keep three times as
+// many waves in flight as one direction's.
+
+template <int NS>
+__global__ void __launch_bounds__(BLK) k_stoch3(DevParams P, Arr prim, Arr eta, Arr zeta, Arr kappa, Arr chi,
+                                                Arr Dm, Arr f0, Arr f1, Arr f2, Arr r0, Arr r1, Arr r2, Arr rc,
+                                                double volinv, double dtinv, double dt)
+{
+    const long t = (long)blockIdx.x * BLK + threadIdx.x;
+    if (blockIdx.y == 0)
+        stoch_body<NS, 0>(P, prim, eta, zeta, kappa, chi, Dm, f0, r0, rc, volinv, dtinv, dt, t);
+    else if (blockIdx.y == 1)
+        stoch_body<NS, 1>(P, prim, eta, zeta, kappa, chi, Dm, f1, r1, rc, volinv, dtinv, dt, t);
+    else
+        stoch_body<NS, 2>(P, prim, eta, zeta, kappa, chi, Dm, f2, r2, rc, volinv, dtinv, dt, t);
+}
+
+template <int NS>
+__global__ void __launch_bounds__(BLK) k_diff3(DevParams P, Arr prim, Arr eta, Arr kappa, Arr chi, Arr Dm,
+                                               Arr f0, Arr f1, Arr f2)
+{
+    const long t = (long)blockIdx.x * BLK + threadIdx.x;
+    if (blockIdx.y == 0)
+        diff_body<NS, 0>(P, prim, eta, kappa, chi, Dm, f0, t);
+    else if (blockIdx.y == 1)
+        diff_body<NS, 1>(P, prim, eta, kappa, chi, Dm, f1, t);
+    else
+        diff_body<NS, 2>(P, prim, eta, kappa, chi, Dm, f2, t);
+}
+
+// Per direction: the flux, the divergence pair and the tangential pair of corner arrays.
+struct CornerFaceSet {
+    Arr fl[3], c1[3], c2[3], ta1[3], ta2[3];
+};
+
+template <int NS>
+__global__ void __launch_bounds__(BLK) k_corner_face3(DevParams P, Arr prim, Arr vcorn, CornerFaceSet C)
+{
+    const long t = (long)blockIdx.x * BLK + threadIdx.x;
+    if (blockIdx.y == 0)
+        corner_face_body<NS, 0>(P, prim, C.fl[0], vcorn, C.c1[0], C.c2[0], C.ta1[0], C.ta2[0], t);
+    else if (blockIdx.y == 1)
+        corner_face_body<NS, 1>(P, prim, C.fl[1], vcorn, C.c1[1], C.c2[1], C.ta1[1], C.ta2[1], t);
+    else
+        corner_face_body<NS, 2>(P, prim, C.fl[2], vcorn, C.c1[2], C.c2[2], C.ta1[2], C.ta2[2], t);
+}
+
+template <int NS>
+__global__ void __launch_bounds__(BLK) k_hyp3(DevParams P, Arr cons, Arr prim, Arr f0, Arr f1, Arr f2)
+{
+    const long t = (long)blockIdx.x * BLK + threadIdx.x;
+    if (blockIdx.y == 0)
+        hyp_body<NS, 0>(P, cons, prim, f0, t);
+    else if (blockIdx.y == 1)
+        hyp_body<NS, 1>(P, cons, prim, f1, t);
+    else
+        hyp_body<NS, 2>(P, cons, prim, f2, t);
+}
+
 // ---- host launchers ---------------------------------------------------------------------
 
 #define NS_DISPATCH(NSV, CALL)                          \
@@ -1030,34 +1089,27 @@ static void flux_passes(const DevParams& P, const FluxArgs& a, hipStream_t s)
 {
     const double volinv = 1. / (P.dx[0] * P.dx[1] * P.dx[2]);
     const double dtinv = 1. / a.dt;
-    if (P.stoch_on) {
-        k_stoch<NS, 0><<<grid_for(nface(P, 0), BLK), BLK, 0, s>>>(P, a.prim, a.eta, a.zeta, a.kappa, a.chi, a.D,
-                                                                  a.flux[0], a.ran[0], a.rancorn, volinv, dtinv, a.dt);
-        k_stoch<NS, 1><<<grid_for(nface(P, 1), BLK), BLK, 0, s>>>(P, a.prim, a.eta, a.zeta, a.kappa, a.chi, a.D,
-                                                                  a.flux[1], a.ran[1], a.rancorn, volinv, dtinv, a.dt);
-        k_stoch<NS, 2><<<grid_for(nface(P, 2), BLK), BLK, 0, s>>>(P, a.prim, a.eta, a.zeta, a.kappa, a.chi, a.D,
-                                                                  a.flux[2], a.ran[2], a.rancorn, volinv, dtinv, a.dt);
-    }
-    k_diff<NS, 0><<<grid_for(nface(P, 0), BLK), BLK, 0, s>>>(P, a.prim, a.eta, a.kappa, a.chi, a.D, a.flux[0]);
-    k_diff<NS, 1><<<grid_for(nface(P, 1), BLK), BLK, 0, s>>>(P, a.prim, a.eta, a.kappa, a.chi, a.D, a.flux[1]);
-    k_diff<NS, 2><<<grid_for(nface(P, 2), BLK), BLK, 0, s>>>(P, a.prim, a.eta, a.kappa, a.chi, a.D, a.flux[2]);
+    long mf = nface(P, 0);
+    for (int d = 1; d < 3; ++d) mf = nface(P, d) > mf ? nface(P, d) : mf;
+    const dim3 grid(grid_for(mf, BLK), 3);   // every body checks its own direction's face count
+    if (P.stoch_on)
+        k_stoch3<NS><<<grid, BLK, 0, s>>>(P, a.prim, a.eta, a.zeta, a.kappa, a.chi, a.D, a.flux[0], a.flux[1],
+                                          a.flux[2], a.ran[0], a.ran[1], a.ran[2], a.rancorn, volinv, dtinv, a.dt);
+    k_diff3<NS><<<grid, BLK, 0, s>>>(P, a.prim, a.eta, a.kappa, a.chi, a.D, a.flux[0], a.flux[1], a.flux[2]);
     // corn[a][b]: derivative direction a, velocity component b (cornx/corny/cornz of flux.cpp)
     const Arr(&c)[3][3] = a.corn;
     k_corner_node<<<grid_for(nnode(P), BLK), BLK, 0, s>>>(P, a.prim, a.eta, a.zeta, c[0][0], c[0][1], c[0][2],
                                                           c[1][0], c[1][1], c[1][2], c[2][0], c[2][1], c[2][2],
                                                           a.visccorn);
     // x face: div pair (vy, wz), tangential (uy, uz); y: (ux, wz), (vx, vz); z: (vy, ux), (wx, wy)
-    k_corner_face<NS, 0><<<grid_for(nface(P, 0), BLK), BLK, 0, s>>>(P, a.prim, a.flux[0], a.visccorn, c[1][1],
-                                                                    c[2][2], c[1][0], c[2][0]);
-    k_corner_face<NS, 1><<<grid_for(nface(P, 1), BLK), BLK, 0, s>>>(P, a.prim, a.flux[1], a.visccorn, c[0][0],
-                                                                    c[2][2], c[0][1], c[2][1]);
-    k_corner_face<NS, 2><<<grid_for(nface(P, 2), BLK), BLK, 0, s>>>(P, a.prim, a.flux[2], a.visccorn, c[1][1],
-                                                                    c[0][0], c[0][2], c[1][2]);
-    if (P.advection_type == 1 || P.advection_type == 2) {
-        k_hyp<NS, 0><<<grid_for(nface(P, 0), BLK), BLK, 0, s>>>(P, a.cons, a.prim, a.flux[0]);
-        k_hyp<NS, 1><<<grid_for(nface(P, 1), BLK), BLK, 0, s>>>(P, a.cons, a.prim, a.flux[1]);
-        k_hyp<NS, 2><<<grid_for(nface(P, 2), BLK), BLK, 0, s>>>(P, a.cons, a.prim, a.flux[2]);
-    }
+    CornerFaceSet C;
+    for (int d = 0; d < 3; ++d) C.fl[d] = a.flux[d];
+    C.c1[0] = c[1][1]; C.c2[0] = c[2][2]; C.ta1[0] = c[1][0]; C.ta2[0] = c[2][0];
+    C.c1[1] = c[0][0]; C.c2[1] = c[2][2]; C.ta1[1] = c[0][1]; C.ta2[1] = c[2][1];
+    C.c1[2] = c[1][1]; C.c2[2] = c[0][0]; C.ta1[2] = c[0][2]; C.ta2[2] = c[1][2];
+    k_corner_face3<NS><<<grid, BLK, 0, s>>>(P, a.prim, a.visccorn, C);
+    if (P.advection_type == 1 || P.advection_type == 2)
+        k_hyp3<NS><<<grid, BLK, 0, s>>>(P, a.cons, a.prim, a.flux[0], a.flux[1], a.flux[2]);
 }
 
 int ref_calculate_flux(const DevParams& P, const FluxArgs& a, int flux_ncomp[3], hipStream_t s)
