@@ -909,10 +909,9 @@ __global__ void __launch_bounds__(BLK) k_lincomb(double* dst, const double* a, c
 // Fills every component of direction D's face noise for sets A and B: component 0 (density)
 // is zero (timeStep.cpp:74-76 skips it), components 1..nvars-1 are N(0,1) * their scale.
 template <int D>
-__global__ void __launch_bounds__(BLK) k_noise_face(DevParams P, NoiseKey K, Arr A, Arr B)
+__device__ __forceinline__ void noise_face_body(const DevParams& P, const NoiseKey& K, Arr A, Arr B, long t)
 {
     const int fx = P.n[0] + (D == 0), fy = P.n[1] + (D == 1), fz = P.n[2] + (D == 2);
-    long t = (long)blockIdx.x * BLK + threadIdx.x;
     if (t >= (long)fx * fy * fz) return;
     int i, j, k;
     face_index(t, fx, fy, i, j, k);
@@ -934,10 +933,10 @@ __global__ void __launch_bounds__(BLK) k_noise_face(DevParams P, NoiseKey K, Arr
     }
 }
 
-__global__ void __launch_bounds__(BLK) k_noise_node(DevParams P, NoiseKey K, Arr A, Arr B, double scale)
+__device__ __forceinline__ void noise_node_body(const DevParams& P, const NoiseKey& K, Arr A, Arr B, double scale,
+                                                long t)
 {
     const int nx = P.n[0] + 1, ny = P.n[1] + 1, nz = P.n[2] + 1;
-    long t = (long)blockIdx.x * BLK + threadIdx.x;
     if (t >= (long)nx * ny * nz) return;
     int i, j, k;
     face_index(t, nx, ny, i, j, k);
@@ -951,6 +950,20 @@ __global__ void __launch_bounds__(BLK) k_noise_node(DevParams P, NoiseKey K, Arr
     noise_ab(K, NODE_FIELD + mask, lin, za, zb);
     A(i, j, k) = za * scale;
     B(i, j, k) = zb * scale;
+}
+
+// One launch for a step's whole noise set (blockIdx.y: face direction 0..2, 3 = nodes); every
+// value is drawn by the same body and key as in a launch of its own.
+__global__ void __launch_bounds__(BLK) k_noise_all(DevParams P, NoiseKey K, Arr A0, Arr B0, Arr A1, Arr B1,
+                                                   Arr A2, Arr B2, Arr rA, Arr rB)
+{
+    const long t = (long)blockIdx.x * BLK + threadIdx.x;
+    switch (blockIdx.y) {
+        case 0: noise_face_body<0>(P, K, A0, B0, t); break;
+        case 1: noise_face_body<1>(P, K, A1, B1, t); break;
+        case 2: noise_face_body<2>(P, K, A2, B2, t); break;
+        default: noise_node_body(P, K, rA, rB, P.var_mom, t); break;
+    }
 }
 
 // ---- the face passes, one launch for all three directions ------------------------------------
@@ -1182,10 +1195,9 @@ int ref_lincomb(double* dst, const double* a, const double* b, long npts, int nc
 int ref_fill_noise(const DevParams& P, const NoiseKey& K, const Arr A[3], const Arr B[3], Arr rA, Arr rB,
                    hipStream_t s)
 {
-    k_noise_face<0><<<grid_for(nface(P, 0), BLK), BLK, 0, s>>>(P, K, A[0], B[0]);
-    k_noise_face<1><<<grid_for(nface(P, 1), BLK), BLK, 0, s>>>(P, K, A[1], B[1]);
-    k_noise_face<2><<<grid_for(nface(P, 2), BLK), BLK, 0, s>>>(P, K, A[2], B[2]);
-    k_noise_node<<<grid_for(nnode(P), BLK), BLK, 0, s>>>(P, K, rA, rB, P.var_mom);
+    long mx = nnode(P);                       // the node box is the largest; each body checks its own
+    for (int d = 0; d < 3; ++d) mx = nface(P, d) > mx ? nface(P, d) : mx;
+    k_noise_all<<<dim3(grid_for(mx, BLK), 4), BLK, 0, s>>>(P, K, A[0], B[0], A[1], B[1], A[2], B[2], rA, rB);
     FHDX_HIP_TRY(hipGetLastError());
     return FHDX_OK;
 }
