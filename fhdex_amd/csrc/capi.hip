@@ -434,8 +434,7 @@ int rk3_ref(fhdx_ctx* c, fhdx_fab cu, fhdx_fab cup, fhdx_fab cup2, fhdx_fab prim
         }
         if ((r = ref_stage(P, make_arr(out[st]), make_arr(cu), make_arr(in[st]), fl, src, dt, st, s))) return r;
         if ((r = ref_c2p(P, a.prim, make_arr(out[st]), s))) return r;
-        if ((r = ref_fill_boundary(P, make_arr(out[st]), nv, s))) return r;
-        if ((r = ref_fill_boundary(P, a.prim, P.nprim, s))) return r;
+        if ((r = ref_fill_boundary2(P, make_arr(out[st]), nv, a.prim, P.nprim, s))) return r;
         if ((r = ref_set_bc(P, a.prim, make_arr(out[st]), s))) return r;  // timeStep.cpp:209, 314, 425
     }
     return FHDX_OK;

@@ -18,6 +18,8 @@ int ref_c2p(const DevParams& P, Arr prim, Arr cons, hipStream_t s);
 int ref_transport(const DevParams& P, Arr prim, Arr eta, Arr zeta, Arr kappa, Arr chi, Arr Dm, int* err,
                   hipStream_t s);
 int ref_fill_boundary(const DevParams& P, Arr a, int nc, hipStream_t s);
+// The same for two arrays in one launch (a stage's state and its primitive variables).
+int ref_fill_boundary2(const DevParams& P, Arr a, int nca, Arr b, int ncb, hipStream_t s);
 // setBC: the ghost cells outside walls, faces lo x .. hi z in turn (no-op when periodic).
 int ref_set_bc(const DevParams& P, Arr prim, Arr cons, hipStream_t s);
 int fill_value(double* p, long n, double v, hipStream_t s);

@@ -122,8 +122,12 @@ __global__ void __launch_bounds__(BLK) k_transport(DevParams P, Arr prim, Arr et
 
 // Periodic ghost fill of a cell FAB: ghost (i,j,k) <- valid (i mod n, ...) along periodic
 // directions; a ghost cell outside a wall has no periodic image and is left to setBC.
-__global__ void __launch_bounds__(BLK) k_fill_boundary(DevParams P, Arr a, int nc)
+// blockIdx.y selects the array: the step fills the stage state and its primitive variables
+// (independent arrays) in one launch.
+__global__ void __launch_bounds__(BLK) k_fill_boundary(DevParams P, Arr a0, int nc0, Arr a1, int nc1)
 {
+    const Arr a = blockIdx.y == 0 ? a0 : a1;
+    const int nc = blockIdx.y == 0 ? nc0 : nc1;
     const int g = P.ngc, n0 = P.n[0], n1 = P.n[1], n2 = P.n[2];
     const int gx = n0 + 2 * g, gy = n1 + 2 * g, gz = n2 + 2 * g;
     long t = (long)blockIdx.x * BLK + threadIdx.x;
@@ -1069,7 +1073,14 @@ int ref_transport(const DevParams& P, Arr prim, Arr eta, Arr zeta, Arr kappa, Ar
 
 int ref_fill_boundary(const DevParams& P, Arr a, int nc, hipStream_t s)
 {
-    k_fill_boundary<<<grid_for(ngrown(P), BLK), BLK, 0, s>>>(P, a, nc);
+    k_fill_boundary<<<grid_for(ngrown(P), BLK), BLK, 0, s>>>(P, a, nc, a, nc);
+    FHDX_HIP_TRY(hipGetLastError());
+    return FHDX_OK;
+}
+
+int ref_fill_boundary2(const DevParams& P, Arr a, int nca, Arr b, int ncb, hipStream_t s)
+{
+    k_fill_boundary<<<dim3(grid_for(ngrown(P), BLK), 2), BLK, 0, s>>>(P, a, nca, b, ncb);
     FHDX_HIP_TRY(hipGetLastError());
     return FHDX_OK;
 }
