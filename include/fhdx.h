@@ -114,7 +114,9 @@ void fhdx_params_default(fhdx_params* p);
 /* GetHcGas (src_compressible/compressible_functions.cpp:91-100): hcv/hcp < 0 -> from dof;
  * SetupBC (boundary.cpp:3-12) and the wall compositions of SetupCWall (boundary.cpp:14-170). */
 void fhdx_params_finalize(fhdx_params* p);
-/* 0 if the parameters are inside the implemented scope, else an fhdx_status. */
+/* 0 if the parameters are inside the implemented scope, else an fhdx_status.  Cell counts:
+ * at least 1 per periodic direction and 2 ngc per wall direction; the fused step, the
+ * ensemble engine and the server further need max(4, 2 ngc) per direction and no walls. */
 int fhdx_params_check(const fhdx_params* p);
 
 /* A context owns per-run device workspace; one per host thread.  Creating it selects nothing:
