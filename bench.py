@@ -27,8 +27,10 @@ def b_alg(nvars: int) -> int:
 
 
 def measured_traffic(n: int):
-    """HBM bytes per step from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this
-    workload (profiles/traffic_argon<n>.json), or None if there is none."""
+    """HBM-side bytes per step from the committed rocprofv3 counter summary of this workload
+    (profiles/traffic_argon<n>.json): the TCC_EA0 read requests by size (TCC_EA0_RDREQ_32B,
+    _64B and the 128-byte remainder) and write requests (TCC_EA0_WRREQ, _64B) times their sizes,
+    calibrated against a plain device copy of known size; None if there is no summary."""
     path = os.path.join(ROOT, "profiles", f"traffic_argon{n}.json")
     if not os.path.exists(path):
         return None
@@ -36,12 +38,26 @@ def measured_traffic(n: int):
         return json.load(f)["bytes_per_step"]
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(n: int, steps: int, n1: int = 32, steps1: int = 2, n4: int = 64, steps4: int = 4):
     """The CPU oracle (the reference's arithmetic in the reference's operation order, OpenMP over
-    z planes) on the host cores: all threads on argon n^3 x steps, one thread on argon
-    n1^3 x steps1, and all threads on the four-species gas n4^3 x steps4."""
+    z planes), built here with gcc -O3 -march=native -ffp-contract=off for this host, on the host
+    cores: all threads on argon n^3 x steps, one thread on argon n1^3 x steps1, and all threads
+    on the four-species gas n4^3 x steps4."""
     from fhdex_amd import params as P
     from oracle import ffi as orc
+
+    orc.use_native_build()
 
     def run(nn, ss, gas=P.argon_gas):
         cfg = gas(nn)
@@ -60,9 +76,11 @@ def cpu_baseline(n: int, steps: int, n1: int = 32, steps1: int = 2, n4: int = 64
     v1, t1 = run(n1, steps1)
     orc.set_num_threads(threads)
     v4, t4 = run(n4, steps4, P.four_species)
+    model = cpu_model()
     return {"value": v, "unit": "cell-updates/s", "cores": threads, "kind": "port",
-            "sample": f"oracle (CPU restatement of the reference step, gcc -O3, OpenMP), argon {n}^3 x "
-                      f"{steps} RK3 steps on {threads} threads, {t:.1f} s",
+            "cpu_model": model, "host_cpus": os.cpu_count(),
+            "sample": f"oracle (CPU restatement of the reference step, gcc -O3 -march=native, OpenMP), argon "
+                      f"{n}^3 x {steps} RK3 steps on {threads} threads of {model}, {t:.1f} s",
             "value_1thread": v1,
             "sample_1thread": f"argon {n1}^3 x {steps1} RK3 steps on 1 thread, {t1:.1f} s",
             "value_species4": v4,

@@ -26,14 +26,13 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 
 using namespace fhdx;
 
-// Device buffers a context owns: the reference's scratch MultiFabs for the drop-in RK3 step
-// and the noise sets when the caller supplies none.
+// Device buffers a context owns for the reference-shaped RK3 step: the step's two noise sets
+// when the caller supplies none, and the chemistry arrays.  Every other array of that step is
+// the caller's (the reference's RK3step argument list).
 struct RefWorkspace {
     bool ready = false;
     std::vector<double*> blocks;
-    fhdx_fab cup, cup2, prim, eta, zeta, kappa, chi, D;
-    fhdx_fab flux[3], stoch[3], sA[3], sB[3], corn[3][3];
-    fhdx_fab visccorn, rancorn, rA, rB;
+    fhdx_fab sA[3], sB[3], rA, rB;
     fhdx_fab source, ranA, ranB, ran;  // chemistry (nreaction > 0): source, reaction noise A, B, A + w B
 };
 
@@ -333,23 +332,10 @@ int ensure_ref_workspace(fhdx_ctx* c)
     auto node = [&](fhdx_fab& f) {
         if (!r) r = alloc_fab(w.blocks, f, 0, 0, 0, n0, n1, n2, 1);
     };
-    cell(w.cup, nv);
-    cell(w.cup2, nv);
-    cell(w.prim, 6 + 2 * NS);
-    cell(w.eta, 1);
-    cell(w.zeta, 1);
-    cell(w.kappa, 1);
-    cell(w.chi, NS);
-    cell(w.D, NS * NS);
     for (int d = 0; d < 3; ++d) {
-        face(w.flux[d], d, nv + 4);
-        face(w.stoch[d], d, nv);
         face(w.sA[d], d, nv);
         face(w.sB[d], d, nv);
-        for (int a = 0; a < 3; ++a) node(w.corn[a][d]);
     }
-    node(w.visccorn);
-    node(w.rancorn);
     node(w.rA);
     node(w.rB);
     if (c->p.nreaction > 0) {

@@ -169,7 +169,7 @@ struct Transport {
     double eta, zeta, kappa;
 };
 
-// Transport coefficients of one cell (transCoeffs.cpp:135-178).  chi[NS], D[NS*NS] out.
+// Transport coefficients of one cell (transCoeffs.cpp:37-88).  chi[NS], D[NS*NS] out.
 // Returns true if a mass fraction lies outside (0,1) (the reference aborts there).
 template <int NS>
 __device__ inline bool transport_cell(const DevParams& P, double rho, double T, double p,

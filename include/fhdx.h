@@ -10,7 +10,7 @@
  *   cell data  : indices [-ngc, n_d + ngc) per direction (ngc ghost layers);
  *   face data  : direction d nodal, indices [0, n_d] in d and [0, n_t) across, no ghosts;
  *   node data  : [0, n_d] in every direction, no ghosts.
- * Component maps (reference: main_driver.cpp:172-190, 248-260; transCoeffs.cpp:168-173):
+ * Component maps (reference: main_driver.cpp:172-190, 248-260; transCoeffs.cpp:76-80):
  *   cons  : 0 rho, 1-3 momentum, 4 rho*E, 5.. rho*Y_k                         (nvars = 5+NS)
  *   prim  : 0 rho, 1-3 velocity, 4 T, 5 p, 6.. Y_k, 6+NS.. X_k                 (nprimvars = 6+2NS)
  *   flux  : 0..nvars-1 as cons, nvars heat, +1 viscous heating (diag), +2 (shear), +3 Dufour

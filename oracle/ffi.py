@@ -23,6 +23,17 @@ def build():
     subprocess.check_call(["make", "-s", "-C", HERE])
 
 
+def use_native_build():
+    """Build liborc_native.so (gcc -march=native for this host, same operation order) and load
+    it instead of liborc.so.  For timing the CPU baseline on the machine that runs it; must be
+    called before the oracle is first used in the process."""
+    global LIB
+    if _lib is not None:
+        raise RuntimeError("the oracle is already loaded in this process")
+    subprocess.check_call(["make", "-s", "-C", HERE, "-B", "native"])
+    LIB = os.path.join(HERE, "liborc_native.so")
+
+
 def lib():
     global _lib
     if _lib is None:

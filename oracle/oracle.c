@@ -8,12 +8,12 @@
  * (left-to-right sums, the same parenthesisation, divisions where the reference divides),
  * because the species-noise Cholesky factors a matrix that is singular in exact arithmetic:
  * its last pivot is rounding noise, so any reordering shows up at ~1e-11 in rho*Y_k, j and
- * rho*E.  Build with -ffp-contract=off.  Only the fully periodic branches exist here (every
- * is_*_dirichlet_mass flag of flux.cpp:29-34 is false and every bc_vel factor is 1).
+ * rho*E.  Build with -ffp-contract=off.  The wall branches (every is_*_dirichlet_mass flag of
+ * flux.cpp:29-34, setBC, StochFlux, BCWallSpeciesFlux) and the periodic ones are both here.
  *
  * Functions and the reference code they follow:
  *   cons_to_prim     conservedPrimitiveConversions.cpp:48-89, compressible_functions.H:138-199
- *   transport        transCoeffs.cpp:129-181, compressible_functions.H:1153-1352 (GIO),
+ *   transport        transCoeffs.cpp:37-88, compressible_functions.H:1153-1352 (GIO),
  *                    :647-778 (D_GIO1)
  *   stoch_face       flux.cpp:71-1145 (+ CholeskyDecomp compressible_functions.H:460-509)
  *   diff_face        flux.cpp:1193-1623
@@ -21,9 +21,19 @@
  *   corner_face      flux.cpp:1770-1980
  *   hyp_face         flux.cpp:2016-2175 (advection_type 1), :2180-2372 (advection_type 2)
  *   rk3_step         timeStep.cpp:11-427 (LinComb of the noise sets at :118-128 etc.)
- *   fill_boundary    MultiFab::FillBoundary(periodicity) on one periodic box
- *   init_cons        compressible_functions.cpp:131-147, :292-302
+ *   fill_boundary    MultiFab::FillBoundary(periodicity) (periodic directions only)
+ *   set_bc           boundary.cpp:172-1209; stoch_flux_walls :1392-1970; wall_species_flux
+ *                    :1212-1390
+ *   init_cons        compressible_functions.cpp:103-413 (prob_types 1-4, 100)
  *   noise            replaces MultiFabFillRandom.cpp:5-31 (Philox4x32-10 + Box-Muller)
+ *   chem_source      src_chemistry/chemistry_functions.cpp:128-193
+ *   evaluate_stats   src_compressible/stats.cpp:5-363
+ *
+ * How each is pinned is tested under tests/test_oracle*.py (Random123 known answers, closed
+ * forms, invariants, textbook first approximations of the mixture transport).  Parity unpinned:
+ * for three or more species, D_GIO1's mixture-averaged preconditioner changes D only within the
+ * truncation of its three-term series (<= 2e-3 against the exact Stefan-Maxwell solution), so
+ * that sub-expression is pinned to the truncation level, not to the bit.
  */
 #include "oracle.h"
 
@@ -243,8 +253,8 @@ static void d_gio1(int NS, double Dbin[MS][MS], const double* Ykp, const double*
 }
 
 /* IdealMixtureTransportGIO (compressible_functions.H:1153-1352) followed by the rho scaling
- * and bulk viscosity of transCoeffs.cpp:167-178.  Returns 1 if a mass fraction is outside
- * (0,1) (the reference aborts at transCoeffs.cpp:137-138). */
+ * (transCoeffs.cpp:76-80) and bulk viscosity (:84-86).  Returns 1 if a mass fraction is outside
+ * (0,1) (the reference aborts at transCoeffs.cpp:45-46). */
 static int transport_cell(const fhdx_params* P, const double* pr, double* eta, double* zeta,
                           double* kappa, double* chi, double* D)
 {

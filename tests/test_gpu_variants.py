@@ -2,7 +2,7 @@
 
 Each variant changes one switch of the reference's namelist on a three-species box: primitive
 (advection_type 1) or no (0) hyperbolic interpolation (flux.cpp:2016-2372), bulk viscosity
-(visc_type 3 with zeta = zeta_ratio * eta, transCoeffs.cpp:168-173), the non-symmetric stress
+(visc_type 3 with zeta = zeta_ratio * eta, transCoeffs.cpp:84-86), the non-symmetric stress
 (visc_type 1), single-component transport (algorithm_type 1: no species fluxes), gravity
 (timeStep.cpp:190-201) and deterministic fluxes (stoch_stress_form 0).  One step of the
 fused path and of the reference-shaped path, with the same injected noise, against the oracle;

@@ -342,7 +342,7 @@ def test_sound_wave_speed_and_damping(orc, adv, visc, zr):
     Gamma = k^2/(2 rho) (4/3 eta + zeta + kappa (1/c_v - 1/c_p)): pins the hyperbolic flux, the
     corner viscous terms and heat conduction together, for conserved (advection_type 2) and
     primitive (1) interpolation, which agree at linear order, with bulk viscosity
-    zeta = zeta_ratio * eta (visc_type 3, transCoeffs.cpp:168-173).  visc_type 1 damps like 2:
+    zeta = zeta_ratio * eta (visc_type 3, transCoeffs.cpp:84-86).  visc_type 1 damps like 2:
     the collocated flux only tests |visc_type| == 3 (flux.cpp:182, 546, 912, 1635), so the
     inputs files' "not-symmetric" option is the symmetric stress in this scheme."""
     from scipy.optimize import curve_fit
