@@ -202,15 +202,6 @@ def measured_flops(n: int):
         return json.load(f)["flops_per_cell_update"]
 
 
-def ensemble_members(count: int, seed: int = 2024, lo: int = 16, hi: int = 64):
-    """Box shapes of the batched-ensemble workload: per-axis edge round(exp(U(ln lo, ln hi)))
-    drawn independently per axis from a fixed seed."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    e = np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), size=(count, 3)))).astype(int)
-    return [tuple(int(v) for v in row) for row in e]
-
-
 def ensemble_bench(count: int, steps: int, loop_steps: int):
     """512 independent argon boxes of uneven size, one noise seed each: the engine's
     graph-captured batched step against the plain loop (the single-box fast path called for
@@ -222,7 +213,7 @@ def ensemble_bench(count: int, steps: int, loop_steps: int):
     from fhdex_amd import lib as L
     from fhdex_amd import params as P
 
-    shapes = ensemble_members(count)
+    shapes = E.ensemble_members(count)
     cfgs = [P.argon_gas(0, shape=sh) for sh in shapes]
     cells = sum(a * b * c for a, b, c in shapes)
     budget = sum(E.bytes_needed(c.params) for c in cfgs)

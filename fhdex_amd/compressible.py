@@ -15,7 +15,8 @@ from .params import Fab, Params
 
 
 class MultiFab:
-    """One FAB of a single-box periodic domain: a device tensor plus its index box."""
+    """One FAB: a device tensor plus its index box.  The constructors take the valid box as
+    (lo, hi) global cell indices, inclusive; by default the whole domain (a single-box MultiFab)."""
 
     def __init__(self, t, lo):
         self.t = t
@@ -26,27 +27,36 @@ class MultiFab:
         return fab_of_tensor(self.t, self.lo)
 
     @staticmethod
-    def cell(p: Params, ncomp: int, device="cuda"):
-        import torch
-        n0, n1, n2 = p.n_cells
-        g = p.ngc
-        t = torch.zeros((ncomp, n2 + 2 * g, n1 + 2 * g, n0 + 2 * g), dtype=torch.float64, device=device)
-        return MultiFab(t, (-g, -g, -g))
+    def _box(p: Params, box):
+        if box is None:
+            return (0, 0, 0), tuple(n - 1 for n in p.n_cells)
+        return tuple(box[0]), tuple(box[1])
 
     @staticmethod
-    def face(p: Params, d: int, ncomp: int, device="cuda"):
+    def cell(p: Params, ncomp: int, device="cuda", box=None):
         import torch
-        n0, n1, n2 = p.n_cells
+        lo, hi = MultiFab._box(p, box)
+        g = p.ngc
+        n0, n1, n2 = (hi[d] - lo[d] + 1 for d in range(3))
+        t = torch.zeros((ncomp, n2 + 2 * g, n1 + 2 * g, n0 + 2 * g), dtype=torch.float64, device=device)
+        return MultiFab(t, tuple(l - g for l in lo))
+
+    @staticmethod
+    def face(p: Params, d: int, ncomp: int, device="cuda", box=None):
+        import torch
+        lo, hi = MultiFab._box(p, box)
+        n0, n1, n2 = (hi[e] - lo[e] + 1 for e in range(3))
         t = torch.zeros((ncomp, n2 + (d == 2), n1 + (d == 1), n0 + (d == 0)), dtype=torch.float64,
                         device=device)
-        return MultiFab(t, (0, 0, 0))
+        return MultiFab(t, lo)
 
     @staticmethod
-    def node(p: Params, ncomp: int = 1, device="cuda"):
+    def node(p: Params, ncomp: int = 1, device="cuda", box=None):
         import torch
-        n0, n1, n2 = p.n_cells
+        lo, hi = MultiFab._box(p, box)
+        n0, n1, n2 = (hi[e] - lo[e] + 1 for e in range(3))
         t = torch.zeros((ncomp, n2 + 1, n1 + 1, n0 + 1), dtype=torch.float64, device=device)
-        return MultiFab(t, (0, 0, 0))
+        return MultiFab(t, lo)
 
 
 def _fa(mfs):
@@ -54,34 +64,43 @@ def _fa(mfs):
 
 
 class DeviceState:
-    """Every MultiFab of RK3step's argument list (main_driver.cpp:160-297) on the device."""
+    """Every MultiFab of RK3step's argument list (main_driver.cpp:160-297) on the device, for one
+    box (the whole domain by default); with chemistry also source (nvars) and ranchem
+    (nreaction), as main_driver.cpp:241-245 defines them."""
 
-    def __init__(self, p: Params, device="cuda"):
+    def __init__(self, p: Params, device="cuda", box=None):
         self.p = p
+        self.box = box
         nv, NS = p.nvars, p.nspecies
-        cell = lambda nc: MultiFab.cell(p, nc, device)  # noqa: E731
+        cell = lambda nc: MultiFab.cell(p, nc, device, box)  # noqa: E731
+        node = lambda: MultiFab.node(p, 1, device, box)  # noqa: E731
         self.cu, self.cup, self.cup2 = cell(nv), cell(nv), cell(nv)
         self.prim = cell(p.nprimvars)
         self.eta, self.zeta, self.kappa = cell(1), cell(1), cell(1)
         self.chi, self.D = cell(NS), cell(NS * NS)
-        self.flux = [MultiFab.face(p, d, nv + 4, device) for d in range(3)]
-        self.stochFlux = [MultiFab.face(p, d, nv, device) for d in range(3)]
-        self.cornx = [MultiFab.node(p, 1, device) for _ in range(3)]
-        self.corny = [MultiFab.node(p, 1, device) for _ in range(3)]
-        self.cornz = [MultiFab.node(p, 1, device) for _ in range(3)]
-        self.visccorn = MultiFab.node(p, 1, device)
-        self.rancorn = MultiFab.node(p, 1, device)
+        self.flux = [MultiFab.face(p, d, nv + 4, device, box) for d in range(3)]
+        self.stochFlux = [MultiFab.face(p, d, nv, device, box) for d in range(3)]
+        self.cornx = [node() for _ in range(3)]
+        self.corny = [node() for _ in range(3)]
+        self.cornz = [node() for _ in range(3)]
+        self.visccorn = node()
+        self.rancorn = node()
+        self.source = cell(nv) if p.nreaction > 0 else None
+        self.ranchem = cell(p.nreaction) if p.nreaction > 0 else None
 
 
 class NoiseSets:
-    """The A/B noise of one step (stochFlux_A/B, rancorn_A/B of timeStep.cpp:38-53)."""
+    """The A/B noise of one step (stochFlux_A/B, rancorn_A/B of timeStep.cpp:38-53) for one box;
+    with chemistry also ranchem_A/B (timeStep.cpp:64-72), as cell FABs with ngc ghosts."""
 
-    def __init__(self, p: Params, device="cuda"):
+    def __init__(self, p: Params, device="cuda", box=None):
         nv = p.nvars
-        self.A = [MultiFab.face(p, d, nv, device) for d in range(3)]
-        self.B = [MultiFab.face(p, d, nv, device) for d in range(3)]
-        self.rA = MultiFab.node(p, 1, device)
-        self.rB = MultiFab.node(p, 1, device)
+        self.A = [MultiFab.face(p, d, nv, device, box) for d in range(3)]
+        self.B = [MultiFab.face(p, d, nv, device, box) for d in range(3)]
+        self.rA = MultiFab.node(p, 1, device, box)
+        self.rB = MultiFab.node(p, 1, device, box)
+        self.chemA = MultiFab.cell(p, p.nreaction, device, box) if p.nreaction > 0 else None
+        self.chemB = MultiFab.cell(p, p.nreaction, device, box) if p.nreaction > 0 else None
 
 
 def conservedToPrimitive(ctx: Context, prim: MultiFab, cons: MultiFab, stream=None):
@@ -130,6 +149,22 @@ def RK3step(ctx: Context, st: DeviceState, dt: float, noise: NoiseSets | None = 
                                   _fa(st.stochFlux), _fa(st.cornx), _fa(st.corny), _fa(st.cornz),
                                   st.visccorn.fab, st.rancorn.fab, *args, dt, seed, step, stream),
           "RK3step")
+
+
+def RK3stage(ctx: Context, stage: int, st: DeviceState, noise: NoiseSets, dt: float, stream=None):
+    """Stage `stage` (0, 1, 2) of RK3step on one FAB (fhdx_rk3_stage_ref): no ghost cell is
+    filled; between stages the caller exchanges the ghosts of the stage output and prim across
+    the FABs and applies setBC, as timeStep.cpp:204-209 does."""
+    z = Fab()
+    chem = ctx.p.nreaction > 0
+    check(lib().fhdx_rk3_stage_ref(ctx.h, int(stage), st.cu.fab, st.cup.fab, st.cup2.fab, st.prim.fab,
+                                   st.eta.fab, st.zeta.fab, st.kappa.fab, st.chi.fab, st.D.fab,
+                                   _fa(st.flux), _fa(st.stochFlux), _fa(st.cornx), _fa(st.corny),
+                                   _fa(st.cornz), st.visccorn.fab, st.rancorn.fab, _fa(noise.A),
+                                   _fa(noise.B), noise.rA.fab, noise.rB.fab,
+                                   st.source.fab if chem else z, st.ranchem.fab if chem else z,
+                                   noise.chemA.fab if chem else z, noise.chemB.fab if chem else z,
+                                   float(dt), stream), "RK3stage")
 
 
 def rk3_step(ctx: Context, cu: MultiFab, prim: MultiFab, dt: float, seed: int, step: int, stream=None):

@@ -270,29 +270,125 @@ int fhdx_params_check(const fhdx_params* p)
 
 namespace {
 
-bool check_cell(const fhdx_ctx* c, const fhdx_fab& f, int ncomp, const char* what)
+// A valid box of the domain (global cell indices): lower corner and cells per direction.  A
+// MultiFab split into several boxes hands each FAB to the library on its own; every array of
+// one call belongs to the same box.
+struct Box {
+    int lo[3], n[3];
+};
+
+Box domain_box(const fhdx_ctx* c)
+{
+    Box b;
+    for (int d = 0; d < 3; ++d) {
+        b.lo[d] = 0;
+        b.n[d] = c->p.n_cells[d];
+    }
+    return b;
+}
+
+bool is_domain(const fhdx_ctx* c, const Box& b)
+{
+    for (int d = 0; d < 3; ++d)
+        if (b.lo[d] != 0 || b.n[d] != c->p.n_cells[d]) return false;
+    return true;
+}
+
+// A box must lie inside the domain, and a box touching a wall must hold the ngc cells that setBC
+// mirrors into the ghosts there (boundary.cpp:172-1209).
+bool box_ok(const fhdx_ctx* c, const Box& b, const char* what)
+{
+    bool ok = true;
+    for (int d = 0; d < 3; ++d) {
+        ok = ok && b.n[d] >= 1 && b.lo[d] >= 0 && b.lo[d] + b.n[d] <= c->p.n_cells[d];
+        const bool at_wall = !c->dp.periodic[d] && (b.lo[d] == 0 || b.lo[d] + b.n[d] == c->p.n_cells[d]);
+        ok = ok && !(at_wall && b.n[d] < c->p.ngc);
+    }
+    if (!ok)
+        set_error(std::string("bad box: ") + what +
+                  " (a box must lie inside the domain and hold >= ngc cells across a wall it touches)");
+    return ok;
+}
+
+// The valid box of a cell FAB with ngc ghost layers.
+bool cell_box(const fhdx_ctx* c, const fhdx_fab& f, int ncomp, const char* what, Box& b)
+{
+    const int g = c->p.ngc;
+    if (!f.p || f.ncomp < ncomp) {
+        set_error(std::string("bad cell array: ") + what);
+        return false;
+    }
+    for (int d = 0; d < 3; ++d) {
+        b.lo[d] = f.lo[d] + g;
+        b.n[d] = f.hi[d] - f.lo[d] + 1 - 2 * g;
+    }
+    return box_ok(c, b, what);
+}
+
+// The valid box of a node FAB (no ghosts).
+bool node_box(const fhdx_ctx* c, const fhdx_fab& f, const char* what, Box& b)
+{
+    if (!f.p || f.ncomp < 1) {
+        set_error(std::string("bad node array: ") + what);
+        return false;
+    }
+    for (int d = 0; d < 3; ++d) {
+        b.lo[d] = f.lo[d];
+        b.n[d] = f.hi[d] - f.lo[d];
+    }
+    return box_ok(c, b, what);
+}
+
+bool check_cell(const fhdx_ctx* c, const Box& b, const fhdx_fab& f, int ncomp, const char* what)
 {
     const int g = c->p.ngc;
     bool ok = f.p != nullptr && f.ncomp >= ncomp;
-    for (int d = 0; d < 3; ++d) ok = ok && f.lo[d] == -g && f.hi[d] == c->p.n_cells[d] - 1 + g;
+    for (int d = 0; d < 3; ++d) ok = ok && f.lo[d] == b.lo[d] - g && f.hi[d] == b.lo[d] + b.n[d] - 1 + g;
     if (!ok) set_error(std::string("bad cell array shape: ") + what);
     return ok;
 }
 
-bool check_face(const fhdx_ctx* c, const fhdx_fab& f, int dir, int ncomp, const char* what)
+bool check_face(const fhdx_ctx* c, const Box& b, const fhdx_fab& f, int dir, int ncomp, const char* what)
 {
     bool ok = f.p != nullptr && f.ncomp >= ncomp;
-    for (int d = 0; d < 3; ++d) ok = ok && f.lo[d] == 0 && f.hi[d] == c->p.n_cells[d] - 1 + (d == dir);
+    for (int d = 0; d < 3; ++d) ok = ok && f.lo[d] == b.lo[d] && f.hi[d] == b.lo[d] + b.n[d] - 1 + (d == dir);
     if (!ok) set_error(std::string("bad face array shape: ") + what);
     return ok;
 }
 
-bool check_node(const fhdx_ctx* c, const fhdx_fab& f, const char* what)
+bool check_node(const fhdx_ctx* c, const Box& b, const fhdx_fab& f, const char* what)
 {
     bool ok = f.p != nullptr && f.ncomp >= 1;
-    for (int d = 0; d < 3; ++d) ok = ok && f.lo[d] == 0 && f.hi[d] == c->p.n_cells[d];
+    for (int d = 0; d < 3; ++d) ok = ok && f.lo[d] == b.lo[d] && f.hi[d] == b.lo[d] + b.n[d];
     if (!ok) set_error(std::string("bad node array shape: ") + what);
     return ok;
+}
+
+// Whole-domain forms (entry points that fill periodic ghosts themselves take one box only).
+bool check_cell(const fhdx_ctx* c, const fhdx_fab& f, int ncomp, const char* what)
+{
+    return check_cell(c, domain_box(c), f, ncomp, what);
+}
+
+bool check_face(const fhdx_ctx* c, const fhdx_fab& f, int dir, int ncomp, const char* what)
+{
+    return check_face(c, domain_box(c), f, dir, ncomp, what);
+}
+
+bool check_node(const fhdx_ctx* c, const fhdx_fab& f, const char* what)
+{
+    return check_node(c, domain_box(c), f, what);
+}
+
+// The context's device parameters with the launch box set to b.
+DevParams box_params(const fhdx_ctx* c, const Box& b)
+{
+    DevParams P = c->dp;
+    for (int d = 0; d < 3; ++d) {
+        P.bo[d] = b.lo[d];
+        P.bn[d] = b.n[d];
+    }
+    return P;
 }
 
 hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
@@ -362,66 +458,84 @@ long npts(const fhdx_fab& f)
     return (long)(f.hi[0] - f.lo[0] + 1) * (f.hi[1] - f.lo[1] + 1) * (f.hi[2] - f.lo[2] + 1);
 }
 
-// The reference RK3step with the reference's arrays (timeStep.cpp:11-427).
-int rk3_ref(fhdx_ctx* c, fhdx_fab cu, fhdx_fab cup, fhdx_fab cup2, fhdx_fab prim, fhdx_fab eta,
-            fhdx_fab zeta, fhdx_fab kappa, fhdx_fab chi, fhdx_fab D, const fhdx_fab flux[3],
-            const fhdx_fab stochFlux[3], const fhdx_fab cornx[3], const fhdx_fab corny[3],
-            const fhdx_fab cornz[3], fhdx_fab visccorn, fhdx_fab rancorn, const fhdx_fab sA[3],
-            const fhdx_fab sB[3], fhdx_fab rA, fhdx_fab rB, double dt, hipStream_t s)
+// The arrays of one reference-shaped RK3 stage on one box: the reference's own argument list
+// of RK3step (compressible_functions.H:41-51), the step's noise sets (stochFlux_A/B,
+// rancorn_A/B of timeStep.cpp:38-53) and, with chemistry, source, ranchem and its noise sets.
+struct StageArrays {
+    fhdx_fab cu, cup, cup2, prim, eta, zeta, kappa, chi, D;
+    fhdx_fab flux[3], stochFlux[3], cornx[3], corny[3], cornz[3], visccorn, rancorn;
+    fhdx_fab sA[3], sB[3], rA, rB;
+    fhdx_fab source, ranchem, ranchemA, ranchemB;
+};
+
+// Stage st (0, 1, 2) of RK3step on the launch box of P, from the transport coefficients of the
+// stage input to conservedToPrimitive of its output (timeStep.cpp:99-202, 211-307, 316-415):
+// transport on the grown box, the stage-weighted noise, calculateFlux, the chemistry source,
+// the stage combination and prim on the valid cells.  The ghost cells of the output state and
+// of prim are left to the caller (FillBoundary, then setBC: timeStep.cpp:204-209, 309-314,
+// 417-425).
+int rk3_stage(const fhdx_ctx* c, const DevParams& P, const StageArrays& A, int st, double dt, hipStream_t s)
 {
     const bool chem = c->p.nreaction > 0;
-    int r0 = chem ? ensure_ref_workspace(c) : FHDX_OK;
-    if (r0) return r0;
-    const RefWorkspace& rw = c->ref;
-    Arr src{};
-    if (chem) src = make_arr(rw.source);
-    const DevParams& P = c->dp;
     const int nv = P.nvars;
     double ws[3];
     stage_weights(ws);
-    fhdx_fab in[3] = {cu, cup, cup2};
-    fhdx_fab out[3] = {cup, cup2, cu};
+    const fhdx_fab in[3] = {A.cu, A.cup, A.cup2};
+    const fhdx_fab out[3] = {A.cup, A.cup2, A.cu};
     FluxArgs a;
-    a.prim = make_arr(prim);
-    a.eta = make_arr(eta);
-    a.zeta = make_arr(zeta);
-    a.kappa = make_arr(kappa);
-    a.chi = make_arr(chi);
-    a.D = make_arr(D);
+    a.prim = make_arr(A.prim);
+    a.eta = make_arr(A.eta);
+    a.zeta = make_arr(A.zeta);
+    a.kappa = make_arr(A.kappa);
+    a.chi = make_arr(A.chi);
+    a.D = make_arr(A.D);
     Arr fl[3];
     for (int d = 0; d < 3; ++d) {
-        a.flux[d] = fl[d] = make_arr(flux[d]);
-        a.ran[d] = make_arr(stochFlux[d]);
-        a.corn[0][d] = make_arr(cornx[d]);
-        a.corn[1][d] = make_arr(corny[d]);
-        a.corn[2][d] = make_arr(cornz[d]);
+        a.flux[d] = fl[d] = make_arr(A.flux[d]);
+        a.ran[d] = make_arr(A.stochFlux[d]);
+        a.corn[0][d] = make_arr(A.cornx[d]);
+        a.corn[1][d] = make_arr(A.corny[d]);
+        a.corn[2][d] = make_arr(A.cornz[d]);
     }
-    a.visccorn = make_arr(visccorn);
-    a.rancorn = make_arr(rancorn);
+    a.visccorn = make_arr(A.visccorn);
+    a.rancorn = make_arr(A.rancorn);
     a.dt = dt;
+    a.cons = make_arr(in[st]);
+    int r;
+    if ((r = ref_transport(P, a.prim, a.eta, a.zeta, a.kappa, a.chi, a.D, c->d_err, s))) return r;
+    {
+        double* const dst[4] = {A.stochFlux[0].p, A.stochFlux[1].p, A.stochFlux[2].p, A.rancorn.p};
+        const double* const la[4] = {A.sA[0].p, A.sA[1].p, A.sA[2].p, A.rA.p};
+        const double* const lb[4] = {A.sB[0].p, A.sB[1].p, A.sB[2].p, A.rB.p};
+        const long np[4] = {npts(A.stochFlux[0]), npts(A.stochFlux[1]), npts(A.stochFlux[2]), npts(A.rancorn)};
+        const int lnc[4] = {nv, nv, nv, 1}, lc0[4] = {1, 1, 1, 0};
+        if ((r = ref_lincomb4(dst, la, lb, np, lnc, lc0, ws[st], s))) return r;
+    }
+    const int fnc[3] = {A.flux[0].ncomp, A.flux[1].ncomp, A.flux[2].ncomp};
+    if ((r = ref_calculate_flux(P, a, const_cast<int*>(fnc), s))) return r;
+    Arr src{};
+    if (chem) {  // timeStep.cpp:135-143: ranchem = A + w B, then the source from this stage's prim
+        src = make_arr(A.source);
+        if ((r = ref_lincomb(A.ranchem.p, A.ranchemA.p, A.ranchemB.p, npts(A.ranchem), c->p.nreaction, 0, ws[st],
+                             s)))
+            return r;
+        if ((r = ref_chem_source(P, a.prim, src, make_arr(A.ranchem), dt, s))) return r;
+    }
+    if ((r = ref_stage(P, make_arr(out[st]), make_arr(A.cu), make_arr(in[st]), fl, src, dt, st, s))) return r;
+    return ref_c2p(P, a.prim, make_arr(out[st]), s);
+}
+
+// The reference RK3step with the reference's arrays (timeStep.cpp:11-427) on a single-box
+// domain: each stage, then the periodic images and setBC of the new state and prim.
+int rk3_ref(fhdx_ctx* c, const StageArrays& A, double dt, hipStream_t s)
+{
+    const DevParams& P = c->dp;
+    const fhdx_fab out[3] = {A.cup, A.cup2, A.cu};
     for (int st = 0; st < 3; ++st) {
         int r;
-        if ((r = ref_transport(P, a.prim, a.eta, a.zeta, a.kappa, a.chi, a.D, c->d_err, s))) return r;
-        {
-            double* const dst[4] = {stochFlux[0].p, stochFlux[1].p, stochFlux[2].p, rancorn.p};
-            const double* const la[4] = {sA[0].p, sA[1].p, sA[2].p, rA.p};
-            const double* const lb[4] = {sB[0].p, sB[1].p, sB[2].p, rB.p};
-            const long np[4] = {npts(stochFlux[0]), npts(stochFlux[1]), npts(stochFlux[2]), npts(rancorn)};
-            const int lnc[4] = {nv, nv, nv, 1}, lc0[4] = {1, 1, 1, 0};
-            if ((r = ref_lincomb4(dst, la, lb, np, lnc, lc0, ws[st], s))) return r;
-        }
-        a.cons = make_arr(in[st]);
-        int fnc[3] = {flux[0].ncomp, flux[1].ncomp, flux[2].ncomp};
-        if ((r = ref_calculate_flux(P, a, fnc, s))) return r;
-        if (chem) {  // timeStep.cpp:135-143: ranchem = A + w B, then the source from this stage's prim
-            if ((r = ref_lincomb(rw.ran.p, rw.ranA.p, rw.ranB.p, npts(rw.ran), c->p.nreaction, 0, ws[st], s)))
-                return r;
-            if ((r = ref_chem_source(P, a.prim, src, make_arr(rw.ran), dt, s))) return r;
-        }
-        if ((r = ref_stage(P, make_arr(out[st]), make_arr(cu), make_arr(in[st]), fl, src, dt, st, s))) return r;
-        if ((r = ref_c2p(P, a.prim, make_arr(out[st]), s))) return r;
-        if ((r = ref_fill_boundary2(P, make_arr(out[st]), nv, a.prim, P.nprim, s))) return r;
-        if ((r = ref_set_bc(P, a.prim, make_arr(out[st]), s))) return r;  // timeStep.cpp:209, 314, 425
+        if ((r = rk3_stage(c, P, A, st, dt, s))) return r;
+        if ((r = ref_fill_boundary2(P, make_arr(out[st]), P.nvars, make_arr(A.prim), P.nprim, s))) return r;
+        if ((r = ref_set_bc(P, make_arr(A.prim), make_arr(out[st]), s))) return r;  // timeStep.cpp:209, 314, 425
     }
     return FHDX_OK;
 }
@@ -503,8 +617,10 @@ int fhdx_ctx_check(fhdx_ctx* c, void* stream)
 int fhdx_cons_to_prim(fhdx_ctx* c, fhdx_fab prim, fhdx_fab cons, void* stream)
 {
     if (!c) return FHDX_ERR_ARG;
-    if (!check_cell(c, prim, c->dp.nprim, "prim") || !check_cell(c, cons, c->dp.nvars, "cons")) return FHDX_ERR_ARG;
-    return ref_c2p(c->dp, make_arr(prim), make_arr(cons), S(stream));
+    Box b;
+    if (!cell_box(c, prim, c->dp.nprim, "prim", b) || !check_cell(c, b, cons, c->dp.nvars, "cons"))
+        return FHDX_ERR_ARG;
+    return ref_c2p(box_params(c, b), make_arr(prim), make_arr(cons), S(stream));
 }
 
 int fhdx_transport(fhdx_ctx* c, fhdx_fab prim, fhdx_fab eta, fhdx_fab zeta, fhdx_fab kappa, fhdx_fab chi,
@@ -512,27 +628,30 @@ int fhdx_transport(fhdx_ctx* c, fhdx_fab prim, fhdx_fab eta, fhdx_fab zeta, fhdx
 {
     if (!c) return FHDX_ERR_ARG;
     const int NS = c->p.nspecies;
-    if (!check_cell(c, prim, c->dp.nprim, "prim") || !check_cell(c, eta, 1, "eta") ||
-        !check_cell(c, zeta, 1, "zeta") || !check_cell(c, kappa, 1, "kappa") || !check_cell(c, chi, NS, "chi") ||
-        !check_cell(c, D, NS * NS, "D"))
+    Box b;
+    if (!cell_box(c, prim, c->dp.nprim, "prim", b) || !check_cell(c, b, eta, 1, "eta") ||
+        !check_cell(c, b, zeta, 1, "zeta") || !check_cell(c, b, kappa, 1, "kappa") ||
+        !check_cell(c, b, chi, NS, "chi") || !check_cell(c, b, D, NS * NS, "D"))
         return FHDX_ERR_ARG;
-    return ref_transport(c->dp, make_arr(prim), make_arr(eta), make_arr(zeta), make_arr(kappa), make_arr(chi),
+    return ref_transport(box_params(c, b), make_arr(prim), make_arr(eta), make_arr(zeta), make_arr(kappa), make_arr(chi),
                          make_arr(D), c->d_err, S(stream));
 }
 
 int fhdx_fill_boundary(fhdx_ctx* c, fhdx_fab mf, void* stream)
 {
     if (!c) return FHDX_ERR_ARG;
-    if (!check_cell(c, mf, 1, "mf")) return FHDX_ERR_ARG;
+    if (!check_cell(c, mf, 1, "mf (the whole domain: a box of a split domain takes its ghosts from the others)"))
+        return FHDX_ERR_ARG;
     return ref_fill_boundary(c->dp, make_arr(mf), mf.ncomp, S(stream));
 }
 
 int fhdx_set_bc(fhdx_ctx* c, fhdx_fab prim, fhdx_fab cons, void* stream)
 {
     if (!c) return FHDX_ERR_ARG;
-    if (!check_cell(c, prim, c->dp.nprim, "prim") || !check_cell(c, cons, c->dp.nvars, "cons"))
+    Box b;
+    if (!cell_box(c, prim, c->dp.nprim, "prim", b) || !check_cell(c, b, cons, c->dp.nvars, "cons"))
         return FHDX_ERR_ARG;
-    return ref_set_bc(c->dp, make_arr(prim), make_arr(cons), S(stream));
+    return ref_set_bc(box_params(c, b), make_arr(prim), make_arr(cons), S(stream));
 }
 
 int fhdx_calculate_flux(fhdx_ctx* c, fhdx_fab cons, fhdx_fab prim, fhdx_fab eta, fhdx_fab zeta,
@@ -542,13 +661,16 @@ int fhdx_calculate_flux(fhdx_ctx* c, fhdx_fab cons, fhdx_fab prim, fhdx_fab eta,
 {
     if (!c || !flux || !stochFlux || !cornx || !corny || !cornz) return FHDX_ERR_ARG;
     const int NS = c->p.nspecies, nv = c->dp.nvars;
-    bool ok = check_cell(c, cons, nv, "cons") && check_cell(c, prim, c->dp.nprim, "prim") &&
-              check_cell(c, eta, 1, "eta") && check_cell(c, zeta, 1, "zeta") && check_cell(c, kappa, 1, "kappa") &&
-              check_cell(c, chi, NS, "chi") && check_cell(c, D, NS * NS, "D") &&
-              check_node(c, visccorn, "visccorn") && check_node(c, rancorn, "rancorn");
+    Box b;
+    bool ok = cell_box(c, cons, nv, "cons", b) && check_cell(c, b, prim, c->dp.nprim, "prim") &&
+              check_cell(c, b, eta, 1, "eta") && check_cell(c, b, zeta, 1, "zeta") &&
+              check_cell(c, b, kappa, 1, "kappa") && check_cell(c, b, chi, NS, "chi") &&
+              check_cell(c, b, D, NS * NS, "D") && check_node(c, b, visccorn, "visccorn") &&
+              check_node(c, b, rancorn, "rancorn");
     for (int d = 0; ok && d < 3; ++d)
-        ok = check_face(c, flux[d], d, nv + 4, "flux") && check_face(c, stochFlux[d], d, nv, "stochFlux") &&
-             check_node(c, cornx[d], "cornx") && check_node(c, corny[d], "corny") && check_node(c, cornz[d], "cornz");
+        ok = check_face(c, b, flux[d], d, nv + 4, "flux") && check_face(c, b, stochFlux[d], d, nv, "stochFlux") &&
+             check_node(c, b, cornx[d], "cornx") && check_node(c, b, corny[d], "corny") &&
+             check_node(c, b, cornz[d], "cornz");
     if (!ok) return FHDX_ERR_ARG;
     if (!(dt > 0.)) return FHDX_ERR_ARG;
     FluxArgs a;
@@ -570,7 +692,7 @@ int fhdx_calculate_flux(fhdx_ctx* c, fhdx_fab cons, fhdx_fab prim, fhdx_fab eta,
     a.rancorn = make_arr(rancorn);
     a.dt = dt;
     int fnc[3] = {flux[0].ncomp, flux[1].ncomp, flux[2].ncomp};
-    return ref_calculate_flux(c->dp, a, fnc, S(stream));
+    return ref_calculate_flux(box_params(c, b), a, fnc, S(stream));
 }
 
 int fhdx_fill_noise(fhdx_ctx* c, uint64_t seed, uint64_t step, uint32_t instance, const fhdx_fab stochA[3],
@@ -578,9 +700,10 @@ int fhdx_fill_noise(fhdx_ctx* c, uint64_t seed, uint64_t step, uint32_t instance
 {
     if (!c || !stochA || !stochB) return FHDX_ERR_ARG;
     const int nv = c->dp.nvars;
-    bool ok = check_node(c, rancornA, "rancornA") && check_node(c, rancornB, "rancornB");
+    Box b;
+    bool ok = node_box(c, rancornA, "rancornA", b) && check_node(c, b, rancornB, "rancornB");
     for (int d = 0; ok && d < 3; ++d)
-        ok = check_face(c, stochA[d], d, nv, "stochA") && check_face(c, stochB[d], d, nv, "stochB");
+        ok = check_face(c, b, stochA[d], d, nv, "stochA") && check_face(c, b, stochB[d], d, nv, "stochB");
     if (!ok) return FHDX_ERR_ARG;
     hipStream_t s = S(stream);
     Arr A[3], B[3];
@@ -593,7 +716,7 @@ int fhdx_fill_noise(fhdx_ctx* c, uint64_t seed, uint64_t step, uint32_t instance
         if (!r && stochB[d].ncomp > c->dp.nvars) r = fill_value(stochB[d].p, B[d].sn * stochB[d].ncomp, 0.0, s);
         if (r) return r;
     }
-    return ref_fill_noise(c->dp, make_noise_key(seed, step, instance), A, B, make_arr(rancornA),
+    return ref_fill_noise(box_params(c, b), make_noise_key(seed, step, instance), A, B, make_arr(rancornA),
                           make_arr(rancornB), s);
 }
 
@@ -621,6 +744,25 @@ int fhdx_rk3_step_ref(fhdx_ctx* c, fhdx_fab cu, fhdx_fab cup, fhdx_fab cup2, fhd
         set_error("nreaction > 0: call fhdx_ctx_set_chemistry before stepping");
         return FHDX_ERR_ARG;
     }
+    StageArrays A{};
+    A.cu = cu;
+    A.cup = cup;
+    A.cup2 = cup2;
+    A.prim = prim;
+    A.eta = eta;
+    A.zeta = zeta;
+    A.kappa = kappa;
+    A.chi = chi;
+    A.D = D;
+    for (int d = 0; d < 3; ++d) {
+        A.flux[d] = flux[d];
+        A.stochFlux[d] = stochFlux[d];
+        A.cornx[d] = cornx[d];
+        A.corny[d] = corny[d];
+        A.cornz[d] = cornz[d];
+    }
+    A.visccorn = visccorn;
+    A.rancorn = rancorn;
     if (injected) {
         if (c->p.nreaction > 0) {
             set_error("caller noise with chemistry: the reaction noise has no array argument");
@@ -631,8 +773,13 @@ int fhdx_rk3_step_ref(fhdx_ctx* c, fhdx_fab cu, fhdx_fab cup, fhdx_fab cup2, fhd
             if (!check_face(c, stochA[d], d, nv, "stochA") || !check_face(c, stochB[d], d, nv, "stochB"))
                 return FHDX_ERR_ARG;
         if (!check_node(c, *rancornA, "rancornA") || !check_node(c, *rancornB, "rancornB")) return FHDX_ERR_ARG;
-        return rk3_ref(c, cu, cup, cup2, prim, eta, zeta, kappa, chi, D, flux, stochFlux, cornx, corny, cornz,
-                       visccorn, rancorn, stochA, stochB, *rancornA, *rancornB, dt, s);
+        for (int d = 0; d < 3; ++d) {
+            A.sA[d] = stochA[d];
+            A.sB[d] = stochB[d];
+        }
+        A.rA = *rancornA;
+        A.rB = *rancornB;
+        return rk3_ref(c, A, dt, s);
     }
     int r = ensure_ref_workspace(c);
     if (r) return r;
@@ -641,8 +788,85 @@ int fhdx_rk3_step_ref(fhdx_ctx* c, fhdx_fab cu, fhdx_fab cup, fhdx_fab cup2, fhd
     if (c->p.nreaction > 0 &&
         (r = ref_fill_chem_noise(c->dp, make_noise_key(seed, step, 0), make_arr(w.ranA), make_arr(w.ranB), s)))
         return r;
-    return rk3_ref(c, cu, cup, cup2, prim, eta, zeta, kappa, chi, D, flux, stochFlux, cornx, corny, cornz,
-                   visccorn, rancorn, w.sA, w.sB, w.rA, w.rB, dt, s);
+    for (int d = 0; d < 3; ++d) {
+        A.sA[d] = w.sA[d];
+        A.sB[d] = w.sB[d];
+    }
+    A.rA = w.rA;
+    A.rB = w.rB;
+    A.source = w.source;
+    A.ranchem = w.ran;
+    A.ranchemA = w.ranA;
+    A.ranchemB = w.ranB;
+    return rk3_ref(c, A, dt, s);
+}
+
+int fhdx_rk3_stage_ref(fhdx_ctx* c, int stage, fhdx_fab cu, fhdx_fab cup, fhdx_fab cup2, fhdx_fab prim,
+                       fhdx_fab eta, fhdx_fab zeta, fhdx_fab kappa, fhdx_fab chi, fhdx_fab D, const fhdx_fab flux[3],
+                       const fhdx_fab stochFlux[3], const fhdx_fab cornx[3], const fhdx_fab corny[3],
+                       const fhdx_fab cornz[3], fhdx_fab visccorn, fhdx_fab rancorn, const fhdx_fab stochA[3],
+                       const fhdx_fab stochB[3], fhdx_fab rancornA, fhdx_fab rancornB, fhdx_fab source,
+                       fhdx_fab ranchem, fhdx_fab ranchemA, fhdx_fab ranchemB, double dt, void* stream)
+{
+    if (!c || !flux || !stochFlux || !cornx || !corny || !cornz || !stochA || !stochB) return FHDX_ERR_ARG;
+    if (stage < 0 || stage > 2 || !(dt > 0.)) {
+        set_error("fhdx_rk3_stage_ref: stage must be 0, 1 or 2 and dt > 0");
+        return FHDX_ERR_ARG;
+    }
+    const int NS = c->p.nspecies, nv = c->dp.nvars, nr = c->p.nreaction;
+    Box b;
+    bool ok = cell_box(c, cu, nv, "cu", b) && check_cell(c, b, cup, nv, "cup") &&
+              check_cell(c, b, cup2, nv, "cup2") && check_cell(c, b, prim, c->dp.nprim, "prim") &&
+              check_cell(c, b, eta, 1, "eta") && check_cell(c, b, zeta, 1, "zeta") &&
+              check_cell(c, b, kappa, 1, "kappa") && check_cell(c, b, chi, NS, "chi") &&
+              check_cell(c, b, D, NS * NS, "D") && check_node(c, b, visccorn, "visccorn") &&
+              check_node(c, b, rancorn, "rancorn") && check_node(c, b, rancornA, "rancornA") &&
+              check_node(c, b, rancornB, "rancornB");
+    for (int d = 0; ok && d < 3; ++d)
+        ok = check_face(c, b, flux[d], d, nv + 4, "flux") && check_face(c, b, stochFlux[d], d, nv, "stochFlux") &&
+             check_face(c, b, stochA[d], d, nv, "stochA") && check_face(c, b, stochB[d], d, nv, "stochB") &&
+             check_node(c, b, cornx[d], "cornx") && check_node(c, b, corny[d], "corny") &&
+             check_node(c, b, cornz[d], "cornz");
+    if (ok && nr > 0) {
+        if (!c->chem_set) {
+            set_error("nreaction > 0: call fhdx_ctx_set_chemistry before stepping");
+            return FHDX_ERR_ARG;
+        }
+        ok = check_cell(c, b, source, nv, "source") && check_cell(c, b, ranchem, nr, "ranchem") &&
+             check_cell(c, b, ranchemA, nr, "ranchemA") && check_cell(c, b, ranchemB, nr, "ranchemB") &&
+             ranchemA.ncomp == ranchem.ncomp && ranchemB.ncomp == ranchem.ncomp;
+    }
+    if (!ok) return FHDX_ERR_ARG;
+    StageArrays A{};
+    A.cu = cu;
+    A.cup = cup;
+    A.cup2 = cup2;
+    A.prim = prim;
+    A.eta = eta;
+    A.zeta = zeta;
+    A.kappa = kappa;
+    A.chi = chi;
+    A.D = D;
+    for (int d = 0; d < 3; ++d) {
+        A.flux[d] = flux[d];
+        A.stochFlux[d] = stochFlux[d];
+        A.cornx[d] = cornx[d];
+        A.corny[d] = corny[d];
+        A.cornz[d] = cornz[d];
+        A.sA[d] = stochA[d];
+        A.sB[d] = stochB[d];
+    }
+    A.visccorn = visccorn;
+    A.rancorn = rancorn;
+    A.rA = rancornA;
+    A.rB = rancornB;
+    if (nr > 0) {
+        A.source = source;
+        A.ranchem = ranchem;
+        A.ranchemA = ranchemA;
+        A.ranchemB = ranchemB;
+    }
+    return rk3_stage(c, box_params(c, b), A, stage, dt, S(stream));
 }
 
 int fhdx_rk3_step(fhdx_ctx* c, fhdx_fab cu, fhdx_fab prim, double dt, uint64_t seed, uint64_t step, void* stream)
@@ -727,10 +951,11 @@ int fhdx_chemistry_source(fhdx_ctx* c, fhdx_fab prim, fhdx_fab source, fhdx_fab 
         set_error("fhdx_chemistry_source: call fhdx_ctx_set_chemistry first");
         return FHDX_ERR_ARG;
     }
-    if (!check_cell(c, prim, c->dp.nprim, "prim") || !check_cell(c, source, c->dp.nvars, "source") ||
-        !check_cell(c, ranchem, c->p.nreaction, "ranchem"))
+    Box b;
+    if (!cell_box(c, prim, c->dp.nprim, "prim", b) || !check_cell(c, b, source, c->dp.nvars, "source") ||
+        !check_cell(c, b, ranchem, c->p.nreaction, "ranchem"))
         return FHDX_ERR_ARG;
-    return ref_chem_source(c->dp, make_arr(prim), make_arr(source), make_arr(ranchem), dt, S(stream));
+    return ref_chem_source(box_params(c, b), make_arr(prim), make_arr(source), make_arr(ranchem), dt, S(stream));
 }
 
 int fhdx_fill_chem_noise(fhdx_ctx* c, uint64_t seed, uint64_t step, uint32_t instance, fhdx_fab ranchemA,
@@ -741,10 +966,12 @@ int fhdx_fill_chem_noise(fhdx_ctx* c, uint64_t seed, uint64_t step, uint32_t ins
         set_error("fhdx_fill_chem_noise: call fhdx_ctx_set_chemistry first");
         return FHDX_ERR_ARG;
     }
-    if (!check_cell(c, ranchemA, c->p.nreaction, "ranchemA") || !check_cell(c, ranchemB, c->p.nreaction, "ranchemB"))
+    Box b;
+    if (!cell_box(c, ranchemA, c->p.nreaction, "ranchemA", b) ||
+        !check_cell(c, b, ranchemB, c->p.nreaction, "ranchemB"))
         return FHDX_ERR_ARG;
-    return ref_fill_chem_noise(c->dp, make_noise_key(seed, step, instance), make_arr(ranchemA), make_arr(ranchemB),
-                               S(stream));
+    return ref_fill_chem_noise(box_params(c, b), make_noise_key(seed, step, instance), make_arr(ranchemA),
+                               make_arr(ranchemB), S(stream));
 }
 
 // ---- evaluateStats ----------------------------------------------------------------------

@@ -45,6 +45,8 @@ inline DevParams make_dev_params(const fhdx_params& p)
     d.ngc = p.ngc;
     for (int i = 0; i < 3; ++i) {
         d.n[i] = p.n_cells[i];
+        d.bo[i] = 0;
+        d.bn[i] = p.n_cells[i];
         d.dx[i] = p.dx[i];
         d.grav[i] = p.grav[i];
     }

@@ -30,7 +30,10 @@ struct DevChem {
 
 struct DevParams {
     int ns, nvars, nprim, ngc;
-    int n[3];
+    int n[3];       // domain cells per direction
+    // The valid box a launch covers: lower corner bo and extent bn (global cell indices).  The
+    // whole domain unless the arrays are one FAB of a MultiFab split into several boxes.
+    int bo[3], bn[3];
     int algorithm_type, advection_type, visc_type, stoch_on;
     double dx[3];
     double molmass[MS], diameter[MS], hcv[MS], hcp[MS], e0[MS];

@@ -74,15 +74,28 @@ __device__ inline void face_index(long t, int fx, int fy, int& i, int& j, int& k
     k = (int)(r / fy);
 }
 
+// Thread t of a launch over the launch box's cells grown by (ex, ey, ez) on the hi side (faces
+// of direction d: 1 along d; nodes: 1 along every axis), as global indices (i, j, k); false
+// past the end.  Wall and periodic tests keep using the domain size P.n.
+__device__ inline bool box_index(const DevParams& P, long t, int ex, int ey, int ez, int& i, int& j, int& k)
+{
+    const int fx = P.bn[0] + ex, fy = P.bn[1] + ey, fz = P.bn[2] + ez;
+    if (t >= (long)fx * fy * fz) return false;
+    face_index(t, fx, fy, i, j, k);
+    i += P.bo[0];
+    j += P.bo[1];
+    k += P.bo[2];
+    return true;
+}
+
 // ---- cell kernels -------------------------------------------------------------------------
 
 template <int NS>
 __global__ void __launch_bounds__(BLK) k_c2p(DevParams P, Arr prim, Arr cons)
 {
     long t = (long)blockIdx.x * BLK + threadIdx.x;
-    if (t >= (long)P.n[0] * P.n[1] * P.n[2]) return;
     int i, j, k;
-    face_index(t, P.n[0], P.n[1], i, j, k);
+    if (!box_index(P, t, 0, 0, 0, i, j, k)) return;
     double c[NS + 5], pr[2 * NS + 6];
 #pragma unroll
     for (int n = 0; n < NS + 5; ++n) c[n] = cons(i, j, k, n);
@@ -96,14 +109,14 @@ __global__ void __launch_bounds__(BLK) k_transport(DevParams P, Arr prim, Arr et
                                                    Arr kappa, Arr chi, Arr Dm, int* err)
 {
     const int g = P.ngc;
-    const int gx = P.n[0] + 2 * g, gy = P.n[1] + 2 * g, gz = P.n[2] + 2 * g;
+    const int gx = P.bn[0] + 2 * g, gy = P.bn[1] + 2 * g, gz = P.bn[2] + 2 * g;
     long t = (long)blockIdx.x * BLK + threadIdx.x;
     if (t >= (long)gx * gy * gz) return;
     int i, j, k;
     face_index(t, gx, gy, i, j, k);
-    i -= g;
-    j -= g;
-    k -= g;
+    i += P.bo[0] - g;
+    j += P.bo[1] - g;
+    k += P.bo[2] - g;
     double Y[NS], chiv[NS], Dv[NS * NS];
 #pragma unroll
     for (int n = 0; n < NS; ++n) Y[n] = prim(i, j, k, 6 + n);
@@ -158,15 +171,17 @@ template <int NS>
 __global__ void __launch_bounds__(BLK) k_set_bc(DevParams P, Arr prim, Arr cons, int D)
 {
     const int g = P.ngc, n = P.n[D];
-    int ext[3] = {P.n[0] + 2 * g, P.n[1] + 2 * g, P.n[2] + 2 * g};
+    int ext[3] = {P.bn[0] + 2 * g, P.bn[1] + 2 * g, P.bn[2] + 2 * g};
     ext[D] = 2 * g;
     long t = (long)blockIdx.x * BLK + threadIdx.x;
     if (t >= (long)ext[0] * ext[1] * ext[2]) return;
     int c[3];
     face_index(t, ext[0], ext[1], c[0], c[1], c[2]);
     for (int d = 0; d < 3; ++d)
-        if (d != D) c[d] -= g;
+        if (d != D) c[d] += P.bo[d] - g;
     const int side = c[D] >= g;  // layers 0..g-1: lo ghosts -g..-1; g..2g-1: hi ghosts n..n+g-1
+    // a box that does not reach this side of the domain has no wall ghosts there
+    if (side ? P.bo[D] + P.bn[D] != n : P.bo[D] != 0) return;
     c[D] = side ? n + (c[D] - g) : c[D] - g;
     int m[3] = {c[0], c[1], c[2]};
     m[D] = side ? 2 * (n - 1) - c[D] + 1 : -c[D] - 1;
@@ -276,10 +291,8 @@ __device__ __forceinline__ void stoch_body(const DevParams& P, Arr prim, Arr eta
                                            Arr Dm, Arr fl, Arr ran, Arr rc, double volinv, double dtinv,
                                            double dt, long t)
 {
-    const int fx = P.n[0] + (D == 0), fy = P.n[1] + (D == 1), fz = P.n[2] + (D == 2);
-    if (t >= (long)fx * fy * fz) return;
     int i, j, k;
-    face_index(t, fx, fy, i, j, k);
+    if (!box_index(P, t, D == 0, D == 1, D == 2, i, j, k)) return;
     const WallFace W = wall_face<D>(P, i, j, k);
     // lower and upper cells of the face averages; both the wall-side cell at a wall
     const int il = W.any() ? W.wi : i - (D == 0), jl = W.any() ? W.wj : j - (D == 1),
@@ -490,10 +503,8 @@ template <int NS, int D>
 __device__ __forceinline__ void diff_body(const DevParams& P, Arr prim, Arr eta, Arr kappa, Arr chi, Arr Dm,
                                           Arr fl, long t)
 {
-    const int fx = P.n[0] + (D == 0), fy = P.n[1] + (D == 1), fz = P.n[2] + (D == 2);
-    if (t >= (long)fx * fy * fz) return;
     int i, j, k;
-    face_index(t, fx, fy, i, j, k);
+    if (!box_index(P, t, D == 0, D == 1, D == 2, i, j, k)) return;
     const int il = i - (D == 0), jl = j - (D == 1), kl = k - (D == 2);
     const WallFace W = wall_face<D>(P, i, j, k);
     const bool wall = W.any();
@@ -580,11 +591,9 @@ __global__ void __launch_bounds__(BLK) k_corner_node(DevParams P, Arr prim, Arr 
                                                      Arr cux, Arr cvx, Arr cwx, Arr cuy, Arr cvy,
                                                      Arr cwy, Arr cuz, Arr cvz, Arr cwz, Arr vcorn)
 {
-    const int nx = P.n[0] + 1, ny = P.n[1] + 1, nz = P.n[2] + 1;
     long t = (long)blockIdx.x * BLK + threadIdx.x;
-    if (t >= (long)nx * ny * nz) return;
     int i, j, k;
-    face_index(t, nx, ny, i, j, k);
+    if (!box_index(P, t, 1, 1, 1, i, j, k)) return;
     double muxp = 0.125 * (eta(i, j - 1, k - 1) + eta(i - 1, j - 1, k - 1) + eta(i, j, k - 1) +
                            eta(i - 1, j, k - 1) + eta(i, j - 1, k) + eta(i - 1, j - 1, k) + eta(i, j, k) +
                            eta(i - 1, j, k));
@@ -684,10 +693,8 @@ template <int NS, int D>
 __device__ __forceinline__ void corner_face_body(const DevParams& P, Arr prim, Arr fl, Arr vcorn, Arr c1,
                                                  Arr c2, Arr ta1, Arr ta2, long t)
 {
-    const int fx = P.n[0] + (D == 0), fy = P.n[1] + (D == 1), fz = P.n[2] + (D == 2);
-    if (t >= (long)fx * fy * fz) return;
     int i, j, k;
-    face_index(t, fx, fy, i, j, k);
+    if (!box_index(P, t, D == 0, D == 1, D == 2, i, j, k)) return;
     const WallFace W = wall_face<D>(P, i, j, k);
     // velocity cells of the heating term: the two neighbours, or the wall-side cell twice
     // (flux.cpp:1791-1822: 0.5*x*u_wall equals 0.25*x*(u_wall + u_wall) exactly)
@@ -718,10 +725,8 @@ __device__ __forceinline__ void corner_face_body(const DevParams& P, Arr prim, A
 template <int NS, int D>
 __device__ __forceinline__ void hyp_body(const DevParams& P, Arr cons, Arr prim, Arr fl, long t)
 {
-    const int fx = P.n[0] + (D == 0), fy = P.n[1] + (D == 1), fz = P.n[2] + (D == 2);
-    if (t >= (long)fx * fy * fz) return;
     int i, j, k;
-    face_index(t, fx, fy, i, j, k);
+    if (!box_index(P, t, D == 0, D == 1, D == 2, i, j, k)) return;
     constexpr int ex = (D == 0), ey = (D == 1), ez = (D == 2);
     const int nv = NS + 5;
     const double wgt2 = 1. / 12.;
@@ -805,9 +810,8 @@ __global__ void __launch_bounds__(BLK) k_stage(DevParams P, Arr out, Arr u0, Arr
                                                Arr fz, Arr srcf, double dt, int mode)
 {
     long t = (long)blockIdx.x * BLK + threadIdx.x;
-    if (t >= (long)P.n[0] * P.n[1] * P.n[2]) return;
     int i, j, k;
-    face_index(t, P.n[0], P.n[1], i, j, k);
+    if (!box_index(P, t, 0, 0, 0, i, j, k)) return;
     double o[NS + 5];
 #pragma unroll
     for (int n = 0; n < NS + 5; ++n) {
@@ -845,9 +849,8 @@ __global__ void __launch_bounds__(BLK) k_chem_source(DevParams P, Arr prim, Arr 
                                                      double dV)
 {
     long t = (long)blockIdx.x * BLK + threadIdx.x;
-    if (t >= (long)P.n[0] * P.n[1] * P.n[2]) return;
     int i, j, k;
-    face_index(t, P.n[0], P.n[1], i, j, k);
+    if (!box_index(P, t, 0, 0, 0, i, j, k)) return;
     double X[NS], src[NS];
 #pragma unroll
     for (int n = 0; n < NS; ++n) X[n] = prim(i, j, k, 6 + NS + n);
@@ -861,9 +864,8 @@ __global__ void __launch_bounds__(BLK) k_chem_source(DevParams P, Arr prim, Arr 
 __global__ void __launch_bounds__(BLK) k_chem_noise(DevParams P, NoiseKey K, Arr A, Arr B)
 {
     long t = (long)blockIdx.x * BLK + threadIdx.x;
-    if (t >= (long)P.n[0] * P.n[1] * P.n[2]) return;
     int i, j, k;
-    face_index(t, P.n[0], P.n[1], i, j, k);
+    if (!box_index(P, t, 0, 0, 0, i, j, k)) return;
     const uint32_t lin = (uint32_t)(((long)k * P.n[1] + j) * P.n[0] + i);
     for (int m = 0; m < P.chem.nreaction; ++m) {
         double za, zb;
@@ -915,10 +917,8 @@ __global__ void __launch_bounds__(BLK) k_lincomb(double* dst, const double* a, c
 template <int D>
 __device__ __forceinline__ void noise_face_body(const DevParams& P, const NoiseKey& K, Arr A, Arr B, long t)
 {
-    const int fx = P.n[0] + (D == 0), fy = P.n[1] + (D == 1), fz = P.n[2] + (D == 2);
-    if (t >= (long)fx * fy * fz) return;
     int i, j, k;
-    face_index(t, fx, fy, i, j, k);
+    if (!box_index(P, t, D == 0, D == 1, D == 2, i, j, k)) return;
     const int iw = (D == 0 && i == P.n[0]) ? 0 : i;
     const int jw = (D == 1 && j == P.n[1]) ? 0 : j;
     const int kw = (D == 2 && k == P.n[2]) ? 0 : k;
@@ -940,10 +940,8 @@ __device__ __forceinline__ void noise_face_body(const DevParams& P, const NoiseK
 __device__ __forceinline__ void noise_node_body(const DevParams& P, const NoiseKey& K, Arr A, Arr B, double scale,
                                                 long t)
 {
-    const int nx = P.n[0] + 1, ny = P.n[1] + 1, nz = P.n[2] + 1;
-    if (t >= (long)nx * ny * nz) return;
     int i, j, k;
-    face_index(t, nx, ny, i, j, k);
+    if (!box_index(P, t, 1, 1, 1, i, j, k)) return;
     const int iw = (i == P.n[0]) ? 0 : i, jw = (j == P.n[1]) ? 0 : j, kw = (k == P.n[2]) ? 0 : k;
     uint32_t lin = (uint32_t)(((long)kw * P.n[1] + jw) * P.n[0] + iw);
     // nodes on the hi plane of a wall direction are distinct: field NODE_FIELD + mask
@@ -1043,16 +1041,17 @@ __global__ void __launch_bounds__(BLK) k_hyp3(DevParams P, Arr cons, Arr prim, A
         default: return FHDX_ERR_UNSUPPORTED;           \
     }
 
-static long ncell(const DevParams& P) { return (long)P.n[0] * P.n[1] * P.n[2]; }
+// counts over the launch box (cells, faces of direction d, nodes, cells grown by ngc)
+static long ncell(const DevParams& P) { return (long)P.bn[0] * P.bn[1] * P.bn[2]; }
 static long nface(const DevParams& P, int d)
 {
-    return (long)(P.n[0] + (d == 0)) * (P.n[1] + (d == 1)) * (P.n[2] + (d == 2));
+    return (long)(P.bn[0] + (d == 0)) * (P.bn[1] + (d == 1)) * (P.bn[2] + (d == 2));
 }
-static long nnode(const DevParams& P) { return (long)(P.n[0] + 1) * (P.n[1] + 1) * (P.n[2] + 1); }
+static long nnode(const DevParams& P) { return (long)(P.bn[0] + 1) * (P.bn[1] + 1) * (P.bn[2] + 1); }
 static long ngrown(const DevParams& P)
 {
     const int g = P.ngc;
-    return (long)(P.n[0] + 2 * g) * (P.n[1] + 2 * g) * (P.n[2] + 2 * g);
+    return (long)(P.bn[0] + 2 * g) * (P.bn[1] + 2 * g) * (P.bn[2] + 2 * g);
 }
 
 int ref_c2p(const DevParams& P, Arr prim, Arr cons, hipStream_t s)
@@ -1093,7 +1092,7 @@ int ref_set_bc(const DevParams& P, Arr prim, Arr cons, hipStream_t s)
         if (P.periodic[d]) continue;
         long slab = 2L * g;
         for (int e = 0; e < 3; ++e)
-            if (e != d) slab *= P.n[e] + 2 * g;
+            if (e != d) slab *= P.bn[e] + 2 * g;
         NS_DISPATCH(P.ns, (k_set_bc<NS><<<grid_for(slab, BLK), BLK, 0, s>>>(P, prim, cons, d)));
     }
     FHDX_HIP_TRY(hipGetLastError());

@@ -75,8 +75,6 @@ struct fhdx_server {
     std::deque<Pending> pending;
     std::vector<Running> running;
     std::vector<int> flags;
-    double* staging = nullptr;  // pinned host buffer for results nobody asked to keep
-    size_t staging_bytes = 0;
     std::atomic<uint64_t> steps_done{0}, cell_updates{0}, bytes_used{0};
 
     void complete(const Pending& q, int status, int64_t t_admit, uint64_t steps, const char* msg)
@@ -138,34 +136,14 @@ struct fhdx_server {
 
     int finish(Running& r, int status, const char* msg)
     {
-        const fhdx_params& p = r.q.r.params;
         const uint64_t step = e->slots[r.slot].desc.step;
-        if (status == FHDX_OK) {
-            const int g = p.ngc;
-            const size_t pts =
-                (size_t)(p.n_cells[0] + 2 * g) * (p.n_cells[1] + 2 * g) * (p.n_cells[2] + 2 * g);
-            double* cu_out = r.q.r.cu_out;
-            if (!cu_out) {
-                const size_t bytes = pts * (5 + p.nspecies) * sizeof(double);
-                if (bytes > staging_bytes) {
-                    if (staging) (void)hipHostFree(staging);
-                    staging = nullptr;
-                    staging_bytes = 0;
-                    if (hipHostMalloc((void**)&staging, bytes, hipHostMallocDefault) != hipSuccess) {
-                        status = FHDX_ERR_NOMEM;
-                        msg = "pinned result buffer";
-                    } else {
-                        staging_bytes = bytes;
-                    }
-                }
-                cu_out = staging;
-            }
-            if (status == FHDX_OK) {
-                int rc = engine_read(e, r.slot, cu_out, r.q.r.prim_out);
-                if (rc) {
-                    status = rc;
-                    msg = fhdx_last_error();
-                }
+        // the result goes where the request asked for it; a request with neither buffer only
+        // wants its completion (the state is dropped with the slot)
+        if (status == FHDX_OK && (r.q.r.cu_out || r.q.r.prim_out)) {
+            int rc = engine_read(e, r.slot, r.q.r.cu_out, r.q.r.prim_out);
+            if (rc) {
+                status = rc;
+                msg = fhdx_last_error();
             }
         }
         const std::string m = msg ? msg : "";
@@ -281,10 +259,15 @@ int fhdx_server_submit(fhdx_server* s, const fhdx_request* r, int64_t* id)
     q.r = *r;
     const fhdx_params& p = r->params;
     if (!r->cu_init) {
-        // uniform start (InitConsVar prob_type 1): rho, zero momentum, rho*sum_k Y_k(e0_k + cv_k T), rho*Y_k
+        // uniform start (InitConsVar prob_type 1): rho, zero momentum, rho*e, rho*Y_k, with
+        // e = e0mix + T*cvmix summed as GetEnergy does (compressible_functions.H:1498-1511)
         if (p.nspecies < 1 || p.nspecies > FHDX_MAX_SPECIES) return FHDX_ERR_ARG;
-        double ie = 0.;
-        for (int k = 0; k < p.nspecies; ++k) ie += r->Y0[k] * (p.e0[k] + p.hcv[k] * r->T0);
+        double cvmix = 0., e0mix = 0.;
+        for (int k = 0; k < p.nspecies; ++k) {
+            cvmix += r->Y0[k] * p.hcv[k];
+            e0mix += r->Y0[k] * p.e0[k];
+        }
+        const double ie = e0mix + r->T0 * cvmix;
         q.uniform[0] = r->rho0;
         q.uniform[1] = q.uniform[2] = q.uniform[3] = 0.;
         q.uniform[4] = r->rho0 * ie;
@@ -343,7 +326,6 @@ int fhdx_server_destroy(fhdx_server* s)
     }
     s->cv.notify_all();
     if (s->worker.joinable()) s->worker.join();
-    if (s->staging) (void)hipHostFree(s->staging);
     int rc = engine_destroy(s->e);
     delete s;
     return rc;

@@ -21,15 +21,28 @@ def grown_shape(p: Params, ncomp: int):
     return (ncomp, n2 + 2 * g, n1 + 2 * g, n0 + 2 * g)
 
 
+def ensemble_members(count: int, seed: int = 2024, lo: int = 16, hi: int = 64):
+    """Box shapes of the batched-ensemble workload: per-axis edge round(exp(U(ln lo, ln hi)))
+    drawn independently per axis from a fixed seed (512 argon boxes, seeds 1000 + i, in the
+    benchmark)."""
+    rng = np.random.default_rng(seed)
+    e = np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), size=(count, 3)))).astype(int)
+    return [tuple(int(v) for v in row) for row in e]
+
+
 def uniform_values(cfg: Config):
     """Per-component values of the uniform start (InitConsVar prob_type 1,
-    compressible_functions.cpp:163-196): rho0, zero momentum, rho0 * sum_k Y_k (e0_k + cv_k T),
-    rho0 * Y_k."""
+    compressible_functions.cpp:163-196): rho0, zero momentum, rho0 * e, rho0 * Y_k, with the
+    internal energy e = e0mix + T cvmix of GetEnergy (compressible_functions.H:1498-1511):
+    cvmix = sum_k Y_k cv_k and e0mix = sum_k Y_k e0_k summed separately, in that order."""
     p = cfg.params
     ns = p.nspecies
-    ie = 0.0
+    cvmix = 0.0
+    e0mix = 0.0
     for k in range(ns):
-        ie += cfg.rhobar[k] * (p.e0[k] + p.hcv[k] * cfg.T_init)
+        cvmix += cfg.rhobar[k] * p.hcv[k]
+        e0mix += cfg.rhobar[k] * p.e0[k]
+    ie = e0mix + cfg.T_init * cvmix
     return [cfg.rho0, 0.0, 0.0, 0.0, cfg.rho0 * ie] + [cfg.rho0 * cfg.rhobar[k] for k in range(ns)]
 
 

@@ -13,6 +13,7 @@ import time
 import numpy as np
 
 from .lib import check, lib
+from .ensemble import grown_shape
 from .params import MAX_SPECIES, Config, Params
 
 
@@ -164,9 +165,12 @@ def request_stream(count: int, seed: int, edge=(16, 128), species=(1, 4), nsteps
     return out
 
 
-def run_stream(server: Server, reqs, rate: float, seed: int = 1, timeout_s: float = 900.0):
+def run_stream(server: Server, reqs, rate: float, seed: int = 1, timeout_s: float = 900.0,
+               results: bool = True):
     """Submit `reqs` with Poisson arrivals at `rate` per second (exponential gaps from `seed`),
-    wait for all of them and return the completions plus the wall time."""
+    wait for all of them and return the completions plus the wall time.  results: each request
+    gets a host buffer for its final conserved state, so a completion time is the time its
+    result reached host memory."""
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate, size=len(reqs))
     t0 = time.perf_counter()
@@ -177,7 +181,8 @@ def run_stream(server: Server, reqs, rate: float, seed: int = 1, timeout_s: floa
         delay = t_next - time.perf_counter()
         if delay > 0:
             time.sleep(delay)
-        ids[server.submit(cfg, s, n)] = (cfg, ns, n)
+        out = np.empty(grown_shape(cfg.params, cfg.params.nvars)) if results else None
+        ids[server.submit(cfg, s, n, cu_out=out)] = (cfg, ns, n)
     open_n = server.wait(timeout_s)
     wall = time.perf_counter() - t0
     done = server.poll(len(reqs) + 16)

@@ -6,10 +6,15 @@
  * `void* stream` (NULL = the null stream).  All arrays are FP64 device pointers laid out like
  * an AMReX FArrayBox / Array4: x fastest, then y, then z, then component.
  *
- * Index conventions (reference: src_compressible/main_driver.cpp:160-297):
- *   cell data  : indices [-ngc, n_d + ngc) per direction (ngc ghost layers);
- *   face data  : direction d nodal, indices [0, n_d] in d and [0, n_t) across, no ghosts;
- *   node data  : [0, n_d] in every direction, no ghosts.
+ * Index conventions (reference: src_compressible/main_driver.cpp:160-297), for a valid box
+ * [lo_d, hi_d] of the domain [0, n_d):
+ *   cell data  : indices [lo_d - ngc, hi_d + ngc] per direction (ngc ghost layers);
+ *   face data  : direction d nodal, indices [lo_d, hi_d + 1] in d and [lo_t, hi_t] across;
+ *   node data  : [lo_d, hi_d + 1] in every direction, no ghosts.
+ * fhdx_cons_to_prim, fhdx_transport, fhdx_set_bc, fhdx_calculate_flux, fhdx_fill_noise,
+ * fhdx_chemistry_source, fhdx_fill_chem_noise and fhdx_rk3_stage_ref take any box of the domain
+ * (one FAB of a MultiFab split into boxes, main_driver.cpp:90-117); the other entry points take
+ * the whole domain as one box (lo = 0, hi = n - 1) and fill its periodic ghosts themselves.
  * Component maps (reference: main_driver.cpp:172-190, 248-260; transCoeffs.cpp:76-80):
  *   cons  : 0 rho, 1-3 momentum, 4 rho*E, 5.. rho*Y_k                         (nvars = 5+NS)
  *   prim  : 0 rho, 1-3 velocity, 4 T, 5 p, 6.. Y_k, 6+NS.. X_k                 (nprimvars = 6+2NS)
@@ -137,8 +142,9 @@ int fhdx_cons_to_prim(fhdx_ctx* ctx, fhdx_fab prim, fhdx_fab cons, void* stream)
 int fhdx_transport(fhdx_ctx* ctx, fhdx_fab prim, fhdx_fab eta, fhdx_fab zeta, fhdx_fab kappa,
                    fhdx_fab chi, fhdx_fab D, void* stream);
 
-/* MultiFab::FillBoundary(periodicity) for one cell FAB (all components): every ghost cell whose
- * periodic image is a valid cell gets that image; ghost cells outside a wall are left alone. */
+/* MultiFab::FillBoundary(periodicity) for a single-box domain (all components): every ghost cell
+ * whose periodic image is a valid cell gets that image; ghost cells outside a wall are left
+ * alone.  A domain split into several FABs exchanges ghosts between them (AMReX's FillBoundary). */
 int fhdx_fill_boundary(fhdx_ctx* ctx, fhdx_fab mf, void* stream);
 
 /* setBC (src_compressible/boundary.cpp:172-1209): the physical ghost cells of prim and cons at
@@ -181,6 +187,34 @@ int fhdx_rk3_step_ref(fhdx_ctx* ctx, fhdx_fab cu, fhdx_fab cup, fhdx_fab cup2, f
                       const fhdx_fab* stochA, const fhdx_fab* stochB,
                       const fhdx_fab* rancornA, const fhdx_fab* rancornB,
                       double dt, uint64_t seed, uint64_t step, void* stream);
+
+/* One stage of RK3step (src_compressible/timeStep.cpp:99-202 stage 0, 211-307 stage 1, 316-415
+ * stage 2) on ONE FAB of MultiFabs that may be split into several boxes (main_driver.cpp:90-117;
+ * e.g. inputs_equil_3d's max_grid_size 32 32 64): the transport coefficients of the stage input
+ * on the grown box, stochFlux/rancorn = A + w_s B (LinComb, timeStep.cpp:118-128), calculateFlux,
+ * with chemistry ranchem = A + w_s B and the source (timeStep.cpp:135-143), the stage combination
+ * (stage 0: cu -> cup, 1: cup -> cup2, 2: cup2 -> cu) and conservedToPrimitive of the output on
+ * the valid cells.  It fills no ghost cell: between stages the caller runs
+ * FillBoundary(periodicity) on the output state and prim across all FABs, then setBC
+ * (fhdx_set_bc per FAB), as timeStep.cpp:204-209, 309-314 and 417-425 do; before stage 0 cu and
+ * prim enter with their ghosts set (main_driver.cpp:535-542).  The A/B sets are filled once per
+ * step per FAB by fhdx_fill_noise (and fhdx_fill_chem_noise): the noise is keyed by the global
+ * face / node index, so a face two FABs share draws one value in both, which is what
+ * OverrideSync leaves in the reference (MultiFabFillRandom.cpp:25-28).  Each FAB is the
+ * reference's: cell arrays on the box grown by ngc, face arrays nodal in their direction, node
+ * arrays nodal in all, no ghosts.  A box must lie in the domain and hold >= ngc cells across a
+ * wall it touches.  source (nvars comps), ranchem and ranchemA/B (nreaction comps, cell FABs
+ * with ngc ghosts) are read only when nreaction > 0 (pass zeroed fabs otherwise); the reference
+ * fills source and ranchem inside RK3step (timeStep.cpp:135-143). */
+int fhdx_rk3_stage_ref(fhdx_ctx* ctx, int stage, fhdx_fab cu, fhdx_fab cup, fhdx_fab cup2,
+                       fhdx_fab prim, fhdx_fab eta, fhdx_fab zeta, fhdx_fab kappa, fhdx_fab chi,
+                       fhdx_fab D, const fhdx_fab flux[3], const fhdx_fab stochFlux[3],
+                       const fhdx_fab cornx[3], const fhdx_fab corny[3], const fhdx_fab cornz[3],
+                       fhdx_fab visccorn, fhdx_fab rancorn,
+                       const fhdx_fab stochA[3], const fhdx_fab stochB[3],
+                       fhdx_fab rancornA, fhdx_fab rancornB,
+                       fhdx_fab source, fhdx_fab ranchem, fhdx_fab ranchemA, fhdx_fab ranchemB,
+                       double dt, void* stream);
 
 /* evaluateStats (src_compressible/stats.cpp:5-363; called at main_driver.cpp:613): running
  * means consMean (nvars) and primMean (nprimvars), variances consVar (nvars) and primVar
@@ -344,7 +378,8 @@ uint64_t fhdx_engine_bytes_used(fhdx_engine* e);
 /* No reference counterpart.  A worker thread owns an engine: submitted requests are admitted
  * in arrival order as the memory budget and a slot allow, every admitted instance advances in
  * the same launches (new arrivals join at the next chunk of at most chunk_steps steps), and a
- * finished instance's state is copied to host memory before its completion is posted.  A
+ * finished instance's state is copied to the request's cu_out / prim_out before its completion
+ * is posted.  A
  * request whose instance raises an error flag is evicted alone.  Results are bit-identical to a
  * solo fhdx_rk3_step run with the same (state, dt, seed), steps 0..nsteps-1. */
 typedef struct fhdx_server fhdx_server;
@@ -356,8 +391,8 @@ typedef struct fhdx_request {
     uint64_t nsteps;          /* >= 1                                                       */
     const double* cu_init;    /* grown conserved start state (host or device), kept alive by
                                  the caller until completion; NULL: uniform start below     */
-    double* cu_out;           /* host/device buffer for the grown final cu, or NULL (the
-                                 result is then copied to a server-owned host buffer)       */
+    double* cu_out;           /* host/device buffer for the grown final cu, or NULL; with
+                                 cu_out and prim_out both NULL the final state is dropped   */
     double* prim_out;         /* grown final prim, or NULL                                  */
     double rho0, T0;          /* uniform start: density, temperature ...                    */
     double Y0[FHDX_MAX_SPECIES]; /* ... and mass fractions (InitConsVar prob_type 1)       */
@@ -369,7 +404,7 @@ typedef struct fhdx_completion {
     int32_t pad;
     int64_t arrival_ns;       /* steady-clock times: submission, admission (0: never) ...   */
     int64_t admit_ns;
-    int64_t done_ns;          /* ... and result on host                                     */
+    int64_t done_ns;          /* ... and result copied to cu_out / prim_out                 */
     uint64_t steps;           /* steps taken                                                */
     char message[96];
 } fhdx_completion;

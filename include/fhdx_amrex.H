@@ -6,15 +6,23 @@
 // Include it from src_compressible instead of linking timeStep.cpp, flux.cpp, transCoeffs.cpp,
 // conservedPrimitiveConversions.cpp, stats.cpp, writePlotFile.cpp and checkpoint.cpp, build
 // AMReX with USE_HIP=TRUE for gfx950, and link
-// fhdex_amd/libfhdx.so.  Requirements: a fully periodic domain and one box per rank
-// (max_grid_size >= n_cells), which is what the library's index checks accept.  Compiled only
-// where AMREX_HOME exists; it is not part of the library build.
+// fhdex_amd/libfhdx.so.  Any BoxArray works (max_grid_size may split the domain, main_driver.cpp:
+// 90-117): RK3step runs each stage per FAB (fhdx_rk3_stage_ref) and exchanges ghosts with
+// MultiFab::FillBoundary between stages, as timeStep.cpp does; periodic directions and walls
+// (setBC per FAB) as fhdx_params_check accepts them.  A device error flag (a mass fraction
+// outside (0,1), where the reference aborts at transCoeffs.cpp:45-46) ends in amrex::Abort after
+// RK3step and calculateTransportCoeffs.  WritePlotFile / WriteCheckPoint / ReadCheckPoint and
+// evaluateStats take a single-box domain.  Compiled only where AMREX_HOME exists; it is not part
+// of the library build.
 #pragma once
 
 #include <AMReX_MultiFab.H>
 #include <AMReX_Geometry.H>
 
 #include <array>
+#include <chrono>
+#include <cstring>
+#include <vector>
 
 #include "chemistry_namespace.H"
 #include "common_functions.H"
@@ -99,29 +107,40 @@ inline fhdx_params params_from_namespaces(amrex::Geometry const& geom)
     return p;
 }
 
+inline void check(int status);
+
+// One context per parameter set (the namespaces can change between calls, e.g. a test driver that
+// runs several decks in one process): contexts are kept for the process, keyed by the finalized
+// parameters.
 inline fhdx_ctx* context(amrex::Geometry const& geom)
 {
-    static fhdx_ctx* ctx = nullptr;
-    if (!ctx) {
-        fhdx_params p = params_from_namespaces(geom);
-        if (fhdx_ctx_create(&p, &ctx) != FHDX_OK) amrex::Abort(fhdx_last_error());
-        if (p.nreaction > 0) {  // reactions of chemistry_namespace.H
-            fhdx_chemistry c{};
-            c.nreaction = chemistry::nreaction;
-            c.reaction_type = chemistry::reaction_type;
-            for (int m = 0; m < c.nreaction; ++m) {
-                for (int n = 0; n < p.nspecies; ++n) {
-                    c.stoich_R[m][n] = chemistry::stoich_coeffs_R(m, n);
-                    c.stoich_P[m][n] = chemistry::stoich_coeffs_P(m, n);
-                }
-                c.rate_const[m] = chemistry::rate_const[m];
-                c.alpha_param[m] = chemistry::alpha_param[m];
-                c.beta_param[m] = chemistry::beta_param[m];
+    struct Entry {
+        fhdx_params p;
+        fhdx_ctx* ctx;
+    };
+    static std::vector<Entry> cache;
+    fhdx_params p = params_from_namespaces(geom);
+    for (const Entry& e : cache)
+        if (std::memcmp(&e.p, &p, sizeof(fhdx_params)) == 0) return e.ctx;
+    fhdx_ctx* ctx = nullptr;
+    check(fhdx_ctx_create(&p, &ctx));
+    if (p.nreaction > 0) {  // reactions of chemistry_namespace.H
+        fhdx_chemistry c{};
+        c.nreaction = chemistry::nreaction;
+        c.reaction_type = chemistry::reaction_type;
+        for (int m = 0; m < c.nreaction; ++m) {
+            for (int n = 0; n < p.nspecies; ++n) {
+                c.stoich_R[m][n] = chemistry::stoich_coeffs_R(m, n);
+                c.stoich_P[m][n] = chemistry::stoich_coeffs_P(m, n);
             }
-            c.T0_chem = chemistry::T0_chem;
-            if (fhdx_ctx_set_chemistry(ctx, &c) != FHDX_OK) amrex::Abort(fhdx_last_error());
+            c.rate_const[m] = chemistry::rate_const[m];
+            c.alpha_param[m] = chemistry::alpha_param[m];
+            c.beta_param[m] = chemistry::beta_param[m];
         }
+        c.T0_chem = chemistry::T0_chem;
+        check(fhdx_ctx_set_chemistry(ctx, &c));
     }
+    cache.push_back({p, ctx});
     return ctx;
 }
 
@@ -133,10 +152,20 @@ inline void check(int status)
 inline void* stream() { return (void*)amrex::Gpu::gpuStream(); }
 
 // Philox key of the next RK3step: the seed and the number of steps taken.  ReadCheckPoint sets
-// both from the checkpoint, so a restarted run continues the noise sequence exactly.
+// both from the checkpoint, so a restarted run continues the noise sequence exactly.  As
+// main_driver.cpp:123-140: seed > 0 is the key, seed = 0 takes the clock (the same value on every
+// rank: the noise is keyed by the global face index, not per rank), seed < 0 aborts.
 inline uint64_t& philox_seed()
 {
-    static uint64_t s = (uint64_t)common::seed;
+    static uint64_t s = [] {
+        if (common::seed > 0) return (uint64_t)common::seed;
+        if (common::seed < 0) amrex::Abort("Must supply non-negative seed");
+        long long now = (long long)std::chrono::duration_cast<std::chrono::nanoseconds>(
+                            std::chrono::system_clock::now().time_since_epoch())
+                            .count();
+        amrex::ParallelDescriptor::Bcast(&now, 1, amrex::ParallelDescriptor::IOProcessorNumber());
+        return (uint64_t)now;
+    }();
     return s;
 }
 inline uint64_t& philox_step()
@@ -189,11 +218,13 @@ inline void setBC(amrex::MultiFab& prim, amrex::MultiFab& cons)
 inline void calculateTransportCoeffs(const amrex::MultiFab& prim, amrex::MultiFab& eta, amrex::MultiFab& zeta,
                                      amrex::MultiFab& kappa, amrex::MultiFab& chi, amrex::MultiFab& D)
 {
+    fhdx_ctx* ctx = fhdx_amrex::context(amrex::DefaultGeometry());
     for (amrex::MFIter mfi(prim); mfi.isValid(); ++mfi)
-        fhdx_amrex::check(fhdx_transport(fhdx_amrex::context(amrex::DefaultGeometry()), fhdx_amrex::fab(prim, mfi),
-                                         fhdx_amrex::fab(eta, mfi), fhdx_amrex::fab(zeta, mfi),
-                                         fhdx_amrex::fab(kappa, mfi), fhdx_amrex::fab(chi, mfi),
-                                         fhdx_amrex::fab(D, mfi), fhdx_amrex::stream()));
+        fhdx_amrex::check(fhdx_transport(ctx, fhdx_amrex::fab(prim, mfi), fhdx_amrex::fab(eta, mfi),
+                                         fhdx_amrex::fab(zeta, mfi), fhdx_amrex::fab(kappa, mfi),
+                                         fhdx_amrex::fab(chi, mfi), fhdx_amrex::fab(D, mfi), fhdx_amrex::stream()));
+    // transCoeffs.cpp:45-46 aborts on a mass fraction outside (0,1)
+    fhdx_amrex::check(fhdx_ctx_check(ctx, fhdx_amrex::stream()));
 }
 
 inline void calculateFlux(const amrex::MultiFab& cons, const amrex::MultiFab& prim, const amrex::MultiFab& eta,
@@ -220,30 +251,71 @@ inline void calculateFlux(const amrex::MultiFab& cons, const amrex::MultiFab& pr
     }
 }
 
-// RK3step with Philox noise keyed by (philox_seed(), philox_step()).
+// RK3step (timeStep.cpp:11-427) with Philox noise keyed by (philox_seed(), philox_step()): the
+// step's noise sets defined like the reference's temporaries (timeStep.cpp:38-72) and filled per
+// FAB, then per stage every FAB (fhdx_rk3_stage_ref: transport, weighted noise, calculateFlux,
+// chemistry source, stage combination, conservedToPrimitive) followed by FillBoundary of the
+// stage output and prim and setBC (timeStep.cpp:204-209, 309-314, 417-425).
 inline void RK3step(amrex::MultiFab& cu, amrex::MultiFab& cup, amrex::MultiFab& cup2, amrex::MultiFab& /*cup3*/,
-                    amrex::MultiFab& prim, amrex::MultiFab& /*source*/, amrex::MultiFab& eta, amrex::MultiFab& zeta,
+                    amrex::MultiFab& prim, amrex::MultiFab& source, amrex::MultiFab& eta, amrex::MultiFab& zeta,
                     amrex::MultiFab& kappa, amrex::MultiFab& chi, amrex::MultiFab& D,
                     std::array<amrex::MultiFab, 3>& flux, std::array<amrex::MultiFab, 3>& stochFlux,
                     std::array<amrex::MultiFab, 3>& cornx, std::array<amrex::MultiFab, 3>& corny,
                     std::array<amrex::MultiFab, 3>& cornz, amrex::MultiFab& visccorn, amrex::MultiFab& rancorn,
-                    amrex::MultiFab& /*ranchem*/, const amrex::Geometry& geom, const amrex::Real dt)
+                    amrex::MultiFab& ranchem, const amrex::Geometry& geom, const amrex::Real dt)
 {
     using namespace fhdx_amrex;
-    for (amrex::MFIter mfi(cu); mfi.isValid(); ++mfi) {
-        fhdx_fab fl[3], sf[3], cx[3], cy[3], cz[3];
-        for (int d = 0; d < 3; ++d) {
-            fl[d] = fab(flux[d], mfi);
-            sf[d] = fab(stochFlux[d], mfi);
-            cx[d] = fab(cornx[d], mfi);
-            cy[d] = fab(corny[d], mfi);
-            cz[d] = fab(cornz[d], mfi);
-        }
-        check(fhdx_rk3_step_ref(context(geom), fab(cu, mfi), fab(cup, mfi), fab(cup2, mfi), fab(prim, mfi),
-                                fab(eta, mfi), fab(zeta, mfi), fab(kappa, mfi), fab(chi, mfi), fab(D, mfi), fl, sf,
-                                cx, cy, cz, fab(visccorn, mfi), fab(rancorn, mfi), nullptr, nullptr, nullptr,
-                                nullptr, dt, philox_seed(), philox_step(), stream()));
+    fhdx_ctx* ctx = context(geom);
+    const int nv = cu.nComp(), nr = chemistry::nreaction, g = cu.nGrow();
+    std::array<amrex::MultiFab, 3> sA, sB;
+    for (int d = 0; d < 3; ++d) {
+        sA[d].define(stochFlux[d].boxArray(), stochFlux[d].DistributionMap(), nv, 0);
+        sB[d].define(stochFlux[d].boxArray(), stochFlux[d].DistributionMap(), nv, 0);
     }
+    amrex::MultiFab rA(rancorn.boxArray(), rancorn.DistributionMap(), 1, 0);
+    amrex::MultiFab rB(rancorn.boxArray(), rancorn.DistributionMap(), 1, 0);
+    amrex::MultiFab chA, chB;  // the reaction noise sets, with the ghost width of ranchem
+    if (nr > 0) {
+        chA.define(ranchem.boxArray(), ranchem.DistributionMap(), nr, g);
+        chB.define(ranchem.boxArray(), ranchem.DistributionMap(), nr, g);
+    }
+    for (amrex::MFIter mfi(cu); mfi.isValid(); ++mfi) {
+        fhdx_fab a[3], b[3];
+        for (int d = 0; d < 3; ++d) {
+            a[d] = fab(sA[d], mfi);
+            b[d] = fab(sB[d], mfi);
+        }
+        check(fhdx_fill_noise(ctx, philox_seed(), philox_step(), 0, a, b, fab(rA, mfi), fab(rB, mfi), stream()));
+        if (nr > 0)
+            check(fhdx_fill_chem_noise(ctx, philox_seed(), philox_step(), 0, fab(chA, mfi), fab(chB, mfi), stream()));
+    }
+    amrex::MultiFab* out[3] = {&cup, &cup2, &cu};
+    for (int st = 0; st < 3; ++st) {
+        for (amrex::MFIter mfi(cu); mfi.isValid(); ++mfi) {
+            fhdx_fab fl[3], sf[3], cx[3], cy[3], cz[3], a[3], b[3];
+            for (int d = 0; d < 3; ++d) {
+                fl[d] = fab(flux[d], mfi);
+                sf[d] = fab(stochFlux[d], mfi);
+                cx[d] = fab(cornx[d], mfi);
+                cy[d] = fab(corny[d], mfi);
+                cz[d] = fab(cornz[d], mfi);
+                a[d] = fab(sA[d], mfi);
+                b[d] = fab(sB[d], mfi);
+            }
+            const fhdx_fab none{};
+            check(fhdx_rk3_stage_ref(ctx, st, fab(cu, mfi), fab(cup, mfi), fab(cup2, mfi), fab(prim, mfi),
+                                     fab(eta, mfi), fab(zeta, mfi), fab(kappa, mfi), fab(chi, mfi), fab(D, mfi), fl,
+                                     sf, cx, cy, cz, fab(visccorn, mfi), fab(rancorn, mfi), a, b, fab(rA, mfi),
+                                     fab(rB, mfi), nr > 0 ? fab(source, mfi) : none,
+                                     nr > 0 ? fab(ranchem, mfi) : none, nr > 0 ? fab(chA, mfi) : none,
+                                     nr > 0 ? fab(chB, mfi) : none, dt, stream()));
+        }
+        out[st]->FillBoundary(geom.periodicity());
+        prim.FillBoundary(geom.periodicity());
+        setBC(prim, *out[st]);
+    }
+    // the reference aborts inside the step on a mass fraction outside (0,1) (transCoeffs.cpp:45-46)
+    check(fhdx_ctx_check(ctx, stream()));
     ++philox_step();
 }
 

@@ -96,31 +96,36 @@ def test_equilibrium_variances():
 
 
 def test_equilibrium_variances_c1_cells_and_dt_bias():
+    """At the 32^3 argon deck's 8e-6 cm cells: rho and j within 1 % at the CFL-0.1 dt; Var rho E
+    carries the scheme's time-step bias (the diffusion number is 100x that at 8e-5 cm), which
+    at least halves when dt is halved (second order in dt), and at dt/2 all five are within 1 %."""
     r1 = measure(argon(8e-6, 32))
     r2 = measure(argon(8e-6, 32), div=2)
-    assert np.all(np.abs(r1 - 1) < 0.015), r1
+    assert np.all(np.abs(r1[:4] - 1) < 0.01), r1
+    assert abs(r1[4] - 1) < 0.015, r1
     assert np.all(np.abs(r2 - 1) < 0.01), r2
     assert abs(r2[4] - 1) <= 0.5 * abs(r1[4] - 1), (r1[4], r2[4])
 
 
 def test_mixture_equilibrium_variances():
-    """Four species (the He/Ne/Ar/Kr mixture on a 32^3 box of 1.6e-5 cm cells, dt 4e-12 s): at equilibrium
-    each partial density fluctuates independently, Var rho_k = rho_k m_k / dV with
+    """Four species (the He/Ne/Ar/Kr mixture on a 32^3 box of 3.2e-5 cm cells, dt 1.6e-11 s): at
+    equilibrium each partial density fluctuates independently, Var rho_k = rho_k m_k / dV with
     m_k = M_k k_B / R; Var rho is their sum; Var j_d = rho k_B T / dV; Var rho E =
-    sum_k e_k^2 Var rho_k + rho c_v k_B T^2 / dV with e_k = e0_k + c_v,k T.  The cells hold
-    ~3500 molecules of the heaviest species; at 8e-6 cm (~440) the nonlinear scheme's variances
-    fall short of these linear-theory values by about 20/N_k, a finite-number effect that shrinks
-    8x with 8x the molecules (the oracle shows the same numbers on the CPU)."""
+    sum_k e_k^2 Var rho_k + rho c_v k_B T^2 / dV with e_k = e0_k + c_v,k T.  All nine within 1 %.
+    The cells hold ~28000 molecules of the heaviest species: with few molecules per cell (~440
+    at the 8e-6 cm cells of the 128^3 benchmark) the reference scheme's variances fall short of
+    these values, a finite-number effect the oracle shows on the CPU
+    (test_oracle_stats.py::test_mixture_variance_shortfall_is_a_finite_number_effect)."""
     import torch
     from fhdex_amd import compressible as fc
     from fhdex_amd import ensemble as E
     from fhdex_amd import lib as L
     cfg = P.four_species(32)
-    cfg.params = P.make_params(4, (32, 32, 32), 1.6e-5, [4.0026, 20.1797, 39.9480, 83.8000],
+    cfg.params = P.make_params(4, (32, 32, 32), 3.2e-5, [4.0026, 20.1797, 39.9480, 83.8000],
                                [2.18e-8, 2.58e-8, 3.63e-8, 4.16e-8])
-    cfg.dt = 4e-12
+    cfg.dt = 1.6e-11
     p = cfg.params
-    say(f"case equilibrium four species 32^3 dx=1.6e-5 dt={cfg.dt:.1e}")
+    say(f"case equilibrium four species 32^3 dx=3.2e-5 dt={cfg.dt:.1e}")
     ctx = L.Context(p)
     cu = fc.MultiFab(torch.from_numpy(E.uniform_cons(cfg)).cuda(), (-p.ngc,) * 3)
     prim = fc.MultiFab.cell(p, p.nprimvars)
@@ -159,4 +164,4 @@ def test_mixture_equilibrium_variances():
     say(f"  mean T {T:.4f} K")
     for name, gv, w, r in zip(names, got, want, ratio):
         say(f"  Var {name}: {gv:.5e} theory {w:.5e} ratio {r:.4f}")
-    assert np.all(np.abs(ratio - 1) < 0.03), ratio
+    assert np.all(np.abs(ratio - 1) < 0.01), ratio

@@ -551,3 +551,32 @@ def test_deterministic_wave_start(orc):
     assert np.allclose(pr[0], wave[None, :, None], rtol=1e-14, atol=0)
     assert np.allclose(pr[0] * pr[4], (pr[0] * pr[4]).mean(), rtol=1e-14, atol=0)   # p = rho R T / M
     assert np.all(pr[1:4] == 0.0) and np.allclose(pr[6:8], 0.5, rtol=1e-15)
+
+
+def test_uniform_start_follows_get_energy(orc):
+    """The host uniform start (ensemble.uniform_cons, also the server's) equals InitConsVar's
+    prob_type 1 bit for bit for a gas with nonzero, unequal e0 (the NO2/N2O4 deck and a
+    four-species variant with e0 of mixed sign): the internal energy is e0mix + T cvmix with the
+    two mixture sums taken separately (GetEnergy, compressible_functions.H:1498-1511)."""
+    from fhdex_amd import ensemble as E
+    from fhdex_amd import params as P
+
+    chem = P.no2_n2o4(6)
+    mix = P.four_species(6)
+    for q, e0 in enumerate([3.1e9, -2.7e9, 1.3e8, -4.9e10]):
+        mix.params.e0[q] = e0
+    for cfg in (chem, mix):
+        p = cfg.params
+        st = orc.HostState(p)
+        orc.init_cons(p, st.cu, cfg.rho0, cfg.rhobar, cfg.T_init)
+        assert np.array_equal(E.uniform_cons(cfg), st.cu), cfg.name
+    # the order matters: summing Y_k (e0_k + cv_k T) per species instead differs in the last
+    # bits for this gas, so the equality above pins the order
+    p = mix.params
+    per_species, cvmix, e0mix = 0.0, 0.0, 0.0
+    for k in range(4):
+        per_species += mix.rhobar[k] * (p.e0[k] + p.hcv[k] * mix.T_init)
+        cvmix += mix.rhobar[k] * p.hcv[k]
+        e0mix += mix.rhobar[k] * p.e0[k]
+    assert per_species != e0mix + mix.T_init * cvmix
+    assert abs(per_species - (e0mix + mix.T_init * cvmix)) <= 1e-12 * abs(per_species)
