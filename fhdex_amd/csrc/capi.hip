@@ -579,6 +579,12 @@ int fhdx_ctx_create(const fhdx_params* p, fhdx_ctx** out)
         delete c;
         return FHDX_ERR_HIP;
     }
+    // this device's copy of the no-allocation scratch, looked up once per device
+    if ((r = noalloc_scratch_init(c->device))) {
+        (void)hipFree(c->d_err);
+        delete c;
+        return r;
+    }
     *out = c;
     return FHDX_OK;
 }
@@ -881,6 +887,32 @@ int fhdx_rk3_step(fhdx_ctx* c, fhdx_fab cu, fhdx_fab prim, double dt, uint64_t s
     if (int r = reject_unfused(c, "fhdx_rk3_step")) return r;
     return fused_rk3_step(c->p, c->dp, c->fused, cu, prim, dt, make_noise_key(seed, step, 0), nullptr, nullptr,
                           nullptr, nullptr, c->d_err, S(stream));
+}
+
+int fhdx_rk3_step_noalloc(fhdx_ctx* c, fhdx_fab cu, fhdx_fab prim, double dt, uint64_t seed, uint64_t step,
+                          void* stream)
+{
+    if (!c) return FHDX_ERR_ARG;
+    if (!check_cell(c, cu, c->dp.nvars, "cu") || !check_cell(c, prim, c->dp.nprim, "prim") || !(dt > 0.))
+        return FHDX_ERR_ARG;
+    if (c->p.nreaction > 0 && !c->chem_set) {
+        set_error("nreaction > 0: call fhdx_ctx_set_chemistry before stepping");
+        return FHDX_ERR_ARG;
+    }
+    if (int r = reject_unfused(c, "fhdx_rk3_step_noalloc")) return r;
+    int dev = -1;
+    FHDX_HIP_TRY(hipGetDevice(&dev));
+    if (dev != c->device) {
+        set_error("fhdx_rk3_step_noalloc: the current device is not the context's (its scratch lives on that device)");
+        return FHDX_ERR_ARG;
+    }
+    double* base = noalloc_scratch(dev);
+    if (!base) {
+        set_error("fhdx_rk3_step_noalloc: the scratch of this device was not looked up");
+        return FHDX_ERR_HIP;
+    }
+    return fused_rk3_step_in(c->p, c->dp, c->fused, base, kNoallocScratchBytes, cu, prim, dt,
+                             make_noise_key(seed, step, 0), c->d_err, S(stream));
 }
 
 int fhdx_rk3_step_injected(fhdx_ctx* c, fhdx_fab cu, fhdx_fab prim, const fhdx_fab stochA[3],

@@ -78,10 +78,28 @@ struct FusedWorkspace {
 
 void fused_free(FusedWorkspace& w);
 
+// The file-scope device scratch of fhdx_rk3_step_noalloc (noalloc.hip): its size, the lookup of
+// a device's copy (once per device, with that device current; fhdx_ctx_create runs it) and the
+// pointer found (nullptr before the lookup).
+constexpr size_t kNoallocScratchBytes = size_t(64) << 20;
+int noalloc_scratch_init(int device);
+double* noalloc_scratch(int device);
+
+// Device bytes of the fast path's workspace for this box with the noise evaluated in place:
+// two stage states, the cell records, the node terms and three face-flux arrays.
+size_t fused_workspace_bytes(const DevParams& P);
+
 // One RK3 step: cu advanced in place, prim recomputed, ghosts of both filled.  Noise from
 // Philox (key K) unless sA is non-null (caller's A/B sets).
 int fused_rk3_step(const fhdx_params& p, const DevParams& P, FusedWorkspace& w, fhdx_fab cu, fhdx_fab prim,
                    double dt, const NoiseKey& K, const fhdx_fab* sA, const fhdx_fab* sB, const fhdx_fab* rA,
                    const fhdx_fab* rB, int* d_err, hipStream_t s);
+
+// The same step (Philox noise in place) with its workspace laid out in `base` (`bytes` long)
+// instead of allocated, and the launch shape of `shape`.  Allocates nothing; every workspace
+// value a kernel reads is written earlier in the same step, so `base` needs no clearing.
+int fused_rk3_step_in(const fhdx_params& p, const DevParams& P, const FusedWorkspace& shape, double* base,
+                      size_t bytes, fhdx_fab cu, fhdx_fab prim, double dt, const NoiseKey& K, int* d_err,
+                      hipStream_t s);
 
 }  // namespace fhdx

@@ -415,6 +415,47 @@ int fused_rk3_step(const fhdx_params& p, const DevParams& P, FusedWorkspace& w, 
     return FHDX_OK;
 }
 
+size_t fused_workspace_bytes(const DevParams& P)
+{
+    const Grid G = grid_of(P);
+    const RecLayout L{P.ns};
+    const size_t comps = 2 * (size_t)P.nvars + L.ncomp() + NODE_NCOMP + 3 * (size_t)P.nvars;
+    return comps * (size_t)G.sn * sizeof(double);
+}
+
+int fused_rk3_step_in(const fhdx_params& p, const DevParams& P, const FusedWorkspace& shape, double* base,
+                      size_t bytes, fhdx_fab cu, fhdx_fab prim, double dt, const NoiseKey& K, int* d_err,
+                      hipStream_t s)
+{
+    const size_t need = fused_workspace_bytes(P);
+    if (need > bytes) {
+        set_error("the box needs " + std::to_string(need >> 20) + " MiB of workspace; the no-allocation scratch holds " +
+                  std::to_string(bytes >> 20) + " MiB");
+        return FHDX_ERR_UNSUPPORTED;
+    }
+    FusedWorkspace w;
+    w.tw_max = shape.tw_max;
+    w.rh = shape.rh;
+    w.kz = shape.kz;
+    w.face_bpc = shape.face_bpc;
+    w.rec_in_update = shape.rec_in_update;
+    w.noise_inline = true;
+    const Grid G = grid_of(P);
+    const RecLayout L{P.ns};
+    double* q = base;
+    auto take = [&](double*& a, int nc) {
+        a = q;
+        q += (size_t)G.sn * nc;
+    };
+    take(w.cup, P.nvars);
+    take(w.cup2, P.nvars);
+    take(w.rec, L.ncomp());
+    take(w.node, NODE_NCOMP);
+    for (int d = 0; d < 3; ++d) take(w.F[d], P.nvars);
+    w.ready = true;
+    return fused_rk3_step(p, P, w, cu, prim, dt, K, nullptr, nullptr, nullptr, nullptr, d_err, s);
+}
+
 }  // namespace fhdx
 
 // ---- timing ------------------------------------------------------------------------------

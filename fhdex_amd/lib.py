@@ -29,7 +29,7 @@ EXPORTS = [
     "fhdx_server_destroy", "fhdx_ctx_autotune", "fhdx_ctx_set_tune", "fhdx_ctx_get_tune",
     "fhdx_evaluate_stats", "fhdx_write_plotfile", "fhdx_write_checkpoint", "fhdx_read_checkpoint",
     "fhdx_ctx_set_chemistry", "fhdx_chemistry_source", "fhdx_fill_chem_noise",
-    "fhdx_rk3_stage_ref",
+    "fhdx_rk3_stage_ref", "fhdx_rk3_step_noalloc",
 ]
 
 _lib = None
@@ -103,6 +103,7 @@ def lib():
                                          FA, FA, FA, FA, Fab, Fab, FA, FA, Fab, Fab, Fab, Fab, Fab,
                                          Fab, C.c_double, vp]
         L.fhdx_rk3_step.argtypes = [vp, Fab, Fab, C.c_double, C.c_uint64, C.c_uint64, vp]
+        L.fhdx_rk3_step_noalloc.argtypes = [vp, Fab, Fab, C.c_double, C.c_uint64, C.c_uint64, vp]
         L.fhdx_rk3_step_injected.argtypes = [vp, Fab, Fab, FA, FA, Fab, Fab, C.c_double, vp]
         L.fhdx_ctx_timing_enable.argtypes = [vp, C.c_int]
         L.fhdx_ctx_timing_read.argtypes = [vp, C.c_int, C.c_char_p, C.c_int,

@@ -247,6 +247,20 @@ int fhdx_rk3_step_injected(fhdx_ctx* ctx, fhdx_fab cu, fhdx_fab prim,
                            const fhdx_fab stochA[3], const fhdx_fab stochB[3],
                            fhdx_fab rancornA, fhdx_fab rancornB, double dt, void* stream);
 
+/* No counterpart in the reference: fhdx_rk3_step for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * arithmetic and bits as fhdx_rk3_step; its workspace (two stage states, cell records, node
+ * terms, face fluxes: (2 nvars + (8 + 4 NS + NS^2) + 14 + 3 nvars) x (n_0+2 ngc)(n_1+2 ngc)
+ * (n_2+2 ngc) doubles) lives in a file-scope device array of 64 MiB per GPU, held until the
+ * process ends and looked up per device by fhdx_ctx_create, so the call allocates nothing.  A
+ * box whose workspace exceeds 64 MiB (45^3 cells hold with two species, 39^3 with four) returns
+ * FHDX_ERR_UNSUPPORTED.  The context's device must be current.  Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (two streams, two threads) race and give
+ * wrong results; serialise them.  Nothing needs clearing between calls (each step writes every
+ * workspace value before reading it). */
+int fhdx_rk3_step_noalloc(fhdx_ctx* ctx, fhdx_fab cu, fhdx_fab prim, double dt, uint64_t seed,
+                          uint64_t step, void* stream);
+
 /* ---- chemistry: the +dt*S source of the RK3 stages ------------------------------------ */
 /* Reaction data of src_chemistry/chemistry_namespace.H:1-39 as compute_compressible_chemistry_
  * source_CLE (chemistry_functions.cpp:128-193) uses it: law-of-mass-action rates
