@@ -275,12 +275,12 @@ def ensemble_bench(count: int, steps: int, loop_steps: int):
 
 
 def serve_bench(rate: float, count: int, max_instances: int = 256, budget_gb: float = 160.0,
-                chunk: int = 8, seed: int = 7, cap_cells: int = 2 << 20):
+                chunk: int = 8, seed: int = 7, cap_cells: int = 2 << 20, variants=("batched", "smallest", "plain")):
     """The served stream: `count` requests with Poisson arrivals at `rate`/s (edge log-uniform
     in [16,128], 1-4 species, nsteps in [200,2000]), served by the continuous-batching server
     (admission in arrival order, and smallest work first with at most `cap_cells` cells in
     flight) and, for comparison, by the same server with one slot (requests strictly one after
-    another).  Latency is arrival to result in host memory."""
+    another).  Latency is arrival to result in host memory; service is admission to result."""
     import numpy as np
     from fhdex_amd import serving as S
 
@@ -295,26 +295,47 @@ def serve_bench(rate: float, count: int, max_instances: int = 256, budget_gb: fl
         ok = [c for c in done if c["status"] == 0]
         lat = np.array([(c["done_ns"] - c["arrival_ns"]) * 1e-9 for c in ok])
         queue = np.array([(c["admit_ns"] - c["arrival_ns"]) * 1e-9 for c in ok])
+        serv = np.array([(c["done_ns"] - c["admit_ns"]) * 1e-9 for c in ok])
         t_first = min(c["arrival_ns"] for c in done)
         t_last = max(c["done_ns"] for c in done)
         span = (t_last - t_first) * 1e-9
+        pct = lambda a, q: float(np.percentile(a, q)) if len(a) else None  # noqa: E731
         return {"completed": len(ok), "failed": len(done) - len(ok), "open": open_n,
-                "p50_latency_s": float(np.percentile(lat, 50)) if len(lat) else None,
-                "p99_latency_s": float(np.percentile(lat, 99)) if len(lat) else None,
+                "p50_latency_s": pct(lat, 50), "p99_latency_s": pct(lat, 99),
+                "p50_service_s": pct(serv, 50), "p99_service_s": pct(serv, 99),
                 "mean_queue_s": float(queue.mean()) if len(queue) else None,
+                "max_queue_s": float(queue.max()) if len(queue) else None,
                 "value": work / span, "unit": "cell-updates/s", "span_s": span}
 
-    batched = one(max_instances)
-    smallest = one(max_instances, S.Server.SMALLEST_FIRST, cap_cells)
-    plain = one(1)
-    return {"workload": f"{count} requests, Poisson {rate}/s, edge log-uniform [16,128], "
-                        f"species 1-4, nsteps 200-2000 (stream seed {seed})",
-            "rate_per_s": rate, "requests": count, "cell_updates": work,
-            "offered_load": work * rate / count,
-            "batched": batched, "batched_smallest_first": dict(smallest, max_active_cells=cap_cells),
-            "one_at_a_time": plain,
-            "speedup_vs_one_at_a_time": batched["value"] / plain["value"],
-            "slots": max_instances, "budget_GiB": budget_gb, "chunk_steps": chunk}
+    out = {"workload": f"{count} requests, Poisson {rate:.3g}/s, edge log-uniform [16,128], "
+                       f"species 1-4, nsteps 200-2000 (stream seed {seed})",
+           "rate_per_s": rate, "requests": count, "cell_updates": work,
+           "offered_load": work * rate / count, "slots": max_instances, "budget_GiB": budget_gb,
+           "chunk_steps": chunk}
+    if "batched" in variants:
+        out["batched"] = one(max_instances)
+    if "smallest" in variants:
+        out["batched_smallest_first"] = dict(one(max_instances, S.Server.SMALLEST_FIRST, cap_cells),
+                                             max_active_cells=cap_cells)
+    if "plain" in variants:
+        out["one_at_a_time"] = one(1)
+        if "batched" in out:
+            out["speedup_vs_one_at_a_time"] = out["batched"]["value"] / out["one_at_a_time"]["value"]
+    return out
+
+
+def serve_unsaturated(capacity: float, count: int = 64, load: float = 0.6, seed: int = 11):
+    """`count` requests of the same mix arriving at the rate whose offered load is `load` of a
+    measured capacity (cell-updates/s): below saturation, so a latency is the request's own
+    service time, not backlog.  Batched (arrival order) and one at a time."""
+    from fhdex_amd import serving as S
+    reqs = S.request_stream(count, seed)
+    mean_work = sum(c.cells * n for c, _, n, _ in reqs) / count
+    rate = load * capacity / mean_work
+    r = serve_bench(rate, count, seed=seed, variants=("batched", "plain"))
+    r["offered_load_fraction"] = load
+    r["capacity_cell_updates_per_s"] = capacity
+    return r
 
 
 def main():
@@ -339,6 +360,8 @@ def main():
     ap.add_argument("--serve-cap", type=int, default=2 << 20,
                     help="cells in flight of the smallest-first served run")
     ap.add_argument("--serve-only", action="store_true")
+    ap.add_argument("--serve-unsaturated", type=int, default=64,
+                    help="requests of a stream at 0.6 of the measured serving capacity (0: skip)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--no-configs", action="store_true", help="skip the 32^3 argon and 128^3 four-species rates")
     ap.add_argument("--shape", type=str, default="", help="launch shape tile_width,region_tiles,z_levels,noise_inline"
@@ -376,8 +399,13 @@ def main():
         return
 
     if a.serve_only:
+        caps = []
         for r in [float(v) for v in a.serve_rates.split(",") if v]:
-            print(json.dumps({"serving": serve_bench(r, a.serve_requests, cap_cells=a.serve_cap)}), flush=True)
+            e = serve_bench(r, a.serve_requests, cap_cells=a.serve_cap)
+            caps.append(e["batched"]["value"])
+            print(json.dumps({"serving": e}), flush=True)
+        if a.serve_unsaturated and caps:
+            print(json.dumps({"serving": serve_unsaturated(max(caps), a.serve_unsaturated)}), flush=True)
         return
 
     if a.ensemble_only:
@@ -485,6 +513,9 @@ def main():
     rates = [float(v) for v in a.serve_rates.split(",") if v]
     if rates:
         line["serving"] = [serve_bench(r, a.serve_requests, cap_cells=a.serve_cap) for r in rates]
+        if a.serve_unsaturated:
+            cap = max(e["batched"]["value"] for e in line["serving"])
+            line["serving"].append(serve_unsaturated(cap, a.serve_unsaturated))
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(a.cpu_n, a.cpu_steps)
     print(json.dumps(line), flush=True)
