@@ -75,6 +75,7 @@ def lib():
         L.orc_fill_chem_noise.argtypes = [P, C.c_uint64, C.c_uint64, C.c_uint32, Fab, Fab]
         L.orc_advance_chem.argtypes = [P, CH, Fab, Fab, C.c_int, C.c_double, C.c_uint64,
                                        C.c_uint64, C.c_uint32]
+        _bind_stag(L)
         _lib = L
     return _lib
 
@@ -271,3 +272,127 @@ def evaluate_stats(p, cons, prim, ss: StatsState, steps, cross_cell):
                                   ss.miscVals.ctypes.data_as(C.POINTER(C.c_double)), int(steps),
                                   int(cross_cell))
     assert rc == 0, rc
+
+
+# ---- staggered scheme (oracle_stag.c) ------------------------------------------------------
+
+EDGE_AXES = ((0, 1), (0, 2), (1, 2))       # edge types xy, xz, yz
+EDGE_FLUX_TYPES = (0, 1, 0, 2, 1, 2)        # edgeflux_x[0..1], _y[0..1], _z[0..1]
+
+
+class StagHost:
+    """Host arrays of RK3stepStag's argument list (timeStepStag.cpp:13-26) and its noise sets:
+    cu, prim, transport coefficients (cells, ngc ghosts), cumom/vel (faces nodal in d, ngc
+    ghosts), faceflux (faces, no ghosts), the six edge fluxes (edges, no ghosts), cenflux
+    (cells, one ghost); noise: face[d] (nvars comps), edge[e] (xy, xz, yz), cen[d] (cells,
+    one ghost).  Arrays are (ncomp, z, y, x), C order."""
+
+    def __init__(self, p: Params):
+        self.p = p
+        g, NS, nv = p.ngc, p.nspecies, p.nvars
+        self.g = g
+        self.cu, self.prim = self.cell(nv), self.cell(p.nprimvars)
+        self.eta, self.zeta, self.kappa = self.cell(1), self.cell(1), self.cell(1)
+        self.chi, self.D = self.cell(NS), self.cell(NS * NS)
+        self.mom = [self.face(d, 1) for d in range(3)]
+        self.vel = [self.face(d, 1) for d in range(3)]
+        self.ff = [self.face(d, nv + 4, 0) for d in range(3)]
+        self.ef = [self.edge(e) for e in EDGE_FLUX_TYPES]
+        self.cf = [self.cell(1, 1) for _ in range(3)]
+        self.fA = [self.face(d, nv, 0) for d in range(3)]
+        self.fB = [self.face(d, nv, 0) for d in range(3)]
+        self.eA = [self.edge(e) for e in range(3)]
+        self.eB = [self.edge(e) for e in range(3)]
+        self.cA = [self.cell(1, 1) for _ in range(3)]
+        self.cB = [self.cell(1, 1) for _ in range(3)]
+
+    def cell(self, nc, g=None):
+        g = self.g if g is None else g
+        n0, n1, n2 = self.p.n_cells
+        return np.zeros((nc, n2 + 2 * g, n1 + 2 * g, n0 + 2 * g))
+
+    def face(self, d, nc, g=None):
+        g = self.g if g is None else g
+        n0, n1, n2 = self.p.n_cells
+        return np.zeros((nc, n2 + 2 * g + (d == 2), n1 + 2 * g + (d == 1), n0 + 2 * g + (d == 0)))
+
+    def edge(self, e, nc=1):
+        a, b = EDGE_AXES[e]
+        n0, n1, n2 = self.p.n_cells
+        return np.zeros((nc, n2 + (2 in (a, b)), n1 + (1 in (a, b)), n0 + (0 in (a, b))))
+
+    @staticmethod
+    def ghosts_of(arr, p, nodal=-1):
+        """Ghost width of a cell (nodal -1) or face array from its shape."""
+        n0 = p.n_cells[0]
+        return (arr.shape[3] - n0 - (nodal == 0)) // 2
+
+    def F(self, arr, nodal=-1):
+        g = self.ghosts_of(arr, self.p, nodal)
+        return fab_of(arr, (-g, -g, -g))
+
+    def faces(self, arrs):
+        return (Fab * 3)(*[self.F(a, d) for d, a in enumerate(arrs)])
+
+    def cells(self, arrs):
+        return (Fab * 3)(*[self.F(a) for a in arrs])
+
+    def edges(self, arrs):
+        return (Fab * len(arrs))(*[fab_of(a, (0, 0, 0)) for a in arrs])
+
+    def fill_noise(self, seed, step, instance=0):
+        rc = lib().orc_fill_noise_stag(C.byref(self.p), seed, step, instance, self.faces(self.fA),
+                                       self.faces(self.fB), self.edges(self.eA), self.edges(self.eB),
+                                       self.cells(self.cA), self.cells(self.cB))
+        assert rc == 0, rc
+
+    def c2p(self):
+        rc = lib().orc_c2p_stag(C.byref(self.p), self.F(self.prim), self.faces(self.vel), self.F(self.cu),
+                                self.faces(self.mom))
+        assert rc == 0, rc
+
+    def transport(self):
+        return lib().orc_transport(C.byref(self.p), self.F(self.prim), self.F(self.eta), self.F(self.zeta),
+                                   self.F(self.kappa), self.F(self.chi), self.F(self.D))
+
+    def flux(self, dt, w):
+        """calculateFluxStag of the current state with noise A + w B of the held sets."""
+        nv = self.p.nvars
+        sf = [self.fA[d] + 0.0 for d in range(3)]
+        for d in range(3):
+            sf[d][4:nv] = 1.0 * self.fA[d][4:nv] + w * self.fB[d][4:nv]
+        se = [1.0 * self.eA[e] + w * self.eB[e] for e in range(3)]
+        sc = [1.0 * self.cA[d] + w * self.cB[d] for d in range(3)]
+        rc = lib().orc_flux_stag(C.byref(self.p), self.F(self.cu), self.faces(self.mom), self.F(self.prim),
+                                 self.faces(self.vel), self.F(self.eta), self.F(self.zeta), self.F(self.kappa),
+                                 self.F(self.chi), self.F(self.D), self.faces(self.ff), self.edges(self.ef),
+                                 self.cells(self.cf), self.faces(sf), self.edges(se), self.cells(sc), dt)
+        assert rc == 0, rc
+
+    def step(self, dt):
+        """One RK3stepStag with the held noise sets."""
+        return lib().orc_rk3_step_stag(
+            C.byref(self.p), self.F(self.cu), self.faces(self.mom), self.F(self.prim), self.faces(self.vel),
+            self.F(self.eta), self.F(self.zeta), self.F(self.kappa), self.F(self.chi), self.F(self.D),
+            self.faces(self.ff), self.edges(self.ef), self.cells(self.cf), self.faces(self.fA),
+            self.faces(self.fB), self.edges(self.eA), self.edges(self.eB), self.cells(self.cA),
+            self.cells(self.cB), dt)
+
+    def advance(self, nsteps, dt, seed=1, step0=0, instance=0):
+        """nsteps of RK3stepStag with Philox noise (seed, step0 + s, instance)."""
+        return lib().orc_advance_stag(C.byref(self.p), self.F(self.cu), self.faces(self.mom), self.F(self.prim),
+                                      self.faces(self.vel), nsteps, dt, seed, step0, instance)
+
+
+def _bind_stag(L):
+    P = C.POINTER(Params)
+    FA = Fab * 3
+    E6 = Fab * 6
+    L.orc_c2p_stag.argtypes = [P, Fab, FA, Fab, FA]
+    L.orc_flux_stag.argtypes = [P, Fab, FA, Fab, FA, Fab, Fab, Fab, Fab, Fab, FA, E6, FA, FA, FA, FA,
+                                C.c_double]
+    L.orc_fill_noise_stag.argtypes = [P, C.c_uint64, C.c_uint64, C.c_uint32, FA, FA, FA, FA, FA, FA]
+    L.orc_rk3_step_stag.argtypes = [P, Fab, FA, Fab, FA, Fab, Fab, Fab, Fab, Fab, FA, E6, FA, FA, FA, FA,
+                                    FA, FA, FA, C.c_double]
+    L.orc_advance_stag.argtypes = [P, Fab, FA, Fab, FA, C.c_int, C.c_double, C.c_uint64, C.c_uint64,
+                                   C.c_uint32]
