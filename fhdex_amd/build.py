@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfhdx.so")
 SOURCES = ["capi.hip", "ref_kernels.hip", "fused_kernels.hip", "engine.hip", "server.hip", "stats.hip",
-           "io.hip", "noalloc.hip"]
+           "io.hip", "noalloc.hip", "stag_kernels.hip", "capi_stag.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FHDX_ARCH", "gfx950")

@@ -13,8 +13,7 @@
 #include <tuple>
 #include <vector>
 
-#include "fused.hpp"
-#include "kernels.hpp"
+#include "ctx.hpp"
 
 namespace fhdx {
 
@@ -25,28 +24,6 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 }  // namespace fhdx
 
 using namespace fhdx;
-
-// Device buffers a context owns for the reference-shaped RK3 step: the step's two noise sets
-// when the caller supplies none, and the chemistry arrays.  Every other array of that step is
-// the caller's (the reference's RK3step argument list).
-struct RefWorkspace {
-    bool ready = false;
-    std::vector<double*> blocks;
-    fhdx_fab sA[3], sB[3], rA, rB;
-    fhdx_fab source, ranA, ranB, ran;  // chemistry (nreaction > 0): source, reaction noise A, B, A + w B
-};
-
-struct fhdx_ctx {
-    fhdx_params p;
-    DevParams dp;
-    int device = 0;
-    int* d_err = nullptr;  // device-side error flags (bit 0: mass fraction outside (0,1))
-    RefWorkspace ref;
-    FusedWorkspace fused;
-    double *st_yz = nullptr, *st_part = nullptr, *st_misc = nullptr;  // evaluateStats scratch
-    int st_nparts = 0;
-    bool chem_set = false;  // fhdx_ctx_set_chemistry called (required when p.nreaction > 0)
-};
 
 extern "C" {
 
@@ -595,6 +572,7 @@ int fhdx_ctx_destroy(fhdx_ctx* c)
     (void)hipDeviceSynchronize();
     free_ref_workspace(c->ref);
     fused_free(c->fused);
+    stag_workspace_free(c->stag);
     if (c->st_yz) (void)hipFree(c->st_yz);
     if (c->st_part) (void)hipFree(c->st_part);
     if (c->st_misc) (void)hipFree(c->st_misc);

@@ -29,7 +29,8 @@ EXPORTS = [
     "fhdx_server_destroy", "fhdx_ctx_autotune", "fhdx_ctx_set_tune", "fhdx_ctx_get_tune",
     "fhdx_evaluate_stats", "fhdx_write_plotfile", "fhdx_write_checkpoint", "fhdx_read_checkpoint",
     "fhdx_ctx_set_chemistry", "fhdx_chemistry_source", "fhdx_fill_chem_noise",
-    "fhdx_rk3_stage_ref", "fhdx_rk3_step_noalloc",
+    "fhdx_rk3_stage_ref", "fhdx_rk3_step_noalloc", "fhdx_cons_to_prim_stag",
+    "fhdx_calculate_flux_stag", "fhdx_rk3_step_stag",
 ]
 
 _lib = None
@@ -105,6 +106,12 @@ def lib():
         L.fhdx_rk3_step.argtypes = [vp, Fab, Fab, C.c_double, C.c_uint64, C.c_uint64, vp]
         L.fhdx_rk3_step_noalloc.argtypes = [vp, Fab, Fab, C.c_double, C.c_uint64, C.c_uint64, vp]
         L.fhdx_rk3_step_injected.argtypes = [vp, Fab, Fab, FA, FA, Fab, Fab, C.c_double, vp]
+        E6 = Fab * 6
+        L.fhdx_cons_to_prim_stag.argtypes = [vp, Fab, FA, Fab, FA, vp]
+        L.fhdx_calculate_flux_stag.argtypes = [vp, Fab, FA, Fab, FA, Fab, Fab, Fab, Fab, Fab, FA, E6, FA,
+                                               C.c_double, C.c_double, C.c_uint64, C.c_uint64, vp]
+        L.fhdx_rk3_step_stag.argtypes = [vp, Fab, FA, Fab, FA, Fab, Fab, Fab, Fab, Fab, FA, E6, FA,
+                                         C.c_double, C.c_uint64, C.c_uint64, vp]
         L.fhdx_ctx_timing_enable.argtypes = [vp, C.c_int]
         L.fhdx_ctx_timing_read.argtypes = [vp, C.c_int, C.c_char_p, C.c_int,
                                            C.POINTER(C.c_int64), C.POINTER(C.c_double)]

@@ -486,6 +486,47 @@ int fhdx_read_checkpoint(const fhdx_params* p, const char* chk_base_name, int re
                          uint64_t* seed, int* has_seed, const fhdx_fields* f, int flags,
                          void* stream);
 
+/* ---- staggered scheme (src_compressible_stag) -------------------------------------------
+ * Momentum and velocity on faces, the rest in cells.  Arrays, as main_driver.cpp:616-631 and
+ * 1023-1050 of src_compressible_stag define them: cu (nvars) and prim (nprimvars) cell FABs
+ * with ngc ghosts; cumom[d] and vel[d] face FABs nodal in d with ngc ghosts; eta, zeta, kappa,
+ * chi (nspecies), D (nspecies^2) cell FABs with ngc ghosts; faceflux[d] faces nodal in d, no
+ * ghosts, nvars+4 comps; edgeflux[6] = edgeflux_x[0] (xy), edgeflux_x[1] (xz), edgeflux_y[0]
+ * (xy), edgeflux_y[1] (yz), edgeflux_z[0] (xz), edgeflux_z[1] (yz), edges nodal along both of
+ * their axes, no ghosts; cenflux[d] cells with one ghost layer.  Scope: the 3-D branch on a box
+ * periodic in every direction, without chemistry (other settings: FHDX_ERR_UNSUPPORTED).  The
+ * noise replaces the MultiFabFillRandomNormal calls of timeStepStag.cpp:171-223 with a
+ * counter-based Philox4x32-10 + Box-Muller keyed by (seed, step, instance 0) and the wrapped
+ * global index of each face, edge and cell, so duplicate periodic faces and edges share a
+ * value (OverrideSync's result); the stage noise is A + w_s B of the step's two sets. */
+
+/* conservedToPrimitiveStag (conservedPrimitiveConversionsStag.cpp:4-144) with the
+ * FillBoundary calls around it (timeStepStag.cpp:526-535): face velocities, prim and cu's
+ * cell-centred momentum components on the valid box, then the periodic ghosts of cu, cumom,
+ * vel and prim.  Also the start of a run (main_driver.cpp:766-794). */
+int fhdx_cons_to_prim_stag(fhdx_ctx* ctx, fhdx_fab prim, const fhdx_fab vel[3], fhdx_fab cu,
+                           const fhdx_fab cumom[3], void* stream);
+
+/* calculateFluxStag (fluxStag.cpp:5-1868; signature compressible_functions_stag.H:127-143)
+ * with the noise A + weight B of the (seed, step) sets; eta..D are inputs (the caller's
+ * calculateTransportCoeffs, fhdx_transport). */
+int fhdx_calculate_flux_stag(fhdx_ctx* ctx, fhdx_fab cu, const fhdx_fab cumom[3], fhdx_fab prim,
+                             const fhdx_fab vel[3], fhdx_fab eta, fhdx_fab zeta, fhdx_fab kappa,
+                             fhdx_fab chi, fhdx_fab D, const fhdx_fab faceflux[3],
+                             const fhdx_fab edgeflux[6], const fhdx_fab cenflux[3], double weight,
+                             double dt, uint64_t seed, uint64_t step, void* stream);
+
+/* RK3stepStag (timeStepStag.cpp:13-1245; signature compressible_functions_stag.H:113-125):
+ * cu, cumom advanced in place; prim, vel, the transport coefficients and the fluxes end as the
+ * reference leaves them.  cu, cumom, prim and vel enter consistent, ghosts included (as after
+ * fhdx_cons_to_prim_stag).  The stage arrays (cup, cup2 and their face momenta) are the
+ * context's, allocated on the first call. */
+int fhdx_rk3_step_stag(fhdx_ctx* ctx, fhdx_fab cu, const fhdx_fab cumom[3], fhdx_fab prim,
+                       const fhdx_fab vel[3], fhdx_fab eta, fhdx_fab zeta, fhdx_fab kappa,
+                       fhdx_fab chi, fhdx_fab D, const fhdx_fab faceflux[3],
+                       const fhdx_fab edgeflux[6], const fhdx_fab cenflux[3], double dt,
+                       uint64_t seed, uint64_t step, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
