@@ -192,6 +192,51 @@ def ref_rate(n: int, steps: int, warmup: int = 1):
             "steps": steps}
 
 
+def stag_rate(cfg, steps: int, warmup: int = 2, cpu_steps: int = 0):
+    """cell-updates/s of the staggered step (fhdx_rk3_step_stag, RK3stepStag's arithmetic) on a
+    periodic configuration from the uniform start, Philox noise, HIP events around the timed
+    steps on the launch stream.  With cpu_steps > 0 also the oracle's staggered step on the
+    same box for that many steps on the host cores."""
+    import torch
+    from fhdex_amd import ensemble as E
+    from fhdex_amd import lib as L
+    from fhdex_amd import stag as S
+    p = cfg.params
+    ctx = L.Context(p)
+    st = S.StagState(p)
+    for c, v in enumerate(E.uniform_values(cfg)):
+        st.cu.t[c] = v
+    S.conservedToPrimitiveStag(ctx, st.prim, st.vel, st.cu, st.cumom)
+    for s in range(warmup):
+        S.RK3stepStag(ctx, st, cfg.dt, 1, s)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for s in range(steps):
+        S.RK3stepStag(ctx, st, cfg.dt, 1, warmup + s)
+    stop.record()
+    torch.cuda.synchronize()
+    ctx.check()
+    ms = start.elapsed_time(stop) / steps
+    cells = p.n_cells[0] * p.n_cells[1] * p.n_cells[2]
+    out = {"workload": cfg.name + " (staggered scheme, RK3stepStag)", "value": cells / (ms * 1e-3),
+           "unit": "cell-updates/s", "ms_per_step": ms, "steps": steps, "nvars": p.nvars}
+    ctx.close()
+    del st
+    if cpu_steps:
+        sys.path.insert(0, ROOT)
+        from oracle import ffi as orc
+        from tests.helpers import stag_uniform_state
+        host = stag_uniform_state(p, cfg, orc)
+        t0 = time.perf_counter()
+        assert host.advance(cpu_steps, cfg.dt, 1, 0) == 0
+        t = time.perf_counter() - t0
+        out["cpu_oracle"] = {"value": cells * cpu_steps / t, "unit": "cell-updates/s",
+                             "cores": orc.num_threads(), "kind": "port",
+                             "sample": f"{cpu_steps} staggered steps of the same box"}
+    return out
+
+
 def measured_flops(n: int):
     """FP64 flops per cell-update from the committed rocprofv3 counter summary of this workload
     (profiles/fp64_argon<n>.json), or None."""
@@ -338,6 +383,18 @@ def serve_unsaturated(capacity: float, count: int = 64, load: float = 0.6, seed:
     return r
 
 
+def stag_configs(a):
+    """The staggered scheme on the reference's periodic regression deck (four species 32^3,
+    exec/compressible_stag/inputs_regression_periodic_3D, with the oracle beside it), its
+    equilibrium deck size (four species 64^3, inputs_equil_3d), and argon at the headline box."""
+    from fhdex_amd import params as P
+    cases = {"deck32": lambda: stag_rate(P.four_species(32), 20, cpu_steps=0 if a.no_cpu else 3),
+             "equil64": lambda: stag_rate(P.four_species(64), 20),
+             "argon": lambda: stag_rate(P.argon_gas(a.n), 10)}
+    pick = list(cases) if a.stag_case == "all" else [a.stag_case]
+    return [cases[c]() for c in pick]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -364,6 +421,9 @@ def main():
                     help="requests of a stream at 0.6 of the measured serving capacity (0: skip)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--no-configs", action="store_true", help="skip the 32^3 argon and 128^3 four-species rates")
+    ap.add_argument("--no-stag", action="store_true", help="skip the staggered-scheme rates")
+    ap.add_argument("--stag-only", action="store_true", help="only time the staggered-scheme step")
+    ap.add_argument("--stag-case", type=str, default="all", choices=["all", "deck32", "equil64", "argon"])
     ap.add_argument("--shape", type=str, default="", help="launch shape tile_width,region_tiles,z_levels,noise_inline"
                                                          "[,face_blocks_per_cu[,records_in_update]]")
     ap.add_argument("--copy-probe", action="store_true",
@@ -415,6 +475,10 @@ def main():
 
     if a.ref_only:
         print(json.dumps({"reference_shaped": ref_rate(a.n, a.ref_steps, a.warmup)}), flush=True)
+        return
+
+    if a.stag_only:
+        print(json.dumps({"staggered": stag_configs(a)}), flush=True)
         return
 
     from fhdex_amd import compressible as fc
@@ -508,6 +572,8 @@ def main():
                            wall_rate(P.wall_deck("giantfluct_3d"))]
     if a.ref_steps:
         line["reference_shaped"] = ref_rate(a.n, a.ref_steps)
+    if not a.no_stag:
+        line["staggered"] = stag_configs(a)
     if a.ensemble:
         line["ensemble"] = ensemble_bench(a.ensemble, a.ensemble_steps, a.loop_steps)
     rates = [float(v) for v in a.serve_rates.split(",") if v]

@@ -429,8 +429,12 @@ __global__ void __launch_bounds__(SB) k_stag_update(DevParams P, StagUpdate U, i
         int i, j, k;
         cell_of(tid, n0, n1, i, j, k);
         const double src = 0.0;
+        // Components 1-3 (cell-centred momentum) are skipped: the reference computes them here
+        // and conservedToPrimitiveStag overwrites them (and FillBoundary their ghosts) before
+        // anything reads them, so the result does not depend on them.
 #pragma unroll
         for (int n = 0; n < nv; ++n) {
+            if (n >= 1 && n <= 3) continue;
             const double div = (U.ff[0](i + 1, j, k, n) - U.ff[0](i, j, k, n)) / dx[0] +
                                (U.ff[1](i, j + 1, k, n) - U.ff[1](i, j, k, n)) / dx[1] +
                                (U.ff[2](i, j, k + 1, n) - U.ff[2](i, j, k, n)) / dx[2];
