@@ -74,10 +74,6 @@ int stag_scope(const fhdx_ctx* c, const char* who)
             set_error(std::string(who) + ": the staggered scheme is implemented for periodic boxes only");
             return FHDX_ERR_UNSUPPORTED;
         }
-    if (c->p.nreaction > 0) {
-        set_error(std::string(who) + ": the staggered scheme runs without chemistry here (nreaction 0)");
-        return FHDX_ERR_UNSUPPORTED;
-    }
     return FHDX_OK;
 }
 
@@ -238,13 +234,24 @@ int fhdx_calculate_flux_stag(fhdx_ctx* c, fhdx_fab cu, const fhdx_fab cumom[3], 
 
 int fhdx_rk3_step_stag(fhdx_ctx* c, fhdx_fab cu, const fhdx_fab cumom[3], fhdx_fab prim, const fhdx_fab vel[3],
                        fhdx_fab eta, fhdx_fab zeta, fhdx_fab kappa, fhdx_fab chi, fhdx_fab D,
-                       const fhdx_fab faceflux[3], const fhdx_fab edgeflux[6], const fhdx_fab cenflux[3], double dt,
-                       uint64_t seed, uint64_t step, void* stream)
+                       const fhdx_fab faceflux[3], const fhdx_fab edgeflux[6], const fhdx_fab cenflux[3],
+                       fhdx_fab source, fhdx_fab ranchem, double dt, uint64_t seed, uint64_t step, void* stream)
 {
     if (!c) return FHDX_ERR_ARG;
     if (int r = stag_scope(c, "fhdx_rk3_step_stag")) return r;
     if (int r = check_stag_step_args(c, cu, cumom, prim, vel, eta, zeta, kappa, chi, D, faceflux, edgeflux, cenflux, dt))
         return r;
+    const bool chem = c->p.nreaction > 0;
+    if (chem) {
+        if (!c->chem_set) {
+            set_error("nreaction > 0: call fhdx_ctx_set_chemistry before stepping");
+            return FHDX_ERR_ARG;
+        }
+        const int g = c->p.ngc;
+        if ((source.p && !stag_cell(c, source, c->dp.nvars, g, "source")) ||
+            (ranchem.p && !stag_cell(c, ranchem, c->p.nreaction, g, "ranchem")))
+            return FHDX_ERR_ARG;
+    }
     if (int r = ensure_stag(c)) return r;
     hipStream_t s = SS(stream);
     const NoiseKey K = make_noise_key(seed, step, 0);
@@ -270,6 +277,12 @@ int fhdx_rk3_step_stag(fhdx_ctx* c, fhdx_fab cu, const fhdx_fab cumom[3], fhdx_f
         for (int d = 0; d < 3; ++d) A.mom[d] = ms[d];
         if ((r = stag_flux(c->dp, A, K, wts[st], dt, s))) return r;
         StagUpdate U;
+        U.chem = chem;
+        U.prim = A.prim;
+        U.source = source.p && chem ? make_arr(source) : Arr{};
+        U.ranchem = ranchem.p && chem ? make_arr(ranchem) : Arr{};
+        U.K = K;
+        U.w = wts[st];
         U.out = out;
         U.u0 = u0;
         U.us = us;

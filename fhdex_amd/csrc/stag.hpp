@@ -17,10 +17,17 @@ struct StagArrays {
     Arr td, to[3];
 };
 
-// The arrays of one stage combination: out/outm = f(u0/m0, us/ms, fluxes).
+// The arrays of one stage combination: out/outm = f(u0/m0, us/ms, fluxes).  With chemistry
+// (chem != 0) the cell update adds dt times the reaction source of the stage's prim, with the
+// reaction noise A + w B of key K; source (nvars comps) and ranchem (nreaction) receive what
+// the reference leaves in them when their pointers are set.
 struct StagUpdate {
     Arr out, outm[3], u0, m0[3], us, ms[3];
     Arr ff[3], ef[6], cf[3];
+    int chem;
+    Arr prim, source, ranchem;
+    NoiseKey K;
+    double w;
 };
 
 // Periodic ghost fill: array a (cell: nodal -1, face: nodal d), components c0 .. c0+nc-1.

@@ -12,7 +12,8 @@
 // evaluated in place from a counter-based Philox keyed by the wrapped global index, so duplicate
 // periodic faces and edges draw one value (OverrideSync's result) and nothing is stored.  Every
 // expression keeps the reference's operation order: the results equal the oracle's
-// (oracle/oracle_stag.c) bit for bit.  Scope: 3-D, periodic in every direction, no chemistry.
+// (oracle/oracle_stag.c) bit for bit.  Scope: 3-D, periodic in every direction; with chemistry
+// the update adds the reaction source (timeStepStag.cpp:389-396).
 #include "stag.hpp"
 
 namespace fhdx {
@@ -428,7 +429,23 @@ __global__ void __launch_bounds__(SB) k_stag_update(DevParams P, StagUpdate U, i
         if (tid >= (long)n0 * n1 * n2) return;
         int i, j, k;
         cell_of(tid, n0, n1, i, j, k);
-        const double src = 0.0;
+        double src[nv];
+#pragma unroll
+        for (int n = 0; n < nv; ++n) src[n] = 0.0;
+        if (U.chem) {  // compute_compressible_chemistry_source_CLE from the stage's prim
+            const double dV = P.dx[0] * P.dx[1] * P.dx[2];
+            double X[NS];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) X[s] = U.prim(i, j, k, 6 + NS + s);
+            const uint32_t lin = (uint32_t)(((long)k * n1 + j) * n0 + i);
+            chem_source_cell<NS>(P, U.prim(i, j, k, 4), U.prim(i, j, k, 5), X,
+                                 [&](int m) { return chem_noise(U.K, m, lin, U.w); }, dt, dV, &src[5]);
+            if (U.ranchem.p)
+                for (int m = 0; m < P.chem.nreaction; ++m) U.ranchem(i, j, k, m) = chem_noise(U.K, m, lin, U.w);
+            if (U.source.p)
+#pragma unroll
+                for (int n = 5; n < nv; ++n) U.source(i, j, k, n) = src[n];
+        }
         // Components 1-3 (cell-centred momentum) are skipped: the reference computes them here
         // and conservedToPrimitiveStag overwrites them (and FillBoundary their ghosts) before
         // anything reads them, so the result does not depend on them.
@@ -440,11 +457,11 @@ __global__ void __launch_bounds__(SB) k_stag_update(DevParams P, StagUpdate U, i
                                (U.ff[2](i, j, k + 1, n) - U.ff[2](i, j, k, n)) / dx[2];
             double v;
             if (st == 0)
-                v = U.u0(i, j, k, n) - dt * div + dt * src;
+                v = U.u0(i, j, k, n) - dt * div + dt * src[n];
             else if (st == 1)
-                v = 0.25 * (3.0 * U.u0(i, j, k, n) + U.us(i, j, k, n) - dt * div + dt * src);
+                v = 0.25 * (3.0 * U.u0(i, j, k, n) + U.us(i, j, k, n) - dt * div + dt * src[n]);
             else
-                v = (2. / 3.) * (0.5 * U.u0(i, j, k, n) + U.us(i, j, k, n) - dt * div + dt * src);
+                v = (2. / 3.) * (0.5 * U.u0(i, j, k, n) + U.us(i, j, k, n) - dt * div + dt * src[n]);
             U.out(i, j, k, n) = v;
         }
         const double gs = grav[0] * (U.ms[0](i + 1, j, k) + U.ms[0](i, j, k)) +

@@ -110,8 +110,8 @@ def lib():
         L.fhdx_cons_to_prim_stag.argtypes = [vp, Fab, FA, Fab, FA, vp]
         L.fhdx_calculate_flux_stag.argtypes = [vp, Fab, FA, Fab, FA, Fab, Fab, Fab, Fab, Fab, FA, E6, FA,
                                                C.c_double, C.c_double, C.c_uint64, C.c_uint64, vp]
-        L.fhdx_rk3_step_stag.argtypes = [vp, Fab, FA, Fab, FA, Fab, Fab, Fab, Fab, Fab, FA, E6, FA,
-                                         C.c_double, C.c_uint64, C.c_uint64, vp]
+        L.fhdx_rk3_step_stag.argtypes = [vp, Fab, FA, Fab, FA, Fab, Fab, Fab, Fab, Fab, FA, E6, FA, Fab,
+                                         Fab, C.c_double, C.c_uint64, C.c_uint64, vp]
         L.fhdx_ctx_timing_enable.argtypes = [vp, C.c_int]
         L.fhdx_ctx_timing_read.argtypes = [vp, C.c_int, C.c_char_p, C.c_int,
                                            C.POINTER(C.c_int64), C.POINTER(C.c_double)]

@@ -65,6 +65,9 @@ class StagState:
         self.faceflux = [face_mf(p, d, nv + 4, 0, device) for d in range(3)]
         self.edgeflux = [edge_mf(p, e, device) for e in EDGE_FLUX_TYPES]
         self.cenflux = [cell_mf(p, 1, 1, device) for _ in range(3)]
+        # chemistry (main_driver.cpp defines source with nvars and ranchem with nreaction comps)
+        self.source = cell_mf(p, nv, g, device) if p.nreaction > 0 else None
+        self.ranchem = cell_mf(p, p.nreaction, g, device) if p.nreaction > 0 else None
 
 
 def _f3(mfs):
@@ -95,10 +98,14 @@ def calculateFluxStag(ctx: Context, st: StagState, weight: float, dt: float, see
 
 def RK3stepStag(ctx: Context, st: StagState, dt: float, seed: int = 1, step: int = 0, stream=None):
     """RK3stepStag (timeStepStag.cpp:13-1245): cu and cumom advanced in place, prim, vel, the
-    transport coefficients and the fluxes left as the reference leaves them."""
+    transport coefficients and the fluxes left as the reference leaves them; with chemistry
+    (Context.set_chemistry) also source and ranchem."""
+    z = Fab()
+    src = st.source.fab if st.source is not None else z
+    ran = st.ranchem.fab if st.ranchem is not None else z
     check(lib().fhdx_rk3_step_stag(ctx.h, st.cu.fab, _f3(st.cumom), st.prim.fab, _f3(st.vel), st.eta.fab,
                                    st.zeta.fab, st.kappa.fab, st.chi.fab, st.D.fab, _f3(st.faceflux),
-                                   _f6(st.edgeflux), _f3(st.cenflux), float(dt), int(seed), int(step),
+                                   _f6(st.edgeflux), _f3(st.cenflux), src, ran, float(dt), int(seed), int(step),
                                    stream), "RK3stepStag")
 
 

@@ -494,7 +494,7 @@ int fhdx_read_checkpoint(const fhdx_params* p, const char* chk_base_name, int re
  * ghosts, nvars+4 comps; edgeflux[6] = edgeflux_x[0] (xy), edgeflux_x[1] (xz), edgeflux_y[0]
  * (xy), edgeflux_y[1] (yz), edgeflux_z[0] (xz), edgeflux_z[1] (yz), edges nodal along both of
  * their axes, no ghosts; cenflux[d] cells with one ghost layer.  Scope: the 3-D branch on a box
- * periodic in every direction, without chemistry (other settings: FHDX_ERR_UNSUPPORTED).  The
+ * periodic in every direction (walls: FHDX_ERR_UNSUPPORTED), chemistry included.  The
  * noise replaces the MultiFabFillRandomNormal calls of timeStepStag.cpp:171-223 with a
  * counter-based Philox4x32-10 + Box-Muller keyed by (seed, step, instance 0) and the wrapped
  * global index of each face, edge and cell, so duplicate periodic faces and edges share a
@@ -520,12 +520,19 @@ int fhdx_calculate_flux_stag(fhdx_ctx* ctx, fhdx_fab cu, const fhdx_fab cumom[3]
  * cu, cumom advanced in place; prim, vel, the transport coefficients and the fluxes end as the
  * reference leaves them.  cu, cumom, prim and vel enter consistent, ghosts included (as after
  * fhdx_cons_to_prim_stag).  The stage arrays (cup, cup2 and their face momenta) are the
- * context's, allocated on the first call. */
+ * context's, allocated on the first call.  With nreaction > 0 (fhdx_ctx_set_chemistry first)
+ * every stage adds dt times compute_compressible_chemistry_source_CLE of its prim, with the
+ * reaction noise A + w_s B keyed as fhdx_fill_chem_noise keys it (timeStepStag.cpp:389-396 and
+ * the stage-2/3 forms); source (>= nvars comps; the staggered driver allocates nprimvars) and
+ * ranchem (nreaction comps), cell FABs with ngc ghosts, then end as the reference leaves them on
+ * the valid cells (source components 5.. written; 0-4, the drivers' zero "external source", are
+ * taken as zero); either may be passed with p = NULL (not written, and ignored without
+ * chemistry). */
 int fhdx_rk3_step_stag(fhdx_ctx* ctx, fhdx_fab cu, const fhdx_fab cumom[3], fhdx_fab prim,
                        const fhdx_fab vel[3], fhdx_fab eta, fhdx_fab zeta, fhdx_fab kappa,
                        fhdx_fab chi, fhdx_fab D, const fhdx_fab faceflux[3],
-                       const fhdx_fab edgeflux[6], const fhdx_fab cenflux[3], double dt,
-                       uint64_t seed, uint64_t step, void* stream);
+                       const fhdx_fab edgeflux[6], const fhdx_fab cenflux[3], fhdx_fab source,
+                       fhdx_fab ranchem, double dt, uint64_t seed, uint64_t step, void* stream);
 
 #ifdef __cplusplus
 }

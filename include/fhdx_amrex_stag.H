@@ -8,8 +8,8 @@
 // shares: calculateTransportCoeffs, the context cache, the Philox seed), build AMReX with
 // USE_HIP=TRUE for gfx950 and link fhdex_amd/libfhdx.so.  Scope of the library's staggered
 // path: one box per rank covering the whole periodic domain (max_grid_size >= n_cells), the
-// 3-D branch, no reservoir, membrane, turbulence forcing or chemistry; anything else ends in
-// amrex::Abort with the library's message.  Compiled only where AMREX_HOME exists.
+// 3-D branch with chemistry, no reservoir, membrane or turbulence forcing; anything else ends
+// in amrex::Abort with the library's message.  Compiled only where AMREX_HOME exists.
 #pragma once
 
 #include <array>
@@ -103,12 +103,12 @@ inline void calculateFluxStag(const amrex::MultiFab& cons_in, const std::array<a
 // step, where the reference aborts at transCoeffs.cpp:45-46) ends in amrex::Abort.
 inline void RK3stepStag(amrex::MultiFab& cu, std::array<amrex::MultiFab, AMREX_SPACEDIM>& cumom,
                         amrex::MultiFab& prim, std::array<amrex::MultiFab, AMREX_SPACEDIM>& facevel,
-                        amrex::MultiFab& /*source*/, amrex::MultiFab& eta, amrex::MultiFab& zeta,
+                        amrex::MultiFab& source, amrex::MultiFab& eta, amrex::MultiFab& zeta,
                         amrex::MultiFab& kappa, amrex::MultiFab& chi, amrex::MultiFab& D,
                         std::array<amrex::MultiFab, AMREX_SPACEDIM>& faceflux,
                         std::array<amrex::MultiFab, 2>& edgeflux_x, std::array<amrex::MultiFab, 2>& edgeflux_y,
                         std::array<amrex::MultiFab, 2>& edgeflux_z,
-                        std::array<amrex::MultiFab, AMREX_SPACEDIM>& cenflux, amrex::MultiFab& /*ranchem*/,
+                        std::array<amrex::MultiFab, AMREX_SPACEDIM>& cenflux, amrex::MultiFab& ranchem,
                         const amrex::Geometry& geom, const amrex::Real dt, const int /*step*/,
                         TurbForcingComp& /*turbforce*/)
 {
@@ -122,9 +122,11 @@ inline void RK3stepStag(amrex::MultiFab& cu, std::array<amrex::MultiFab, AMREX_S
         fabs3(faceflux, mfi, ff);
         fabs3(cenflux, mfi, cf);
         edge6(edgeflux_x, edgeflux_y, edgeflux_z, mfi, ef);
+        const bool chem = chemistry::nreaction > 0;
         check(fhdx_rk3_step_stag(ctx, fab(cu, mfi), m, fab(prim, mfi), v, fab(eta, mfi), fab(zeta, mfi),
-                                 fab(kappa, mfi), fab(chi, mfi), fab(D, mfi), ff, ef, cf, dt, philox_seed(),
-                                 philox_step(), stream()));
+                                 fab(kappa, mfi), fab(chi, mfi), fab(D, mfi), ff, ef, cf,
+                                 chem ? fab(source, mfi) : fhdx_fab{}, chem ? fab(ranchem, mfi) : fhdx_fab{}, dt,
+                                 philox_seed(), philox_step(), stream()));
     }
     ++philox_step();
     check(fhdx_ctx_check(ctx, stream()));

@@ -378,8 +378,13 @@ class StagHost:
             self.faces(self.fB), self.edges(self.eA), self.edges(self.eB), self.cells(self.cA),
             self.cells(self.cB), dt)
 
-    def advance(self, nsteps, dt, seed=1, step0=0, instance=0):
-        """nsteps of RK3stepStag with Philox noise (seed, step0 + s, instance)."""
+    def advance(self, nsteps, dt, seed=1, step0=0, instance=0, chem=None):
+        """nsteps of RK3stepStag with Philox noise (seed, step0 + s, instance); with `chem` (a
+        Chemistry) the reaction source of every stage too."""
+        if chem is not None:
+            return lib().orc_advance_stag_chem(C.byref(self.p), C.byref(chem), self.F(self.cu), self.faces(self.mom),
+                                               self.F(self.prim), self.faces(self.vel), nsteps, dt, seed, step0,
+                                               instance)
         return lib().orc_advance_stag(C.byref(self.p), self.F(self.cu), self.faces(self.mom), self.F(self.prim),
                                       self.faces(self.vel), nsteps, dt, seed, step0, instance)
 
@@ -396,3 +401,5 @@ def _bind_stag(L):
                                     FA, FA, FA, C.c_double]
     L.orc_advance_stag.argtypes = [P, Fab, FA, Fab, FA, C.c_int, C.c_double, C.c_uint64, C.c_uint64,
                                    C.c_uint32]
+    L.orc_advance_stag_chem.argtypes = [P, C.POINTER(Chemistry), Fab, FA, Fab, FA, C.c_int, C.c_double,
+                                        C.c_uint64, C.c_uint64, C.c_uint32]

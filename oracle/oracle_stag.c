@@ -629,20 +629,21 @@ static void lincomb(fhdx_fab dst_, fhdx_fab a_, fhdx_fab b_, int c0, int nc, dou
 
 /* stage 0: out = u0 - dt div F; 1: out = 1/4 (3 u0 + us - dt div F); 2: out = 2/3 (1/2 u0 + us
  * - dt div F), with the momenta updated from the cell-centred and edge fluxes, the gravity terms
- * and dt * source (zero here: no chemistry). */
+ * and dt * source (source NULL: zero, no chemistry). */
 static void stag_update(const fhdx_params* P, int st, fab_t out, fab_t* outm, fab_t u0, fab_t* m0,
-                        fab_t us, fab_t* ms, const fab_t* ff, const fab_t* ef, const fab_t* cf, double dt)
+                        fab_t us, fab_t* ms, const fab_t* ff, const fab_t* ef, const fab_t* cf,
+                        const fab_t* source, double dt)
 {
     const int nv = nvars_of(P);
     const int n0 = P->n_cells[0], n1 = P->n_cells[1], n2 = P->n_cells[2];
     const double* dx = P->dx;
     const double* grav = P->grav;
-    const double src = 0.0;
 #pragma omp parallel for schedule(static)
     for (int k = 0; k < n2; ++k)
         for (int j = 0; j < n1; ++j)
             for (int i = 0; i < n0; ++i) {
                 for (int n = 0; n < nv; ++n) {
+                    const double src = source ? AT(*source, i, j, k, n) : 0.0;
                     const double div = (AT(ff[0], i + 1, j, k, n) - AT(ff[0], i, j, k, n)) / dx[0] +
                                        (AT(ff[1], i, j + 1, k, n) - AT(ff[1], i, j, k, n)) / dx[1] +
                                        (AT(ff[2], i, j, k + 1, n) - AT(ff[2], i, j, k, n)) / dx[2];
@@ -737,7 +738,16 @@ static void scratch_free(stag_scratch* S)
     free(S->td.p);
 }
 
-static int rk3_step_stag(const fhdx_params* P, stag_scratch* S, fhdx_fab cu, const fhdx_fab cumom[3],
+/* Chemistry of a staggered step (timeStepStag.cpp:389-396 and the stage-2/3 forms): the
+ * reactions, the caller's source (nvars comps) and ranchem (nreaction) and the reaction noise
+ * sets A, B of the step. */
+typedef struct {
+    const fhdx_chemistry* chem;
+    fhdx_fab source, ranchem, A, B;
+} stag_chem;
+
+static int rk3_step_stag(const fhdx_params* P, const stag_chem* ch, stag_scratch* S, fhdx_fab cu,
+                         const fhdx_fab cumom[3],
                          fhdx_fab prim, const fhdx_fab vel[3], fhdx_fab eta, fhdx_fab zeta,
                          fhdx_fab kappa, fhdx_fab chi, fhdx_fab D, const fhdx_fab ff[3],
                          const fhdx_fab ef[6], const fhdx_fab cf[3], const fhdx_fab faceA[3],
@@ -797,7 +807,14 @@ static int rk3_step_stag(const fhdx_params* P, stag_scratch* S, fhdx_fab cu, con
         A.cu = us;
         for (int d = 0; d < 3; ++d) A.mom[d] = ms[d];
         flux_stag(&A);
-        stag_update(P, st, out, outm, u0, m0, us, ms, A.ff, A.ef, A.cf, dt);
+        fab_t src;
+        if (ch) { /* ranchem = 1*A + w*B, then the source from this stage's prim */
+            lincomb(ch->ranchem, ch->A, ch->B, 0, ch->chem->nreaction, w);
+            rc = orc_chem_source(P, ch->chem, prim, ch->source, ch->ranchem, dt);
+            if (rc != FHDX_OK) return rc;
+            src = F(ch->source);
+        }
+        stag_update(P, st, out, outm, u0, m0, us, ms, A.ff, A.ef, A.cf, ch ? &src : NULL, dt);
         stage_end(P, A.prim, vv, out, outm);
     }
     return FHDX_OK;
@@ -813,7 +830,7 @@ int orc_rk3_step_stag(const fhdx_params* P, fhdx_fab cu, const fhdx_fab cumom[3]
 {
     stag_scratch S;
     scratch_alloc(P, &S);
-    int rc = rk3_step_stag(P, &S, cu, cumom, prim, vel, eta, zeta, kappa, chi, D, faceflux, edgeflux,
+    int rc = rk3_step_stag(P, NULL, &S, cu, cumom, prim, vel, eta, zeta, kappa, chi, D, faceflux, edgeflux,
                            cenflux, faceA, faceB, edgeA, edgeB, cenA, cenB, dt);
     scratch_free(&S);
     return rc;
@@ -860,9 +877,9 @@ int orc_flux_stag(const fhdx_params* P, fhdx_fab cu, const fhdx_fab cumom[3], fh
     return FHDX_OK;
 }
 
-int orc_advance_stag(const fhdx_params* P, fhdx_fab cu, const fhdx_fab cumom[3], fhdx_fab prim,
-                     const fhdx_fab vel[3], int nsteps, double dt, uint64_t seed, uint64_t step0,
-                     uint32_t instance)
+static int advance_stag(const fhdx_params* P, const fhdx_chemistry* chem, fhdx_fab cu,
+                        const fhdx_fab cumom[3], fhdx_fab prim, const fhdx_fab vel[3], int nsteps,
+                        double dt, uint64_t seed, uint64_t step0, uint32_t instance)
 {
     const int g = P->ngc, NS = P->nspecies, nv = nvars_of(P);
     fhdx_fab eta = mk_cell(P, g, 1), zeta = mk_cell(P, g, 1), kappa = mk_cell(P, g, 1);
@@ -883,13 +900,30 @@ int orc_advance_stag(const fhdx_params* P, fhdx_fab cu, const fhdx_fab cumom[3],
     for (int e = 0; e < 6; ++e) ef[e] = mk_edge(P, et[e], 1);
     stag_scratch S;
     scratch_alloc(P, &S);
+    stag_chem ch;
+    memset(&ch, 0, sizeof(ch));
+    if (chem) {
+        const int nr = chem->nreaction;
+        ch.chem = chem;
+        ch.source = mk_cell(P, g, nv);
+        ch.ranchem = mk_cell(P, g, nr);
+        ch.A = mk_cell(P, g, nr);
+        ch.B = mk_cell(P, g, nr);
+    }
     int rc = FHDX_OK;
     for (int s = 0; s < nsteps && rc == FHDX_OK; ++s) {
         orc_fill_noise_stag(P, seed, step0 + (uint64_t)s, instance, fA, fB, eA, eB, cA, cB);
-        rc = rk3_step_stag(P, &S, cu, cumom, prim, vel, eta, zeta, kappa, chi, D, ff, ef, cf, fA, fB, eA,
-                           eB, cA, cB, dt);
+        if (chem) orc_fill_chem_noise(P, seed, step0 + (uint64_t)s, instance, ch.A, ch.B);
+        rc = rk3_step_stag(P, chem ? &ch : NULL, &S, cu, cumom, prim, vel, eta, zeta, kappa, chi, D, ff, ef, cf,
+                           fA, fB, eA, eB, cA, cB, dt);
     }
     scratch_free(&S);
+    if (chem) {
+        free(ch.source.p);
+        free(ch.ranchem.p);
+        free(ch.A.p);
+        free(ch.B.p);
+    }
     free(eta.p);
     free(zeta.p);
     free(kappa.p);
@@ -907,4 +941,18 @@ int orc_advance_stag(const fhdx_params* P, fhdx_fab cu, const fhdx_fab cumom[3],
     }
     for (int e = 0; e < 6; ++e) free(ef[e].p);
     return rc;
+}
+
+int orc_advance_stag(const fhdx_params* P, fhdx_fab cu, const fhdx_fab cumom[3], fhdx_fab prim,
+                     const fhdx_fab vel[3], int nsteps, double dt, uint64_t seed, uint64_t step0,
+                     uint32_t instance)
+{
+    return advance_stag(P, NULL, cu, cumom, prim, vel, nsteps, dt, seed, step0, instance);
+}
+
+int orc_advance_stag_chem(const fhdx_params* P, const fhdx_chemistry* chem, fhdx_fab cu,
+                          const fhdx_fab cumom[3], fhdx_fab prim, const fhdx_fab vel[3], int nsteps,
+                          double dt, uint64_t seed, uint64_t step0, uint32_t instance)
+{
+    return advance_stag(P, chem, cu, cumom, prim, vel, nsteps, dt, seed, step0, instance);
 }

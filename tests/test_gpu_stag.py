@@ -240,3 +240,37 @@ def test_stag_equilibrium_variances():
     for name, gv, w, r in zip(["rho", "jx", "jy", "jz", "rhoE"], got, want, ratio):
         say(f"  Var {name}: {gv:.5e} theory {w:.5e} ratio {r:.4f}")
     assert np.all(np.abs(ratio - 1) < 0.01), ratio
+
+
+@pytest.mark.parametrize("rtype", [1, 0])
+def test_step_stag_chemistry_matches_oracle(orc, rtype):
+    """The staggered step with the NO2/N2O4 dimerization of the reference's staggered deck
+    (exec/compressible_stag/test_NO2_dimerization_eq): per stage the chemical-Langevin source of
+    the stage's prim (reaction noise A + w_s B) enters the cell update; three steps from a
+    perturbed state equal the oracle's bit for bit, and the source / ranchem arrays hold the
+    last stage's values."""
+    from fhdex_amd import lib as L
+    from fhdex_amd import stag as S
+    cfg = P.no2_n2o4(4, reaction_type=rtype)
+    p = cfg.params
+    p.n_cells[:] = (6, 5, 4)
+    say(f"case step_stag chemistry reaction_type {rtype}")
+    host = perturbed_stag_state(p, cfg, orc, seed=6, amp=0.02, vel=3e3)
+    st = _device(p, host)
+    ctx = L.Context(p)
+    ctx.set_chemistry(cfg.chem)
+    for s in range(3):
+        S.RK3stepStag(ctx, st, cfg.dt, 4, s)
+    ctx.check()
+    assert host.advance(3, cfg.dt, 4, 0, chem=cfg.chem) == 0
+    _same("cu", st.cu.t, host.cu)
+    _same("prim", st.prim.t, host.prim)
+    for d in range(3):
+        _same(f"cumom{d}", st.cumom[d].t, host.mom[d])
+        _same(f"vel{d}", st.vel[d].t, host.vel[d])
+    g = p.ngc
+    src = st.source.t[5:, g:-g, g:-g, g:-g]
+    assert float(src.abs().max()) > 0
+    assert float(st.source.t[:5].abs().max()) == 0.0
+    if rtype == 1:
+        assert float(st.ranchem.t[:, g:-g, g:-g, g:-g].abs().max()) > 0

@@ -485,3 +485,61 @@ def test_stag_trace_species_gets_no_species_noise(orc):
         assert np.array_equal(noisy[d][7], quiet[d][7]), d
         for s in (5, 6):
             assert np.abs(noisy[d][s] - quiet[d][s]).max() > 0, (d, s)
+
+
+# ---- chemistry ------------------------------------------------------------------------------
+
+def _no2(shape, **kw):
+    cfg = P.no2_n2o4(4, **kw)
+    p = cfg.params
+    p.n_cells[:] = shape
+    return cfg, p
+
+
+def test_stag_chemistry_source_enters_as_dt_times_source(orc):
+    """A uniform NO2/N2O4 gas off equilibrium with the stochastic fluxes off and reaction_type 0
+    (no reaction noise): every flux vanishes, so one staggered step changes rho Y_k by the RK3
+    integral of the reaction source, dt S_k up to the source's own change within the step (S from
+    the reference's rate law at the start, oracle/oracle.c::orc_chem_source; the composition and,
+    through e0, the temperature move by ~1e-4 of the rate over 1e-12 s), and keeps rho, rho E and
+    the face momenta (zero)."""
+    cfg, p = _no2((4, 4, 4), reaction_type=0)
+    p.stoch_stress_form = 0
+    Y = [0.9, 0.1]                           # far from the deck's 0.724 / 0.276 balance
+    cfg.rhobar = Y
+    st = stag_uniform_state(p, cfg, orc)
+    cu0 = valid(p, st.cu).copy()
+    # the source of the start
+    S = np.zeros((p.nvars,) + st.cu.shape[1:])
+    ran = np.zeros((p.nreaction,) + st.cu.shape[1:])
+    g = p.ngc
+    assert orc.chem_source(p, cfg.chem, st.prim, S, ran, cfg.dt) == 0
+    assert st.advance(1, cfg.dt, 1, 0, chem=cfg.chem) == 0
+    d = valid(p, st.cu) - cu0
+    want = cfg.dt * valid(p, S)
+    for k in range(2):
+        assert np.abs(d[5 + k] / want[5 + k] - 1).max() < 1e-3, k
+    assert np.abs(want[5]).min() > 0
+    assert np.abs(d[0]).max() <= 1e-15 * cfg.rho0
+    assert np.abs(d[4]).max() <= 1e-13 * abs(cu0[4]).max()
+    for dd in range(3):
+        assert not np.any(st.mom[dd])
+
+
+def test_stag_chemistry_conserves_mass_with_noise(orc):
+    """With fluctuating hydrodynamics and the chemical Langevin noise on (reaction_type 1), three
+    steps keep sum rho, sum rho E and the face momentum sums to rounding (M(N2O4) = 2 M(NO2):
+    each reaction conserves mass), while the species totals move."""
+    cfg, p = _no2((6, 5, 4), reaction_type=1)
+    st = perturbed_stag_state(p, cfg, orc, seed=4, amp=0.02, vel=3e3)
+    tot0 = valid(p, st.cu).sum(axis=(1, 2, 3))
+    abs0 = np.abs(valid(p, st.cu)).sum(axis=(1, 2, 3))
+    m0 = [face_valid(p, st.mom[d], d).sum() for d in range(3)]
+    ma = [np.abs(face_valid(p, st.mom[d], d)).sum() for d in range(3)]
+    assert st.advance(3, cfg.dt, 5, 0, chem=cfg.chem) == 0
+    tot1 = valid(p, st.cu).sum(axis=(1, 2, 3))
+    assert abs(tot1[0] - tot0[0]) <= 2e-14 * abs0[0]
+    assert abs(tot1[4] - tot0[4]) <= 2e-14 * abs0[4]
+    assert abs(tot1[5] - tot0[5]) > 1e-10 * abs0[5]
+    for d in range(3):
+        assert abs(face_valid(p, st.mom[d], d).sum() - m0[d]) <= 2e-14 * ma[d]
