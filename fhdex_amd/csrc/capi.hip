@@ -223,8 +223,12 @@ int fhdx_params_check(const fhdx_params* p)
             return FHDX_ERR_UNSUPPORTED;
         }
     }
-    if (p->transport_type != 1) {
-        set_error("only transport_type 1 (Giovangigli) is implemented");
+    if (p->transport_type != 1 && p->transport_type != 3) {
+        set_error("transport_type must be 1 (Giovangigli) or 3 (Hirschfelder-Curtiss-Bird, binary)");
+        return FHDX_ERR_UNSUPPORTED;
+    }
+    if (p->transport_type == 3 && p->nspecies != 2) {
+        set_error("transport_type 3 (Hirschfelder-Curtiss-Bird) is for binary mixtures (nspecies 2)");
         return FHDX_ERR_UNSUPPORTED;
     }
     if (p->nreaction < 0 || p->nreaction > FHDX_MAX_REACTION) {

@@ -54,6 +54,7 @@ inline DevParams make_dev_params(const fhdx_params& p)
     d.advection_type = p.advection_type;
     d.visc_type = p.visc_type;
     d.stoch_on = (p.stoch_stress_form == 1);
+    d.transport_type = p.transport_type;
     for (int n = 0; n < MS; ++n) {
         d.molmass[n] = p.molmass[n];
         d.diameter[n] = p.diameter[n];

@@ -35,6 +35,7 @@ struct DevParams {
     // whole domain unless the arrays are one FAB of a MultiFab split into several boxes.
     int bo[3], bn[3];
     int algorithm_type, advection_type, visc_type, stoch_on;
+    int transport_type;  // 1 Giovangigli, 3 Hirschfelder-Curtiss-Bird binary
     double dx[3];
     double molmass[MS], diameter[MS], hcv[MS], hcp[MS], e0[MS];
     double k_B, Runiv, zeta_ratio;
@@ -172,6 +173,75 @@ struct Transport {
     double eta, zeta, kappa;
 };
 
+// IdealMixtureTransportHCBBin (compressible_functions.H:1357-1478): Hirschfelder-Curtiss-Bird
+// first approximations for a binary mixture of hard spheres with the correction factors 1.016
+// (viscosity), 1.025 (conductivity), 1.133 (diffusion) and 1.299 (thermal diffusion).  Xk are the
+// mole fractions of the clamped mass fractions (transCoeffs.cpp:60-61); D[4] before the density
+// factor, chi[2].
+__device__ inline void hcb_binary(const DevParams& P, double density, double temperature, const double* Xk,
+                                  Transport& tr, double* chi, double* D)
+{
+    const double pi = 3.1415926535897932;
+    const double kB = P.k_B;
+    const double m0 = P.molmass[0] * (kB / P.Runiv), m1 = P.molmass[1] * (kB / P.Runiv);
+    double xxtr[2], yytr[2];
+#pragma unroll
+    for (int ii = 0; ii < 2; ++ii) {
+        double sumXk = 0.;
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj) sumXk += Xk[jj];
+        xxtr[ii] = Xk[ii] + 1.e-15 * (sumXk / 2.0 - Xk[ii]);
+    }
+    double MWmix = 0.;
+#pragma unroll
+    for (int ii = 0; ii < 2; ++ii) MWmix += xxtr[ii] * P.molmass[ii];
+#pragma unroll
+    for (int ii = 0; ii < 2; ++ii) yytr[ii] = P.molmass[ii] / MWmix * xxtr[ii];
+    const double d0 = P.diameter[0], d1 = P.diameter[1];
+    const double mx = 2.0 * m0 * m1 / (m0 + m1);
+    const double dx = 0.5 * (d0 + d1);
+    const double conc = xxtr[1], onc = xxtr[0];
+    const double sm = (m0 + m1) * (m0 + m1), dm = (m0 - m1) * (m0 - m1);
+    const double eta0 = (5.0 / (16.0 * d0 * d0)) * sqrt(m0 * kB * temperature / pi);
+    const double eta1 = (5.0 / (16.0 * d1 * d1)) * sqrt(m1 * kB * temperature / pi);
+    const double etax = (5.0 / (16.0 * dx * dx)) * sqrt(mx * kB * temperature / pi);
+    const double Xeta = (onc * onc / eta0) + (2.0 * conc * onc / etax) + (conc * conc / eta1);
+    const double Yeta = (3.0 / 5.0) * ((onc * onc * m0) / (eta0 * m1) +
+                                       (2.0 * conc * onc * etax * (m0 + m1) * (m0 + m1)) / (4.0 * eta0 * eta1 * m0 * m1) +
+                                       (conc * conc * m1) / (eta1 * m0));
+    const double Zeta = (3.0 / 5.0) * ((onc * onc * m0 / m1) +
+                                       2.0 * conc * onc * ((sm / (4.0 * m0 * m1)) * ((etax / eta0) + (etax / eta1)) - 1.0) +
+                                       conc * conc * m1 / m0);
+    tr.eta = 1.016 * (1.0 + Zeta) / (Xeta + Yeta);
+    const double kappa0 = 15.0 * kB * eta0 / (4.0 * m0);
+    const double kappa1 = 15.0 * kB * eta1 / (4.0 * m1);
+    const double kappax = 15.0 * kB * etax / (4.0 * mx);
+    const double U0 = 4.0 / 15.0 - 17.0 * m0 / (60.0 * m1) + dm / (2.0 * m0 * m1);
+    const double U1 = 4.0 / 15.0 - 17.0 * m1 / (60.0 * m0) + dm / (2.0 * m0 * m1);
+    const double UY = 4.0 * (m0 + m1) * (m0 + m1) * kappax * kappax / (15.0 * 4.0 * m0 * m1 * kappa0 * kappa1) -
+                      17.0 / 60.0 + 13.0 * (m0 - m1) * (m0 - m1) / (32.0 * m0 * m1);
+    const double UZ = (4.0 / 15.0) * ((sm / (4.0 * m0 * m1)) * ((kappax / kappa0) + (kappax / kappa1)) - 1.0) - 17.0 / 60.0;
+    const double Xkappa = onc * onc / kappa0 + 2.0 * conc * onc / kappax + conc * conc / kappa1;
+    const double Ykappa = onc * onc * U0 / kappa0 + 2.0 * conc * onc * UY / kappax + conc * conc * U1 / kappa1;
+    const double Zkappa = onc * onc * U0 + 2.0 * conc * onc * UZ + conc * conc * U1;
+    tr.kappa = 1.025 * (1.0 + Zkappa) / (Xkappa + Ykappa);
+    tr.zeta = 0.0;
+    const double diff = 6.0 * 1.133 * etax / (5.0 * density);
+    const double S0 = (m0 + m1) * kappax / (2.0 * m1 * kappa0) - 15 * (m1 - m0) / (8.0 * m0) - 1.0;
+    const double S1 = (m0 + m1) * kappax / (2.0 * m0 * kappa1) - 15 * (m0 - m1) / (8.0 * m1) - 1.0;
+    double thermal_diff = 1.299 * conc * onc * (S1 * conc - S0 * onc) / (6.0 * kappax * (Xkappa + Ykappa));
+    const double mR = m0 / m1;
+    const double d0t = mR * diff * yytr[1] * yytr[1] * yytr[1] / (yytr[0] * xxtr[1] * xxtr[1]);
+    const double d1t = mR * diff * yytr[1] * yytr[1] / (xxtr[1] * xxtr[1]);
+    D[0] = 1.0 * yytr[0] * d0t;
+    D[1] = -1.0 * yytr[0] * d0t;
+    D[2] = -1.0 * yytr[0] * d1t;
+    D[3] = 1.0 * yytr[0] * d1t;
+    thermal_diff = thermal_diff * mR * yytr[1] * yytr[1] / (xxtr[1] * xxtr[1]);
+    chi[0] = -1.0 * xxtr[1] * xxtr[1] * thermal_diff / (mR * xxtr[0] * yytr[1] * yytr[1]);
+    chi[1] = 1.0 * xxtr[1] * thermal_diff / (mR * yytr[1] * yytr[1]);
+}
+
 // Transport coefficients of one cell (transCoeffs.cpp:37-88).  chi[NS], D[NS*NS] out.
 // Returns true if a mass fraction lies outside (0,1) (the reference aborts there).
 template <int NS>
@@ -190,6 +260,22 @@ __device__ inline bool transport_cell(const DevParams& P, double rho, double T, 
     }
 #pragma unroll
     for (int n = 0; n < NS; ++n) Yk[n] /= sumYk;
+
+    if constexpr (NS == 2) {
+        if (P.transport_type == 3) {  // Hirschfelder-Curtiss-Bird, binary (transCoeffs.cpp:71-75)
+            double molmix = 0., Xf[2];
+#pragma unroll
+            for (int n = 0; n < 2; ++n) molmix += Yk[n] / P.molmass[n];
+            molmix = 1. / molmix;
+#pragma unroll
+            for (int n = 0; n < 2; ++n) Xf[n] = Yk[n] * molmix / P.molmass[n];
+            hcb_binary(P, rho, T, Xf, tr, chi, D);
+#pragma unroll
+            for (int n = 0; n < 4; ++n) D[n] *= rho;
+            if (P.visc_type == 3 || P.visc_type == -3) tr.zeta = P.zeta_ratio * tr.eta;
+            return bad;
+        }
+    }
 
     const double pi = 3.1415926535897932;
     double mm[NS], Xk[NS], Xkp[NS], Ykp[NS], eta1[NS], rhs[NS], Bi[NS];

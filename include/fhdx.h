@@ -55,7 +55,7 @@ typedef struct fhdx_params {
     int32_t algorithm_type;    /* 2: multispecies (species fluxes on), else species off      */
     int32_t advection_type;    /* 0: none, 1: interpolate prim, 2: interpolate cons          */
     int32_t visc_type;         /* |3| adds bulk viscosity zeta = zeta_ratio*eta              */
-    int32_t transport_type;    /* 1: Giovangigli (the only one implemented)                  */
+    int32_t transport_type;    /* 1: Giovangigli, 3: Hirschfelder-Curtiss-Bird (binary)      */
     int32_t stoch_stress_form; /* 1: stochastic fluxes on, 0: deterministic                  */
     int32_t nreaction;         /* 0, or the reaction count given to fhdx_ctx_set_chemistry   */
     /* Boundary codes per direction (common_namespace.H, boundary.cpp:172-2001).  bc_vel -1:

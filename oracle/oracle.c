@@ -14,7 +14,7 @@
  * Functions and the reference code they follow:
  *   cons_to_prim     conservedPrimitiveConversions.cpp:48-89, compressible_functions.H:138-199
  *   transport        transCoeffs.cpp:37-88, compressible_functions.H:1153-1352 (GIO),
- *                    :647-778 (D_GIO1)
+ *                    :647-778 (D_GIO1), :1357-1478 (HCB binary, transport_type 3)
  *   stoch_face       flux.cpp:71-1145 (+ CholeskyDecomp compressible_functions.H:460-509)
  *   diff_face        flux.cpp:1193-1623
  *   corner_node      flux.cpp:1627-1713
@@ -182,6 +182,75 @@ static void d_gio1(int NS, double Dbin[MS][MS], const double* Ykp, const double*
 /* IdealMixtureTransportGIO (compressible_functions.H:1153-1352) followed by the rho scaling
  * (transCoeffs.cpp:76-80) and bulk viscosity (:84-86).  Returns 1 if a mass fraction is outside
  * (0,1) (the reference aborts at transCoeffs.cpp:45-46). */
+/* IdealMixtureTransportHCBBin (compressible_functions.H:1357-1478): Hirschfelder-Curtiss-Bird
+ * first approximations for a binary mixture of hard spheres with the correction factors 1.016
+ * (viscosity), 1.025 (conductivity), 1.133 (diffusion) and 1.299 (thermal diffusion); Xk from
+ * GetMolfrac of the clamped mass fractions (transCoeffs.cpp:60-61).  diff_ij[4] and chitil[2]
+ * out (diff_ij before the density factor). */
+static void hcb_bin(const fhdx_params* P, double density, double temperature, const double* Xk,
+                    double* eta, double* kappa, double* zeta, double* diff_ij, double* chitil)
+{
+    const double pi = 3.1415926535897932;
+    const double kB = P->k_B;
+    double mm[2], xxtr[2], yytr[2];
+    for (int n = 0; n < 2; ++n) mm[n] = P->molmass[n] * (kB / P->Runiv);
+    for (int ii = 0; ii < 2; ++ii) {
+        double sumXk = 0.;
+        for (int jj = 0; jj < 2; ++jj) sumXk += Xk[jj];
+        xxtr[ii] = Xk[ii] + 1.e-15 * (sumXk / 2.0 - Xk[ii]);
+    }
+    double MWmix = 0.;
+    for (int ii = 0; ii < 2; ++ii) MWmix += xxtr[ii] * P->molmass[ii];
+    for (int ii = 0; ii < 2; ++ii) yytr[ii] = P->molmass[ii] / MWmix * xxtr[ii];
+    const double m0 = mm[0], m1 = mm[1];
+    const double d0 = P->diameter[0], d1 = P->diameter[1];
+    const double mx = 2.0 * m0 * m1 / (m0 + m1);
+    const double dx = 0.5 * (d0 + d1);
+    const double conc = xxtr[1], onc = xxtr[0];
+    const double sm = (m0 + m1) * (m0 + m1), dm = (m0 - m1) * (m0 - m1);
+    /* viscosity */
+    const double eta0 = (5.0 / (16.0 * d0 * d0)) * sqrt(m0 * kB * temperature / pi);
+    const double eta1 = (5.0 / (16.0 * d1 * d1)) * sqrt(m1 * kB * temperature / pi);
+    const double etax = (5.0 / (16.0 * dx * dx)) * sqrt(mx * kB * temperature / pi);
+    const double Xeta = (onc * onc / eta0) + (2.0 * conc * onc / etax) + (conc * conc / eta1);
+    const double Yeta = (3.0 / 5.0) * ((onc * onc * m0) / (eta0 * m1) +
+                                       (2.0 * conc * onc * etax * (m0 + m1) * (m0 + m1)) / (4.0 * eta0 * eta1 * m0 * m1) +
+                                       (conc * conc * m1) / (eta1 * m0));
+    const double Zeta = (3.0 / 5.0) * ((onc * onc * m0 / m1) +
+                                       2.0 * conc * onc * ((sm / (4.0 * m0 * m1)) * ((etax / eta0) + (etax / eta1)) - 1.0) +
+                                       conc * conc * m1 / m0);
+    *eta = 1.016 * (1.0 + Zeta) / (Xeta + Yeta);
+    /* thermal conductivity */
+    const double kappa0 = 15.0 * kB * eta0 / (4.0 * m0);
+    const double kappa1 = 15.0 * kB * eta1 / (4.0 * m1);
+    const double kappax = 15.0 * kB * etax / (4.0 * mx);
+    const double U0 = 4.0 / 15.0 - 17.0 * m0 / (60.0 * m1) + dm / (2.0 * m0 * m1);
+    const double U1 = 4.0 / 15.0 - 17.0 * m1 / (60.0 * m0) + dm / (2.0 * m0 * m1);
+    const double UY = 4.0 * (m0 + m1) * (m0 + m1) * kappax * kappax / (15.0 * 4.0 * m0 * m1 * kappa0 * kappa1) -
+                      17.0 / 60.0 + 13.0 * (m0 - m1) * (m0 - m1) / (32.0 * m0 * m1);
+    const double UZ = (4.0 / 15.0) * ((sm / (4.0 * m0 * m1)) * ((kappax / kappa0) + (kappax / kappa1)) - 1.0) - 17.0 / 60.0;
+    const double Xkappa = onc * onc / kappa0 + 2.0 * conc * onc / kappax + conc * conc / kappa1;
+    const double Ykappa = onc * onc * U0 / kappa0 + 2.0 * conc * onc * UY / kappax + conc * conc * U1 / kappa1;
+    const double Zkappa = onc * onc * U0 + 2.0 * conc * onc * UZ + conc * conc * U1;
+    *kappa = 1.025 * (1.0 + Zkappa) / (Xkappa + Ykappa);
+    *zeta = 0.0;
+    /* diffusion coefficient and thermal diffusion ratio */
+    const double diff = 6.0 * 1.133 * etax / (5.0 * density);
+    const double S0 = (m0 + m1) * kappax / (2.0 * m1 * kappa0) - 15 * (m1 - m0) / (8.0 * m0) - 1.0;
+    const double S1 = (m0 + m1) * kappax / (2.0 * m0 * kappa1) - 15 * (m0 - m1) / (8.0 * m1) - 1.0;
+    double thermal_diff = 1.299 * conc * onc * (S1 * conc - S0 * onc) / (6.0 * kappax * (Xkappa + Ykappa));
+    const double mR = m0 / m1;
+    const double d0t = mR * diff * yytr[1] * yytr[1] * yytr[1] / (yytr[0] * xxtr[1] * xxtr[1]);
+    const double d1t = mR * diff * yytr[1] * yytr[1] / (xxtr[1] * xxtr[1]);
+    diff_ij[0] = 1.0 * yytr[0] * d0t;
+    diff_ij[1] = -1.0 * yytr[0] * d0t;
+    diff_ij[2] = -1.0 * yytr[0] * d1t;
+    diff_ij[3] = 1.0 * yytr[0] * d1t;
+    thermal_diff = thermal_diff * mR * yytr[1] * yytr[1] / (xxtr[1] * xxtr[1]);
+    chitil[0] = -1.0 * xxtr[1] * xxtr[1] * thermal_diff / (mR * xxtr[0] * yytr[1] * yytr[1]);
+    chitil[1] = 1.0 * xxtr[1] * thermal_diff / (mR * yytr[1] * yytr[1]);
+}
+
 static int transport_cell(const fhdx_params* P, const double* pr, double* eta, double* zeta,
                           double* kappa, double* chi, double* D)
 {
@@ -197,6 +266,15 @@ static int transport_cell(const fhdx_params* P, const double* pr, double* eta, d
         sumYk += Yk[n];
     }
     for (int n = 0; n < NS; ++n) Yk[n] /= sumYk;
+
+    if (P->transport_type == 3) { /* Hirschfelder-Curtiss-Bird, binary (transCoeffs.cpp:71-75) */
+        double Xf[MS];
+        get_molfrac(P, Yk, Xf);
+        hcb_bin(P, pr[0], pr[4], Xf, eta, kappa, zeta, D, chi);
+        for (int n = 0; n < 4; ++n) D[n] *= pr[0];
+        if (abs(P->visc_type) == 3) *zeta = P->zeta_ratio * *eta;
+        return bad;
+    }
 
     const double T = pr[4], p = pr[5];
     const double pi = 3.1415926535897932;

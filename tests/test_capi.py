@@ -96,6 +96,17 @@ def test_params_default_finalize_check():
     lib.fhdx_params_finalize(C.byref(p))
     assert p.hcv[0] == pytest.approx(1.5 * P.R_UNIV / 39.95)
     assert lib.fhdx_params_check(C.byref(p)) == 0
+    t = p.copy()
+    t.transport_type = 3                                   # Hirschfelder-Curtiss-Bird: binary only
+    assert lib.fhdx_params_check(C.byref(t)) == 0
+    t.nspecies = 3
+    t.molmass[2], t.diameter[2] = 39.95, 3.66e-8
+    lib.fhdx_params_finalize(C.byref(t))
+    assert lib.fhdx_params_check(C.byref(t)) == 2
+    assert b"binary" in lib.fhdx_last_error()
+    t = p.copy()
+    t.transport_type = 4
+    assert lib.fhdx_params_check(C.byref(t)) == 2
     q = p.copy()
     q.bc_vel_lo[0] = 1
     assert lib.fhdx_params_check(C.byref(q)) == 1          # one side periodic, the other not

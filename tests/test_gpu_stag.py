@@ -26,6 +26,18 @@ def say(msg):
 
 
 def _gas(name, shape, **kw):
+    tt = kw.pop("transport_type", 1)
+    cfg, p = _gas_params(name, shape, **kw)
+    p.transport_type = tt
+    return cfg, p
+
+
+def _gas_params(name, shape, **kw):
+    if name == "heAr":
+        cfg = P.species_prefix(2, 4, 1)
+        p = P.make_params(2, shape, 8e-6, [4.0026, 39.948], [2.18e-8, 3.63e-8], **kw)
+        cfg.rhobar = [0.3, 0.7]
+        return cfg, p
     if name == "argon":
         cfg = P.argon_gas(4)
         p = P.make_params(2, shape, 8e-6, [39.95, 39.95], [3.66e-8, 3.66e-8], **kw)
@@ -57,8 +69,9 @@ def _same(name, got, want):
 
 CASES = [("argon", dict()), ("four", dict()), ("argon", dict(advection_type=0, visc_type=3, zeta_ratio=1.5)),
          ("four", dict(advection_type=1)), ("four", dict(grav=(0.0, -3.0e9, 1.0e9), advection_type=0)),
-         ("argon", dict(stoch_stress_form=0))]
-IDS = ["argon", "four", "argon-adv0-visc3", "four-adv1", "four-gravity-adv0", "argon-deterministic"]
+         ("argon", dict(stoch_stress_form=0)), ("heAr", dict(transport_type=3))]
+IDS = ["argon", "four", "argon-adv0-visc3", "four-adv1", "four-gravity-adv0", "argon-deterministic",
+       "heAr-hcb-transport"]
 
 
 @pytest.mark.parametrize("gas", ["argon", "four"])

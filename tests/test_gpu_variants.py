@@ -133,3 +133,43 @@ def test_trace_species_branch(orc):
     ctx.check()
     assert orc.rk3_step(p, st, cfg.dt) == 0
     check("fused cu", valid(p, cu.t.cpu().numpy()), valid(p, st.cu))
+
+
+def test_hcb_binary_transport_one_step(orc):
+    """transport_type 3 (Hirschfelder-Curtiss-Bird binary forms, compressible_functions.H:
+    1357-1478) on a helium-argon mixture: the transport arrays of fhdx_transport, one fused step
+    and one reference-shaped step with the same injected noise, against the oracle."""
+    import torch
+    from fhdex_amd import compressible as fc
+    from fhdex_amd import lib as L
+    base = P.species_prefix(2, 10, 1)
+    p = P.make_params(2, (10, 9, 8), 8e-6, [4.0026, 39.948], [2.18e-8, 3.63e-8])
+    p.transport_type = 3
+    cfg = P.Config("heAr_hcb", p, base.rho0, [0.3, 0.7], base.T_init, base.dt, 1)
+    g = p.ngc
+    say(f"case HCB binary transport {tuple(p.n_cells)}")
+    st = perturbed_state(p, cfg, orc, seed=19, amp=0.02, vel=400.0)
+    ctx = L.Context(p)
+    ds = fc.DeviceState(p)
+    ds.cu.t.copy_(torch.from_numpy(st.cu))
+    ds.prim.t.copy_(torch.from_numpy(st.prim))
+    fc.calculateTransportCoeffs(ctx, ds.prim, ds.eta, ds.zeta, ds.kappa, ds.chi, ds.D)
+    torch.cuda.synchronize()
+    assert orc.transport(p, st) == 0
+    for name in ("eta", "zeta", "kappa", "chi", "D"):
+        check(name, getattr(ds, name).t.cpu().numpy(), getattr(st, name))
+    orc.fill_noise(p, st, 5, 2)
+    cu = fc.MultiFab(torch.from_numpy(st.cu.copy()).cuda(), (-g,) * 3)
+    prim = fc.MultiFab.cell(p, p.nprimvars)
+    ns = fc.NoiseSets(p)
+    for d in range(3):
+        ns.A[d].t.copy_(torch.from_numpy(st.sA[d]))
+        ns.B[d].t.copy_(torch.from_numpy(st.sB[d]))
+    ns.rA.t.copy_(torch.from_numpy(st.rA))
+    ns.rB.t.copy_(torch.from_numpy(st.rB))
+    fc.rk3_step_injected(ctx, cu, prim, ns, cfg.dt)
+    fc.RK3step(ctx, ds, cfg.dt, noise=ns)
+    ctx.check()
+    assert orc.rk3_step(p, st, cfg.dt) == 0
+    check("fused cu", valid(p, cu.t.cpu().numpy()), valid(p, st.cu))
+    check("reference-shaped cu", valid(p, ds.cu.t.cpu().numpy()), valid(p, st.cu))

@@ -40,10 +40,11 @@ BINARY_Y = [(0.5, 0.5), (0.2, 0.8), (0.9, 0.1), (0.01, 0.99)]
 T0, RHO0 = 300.0, 4.83e-4
 
 
-def uniform_transport(orc, M, d, Y, T=T0, rho=RHO0):
+def uniform_transport(orc, M, d, Y, T=T0, rho=RHO0, transport_type=1):
     """The oracle's eta, kappa, chi, D (D(l*NS+k) = rho Diff_kl Y_k) and X, p for a uniform state."""
     NS = len(M)
     p = P.make_params(NS, (4, 4, 4), 8e-6, M, d)
+    p.transport_type = transport_type
     st = orc.HostState(p)
     Y = np.asarray(Y, float)
     Mv = np.asarray(M, float)
@@ -262,3 +263,51 @@ def test_four_species_diffusion_solves_stefan_maxwell(orc, Y, tol):
         dv -= dv.mean()
         V = np.linalg.lstsq(K, np.concatenate([dv, [0.0]]), rcond=None)[0]
         assert np.abs(-Diff @ dv - V).max() <= tol * np.abs(V).max()
+
+
+# ---- transport_type 3: the reference's Hirschfelder-Curtiss-Bird binary forms ------------------
+
+AR_AR = ((39.95, 39.95), (3.66e-8, 3.66e-8))
+
+
+@pytest.mark.parametrize("gas", PAIRS + [AR_AR], ids=["He-Ar", "Ne-Kr", "Ar-He", "Ar-Ar"])
+@pytest.mark.parametrize("Y", BINARY_Y)
+def test_hcb_binary_transport_matches_textbook_times_corrections(orc, gas, Y):
+    """IdealMixtureTransportHCBBin (compressible_functions.H:1357-1478) against HCB sec. 8.2
+    written independently above: viscosity 1.016 (1 + Z_eta)/(X_eta + Y_eta), conductivity
+    1.025 (1 + Z_lambda)/(X_lambda + Y_lambda), thermal diffusion X_2 chi_2 = -X_1 chi_1 =
+    1.299 k_T(2); the Fick coefficient is 1.133 rho D_12 M_1 M_2 / Mbar^2 times 2 mu / mbar,
+    because the reference divides the interaction viscosity by rho where HCB's (6/5) eta_12 /
+    (n m_12) has n m_12 = n 2 mu (the two agree at equal mass fractions and for one gas)."""
+    M, d = gas
+    eta, kap, chi, D, X, pres = uniform_transport(orc, M, d, Y, transport_type=3)
+    he, hl, kT1 = hcb_binary(M, d, X, T0)
+    assert eta == pytest.approx(1.016 * he, rel=1e-13)
+    assert kap == pytest.approx(1.025 * hl, rel=1e-13)
+    kT2 = -kT1
+    if M[0] == M[1]:
+        assert chi[0] == 0.0 and chi[1] == 0.0
+    else:
+        assert X[1] * chi[1] == pytest.approx(1.299 * kT2, rel=1e-12)
+        assert X[0] * chi[0] == pytest.approx(-1.299 * kT2, rel=1e-12)
+    m1, m2 = (Mi * KB / RU for Mi in M)
+    mu = m1 * m2 / (m1 + m2)
+    Mbar = float(np.dot(X, M))
+    mbar = Mbar * KB / RU
+    fick = RHO0 * binary_D12(M, d, T0, pres) * M[0] * M[1] / Mbar ** 2
+    assert D[0] - D[2] == pytest.approx(1.133 * (2.0 * mu / mbar) * fick, rel=1e-12)
+    assert D[3] - D[1] == pytest.approx(D[0] - D[2], rel=1e-12)
+    assert D[0] + D[1] == pytest.approx(0.0, abs=1e-15 * abs(D[0]))
+    assert D[2] + D[3] == pytest.approx(0.0, abs=1e-15 * abs(D[3]))
+
+
+def test_hcb_and_gio_agree_up_to_the_correction_factors(orc):
+    """For rigid spheres both are the first Chapman-Enskog approximation: eta, kappa and chi of
+    transport_type 3 are those of the Giovangigli solve (type 1) times 1.016, 1.025 and 1.299."""
+    for M, d in (HE_AR, NE_KR):
+        for Y in BINARY_Y:
+            e1, k1, c1, D1, X, _ = uniform_transport(orc, M, d, Y)
+            e3, k3, c3, D3, _, _ = uniform_transport(orc, M, d, Y, transport_type=3)
+            assert e3 == pytest.approx(1.016 * e1, rel=1e-13)
+            assert k3 == pytest.approx(1.025 * k1, rel=1e-13)
+            assert np.allclose(c3, 1.299 * c1, rtol=1e-12, atol=0)
