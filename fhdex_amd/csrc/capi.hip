@@ -127,8 +127,8 @@ void fhdx_params_finalize(fhdx_params* p)
 int fhdx_params_check(const fhdx_params* p)
 {
     if (!p) return FHDX_ERR_ARG;
-    if (p->nspecies < 2 || p->nspecies > 4) {
-        set_error("nspecies must be 2..4 (a single gas is written as two identical species)");
+    if (p->nspecies < 2 || p->nspecies > FHDX_MAX_SPECIES) {
+        set_error("nspecies must be 2..8 (a single gas is written as two identical species)");
         return FHDX_ERR_UNSUPPORTED;
     }
     for (int d = 0; d < 3; ++d) {

@@ -122,6 +122,10 @@ static int compute_prim(Engine* e, const EngineSlot& s)
         case 2: kb_prim<2><<<nb, EB, 0, e->stream>>>(d.P, d.G, d.U, d.prim, d.ncell); break;
         case 3: kb_prim<3><<<nb, EB, 0, e->stream>>>(d.P, d.G, d.U, d.prim, d.ncell); break;
         case 4: kb_prim<4><<<nb, EB, 0, e->stream>>>(d.P, d.G, d.U, d.prim, d.ncell); break;
+        case 5: kb_prim<5><<<nb, EB, 0, e->stream>>>(d.P, d.G, d.U, d.prim, d.ncell); break;
+        case 6: kb_prim<6><<<nb, EB, 0, e->stream>>>(d.P, d.G, d.U, d.prim, d.ncell); break;
+        case 7: kb_prim<7><<<nb, EB, 0, e->stream>>>(d.P, d.G, d.U, d.prim, d.ncell); break;
+        case 8: kb_prim<8><<<nb, EB, 0, e->stream>>>(d.P, d.G, d.U, d.prim, d.ncell); break;
         default: return FHDX_ERR_UNSUPPORTED;
     }
     FHDX_HIP_TRY(hipGetLastError());
@@ -221,7 +225,7 @@ static void drop_graph(Engine* e)
 
 static void drop_tables(Engine* e)
 {
-    for (int g = 0; g < 3; ++g) {
+    for (int g = 0; g < kEngineGroups; ++g) {
         if (e->d_cells[g]) (void)hipFree(e->d_cells[g]);
         e->d_cells[g] = nullptr;
         e->ncells_items[g] = 0;
@@ -346,7 +350,7 @@ static int rebuild(Engine* e)
     drop_graph(e);
     drop_tables(e);
     std::vector<int> act;
-    for (int g = 0; g < 3; ++g) {
+    for (int g = 0; g < kEngineGroups; ++g) {
         std::vector<WorkItem> cells;
         for (int q = 0; q < e->max_inst; ++q) {
             const EngineSlot& s = e->slots[q];
@@ -393,6 +397,10 @@ static int enqueue_step(Engine* e)
     enqueue_group<2>(e, 0, ws);
     enqueue_group<3>(e, 1, ws);
     enqueue_group<4>(e, 2, ws);
+    enqueue_group<5>(e, 3, ws);
+    enqueue_group<6>(e, 4, ws);
+    enqueue_group<7>(e, 5, ws);
+    enqueue_group<8>(e, 6, ws);
     if (e->nactive) kb_advance<<<(e->nactive + 255) / 256, 256, 0, e->stream>>>(e->d_inst, e->d_active, e->nactive);
     FHDX_HIP_TRY(hipGetLastError());
     return FHDX_OK;

@@ -42,6 +42,9 @@ struct EngineSlot {
     uint64_t bytes = 0;
 };
 
+// species counts 2..FHDX_MAX_SPECIES, one work table each
+constexpr int kEngineGroups = FHDX_MAX_SPECIES - 1;
+
 struct Engine {
     int max_inst = 0;
     uint64_t budget = 0, used = 0;
@@ -49,9 +52,9 @@ struct Engine {
     Inst* d_inst = nullptr;
     int* d_err = nullptr;
     hipStream_t stream = nullptr;
-    // work tables per species count (index ns-2): cell chunks and ghost chunks
-    WorkItem* d_cells[3] = {nullptr, nullptr, nullptr};
-    int ncells_items[3] = {0, 0, 0};
+    // work tables per species count (index ns-2): cell chunks
+    WorkItem* d_cells[kEngineGroups] = {};
+    int ncells_items[kEngineGroups] = {};
     int* d_active = nullptr;  // active slot list for the step counter kernel
     int nactive = 0;
     bool dirty = true;        // membership changed: rebuild tables, recapture the graph

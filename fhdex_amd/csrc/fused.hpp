@@ -78,6 +78,13 @@ struct FusedWorkspace {
 
 void fused_free(FusedWorkspace& w);
 
+// Species counts above this evaluate the step's Philox noise into arrays (k_noise) before the
+// stages, whatever noise_inline says.  At NS = 8 the face kernel with the noise evaluated in
+// place (256 VGPRs + 236 AGPRs, no scratch) returned values that differ from the same kernel
+// reading the noise from arrays, and from the oracle; its cause was not found.  NS 6 and 7 take
+// the array kernels too, so one kernel family serves every wide mixture.
+constexpr int kInlineNoiseMaxNS = 5;
+
 // The file-scope device scratch of fhdx_rk3_step_noalloc (noalloc.hip): its size, the lookup of
 // a device's copy (once per device, with that device current; fhdx_ctx_create runs it) and the
 // pointer found (nullptr before the lookup).
@@ -86,7 +93,8 @@ int noalloc_scratch_init(int device);
 double* noalloc_scratch(int device);
 
 // Device bytes of the fast path's workspace for this box with the noise evaluated in place:
-// two stage states, the cell records, the node terms and three face-flux arrays.
+// two stage states, the cell records, the node terms and three face-flux arrays (plus the
+// step's noise arrays for species counts above kInlineNoiseMaxNS).
 size_t fused_workspace_bytes(const DevParams& P);
 
 // One RK3 step: cu advanced in place, prim recomputed, ghosts of both filled.  Noise from

@@ -360,7 +360,7 @@ int fused_rk3_step(const fhdx_params& p, const DevParams& P, FusedWorkspace& w, 
     const Grid G = grid_of(P);
     const long ncell = (long)P.n[0] * P.n[1] * P.n[2];
     const bool philox = (sA == nullptr);
-    const bool inl = philox && w.noise_inline;
+    const bool inl = philox && w.noise_inline && P.ns <= kInlineNoiseMaxNS;
     if (philox && !inl) {
         if ((r = ensure_noise_arrays(P, w))) return r;
         w.timing.begin(KT_NOISE, s);
@@ -368,6 +368,10 @@ int fused_rk3_step(const fhdx_params& p, const DevParams& P, FusedWorkspace& w, 
             case 2: k_noise<7><<<grid_for(ncell, FB), FB, 0, s>>>(P, K, G, w.sA[0], w.sA[1], w.sA[2], w.sB[0], w.sB[1], w.sB[2], w.rA, w.rB); break;
             case 3: k_noise<8><<<grid_for(ncell, FB), FB, 0, s>>>(P, K, G, w.sA[0], w.sA[1], w.sA[2], w.sB[0], w.sB[1], w.sB[2], w.rA, w.rB); break;
             case 4: k_noise<9><<<grid_for(ncell, FB), FB, 0, s>>>(P, K, G, w.sA[0], w.sA[1], w.sA[2], w.sB[0], w.sB[1], w.sB[2], w.rA, w.rB); break;
+            case 5: k_noise<10><<<grid_for(ncell, FB), FB, 0, s>>>(P, K, G, w.sA[0], w.sA[1], w.sA[2], w.sB[0], w.sB[1], w.sB[2], w.rA, w.rB); break;
+            case 6: k_noise<11><<<grid_for(ncell, FB), FB, 0, s>>>(P, K, G, w.sA[0], w.sA[1], w.sA[2], w.sB[0], w.sB[1], w.sB[2], w.rA, w.rB); break;
+            case 7: k_noise<12><<<grid_for(ncell, FB), FB, 0, s>>>(P, K, G, w.sA[0], w.sA[1], w.sA[2], w.sB[0], w.sB[1], w.sB[2], w.rA, w.rB); break;
+            case 8: k_noise<13><<<grid_for(ncell, FB), FB, 0, s>>>(P, K, G, w.sA[0], w.sA[1], w.sA[2], w.sB[0], w.sB[1], w.sB[2], w.rA, w.rB); break;
             default: return FHDX_ERR_UNSUPPORTED;
         }
         w.timing.end(s);
@@ -407,6 +411,11 @@ int fused_rk3_step(const fhdx_params& p, const DevParams& P, FusedWorkspace& w, 
             case 7: FHDX_STAGE(3, true); break;
             case 8: FHDX_STAGE(4, false); break;
             case 9: FHDX_STAGE(4, true); break;
+            case 10: FHDX_STAGE(5, false); break;
+            case 11: FHDX_STAGE(5, true); break;
+            case 12: FHDX_STAGE(6, false); break;
+            case 14: FHDX_STAGE(7, false); break;
+            case 16: FHDX_STAGE(8, false); break;
             default: return FHDX_ERR_UNSUPPORTED;
         }
         FHDX_HIP_TRY(hipGetLastError());
@@ -419,7 +428,8 @@ size_t fused_workspace_bytes(const DevParams& P)
 {
     const Grid G = grid_of(P);
     const RecLayout L{P.ns};
-    const size_t comps = 2 * (size_t)P.nvars + L.ncomp() + NODE_NCOMP + 3 * (size_t)P.nvars;
+    size_t comps = 2 * (size_t)P.nvars + L.ncomp() + NODE_NCOMP + 3 * (size_t)P.nvars;
+    if (P.ns > kInlineNoiseMaxNS) comps += 6 * (size_t)P.nvars + 2;  // the step's noise arrays
     return comps * (size_t)G.sn * sizeof(double);
 }
 
@@ -452,6 +462,14 @@ int fused_rk3_step_in(const fhdx_params& p, const DevParams& P, const FusedWorks
     take(w.rec, L.ncomp());
     take(w.node, NODE_NCOMP);
     for (int d = 0; d < 3; ++d) take(w.F[d], P.nvars);
+    if (P.ns > kInlineNoiseMaxNS) {
+        for (int d = 0; d < 3; ++d) {
+            take(w.sA[d], P.nvars);
+            take(w.sB[d], P.nvars);
+        }
+        take(w.rA, 1);
+        take(w.rB, 1);
+    }
     w.ready = true;
     return fused_rk3_step(p, P, w, cu, prim, dt, K, nullptr, nullptr, nullptr, nullptr, d_err, s);
 }

@@ -639,6 +639,10 @@ __global__ void __launch_bounds__(SB) k_stag_fill(DevParams P, FillList L)
         case 2: { constexpr int NS = 2; CALL; } break;  \
         case 3: { constexpr int NS = 3; CALL; } break;  \
         case 4: { constexpr int NS = 4; CALL; } break;  \
+        case 5: { constexpr int NS = 5; CALL; } break;  \
+        case 6: { constexpr int NS = 6; CALL; } break;  \
+        case 7: { constexpr int NS = 7; CALL; } break;  \
+        case 8: { constexpr int NS = 8; CALL; } break;  \
         default: return FHDX_ERR_UNSUPPORTED;           \
     }
 

@@ -208,6 +208,10 @@ int stats_launch(const DevParams& P, const StatsArgs& a, hipStream_t s)
         case 2: stats_passes<2>(P, a, s); break;
         case 3: stats_passes<3>(P, a, s); break;
         case 4: stats_passes<4>(P, a, s); break;
+        case 5: stats_passes<5>(P, a, s); break;
+        case 6: stats_passes<6>(P, a, s); break;
+        case 7: stats_passes<7>(P, a, s); break;
+        case 8: stats_passes<8>(P, a, s); break;
         default: return FHDX_ERR_UNSUPPORTED;
     }
     FHDX_HIP_TRY(hipGetLastError());

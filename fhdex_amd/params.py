@@ -272,16 +272,26 @@ def taylor_green(n=64, nsteps=100):
     return Config(f"taylor_green_{n}^3", p, rho, [0.5, 0.5], 273.0, 1.0e-9, nsteps, prob_type=4)
 
 
-def species_prefix(n_species, edge, nsteps):
-    """Species 1..n_species of the four-species list with equal mass fractions (the served
-    request mix); n_species == 1 is written as two identical species (argon form)."""
+# Eight hard-sphere species: the four of exec/compressible/inputs_profiling_gpu:82-83 (He, Ne,
+# Ar, Kr), then Xe, N2, O2 and CO2 with rigid-sphere diameters of the size room-temperature
+# viscosities give (approximate values: they only have to be distinct and physical).
+EIGHT_SPECIES_M = [4.0026, 20.1797, 39.9480, 83.8000, 131.293, 28.0134, 31.9988, 44.0095]
+EIGHT_SPECIES_D = [2.18e-8, 2.58e-8, 3.63e-8, 4.16e-8, 4.85e-8, 3.75e-8, 3.61e-8, 4.59e-8]
+
+
+def species_prefix(n_species, edge, nsteps, shape=None):
+    """Species 1..n_species of the eight-species list with equal mass fractions (1..4 is the
+    served request mix); n_species == 1 is written as two identical species (argon form).
+    The box is edge^3 cells, or `shape`."""
     if n_species <= 1:
-        return argon_gas(edge, nsteps)
-    M = [4.0026, 20.1797, 39.9480, 83.8000][:n_species]
-    d = [2.18e-8, 2.58e-8, 3.63e-8, 4.16e-8][:n_species]
-    p = make_params(n_species, (edge, edge, edge), 8e-6, M, d)
+        return argon_gas(edge, nsteps, shape)
+    M = EIGHT_SPECIES_M[:n_species]
+    d = EIGHT_SPECIES_D[:n_species]
+    shape = tuple(shape) if shape is not None else (edge, edge, edge)
+    p = make_params(n_species, shape, 8e-6, M, d)
     rhobar = [1.0 / n_species] * n_species
-    return Config(f"species{n_species}_{edge}^3", p, 4.83e-4, rhobar, 300.0, MIXTURE_DT, nsteps)
+    box = f"{edge}^3" if shape is None else "x".join(str(s) for s in shape)
+    return Config(f"species{n_species}_{box}", p, 4.83e-4, rhobar, 300.0, MIXTURE_DT, nsteps)
 
 
 def no2_n2o4(n, nsteps=100, reaction_type=1):

@@ -49,7 +49,7 @@ enum fhdx_status {
 /* Run parameters: the values the reference reads from its AMREX_GPU_MANAGED namespaces
  * (src_common/common_namespace.H:24-115, src_compressible/compressible_namespace.H:1-15). */
 typedef struct fhdx_params {
-    int32_t nspecies;          /* NS, 1..FHDX_MAX_SPECIES                                   */
+    int32_t nspecies;          /* NS, 2..FHDX_MAX_SPECIES (one gas: two identical species)  */
     int32_t n_cells[3];        /* domain cells per direction                                 */
     int32_t ngc;               /* ghost layers of cell data (>= 2; <= n_cells/2 at walls)    */
     int32_t algorithm_type;    /* 2: multispecies (species fluxes on), else species off      */
@@ -301,7 +301,7 @@ int fhdx_fill_chem_noise(fhdx_ctx* ctx, uint64_t seed, uint64_t step, uint32_t i
 /* Launch shape of the fast path: tile width in x (16, 32 or 64 cells; the tile has 256 cells),
  * tiles per region of the region-major block order, how many z levels a tile block marches
  * through, whether the Philox noise is evaluated in place by every stage (1) or written once per
- * step to arrays (0), and an optional cap on resident face-kernel blocks per CU (fewer blocks
+ * step to arrays (0; always so for more than five species), and an optional cap on resident face-kernel blocks per CU (fewer blocks
  * in flight keep more of their neighbours' data in L2). */
 typedef struct fhdx_tune {
     int32_t tile_width;

@@ -157,6 +157,8 @@ def test_params_default_finalize_check():
     q = p.copy()
     q.nspecies = 1
     assert lib.fhdx_params_check(C.byref(q)) == 2
+    q.nspecies = 9                                         # above MAX_SPECIES = 8
+    assert lib.fhdx_params_check(C.byref(q)) == 2
     q = p.copy()
     q.ngc = 1
     assert lib.fhdx_params_check(C.byref(q)) == 2
@@ -164,7 +166,8 @@ def test_params_default_finalize_check():
 
 def test_config_params_pass_check():
     lib = L.lib()
-    for cfg in (P.argon_gas(32), P.four_species(16), P.species_prefix(3, 16, 10), P.no2_n2o4(16),
+    for cfg in (P.argon_gas(32), P.four_species(16), P.species_prefix(3, 16, 10), P.species_prefix(5, 8, 1),
+                P.species_prefix(8, 8, 1), P.no2_n2o4(16),
                 P.wall_deck("slip_wall_adiabatic"), P.wall_deck("noslip_reservoir_isothermal"),
                 P.wall_deck("temp_grad"), P.wall_deck("RT_3d"), P.wall_deck("giantfluct_3d")):
         assert lib.fhdx_params_check(C.byref(cfg.params)) == 0, cfg.name

@@ -46,9 +46,11 @@ def solo(dev, cfg, cu0, seed, step0, nsteps):
 
 
 def members(orc):
-    """Uneven boxes and species counts 2, 3, 4, each with a perturbed state."""
+    """Uneven boxes and species counts 2 to 8, each with a perturbed state."""
     cfgs = [P.argon_gas(0, shape=(12, 8, 16)), P.argon_gas(0, shape=(5, 9, 7)),
-            P.species_prefix(3, 8, 10), P.four_species(6), P.argon_gas(16)]
+            P.species_prefix(3, 8, 10), P.four_species(6), P.argon_gas(16),
+            P.species_prefix(5, 0, 10, shape=(6, 7, 5)), P.species_prefix(6, 5, 10),
+            P.species_prefix(7, 0, 10, shape=(4, 6, 5)), P.species_prefix(8, 6, 10)]
     return [(cfg, perturbed_state(cfg.params, cfg, orc, seed=40 + q, amp=0.01, vel=500.0).cu)
             for q, cfg in enumerate(cfgs)]
 
@@ -60,9 +62,9 @@ def budget_for(E, cfgs):
 @pytest.mark.parametrize("graph", [True, False])
 def test_engine_instances_equal_solo_runs(dev, orc, graph):
     torch, fc, L, E = dev
-    say(f"case engine vs solo, 5 uneven members, graph={graph}")
     mem = members(orc)
-    ens = E.Ensemble(8, budget_for(E, [c for c, _ in mem]))
+    say(f"case engine vs solo, {len(mem)} uneven members, graph={graph}")
+    ens = E.Ensemble(12, budget_for(E, [c for c, _ in mem]))
     slots = [ens.add(cfg.params, cfg.dt, 100 + q, cu0, step0=3) for q, (cfg, cu0) in enumerate(mem)]
     ens.step(2, graph=graph)
     ens.step(2, graph=graph)

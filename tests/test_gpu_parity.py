@@ -80,7 +80,9 @@ def report(tag, got, want):
     return e
 
 
-CASES = [("argon", lambda: P.argon_gas(12)), ("species4", lambda: P.four_species(10))]
+CASES = [("argon", lambda: P.argon_gas(12)), ("species4", lambda: P.four_species(10)),
+         ("species5", lambda: P.species_prefix(5, 0, 1, shape=(9, 7, 8))),
+         ("species8", lambda: P.species_prefix(8, 0, 1, shape=(8, 6, 7)))]
 
 
 @pytest.mark.parametrize("name,mk", CASES)
@@ -285,6 +287,20 @@ def test_four_species_16_cubed_20_steps(dev, orc):
     assert e.max() <= TOL100
 
 
+@pytest.mark.parametrize("ns", [5, 6, 7, 8])
+def test_wide_mixture_8_cubed_10_steps(dev, orc, ns):
+    """Five to eight species (the reference's MAX_SPECIES is 8) through the fused path with its
+    Philox noise, the oracle stepped with the same noise."""
+    torch, fc, L = dev
+    cfg = P.species_prefix(ns, 8, 10)
+    say(f"case {ns} species 8^3 x 10 steps")
+    p, st, cu, prim = _multi_step(torch, fc, L, orc, cfg, 10, 9)
+    e = report("cu after 10", valid(p, cu), valid(p, st.cu))
+    assert e.max() <= TOL100
+    e = report("prim after 10", valid(p, prim), valid(p, st.prim))
+    assert e.max() <= TOL100
+
+
 def test_fused_equals_reference_shaped_path_64_cubed(dev, orc):
     """Both GPU paths, Philox noise from the same key, 3 steps at 64^3: identical bits."""
     torch, fc, L = dev
@@ -304,12 +320,15 @@ def test_fused_equals_reference_shaped_path_64_cubed(dev, orc):
     assert torch.equal(ds.prim.t, prim.t)
 
 
-def test_launch_shapes_give_identical_bits(dev, orc):
-    """Every launch shape the tuner can pick computes the same bits; the tuner leaves cu alone."""
+@pytest.mark.parametrize("name,mk", [("argon", lambda: P.argon_gas(0, shape=(40, 24, 20))),
+                                     ("species8", lambda: P.species_prefix(8, 0, 1, shape=(24, 12, 10)))])
+def test_launch_shapes_give_identical_bits(dev, orc, name, mk):
+    """Every launch shape the tuner can pick computes the same bits, and those of the oracle
+    stepped with the same noise; the tuner leaves cu alone."""
     torch, fc, L = dev
-    cfg = P.argon_gas(0, shape=(40, 24, 20))
+    cfg = mk()
     p = cfg.params
-    say("case launch shapes, argon 40x24x20 x 2 steps")
+    say(f"case launch shapes, {name} {tuple(p.n_cells)} x 2 steps")
     st = perturbed_state(p, cfg, orc, seed=21, amp=0.01, vel=500.0)
     ctx = L.Context(p)
     ref = None
@@ -326,6 +345,13 @@ def test_launch_shapes_give_identical_bits(dev, orc):
         if ref is None:
             ref = got
         assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), shape
+    ns = fc.NoiseSets(p)
+    for step in range(2):
+        fc.fill_noise(ctx, ns, 9, step)
+        noise_to_host(st, ns)
+        assert orc.rk3_step(p, st, cfg.dt) == 0
+    e = report("cu after 2 (every shape)", valid(p, ref[0]), valid(p, st.cu))
+    assert e.max() <= TOL1
     cu = to_dev(torch, fc, st.cu, (-p.ngc,) * 3)
     best, trials = ctx.autotune(cu.fab, cfg.dt, trial_steps=1)
     assert np.array_equal(host(cu), st.cu)

@@ -41,6 +41,9 @@ def _gas_params(name, shape, **kw):
     if name == "argon":
         cfg = P.argon_gas(4)
         p = P.make_params(2, shape, 8e-6, [39.95, 39.95], [3.66e-8, 3.66e-8], **kw)
+    elif name == "eight":
+        cfg = P.species_prefix(8, 4, 1)
+        p = P.make_params(8, shape, 8e-6, P.EIGHT_SPECIES_M, P.EIGHT_SPECIES_D, **kw)
     else:
         cfg = P.four_species(4)
         p = P.make_params(4, shape, 8e-6, list(cfg.params.molmass[:4]), list(cfg.params.diameter[:4]), **kw)
@@ -69,12 +72,13 @@ def _same(name, got, want):
 
 CASES = [("argon", dict()), ("four", dict()), ("argon", dict(advection_type=0, visc_type=3, zeta_ratio=1.5)),
          ("four", dict(advection_type=1)), ("four", dict(grav=(0.0, -3.0e9, 1.0e9), advection_type=0)),
-         ("argon", dict(stoch_stress_form=0)), ("heAr", dict(transport_type=3))]
+         ("argon", dict(stoch_stress_form=0)), ("heAr", dict(transport_type=3)), ("eight", dict()),
+         ("eight", dict(advection_type=0))]
 IDS = ["argon", "four", "argon-adv0-visc3", "four-adv1", "four-gravity-adv0", "argon-deterministic",
-       "heAr-hcb-transport"]
+       "heAr-hcb-transport", "eight", "eight-adv0"]
 
 
-@pytest.mark.parametrize("gas", ["argon", "four"])
+@pytest.mark.parametrize("gas", ["argon", "four", "eight"])
 def test_c2p_stag_matches_oracle(orc, gas):
     from fhdex_amd import lib as L
     from fhdex_amd import stag as S

@@ -24,6 +24,7 @@ def say(msg):
 
 @pytest.mark.parametrize("name,mk", [("argon", lambda: P.argon_gas(0, shape=(20, 12, 9))),
                                      ("species4", lambda: P.four_species(10)),
+                                     ("species8", lambda: P.species_prefix(8, 7, 1)),
                                      ("temp_grad walls", lambda: P.wall_deck("temp_grad", 16)),
                                      ("giantfluct one cell deep", lambda: P.wall_deck("giantfluct_3d", 8))])
 def test_evaluate_stats_matches_oracle(orc, name, mk):

@@ -177,7 +177,11 @@ def test_cons_to_prim_recovers_construction(orc):
 
 # ---- whole step invariants --------------------------------------------------------------
 
-@pytest.mark.parametrize("maker,n", [(P.argon_gas, 8), (P.four_species, 6)])
+def _eight_species(n):
+    return P.species_prefix(8, n, 1)
+
+
+@pytest.mark.parametrize("maker,n", [(P.argon_gas, 8), (P.four_species, 6), (_eight_species, 5)])
 def test_step_conserves_totals(orc, maker, n):
     cfg = maker(n)
     p = cfg.params

@@ -28,13 +28,22 @@ def _argon(shape, **kw):
     return cfg, p
 
 
-@pytest.mark.parametrize("gas", ["argon", "four"])
+def _eight(shape, **kw):
+    cfg = P.species_prefix(8, 4, 1)
+    p = P.make_params(8, shape, 8e-6, P.EIGHT_SPECIES_M, P.EIGHT_SPECIES_D, **kw)
+    return cfg, p
+
+
+GASES = {"argon": _argon, "four": _four, "eight": _eight}
+
+
+@pytest.mark.parametrize("gas", ["argon", "four", "eight"])
 def test_stag_step_conserves_mass_momentum_energy_species(orc, gas):
     """Flux form on a periodic box: sum rho, rho E, rho Y_k over cells and sum of each momentum
     component over the n distinct faces of its direction are unchanged by three noisy steps to
     rounding; the duplicate face n keeps equal to face 0 bit for bit (what OverrideSync and
     FillBoundary leave in the reference)."""
-    cfg, p = (_argon if gas == "argon" else _four)((6, 5, 4))
+    cfg, p = GASES[gas]((6, 5, 4))
     st = perturbed_stag_state(p, cfg, orc, seed=3)
     tot0 = valid(p, st.cu).sum(axis=(1, 2, 3))
     abs0 = np.abs(valid(p, st.cu)).sum(axis=(1, 2, 3))

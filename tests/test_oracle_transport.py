@@ -311,3 +311,67 @@ def test_hcb_and_gio_agree_up_to_the_correction_factors(orc):
             assert e3 == pytest.approx(1.016 * e1, rel=1e-13)
             assert k3 == pytest.approx(1.025 * k1, rel=1e-13)
             assert np.allclose(c3, 1.299 * c1, rtol=1e-12, atol=0)
+
+
+# ---- five to eight species (the reference's MAX_SPECIES is 8) ----------------------------------
+
+def _wide_Y(NS, kind):
+    if kind == "equal":
+        return [1.0 / NS] * NS
+    if kind == "dirichlet":
+        return list(np.random.default_rng(NS).dirichlet(np.ones(NS)))
+    Y = [1e-6] * NS  # one host gas with the others in trace
+    Y[2] = 1.0 - (NS - 1) * 1e-6
+    return Y
+
+
+@pytest.mark.parametrize("NS", [5, 6, 7, 8])
+@pytest.mark.parametrize("kind", ["equal", "dirichlet", "trace"])
+def test_many_species_match_hcb_determinants(orc, NS, kind):
+    """Up to eight distinct species (params.EIGHT_SPECIES_*): GIO's eta and kappa equal the HCB
+    determinant forms, and sum_i X_i chi_i = 0."""
+    M, d = P.EIGHT_SPECIES_M[:NS], P.EIGHT_SPECIES_D[:NS]
+    eta, kap, chi, D, X, pres = uniform_transport(orc, M, d, _wide_Y(NS, kind))
+    he, hl = hcb_matrices(M, d, X, T0)
+    assert eta == pytest.approx(he, rel=1e-13)
+    assert kap == pytest.approx(hl, rel=1e-13)
+    assert abs(np.dot(X, chi)) <= 1e-13 * np.abs(chi).max()
+
+
+@pytest.mark.parametrize("NS", [5, 8])
+@pytest.mark.parametrize("kind", ["equal", "dirichlet"])
+def test_many_species_diffusion_solves_stefan_maxwell(orc, NS, kind):
+    """D_GIO1's species velocities solve the Stefan-Maxwell relations for five and eight
+    species, to the truncation of its three-term series (2e-3 here)."""
+    M, d = P.EIGHT_SPECIES_M[:NS], P.EIGHT_SPECIES_D[:NS]
+    Y = _wide_Y(NS, kind)
+    eta, kap, chi, D, X, pres = uniform_transport(orc, M, d, Y)
+    Diff = D.reshape(NS, NS).T / (RHO0 * np.asarray(Y)[:, None])
+    A = np.zeros((NS, NS))
+    for i in range(NS):
+        for j in range(NS):
+            if i != j:
+                a = X[i] * X[j] / binary_D12((M[i], M[j]), (d[i], d[j]), T0, pres)
+                A[i, j] += a
+                A[i, i] -= a
+    K = np.vstack([A, np.asarray(Y)])
+    rng = np.random.default_rng(11)
+    for _ in range(8):
+        dv = rng.standard_normal(NS)
+        dv -= dv.mean()
+        V = np.linalg.lstsq(K, np.concatenate([dv, [0.0]]), rcond=None)[0]
+        assert np.abs(-Diff @ dv - V).max() <= 2e-3 * np.abs(V).max()
+
+
+def test_eight_labels_of_two_gases_leave_transport_unchanged(orc):
+    """He-Ar written as eight species (four labels of each gas) gives the binary eta, kappa and
+    the binary chi on every label."""
+    M, d = HE_AR
+    Y = (0.2, 0.8)
+    eb, kb, cb, Db, Xb, _ = uniform_transport(orc, M, d, Y)
+    f = (0.1, 0.2, 0.3, 0.4)
+    Y8 = tuple(Y[0] * x for x in f) + tuple(Y[1] * x for x in f)
+    e8, k8, c8, D8, X8, _ = uniform_transport(orc, (M[0],) * 4 + (M[1],) * 4, (d[0],) * 4 + (d[1],) * 4, Y8)
+    assert e8 == pytest.approx(eb, rel=1e-13)
+    assert k8 == pytest.approx(kb, rel=1e-13)
+    np.testing.assert_allclose(c8, cb[[0] * 4 + [1] * 4], rtol=1e-12)
