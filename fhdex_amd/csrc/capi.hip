@@ -35,6 +35,7 @@ const char* fhdx_status_string(int s)
         case FHDX_OK: return "ok";
         case FHDX_ERR_ARG: return "invalid argument or array shape";
         case FHDX_ERR_UNSUPPORTED: return "parameters outside the implemented scope";
+        case FHDX_ERR_SINGULAR: return "singular matrix in the Valk-Waldmann transport solves";
         case FHDX_ERR_HIP: return "HIP runtime error";
         case FHDX_ERR_MASSFRAC: return "mass fraction outside (0,1) in the transport step";
         case FHDX_ERR_NOMEM: return "device memory budget exhausted";
@@ -223,8 +224,9 @@ int fhdx_params_check(const fhdx_params* p)
             return FHDX_ERR_UNSUPPORTED;
         }
     }
-    if (p->transport_type != 1 && p->transport_type != 3) {
-        set_error("transport_type must be 1 (Giovangigli) or 3 (Hirschfelder-Curtiss-Bird, binary)");
+    if (p->transport_type < 1 || p->transport_type > 3) {
+        set_error("transport_type must be 1 (Giovangigli), 2 (Valk-Waldmann) or 3 (Hirschfelder-Curtiss-Bird, "
+                  "binary)");
         return FHDX_ERR_UNSUPPORTED;
     }
     if (p->transport_type == 3 && p->nspecies != 2) {
@@ -596,6 +598,10 @@ int fhdx_ctx_check(fhdx_ctx* c, void* stream)
         if (h & 1) {
             set_error("mass fraction outside (0,1) met in the transport step");
             return FHDX_ERR_MASSFRAC;
+        }
+        if (h & 2) {
+            set_error("singular matrix in the Valk-Waldmann transport solves (transport_type 2)");
+            return FHDX_ERR_SINGULAR;
         }
         return FHDX_ERR_NONFINITE;
     }

@@ -260,6 +260,11 @@ int engine_add(Engine* e, const fhdx_params& p, double dt, uint64_t seed, uint64
         set_error("ensemble instances with chemistry (nreaction > 0) are not supported");
         return FHDX_ERR_UNSUPPORTED;
     }
+    if (p.transport_type == 2) {
+        set_error("ensemble instances with Valk-Waldmann transport (transport_type 2) are not supported "
+                  "(fhdx_rk3_step and fhdx_rk3_step_ref run it)");
+        return FHDX_ERR_UNSUPPORTED;
+    }
     if (p.bc_vel_lo[0] != -1 || p.bc_vel_lo[1] != -1 || p.bc_vel_lo[2] != -1) {
         set_error("ensemble instances are periodic boxes (walls: fhdx_rk3_step_ref)");
         return FHDX_ERR_UNSUPPORTED;

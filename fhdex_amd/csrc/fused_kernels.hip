@@ -112,14 +112,14 @@ __global__ void __launch_bounds__(FB) k_noise(DevParams P, NoiseKey K, Grid G, d
 
 // ---- K1: cell records ------------------------------------------------------------------
 
-template <int NS>
+template <int NS, bool VW>
 __global__ void __launch_bounds__(FB) k_cells(DevParams P, Grid G, double* U, double* R, int* err)
 {
     long t = (long)blockIdx.x * FB + threadIdx.x;
     if (t >= (long)P.n[0] * P.n[1] * P.n[2]) return;
     int i, j, k;
     decode3(t, P.n[0], P.n[1], i, j, k);
-    cell_body<NS>(P, G.fld(U), G.fld(R), err, i, j, k, G.off(i, j, k));
+    cell_body<NS, VW>(P, G.fld(U), G.fld(R), err, i, j, k, G.off(i, j, k));
 }
 
 // ---- tiles: 2-D (x, y) blocks of 256 threads, one z level per block, region-major order ----
@@ -299,9 +299,14 @@ static void launch_stage(const DevParams& P, FusedWorkspace& w, const Grid& G, d
     const TileMap tm = make_tiles(P.n[0], P.n[1], P.n[2], w.rh, w.tw_max, w.kz);
     const unsigned ntile = tm.blocks();
     Timing& T = w.timing;
-    if (st == 0 || !w.rec_in_update) {  // else the previous stage's k_update wrote them
+    // transport_type 2 takes its own cell kernel; the update kernels carry the default transport
+    // only, so its records are never written there
+    const bool vw = P.transport_type == 2;
+    const bool rec_in_update = w.rec_in_update && !vw;
+    if (st == 0 || !rec_in_update) {  // else the previous stage's k_update wrote them
         T.begin(KT_CELLS, s);
-        k_cells<NS><<<grid_for(ncell, FB), FB, 0, s>>>(P, G, us, w.rec, d_err);  // records + ghost images
+        if (vw) k_cells<NS, true><<<grid_for(ncell, FB), FB, 0, s>>>(P, G, us, w.rec, d_err);
+        else k_cells<NS, false><<<grid_for(ncell, FB), FB, 0, s>>>(P, G, us, w.rec, d_err);  // records + ghost images
         T.end(s);
     }
     const bool zm = tm.kz > 1;
@@ -334,7 +339,7 @@ static void launch_stage(const DevParams& P, FusedWorkspace& w, const Grid& G, d
     k_update<NS, ZMV, RECV, CHEMV><<<ntile, FB, 0, s>>>(P, tm, G, out, cu, us, w.F[0], w.F[1], w.F[2], dt, st, \
                                                         prim, st == 2, w.rec, d_err, K, wst,                   \
                                                         P.dx[0] * P.dx[1] * P.dx[2])
-    const bool rec_next = w.rec_in_update && st < 2;
+    const bool rec_next = rec_in_update && st < 2;
     const int variant = (zm ? 4 : 0) + (rec_next ? 2 : 0) + (P.chem.nreaction > 0 ? 1 : 0);
     switch (variant) {
         case 0: FHDX_UPD(false, false, false); break;

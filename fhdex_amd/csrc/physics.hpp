@@ -242,6 +242,378 @@ __device__ inline void hcb_binary(const DevParams& P, double density, double tem
     chi[1] = 1.0 * xxtr[1] * thermal_diff / (mR * yytr[1] * yytr[1]);
 }
 
+// D_GIO1 (compressible_functions.H:647-778; D_GIO at :513-646 is the same arithmetic): the
+// three-term series of the multicomponent diffusion matrix, D[j*NS+i] = Diff_ij Ykp_i.
+template <int NS>
+__device__ __forceinline__ void d_gio_series(const double (&Dbin)[NS][NS], const double* Ykp, const double* Xkp,
+                                             double* D)
+{
+    double Di[NS], Minv[NS], Mmat[NS];
+    double Pm[NS][NS], Zm[NS][NS], PJ[NS][NS], m1[NS][NS], m2[NS][NS], Diff[NS][NS];
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        double term2 = 0.;
+#pragma unroll
+        for (int j = 0; j < NS; ++j)
+            if (j != i) term2 = term2 + Xkp[j] / Dbin[i][j];
+        Di[i] = (1. - Ykp[i]) / term2;
+    }
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        Mmat[i] = Xkp[i] / Di[i];
+        Minv[i] = Di[i] / Xkp[i];
+    }
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            Pm[i][j] = -Ykp[j];
+            if (i == j) Pm[i][j] = Pm[i][j] + 1.;
+        }
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            double Delta;
+            if (i == j) {
+                double term1 = 0.;
+#pragma unroll
+                for (int k = 0; k < NS; ++k)
+                    if (k != i) term1 = term1 + Xkp[i] * Xkp[k] / Dbin[i][k];
+                Delta = term1;
+            } else {
+                Delta = -Xkp[i] * Xkp[j] / Dbin[i][j];
+            }
+            Zm[i][j] = -Delta;
+        }
+#pragma unroll
+    for (int i = 0; i < NS; ++i) Zm[i][i] = Zm[i][i] + Mmat[i];
+    // Jmat = Minv_i * Zmat_ij, PJ = P * J
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            double s = 0.;
+#pragma unroll
+            for (int k = 0; k < NS; ++k) s = s + Pm[i][k] * (Minv[k] * Zm[k][j]);
+            PJ[i][j] = s;
+        }
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            double scr = 0.;
+#pragma unroll
+            for (int k = 0; k < NS; ++k) scr = scr + Pm[i][k] * Minv[k] * Pm[j][k];
+            m2[i][j] = scr;
+            Diff[i][j] = scr;
+        }
+#pragma unroll
+    for (int ll = 0; ll < 3; ++ll) {
+#pragma unroll
+        for (int i = 0; i < NS; ++i)
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+                double scr = 0.;
+#pragma unroll
+                for (int k = 0; k < NS; ++k) scr = scr + PJ[i][k] * Diff[k][j];
+                m1[i][j] = scr + m2[i][j];
+            }
+#pragma unroll
+        for (int i = 0; i < NS; ++i)
+#pragma unroll
+            for (int j = 0; j < NS; ++j) Diff[i][j] = m1[i][j];
+    }
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+#pragma unroll
+        for (int j = 0; j < NS; ++j) D[j * NS + i] = Diff[i][j] * Ykp[i];
+}
+
+// ---- transport_type 2: IdealMixtureTransportVW (compressible_functions.H:1047-1150) ----------
+// The reference's pow(a, 2.) is evaluated as a*a (the correctly rounded square libm's pow
+// returns) and pow(a, 3.) as (a*a)*a (within an ulp of libm's), the same sequence as the oracle.
+// Matrices are flat with rows `stride` apart; the pivot rows are chosen at run time, so these
+// arrays live in private memory.  The functions return 1 where the reference aborts with
+// "singular matrix in decomp".
+__device__ inline double vw_sq(double a) { return a * a; }
+__device__ inline double vw_cube(double a) { return (a * a) * a; }
+
+// Decomp / Decomp2 (H:221-320): LU with partial pivoting, 1-based row numbers in ip.
+__device__ inline int vw_decomp(int neq, int stride, double* a, int* ip)
+{
+    for (int n = 0; n < neq; ++n) ip[n] = n + 1;
+    for (int i = 0; i < neq - 1; ++i) {
+        int k = i + 1;
+        double apm = fabs(a[(ip[i] - 1) * stride + i]);
+        for (int j = i + 1; j < neq; ++j) {
+            const double apn = fabs(a[(ip[j] - 1) * stride + i]);
+            if (apm >= apn) continue;
+            apm = apn;
+            k = j + 1;
+        }
+        const int piv = ip[k - 1];
+        ip[k - 1] = ip[i];
+        ip[i] = piv;
+        if (apm < 1.e-30) return 1;
+        for (int l = i + 1; l < neq; ++l) {
+            const int n = ip[l];
+            const double amult = a[(n - 1) * stride + i] / a[(piv - 1) * stride + i];
+            a[(n - 1) * stride + i] = amult;
+            if (fabs(amult) < 1.e-60) continue;
+            for (int c = i + 1; c < neq; ++c)
+                a[(n - 1) * stride + c] = a[(n - 1) * stride + c] - amult * a[(piv - 1) * stride + c];
+        }
+    }
+    return fabs(a[(ip[neq - 1] - 1) * stride + neq - 1]) < 1.e-30;
+}
+
+// Solve / Solve2 (H:322-402); the closing loop undoes the permutation cycle by cycle (each pass
+// fixes one more row, so it ends after at most neq passes for any permutation in ip).
+__device__ inline void vw_solve(int neq, int stride, const double* a, double* b, int* ip)
+{
+    for (int l = 1; l < neq; ++l) {
+        const int n = ip[l];
+        for (int k = 0; k < l; ++k) b[n - 1] = b[n - 1] - a[(n - 1) * stride + k] * b[ip[k] - 1];
+    }
+    b[ip[neq - 1] - 1] = b[ip[neq - 1] - 1] / a[(ip[neq - 1] - 1) * stride + neq - 1];
+    for (int l = 0; l < neq - 1; ++l) {
+        const int j = neq - (l + 1);
+        const int n = ip[j - 1];
+        for (int k = j; k < neq; ++k) b[n - 1] = b[n - 1] - a[(n - 1) * stride + k] * b[ip[k] - 1];
+        b[n - 1] = b[n - 1] / a[(n - 1) * stride + j - 1];
+    }
+    for (int n = 0; n < neq; ++n)
+        while (ip[n] != n + 1) {
+            const int j = ip[n];
+            const double scr = b[j - 1];
+            ip[n] = ip[j - 1];
+            b[j - 1] = b[ip[j - 1] - 1];
+            b[ip[j - 1] - 1] = scr;
+            ip[j - 1] = j;
+        }
+}
+
+// Decompnp / Solvenp (H:404-458): without pivoting.
+template <int N>
+__device__ inline int vw_decomp_np(double* a)
+{
+    for (int i = 0; i < N - 1; ++i)
+        for (int l = i + 1; l < N; ++l) {
+            const double amult = a[l * N + i] / a[i * N + i];
+            a[l * N + i] = amult;
+            if (fabs(amult) < 1.e-60) continue;
+            for (int k = i + 1; k < N; ++k) a[l * N + k] = a[l * N + k] - amult * a[i * N + k];
+        }
+    return fabs(a[(N - 1) * N + N - 1]) < 1.e-30;
+}
+
+template <int N>
+__device__ inline void vw_solve_np(const double* a, double* b)
+{
+    for (int l = 1; l < N; ++l)
+        for (int k = 0; k < l; ++k) b[l] = b[l] - a[l * N + k] * b[k];
+    b[N - 1] = b[N - 1] / a[(N - 1) * N + N - 1];
+    for (int l = 0; l < N - 1; ++l) {
+        const int j = N - (l + 1);
+        for (int k = j; k < N; ++k) b[j - 1] = b[j - 1] - a[(j - 1) * N + k] * b[k];
+        b[j - 1] = b[j - 1] / a[(j - 1) * N + j - 1];
+    }
+}
+
+// ViscLin (H:969-1045): the first Sonine approximation of the viscosity (HCB 7.4-56, 62, 63).
+template <int NS>
+__device__ inline int vw_visc_lin(const double* om, const double* Ykp, double T, double rho, const double* mk,
+                                  double kB, double& eta)
+{
+    double nk[NS], bS[NS], diag[NS], Q[NS * NS];
+    int ip[NS];
+    for (int i = 0; i < NS; ++i) nk[i] = rho * Ykp[i] / mk[i];
+    for (int i = 0; i < NS; ++i) {
+        diag[i] = 0.;
+        for (int k = 0; k < NS; ++k)
+            diag[i] = diag[i] + nk[k] * mk[k] / vw_sq(mk[i] + mk[k]) *
+                                    (5. * mk[i] * om[i * NS + k] + 3. * mk[k] * om[i * NS + k]);
+    }
+    for (int i = 0; i < NS; ++i) {
+        for (int j = 0; j < NS; ++j) {
+            double s1 = -nk[j] * mk[j] / vw_sq(mk[i] + mk[j]) * 2. * mk[j] * om[i * NS + j];
+            if (i == j) s1 = s1 + diag[i];
+            Q[i * NS + j] = -(16. / 15.) * (mk[i] / mk[j]) * s1;
+        }
+        bS[i] = -1.;
+    }
+    if (vw_decomp(NS, NS, Q, ip)) return 1;
+    vw_solve(NS, NS, Q, bS, ip);
+    double s = 0.;
+    for (int i = 0; i < NS; ++i) s = s + nk[i] * bS[i];
+    eta = 0.5 * kB * T * s;
+    return 0;
+}
+
+// LambdaLin (H:782-912): the 2NS Sonine system for conductivity and thermal diffusion (HCB
+// 7.4-50..54), lambda' (7.4-33) and lambda (7.4-65).
+template <int NS>
+__device__ inline int vw_lambda_lin(const double* Dbin, const double* om, const double* Ykp, double T, double rho,
+                                    const double* mk, double kB, double& lam)
+{
+    constexpr int S = 2 * NS;
+    double nk[NS], beta[NS], d1[NS], d2[NS], d3[NS], d4[NS], DT[NS], QQ[S * S], aS[S];
+    int ip[S];
+    for (int i = 0; i < NS; ++i) {
+        nk[i] = rho * Ykp[i] / mk[i];
+        d1[i] = d2[i] = d3[i] = d4[i] = 0.;
+    }
+    for (int i = 0; i < NS; ++i)
+        for (int k = 0; k < NS; ++k) {
+            const double o = om[i * NS + k], s = mk[i] + mk[k];
+            d1[i] = d1[i] + nk[k] * mk[k] / s * nk[i] * mk[i] * o;
+            if (k != i) d2[i] = d2[i] + nk[k] * mk[k] / s * o;
+            d3[i] = d3[i] + nk[i] * nk[k] * vw_sq(mk[k]) / vw_sq(s) * .5 * o;
+            d4[i] = d4[i] + nk[i] * nk[k] * mk[k] / vw_cube(s) *
+                                ((5. / 4. * (6. * vw_sq(mk[i]) + 5. * vw_sq(mk[k])) * o - 3. * vw_sq(mk[k]) * o) +
+                                 4. * mk[i] * mk[k] * o);
+        }
+    for (int i = 0; i < NS; ++i)
+        for (int j = 0; j < NS; ++j) {
+            const double o = om[i * NS + j], s = mk[i] + mk[j];
+            double s00 = -nk[j] * mk[j] * d2[i] - nk[j] * mk[j] / s * nk[i] * mk[i] * o;
+            double s01 = -nk[i] * nk[j] * vw_sq(mk[j]) / vw_sq(s) * .5 * o;
+            // the off-diagonal [11] term as the reference writes it: m_j throughout
+            double s11 = nk[i] * nk[j] * mk[j] / vw_cube(s) *
+                         (-(5. / 4. * (6. * vw_sq(mk[j]) + 5. * vw_sq(mk[j])) * o - 3. * vw_sq(mk[j]) * o) +
+                          4. * mk[j] * mk[j] * o);
+            if (i == j) {
+                s00 = s00 + d1[i];
+                s01 = s01 + d3[i];
+                s11 = s11 + d4[i];
+            }
+            const double ratm = mk[i] / mk[j], sqratm = sqrt(ratm), scr = ratm * sqratm;
+            QQ[i * S + j] = 8. * sqratm / mk[i] * s00;
+            QQ[i * S + j + NS] = -8. * scr * s01;
+            QQ[(i + NS) * S + j] = -8. * sqratm * s01;
+            QQ[(i + NS) * S + j + NS] = 8. * ratm * sqratm * s11;
+        }
+    for (int i = 0; i < NS; ++i) {
+        aS[i] = 0.;
+        beta[i] = sqrt(2. * kB * T / mk[i]);
+        aS[i + NS] = -(15. / 4.) * nk[i] * beta[i];
+    }
+    if (vw_decomp(S, S, QQ, ip)) return 1;
+    vw_solve(S, S, QQ, aS, ip);
+    for (int i = 0; i < NS; ++i) DT[i] = 0.5 * nk[i] * beta[i] * mk[i] * aS[i];
+    double s1 = 0.;
+    for (int i = 0; i < NS; ++i) s1 = s1 + nk[i] * beta[i] * aS[NS + i];
+    const double lp = -5. / 4. * kB * s1;
+    s1 = 0.;
+    for (int i = 0; i < NS; ++i)
+        for (int j = 0; j < NS; ++j)
+            s1 = s1 + nk[i] * nk[j] / Dbin[i * NS + j] * vw_sq(DT[i] / (nk[i] * mk[i]) - DT[j] / (nk[j] * mk[j]));
+    double ntotal = 0.;
+    for (int i = 0; i < NS; ++i) ntotal += nk[i];
+    lam = lp - 0.5 * kB / ntotal * s1;
+    return 0;
+}
+
+// ThermalDiff (H:914-967): Valk's thermal diffusion ratios into chi.
+template <int NS>
+__device__ inline int vw_thermal_diff(const double* a1, const double* a2, const double* ab, const double* Xkp,
+                                      double sqrtT, const double* mk, double* kT)
+{
+    double A[NS * NS], AA[NS], al[NS * NS];
+    for (int i = 0; i < NS; ++i) {
+        for (int j = 0; j < NS; ++j) A[i * NS + j] = Xkp[j] * a2[i * NS + j];
+        AA[i] = 15. / 4.;
+    }
+    for (int i = 0; i < NS; ++i) {
+        double s = 0.;
+        for (int k = 0; k < NS; ++k) s = s + Xkp[k] * a1[i * NS + k];
+        A[i * NS + i] = A[i * NS + i] + s;
+    }
+    if (vw_decomp_np<NS>(A)) return 1;
+    vw_solve_np<NS>(A, AA);
+    for (int i = 0; i < NS; ++i)
+        for (int j = 0; j < NS; ++j) al[i * NS + j] = ab[i * NS + j] * (AA[i] / mk[i] - AA[j] / mk[j]) / sqrtT;
+    for (int k = 0; k < NS; ++k) {
+        double s = 0.;
+        for (int j = 0; j < NS; ++j) s = s + Xkp[j] * al[k * NS + j];
+        kT[k] = s;
+    }
+    return 0;
+}
+
+// IdealMixtureTransportVW (H:1047-1150).  Xk: mole fractions of the clamped mass fractions
+// (transCoeffs.cpp:60-61).  D[j*NS+i] before the density factor.  Returns 1 on a singular matrix.
+template <int NS>
+__device__ inline int vw_transport(const DevParams& P, double density, double T, double pressure, const double* Xk,
+                                   Transport& tr, double* chi, double* D)
+{
+    const double kB = P.k_B, pi = 3.1415926535897932;
+    double mk[NS], xx[NS], yy[NS];
+    double Dflat[NS * NS], om[NS * NS], a1[NS * NS], a2[NS * NS], ab[NS * NS], Dbin[NS][NS];
+    for (int n = 0; n < NS; ++n) mk[n] = P.molmass[n] * (kB / P.Runiv);
+    for (int ii = 0; ii < NS; ++ii) {
+        double sumXk = 0.;
+        for (int jj = 0; jj < NS; ++jj) sumXk += Xk[jj];
+        xx[ii] = Xk[ii] + 1.e-15 * (sumXk / NS - Xk[ii]);
+    }
+    double MWmix = 0.;
+    for (int ii = 0; ii < NS; ++ii) MWmix += xx[ii] * P.molmass[ii];
+    for (int ii = 0; ii < NS; ++ii) yy[ii] = P.molmass[ii] / MWmix * xx[ii];
+    const double Fijstar = -27. / 5.;
+    const double sqrtT = sqrt(T);
+    tr.eta = tr.kappa = 0.;  // what a singular solve leaves (the reference aborts there)
+    for (int n = 0; n < NS; ++n) chi[n] = 0.;
+    for (int i = 0; i < NS; ++i)
+        for (int j = 0; j < NS; ++j) {
+            const double diamat = 0.5 * (P.diameter[i] + P.diameter[j]);
+            const double mi = mk[i], mj = mk[j];
+            const double mu = mi * mj / (mi + mj);
+            const double Fij = (6.0 * mi * mi + 13.0 / 5.0 * mj * mj + 16.0 / 5.0 * mi * mj) / vw_sq(mi + mj);
+            const double s11bar = sqrt(kB / (2.0 * pi * mu)) * pi * vw_sq(diamat);
+            ab[i * NS + j] = 8.0 / (3.0 * kB) * mu * mu * (-.5 * s11bar);
+            Dflat[i * NS + j] = 3.0 / 16.0 * sqrt(2.0 * pi * vw_cube(kB) * (mi + mj) / mi / mj) /
+                                (pi * vw_sq(diamat)) * T * sqrtT / pressure;
+            Dbin[i][j] = Dflat[i * NS + j];
+            om[i * NS + j] = sqrt(pi * kB / (2.0 * mu)) * vw_sq(diamat) * sqrtT;
+            a1[i * NS + j] = (5.0 / (kB) * mi * mj / (mi + mj) * Fij * s11bar) / sqrtT;
+            a2[i * NS + j] = (5.0 / (kB) * mi * mj * mi * mj / (vw_cube(mi + mj)) * Fijstar * s11bar) / sqrtT;
+        }
+    int bad = vw_visc_lin<NS>(om, yy, T, density, mk, kB, tr.eta);
+    tr.zeta = 0.;
+    bad |= vw_lambda_lin<NS>(Dflat, om, yy, T, density, mk, kB, tr.kappa);
+    d_gio_series<NS>(Dbin, yy, xx, D);  // D_GIO (H:513-646)
+    bad |= vw_thermal_diff<NS>(a1, a2, ab, xx, sqrtT, mk, chi);
+    return bad;
+}
+
+// transport_type 2 for one cell (transCoeffs.cpp:37-88 with :66-69).  Returns flags: bit 0 a mass
+// fraction outside (0,1), bit 1 a singular matrix in the Valk-Waldmann solves (the reference
+// aborts on either).
+template <int NS>
+__device__ inline int transport_cell_vw(const DevParams& P, double rho, double T, double p, const double* Yraw,
+                                        Transport& tr, double* chi, double* D)
+{
+    double Yk[NS], Xf[NS];
+    double sumYk = 0.;
+    int flags = 0;
+    for (int n = 0; n < NS; ++n) {
+        const double y = Yraw[n];
+        if ((y <= 0.0) | (y >= 1.0)) flags = 1;
+        Yk[n] = clamp01(y);
+        sumYk += Yk[n];
+    }
+    for (int n = 0; n < NS; ++n) Yk[n] /= sumYk;
+    double molmix = 0.;
+    for (int n = 0; n < NS; ++n) molmix += Yk[n] / P.molmass[n];
+    molmix = 1. / molmix;
+    for (int n = 0; n < NS; ++n) Xf[n] = Yk[n] * molmix / P.molmass[n];
+    if (vw_transport<NS>(P, rho, T, p, Xf, tr, chi, D)) flags |= 2;
+    for (int n = 0; n < NS * NS; ++n) D[n] *= rho;
+    if (P.visc_type == 3 || P.visc_type == -3) tr.zeta = P.zeta_ratio * tr.eta;
+    return flags;
+}
+
 // Transport coefficients of one cell (transCoeffs.cpp:37-88).  chi[NS], D[NS*NS] out.
 // Returns true if a mass fraction lies outside (0,1) (the reference aborts there).
 template <int NS>
@@ -395,89 +767,7 @@ __device__ inline bool transport_cell(const DevParams& P, double rho, double T, 
         chi[i] = c;
     }
 
-    // D_GIO1 (H:647-778)
-    {
-        double Di[NS], Minv[NS], Mmat[NS];
-        double Pm[NS][NS], Zm[NS][NS], PJ[NS][NS], m1[NS][NS], m2[NS][NS], Diff[NS][NS];
-#pragma unroll
-        for (int i = 0; i < NS; ++i) {
-            double term2 = 0.;
-#pragma unroll
-            for (int j = 0; j < NS; ++j)
-                if (j != i) term2 = term2 + Xkp[j] / Dbin[i][j];
-            Di[i] = (1. - Ykp[i]) / term2;
-        }
-#pragma unroll
-        for (int i = 0; i < NS; ++i) {
-            Mmat[i] = Xkp[i] / Di[i];
-            Minv[i] = Di[i] / Xkp[i];
-        }
-#pragma unroll
-        for (int i = 0; i < NS; ++i)
-#pragma unroll
-            for (int j = 0; j < NS; ++j) {
-                Pm[i][j] = -Ykp[j];
-                if (i == j) Pm[i][j] = Pm[i][j] + 1.;
-            }
-#pragma unroll
-        for (int i = 0; i < NS; ++i)
-#pragma unroll
-            for (int j = 0; j < NS; ++j) {
-                double Delta;
-                if (i == j) {
-                    double term1 = 0.;
-#pragma unroll
-                    for (int k = 0; k < NS; ++k)
-                        if (k != i) term1 = term1 + Xkp[i] * Xkp[k] / Dbin[i][k];
-                    Delta = term1;
-                } else {
-                    Delta = -Xkp[i] * Xkp[j] / Dbin[i][j];
-                }
-                Zm[i][j] = -Delta;
-            }
-#pragma unroll
-        for (int i = 0; i < NS; ++i) Zm[i][i] = Zm[i][i] + Mmat[i];
-        // Jmat = Minv_i * Zmat_ij, PJ = P * J
-#pragma unroll
-        for (int i = 0; i < NS; ++i)
-#pragma unroll
-            for (int j = 0; j < NS; ++j) {
-                double s = 0.;
-#pragma unroll
-                for (int k = 0; k < NS; ++k) s = s + Pm[i][k] * (Minv[k] * Zm[k][j]);
-                PJ[i][j] = s;
-            }
-#pragma unroll
-        for (int i = 0; i < NS; ++i)
-#pragma unroll
-            for (int j = 0; j < NS; ++j) {
-                double scr = 0.;
-#pragma unroll
-                for (int k = 0; k < NS; ++k) scr = scr + Pm[i][k] * Minv[k] * Pm[j][k];
-                m2[i][j] = scr;
-                Diff[i][j] = scr;
-            }
-#pragma unroll
-        for (int ll = 0; ll < 3; ++ll) {
-#pragma unroll
-            for (int i = 0; i < NS; ++i)
-#pragma unroll
-                for (int j = 0; j < NS; ++j) {
-                    double scr = 0.;
-#pragma unroll
-                    for (int k = 0; k < NS; ++k) scr = scr + PJ[i][k] * Diff[k][j];
-                    m1[i][j] = scr + m2[i][j];
-                }
-#pragma unroll
-            for (int i = 0; i < NS; ++i)
-#pragma unroll
-                for (int j = 0; j < NS; ++j) Diff[i][j] = m1[i][j];
-        }
-#pragma unroll
-        for (int i = 0; i < NS; ++i)
-#pragma unroll
-            for (int j = 0; j < NS; ++j) D[j * NS + i] = Diff[i][j] * Ykp[i];
-    }
+    d_gio_series<NS>(Dbin, Ykp, Xkp, D);  // D_GIO1 (H:647-778)
 #pragma unroll
     for (int n = 0; n < NS * NS; ++n) D[n] *= rho;
     if (P.visc_type == 3 || P.visc_type == -3) tr.zeta = P.zeta_ratio * tr.eta;

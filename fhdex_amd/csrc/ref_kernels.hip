@@ -104,7 +104,7 @@ __global__ void __launch_bounds__(BLK) k_c2p(DevParams P, Arr prim, Arr cons)
     for (int n = 0; n < 2 * NS + 6; ++n) prim(i, j, k, n) = pr[n];
 }
 
-template <int NS>
+template <int NS, bool VW>
 __global__ void __launch_bounds__(BLK) k_transport(DevParams P, Arr prim, Arr eta, Arr zeta,
                                                    Arr kappa, Arr chi, Arr Dm, int* err)
 {
@@ -121,9 +121,14 @@ __global__ void __launch_bounds__(BLK) k_transport(DevParams P, Arr prim, Arr et
 #pragma unroll
     for (int n = 0; n < NS; ++n) Y[n] = prim(i, j, k, 6 + n);
     Transport tr;
-    bool bad = transport_cell<NS>(P, prim(i, j, k, 0), prim(i, j, k, 4), prim(i, j, k, 5), Y, tr,
-                                  chiv, Dv);
-    if (bad) atomicOr(err, 1);
+    if constexpr (VW) {  // transport_type 2
+        const int fl = transport_cell_vw<NS>(P, prim(i, j, k, 0), prim(i, j, k, 4), prim(i, j, k, 5), Y, tr, chiv, Dv);
+        if (fl) atomicOr(err, fl);
+    } else {
+        bool bad = transport_cell<NS>(P, prim(i, j, k, 0), prim(i, j, k, 4), prim(i, j, k, 5), Y, tr,
+                                      chiv, Dv);
+        if (bad) atomicOr(err, 1);
+    }
     eta(i, j, k) = tr.eta;
     zeta(i, j, k) = tr.zeta;
     kappa(i, j, k) = tr.kappa;
@@ -1068,8 +1073,13 @@ int ref_c2p(const DevParams& P, Arr prim, Arr cons, hipStream_t s)
 int ref_transport(const DevParams& P, Arr prim, Arr eta, Arr zeta, Arr kappa, Arr chi, Arr Dm,
                   int* err, hipStream_t s)
 {
-    NS_DISPATCH(P.ns, (k_transport<NS><<<grid_for(ngrown(P), BLK), BLK, 0, s>>>(
-                          P, prim, eta, zeta, kappa, chi, Dm, err)));
+    if (P.transport_type == 2) {
+        NS_DISPATCH(P.ns, (k_transport<NS, true><<<grid_for(ngrown(P), BLK), BLK, 0, s>>>(
+                              P, prim, eta, zeta, kappa, chi, Dm, err)));
+    } else {
+        NS_DISPATCH(P.ns, (k_transport<NS, false><<<grid_for(ngrown(P), BLK), BLK, 0, s>>>(
+                              P, prim, eta, zeta, kappa, chi, Dm, err)));
+    }
     FHDX_HIP_TRY(hipGetLastError());
     return FHDX_OK;
 }

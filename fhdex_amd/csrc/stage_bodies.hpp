@@ -129,7 +129,8 @@ __device__ inline void ghost_body(const Arr& a, int nc, int g, int n0, int n1, i
 // ---- conservedToPrimitive + transport coefficients into the cell record -------------------
 
 // The record of cell (i,j,k) from its conserved values c[] (held in registers by the caller).
-template <int NS>
+// VW: transport_type 2 (Valk-Waldmann); otherwise the type the parameters name (1 or 3).
+template <int NS, bool VW = false>
 __device__ inline void cell_record(const DevParams& P, const double* c, const Fld& R, int* err, int i, int j, int k,
                                    unsigned cb)
 {
@@ -138,8 +139,13 @@ __device__ inline void cell_record(const DevParams& P, const double* c, const Fl
     cons_to_prim_cell<NS>(P, c, pr);
     Transport tr;
     double chi[NS], D[NS * NS];
-    bool bad = transport_cell<NS>(P, pr[0], pr[4], pr[5], &pr[6], tr, chi, D);
-    if (bad) atomicOr(err, 1);
+    if constexpr (VW) {
+        const int fl = transport_cell_vw<NS>(P, pr[0], pr[4], pr[5], &pr[6], tr, chi, D);
+        if (fl) atomicOr(err, fl);
+    } else {
+        bool bad = transport_cell<NS>(P, pr[0], pr[4], pr[5], &pr[6], tr, chi, D);
+        if (bad) atomicOr(err, 1);
+    }
     // clamped and renormalised mass fractions of the stochastic species flux (flux.cpp:343-365)
     double yn[NS], sumy = 0.;
 #pragma unroll
@@ -169,14 +175,14 @@ __device__ inline void cell_record(const DevParams& P, const double* c, const Fl
     store_cell<8 + 4 * NS + NS * NS>(P, R, [&](int q) { return rec[q]; }, i, j, k, cb);
 }
 
-template <int NS>
+template <int NS, bool VW = false>
 __device__ inline void cell_body(const DevParams& P, const Fld& U, const Fld& R, int* err, int i, int j, int k,
                                  unsigned cb)
 {
     double c[NS + 5];
 #pragma unroll
     for (int n = 0; n < NS + 5; ++n) c[n] = U(cb, n);
-    cell_record<NS>(P, c, R, err, i, j, k, cb);
+    cell_record<NS, VW>(P, c, R, err, i, j, k, cb);
 }
 
 // ---- nodes (flux.cpp:1627-1713) and the stochastic edge terms of each node -----------------

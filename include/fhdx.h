@@ -43,7 +43,9 @@ enum fhdx_status {
                                  there (src_compressible/transCoeffs.cpp:45-46) */
     FHDX_ERR_NOMEM = 5,       /* device memory budget exhausted */
     FHDX_ERR_NONFINITE = 6,   /* non-finite state detected */
-    FHDX_ERR_BUSY = 7         /* serving engine: no free slot for the request */
+    FHDX_ERR_BUSY = 7,        /* serving engine: no free slot for the request */
+    FHDX_ERR_SINGULAR = 8     /* transport_type 2: a singular matrix in the Valk-Waldmann solves
+                                 (the reference aborts, compressible_functions.H:251-253) */
 };
 
 /* Run parameters: the values the reference reads from its AMREX_GPU_MANAGED namespaces
@@ -55,7 +57,8 @@ typedef struct fhdx_params {
     int32_t algorithm_type;    /* 2: multispecies (species fluxes on), else species off      */
     int32_t advection_type;    /* 0: none, 1: interpolate prim, 2: interpolate cons          */
     int32_t visc_type;         /* |3| adds bulk viscosity zeta = zeta_ratio*eta              */
-    int32_t transport_type;    /* 1: Giovangigli, 3: Hirschfelder-Curtiss-Bird (binary)      */
+    int32_t transport_type;    /* 1: Giovangigli, 2: Valk-Waldmann, 3: Hirschfelder-Curtiss-
+                                  Bird (binary)                                              */
     int32_t stoch_stress_form; /* 1: stochastic fluxes on, 0: deterministic                  */
     int32_t nreaction;         /* 0, or the reaction count given to fhdx_ctx_set_chemistry   */
     /* Boundary codes per direction (common_namespace.H, boundary.cpp:172-2001).  bc_vel -1:
@@ -137,8 +140,9 @@ int fhdx_ctx_check(fhdx_ctx* ctx, void* stream);
 /* conservedToPrimitive (src_compressible/conservedPrimitiveConversions.cpp:4-92): valid cells. */
 int fhdx_cons_to_prim(fhdx_ctx* ctx, fhdx_fab prim, fhdx_fab cons, void* stream);
 
-/* calculateTransportCoeffs (src_compressible/transCoeffs.cpp:7-90), transport_type 1, on the
- * box grown by ngc.  A mass fraction outside (0,1) raises FHDX_ERR_MASSFRAC (see ctx_check). */
+/* calculateTransportCoeffs (src_compressible/transCoeffs.cpp:7-90), transport_type 1, 2 or 3, on
+ * the box grown by ngc.  A mass fraction outside (0,1) raises FHDX_ERR_MASSFRAC, a singular
+ * matrix in the type-2 solves FHDX_ERR_SINGULAR (see ctx_check). */
 int fhdx_transport(fhdx_ctx* ctx, fhdx_fab prim, fhdx_fab eta, fhdx_fab zeta, fhdx_fab kappa,
                    fhdx_fab chi, fhdx_fab D, void* stream);
 

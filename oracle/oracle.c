@@ -14,7 +14,8 @@
  * Functions and the reference code they follow:
  *   cons_to_prim     conservedPrimitiveConversions.cpp:48-89, compressible_functions.H:138-199
  *   transport        transCoeffs.cpp:37-88, compressible_functions.H:1153-1352 (GIO),
- *                    :647-778 (D_GIO1), :1357-1478 (HCB binary, transport_type 3)
+ *                    :647-778 (D_GIO1), :1357-1478 (HCB binary, transport_type 3),
+ *                    :221-458 and :782-1150 (Valk-Waldmann, transport_type 2)
  *   stoch_face       flux.cpp:71-1145 (+ CholeskyDecomp compressible_functions.H:460-509)
  *   diff_face        flux.cpp:1193-1623
  *   corner_node      flux.cpp:1627-1713
@@ -251,6 +252,262 @@ static void hcb_bin(const fhdx_params* P, double density, double temperature, co
     chitil[1] = 1.0 * xxtr[1] * thermal_diff / (mR * yytr[1] * yytr[1]);
 }
 
+/* ------------------------------------------------------------------------------------- */
+/* transport_type 2: IdealMixtureTransportVW (compressible_functions.H:1047-1150)           */
+/* ------------------------------------------------------------------------------------- */
+/* The reference's pow(a, 2.) is evaluated as a*a (the correctly rounded square, which libm's pow
+ * returns) and pow(a, 3.) as (a*a)*a (within an ulp of libm's).  Matrices are flat with rows
+ * `stride` apart, as the reference keeps them. */
+static inline double vw_sq(double a) { return a * a; }
+static inline double vw_cube(double a) { return (a * a) * a; }
+
+/* Decomp and Decomp2 (H:221-320): LU factorisation with partial pivoting; ip holds 1-based row
+ * numbers.  Returns 1 where the reference aborts ("singular matrix in decomp"). */
+static int vw_decomp(int neq, int stride, double* a, int* ip)
+{
+    for (int n = 0; n < neq; ++n) ip[n] = n + 1;
+    for (int i = 0; i < neq - 1; ++i) {
+        int k = i + 1;
+        double apm = fabs(a[(ip[i] - 1) * stride + i]);
+        for (int j = i + 1; j < neq; ++j) {
+            const double apn = fabs(a[(ip[j] - 1) * stride + i]);
+            if (apm >= apn) continue;
+            apm = apn;
+            k = j + 1;
+        }
+        const int piv = ip[k - 1];
+        ip[k - 1] = ip[i];
+        ip[i] = piv;
+        if (apm < 1.e-30) return 1;
+        for (int l = i + 1; l < neq; ++l) {
+            const int n = ip[l];
+            const double amult = a[(n - 1) * stride + i] / a[(piv - 1) * stride + i];
+            a[(n - 1) * stride + i] = amult;
+            if (fabs(amult) < 1.e-60) continue;
+            for (int c = i + 1; c < neq; ++c)
+                a[(n - 1) * stride + c] = a[(n - 1) * stride + c] - amult * a[(piv - 1) * stride + c];
+        }
+    }
+    return fabs(a[(ip[neq - 1] - 1) * stride + neq - 1]) < 1.e-30;
+}
+
+/* Solve and Solve2 (H:322-402): forward and back substitution through the permutation, then the
+ * permutation undone in place cycle by cycle (ip is reset to the identity). */
+static void vw_solve(int neq, int stride, const double* a, double* b, int* ip)
+{
+    for (int l = 1; l < neq; ++l) {
+        const int n = ip[l];
+        for (int k = 0; k < l; ++k) b[n - 1] = b[n - 1] - a[(n - 1) * stride + k] * b[ip[k] - 1];
+    }
+    b[ip[neq - 1] - 1] = b[ip[neq - 1] - 1] / a[(ip[neq - 1] - 1) * stride + neq - 1];
+    for (int l = 0; l < neq - 1; ++l) {
+        const int j = neq - (l + 1);
+        const int n = ip[j - 1];
+        for (int k = j; k < neq; ++k) b[n - 1] = b[n - 1] - a[(n - 1) * stride + k] * b[ip[k] - 1];
+        b[n - 1] = b[n - 1] / a[(n - 1) * stride + j - 1];
+    }
+    for (int n = 0; n < neq; ++n)
+        while (ip[n] != n + 1) {
+            const int j = ip[n];
+            const double scr = b[j - 1];
+            ip[n] = ip[j - 1];
+            b[j - 1] = b[ip[j - 1] - 1];
+            b[ip[j - 1] - 1] = scr;
+            ip[j - 1] = j;
+        }
+}
+
+/* Decompnp / Solvenp (H:404-458): the same without pivoting. */
+static int vw_decomp_np(int neq, int ndim, double* a)
+{
+    for (int i = 0; i < neq - 1; ++i)
+        for (int l = i + 1; l < neq; ++l) {
+            const double amult = a[l * ndim + i] / a[i * ndim + i];
+            a[l * ndim + i] = amult;
+            if (fabs(amult) < 1.e-60) continue;
+            for (int k = i + 1; k < neq; ++k) a[l * ndim + k] = a[l * ndim + k] - amult * a[i * ndim + k];
+        }
+    return fabs(a[(neq - 1) * ndim + neq - 1]) < 1.e-30;
+}
+
+static void vw_solve_np(int neq, int ndim, const double* a, double* b)
+{
+    for (int l = 1; l < neq; ++l)
+        for (int k = 0; k < l; ++k) b[l] = b[l] - a[l * ndim + k] * b[k];
+    b[neq - 1] = b[neq - 1] / a[(neq - 1) * ndim + neq - 1];
+    for (int l = 0; l < neq - 1; ++l) {
+        const int j = neq - (l + 1);
+        for (int k = j; k < neq; ++k) b[j - 1] = b[j - 1] - a[(j - 1) * ndim + k] * b[k];
+        b[j - 1] = b[j - 1] / a[(j - 1) * ndim + j - 1];
+    }
+}
+
+/* ViscLin (H:969-1045): the first Sonine approximation of the mixture viscosity, HCB 7.4-56,
+ * 62, 63. */
+static int vw_visc_lin(int NS, const double* om, const double* Ykp, double T, double rho, const double* mk,
+                       double kB, double* eta)
+{
+    double nk[MS], bS[MS], diag[MS], Q[MS * MS];
+    int ip[MS];
+    for (int i = 0; i < NS; ++i) nk[i] = rho * Ykp[i] / mk[i];
+    for (int i = 0; i < NS; ++i) {
+        diag[i] = 0.;
+        for (int k = 0; k < NS; ++k)
+            diag[i] = diag[i] + nk[k] * mk[k] / vw_sq(mk[i] + mk[k]) *
+                                    (5. * mk[i] * om[i * NS + k] + 3. * mk[k] * om[i * NS + k]);
+    }
+    for (int i = 0; i < NS; ++i) {
+        for (int j = 0; j < NS; ++j) {
+            double s1 = -nk[j] * mk[j] / vw_sq(mk[i] + mk[j]) * 2. * mk[j] * om[i * NS + j];
+            if (i == j) s1 = s1 + diag[i];
+            Q[i * NS + j] = -(16. / 15.) * (mk[i] / mk[j]) * s1;
+        }
+        bS[i] = -1.;
+    }
+    if (vw_decomp(NS, NS, Q, ip)) return 1;
+    vw_solve(NS, NS, Q, bS, ip);
+    double s = 0.;
+    for (int i = 0; i < NS; ++i) s = s + nk[i] * bS[i];
+    *eta = 0.5 * kB * T * s;
+    return 0;
+}
+
+/* LambdaLin (H:782-912): the 2NS-term Sonine system for the thermal conductivity and the
+ * thermal diffusion coefficients (HCB 7.4-50..54), lambda' (7.4-33) and lambda (7.4-65). */
+static int vw_lambda_lin(int NS, const double* Dbin, const double* om, const double* Ykp, double T, double rho,
+                         const double* mk, double kB, double* lam)
+{
+    const int S = 2 * NS;
+    double nk[MS], beta[MS], d1[MS], d2[MS], d3[MS], d4[MS], DT[MS], QQ[4 * MS * MS], aS[2 * MS];
+    int ip[2 * MS];
+    for (int i = 0; i < NS; ++i) {
+        nk[i] = rho * Ykp[i] / mk[i];
+        d1[i] = d2[i] = d3[i] = d4[i] = 0.;
+    }
+    for (int i = 0; i < NS; ++i)
+        for (int k = 0; k < NS; ++k) {
+            const double o = om[i * NS + k], s = mk[i] + mk[k];
+            d1[i] = d1[i] + nk[k] * mk[k] / s * nk[i] * mk[i] * o;
+            if (k != i) d2[i] = d2[i] + nk[k] * mk[k] / s * o;
+            d3[i] = d3[i] + nk[i] * nk[k] * vw_sq(mk[k]) / vw_sq(s) * .5 * o;
+            d4[i] = d4[i] + nk[i] * nk[k] * mk[k] / vw_cube(s) *
+                                ((5. / 4. * (6. * vw_sq(mk[i]) + 5. * vw_sq(mk[k])) * o - 3. * vw_sq(mk[k]) * o) +
+                                 4. * mk[i] * mk[k] * o);
+        }
+    for (int i = 0; i < NS; ++i)
+        for (int j = 0; j < NS; ++j) {
+            const double o = om[i * NS + j], s = mk[i] + mk[j];
+            double s00 = -nk[j] * mk[j] * d2[i] - nk[j] * mk[j] / s * nk[i] * mk[i] * o;
+            double s01 = -nk[i] * nk[j] * vw_sq(mk[j]) / vw_sq(s) * .5 * o;
+            /* the off-diagonal [11] term as the reference writes it: m_j throughout */
+            double s11 = nk[i] * nk[j] * mk[j] / vw_cube(s) *
+                         (-(5. / 4. * (6. * vw_sq(mk[j]) + 5. * vw_sq(mk[j])) * o - 3. * vw_sq(mk[j]) * o) +
+                          4. * mk[j] * mk[j] * o);
+            if (i == j) {
+                s00 = s00 + d1[i];
+                s01 = s01 + d3[i];
+                s11 = s11 + d4[i];
+            }
+            const double ratm = mk[i] / mk[j], sqratm = sqrt(ratm), scr = ratm * sqratm;
+            QQ[i * S + j] = 8. * sqratm / mk[i] * s00;
+            QQ[i * S + j + NS] = -8. * scr * s01;
+            QQ[(i + NS) * S + j] = -8. * sqratm * s01;
+            QQ[(i + NS) * S + j + NS] = 8. * ratm * sqratm * s11;
+        }
+    for (int i = 0; i < NS; ++i) {
+        aS[i] = 0.;
+        beta[i] = sqrt(2. * kB * T / mk[i]);
+        aS[i + NS] = -(15. / 4.) * nk[i] * beta[i];
+    }
+    if (vw_decomp(S, S, QQ, ip)) return 1;
+    vw_solve(S, S, QQ, aS, ip);
+    for (int i = 0; i < NS; ++i) DT[i] = 0.5 * nk[i] * beta[i] * mk[i] * aS[i];
+    double s1 = 0.;
+    for (int i = 0; i < NS; ++i) s1 = s1 + nk[i] * beta[i] * aS[NS + i];
+    const double lp = -5. / 4. * kB * s1;
+    s1 = 0.;
+    for (int i = 0; i < NS; ++i)
+        for (int j = 0; j < NS; ++j)
+            s1 = s1 + nk[i] * nk[j] / Dbin[i * NS + j] * vw_sq(DT[i] / (nk[i] * mk[i]) - DT[j] / (nk[j] * mk[j]));
+    double ntotal = 0.;
+    for (int i = 0; i < NS; ++i) ntotal += nk[i];
+    *lam = lp - 0.5 * kB / ntotal * s1;
+    return 0;
+}
+
+/* ThermalDiff (H:914-967): Valk's thermal diffusion ratios (kT into chitil). */
+static int vw_thermal_diff(int NS, const double* a1, const double* a2, const double* ab, const double* Xkp,
+                           double sqrtT, const double* mk, double* kT)
+{
+    double A[MS * MS], AA[MS], al[MS * MS];
+    for (int i = 0; i < NS; ++i) {
+        for (int j = 0; j < NS; ++j) A[i * NS + j] = Xkp[j] * a2[i * NS + j];
+        AA[i] = 15. / 4.;
+    }
+    for (int i = 0; i < NS; ++i) {
+        double s = 0.;
+        for (int k = 0; k < NS; ++k) s = s + Xkp[k] * a1[i * NS + k];
+        A[i * NS + i] = A[i * NS + i] + s;
+    }
+    if (vw_decomp_np(NS, NS, A)) return 1;
+    vw_solve_np(NS, NS, A, AA);
+    for (int i = 0; i < NS; ++i)
+        for (int j = 0; j < NS; ++j) al[i * NS + j] = ab[i * NS + j] * (AA[i] / mk[i] - AA[j] / mk[j]) / sqrtT;
+    for (int k = 0; k < NS; ++k) {
+        double s = 0.;
+        for (int j = 0; j < NS; ++j) s = s + Xkp[j] * al[k * NS + j];
+        kT[k] = s;
+    }
+    return 0;
+}
+
+/* IdealMixtureTransportVW (H:1047-1150): Xk from GetMolfrac of the clamped mass fractions
+ * (transCoeffs.cpp:60-61); eta, kappa, zeta = 0, diff_ij (out[j*NS+i] before the density factor)
+ * and chitil.  Returns 1 where one of the linear solves meets a singular matrix (the reference
+ * aborts there). */
+static int vw_transport(const fhdx_params* P, double density, double T, double pressure, const double* Xk,
+                        double* eta, double* kappa, double* zeta, double* diff, double* chi)
+{
+    const int NS = P->nspecies;
+    const double kB = P->k_B, pi = 3.1415926535897932;
+    double mk[MS], xx[MS], yy[MS];
+    double Dflat[MS * MS], om[MS * MS], a1[MS * MS], a2[MS * MS], ab[MS * MS], Dbin[MS][MS];
+    for (int n = 0; n < NS; ++n) mk[n] = P->molmass[n] * (kB / P->Runiv);
+    for (int ii = 0; ii < NS; ++ii) {
+        double sumXk = 0.;
+        for (int jj = 0; jj < NS; ++jj) sumXk += Xk[jj];
+        xx[ii] = Xk[ii] + 1.e-15 * (sumXk / NS - Xk[ii]);
+    }
+    double MWmix = 0.;
+    for (int ii = 0; ii < NS; ++ii) MWmix += xx[ii] * P->molmass[ii];
+    for (int ii = 0; ii < NS; ++ii) yy[ii] = P->molmass[ii] / MWmix * xx[ii];
+    const double Fijstar = -27. / 5.;
+    const double sqrtT = sqrt(T);
+    *eta = *kappa = 0.; /* what a singular solve leaves (the reference aborts there) */
+    for (int n = 0; n < NS; ++n) chi[n] = 0.;
+    for (int i = 0; i < NS; ++i)
+        for (int j = 0; j < NS; ++j) {
+            const double diamat = 0.5 * (P->diameter[i] + P->diameter[j]);
+            const double mi = mk[i], mj = mk[j];
+            const double mu = mi * mj / (mi + mj);
+            const double Fij = (6.0 * mi * mi + 13.0 / 5.0 * mj * mj + 16.0 / 5.0 * mi * mj) / vw_sq(mi + mj);
+            const double s11bar = sqrt(kB / (2.0 * pi * mu)) * pi * vw_sq(diamat);
+            ab[i * NS + j] = 8.0 / (3.0 * kB) * mu * mu * (-.5 * s11bar);
+            Dflat[i * NS + j] = 3.0 / 16.0 * sqrt(2.0 * pi * vw_cube(kB) * (mi + mj) / mi / mj) /
+                                (pi * vw_sq(diamat)) * T * sqrtT / pressure;
+            Dbin[i][j] = Dflat[i * NS + j];
+            om[i * NS + j] = sqrt(pi * kB / (2.0 * mu)) * vw_sq(diamat) * sqrtT;
+            a1[i * NS + j] = (5.0 / (kB) * mi * mj / (mi + mj) * Fij * s11bar) / sqrtT;
+            a2[i * NS + j] = (5.0 / (kB) * mi * mj * mi * mj / (vw_cube(mi + mj)) * Fijstar * s11bar) / sqrtT;
+        }
+    int bad = vw_visc_lin(NS, om, yy, T, density, mk, kB, eta);
+    *zeta = 0.;
+    bad |= vw_lambda_lin(NS, Dflat, om, yy, T, density, mk, kB, kappa);
+    d_gio1(NS, Dbin, yy, xx, diff); /* D_GIO (H:513-646) is D_GIO1's arithmetic on a flat Dbin */
+    bad |= vw_thermal_diff(NS, a1, a2, ab, xx, sqrtT, mk, chi);
+    return bad;
+}
+
 static int transport_cell(const fhdx_params* P, const double* pr, double* eta, double* zeta,
                           double* kappa, double* chi, double* D)
 {
@@ -272,6 +529,14 @@ static int transport_cell(const fhdx_params* P, const double* pr, double* eta, d
         get_molfrac(P, Yk, Xf);
         hcb_bin(P, pr[0], pr[4], Xf, eta, kappa, zeta, D, chi);
         for (int n = 0; n < 4; ++n) D[n] *= pr[0];
+        if (abs(P->visc_type) == 3) *zeta = P->zeta_ratio * *eta;
+        return bad;
+    }
+    if (P->transport_type == 2) { /* Valk-Waldmann (transCoeffs.cpp:66-69) */
+        double Xf[MS];
+        get_molfrac(P, Yk, Xf);
+        if (vw_transport(P, pr[0], pr[4], pr[5], Xf, eta, kappa, zeta, D, chi)) bad |= 2;
+        for (int n = 0; n < NS * NS; ++n) D[n] *= pr[0];
         if (abs(P->visc_type) == 3) *zeta = P->zeta_ratio * *eta;
         return bad;
     }
@@ -402,7 +667,7 @@ int orc_transport(const fhdx_params* P, fhdx_fab prim_, fhdx_fab eta_, fhdx_fab 
                 for (int n = 0; n < NS; ++n) AT(chi, i, j, k, n) = c[n];
                 for (int n = 0; n < NS * NS; ++n) AT(D, i, j, k, n) = d[n];
             }
-    return bad ? FHDX_ERR_MASSFRAC : FHDX_OK;
+    return (bad & 1) ? FHDX_ERR_MASSFRAC : (bad & 2) ? FHDX_ERR_SINGULAR : FHDX_OK;
 }
 
 /* ------------------------------------------------------------------------------------- */

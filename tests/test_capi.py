@@ -105,7 +105,11 @@ def test_params_default_finalize_check():
     assert lib.fhdx_params_check(C.byref(t)) == 2
     assert b"binary" in lib.fhdx_last_error()
     t = p.copy()
+    t.transport_type = 2                                   # Valk-Waldmann: any species count
+    assert lib.fhdx_params_check(C.byref(t)) == 0
     t.transport_type = 4
+    assert lib.fhdx_params_check(C.byref(t)) == 2
+    t.transport_type = 0
     assert lib.fhdx_params_check(C.byref(t)) == 2
     q = p.copy()
     q.bc_vel_lo[0] = 1

@@ -73,9 +73,9 @@ def _same(name, got, want):
 CASES = [("argon", dict()), ("four", dict()), ("argon", dict(advection_type=0, visc_type=3, zeta_ratio=1.5)),
          ("four", dict(advection_type=1)), ("four", dict(grav=(0.0, -3.0e9, 1.0e9), advection_type=0)),
          ("argon", dict(stoch_stress_form=0)), ("heAr", dict(transport_type=3)), ("eight", dict()),
-         ("eight", dict(advection_type=0))]
+         ("eight", dict(advection_type=0)), ("heAr", dict(transport_type=2)), ("four", dict(transport_type=2))]
 IDS = ["argon", "four", "argon-adv0-visc3", "four-adv1", "four-gravity-adv0", "argon-deterministic",
-       "heAr-hcb-transport", "eight", "eight-adv0"]
+       "heAr-hcb-transport", "eight", "eight-adv0", "heAr-vw-transport", "four-vw-transport"]
 
 
 @pytest.mark.parametrize("gas", ["argon", "four", "eight"])

@@ -173,3 +173,83 @@ def test_hcb_binary_transport_one_step(orc):
     assert orc.rk3_step(p, st, cfg.dt) == 0
     check("fused cu", valid(p, cu.t.cpu().numpy()), valid(p, st.cu))
     check("reference-shaped cu", valid(p, ds.cu.t.cpu().numpy()), valid(p, st.cu))
+
+
+def _vw_config(ns):
+    if ns == 2:
+        base = P.species_prefix(2, 10, 1)
+        p = P.make_params(2, (10, 9, 8), 8e-6, [4.0026, 39.948], [2.18e-8, 3.63e-8])
+        rhobar = [0.3, 0.7]
+    else:
+        base = P.species_prefix(ns, 0, 1, shape=(8, 7, 6))
+        p = base.params
+        rhobar = base.rhobar
+    p.transport_type = 2
+    return P.Config(f"vw{ns}", p, base.rho0, rhobar, base.T_init, base.dt, 1)
+
+
+@pytest.mark.parametrize("ns", [2, 4, 8])
+def test_vw_transport_one_step(orc, ns):
+    """transport_type 2 (Valk-Waldmann, compressible_functions.H:1047-1150; pivoted LU solves in
+    private memory): the transport arrays of fhdx_transport, one fused step and one
+    reference-shaped step with the same injected noise, against the oracle."""
+    import torch
+    from fhdex_amd import compressible as fc
+    from fhdex_amd import lib as L
+    cfg = _vw_config(ns)
+    p = cfg.params
+    g = p.ngc
+    say(f"case Valk-Waldmann transport NS={ns} {tuple(p.n_cells)}")
+    st = perturbed_state(p, cfg, orc, seed=23, amp=0.02, vel=400.0)
+    ctx = L.Context(p)
+    ds = fc.DeviceState(p)
+    ds.cu.t.copy_(torch.from_numpy(st.cu))
+    ds.prim.t.copy_(torch.from_numpy(st.prim))
+    fc.calculateTransportCoeffs(ctx, ds.prim, ds.eta, ds.zeta, ds.kappa, ds.chi, ds.D)
+    torch.cuda.synchronize()
+    assert orc.transport(p, st) == 0
+    for name in ("eta", "zeta", "kappa", "chi", "D"):
+        check(name, getattr(ds, name).t.cpu().numpy(), getattr(st, name))
+    orc.fill_noise(p, st, 5, 2)
+    cu = fc.MultiFab(torch.from_numpy(st.cu.copy()).cuda(), (-g,) * 3)
+    prim = fc.MultiFab.cell(p, p.nprimvars)
+    ns_ = fc.NoiseSets(p)
+    for d in range(3):
+        ns_.A[d].t.copy_(torch.from_numpy(st.sA[d]))
+        ns_.B[d].t.copy_(torch.from_numpy(st.sB[d]))
+    ns_.rA.t.copy_(torch.from_numpy(st.rA))
+    ns_.rB.t.copy_(torch.from_numpy(st.rB))
+    fc.rk3_step_injected(ctx, cu, prim, ns_, cfg.dt)
+    fc.RK3step(ctx, ds, cfg.dt, noise=ns_)
+    ctx.check()
+    assert orc.rk3_step(p, st, cfg.dt) == 0
+    check("fused cu", valid(p, cu.t.cpu().numpy()), valid(p, st.cu))
+    check("reference-shaped cu", valid(p, ds.cu.t.cpu().numpy()), valid(p, st.cu))
+
+
+def test_vw_singular_matrix_raises():
+    """A cell at zero temperature zeroes every collision integral, so the pivoted factorisation
+    meets a pivot below 1e-30 (the reference aborts); fhdx_ctx_check raises FHDX_ERR_SINGULAR.
+    Every loop of the solves is bounded, and the solve itself is skipped once the flag is set."""
+    import torch
+    from fhdex_amd import compressible as fc
+    from fhdex_amd import lib as L
+    cfg = _vw_config(2)
+    p = cfg.params
+    say("case Valk-Waldmann singular matrix flag")
+    ctx = L.Context(p)
+    ds = fc.DeviceState(p)
+    pr = ds.prim.t
+    pr[0] = 4.83e-4
+    pr[4] = 300.0
+    pr[5] = 1e5
+    pr[6] = 0.3
+    pr[7] = 0.7
+    pr[8] = 0.8
+    pr[9] = 0.2
+    pr[4, 3, 3, 3] = 0.0
+    fc.calculateTransportCoeffs(ctx, ds.prim, ds.eta, ds.zeta, ds.kappa, ds.chi, ds.D)
+    with pytest.raises(L.FhdxError) as e:
+        ctx.check()
+    assert e.value.status == 8
+    ctx.check()  # the flag is cleared once reported

@@ -433,13 +433,14 @@ def test_gravity_accelerates_a_uniform_gas(orc):
         assert dE == pytest.approx(0.5 * rho * gg * t * t, rel=1e-8)
 
 
-@pytest.mark.parametrize("alg,tt", [(2, 1), (1, 1), (2, 3)])
+@pytest.mark.parametrize("alg,tt", [(2, 1), (1, 1), (2, 3), (2, 2)])
 def test_label_diffusion_of_identical_species(orc, alg, tt):
     """Two identical species at rest with a sinusoidal mass fraction: the mixture is uniform, so
     only the labels move.  With species fluxes (algorithm_type 2) the label profile decays at
     D_12 k_eff^2 (D_12 the hard-sphere binary diffusivity, k_eff^2 = (2 - 2 cos kh)/h^2), times
-    1.133 with the Hirschfelder-Curtiss-Bird forms (transport_type 3); with algorithm_type 1 (no
-    species fluxes) it stays put."""
+    1.133 with the Hirschfelder-Curtiss-Bird forms (transport_type 3), and at D_12 k_eff^2 again
+    with the Valk-Waldmann forms (transport_type 2); with algorithm_type 1 (no species fluxes) it
+    stays put."""
     cfg = P.argon_gas(4)
     n = 32
     p = P.make_params(2, (4, n, 4), 8e-6, [39.95, 39.95], [3.66e-8, 3.66e-8], stoch_stress_form=0,

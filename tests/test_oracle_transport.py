@@ -375,3 +375,112 @@ def test_eight_labels_of_two_gases_leave_transport_unchanged(orc):
     assert e8 == pytest.approx(eb, rel=1e-13)
     assert k8 == pytest.approx(kb, rel=1e-13)
     np.testing.assert_allclose(c8, cb[[0] * 4 + [1] * 4], rtol=1e-12)
+
+
+# ---- transport_type 2: the reference's Valk-Waldmann forms (compressible_functions.H:1047-1150) ---
+#
+# ViscLin solves the first Sonine approximation of the viscosity (HCB 7.4-56, 62, 63), the same
+# approximation as the HCB determinant form, so eta agrees to rounding for any mixture.  ThermalDiff
+# (Valk) gives X_1 chi_1 = k_T of HCB eq. 8.2-50 for a binary gas.  LambdaLin's conductivity
+# (two Sonine terms with the thermal-diffusion correction of HCB 7.4-65) reduces to the pure-gas
+# value for identical species and stays within 3e-3 of HCB's first approximation [lambda]_1 for
+# distinct masses (2.4e-3 at most here, always below); a sign or coefficient slip in its Q
+# matrix, its rhs or the correction moves it by 6e-3 to O(1).  Relabelling the species permutes
+# every output.  D is D_GIO's series on the same binary diffusion
+# coefficients (HCB eq. 8.2-44), exact for two species.
+
+VW_PAIRS = PAIRS + [AR_AR]
+VW_IDS = ["He-Ar", "Ne-Kr", "Ar-He", "Ar-Ar"]
+
+
+@pytest.mark.parametrize("gas", VW_PAIRS, ids=VW_IDS)
+@pytest.mark.parametrize("Y", BINARY_Y)
+def test_vw_binary_matches_hcb(orc, gas, Y):
+    M, d = gas
+    eta, kap, chi, D, X, pres = uniform_transport(orc, M, d, Y, transport_type=2)
+    he, hl, hk = hcb_binary(M, d, X, T0)
+    assert eta == pytest.approx(he, rel=1e-13)
+    assert kap == pytest.approx(hl, rel=3e-3)
+    if M[0] == M[1]:
+        assert kap == pytest.approx(hl, rel=1e-13)
+        assert np.abs(chi).max() <= 1e-14
+    else:
+        assert X[0] * chi[0] == pytest.approx(hk, rel=1e-12)
+    assert abs(np.dot(X, chi)) <= 1e-13 * max(np.abs(chi).max(), 1.0)
+    # Fick coefficient of the binary series: rho D12 M1 M2 / Mbar^2 (the series is exact for NS = 2)
+    Dm = D.reshape(2, 2)  # [l, k]
+    Mbar = 1.0 / (Y[0] / M[0] + Y[1] / M[1])
+    want = RHO0 * binary_D12(M, d, T0, pres) * M[0] * M[1] / Mbar ** 2
+    assert Dm[0, 0] - Dm[1, 0] == pytest.approx(want, rel=1e-10)
+
+
+@pytest.mark.parametrize("NS,kind", [(4, "equal"), (4, "dirichlet"), (8, "equal"), (8, "dirichlet"),
+                                     (8, "trace")])
+def test_vw_many_species(orc, NS, kind):
+    """Four and eight distinct species: eta equals the HCB determinant form, sum_i X_i chi_i = 0, and
+    the diffusion coefficients solve the Stefan-Maxwell relations to the series truncation."""
+    M, d = P.EIGHT_SPECIES_M[:NS], P.EIGHT_SPECIES_D[:NS]
+    Y = _wide_Y(NS, kind)
+    eta, kap, chi, D, X, pres = uniform_transport(orc, M, d, Y, transport_type=2)
+    he, hl = hcb_matrices(M, d, X, T0)
+    assert eta == pytest.approx(he, rel=1e-13)
+    assert kap == pytest.approx(hl, rel=3e-3)
+    assert abs(np.dot(X, chi)) <= 1e-13 * np.abs(chi).max()
+    Diff = D.reshape(NS, NS).T / (RHO0 * np.asarray(Y)[:, None])
+    A = np.zeros((NS, NS))
+    for i in range(NS):
+        for j in range(NS):
+            if i != j:
+                a = X[i] * X[j] / binary_D12((M[i], M[j]), (d[i], d[j]), T0, pres)
+                A[i, j] += a
+                A[i, i] -= a
+    K = np.vstack([A, np.asarray(Y)])
+    rng = np.random.default_rng(5)
+    for _ in range(6):
+        dv = rng.standard_normal(NS)
+        dv -= dv.mean()
+        V = np.linalg.lstsq(K, np.concatenate([dv, [0.0]]), rcond=None)[0]
+        assert np.abs(-Diff @ dv - V).max() <= 3e-3 * np.abs(V).max()
+
+
+@pytest.mark.parametrize("host", range(4))
+def test_vw_trace_species_leave_host_closed_forms(orc, host):
+    """A host gas with three trace species (Y = 1e-9): eta and kappa are the host's pure-gas
+    values to O(1e-7)."""
+    M, d = C3
+    m = [Mi * KB / RU for Mi in M]
+    Y = [1e-9] * 4
+    Y[host] = 1.0 - 3e-9
+    eta, kap, chi, D, X, pres = uniform_transport(orc, M, d, Y, transport_type=2)
+    assert eta == pytest.approx(pure_eta(m[host], d[host], T0), rel=1e-7)
+    assert kap == pytest.approx(pure_lambda(m[host], d[host], T0), rel=1e-6)
+
+
+def test_vw_singular_matrix_is_reported(orc):
+    """A zero temperature makes every collision integral zero: the pivoted factorisation meets a
+    pivot below 1e-30, where the reference aborts ("singular matrix in decomp"); the oracle
+    reports FHDX_ERR_SINGULAR."""
+    M, d = HE_AR
+    p = P.make_params(2, (4, 4, 4), 8e-6, M, d)
+    p.transport_type = 2
+    st = orc.HostState(p)
+    st.prim[0], st.prim[4], st.prim[5] = RHO0, 0.0, 1e5
+    st.prim[6], st.prim[7] = 0.5, 0.5
+    st.prim[8], st.prim[9] = 0.9, 0.1
+    assert orc.transport(p, st) == 8
+
+
+def test_vw_relabelling_permutes_every_output(orc):
+    """The four-species gas listed in reverse with a Dirichlet composition: eta, kappa equal, chi and
+    D permuted (to 1e-12; the pivoted factorisation sees the rows in another order)."""
+    M, d = C3
+    Y = list(np.random.default_rng(3).dirichlet(np.ones(4)))
+    perm = [3, 1, 0, 2]
+    e1, k1, c1, D1, X1, _ = uniform_transport(orc, M, d, Y, transport_type=2)
+    e2, k2, c2, D2, X2, _ = uniform_transport(orc, [M[q] for q in perm], [d[q] for q in perm],
+                                              [Y[q] for q in perm], transport_type=2)
+    assert e2 == pytest.approx(e1, rel=1e-12)
+    assert k2 == pytest.approx(k1, rel=1e-12)
+    np.testing.assert_allclose(c2, c1[perm], rtol=1e-11, atol=1e-14 * np.abs(c1).max())
+    Dm1, Dm2 = D1.reshape(4, 4), D2.reshape(4, 4)
+    np.testing.assert_allclose(Dm2, Dm1[np.ix_(perm, perm)], rtol=1e-10, atol=1e-12 * np.abs(Dm1).max())
