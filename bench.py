@@ -565,7 +565,11 @@ def main():
     ctx.close()
     del cu, prim
     if not a.no_configs:
+        vw = P.four_species(64, 10)
+        vw.params.transport_type = 2
+        vw.name += " Valk-Waldmann transport"
         line["configs"] = [config_rate(P.argon_gas(32), 50), config_rate(P.four_species(128), 10),
+                           config_rate(P.species_prefix(8, 64, 10), 10), config_rate(vw, 10),
                            config_rate(P.no2_n2o4(128), 10), config_rate(P.taylor_green(64), 50),
                            wall_rate(P.wall_deck("slip_wall_adiabatic")),
                            wall_rate(P.wall_deck("temp_grad")), wall_rate(P.wall_deck("RT_3d")),

@@ -287,10 +287,10 @@ def species_prefix(n_species, edge, nsteps, shape=None):
         return argon_gas(edge, nsteps, shape)
     M = EIGHT_SPECIES_M[:n_species]
     d = EIGHT_SPECIES_D[:n_species]
+    box = f"{edge}^3" if shape is None else "x".join(str(s) for s in shape)
     shape = tuple(shape) if shape is not None else (edge, edge, edge)
     p = make_params(n_species, shape, 8e-6, M, d)
     rhobar = [1.0 / n_species] * n_species
-    box = f"{edge}^3" if shape is None else "x".join(str(s) for s in shape)
     return Config(f"species{n_species}_{box}", p, 4.83e-4, rhobar, 300.0, MIXTURE_DT, nsteps)
 
 
