@@ -435,6 +435,12 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if not a.no_cpu or not a.no_stag:
+        # every CPU leg of this run (the staggered comparison, the CPU baseline) times the
+        # host-tuned oracle build, which has to be chosen before the oracle is first loaded
+        from oracle import ffi as orc
+        orc.use_native_build()
+
     import torch
 
     def copy_probe():

@@ -26,12 +26,15 @@ def build():
 def use_native_build():
     """Build liborc_native.so (gcc -march=native for this host, same operation order) and load
     it instead of liborc.so.  For timing the CPU baseline on the machine that runs it; must be
-    called before the oracle is first used in the process."""
+    called before the oracle is first used in the process (a second call is a no-op)."""
     global LIB
+    native = os.path.join(HERE, "liborc_native.so")
     if _lib is not None:
+        if LIB == native:
+            return
         raise RuntimeError("the oracle is already loaded in this process")
     subprocess.check_call(["make", "-s", "-C", HERE, "-B", "native"])
-    LIB = os.path.join(HERE, "liborc_native.so")
+    LIB = native
 
 
 def lib():
