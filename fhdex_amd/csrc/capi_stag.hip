@@ -124,8 +124,10 @@ int ensure_stag(fhdx_ctx* c)
 }
 
 // FillBoundary(periodicity) of a stage's output (timeStepStag.cpp:526-535): state, momenta,
-// velocities, prim.
-int fill_stage_end(const fhdx_ctx* c, Arr cons, const Arr mom[3], const Arr vel[3], Arr prim, hipStream_t s)
+// velocities, prim; with tr, also the transport arrays the conversion filled (a ghost cell's
+// coefficients are those of its periodic image, whose prim it holds).
+int fill_stage_end(const fhdx_ctx* c, Arr cons, const Arr mom[3], const Arr vel[3], Arr prim, const StagTransport* tr,
+                   hipStream_t s)
 {
     FillList L{};
     L.g = c->p.ngc;
@@ -135,7 +137,17 @@ int fill_stage_end(const fhdx_ctx* c, Arr cons, const Arr mom[3], const Arr vel[
         L.item[4 + d] = FillItem{vel[d], d, 0, 1};
     }
     L.item[7] = FillItem{prim, -1, 0, c->dp.nprim};
-    return stag_fill(c->dp, L, 8, s);
+    int count = 8;
+    if (tr) {
+        const int NS = c->p.nspecies;
+        L.item[8] = FillItem{tr->eta, -1, 0, 1};
+        L.item[9] = FillItem{tr->zeta, -1, 0, 1};
+        L.item[10] = FillItem{tr->kappa, -1, 0, 1};
+        L.item[11] = FillItem{tr->chi, -1, 0, NS};
+        L.item[12] = FillItem{tr->D, -1, 0, NS * NS};
+        count = 13;
+    }
+    return stag_fill(c->dp, L, count, s);
 }
 
 // The density ghosts conservedToPrimitiveStag reads (face velocities at index 0).
@@ -147,11 +159,12 @@ int fill_density(const fhdx_ctx* c, Arr cons, hipStream_t s)
     return stag_fill(c->dp, L, 1, s);
 }
 
-int c2p_with_ghosts(const fhdx_ctx* c, Arr prim, const Arr vel[3], Arr cons, const Arr mom[3], hipStream_t s)
+int c2p_with_ghosts(const fhdx_ctx* c, Arr prim, const Arr vel[3], Arr cons, const Arr mom[3], const StagTransport* tr,
+                    hipStream_t s)
 {
     int r = fill_density(c, cons, s);
-    if (!r) r = stag_c2p(c->dp, prim, vel, cons, mom, s);
-    if (!r) r = fill_stage_end(c, cons, mom, vel, prim, s);
+    if (!r) r = stag_c2p(c->dp, prim, vel, cons, mom, tr, s);
+    if (!r) r = fill_stage_end(c, cons, mom, vel, prim, tr, s);
     return r;
 }
 
@@ -174,7 +187,7 @@ int fhdx_cons_to_prim_stag(fhdx_ctx* c, fhdx_fab prim, const fhdx_fab vel[3], fh
         v[d] = make_arr(vel[d]);
         m[d] = make_arr(cumom[d]);
     }
-    return c2p_with_ghosts(c, make_arr(prim), v, make_arr(cu), m, SS(stream));
+    return c2p_with_ghosts(c, make_arr(prim), v, make_arr(cu), m, nullptr, SS(stream));
 }
 
 static int check_stag_step_args(fhdx_ctx* c, fhdx_fab cu, const fhdx_fab cumom[3], fhdx_fab prim, const fhdx_fab vel[3],
@@ -266,9 +279,13 @@ int fhdx_rk3_step_stag(fhdx_ctx* c, fhdx_fab cu, const fhdx_fab cumom[3], fhdx_f
         m1[d] = make_arr(w->cupm[d]);
         m2[d] = make_arr(w->cup2m[d]);
     }
+    // calculateTransportCoeffs of each stage's prim (timeStepStag.cpp): on the caller's prim before
+    // stage 0, then inside the conversion that ends stages 0 and 1.  The last conversion leaves
+    // them alone, so the step returns the stage-2 coefficients, as the reference does.
+    const StagTransport tc{A.eta, A.zeta, A.kappa, A.chi, A.D, c->d_err};
     for (int st = 0; st < 3; ++st) {
-        int r = ref_transport(c->dp, A.prim, A.eta, A.zeta, A.kappa, A.chi, A.D, c->d_err, s);
-        if (r) return r;
+        int r = 0;
+        if (st == 0 && (r = ref_transport(c->dp, A.prim, A.eta, A.zeta, A.kappa, A.chi, A.D, c->d_err, s))) return r;
         const Arr us = st == 0 ? u0 : st == 1 ? c1 : c2;
         const Arr* ms = st == 0 ? m0 : st == 1 ? m1 : m2;
         const Arr out = st == 0 ? c1 : st == 1 ? c2 : u0;
@@ -295,7 +312,7 @@ int fhdx_rk3_step_stag(fhdx_ctx* c, fhdx_fab cu, const fhdx_fab cumom[3], fhdx_f
         }
         for (int e = 0; e < 6; ++e) U.ef[e] = A.ef[e];
         if ((r = stag_update(c->dp, U, st, dt, s))) return r;
-        if ((r = c2p_with_ghosts(c, A.prim, A.vel, out, outm, s))) return r;
+        if ((r = c2p_with_ghosts(c, A.prim, A.vel, out, outm, st < 2 ? &tc : nullptr, s))) return r;
     }
     return FHDX_OK;
 }

@@ -36,16 +36,25 @@ struct FillItem {
     int nodal, c0, nc;
 };
 struct FillList {
-    FillItem item[8];
+    FillItem item[13];
     int g;  // ghost layers
+};
+
+// The transport arrays conservedToPrimitiveStag may also fill (valid cells) from the prim it
+// computes: eta, zeta, kappa, chi, D and the device error flags.
+struct StagTransport {
+    Arr eta, zeta, kappa, chi, D;
+    int* err;
 };
 
 // calculateFluxStag with the noise A + w B of the step's Philox sets (key K).
 int stag_flux(const DevParams& P, const StagArrays& A, const NoiseKey& K, double w, double dt, hipStream_t s);
 // Stage st (0, 1, 2) of RK3stepStag's combination.
 int stag_update(const DevParams& P, const StagUpdate& U, int st, double dt, hipStream_t s);
-// conservedToPrimitiveStag on the valid box (cons' density ghosts must be current).
-int stag_c2p(const DevParams& P, Arr prim, const Arr vel[3], Arr cons, const Arr mom[3], hipStream_t s);
+// conservedToPrimitiveStag on the valid box (cons' density ghosts must be current); with tr set,
+// also calculateTransportCoeffs of each valid cell's new prim (their ghosts are periodic images).
+int stag_c2p(const DevParams& P, Arr prim, const Arr vel[3], Arr cons, const Arr mom[3], const StagTransport* tr,
+             hipStream_t s);
 // FillBoundary(periodicity) of the first `count` items.
 int stag_fill(const DevParams& P, const FillList& L, int count, hipStream_t s);
 

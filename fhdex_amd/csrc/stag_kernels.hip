@@ -509,9 +509,11 @@ __global__ void __launch_bounds__(SB) k_stag_update(DevParams P, StagUpdate U, i
 
 // ---- conservedToPrimitiveStag (conservedPrimitiveConversionsStag.cpp:58-140) ---------------
 
-template <int NS>
+// TR: 0 conversion only; 1 also the transport coefficients of the parameters' type (1 or 3) and
+// 2 of transport_type 2, from the prim values just computed (the same inputs k_transport reads).
+template <int NS, int TR>
 __global__ void __launch_bounds__(SB) k_stag_c2p(DevParams P, Arr prim, Arr vx, Arr vy, Arr vz, Arr cons, Arr mx,
-                                                 Arr my, Arr mz)
+                                                 Arr my, Arr mz, StagTransport tc)
 {
     const int part = blockIdx.y;  // 0 cells, 1..3 face velocities of direction part-1
     const long tid = (long)blockIdx.x * SB + threadIdx.x;
@@ -576,7 +578,25 @@ __global__ void __launch_bounds__(SB) k_stag_c2p(DevParams P, Arr prim, Arr vx, 
         prim(i, j, k, 6 + n) = Yk[n];
         prim(i, j, k, 6 + NS + n) = Yk[n] * molmix / P.molmass[n];
     }
-    prim(i, j, k, 5) = eos_pressure<NS>(P, Yk, rho, T);
+    const double pres = eos_pressure<NS>(P, Yk, rho, T);
+    prim(i, j, k, 5) = pres;
+    if constexpr (TR != 0) {  // calculateTransportCoeffs of this cell (transCoeffs.cpp:37-88)
+        Transport tr;
+        double chiv[NS], Dv[NS * NS];
+        if constexpr (TR == 2) {
+            const int fl = transport_cell_vw<NS>(P, rho, T, pres, Yk, tr, chiv, Dv);
+            if (fl) atomicOr(tc.err, fl);
+        } else {
+            if (transport_cell<NS>(P, rho, T, pres, Yk, tr, chiv, Dv)) atomicOr(tc.err, 1);
+        }
+        tc.eta(i, j, k) = tr.eta;
+        tc.zeta(i, j, k) = tr.zeta;
+        tc.kappa(i, j, k) = tr.kappa;
+#pragma unroll
+        for (int n = 0; n < NS; ++n) tc.chi(i, j, k, n) = chiv[n];
+#pragma unroll
+        for (int n = 0; n < NS * NS; ++n) tc.D(i, j, k, n) = Dv[n];
+    }
 }
 
 // ---- periodic ghost fill of cell and face FABs ----------------------------------------------
@@ -693,12 +713,23 @@ int stag_update(const DevParams& P, const StagUpdate& U, int st, double dt, hipS
     return FHDX_OK;
 }
 
-int stag_c2p(const DevParams& P, Arr prim, const Arr vel[3], Arr cons, const Arr mom[3], hipStream_t s)
+int stag_c2p(const DevParams& P, Arr prim, const Arr vel[3], Arr cons, const Arr mom[3], const StagTransport* tr,
+             hipStream_t s)
 {
     long mx = (long)P.n[0] * P.n[1] * P.n[2];
     for (int d = 0; d < 3; ++d) mx = std::max(mx, (long)(P.n[0] + (d == 0)) * (P.n[1] + (d == 1)) * (P.n[2] + (d == 2)));
-    STAG_NS_DISPATCH(P.ns, (k_stag_c2p<NS><<<dim3(grid_for(mx, SB), 4), SB, 0, s>>>(
-                               P, prim, vel[0], vel[1], vel[2], cons, mom[0], mom[1], mom[2])));
+    const dim3 grid(grid_for(mx, SB), 4);
+    const StagTransport tc = tr ? *tr : StagTransport{};
+    if (!tr) {
+        STAG_NS_DISPATCH(P.ns, (k_stag_c2p<NS, 0><<<grid, SB, 0, s>>>(P, prim, vel[0], vel[1], vel[2], cons, mom[0],
+                                                                       mom[1], mom[2], tc)));
+    } else if (P.transport_type == 2) {
+        STAG_NS_DISPATCH(P.ns, (k_stag_c2p<NS, 2><<<grid, SB, 0, s>>>(P, prim, vel[0], vel[1], vel[2], cons, mom[0],
+                                                                       mom[1], mom[2], tc)));
+    } else {
+        STAG_NS_DISPATCH(P.ns, (k_stag_c2p<NS, 1><<<grid, SB, 0, s>>>(P, prim, vel[0], vel[1], vel[2], cons, mom[0],
+                                                                       mom[1], mom[2], tc)));
+    }
     FHDX_HIP_TRY(hipGetLastError());
     return FHDX_OK;
 }
