@@ -171,3 +171,19 @@ def test_engine_error_flag_is_per_instance(dev, orc):
     assert flags[a] == 0 and flags[b] != 0
     assert not any(ens.errors())   # cleared by the read
     ens.close()
+
+
+def test_engine_rejects_valk_waldmann_transport(dev, orc):
+    """transport_type 2 has kernels of its own on the single-box paths; the engine's shared
+    kernels carry the default transport only, so admission refuses it with a clear error."""
+    torch, fc, L, E = dev
+    say("case engine refuses transport_type 2")
+    cfg = P.four_species(6)
+    cfg.params.transport_type = 2
+    cu0 = perturbed_state(cfg.params, cfg, orc, seed=5, amp=0.01, vel=300.0).cu
+    ens = E.Ensemble(2, E.bytes_needed(cfg.params))
+    with pytest.raises(L.FhdxError) as e:
+        ens.add(cfg.params, cfg.dt, 1, cu0)
+    assert e.value.status == 2
+    assert "Valk-Waldmann" in str(e.value)
+    ens.close()
