@@ -285,6 +285,8 @@ def species_prefix(n_species, edge, nsteps, shape=None):
     The box is edge^3 cells, or `shape`."""
     if n_species <= 1:
         return argon_gas(edge, nsteps, shape)
+    if n_species > len(EIGHT_SPECIES_M):
+        raise ValueError(f"species_prefix: at most {len(EIGHT_SPECIES_M)} species (MAX_SPECIES)")
     M = EIGHT_SPECIES_M[:n_species]
     d = EIGHT_SPECIES_D[:n_species]
     box = f"{edge}^3" if shape is None else "x".join(str(s) for s in shape)
