@@ -486,7 +486,17 @@ int rk3_stage(const fhdx_ctx* c, const DevParams& P, const StageArrays& A, int s
     a.cons = make_arr(in[st]);
     int r;
     if ((r = ref_transport(P, a.prim, a.eta, a.zeta, a.kappa, a.chi, a.D, c->d_err, s))) return r;
-    {
+    if (P.stoch_on) {
+        // the stochastic and node passes form stochFlux = A + w B and rancorn themselves
+        a.weigh = true;
+        for (int d = 0; d < 3; ++d) {
+            a.nA[d] = make_arr(A.sA[d]);
+            a.nB[d] = make_arr(A.sB[d]);
+        }
+        a.rA = make_arr(A.rA);
+        a.rB = make_arr(A.rB);
+        a.w = ws[st];
+    } else {
         double* const dst[4] = {A.stochFlux[0].p, A.stochFlux[1].p, A.stochFlux[2].p, A.rancorn.p};
         const double* const la[4] = {A.sA[0].p, A.sA[1].p, A.sA[2].p, A.rA.p};
         const double* const lb[4] = {A.sB[0].p, A.sB[1].p, A.sB[2].p, A.rB.p};

@@ -6,12 +6,19 @@
 namespace fhdx {
 
 // Views of every array calculateFlux touches (flux.cpp:4-16).
+// With weigh set, ran[] and rancorn are outputs: the stochastic pass reads the step's noise sets
+// A, B and stores ran = 1*A + w*B (component 0 zero) for each face it computes, and the node pass
+// rancorn likewise (the reference's MultiFab::LinComb, timeStep.cpp:99-107), so no separate
+// weighting launch runs; otherwise ran[] and rancorn hold the weighted noise already.
 struct FluxArgs {
     Arr cons, prim, eta, zeta, kappa, chi, D;
     Arr flux[3], ran[3];
     Arr corn[3][3];  // [derivative direction][velocity component]: cornx/corny/cornz
     Arr visccorn, rancorn;
     double dt;
+    bool weigh = false;
+    Arr nA[3], nB[3], rA, rB;
+    double w = 0.;
 };
 
 int ref_c2p(const DevParams& P, Arr prim, Arr cons, hipStream_t s);

@@ -291,13 +291,25 @@ __global__ void __launch_bounds__(BLK) k_fill(double* p, long n, double v)
 // are scaled by sqrt(2) (slip) or 0 (no slip) (:250-280).  StochFlux (boundary.cpp:1392-1970)
 // then scales the face's stochastic fluxes, applied here to the values this thread wrote.
 
-template <int NS, int D>
+// The step's noise of one stage: with WN the weighted values 1*A + w*B are formed here (and
+// stored into ran for the face, as MultiFab::LinComb leaves them); otherwise ran holds them.
+struct NoiseW {
+    Arr A, B, rA, rB;
+    double w;
+};
+
+template <int NS, int D, bool WN>
 __device__ __forceinline__ void stoch_body(const DevParams& P, Arr prim, Arr eta, Arr zeta, Arr kappa, Arr chi,
-                                           Arr Dm, Arr fl, Arr ran, Arr rc, double volinv, double dtinv,
-                                           double dt, long t)
+                                           Arr Dm, Arr fl, Arr ran, Arr rc, const NoiseW& nw, double volinv,
+                                           double dtinv, double dt, long t)
 {
     int i, j, k;
     if (!box_index(P, t, D == 0, D == 1, D == 2, i, j, k)) return;
+    if constexpr (WN) {
+#pragma unroll
+        for (int c = 0; c < NS + 5; ++c)
+            ran(i, j, k, c) = (c < 1) ? 0.0 : 1.0 * nw.A(i, j, k, c) + nw.w * nw.B(i, j, k, c);
+    }
     const WallFace W = wall_face<D>(P, i, j, k);
     // lower and upper cells of the face averages; both the wall-side cell at a wall
     const int il = W.any() ? W.wi : i - (D == 0), jl = W.any() ? W.wj : j - (D == 1),
@@ -381,7 +393,10 @@ __device__ __forceinline__ void stoch_body(const DevParams& P, Arr prim, Arr eta
             }
         }
         const signed char* nd = c_rc_node[D][c];
-        double term = fac[c] * sqrt(mu) * rc(i + nd[0], j + nd[1], k + nd[2]);
+        double rcv;
+        if constexpr (WN) rcv = 1.0 * nw.rA(i + nd[0], j + nd[1], k + nd[2]) + nw.w * nw.rB(i + nd[0], j + nd[1], k + nd[2]);
+        else rcv = rc(i + nd[0], j + nd[1], k + nd[2]);
+        double term = fac[c] * sqrt(mu) * rcv;
         sum = (c == 0) ? term : sum + term;
     }
     w[1 + D] = w[1 + D] + 0.25 * nweight * sum;
@@ -594,11 +609,13 @@ __device__ __forceinline__ void diff_body(const DevParams& P, Arr prim, Arr eta,
 
 __global__ void __launch_bounds__(BLK) k_corner_node(DevParams P, Arr prim, Arr eta, Arr zeta,
                                                      Arr cux, Arr cvx, Arr cwx, Arr cuy, Arr cvy,
-                                                     Arr cwy, Arr cuz, Arr cvz, Arr cwz, Arr vcorn)
+                                                     Arr cwy, Arr cuz, Arr cvz, Arr cwz, Arr vcorn, Arr rancorn,
+                                                     Arr rA, Arr rB, double w, int weigh)
 {
     long t = (long)blockIdx.x * BLK + threadIdx.x;
     int i, j, k;
     if (!box_index(P, t, 1, 1, 1, i, j, k)) return;
+    if (weigh) rancorn(i, j, k) = 1.0 * rA(i, j, k) + w * rB(i, j, k);  // the node's weighted noise
     double muxp = 0.125 * (eta(i, j - 1, k - 1) + eta(i - 1, j - 1, k - 1) + eta(i, j, k - 1) +
                            eta(i - 1, j, k - 1) + eta(i, j - 1, k) + eta(i - 1, j - 1, k) + eta(i, j, k) +
                            eta(i - 1, j, k));
@@ -980,18 +997,19 @@ __global__ void __launch_bounds__(BLK) k_noise_all(DevParams P, NoiseKey K, Arr 
 // On small boxes (the wall decks), three directions' faces in one launch keep three times as
 // many waves in flight as one direction's.
 
-template <int NS>
+template <int NS, bool WN>
 __global__ void __launch_bounds__(BLK) k_stoch3(DevParams P, Arr prim, Arr eta, Arr zeta, Arr kappa, Arr chi,
                                                 Arr Dm, Arr f0, Arr f1, Arr f2, Arr r0, Arr r1, Arr r2, Arr rc,
-                                                double volinv, double dtinv, double dt)
+                                                NoiseW n0, NoiseW n1, NoiseW n2, double volinv, double dtinv,
+                                                double dt)
 {
     const long t = (long)blockIdx.x * BLK + threadIdx.x;
     if (blockIdx.y == 0)
-        stoch_body<NS, 0>(P, prim, eta, zeta, kappa, chi, Dm, f0, r0, rc, volinv, dtinv, dt, t);
+        stoch_body<NS, 0, WN>(P, prim, eta, zeta, kappa, chi, Dm, f0, r0, rc, n0, volinv, dtinv, dt, t);
     else if (blockIdx.y == 1)
-        stoch_body<NS, 1>(P, prim, eta, zeta, kappa, chi, Dm, f1, r1, rc, volinv, dtinv, dt, t);
+        stoch_body<NS, 1, WN>(P, prim, eta, zeta, kappa, chi, Dm, f1, r1, rc, n1, volinv, dtinv, dt, t);
     else
-        stoch_body<NS, 2>(P, prim, eta, zeta, kappa, chi, Dm, f2, r2, rc, volinv, dtinv, dt, t);
+        stoch_body<NS, 2, WN>(P, prim, eta, zeta, kappa, chi, Dm, f2, r2, rc, n2, volinv, dtinv, dt, t);
 }
 
 template <int NS>
@@ -1129,15 +1147,23 @@ static void flux_passes(const DevParams& P, const FluxArgs& a, hipStream_t s)
     long mf = nface(P, 0);
     for (int d = 1; d < 3; ++d) mf = nface(P, d) > mf ? nface(P, d) : mf;
     const dim3 grid(grid_for(mf, BLK), 3);   // every body checks its own direction's face count
-    if (P.stoch_on)
-        k_stoch3<NS><<<grid, BLK, 0, s>>>(P, a.prim, a.eta, a.zeta, a.kappa, a.chi, a.D, a.flux[0], a.flux[1],
-                                          a.flux[2], a.ran[0], a.ran[1], a.ran[2], a.rancorn, volinv, dtinv, a.dt);
+    NoiseW nw[3];
+    for (int d = 0; d < 3; ++d) nw[d] = NoiseW{a.nA[d], a.nB[d], a.rA, a.rB, a.w};
+    if (P.stoch_on && a.weigh)
+        k_stoch3<NS, true><<<grid, BLK, 0, s>>>(P, a.prim, a.eta, a.zeta, a.kappa, a.chi, a.D, a.flux[0], a.flux[1],
+                                                a.flux[2], a.ran[0], a.ran[1], a.ran[2], a.rancorn, nw[0], nw[1],
+                                                nw[2], volinv, dtinv, a.dt);
+    else if (P.stoch_on)
+        k_stoch3<NS, false><<<grid, BLK, 0, s>>>(P, a.prim, a.eta, a.zeta, a.kappa, a.chi, a.D, a.flux[0],
+                                                 a.flux[1], a.flux[2], a.ran[0], a.ran[1], a.ran[2], a.rancorn,
+                                                 nw[0], nw[1], nw[2], volinv, dtinv, a.dt);
     k_diff3<NS><<<grid, BLK, 0, s>>>(P, a.prim, a.eta, a.kappa, a.chi, a.D, a.flux[0], a.flux[1], a.flux[2]);
     // corn[a][b]: derivative direction a, velocity component b (cornx/corny/cornz of flux.cpp)
     const Arr(&c)[3][3] = a.corn;
     k_corner_node<<<grid_for(nnode(P), BLK), BLK, 0, s>>>(P, a.prim, a.eta, a.zeta, c[0][0], c[0][1], c[0][2],
                                                           c[1][0], c[1][1], c[1][2], c[2][0], c[2][1], c[2][2],
-                                                          a.visccorn);
+                                                          a.visccorn, a.rancorn, a.rA, a.rB, a.w,
+                                                          (P.stoch_on && a.weigh) ? 1 : 0);
     // x face: div pair (vy, wz), tangential (uy, uz); y: (ux, wz), (vx, vz); z: (vy, ux), (wx, wy)
     CornerFaceSet C;
     for (int d = 0; d < 3; ++d) C.fl[d] = a.flux[d];
