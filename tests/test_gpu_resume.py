@@ -21,7 +21,8 @@ def say(msg):
 
 
 @pytest.mark.parametrize("name,mk", [("argon", lambda: P.argon_gas(0, shape=(12, 8, 16))),
-                                     ("species4", lambda: P.four_species(8))])
+                                     ("species4", lambda: P.four_species(8)),
+                                     ("species8", lambda: P.species_prefix(8, 8, 1))])
 def test_checkpoint_restart_is_exact(tmp_path, orc, name, mk):
     import torch
     from fhdex_amd import compressible as fc

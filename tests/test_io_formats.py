@@ -58,7 +58,8 @@ def valid(p, t):
 
 
 CASES = [("argon", lambda: P.argon_gas(0, shape=(8, 6, 5)).params),
-         ("species4", lambda: P.four_species(6).params)]
+         ("species4", lambda: P.four_species(6).params),
+         ("species8", lambda: P.species_prefix(8, 0, 1, shape=(5, 4, 6)).params)]
 
 
 @pytest.mark.parametrize("name,mk", CASES)
