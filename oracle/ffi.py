@@ -29,9 +29,9 @@ def use_native_build():
     called before the oracle is first used in the process (a second call is a no-op)."""
     global LIB
     native = os.path.join(HERE, "liborc_native.so")
+    if LIB == native:
+        return
     if _lib is not None:
-        if LIB == native:
-            return
         raise RuntimeError("the oracle is already loaded in this process")
     subprocess.check_call(["make", "-s", "-C", HERE, "-B", "native"])
     LIB = native
